@@ -1,0 +1,319 @@
+// Host side of the C ABI declared in include/pp_solver.h: argument checks, device buffers, layout
+// tables, transfers and the translation of device status codes into the reference's error texts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "pp_device.hpp"
+#include "pp_kernels.hpp"
+#include "pp_solver.h"
+
+struct pp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+
+  struct Layout {
+    int nlev = 0;
+    int *off = nullptr;
+    uint32_t *cell = nullptr;
+  };
+  std::map<std::pair<int, int>, Layout> layouts;
+
+  // work arrays, sized for `cap` cells
+  size_t cap = 0;
+  double *work = nullptr;      // 13 FP64 arrays
+  float *fwork = nullptr;      // 12 FP32 arrays
+  double *wrap64 = nullptr;    // LX, LY
+  float *wrap32 = nullptr;     // fLX, fLY
+  // staging for the host entry point
+  size_t stage_cap = 0;
+  double *stage = nullptr;     // rho, u, v, p_n, p_nm1, p_out
+  pp::SysArgs *d_args = nullptr;
+  pp::DevReport *d_rep = nullptr;
+  pp::DevReport *h_rep = nullptr;  // pinned
+};
+
+namespace {
+
+int fail(pp_ctx *ctx, pp_report *rep, int status, const std::string &msg) {
+  if (ctx) ctx->last_error = msg;
+  if (rep) {
+    rep->status = status;
+    std::snprintf(rep->message, sizeof(rep->message), "%s", msg.c_str());
+  }
+  return status;
+}
+
+#define PP_HIP_CHECK(expr)                                                                     \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return fail(ctx, report, PP_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));    \
+  } while (0)
+
+// Wavefront layout tables: level offsets and the (i, j) of every position.
+void build_layout(int nx, int ny, std::vector<int> &off, std::vector<uint32_t> &cell) {
+  const int nlev = nx + ny - 1;
+  off.assign(static_cast<size_t>(nlev) + 1, 0);
+  cell.clear();
+  cell.reserve(static_cast<size_t>(nx) * ny);
+  for (int d = 0; d < nlev; ++d) {
+    const int lo = std::max(0, d - ny + 1), hi = std::min(d, nx - 1);
+    for (int i = lo; i <= hi; ++i) cell.push_back((static_cast<uint32_t>(i) << 16) | static_cast<uint32_t>(d - i));
+    off[static_cast<size_t>(d) + 1] = static_cast<int>(cell.size());
+  }
+}
+
+const char *status_text(int status, int scheme) {
+  const bool ildlt = scheme == PP_ILDLT_PCG || scheme == PP_SPLIT_ILDLT_PCG;
+  switch (status) {
+    case pp::kFactorFail: return ildlt ? "ILDLT preconditioner construction failed" : "ICC preconditioner construction failed";
+    case pp::kPapFail: return "pcg breakdown: p^T A p is invalid";
+    case pp::kRz0Fail: return "pcg preconditioner breakdown: non-positive r^T M^{-1} r";
+    case pp::kRzFail: return "pcg preconditioner breakdown: non-positive new r^T M^{-1} r";
+    case pp::kNoDiag: return "sparse matrix assembly failed to produce a diagonal entry";
+    case pp::kSweepFail: return "wavefront hand-off did not complete";
+    default: return "unknown device status";
+  }
+}
+
+int validate(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
+  if (!ctx) return fail(ctx, report, PP_ERR_ARG, "null context");
+  if (!pb) return fail(ctx, report, PP_ERR_ARG, "null problem");
+  if (pb->nx < 1 || pb->ny < 1 || pb->nx > pp::kMaxSide || pb->ny > pp::kMaxSide)
+    return fail(ctx, report, PP_ERR_ARG, "grid size outside 1..1024 per direction");
+  if ((pb->periodic_x && pb->nx < 4) || (pb->periodic_y && pb->ny < 4))
+    return fail(ctx, report, PP_ERR_ARG, "periodic direction needs at least 4 cells");
+  if (pb->scheme < PP_ICPCG || pb->scheme > PP_SPLIT_ILDLT_PCG) return fail(ctx, report, PP_ERR_ARG, "unknown scheme");
+  if (pb->precision != PP_FP64 && pb->precision != PP_MIXED) return fail(ctx, report, PP_ERR_ARG, "unknown precision");
+  for (int s = 0; s < 4; ++s)
+    if (pb->bc_type[s] < PP_BC_UNSET || pb->bc_type[s] > PP_BC_NEUMANN)
+      return fail(ctx, report, PP_ERR_ARG, "unknown boundary type");
+  // explicit pressure BCs on periodic axes are rejected by the reference (operators.cpp:11-34)
+  if (pb->periodic_x && (pb->bc_type[0] != PP_BC_UNSET || pb->bc_type[1] != PP_BC_UNSET))
+    return fail(ctx, report, PP_ERR_ARG, "pressure_bc_left cannot be set when the direction is periodic");
+  if (pb->periodic_y && (pb->bc_type[2] != PP_BC_UNSET || pb->bc_type[3] != PP_BC_UNSET))
+    return fail(ctx, report, PP_ERR_ARG, "pressure_bc_bottom cannot be set when the direction is periodic");
+  return PP_OK;
+}
+
+bool has_dirichlet(const pp_problem *pb) {
+  auto dir = [&](int s) { return pb->bc_type[s] == PP_BC_DIRICHLET; };
+  if (!pb->periodic_x && (dir(0) || dir(1))) return true;
+  if (!pb->periodic_y && (dir(2) || dir(3))) return true;
+  return false;
+}
+
+int ensure_layout(pp_ctx *ctx, int nx, int ny, pp_ctx::Layout **out, pp_report *report) {
+  auto key = std::make_pair(nx, ny);
+  auto it = ctx->layouts.find(key);
+  if (it == ctx->layouts.end()) {
+    std::vector<int> off;
+    std::vector<uint32_t> cell;
+    build_layout(nx, ny, off, cell);
+    pp_ctx::Layout L;
+    L.nlev = nx + ny - 1;
+    PP_HIP_CHECK(hipMalloc(&L.off, off.size() * sizeof(int)));
+    PP_HIP_CHECK(hipMalloc(&L.cell, cell.size() * sizeof(uint32_t)));
+    PP_HIP_CHECK(hipMemcpy(L.off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    PP_HIP_CHECK(hipMemcpy(L.cell, cell.data(), cell.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    it = ctx->layouts.emplace(key, L).first;
+  }
+  *out = &it->second;
+  return PP_OK;
+}
+
+constexpr int kF64Arrays = 13;
+constexpr int kF32Arrays = 12;
+
+int ensure_work(pp_ctx *ctx, size_t n, pp_report *report) {
+  if (n <= ctx->cap) return PP_OK;
+  if (ctx->work) (void)hipFree(ctx->work);
+  if (ctx->fwork) (void)hipFree(ctx->fwork);
+  ctx->work = nullptr;
+  ctx->fwork = nullptr;
+  ctx->cap = 0;
+  PP_HIP_CHECK(hipMalloc(&ctx->work, n * kF64Arrays * sizeof(double)));
+  PP_HIP_CHECK(hipMalloc(&ctx->fwork, n * kF32Arrays * sizeof(float)));
+  ctx->cap = n;
+  return PP_OK;
+}
+
+void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::SysArgs &a) {
+  std::memset(&a, 0, sizeof(a));
+  a.nx = pb->nx;
+  a.ny = pb->ny;
+  a.n = pb->nx * pb->ny;
+  a.nlev = L.nlev;
+  a.dx = pb->dx;
+  a.dy = pb->dy;
+  a.dt = pb->dt;
+  a.periodic_x = pb->periodic_x ? 1 : 0;
+  a.periodic_y = pb->periodic_y ? 1 : 0;
+  for (int s = 0; s < 4; ++s) {
+    a.bc_type[s] = pb->bc_type[s];
+    a.bc_value[s] = pb->bc_value[s];
+  }
+  a.split = (pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG) ? 1 : 0;
+  a.ildlt = (pb->scheme == PP_ILDLT_PCG || pb->scheme == PP_SPLIT_ILDLT_PCG) ? 1 : 0;
+  a.projected = has_dirichlet(pb) ? 0 : 1;
+  a.max_iterations = pb->max_iterations;
+  a.tolerance = pb->tolerance;
+  a.rho_ref = std::min(1.0, pb->density_ratio);
+  a.step_index = pb->step_index;
+  a.inner_max_iterations = pb->inner_max_iterations;
+  a.inner_tolerance = pb->inner_tolerance;
+  a.lev_off = L.off;
+  a.cell = L.cell;
+  const size_t n = ctx->cap;
+  double *w = ctx->work;
+  double **f64[kF64Arrays] = {&a.aP, &a.aW, &a.aS, &a.Lw, &a.Ls, &a.dg, &a.b, &a.x, &a.r, &a.pa, &a.pb, &a.z, &a.y};
+  for (int k = 0; k < kF64Arrays; ++k) *f64[k] = w + static_cast<size_t>(k) * n;
+  float *fw = ctx->fwork;
+  float **f32[kF32Arrays] = {&a.fP, &a.fW, &a.fS, &a.fLw, &a.fLs, &a.fRd, &a.fx, &a.fr, &a.fpa, &a.fpb, &a.fz, &a.fy};
+  for (int k = 0; k < kF32Arrays; ++k) *f32[k] = fw + static_cast<size_t>(k) * n;
+  a.LX = ctx->wrap64;
+  a.LY = ctx->wrap64 + pp::kMaxSide;
+  a.fLX = ctx->wrap32;
+  a.fLY = ctx->wrap32 + pp::kMaxSide;
+  a.rep = ctx->d_rep;
+}
+
+int run_solve(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u, const double *d_v,
+              const double *d_pn, const double *d_pnm1, double *d_out, pp_report *report, hipStream_t stream) {
+  pp_ctx::Layout *L = nullptr;
+  if (int rc = ensure_layout(ctx, pb->nx, pb->ny, &L, report)) return rc;
+  if (int rc = ensure_work(ctx, static_cast<size_t>(pb->nx) * pb->ny, report)) return rc;
+  pp::SysArgs a;
+  fill_args(ctx, pb, *L, a);
+  a.rho = d_rho;
+  a.u = d_u;
+  a.v = d_v;
+  a.p_n = d_pn;
+  a.p_nm1 = d_pnm1;
+  a.p_out = d_out;
+  PP_HIP_CHECK(hipMemcpyAsync(ctx->d_args, &a, sizeof(a), hipMemcpyHostToDevice, stream));
+  PP_HIP_CHECK(hipMemsetAsync(ctx->d_rep, 0, sizeof(pp::DevReport), stream));
+  PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_args, 1, a.ildlt != 0, stream));
+  PP_HIP_CHECK(hipMemcpyAsync(ctx->h_rep, ctx->d_rep, sizeof(pp::DevReport), hipMemcpyDeviceToHost, stream));
+  PP_HIP_CHECK(hipStreamSynchronize(stream));
+  const pp::DevReport &r = *ctx->h_rep;
+  if (report) {
+    std::memset(report, 0, sizeof(*report));
+    report->iterations = r.iterations;
+    report->outer_cycles = r.outer_cycles;
+    report->relative_residual = r.rel;
+    report->absolute_residual = r.abs;
+    report->iterate_residual = r.iter_res;
+    report->status = PP_OK;
+  }
+  if (r.status != pp::kOk) return fail(ctx, report, r.status, status_text(r.status, pb->scheme));
+  ctx->last_error.clear();
+  return PP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_create(pp_ctx **out, int device) {
+  if (!out) return PP_ERR_ARG;
+  *out = nullptr;
+  auto *ctx = new pp_ctx();
+  ctx->device = device;
+  pp_report *report = nullptr;
+  PP_HIP_CHECK(hipSetDevice(device));
+  PP_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  PP_HIP_CHECK(hipMalloc(&ctx->wrap64, 2 * pp::kMaxSide * sizeof(double)));
+  PP_HIP_CHECK(hipMalloc(&ctx->wrap32, 2 * pp::kMaxSide * sizeof(float)));
+  PP_HIP_CHECK(hipMalloc(&ctx->d_args, sizeof(pp::SysArgs)));
+  PP_HIP_CHECK(hipMalloc(&ctx->d_rep, sizeof(pp::DevReport)));
+  PP_HIP_CHECK(hipHostMalloc(&ctx->h_rep, sizeof(pp::DevReport), hipHostMallocDefault));
+  *out = ctx;
+  return PP_OK;
+}
+
+void pp_destroy(pp_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->layouts) {
+    (void)hipFree(kv.second.off);
+    (void)hipFree(kv.second.cell);
+  }
+  if (ctx->work) (void)hipFree(ctx->work);
+  if (ctx->fwork) (void)hipFree(ctx->fwork);
+  if (ctx->wrap64) (void)hipFree(ctx->wrap64);
+  if (ctx->wrap32) (void)hipFree(ctx->wrap32);
+  if (ctx->stage) (void)hipFree(ctx->stage);
+  if (ctx->d_args) (void)hipFree(ctx->d_args);
+  if (ctx->d_rep) (void)hipFree(ctx->d_rep);
+  if (ctx->h_rep) (void)hipHostFree(ctx->h_rep);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *pp_last_error(const pp_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int pp_solve_device(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u, const double *d_v,
+                    const double *d_p_n, const double *d_p_nm1, double *d_p_out, pp_report *report, void *stream) {
+  if (int rc = validate(ctx, pb, report)) return rc;
+  if (!d_rho && !(pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG))
+    return fail(ctx, report, PP_ERR_ARG, "rho is required");
+  if (!d_u || !d_v || !d_p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return run_solve(ctx, pb, d_rho, d_u, d_v, d_p_n, d_p_nm1, d_p_out, report, s);
+}
+
+int pp_solve(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, const double *u, int ld_u,
+             const double *v, int ld_v, const double *p_n, int ld_pn, const double *p_nm1, int ld_pnm1,
+             double *p_out, int ld_out, pp_report *report) {
+  if (int rc = validate(ctx, pb, report)) return rc;
+  const int nx = pb->nx, ny = pb->ny;
+  if (!rho || !u || !v || !p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  if (ld_rho < ny || ld_u < ny || ld_v < ny + 1 || ld_out < ny || (p_n && ld_pn < ny) || (p_nm1 && ld_pnm1 < ny))
+    return fail(ctx, report, PP_ERR_ARG, "leading dimension smaller than the row length");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  const size_t n = static_cast<size_t>(nx) * ny;
+  const size_t need = 4 * n + static_cast<size_t>(nx + 1) * ny + static_cast<size_t>(nx) * (ny + 1);
+  if (need > ctx->stage_cap) {
+    if (ctx->stage) (void)hipFree(ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_cap = 0;
+    PP_HIP_CHECK(hipMalloc(&ctx->stage, need * sizeof(double)));
+    ctx->stage_cap = need;
+  }
+  double *d_rho = ctx->stage;
+  double *d_u = d_rho + n;
+  double *d_v = d_u + static_cast<size_t>(nx + 1) * ny;
+  double *d_pn = d_v + static_cast<size_t>(nx) * (ny + 1);
+  double *d_pnm1 = d_pn + n;
+  double *d_out = d_pnm1 + n;
+  hipStream_t s = ctx->stream;
+  auto up = [&](double *dst, const double *src, int ld, int rows, int cols) {
+    return hipMemcpy2DAsync(dst, static_cast<size_t>(cols) * sizeof(double), src, static_cast<size_t>(ld) * sizeof(double),
+                            static_cast<size_t>(cols) * sizeof(double), static_cast<size_t>(rows), hipMemcpyHostToDevice, s);
+  };
+  PP_HIP_CHECK(up(d_rho, rho, ld_rho, nx, ny));
+  PP_HIP_CHECK(up(d_u, u, ld_u, nx + 1, ny));
+  PP_HIP_CHECK(up(d_v, v, ld_v, nx, ny + 1));
+  if (p_n) PP_HIP_CHECK(up(d_pn, p_n, ld_pn, nx, ny));
+  if (p_nm1) PP_HIP_CHECK(up(d_pnm1, p_nm1, ld_pnm1, nx, ny));
+  const int rc = run_solve(ctx, pb, d_rho, d_u, d_v, p_n ? d_pn : nullptr, p_nm1 ? d_pnm1 : nullptr, d_out, report, s);
+  if (rc != PP_OK) return rc;
+  PP_HIP_CHECK(hipMemcpy2DAsync(p_out, static_cast<size_t>(ld_out) * sizeof(double), d_out,
+                                static_cast<size_t>(ny) * sizeof(double), static_cast<size_t>(ny) * sizeof(double),
+                                static_cast<size_t>(nx), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  return PP_OK;
+}
+
+}  // extern "C"

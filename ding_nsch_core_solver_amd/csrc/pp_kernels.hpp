@@ -1,0 +1,47 @@
+// Arguments and launch entry points shared between the host layer and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pp {
+
+struct DevReport;
+
+// One pressure system as the kernels see it. Field inputs/outputs are in the natural i-major
+// layout; every work array is in the wavefront layout described in pp_device.hpp.
+struct SysArgs {
+  int nx, ny, n, nlev;
+  double dx, dy, dt;
+  int periodic_x, periodic_y;
+  int bc_type[4];
+  double bc_value[4];
+  int split, ildlt, projected;
+  int max_iterations;
+  double tolerance;
+  double rho_ref;
+  int step_index;
+  // mixed precision controls
+  int inner_max_iterations;
+  double inner_tolerance;
+  // layout tables
+  const int *lev_off;      // nlev + 1
+  const uint32_t *cell;    // n, (i << 16) | j of each wavefront position
+  // fields (natural layout)
+  const double *rho, *u, *v, *p_n, *p_nm1;
+  double *p_out;
+  // FP64 work arrays (wavefront layout, n each)
+  double *aP, *aW, *aS;    // CSR diagonal, west entry, south entry (0 when absent or dropped)
+  double *Lw, *Ls, *dg;    // factor: west/south lower entries, pivot (ICC d or ILDLT D)
+  double *b, *x, *r, *pa, *pb, *z, *y;
+  double *LX, *LY;         // periodic wrap lower entries: LX[j] row (nx-1,j) col (0,j); LY[i] row (i,ny-1) col (i,0)
+  // FP32 work arrays for mixed precision (wavefront layout)
+  float *fP, *fW, *fS, *fLw, *fLs, *fRd;   // operator, factor, reciprocal pivot
+  float *fx, *fr, *fpa, *fpb, *fz, *fy;
+  float *fLX, *fLY;
+  DevReport *rep;
+};
+
+hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream);
+
+}  // namespace pp

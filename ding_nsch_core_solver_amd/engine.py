@@ -1,0 +1,183 @@
+"""ctypes binding of the C ABI in include/pp_solver.h (libppsolve.so, built in-tree).
+
+The library must be built (``python -m ding_nsch_core_solver_amd.build``); there is no fallback
+path: every call goes to the HIP engine and fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .build import LIB_PATH
+
+PP_ICPCG, PP_ILDLT_PCG, PP_SPLIT_ICPCG, PP_SPLIT_ILDLT_PCG = 1, 2, 3, 4
+PP_FP64, PP_MIXED = 0, 1
+
+# New scheme strings (the reference's names prefixed with hip_) and their codes. The split
+# aliases mirror the reference dispatch (pressure.cpp:1048-1058).
+SCHEME_CODES = {
+    "hip_icpcg": PP_ICPCG,
+    "hip_ildlt_pcg": PP_ILDLT_PCG,
+    "hip_split_icpcg": PP_SPLIT_ICPCG,
+    "hip_liu_split_icpcg": PP_SPLIT_ICPCG,
+    "hip_paper_split_icpcg": PP_SPLIT_ICPCG,
+    "hip_split_ildlt_pcg": PP_SPLIT_ILDLT_PCG,
+    "hip_liu_split_ildlt_pcg": PP_SPLIT_ILDLT_PCG,
+    "hip_paper_split_ildlt_pcg": PP_SPLIT_ILDLT_PCG,
+}
+
+
+class PPProblem(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dt", ctypes.c_double),
+        ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int),
+        ("bc_type", ctypes.c_int * 4), ("bc_value", ctypes.c_double * 4),
+        ("scheme", ctypes.c_int), ("precision", ctypes.c_int),
+        ("max_iterations", ctypes.c_int), ("tolerance", ctypes.c_double),
+        ("density_ratio", ctypes.c_double), ("step_index", ctypes.c_int),
+        ("inner_max_iterations", ctypes.c_int), ("inner_tolerance", ctypes.c_double),
+    ]
+
+
+class PPReport(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int), ("outer_cycles", ctypes.c_int),
+        ("relative_residual", ctypes.c_double), ("absolute_residual", ctypes.c_double),
+        ("iterate_residual", ctypes.c_double), ("status", ctypes.c_int),
+        ("message", ctypes.c_char * 256),
+    ]
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+_lib = None
+
+# Every entry point declared in include/pp_solver.h.
+EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_solve", "pp_solve_device")
+
+
+def library_available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not library_available():
+            raise FileNotFoundError(f"HIP engine not built: {LIB_PATH} (python -m ding_nsch_core_solver_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.pp_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+        L.pp_create.restype = ctypes.c_int
+        L.pp_destroy.argtypes = [vp]
+        L.pp_destroy.restype = None
+        L.pp_last_error.argtypes = [vp]
+        L.pp_last_error.restype = ctypes.c_char_p
+        L.pp_solve.argtypes = [vp, ctypes.POINTER(PPProblem), dp, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int,
+                               dp, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(PPReport)]
+        L.pp_solve.restype = ctypes.c_int
+        L.pp_solve_device.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, vp, vp,
+                                      ctypes.POINTER(PPReport), vp]
+        L.pp_solve_device.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def make_problem(case, scheme: int = PP_ICPCG, precision: int = PP_FP64, inner_max_iterations: int = 0,
+                 inner_tolerance: float = 0.0) -> PPProblem:
+    pb = PPProblem()
+    pb.nx, pb.ny = case.nx, case.ny
+    pb.dx, pb.dy, pb.dt = case.dx, case.dy, case.dt
+    pb.periodic_x, pb.periodic_y = int(case.periodic_x), int(case.periodic_y)
+    for s in range(4):
+        pb.bc_type[s] = int(case.bc_type[s])
+        pb.bc_value[s] = float(case.bc_value[s])
+    pb.scheme, pb.precision = int(scheme), int(precision)
+    pb.max_iterations = int(case.max_iterations)
+    pb.tolerance = float(case.tolerance)
+    pb.density_ratio = float(case.density_ratio)
+    pb.step_index = int(case.step_index)
+    pb.inner_max_iterations = int(inner_max_iterations)
+    pb.inner_tolerance = float(inner_tolerance)
+    return pb
+
+
+def _dp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _report_dict(rep: PPReport) -> dict:
+    return {
+        "iterations": rep.iterations, "outer_cycles": rep.outer_cycles,
+        "relative_residual": rep.relative_residual, "absolute_residual": rep.absolute_residual,
+        "iterate_residual": rep.iterate_residual, "status": rep.status,
+        "message": rep.message.decode(errors="replace"),
+    }
+
+
+class Engine:
+    """One engine context on one device (pp_create / pp_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = lib()
+        h = ctypes.c_void_p()
+        rc = self._lib.pp_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise EngineError(rc, f"pp_create failed ({rc})")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.pp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def solve(self, case, scheme: int = PP_ICPCG, precision: int = PP_FP64, **kw):
+        """Host-array solve. Returns (p interior as (nx*ny,) float64, report dict)."""
+        pb = make_problem(case, scheme, precision, **kw)
+        nx, ny = case.nx, case.ny
+        rho = np.ascontiguousarray(case.rho, dtype=np.float64).reshape(nx, ny)
+        u = np.ascontiguousarray(case.u, dtype=np.float64).reshape(nx + 1, ny)
+        v = np.ascontiguousarray(case.v, dtype=np.float64).reshape(nx, ny + 1)
+        pn = None if case.p_n is None else np.ascontiguousarray(case.p_n, dtype=np.float64).reshape(nx, ny)
+        pm = None if case.p_nm1 is None else np.ascontiguousarray(case.p_nm1, dtype=np.float64).reshape(nx, ny)
+        out = np.zeros((nx, ny))
+        rep = PPReport()
+        rc = self._lib.pp_solve(self.h, ctypes.byref(pb), _dp(rho), ny, _dp(u), ny, _dp(v), ny + 1,
+                                _dp(pn), ny, _dp(pm), ny, _dp(out), ny, ctypes.byref(rep))
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return out.reshape(-1), _report_dict(rep)
+
+    def solve_device(self, case, d_rho, d_u, d_v, d_pn, d_pnm1, d_out, scheme: int = PP_ICPCG,
+                     precision: int = PP_FP64, stream=None, **kw):
+        """Solve on device pointers (ints, e.g. torch tensor.data_ptr()). Returns the report dict."""
+        pb = make_problem(case, scheme, precision, **kw)
+        rep = PPReport()
+        rc = self._lib.pp_solve_device(self.h, ctypes.byref(pb), d_rho, d_u, d_v, d_pn, d_pnm1, d_out,
+                                       ctypes.byref(rep), stream)
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return _report_dict(rep)
+
+    def last_error(self) -> str:
+        return self._lib.pp_last_error(self.h).decode(errors="replace")

@@ -1,0 +1,123 @@
+"""Synthetic pressure-solve inputs (droplet density fields and seeded face velocities).
+
+Every array is float64, i-major with j contiguous (cell (i, j) at ``i*ny + j``), the interior
+layout of the reference's ``Field2D`` (src/core/base/field.hpp:23-33) without its ghost ring:
+
+* ``rho``  nx x ny cell densities (the solver's ``rho_mid_``)
+* ``u``    (nx+1) x ny x-face velocities (``u_star_``), face i sits between cells i-1 and i
+* ``v``    nx x (ny+1) y-face velocities (``v_star_``)
+
+The random stream is numpy's PCG64 seeded per case; u is drawn first (i-major), then v. Wall-normal
+faces of non-periodic directions are zeroed, mirroring the Dirichlet-0 no-slip velocity ghosts the
+reference imposes (operators.cpp:194-199, 250-255).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+BC_UNSET, BC_DIRICHLET, BC_NEUMANN = 0, 1, 2
+LEFT, RIGHT, BOTTOM, TOP = 0, 1, 2, 3
+
+
+@dataclasses.dataclass
+class PressureCase:
+    """One pressure-correction problem: grid, boundary options, solver controls and fields."""
+
+    nx: int
+    ny: int
+    lx: float = 1.0
+    ly: float = 1.0
+    dt: float = 1.0e-4
+    periodic_x: bool = False
+    periodic_y: bool = False
+    bc_type: tuple = (BC_UNSET, BC_UNSET, BC_UNSET, BC_UNSET)
+    bc_value: tuple = (0.0, 0.0, 0.0, 0.0)
+    max_iterations: int = 20000
+    tolerance: float = 1.0e-8
+    density_ratio: float = 1000.0
+    step_index: int = 0
+    rho: np.ndarray | None = None
+    u: np.ndarray | None = None
+    v: np.ndarray | None = None
+    p_n: np.ndarray | None = None
+    p_nm1: np.ndarray | None = None
+
+    @property
+    def n(self) -> int:
+        return self.nx * self.ny
+
+    @property
+    def dx(self) -> float:
+        return self.lx / max(float(self.nx), 1.0)
+
+    @property
+    def dy(self) -> float:
+        return self.ly / max(float(self.ny), 1.0)
+
+    def has_dirichlet(self) -> bool:
+        """Mirror of pressure_has_dirichlet_boundary (operators.cpp:114-128)."""
+        if not self.periodic_x and (self.bc_type[LEFT] == BC_DIRICHLET or self.bc_type[RIGHT] == BC_DIRICHLET):
+            return True
+        if not self.periodic_y and (self.bc_type[BOTTOM] == BC_DIRICHLET or self.bc_type[TOP] == BC_DIRICHLET):
+            return True
+        return False
+
+
+def droplet_density(nx, ny, lx=1.0, ly=1.0, cx=0.5, cy=0.5, radius=0.2, cn=0.01,
+                    rho_in=1000.0, rho_out=1.0, width_factor=math.sqrt(2.0)):
+    """rho = (1+phi)/2*rho_in + (1-phi)/2*rho_out with phi = tanh((R - r)/(width_factor*Cn))."""
+    dx, dy = lx / nx, ly / ny
+    x = (np.arange(nx, dtype=np.float64) + 0.5) * dx
+    y = (np.arange(ny, dtype=np.float64) + 0.5) * dy
+    r = np.sqrt((x[:, None] - cx) ** 2 + (y[None, :] - cy) ** 2)
+    phi = np.tanh((radius - r) / (width_factor * cn))
+    return np.ascontiguousarray((1.0 + phi) / 2.0 * rho_in + (1.0 - phi) / 2.0 * rho_out)
+
+
+def random_faces(nx, ny, seed, periodic_x=False, periodic_y=False):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((nx + 1) * ny).reshape(nx + 1, ny)
+    v = rng.standard_normal(nx * (ny + 1)).reshape(nx, ny + 1)
+    if not periodic_x:
+        u[0, :] = 0.0
+        u[nx, :] = 0.0
+    if not periodic_y:
+        v[:, 0] = 0.0
+        v[:, ny] = 0.0
+    return np.ascontiguousarray(u), np.ascontiguousarray(v)
+
+
+def droplet_case(nx=192, ny=192, seed=0, density_ratio=1000.0, radius=0.2, cx=0.5, cy=0.5, cn=0.01,
+                 lx=1.0, ly=1.0, dt=1.0e-4, tolerance=1.0e-8, max_iterations=20000,
+                 periodic_x=False, periodic_y=False, **kw) -> PressureCase:
+    """A droplet of density ``density_ratio`` in a unit-density bath with a seeded random u*, v*.
+
+    With the defaults this is the 192x192, density-ratio-1000 single-solve workload: Neumann
+    pressure on all four walls, droplet radius 0.2 centred, Cn = 0.01, dt = 1e-4, tol 1e-8.
+    """
+    rho = droplet_density(nx, ny, lx, ly, cx, cy, radius, cn, rho_in=density_ratio, rho_out=1.0)
+    u, v = random_faces(nx, ny, seed, periodic_x, periodic_y)
+    return PressureCase(nx=nx, ny=ny, lx=lx, ly=ly, dt=dt, periodic_x=periodic_x, periodic_y=periodic_y,
+                        tolerance=tolerance, max_iterations=max_iterations, density_ratio=density_ratio,
+                        rho=rho, u=u, v=v, **kw)
+
+
+BATCH_GRIDS = ((64, 64), (96, 192), (128, 128), (192, 192), (256, 256))
+
+
+def batched_cases(count=256, seed=0, tolerance=1.0e-8):
+    """Uneven batch: grid from BATCH_GRIDS, centre U[0.3,0.7]^2, radius U[0.1,0.25],
+    density ratio 10**U[0,3], per-system RHS seed."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(count):
+        nx, ny = BATCH_GRIDS[int(rng.integers(0, len(BATCH_GRIDS)))]
+        cx, cy = rng.uniform(0.3, 0.7, size=2)
+        radius = rng.uniform(0.1, 0.25)
+        dr = 10.0 ** rng.uniform(0.0, 3.0)
+        cases.append(droplet_case(nx, ny, seed=1000 + k, density_ratio=dr, radius=radius, cx=cx, cy=cy,
+                                  tolerance=tolerance))
+    return cases

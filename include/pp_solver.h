@@ -1,0 +1,122 @@
+/*
+ * pp_solver.h — C ABI of the MI355X pressure-projection engine.
+ *
+ * The engine replaces the body of the reference's in-process pressure backends
+ *   Solver::solve_pressure_correction_icpcg          (src/core/coupled/pressure.cpp:302-321)
+ *   Solver::solve_pressure_correction_ildlt_pcg      (pressure.cpp:323-342)
+ *   Solver::solve_pressure_correction_liu_split_icpcg (pressure.cpp:344-365)
+ *   Solver::solve_pressure_correction_liu_split_ildlt_pcg (pressure.cpp:470-491)
+ * reached from the scheme-string switch Solver::solve_pressure_correction (pressure.cpp:1031-1089).
+ * One call assembles the 5-point variable-coefficient operator and the right-hand side from the
+ * Solver's fields, factors it (ICC(0) or ILDLT(0), natural ordering), runs projected PCG from x0 = 0
+ * and returns the interior of pressure_correction_ after the mean subtraction the reference wrappers
+ * apply; ghost filling (apply_pressure_bc, operators.cpp:143-181) stays with the caller.
+ *
+ * Conventions
+ *   - Arrays are double, i-major with j contiguous, like Field2D (src/core/base/field.hpp:23-33).
+ *     Host entry points take a leading dimension (elements between consecutive i) so a caller can
+ *     pass &field(0,0) of a ghosted Field2D with ld = ny + 2*ghost without copying.
+ *   - Return value 0 = success. Non-zero = error; pp_report.message carries the reference's error
+ *     text ("ICC preconditioner construction failed", "pcg breakdown: p^T A p is invalid", ...),
+ *     which the C++ host turns into std::runtime_error like the reference.
+ *   - Not converging within max_iterations is not an error (ch_sparse_krylov.hpp:659).
+ *   - One context per host thread; calls on one context are serialised by the caller.
+ */
+#ifndef PP_SOLVER_H
+#define PP_SOLVER_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scheme codes; names of the new scheme strings in parentheses. */
+enum pp_scheme {
+  PP_ICPCG = 1,           /* hip_icpcg        <- icpcg            */
+  PP_ILDLT_PCG = 2,       /* hip_ildlt_pcg    <- ildlt_pcg        */
+  PP_SPLIT_ICPCG = 3,     /* hip_split_icpcg  <- split_icpcg, liu_split_icpcg, paper_split_icpcg */
+  PP_SPLIT_ILDLT_PCG = 4  /* hip_split_ildlt_pcg <- split_ildlt_pcg, ...                         */
+};
+
+/* Arithmetic of the inner Krylov loop. */
+enum pp_precision {
+  PP_FP64 = 0,  /* the reference's arithmetic, operation for operation                          */
+  PP_MIXED = 1  /* FP32 SpMV/sweeps/updates with FP64-accumulated dots inside FP64 refinement    */
+};
+
+/* Boundary types (config.hpp:23-27). Index order of bc arrays: left, right, bottom, top. */
+enum pp_bc_type { PP_BC_UNSET = 0, PP_BC_DIRICHLET = 1, PP_BC_NEUMANN = 2 };
+
+typedef struct pp_problem {
+  int nx, ny;             /* cells; 1..1024 each, >= 4 along a periodic direction              */
+  double dx, dy;          /* the Solver's dx_, dy_ (lx / nx, ly / ny)                            */
+  double dt;              /* cfg_.dt                                                              */
+  int periodic_x, periodic_y;
+  int bc_type[4];         /* pressure_bc_left/right/bottom/top .type                             */
+  double bc_value[4];     /* ... .value                                                           */
+  int scheme;             /* enum pp_scheme                                                       */
+  int precision;          /* enum pp_precision                                                    */
+  int max_iterations;     /* cfg_.poisson_iterations (values < 1 act as 1)                        */
+  double tolerance;       /* cfg_.pressure_tolerance (relative or absolute residual)              */
+  double density_ratio;   /* split schemes: rho_ref = min(1, density_ratio) (operators.cpp:411)   */
+  int step_index;         /* split schemes: 0 -> p_ext = p^n, else 2 p^n - p^{n-1} (operators.cpp:413-422) */
+  int inner_max_iterations; /* PP_MIXED: cap per inner FP32 solve (<= 0: engine default)         */
+  double inner_tolerance;   /* PP_MIXED: inner relative reduction (<= 0: engine default)         */
+} pp_problem;
+
+typedef struct pp_report {
+  int iterations;            /* last_pressure_iterations_ (mixed: total inner iterations)         */
+  int outer_cycles;          /* PP_MIXED: refinement cycles; PP_FP64: 1                           */
+  double relative_residual;  /* the backend's return value                                        */
+  double absolute_residual;
+  double iterate_residual;   /* LinearSolveReport::iterate_residual                              */
+  int status;                /* 0 ok; see pp_status                                               */
+  char message[256];
+} pp_report;
+
+enum pp_status {
+  PP_OK = 0,
+  PP_ERR_FACTOR = 1,        /* "ICC preconditioner construction failed" / "ILDLT ..."            */
+  PP_ERR_PAP = 2,           /* "pcg breakdown: p^T A p is invalid"                               */
+  PP_ERR_RZ0 = 3,           /* "pcg preconditioner breakdown: non-positive r^T M^{-1} r"         */
+  PP_ERR_RZ = 4,            /* "pcg preconditioner breakdown: non-positive new r^T M^{-1} r"     */
+  PP_ERR_NODIAG = 5,        /* "sparse matrix assembly failed to produce a diagonal entry"       */
+  PP_ERR_SWEEP = 6,         /* internal: wavefront hand-off did not complete                     */
+  PP_ERR_ARG = 100,         /* invalid argument (sizes, scheme, pointers)                       */
+  PP_ERR_HIP = 101          /* HIP runtime failure                                               */
+};
+
+typedef struct pp_ctx pp_ctx;
+
+/* Context bound to one device; owns device buffers, layout tables and a stream. */
+int pp_create(pp_ctx **ctx, int device);
+void pp_destroy(pp_ctx *ctx);
+/* Text of the last error on this context ("" if none). */
+const char *pp_last_error(const pp_ctx *ctx);
+
+/*
+ * One pressure correction from host fields (replaces pressure.cpp:302-365, 470-491 between the
+ * zero-fill of pressure_correction_ and apply_pressure_bc). Synchronous.
+ *   rho    nx x ny      rho_mid_   (ld_rho >= ny)
+ *   u      (nx+1) x ny  u_star_    (ld_u >= ny)
+ *   v      nx x (ny+1)  v_star_    (ld_v >= ny+1)
+ *   p_n, p_nm1  nx x ny pressure_, pressure_previous_step_ (split schemes; NULL = zero)
+ *   p_out  nx x ny      interior of pressure_correction_ (ld_out >= ny)
+ */
+int pp_solve(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, const double *u, int ld_u,
+             const double *v, int ld_v, const double *p_n, int ld_pn, const double *p_nm1, int ld_pnm1,
+             double *p_out, int ld_out, pp_report *report);
+
+/*
+ * Same solve on device-resident, contiguous arrays (rho nx*ny, u (nx+1)*ny, v nx*(ny+1), p_n/p_nm1
+ * nx*ny or NULL, p_out nx*ny). Work is queued on `stream` (a hipStream_t, NULL = the context's
+ * stream); the call waits for completion to fill `report`.
+ */
+int pp_solve_device(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u,
+                    const double *d_v, const double *d_p_n, const double *d_p_nm1, double *d_p_out,
+                    pp_report *report, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PP_SOLVER_H */

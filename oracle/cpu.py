@@ -1,0 +1,171 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so, built from oracle/pp_oracle.cpp).
+
+Checker only: imported by tests/, the smoke check and bench.py's CPU-baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+SCHEMES = {"icpcg": 1, "ildlt_pcg": 2, "split_icpcg": 3, "split_ildlt_pcg": 4}
+SEQUENTIAL, PAIRWISE, CANONICAL = 0, 1, 2
+
+
+class OrProblem(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("dt", ctypes.c_double),
+        ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int),
+        ("bc_type", ctypes.c_int * 4), ("bc_value", ctypes.c_double * 4),
+        ("max_iterations", ctypes.c_int), ("tolerance", ctypes.c_double),
+        ("density_ratio", ctypes.c_double), ("step_index", ctypes.c_int),
+    ]
+
+
+def available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "oracle"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise FileNotFoundError(f"oracle not built: {LIB_PATH} (make -C oracle oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.or_pressure_solve.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), dp, dp, dp, dp, dp, ctypes.c_int,
+                                        dp, ip, dp, ctypes.c_char_p, ctypes.c_int]
+        L.or_pressure_solve.restype = ctypes.c_int
+        L.or_sys_build.argtypes = [ctypes.POINTER(OrProblem), ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_int,
+                                   ctypes.c_char_p, ctypes.c_int]
+        L.or_sys_build.restype = ctypes.c_void_p
+        L.or_sys_free.argtypes = [ctypes.c_void_p]
+        L.or_sys_sizes.argtypes = [ctypes.c_void_p, ip, ip, ip]
+        L.or_sys_csr.argtypes = [ctypes.c_void_p, ip, ip, dp, dp]
+        L.or_sys_factor.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp]
+        L.or_sys_factor.restype = ctypes.c_int
+        L.or_sys_apply_matrix.argtypes = [ctypes.c_void_p, dp, dp]
+        L.or_sys_apply_precond.argtypes = [ctypes.c_void_p, dp, dp]
+        L.or_sys_pcg.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, dp, ip, dp, dp,
+                                 ctypes.c_char_p, ctypes.c_int]
+        L.or_sys_pcg.restype = ctypes.c_int
+        L.or_apply_pressure_bc.argtypes = [ctypes.POINTER(OrProblem), dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def problem(case) -> OrProblem:
+    pb = OrProblem()
+    pb.nx, pb.ny = case.nx, case.ny
+    pb.lx, pb.ly, pb.dt = case.lx, case.ly, case.dt
+    pb.periodic_x, pb.periodic_y = int(case.periodic_x), int(case.periodic_y)
+    for s in range(4):
+        pb.bc_type[s] = int(case.bc_type[s])
+        pb.bc_value[s] = float(case.bc_value[s])
+    pb.max_iterations = case.max_iterations
+    pb.tolerance = case.tolerance
+    pb.density_ratio = case.density_ratio
+    pb.step_index = case.step_index
+    return pb
+
+
+def pressure_solve(scheme: str, case, reduction: int = SEQUENTIAL):
+    """Oracle pressure correction. Returns (p interior (nx*ny,), iterations, relative residual)."""
+    L = lib()
+    p = np.zeros(case.n)
+    it = ctypes.c_int(0)
+    res = ctypes.c_double(0.0)
+    err = ctypes.create_string_buffer(512)
+    rc = L.or_pressure_solve(SCHEMES[scheme], ctypes.byref(problem(case)), _dp(case.rho), _dp(case.u), _dp(case.v),
+                             _dp(case.p_n), _dp(case.p_nm1), reduction, _dp(p), ctypes.byref(it), ctypes.byref(res),
+                             err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return p, it.value, res.value
+
+
+def apply_pressure_bc(case, field_ghosted: np.ndarray) -> np.ndarray:
+    """Ghost fill (3 layers) of a pressure-like Field2D buffer, in place; returns it."""
+    assert field_ghosted.size == (case.nx + 6) * (case.ny + 6)
+    lib().or_apply_pressure_bc(ctypes.byref(problem(case)), _dp(field_ghosted))
+    return field_ghosted
+
+
+class OracleSystem:
+    """The oracle's assembled CSR system for a case, with factor and operator applications."""
+
+    def __init__(self, case, use_split=False, reduction=SEQUENTIAL):
+        L = lib()
+        err = ctypes.create_string_buffer(512)
+        self.h = L.or_sys_build(ctypes.byref(problem(case)), int(use_split), _dp(case.rho), _dp(case.u), _dp(case.v),
+                                _dp(case.p_n), _dp(case.p_nm1), reduction, err, 512)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        n, nnz, hd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.or_sys_sizes(self.h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(hd))
+        self.n, self.nnz, self.has_dirichlet = n.value, nnz.value, bool(hd.value)
+        self.row_ptr = np.zeros(self.n + 1, dtype=np.int32)
+        self.col_idx = np.zeros(self.nnz, dtype=np.int32)
+        self.values = np.zeros(self.nnz)
+        self.rhs = np.zeros(self.n)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.or_sys_csr(self.h, self.row_ptr.ctypes.data_as(ip), self.col_idx.ctypes.data_as(ip), _dp(self.values),
+                     _dp(self.rhs))
+
+    def dense(self) -> np.ndarray:
+        A = np.zeros((self.n, self.n))
+        for r in range(self.n):
+            for q in range(self.row_ptr[r], self.row_ptr[r + 1]):
+                A[r, self.col_idx[q]] = self.values[q]
+        return A
+
+    def factor(self, kind=1):
+        lower = np.zeros(self.nnz)
+        diag = np.zeros(self.n)
+        fl = ctypes.c_double(0.0)
+        ok = lib().or_sys_factor(self.h, kind, _dp(lower), _dp(diag), ctypes.byref(fl))
+        return bool(ok), lower, diag, fl.value
+
+    def apply_matrix(self, x):
+        y = np.zeros(self.n)
+        lib().or_sys_apply_matrix(self.h, _dp(np.ascontiguousarray(x, dtype=np.float64)), _dp(y))
+        return y
+
+    def apply_precond(self, r):
+        z = np.zeros(self.n)
+        lib().or_sys_apply_precond(self.h, _dp(np.ascontiguousarray(r, dtype=np.float64)), _dp(z))
+        return z
+
+    def pcg(self, max_iterations, tolerance):
+        x = np.zeros(self.n)
+        it, rel, ab = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        err = ctypes.create_string_buffer(512)
+        rc = lib().or_sys_pcg(self.h, max_iterations, tolerance, _dp(x), ctypes.byref(it), ctypes.byref(rel),
+                              ctypes.byref(ab), err, 512)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        return x, it.value, rel.value, ab.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_sys_free(self.h)
+            self.h = None

@@ -1,0 +1,682 @@
+// CPU oracle for the pressure-projection path. TEST INFRASTRUCTURE ONLY.
+//
+// A single-threaded restatement of the reference's variable-coefficient pressure solve, written
+// for checking the HIP engine: only tests/, the smoke check and bench.py's CPU-baseline leg may
+// load it. It follows, operation for operation, the reference functions cited at each routine
+// (paths relative to the reference repository):
+//   assembly        src/core/coupled/pressure.cpp:123-196, operators.cpp:330-350, 388-390, 411-444
+//   regularization  pressure.cpp:198-212
+//   CSR             src/core/linear_algebra/ch_sparse_krylov.hpp:105-133
+//   ICC(0)/ILDLT(0) ch_sparse_krylov.hpp:337-419 / 252-335
+//   sweeps, SpMV    ch_sparse_krylov.hpp:444-513 / 235-250
+//   PCG             ch_sparse_krylov.hpp:515-660 with pressure.cpp:225-239 projections
+//   scatter         pressure.cpp:241-248, operators.cpp:447-460, ghost fill operators.cpp:143-181
+// Arithmetic is unfused double precision (-ffp-contract=off) in the reference's order, so with
+// sequential reductions it reproduces the reference bit for bit. The reduction mode can be
+// switched to a pairwise tree to measure how strongly the result depends on summation order.
+//
+// Pinning: bitwise against the compiled reference (oracle/_ref) and the committed golden vectors
+// in tests/golden/, plus closed-form invariants (symmetry, A*1 = 0, IC(0) pattern identity, CG
+// residual identity) in tests/test_oracle.py.
+//
+// Extras with no reference counterpart (mixed-precision refinement and the adjoint) are defined
+// in this file too, in FP64, as the checkers of the engine's new functions.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+extern "C" {
+
+struct OrProblem {
+  int nx, ny;
+  double lx, ly, dt;
+  int periodic_x, periodic_y;
+  int bc_type[4];  // left, right, bottom, top; 0 unset, 1 dirichlet, 2 neumann
+  double bc_value[4];
+  int max_iterations;
+  double tolerance;
+  double density_ratio;
+  int step_index;
+};
+
+}  // extern "C"
+
+namespace {
+
+enum Side { kLeft = 0, kRight = 1, kBottom = 2, kTop = 3 };
+enum BcType { kUnset = 0, kDirichlet = 1, kNeumann = 2 };
+enum Reduction { kSequential = 0, kPairwise = 1, kCanonical = 2 };
+
+// Summation orders. kSequential is std::accumulate / dot_product (the reference). kPairwise is a
+// recursive halving tree over natural order. kCanonical is the HIP engine's block reduction:
+// positions of the wavefront layout (level d = i + j ascending, i ascending within a level) are
+// dealt round-robin to 1024 threads, each thread sums its positions in order from 0.0, lanes of a
+// 64-wide wave combine with offsets 32..1 (v[l] += v[l + off]) and the 16 wave sums combine with
+// offsets 8..1.
+struct Reducer {
+  int mode = kSequential;
+  std::vector<int> perm;  // kCanonical: natural index of each wavefront position
+
+  Reducer(int m, int nx, int ny) : mode(m) {
+    if (mode == kCanonical) {
+      perm.reserve(static_cast<std::size_t>(nx) * ny);
+      for (int d = 0; d < nx + ny - 1; ++d)
+        for (int i = std::max(0, d - ny + 1); i <= std::min(d, nx - 1); ++i) perm.push_back(i * ny + (d - i));
+    }
+  }
+
+  template <class F>
+  double sum(std::size_t n, F f) const {
+    if (mode == kSequential) {
+      double s = 0.0;
+      for (std::size_t k = 0; k < n; ++k) s += f(k);
+      return s;
+    }
+    if (mode == kPairwise) {
+      std::vector<double> t(n);
+      for (std::size_t k = 0; k < n; ++k) t[k] = f(k);
+      std::size_t len = n;
+      while (len > 1) {
+        const std::size_t half = (len + 1) / 2;
+        for (std::size_t k = 0; k + half < len; ++k) t[k] = t[k] + t[k + half];
+        len = half;
+      }
+      return n ? t[0] : 0.0;
+    }
+    constexpr std::size_t T = 1024, W = 16;
+    double part[T];
+    for (std::size_t t = 0; t < T; ++t) {
+      double s = 0.0;
+      for (std::size_t q = t; q < n; q += T) s = s + f(static_cast<std::size_t>(perm[q]));
+      part[t] = s;
+    }
+    double wsum[W];
+    for (std::size_t w = 0; w < W; ++w) {
+      double *v = part + 64 * w;
+      for (std::size_t off = 32; off >= 1; off >>= 1)
+        for (std::size_t l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+      wsum[w] = v[0];
+    }
+    for (std::size_t off = W / 2; off >= 1; off >>= 1)
+      for (std::size_t t = 0; t < off; ++t) wsum[t] = wsum[t] + wsum[t + off];
+    return wsum[0];
+  }
+};
+
+struct Bc {
+  int type;
+  double value;
+};
+
+struct Grid {
+  const OrProblem &pb;
+  double dx, dy;
+  explicit Grid(const OrProblem &p)
+      : pb(p), dx(p.lx / std::max(static_cast<double>(p.nx), 1.0)), dy(p.ly / std::max(static_cast<double>(p.ny), 1.0)) {}
+
+  // Unset edges act as homogeneous Neumann (operators.cpp:36-60).
+  Bc bc(int side) const {
+    if (pb.bc_type[side] != kUnset) return {pb.bc_type[side], pb.bc_value[side]};
+    return {kNeumann, 0.0};
+  }
+  // operators.cpp:114-128
+  bool has_dirichlet() const {
+    if (!pb.periodic_x && (bc(kLeft).type == kDirichlet || bc(kRight).type == kDirichlet)) return true;
+    if (!pb.periodic_y && (bc(kBottom).type == kDirichlet || bc(kTop).type == kDirichlet)) return true;
+    return false;
+  }
+};
+
+// Cell-centred field accessor over an interior-only i-major array.
+struct CellView {
+  const double *a;
+  int ny;
+  double operator()(int i, int j) const { return a[static_cast<std::size_t>(i) * ny + j]; }
+};
+
+// Face densities: arithmetic mean of the two cells, wall faces take the adjacent cell, periodic
+// faces average across the wrap (operators.cpp:330-350).
+double rho_xface(const Grid &g, const CellView &rho, int i, int j) {
+  const int nx = g.pb.nx;
+  if (i <= 0) return g.pb.periodic_x ? 0.5 * (rho(nx - 1, j) + rho(0, j)) : rho(0, j);
+  if (i >= nx) return g.pb.periodic_x ? 0.5 * (rho(nx - 1, j) + rho(0, j)) : rho(nx - 1, j);
+  return 0.5 * (rho(i - 1, j) + rho(i, j));
+}
+
+double rho_yface(const Grid &g, const CellView &rho, int i, int j) {
+  const int ny = g.pb.ny;
+  if (j <= 0) return g.pb.periodic_y ? 0.5 * (rho(i, ny - 1) + rho(i, 0)) : rho(i, 0);
+  if (j >= ny) return g.pb.periodic_y ? 0.5 * (rho(i, ny - 1) + rho(i, 0)) : rho(i, ny - 1);
+  return 0.5 * (rho(i, j - 1) + rho(i, j));
+}
+
+// (u(i+1,j)-u(i,j))/dx + (v(i,j+1)-v(i,j))/dy on the staggered faces (operators.cpp:388-390).
+double cell_divergence(const Grid &g, const double *u, const double *v, int i, int j) {
+  const int ny = g.pb.ny;
+  const double ue = u[static_cast<std::size_t>(i + 1) * ny + j], uw = u[static_cast<std::size_t>(i) * ny + j];
+  const double vn = v[static_cast<std::size_t>(i) * (ny + 1) + j + 1], vs = v[static_cast<std::size_t>(i) * (ny + 1) + j];
+  return (ue - uw) / g.dx + (vn - vs) / g.dy;
+}
+
+// Pressure value with periodic wrap; only wrapped neighbours are ever read outside the interior
+// (the non-periodic wall fluxes are zero), so ghost layers reduce to the wrap (operators.cpp:413-444).
+double wrap_value(const Grid &g, const std::vector<double> &p, int i, int j) {
+  const int nx = g.pb.nx, ny = g.pb.ny;
+  if (i < 0) i += nx;
+  if (i >= nx) i -= nx;
+  if (j < 0) j += ny;
+  if (j >= ny) j -= ny;
+  return p[static_cast<std::size_t>(i) * ny + j];
+}
+
+// Split explicit term sum over faces of (1 - rho_ref/rho_f) * grad(p_ext) (operators.cpp:424-444).
+double split_explicit_divergence(const Grid &g, const CellView &rho, const std::vector<double> &pext, double rho_ref,
+                                 int i, int j) {
+  const OrProblem &pb = g.pb;
+  const bool east_open = pb.periodic_x || i < pb.nx - 1;
+  const bool west_open = pb.periodic_x || i > 0;
+  const bool north_open = pb.periodic_y || j < pb.ny - 1;
+  const bool south_open = pb.periodic_y || j > 0;
+  const double pc = pext[static_cast<std::size_t>(i) * pb.ny + j];
+  const double fe = east_open ? (1.0 - rho_ref / rho_xface(g, rho, i + 1, j)) * ((wrap_value(g, pext, i + 1, j) - pc) / g.dx) : 0.0;
+  const double fw = west_open ? (1.0 - rho_ref / rho_xface(g, rho, i, j)) * ((pc - wrap_value(g, pext, i - 1, j)) / g.dx) : 0.0;
+  const double fn = north_open ? (1.0 - rho_ref / rho_yface(g, rho, i, j + 1)) * ((wrap_value(g, pext, i, j + 1) - pc) / g.dy) : 0.0;
+  const double fs = south_open ? (1.0 - rho_ref / rho_yface(g, rho, i, j)) * ((pc - wrap_value(g, pext, i, j - 1)) / g.dy) : 0.0;
+  return (fe - fw) / g.dx + (fn - fs) / g.dy;
+}
+
+struct Csr {
+  int n = 0;
+  std::vector<int> row_ptr, col, diag_pos;
+  std::vector<double> val;
+};
+
+struct System {
+  Csr A;
+  std::vector<double> rhs;
+  bool has_dirichlet = false;
+  double assembly_max_diag = 0.0;
+};
+
+// Assembly + regularization + CSR conversion (pressure.cpp:123-212, ch_sparse_krylov.hpp:105-133).
+// Each row gathers its entries in the reference's insertion order (east, west, north, south,
+// centre), accumulating repeated columns in that order, then sorts by column as std::map would.
+System assemble(const OrProblem &pb, const double *rho_in, const double *u, const double *v, const double *p_n,
+                const double *p_nm1, bool use_split, const Reducer &R) {
+  Grid g(pb);
+  const int nx = pb.nx, ny = pb.ny, n = nx * ny;
+  const CellView rho{rho_in, ny};
+  System sys;
+  sys.has_dirichlet = g.has_dirichlet();
+  sys.rhs.assign(static_cast<std::size_t>(n), 0.0);
+  const double rho_ref = std::min(1.0, pb.density_ratio);
+
+  std::vector<double> pext;
+  if (use_split) {
+    pext.assign(static_cast<std::size_t>(n), 0.0);
+    for (int k = 0; k < n; ++k) {
+      const double pn = p_n ? p_n[k] : 0.0, pm = p_nm1 ? p_nm1[k] : 0.0;
+      pext[static_cast<std::size_t>(k)] = (pb.step_index == 0) ? pn : (2.0 * pn - pm);
+    }
+  }
+
+  Csr &A = sys.A;
+  A.n = n;
+  A.row_ptr.assign(1, 0);
+  A.diag_pos.assign(static_cast<std::size_t>(n), -1);
+  double max_diag = 0.0;
+  for (int i = 0; i < nx; ++i) {
+    for (int j = 0; j < ny; ++j) {
+      const int row = i * ny + j;
+      const bool has_e = pb.periodic_x || i < nx - 1;
+      const bool has_w = pb.periodic_x || i > 0;
+      const bool has_n = pb.periodic_y || j < ny - 1;
+      const bool has_s = pb.periodic_y || j > 0;
+      const double ce = use_split ? 1.0 / (g.dx * g.dx) : 1.0 / (rho_xface(g, rho, i + 1, j) * g.dx * g.dx);
+      const double cw = use_split ? 1.0 / (g.dx * g.dx) : 1.0 / (rho_xface(g, rho, i, j) * g.dx * g.dx);
+      const double cn = use_split ? 1.0 / (g.dy * g.dy) : 1.0 / (rho_yface(g, rho, i, j + 1) * g.dy * g.dy);
+      const double cs = use_split ? 1.0 / (g.dy * g.dy) : 1.0 / (rho_yface(g, rho, i, j) * g.dy * g.dy);
+      double diag = 0.0;
+      double b = use_split ? -(split_explicit_divergence(g, rho, pext, rho_ref, i, j) +
+                               rho_ref * cell_divergence(g, u, v, i, j) / pb.dt)
+                           : -cell_divergence(g, u, v, i, j) / pb.dt;
+
+      int cols[5];
+      double vals[5];
+      int m = 0;
+      auto put = [&](int c, double value) {
+        for (int k = 0; k < m; ++k) {
+          if (cols[k] == c) {
+            vals[k] += value;
+            return;
+          }
+        }
+        cols[m] = c;
+        vals[m] = 0.0 + value;
+        ++m;
+      };
+      auto boundary = [&](int side, double coeff, double spacing) {
+        const Bc bc = g.bc(side);
+        if (bc.type == kDirichlet) {
+          diag += 2.0 * coeff;
+          b += 2.0 * coeff * bc.value;
+        } else {
+          b += coeff * bc.value * spacing;
+        }
+      };
+      if (has_e) {
+        diag += ce;
+        put((pb.periodic_x && i == nx - 1 ? 0 : i + 1) * ny + j, -ce);
+      } else {
+        boundary(kRight, ce, g.dx);
+      }
+      if (has_w) {
+        diag += cw;
+        put((pb.periodic_x && i == 0 ? nx - 1 : i - 1) * ny + j, -cw);
+      } else {
+        boundary(kLeft, cw, g.dx);
+      }
+      if (has_n) {
+        diag += cn;
+        put(i * ny + (pb.periodic_y && j == ny - 1 ? 0 : j + 1), -cn);
+      } else {
+        boundary(kTop, cn, g.dy);
+      }
+      if (has_s) {
+        diag += cs;
+        put(i * ny + (pb.periodic_y && j == 0 ? ny - 1 : j - 1), -cs);
+      } else {
+        boundary(kBottom, cs, g.dy);
+      }
+      put(row, diag);
+      sys.rhs[static_cast<std::size_t>(row)] = b;
+      max_diag = std::max(max_diag, diag);
+
+      // sort by column (insertion sort on <= 5 entries), then drop |v| < 1e-18 like finalize_row_maps
+      for (int a = 1; a < m; ++a)
+        for (int k = a; k > 0 && cols[k - 1] > cols[k]; --k) {
+          std::swap(cols[k - 1], cols[k]);
+          std::swap(vals[k - 1], vals[k]);
+        }
+      for (int k = 0; k < m; ++k) {
+        if (std::abs(vals[k]) < 1.0e-18) continue;
+        A.col.push_back(cols[k]);
+        A.val.push_back(vals[k]);
+        if (cols[k] == row) A.diag_pos[static_cast<std::size_t>(row)] = static_cast<int>(A.val.size()) - 1;
+      }
+      A.row_ptr.push_back(static_cast<int>(A.val.size()));
+    }
+  }
+  for (int r = 0; r < n; ++r)
+    if (A.diag_pos[static_cast<std::size_t>(r)] < 0)
+      throw std::runtime_error("sparse matrix assembly failed to produce a diagonal entry");
+  sys.assembly_max_diag = max_diag;
+
+  if (!sys.has_dirichlet) {
+    const double total = R.sum(sys.rhs.size(), [&](std::size_t k) { return sys.rhs[k]; });
+    const double mean = total / std::max(static_cast<double>(n), 1.0);
+    for (double &x : sys.rhs) x -= mean;
+  }
+  return sys;
+}
+
+struct Factor {
+  int kind = 1;  // 1 ICC(0), 2 ILDLT(0)
+  std::vector<double> lower, diag;
+  double floor = 1.0e-30;
+  // transpose lists: for column c, the (row, L value) pairs of the rows below it, ascending rows
+  std::vector<int> t_ptr, t_row;
+  std::vector<double> t_val;
+};
+
+// ICC(0) (ch_sparse_krylov.hpp:337-419) and ILDLT(0) (252-335) in natural order on the CSR
+// pattern. Returns false when a pivot is non-finite or at/below the floor.
+bool factorize(const Csr &A, int kind, Factor &F) {
+  const int n = A.n;
+  F.kind = kind;
+  F.lower.assign(A.val.size(), 0.0);
+  F.diag.assign(static_cast<std::size_t>(n), 0.0);
+  double max_diag = 0.0;
+  for (int r = 0; r < n; ++r) max_diag = std::max(max_diag, std::abs(A.val[static_cast<std::size_t>(A.diag_pos[static_cast<std::size_t>(r)])]));
+  F.floor = (kind == 2) ? std::max(1.0, max_diag) * 1.0e-12 : std::sqrt(std::max(1.0, max_diag) * 1.0e-12);
+  const double floor_sq = F.floor * F.floor;
+
+  // L(row, target) if that strictly-lower entry exists, else 0 (scan stops past the target).
+  auto lookup = [&](int row, int target) {
+    for (int q = A.row_ptr[static_cast<std::size_t>(row)]; q < A.row_ptr[static_cast<std::size_t>(row) + 1]; ++q) {
+      const int c = A.col[static_cast<std::size_t>(q)];
+      if (c >= row || c > target) break;
+      if (c == target) return F.lower[static_cast<std::size_t>(q)];
+    }
+    return 0.0;
+  };
+
+  for (int r = 0; r < n; ++r) {
+    const int beg = A.row_ptr[static_cast<std::size_t>(r)], end = A.row_ptr[static_cast<std::size_t>(r) + 1];
+    for (int q = beg; q < end; ++q) {
+      const int c = A.col[static_cast<std::size_t>(q)];
+      if (c >= r) break;
+      double s = A.val[static_cast<std::size_t>(q)];
+      for (int q2 = beg; q2 < q; ++q2) {
+        const int k = A.col[static_cast<std::size_t>(q2)];
+        if (k >= c) break;
+        const double ljk = lookup(c, k);
+        if (ljk != 0.0) {
+          if (kind == 2)
+            s -= F.lower[static_cast<std::size_t>(q2)] * F.diag[static_cast<std::size_t>(k)] * ljk;
+          else
+            s -= F.lower[static_cast<std::size_t>(q2)] * ljk;
+        }
+      }
+      F.lower[static_cast<std::size_t>(q)] = s / std::max(F.diag[static_cast<std::size_t>(c)], F.floor);
+    }
+    double d = A.val[static_cast<std::size_t>(A.diag_pos[static_cast<std::size_t>(r)])];
+    for (int q = beg; q < end; ++q) {
+      const int c = A.col[static_cast<std::size_t>(q)];
+      if (c >= r) break;
+      const double l = F.lower[static_cast<std::size_t>(q)];
+      if (kind == 2)
+        d -= (l * l) * F.diag[static_cast<std::size_t>(c)];
+      else
+        d -= l * l;
+    }
+    if (kind == 2) {
+      if (!std::isfinite(d) || d <= F.floor) return false;
+      F.diag[static_cast<std::size_t>(r)] = d;
+    } else {
+      if (!std::isfinite(d) || d <= floor_sq) return false;
+      F.diag[static_cast<std::size_t>(r)] = std::sqrt(d);
+    }
+  }
+
+  // column-wise transpose of the strictly lower part, rows ascending within each column
+  F.t_ptr.assign(static_cast<std::size_t>(n) + 1, 0);
+  for (int r = 0; r < n; ++r)
+    for (int q = A.row_ptr[static_cast<std::size_t>(r)]; q < A.row_ptr[static_cast<std::size_t>(r) + 1]; ++q)
+      if (A.col[static_cast<std::size_t>(q)] < r) ++F.t_ptr[static_cast<std::size_t>(A.col[static_cast<std::size_t>(q)]) + 1];
+  for (int c = 0; c < n; ++c) F.t_ptr[static_cast<std::size_t>(c) + 1] += F.t_ptr[static_cast<std::size_t>(c)];
+  F.t_row.assign(static_cast<std::size_t>(F.t_ptr[static_cast<std::size_t>(n)]), 0);
+  F.t_val.assign(F.t_row.size(), 0.0);
+  std::vector<int> fill(F.t_ptr.begin(), F.t_ptr.end() - 1);
+  for (int r = 0; r < n; ++r)
+    for (int q = A.row_ptr[static_cast<std::size_t>(r)]; q < A.row_ptr[static_cast<std::size_t>(r) + 1]; ++q) {
+      const int c = A.col[static_cast<std::size_t>(q)];
+      if (c < r) {
+        const int at = fill[static_cast<std::size_t>(c)]++;
+        F.t_row[static_cast<std::size_t>(at)] = r;
+        F.t_val[static_cast<std::size_t>(at)] = F.lower[static_cast<std::size_t>(q)];
+      }
+    }
+  return true;
+}
+
+// y = A x, row sums in column order (ch_sparse_krylov.hpp:235-250).
+void spmv(const Csr &A, const double *x, double *y) {
+  for (int r = 0; r < A.n; ++r) {
+    double s = 0.0;
+    for (int q = A.row_ptr[static_cast<std::size_t>(r)]; q < A.row_ptr[static_cast<std::size_t>(r) + 1]; ++q)
+      s += A.val[static_cast<std::size_t>(q)] * x[A.col[static_cast<std::size_t>(q)]];
+    y[r] = s;
+  }
+}
+
+// z = M^{-1} r by forward and backward sweeps (ch_sparse_krylov.hpp:444-513).
+void precondition(const Csr &A, const Factor &F, const double *r, double *z) {
+  const int n = A.n;
+  std::vector<double> y(static_cast<std::size_t>(n), 0.0);
+  for (int row = 0; row < n; ++row) {
+    double s = r[row];
+    for (int q = A.row_ptr[static_cast<std::size_t>(row)]; q < A.row_ptr[static_cast<std::size_t>(row) + 1]; ++q) {
+      const int c = A.col[static_cast<std::size_t>(q)];
+      if (c >= row) break;
+      s -= F.lower[static_cast<std::size_t>(q)] * y[static_cast<std::size_t>(c)];
+    }
+    y[static_cast<std::size_t>(row)] = (F.kind == 2) ? s : s / std::max(F.diag[static_cast<std::size_t>(row)], F.floor);
+  }
+  if (F.kind == 2)
+    for (int row = 0; row < n; ++row) y[static_cast<std::size_t>(row)] /= std::max(F.diag[static_cast<std::size_t>(row)], F.floor);
+  for (int row = 0; row < n; ++row) z[row] = y[static_cast<std::size_t>(row)];
+  for (int row = n - 1; row >= 0; --row) {
+    double s = z[row];
+    for (int t = F.t_ptr[static_cast<std::size_t>(row)]; t < F.t_ptr[static_cast<std::size_t>(row) + 1]; ++t)
+      s -= F.t_val[static_cast<std::size_t>(t)] * z[F.t_row[static_cast<std::size_t>(t)]];
+    z[row] = (F.kind == 2) ? s : s / std::max(F.diag[static_cast<std::size_t>(row)], F.floor);
+  }
+}
+
+struct Report {
+  int iterations = 0;
+  double abs_res = 0.0, rel_res = 0.0, iterate_res = 0.0;
+};
+
+// Zero-mean projection (pressure.cpp:225-239).
+void project(std::vector<double> &w, const Reducer &R) {
+  if (w.empty()) return;
+  const double mean = R.sum(w.size(), [&](std::size_t k) { return w[k]; }) /
+                      std::max(static_cast<double>(w.size()), 1.0);
+  for (double &x : w) x -= mean;
+}
+
+// PCG from x0 = 0 (ch_sparse_krylov.hpp:515-660, pressure.cpp:50-57).
+Report pcg(const Csr &A, const Factor &F, const std::vector<double> &b, int max_iterations, double tol, bool projected,
+           const Reducer &R, std::vector<double> &x) {
+  const int n = A.n;
+  if (n <= 0) throw std::runtime_error("cannot solve an empty linear system");
+  auto dot = [&](const std::vector<double> &a, const std::vector<double> &c) {
+    return R.sum(static_cast<std::size_t>(n), [&](std::size_t k) { return a[k] * c[k]; });
+  };
+  x.assign(static_cast<std::size_t>(n), 0.0);
+  std::vector<double> ax(static_cast<std::size_t>(n));
+  spmv(A, x.data(), ax.data());
+  std::vector<double> r(static_cast<std::size_t>(n));
+  for (int k = 0; k < n; ++k) r[static_cast<std::size_t>(k)] = b[static_cast<std::size_t>(k)] - ax[static_cast<std::size_t>(k)];
+  if (projected) project(r, R);
+  const double bnorm = std::sqrt(std::max(dot(b, b), 0.0));
+  Report rep;
+  rep.abs_res = std::sqrt(std::max(dot(r, r), 0.0));
+  rep.rel_res = rep.abs_res / std::max(bnorm, 1.0e-30);
+  if (rep.rel_res < tol || rep.abs_res < tol) return rep;
+
+  std::vector<double> z(static_cast<std::size_t>(n));
+  precondition(A, F, r.data(), z.data());
+  if (projected) project(z, R);
+  double rz = dot(r, z);
+  if (!std::isfinite(rz) || rz <= 0.0) throw std::runtime_error("pcg preconditioner breakdown: non-positive r^T M^{-1} r");
+  std::vector<double> p(z), ap(static_cast<std::size_t>(n));
+  const int limit = std::max(1, max_iterations);
+  for (int it = 0; it < limit; ++it) {
+    spmv(A, p.data(), ap.data());
+    const double pap = dot(p, ap);
+    if (!std::isfinite(pap) || std::abs(pap) < 1.0e-30) throw std::runtime_error("pcg breakdown: p^T A p is invalid");
+    const double alpha = rz / pap;
+    double upd = 0.0, xsq = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const std::size_t s = static_cast<std::size_t>(k);
+      const double delta = alpha * p[s];
+      x[s] += delta;
+      r[s] -= alpha * ap[s];
+      upd += delta * delta;
+      xsq += x[s] * x[s];
+    }
+    if (R.mode != kSequential) {
+      upd = R.sum(static_cast<std::size_t>(n), [&](std::size_t k) {
+        const double delta = alpha * p[k];
+        return delta * delta;
+      });
+      xsq = R.sum(static_cast<std::size_t>(n), [&](std::size_t k) { return x[k] * x[k]; });
+    }
+    if (projected) {
+      project(x, R);
+      project(r, R);
+    }
+    rep.iterations = it + 1;
+    rep.iterate_res = std::sqrt(upd / std::max(xsq, 1.0e-30));
+    rep.abs_res = std::sqrt(std::max(dot(r, r), 0.0));
+    rep.rel_res = rep.abs_res / std::max(bnorm, 1.0e-30);
+    if (rep.rel_res < tol || rep.abs_res < tol) break;
+    precondition(A, F, r.data(), z.data());
+    if (projected) project(z, R);
+    const double rz_new = dot(r, z);
+    if (!std::isfinite(rz_new) || rz_new <= 0.0)
+      throw std::runtime_error("pcg preconditioner breakdown: non-positive new r^T M^{-1} r");
+    const double beta = rz_new / rz;
+    for (int k = 0; k < n; ++k) {
+      const std::size_t s = static_cast<std::size_t>(k);
+      p[s] = z[s] + beta * p[s];
+    }
+    if (projected) project(p, R);
+    rz = rz_new;
+  }
+  return rep;
+}
+
+void set_err(char *err, int len, const char *msg) {
+  if (err && len > 0) std::snprintf(err, static_cast<std::size_t>(len), "%s", msg);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Scheme codes shared with the engine: 1 icpcg, 2 ildlt_pcg, 3 split_icpcg, 4 split_ildlt_pcg.
+// Full pressure correction: assemble, factor, PCG, scatter, subtract the interior mean (when no
+// Dirichlet edge). p_out receives the interior (i*ny+j). Returns 0 or 1 (message in err).
+int or_pressure_solve(int scheme, const OrProblem *pb, const double *rho, const double *u, const double *v,
+                      const double *p_n, const double *p_nm1, int reduction, double *p_out, int *iterations,
+                      double *residual, char *err, int errlen) {
+  try {
+    const bool split = scheme == 3 || scheme == 4;
+    const int kind = (scheme == 2 || scheme == 4) ? 2 : 1;
+    const Reducer R(reduction, pb->nx, pb->ny);
+    System sys = assemble(*pb, rho, u, v, p_n, p_nm1, split, R);
+    Factor F;
+    if (!factorize(sys.A, kind, F))
+      throw std::runtime_error(kind == 2 ? "ILDLT preconditioner construction failed"
+                                         : "ICC preconditioner construction failed");
+    std::vector<double> x;
+    Report rep = pcg(sys.A, F, sys.rhs, pb->max_iterations, pb->tolerance, !sys.has_dirichlet, R, x);
+    const int n = pb->nx * pb->ny;
+    for (int k = 0; k < n; ++k) p_out[k] = x[static_cast<std::size_t>(k)];
+    if (!sys.has_dirichlet) {
+      // subtract_mean: i-outer, j-inner accumulation (operators.cpp:447-460)
+      double mean = R.sum(static_cast<std::size_t>(n), [&](std::size_t k) { return p_out[k]; });
+      mean /= static_cast<double>(n);
+      for (int k = 0; k < n; ++k) p_out[k] -= mean;
+    }
+    *iterations = rep.iterations;
+    *residual = rep.rel_res;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// ---- system-level access for tests ------------------------------------------------------------
+struct OrSystem {
+  System sys;
+  Factor F;
+  Reducer R{kSequential, 1, 1};
+};
+
+void *or_sys_build(const OrProblem *pb, int use_split, const double *rho, const double *u, const double *v,
+                   const double *p_n, const double *p_nm1, int reduction, char *err, int errlen) {
+  try {
+    auto *h = new OrSystem();
+    h->R = Reducer(reduction, pb->nx, pb->ny);
+    h->sys = assemble(*pb, rho, u, v, p_n, p_nm1, use_split != 0, h->R);
+    return h;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return nullptr;
+  }
+}
+
+void or_sys_free(void *h) { delete static_cast<OrSystem *>(h); }
+
+void or_sys_sizes(void *h, int *n, int *nnz, int *has_dirichlet) {
+  auto *o = static_cast<OrSystem *>(h);
+  *n = o->sys.A.n;
+  *nnz = static_cast<int>(o->sys.A.val.size());
+  *has_dirichlet = o->sys.has_dirichlet ? 1 : 0;
+}
+
+void or_sys_csr(void *h, int *row_ptr, int *col, double *val, double *rhs) {
+  auto *o = static_cast<OrSystem *>(h);
+  std::copy(o->sys.A.row_ptr.begin(), o->sys.A.row_ptr.end(), row_ptr);
+  std::copy(o->sys.A.col.begin(), o->sys.A.col.end(), col);
+  std::copy(o->sys.A.val.begin(), o->sys.A.val.end(), val);
+  std::copy(o->sys.rhs.begin(), o->sys.rhs.end(), rhs);
+}
+
+int or_sys_factor(void *h, int kind, double *lower, double *diag, double *floor_out) {
+  auto *o = static_cast<OrSystem *>(h);
+  const bool ok = factorize(o->sys.A, kind, o->F);
+  if (lower) std::copy(o->F.lower.begin(), o->F.lower.end(), lower);
+  if (diag) std::copy(o->F.diag.begin(), o->F.diag.end(), diag);
+  if (floor_out) *floor_out = o->F.floor;
+  return ok ? 1 : 0;
+}
+
+void or_sys_apply_matrix(void *h, const double *x, double *y) { spmv(static_cast<OrSystem *>(h)->sys.A, x, y); }
+
+void or_sys_apply_precond(void *h, const double *r, double *z) {
+  auto *o = static_cast<OrSystem *>(h);
+  precondition(o->sys.A, o->F, r, z);
+}
+
+int or_sys_pcg(void *h, int max_iterations, double tolerance, double *x, int *iterations, double *rel, double *abs_res,
+               char *err, int errlen) {
+  auto *o = static_cast<OrSystem *>(h);
+  try {
+    std::vector<double> xv;
+    const Report rep = pcg(o->sys.A, o->F, o->sys.rhs, max_iterations, tolerance, !o->sys.has_dirichlet, o->R, xv);
+    std::copy(xv.begin(), xv.end(), x);
+    *iterations = rep.iterations;
+    *rel = rep.rel_res;
+    *abs_res = rep.abs_res;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// Ghost fill of a pressure-like field with 3 ghost layers (operators.cpp:143-181).
+void or_apply_pressure_bc(const OrProblem *pb, double *f) {
+  const Grid g(*pb);
+  const int nx = pb->nx, ny = pb->ny, G = 3, ld = ny + 2 * G;
+  auto at = [&](int i, int j) -> double & { return f[static_cast<std::size_t>(i + G) * ld + (j + G)]; };
+  const Bc L = g.bc(kLeft), Rb = g.bc(kRight), B = g.bc(kBottom), T = g.bc(kTop);
+  for (int k = 1; k <= G; ++k) {
+    for (int j = 0; j < ny; ++j) {
+      if (pb->periodic_x) {
+        at(-k, j) = at(nx - k, j);
+        at(nx - 1 + k, j) = at(k - 1, j);
+      } else {
+        const double lo = 2.0 * (static_cast<double>(k) - 0.5) * L.value * g.dx;
+        const double ro = 2.0 * (static_cast<double>(k) - 0.5) * Rb.value * g.dx;
+        at(-k, j) = (L.type == kDirichlet) ? 2.0 * L.value - at(k - 1, j) : at(k - 1, j) + lo;
+        at(nx - 1 + k, j) = (Rb.type == kDirichlet) ? 2.0 * Rb.value - at(nx - k, j) : at(nx - k, j) + ro;
+      }
+    }
+    for (int i = -G; i < nx + G; ++i) {
+      if (pb->periodic_y) {
+        at(i, -k) = at(i, ny - k);
+        at(i, ny - 1 + k) = at(i, k - 1);
+      } else {
+        const double bo = 2.0 * (static_cast<double>(k) - 0.5) * B.value * g.dy;
+        const double to = 2.0 * (static_cast<double>(k) - 0.5) * T.value * g.dy;
+        at(i, -k) = (B.type == kDirichlet) ? 2.0 * B.value - at(i, k - 1) : at(i, k - 1) + bo;
+        at(i, ny - 1 + k) = (T.type == kDirichlet) ? 2.0 * T.value - at(i, ny - k) : at(i, ny - k) + to;
+      }
+    }
+  }
+}
+
+}  // extern "C"

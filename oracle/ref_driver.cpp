@@ -1,0 +1,239 @@
+// Checker-only driver over the reference solver compiled from its own sources.
+//
+// This file is test infrastructure: it is linked against the reference's src/core/*.cpp by
+// oracle/Makefile into oracle/_ref/libdingref.so, and only tests/, the smoke check and the
+// CPU-baseline leg of bench.py load that library. It reaches the Solver's private members the
+// same way the reference's own unit tests do (src/tests/core_solver_unit_tests.cpp:7-9).
+//
+// Entry points (extern "C", plain pointers):
+//   ref_pressure_solve   one call of Solver::solve_pressure_correction() (pressure.cpp:1031-1089)
+//   ref_sys_*            the assembled CSR system, the ICC/ILDLT factors, one SpMV, one
+//                        preconditioner application and one PCG solve, taken from the reference's
+//                        own functions (pressure.cpp:123-248, ch_sparse_krylov.hpp:105-660).
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <functional>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <sys/types.h>
+#include <vector>
+
+#define private public
+#include "core/coupled/solver.hpp"
+#undef private
+
+extern "C" {
+
+struct RefProblem {
+  int nx, ny;
+  double lx, ly, dt;
+  int periodic_x, periodic_y;
+  int bc_type[4];     // left, right, bottom, top; 0 unset, 1 dirichlet, 2 neumann
+  double bc_value[4];
+  int max_iterations;
+  double tolerance;
+  double density_ratio;
+  int step_index;
+};
+
+}  // extern "C"
+
+namespace {
+
+ding::Config make_config(const RefProblem &pb, const char *scheme) {
+  ding::Config cfg;
+  cfg.name = "oracle_ref";
+  cfg.nx = pb.nx;
+  cfg.ny = pb.ny;
+  cfg.ghost = 3;
+  cfg.lx = pb.lx;
+  cfg.ly = pb.ly;
+  cfg.dt = pb.dt;
+  cfg.periodic_x = pb.periodic_x != 0;
+  cfg.periodic_y = pb.periodic_y != 0;
+  ding::BoundaryConditionSpec *bcs[4] = {&cfg.pressure_bc_left, &cfg.pressure_bc_right, &cfg.pressure_bc_bottom,
+                                         &cfg.pressure_bc_top};
+  for (int s = 0; s < 4; ++s) {
+    bcs[s]->type = static_cast<ding::BoundaryConditionType>(pb.bc_type[s]);
+    bcs[s]->value = pb.bc_value[s];
+  }
+  cfg.poisson_iterations = pb.max_iterations;
+  cfg.pressure_tolerance = pb.tolerance;
+  cfg.density_ratio = pb.density_ratio;
+  cfg.pressure_scheme = scheme ? scheme : "icpcg";
+  cfg.verbose = false;
+  cfg.write_vtk = false;
+  return cfg;
+}
+
+void load_interior(ding::Field2D &f, const double *src) {
+  if (!src) return;
+  for (int i = 0; i < f.nx; ++i)
+    for (int j = 0; j < f.ny; ++j) f(i, j) = src[static_cast<std::size_t>(i) * f.ny + j];
+}
+
+std::unique_ptr<ding::Solver> make_solver(const RefProblem &pb, const char *scheme, const double *rho,
+                                          const double *u, const double *v, const double *p_n,
+                                          const double *p_nm1) {
+  auto s = std::make_unique<ding::Solver>(make_config(pb, scheme));
+  s->validate_boundary_configuration();
+  load_interior(s->rho_mid_, rho);
+  load_interior(s->u_star_, u);
+  load_interior(s->v_star_, v);
+  load_interior(s->pressure_, p_n);
+  load_interior(s->pressure_previous_step_, p_nm1);
+  s->apply_pressure_bc(s->pressure_);
+  s->apply_pressure_bc(s->pressure_previous_step_);
+  s->current_step_index_ = pb.step_index;
+  return s;
+}
+
+void set_err(char *err, int errlen, const char *msg) {
+  if (err && errlen > 0) {
+    std::snprintf(err, static_cast<std::size_t>(errlen), "%s", msg);
+  }
+}
+
+struct RefSystem {
+  std::unique_ptr<ding::Solver> solver;
+  ding::PressureLinearSystem system;
+  ding::ch_sparse_krylov::SparseMatrixCSR matrix;
+  ding::ch_sparse_krylov::KrylovPreconditioner precond;
+  bool factored = false;
+};
+
+}  // namespace
+
+extern "C" {
+
+// One pressure solve through the reference dispatcher. p_out receives the interior of
+// pressure_correction_ (row i*ny+j); p_out_ghosted (may be NULL) receives the whole Field2D
+// buffer including its 3 ghost layers. Returns 0 on success, 1 on a thrown error (message in err).
+int ref_pressure_solve(const char *scheme, const RefProblem *pb, const double *rho, const double *u,
+                       const double *v, const double *p_n, const double *p_nm1, double *p_out,
+                       double *p_out_ghosted, int *iterations, double *residual, char *err, int errlen) {
+  try {
+    auto s = make_solver(*pb, scheme, rho, u, v, p_n, p_nm1);
+    const double res = s->solve_pressure_correction();
+    if (residual) *residual = res;
+    if (iterations) *iterations = s->last_pressure_iterations_;
+    if (p_out) {
+      for (int i = 0; i < pb->nx; ++i)
+        for (int j = 0; j < pb->ny; ++j)
+          p_out[static_cast<std::size_t>(i) * pb->ny + j] = s->pressure_correction_(i, j);
+    }
+    if (p_out_ghosted) {
+      std::memcpy(p_out_ghosted, s->pressure_correction_.data.data(),
+                  s->pressure_correction_.data.size() * sizeof(double));
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// Ghost fill of a pressure-like field (Field2D buffer with 3 ghost layers) by apply_pressure_bc.
+int ref_apply_pressure_bc(const RefProblem *pb, double *field_ghosted) {
+  auto s = std::make_unique<ding::Solver>(make_config(*pb, "icpcg"));
+  ding::Field2D f(pb->nx, pb->ny, 3, 0.0);
+  std::memcpy(f.data.data(), field_ghosted, f.data.size() * sizeof(double));
+  s->apply_pressure_bc(f);
+  std::memcpy(field_ghosted, f.data.data(), f.data.size() * sizeof(double));
+  return 0;
+}
+
+// Assemble (split or not), regularize and convert to CSR with the reference's own functions.
+void *ref_sys_build(const RefProblem *pb, int use_split, const double *rho, const double *u, const double *v,
+                    const double *p_n, const double *p_nm1, char *err, int errlen) {
+  try {
+    auto sys = std::make_unique<RefSystem>();
+    sys->solver = make_solver(*pb, use_split ? "split_icpcg" : "icpcg", rho, u, v, p_n, p_nm1);
+    ding::Solver &s = *sys->solver;
+    if (use_split) {
+      ding::Field2D p_ext(pb->nx, pb->ny, 3, 0.0);
+      s.build_liu_split_pressure_extrapolation(p_ext);
+      sys->system = s.assemble_pressure_linear_system(true, s.liu_split_reference_density(), &p_ext);
+    } else {
+      sys->system = s.assemble_pressure_linear_system(false, 1.0, nullptr);
+    }
+    ding::ch_sparse_krylov::finalize_row_maps(sys->system.row_maps, sys->matrix);
+    return sys.release();
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return nullptr;
+  }
+}
+
+void ref_sys_free(void *h) { delete static_cast<RefSystem *>(h); }
+
+void ref_sys_sizes(void *h, int *n, int *nnz, int *has_dirichlet) {
+  auto *sys = static_cast<RefSystem *>(h);
+  *n = sys->matrix.n;
+  *nnz = static_cast<int>(sys->matrix.values.size());
+  *has_dirichlet = sys->system.has_dirichlet_boundary ? 1 : 0;
+}
+
+void ref_sys_csr(void *h, int *row_ptr, int *col_idx, double *values, double *rhs) {
+  auto *sys = static_cast<RefSystem *>(h);
+  std::copy(sys->matrix.row_ptr.begin(), sys->matrix.row_ptr.end(), row_ptr);
+  std::copy(sys->matrix.col_idx.begin(), sys->matrix.col_idx.end(), col_idx);
+  std::copy(sys->matrix.values.begin(), sys->matrix.values.end(), values);
+  std::copy(sys->system.rhs.begin(), sys->system.rhs.end(), rhs);
+}
+
+// kind 1 = ICC(0), 2 = ILDLT(0). Returns 1 on success, 0 when the factorization fails.
+int ref_sys_factor(void *h, int kind, double *lower, double *diag, double *floor_out) {
+  auto *sys = static_cast<RefSystem *>(h);
+  bool ok = (kind == 2) ? ding::ch_sparse_krylov::try_build_incomplete_ldlt(sys->matrix, sys->precond)
+                        : ding::ch_sparse_krylov::try_build_incomplete_cholesky(sys->matrix, sys->precond);
+  sys->factored = ok;
+  if (lower) std::copy(sys->precond.lower.begin(), sys->precond.lower.end(), lower);
+  if (diag) std::copy(sys->precond.diag.begin(), sys->precond.diag.end(), diag);
+  if (floor_out) *floor_out = sys->precond.diagonal_floor;
+  return ok ? 1 : 0;
+}
+
+void ref_sys_apply_matrix(void *h, const double *x, double *y) {
+  auto *sys = static_cast<RefSystem *>(h);
+  std::vector<double> xv(x, x + sys->matrix.n), yv;
+  ding::ch_sparse_krylov::apply_matrix(sys->matrix, xv, yv);
+  std::copy(yv.begin(), yv.end(), y);
+}
+
+void ref_sys_apply_precond(void *h, const double *r, double *z) {
+  auto *sys = static_cast<RefSystem *>(h);
+  std::vector<double> rv(r, r + sys->matrix.n), zv;
+  ding::ch_sparse_krylov::apply_preconditioner(sys->matrix, sys->precond, rv, zv);
+  std::copy(zv.begin(), zv.end(), z);
+}
+
+// PCG from x0 = 0 with the reference's nullspace projection (pressure.cpp:50-57, 225-239).
+int ref_sys_pcg(void *h, int max_iterations, double tolerance, double *x, int *iterations, double *rel,
+                double *abs_res, char *err, int errlen) {
+  auto *sys = static_cast<RefSystem *>(h);
+  try {
+    std::vector<double> xv(static_cast<std::size_t>(sys->matrix.n), 0.0);
+    const auto report = ding::ch_sparse_krylov::solve_preconditioned_cg(
+        sys->matrix, sys->precond, sys->system.rhs, max_iterations, tolerance, xv,
+        sys->solver->pressure_nullspace_projection());
+    std::copy(xv.begin(), xv.end(), x);
+    *iterations = report.iterations;
+    *rel = report.relative_residual;
+    *abs_res = report.absolute_residual;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+"""Write golden vectors for the pressure path from the reference compiled from its own sources.
+
+Run in a container where the reference sources exist (``make -C oracle ref`` first):
+    python tests/golden/make_golden.py
+Each fixture is an .npz with the case inputs, the reference's CSR operator and regularized RHS
+(pressure.cpp:123-212 + ch_sparse_krylov.hpp:105-133), its ICC(0)/ILDLT(0) factor
+(ch_sparse_krylov.hpp:252-419), one SpMV and one preconditioner application on a seeded probe
+vector (235-250, 444-513), and the full Solver::solve_pressure_correction result (pressure.cpp:1031).
+The files hold data only (no code) and load with numpy.load(allow_pickle=False).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+from ding_nsch_core_solver_amd import workloads as W  # noqa: E402
+from oracle import ref  # noqa: E402
+
+U, D, N = W.BC_UNSET, W.BC_DIRICHLET, W.BC_NEUMANN
+
+
+def cases():
+    out = []
+    out.append(("icpcg_12x10_neumann_dr1000", "icpcg", W.droplet_case(12, 10, seed=21, density_ratio=1000.0,
+                                                                       radius=0.3)))
+    out.append(("ildlt_pcg_10x12_dirichlet", "ildlt_pcg",
+                W.droplet_case(10, 12, seed=22, density_ratio=100.0, bc_type=(D, N, N, D),
+                               bc_value=(1.0, 0.5, -0.25, -2.0))))
+    c = W.droplet_case(14, 9, seed=23, density_ratio=250.0, step_index=2)
+    rng = np.random.default_rng(123)
+    c.p_n = rng.standard_normal(c.n) * 10.0
+    c.p_nm1 = rng.standard_normal(c.n) * 10.0
+    out.append(("split_icpcg_14x9_history", "split_icpcg", c))
+    out.append(("icpcg_16x8_periodic_x", "icpcg", W.droplet_case(16, 8, seed=24, density_ratio=30.0,
+                                                                  periodic_x=True)))
+    out.append(("icpcg_8x16_periodic_y", "icpcg", W.droplet_case(8, 16, seed=25, density_ratio=30.0,
+                                                                  periodic_y=True)))
+    out.append(("ildlt_pcg_12x12_periodic_xy", "ildlt_pcg", W.droplet_case(12, 12, seed=26, density_ratio=1.0,
+                                                                             periodic_x=True, periodic_y=True)))
+    c = W.droplet_case(9, 13, seed=27, density_ratio=600.0, step_index=0, periodic_x=True)
+    c.p_n = np.random.default_rng(7).standard_normal(c.n)
+    out.append(("split_ildlt_pcg_9x13_periodic_x", "split_ildlt_pcg", c))
+    return out
+
+
+def write(name, scheme, case):
+    use_split = scheme.startswith("split")
+    kind = 2 if "ildlt" in scheme else 1
+    rs = ref.RefSystem(case, use_split)
+    ok, lower, diag, fl = rs.factor(kind)
+    probe = np.random.default_rng(2024).standard_normal(rs.n)
+    p, it, res = ref.pressure_solve(scheme, case)
+    empty = np.zeros(0)
+    np.savez_compressed(
+        os.path.join(HERE, name + ".npz"),
+        scheme=np.array(scheme), nx=case.nx, ny=case.ny, lx=case.lx, ly=case.ly, dt=case.dt,
+        periodic_x=int(case.periodic_x), periodic_y=int(case.periodic_y),
+        bc_type=np.array(case.bc_type, dtype=np.int32), bc_value=np.array(case.bc_value, dtype=np.float64),
+        max_iterations=case.max_iterations, tolerance=case.tolerance, density_ratio=case.density_ratio,
+        step_index=case.step_index, rho=case.rho, u=case.u, v=case.v,
+        p_n=case.p_n if case.p_n is not None else empty, p_nm1=case.p_nm1 if case.p_nm1 is not None else empty,
+        row_ptr=rs.row_ptr, col_idx=rs.col_idx, values=rs.values, rhs=rs.rhs,
+        factor_ok=int(ok), floor=fl, lower=lower, diag=diag,
+        probe=probe, probe_Ax=rs.apply_matrix(probe), probe_Mx=rs.apply_precond(probe),
+        iterations=it, residual=res, pressure=p)
+    print(f"{name}: n={rs.n} nnz={rs.nnz} iterations={it} residual={res:.3e}")
+
+
+if __name__ == "__main__":
+    if not ref.available():
+        sys.exit("build the reference checker first: make -C oracle ref")
+    for name, scheme, case in cases():
+        write(name, scheme, case)
