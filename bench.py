@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Pressure-solve throughput on one MI355X.
+
+Workload (one step = one pressure correction): 192x192 cells on [0,1]^2, Neumann pressure on all
+four walls, a droplet of radius 0.2 at the centre with Cn = 0.01 and density ratio 1000
+(rho = (1+phi)/2*1000 + (1-phi)/2), RHS = -div(u*)/dt of a seeded N(0,1) face velocity field with
+dt = 1e-4, ICC(0)-preconditioned CG to relative residual 1e-8 (the reference's icpcg backend).
+Inputs are resident in HBM when the timed region starts; every step runs assembly, factorization,
+PCG and the output mean subtraction.
+
+Prints one JSON line with solves/s, the roofline of the solve kernel (algorithmic bytes from
+HIP-event kernel time) and the reference CPU path timed on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from ding_nsch_core_solver_amd import engine as E  # noqa: E402
+from ding_nsch_core_solver_amd import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "fp64_solve_192x192_dr1000_traffic.txt")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def setup_bytes(n):
+    """Algorithmic bytes of one solve's setup: assembly, regularization, factorization."""
+    return 120 * n
+
+
+def iteration_bytes(n, fp32_inner=False):
+    """Algorithmic bytes of one PCG iteration: 25 n-length streams (SpMV, x/r update, two sweeps, p update)."""
+    return (100 if fp32_inner else 200) * n
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, seconds):
+    """Reference icpcg (compiled from the reference's sources) on one pinned core, bounded sample."""
+    kind = "reference"
+    try:
+        from oracle import ref as impl
+        if not impl.available():
+            raise FileNotFoundError
+        solve = lambda: impl.pressure_solve("icpcg", case)  # noqa: E731
+    except (ImportError, FileNotFoundError):
+        from oracle import cpu as impl
+        if not impl.available():
+            return None, None
+        kind = "port"
+        solve = lambda: impl.pressure_solve("icpcg", case, impl.SEQUENTIAL)  # noqa: E731
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        result = solve()  # warm-up, also the parity reference
+        times = []
+        t_end = time.perf_counter() + seconds
+        while time.perf_counter() < t_end and len(times) < 200:
+            t0 = time.perf_counter()
+            solve()
+            times.append(time.perf_counter() - t0)
+    finally:
+        os.sched_setaffinity(0, old)
+    med = float(np.median(times))
+    return {
+        "value": 1.0 / med, "unit": "solves/s", "cores": 1, "kind": kind,
+        "sample": f"{len(times)} icpcg solves of the same 192x192 dr=1000 seed-0 input, median "
+                  f"{med * 1e3:.1f} ms; host {cpu_model()}, nproc {os.cpu_count()}",
+    }, result
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this engine runs one pressure system per "
+              "solve on one GPU (independent systems would shard by system); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    dev = torch.device("cuda:0")
+    case = W.droplet_case(192, 192, seed=args.seed, density_ratio=1000.0, tolerance=1e-8, max_iterations=20000)
+    n = case.n
+    d_rho = torch.from_numpy(case.rho.reshape(-1)).to(dev)
+    d_u = torch.from_numpy(case.u.reshape(-1)).to(dev)
+    d_v = torch.from_numpy(case.v.reshape(-1)).to(dev)
+    d_out = torch.zeros(n, dtype=torch.float64, device=dev)
+    eng = E.Engine(0)
+    stream = torch.cuda.current_stream(dev)
+
+    def step(precision=E.PP_FP64):
+        return eng.solve_device(case, d_rho.data_ptr(), d_u.data_ptr(), d_v.data_ptr(), None, None,
+                                d_out.data_ptr(), scheme=E.PP_ICPCG, precision=precision,
+                                stream=stream.cuda_stream)
+
+    log(f"[bench] warmup {args.warmup} steps")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} FP64 steps")
+    reports = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        reports.append(step())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    p_gpu = d_out.cpu().numpy()
+    iters = reports[-1]["iterations"]
+    kernel_ms = float(np.mean([r["kernel_ms"] for r in reports]))
+    alg_bytes = setup_bytes(n) + iters * iteration_bytes(n)
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    log(f"[bench] {args.steps / elapsed:.2f} solves/s, {elapsed / args.steps * 1e3:.3f} ms/solve, "
+        f"kernel {kernel_ms:.3f} ms, iterations {iters}")
+
+    line = {
+        "metric": "pressure solves/s to 1e-8 rel resid at 192^2 dr=1000 (FP64 vs mixed); ms/step",
+        "value": args.steps / elapsed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "single variable-coefficient pressure solve, 192x192, Neumann walls, droplet "
+                               "R=0.2 Cn=0.01 density ratio 1000, seeded N(0,1) u*/v*, dt=1e-4, hip_icpcg "
+                               "FP64 to rel. residual 1e-8",
+                   "nx": 192, "ny": 192, "scheme": "hip_icpcg", "tolerance": 1e-8, "seed": args.seed,
+                   "iterations": iters, "relative_residual": reports[-1]["relative_residual"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "kernel": "fp64_solve_kernel", "kernel_ms": kernel_ms,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+    }
+
+    if not args.no_cpu_baseline:
+        log(f"[bench] CPU baseline (~{args.cpu_seconds:.0f} s on one core)")
+        base, ref_result = cpu_baseline(case, args.cpu_seconds)
+        line["cpu_baseline"] = base
+        if ref_result is not None:
+            p_ref, it_ref, res_ref = ref_result
+            line["parity"] = {"iterations_gpu": iters, "iterations_cpu": it_ref,
+                              "max_abs_diff_over_max_abs": float(np.abs(p_gpu - p_ref).max() / np.abs(p_ref).max())}
+            line["speedup_vs_cpu"] = line["value"] / base["value"]
+    print(json.dumps(line), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -13,7 +13,7 @@
 // takes it from wave w-1 through an LDS ring with a published/consumed counter pair, so the waves
 // form a pipeline without block-wide barriers. The south neighbour is the lane's own previous value.
 //
-// Reductions use one fixed order (the "canonical tree"): thread t of the block accumulates
+// Reductions use one fixed order (the "canonical tree"): thread t of the T = 512 threads accumulates
 // positions t, t + T, t + 2T, ... sequentially from 0.0; lanes of a wave combine with offsets
 // 32, 16, ..., 1 (v[l] += v[l + off]); waves combine the same way with offsets W/2, ..., 1.
 // oracle/pp_oracle.cpp reproduces this order to check whole solves bit for bit.
@@ -24,11 +24,17 @@
 
 namespace pp {
 
-constexpr int kThreads = 1024;            // threads per system block
-constexpr int kWaves = kThreads / 64;     // 16
-constexpr int kMaxSide = 1024;            // nx, ny <= 1024 (sweep waves <= 16)
+constexpr int kThreads = 512;             // threads per system block (8 waves, 256 VGPRs per lane)
+constexpr int kWaves = kThreads / 64;     // 8
+constexpr int kMaxRows = 64 * kWaves;     // nx <= 512: one sweep wave per 64 rows
+constexpr int kMaxSide = 1024;            // ny <= 1024
 constexpr int kMaxLevels = 2 * kMaxSide - 1;
 constexpr int kRing = 64;                 // hand-off ring depth between sweep waves
+constexpr int kPad = 1024;                // padding after every work array (lanes outside the grid)
+constexpr int kStageDoubles = 16384;      // 128 KB of LDS for the sweep operand ring
+constexpr int kMaxRing = 16;              // ring depth in levels (per sweep wave)
+constexpr int kStageUnit = 8;             // levels a loader moves per step
+constexpr int kMaxStagedWaves = 6;        // more sweep waves leave too few loader waves
 constexpr int kSpinLimit = 1 << 24;       // bounded waits: a broken hand-off ends as an error
 
 enum DevStatus : int {
@@ -41,9 +47,27 @@ struct DevReport {
   int outer_cycles;
   int pad;
   double rel, abs, iter_res;
+  unsigned long long phase_ticks[10];
 };
 
 __device__ __forceinline__ int level_lo(int d, int ny) { return d - ny + 1 > 0 ? d - ny + 1 : 0; }
+
+// Explicit address spaces. Pointers that arrive through a struct are generic ("flat"); flat loads
+// count against both the vector-memory and the LDS wait counters, and volatile flat accesses are
+// followed by a full vmcnt(0) drain. Casting to the global (1) and LDS (3) spaces gives global_* and
+// ds_* instructions, so LDS polling never waits on in-flight prefetch loads.
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T *;
+
+template <typename T>
+__device__ __forceinline__ global_ptr<T> as_global(T *p) {
+  return (global_ptr<T>)p;
+}
+
+template <typename T>
+__device__ __forceinline__ volatile __attribute__((address_space(3))) T *lds(volatile T *p) {
+  return (volatile __attribute__((address_space(3))) T *)(p);
+}
 
 // std::max(value, floor) as the reference writes it: floor only when value < floor.
 __device__ __forceinline__ double floor_max(double value, double floor) { return value < floor ? floor : value; }
@@ -51,20 +75,29 @@ __device__ __forceinline__ float floor_max(float value, float floor) { return va
 
 // LDS state shared by the phases of one system block.
 struct SweepShared {
-  int off[kMaxLevels + 1];              // level offsets of the wavefront layout
   double red[2][kWaves * 4];            // reduction slots, double-buffered
   volatile double hand[kWaves][kRing];  // hand[w]: boundary values flowing into wave w
   volatile int cnt[kWaves];             // values published into hand[w]
   volatile int ack[kWaves];             // values consumed from hand[w]
   volatile double wrap[kMaxSide];       // periodic-x wrap line (row 0 -> row nx-1 and back)
+  double lx[kMaxSide];                  // periodic-x wrap lower entries LX[j], copied after factoring
+  // Operand staging ring for the sweeps: loader waves copy the four per-level operands of every
+  // sweep wave into stage[] ahead of use; ready[w][slot] holds step + 1 once a slot is filled and
+  // used[w] counts the steps sweep wave w has finished with.
+  double stage[kStageDoubles];
+  volatile int ready[kWaves * kMaxRing];
+  volatile int used[kWaves];
+  // Write targets for lanes that have nothing to publish, so hand-off stores need no branch.
+  volatile double dummy_d[64];
+  volatile int dummy_i[64];
   volatile int spin_error;
   volatile int status;                  // DevStatus of the system, uniform after each barrier
 };
 
-__device__ __forceinline__ int cell_pos(const SweepShared &sh, int i, int j, int ny) {
-  const int d = i + j;
-  return sh.off[d] + i - level_lo(d, ny);
-}
+// Device clock for per-phase timing (s_memrealtime, 100 MHz).
+__device__ __forceinline__ unsigned long long device_ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
+constexpr int kPhases = 10;  // setup, spmv, update, rr, forward, backward, zsum, rz, p update, output
 
 // Sum of K per-thread partials over the block in the canonical tree; every thread gets the result.
 template <int K>
@@ -124,10 +157,10 @@ __device__ __forceinline__ double block_max1(double v, SweepShared &sh, int &par
 // Bounded wait on an LDS counter; on timeout records an error and returns (never hangs).
 __device__ __forceinline__ void wait_at_least(volatile int *counter, int target, SweepShared &sh) {
   int guard = 0;
-  while (*counter < target) {
+  while (*lds(counter) < target) {
     __builtin_amdgcn_s_sleep(1);
     if (++guard > kSpinLimit) {
-      sh.spin_error = 1;
+      *lds(&sh.spin_error) = 1;
       return;
     }
   }
@@ -135,65 +168,113 @@ __device__ __forceinline__ void wait_at_least(volatile int *counter, int target,
 
 __device__ __forceinline__ void reset_handoff(SweepShared &sh) {
   if (threadIdx.x < kWaves) {
-    sh.cnt[threadIdx.x] = 0;
-    sh.ack[threadIdx.x] = 0;
+    *lds(&sh.cnt[threadIdx.x]) = 0;
+    *lds(&sh.ack[threadIdx.x]) = 0;
   }
 }
 
-// ---- forward wavefront (factorization and forward substitution share the pattern) -----------
-//
-// Helper that moves lane 63's value of wave w to lane 0 of wave w+1 for rows in ascending order.
-// `j_in` is the column lane 0 of this wave works on at this level, `j_out` that of lane 63.
-template <typename T>
-__device__ __forceinline__ T forward_receive(T west_from_shuffle, int wave, int lane, int j_in, int ny,
-                                             SweepShared &sh) {
-  T west = west_from_shuffle;
-  if (wave > 0 && j_in >= 0 && j_in < ny) {
-    wait_at_least(&sh.cnt[wave], j_in + 1, sh);
-    const T h = static_cast<T>(sh.hand[wave][j_in % kRing]);
-    if (lane == 0) west = h;
-    if (lane == 0) sh.ack[wave] = j_in + 1;
+// ---- lane shifts and hand-offs between sweep waves ---------------------------------------------
+
+// Lane l receives lane l-1's value (lane 0 receives 0): one DPP wave_shr per 32-bit half.
+__device__ __forceinline__ double wave_shr1(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Lane l receives lane l+1's value (lane 63 receives 0).
+__device__ __forceinline__ double wave_shl1(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ float wave_shr1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float wave_shl1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
+// Value held by lane k (compile-time k), broadcast through SGPRs.
+__device__ __forceinline__ double lane_value(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ float lane_value(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// Hand-off protocol. Ring hand[w] carries boundary values into wave w; cnt[w] counts values
+// published, ack[w] values consumed. The k-th value of a stream sits in slot k % kRing.
+// Consumers read a batch of P values at once (lane k of the wave holds batch entry k);
+// producers publish one value per level and check ring space once per batch of P levels.
+// With P <= 31 and kRing = 64 a producer never waits on a consumer that waits on it.
+
+template <typename T, int P>
+__device__ __forceinline__ T receive_batch(int stream_wave, int lane, int k0, int count, SweepShared &sh) {
+  T hv = T(0);
+  const int klo = k0 > 0 ? k0 : 0;
+  const int khi = (k0 + P - 1) < (count - 1) ? (k0 + P - 1) : (count - 1);
+  if (klo <= khi) {
+    wait_at_least(&sh.cnt[stream_wave], khi + 1, sh);
+    const int kl = k0 + lane;
+    if (lane < P && kl >= klo && kl <= khi) hv = static_cast<T>(*lds(&sh.hand[stream_wave][kl % kRing]));
+    if (lane == 0) *lds(&sh.ack[stream_wave]) = khi + 1;
   }
-  return west;
+  return hv;
+}
+
+// Before a batch whose last published index is k_last: make sure its slot has been consumed.
+__device__ __forceinline__ void reserve_batch(int stream_wave, int k_last, SweepShared &sh) {
+  if (k_last >= 0) wait_at_least(&sh.ack[stream_wave], k_last + 1 - kRing, sh);
 }
 
 template <typename T>
-__device__ __forceinline__ void forward_send(T value, int wave, int lane, int S, int j_out, int ny, SweepShared &sh) {
-  if (wave + 1 < S && j_out >= 0 && j_out < ny) {
-    wait_at_least(&sh.ack[wave + 1], j_out + 1 - kRing, sh);
-    if (lane == 63) {
-      sh.hand[wave + 1][j_out % kRing] = static_cast<double>(value);
-      sh.cnt[wave + 1] = j_out + 1;
-    }
+__device__ __forceinline__ void publish(int stream_wave, int lane, int src_lane, int k, int count, T value,
+                                        SweepShared &sh) {
+  if (k >= 0 && k < count && lane == src_lane) {
+    *lds(&sh.hand[stream_wave][k % kRing]) = static_cast<double>(value);
+    *lds(&sh.cnt[stream_wave]) = k + 1;
   }
 }
 
-// Backward direction: lane 0 of wave w sends to lane 63 of wave w-1; columns descend, so the
-// k-th hand-off carries column ny-1-k.
+// Branch-free publish: every lane executes the two LDS stores, only src_lane with a valid index
+// targets the ring and counter, the others write their dummy slots.
 template <typename T>
-__device__ __forceinline__ T backward_receive(T east_from_shuffle, int wave, int lane, int S, int j63, int ny,
-                                              SweepShared &sh) {
-  T east = east_from_shuffle;
-  if (wave + 1 < S && j63 >= 0 && j63 < ny) {
-    const int k = ny - 1 - j63;
-    wait_at_least(&sh.cnt[wave], k + 1, sh);
-    const T h = static_cast<T>(sh.hand[wave][k % kRing]);
-    if (lane == 63) east = h;
-    if (lane == 63) sh.ack[wave] = k + 1;
-  }
-  return east;
+__device__ __forceinline__ void publish_nb(int stream_wave, int lane, int src_lane, int k, int count, T value,
+                                           SweepShared &sh) {
+  const bool real = lane == src_lane && k >= 0 && k < count;
+  volatile double *slot = real ? &sh.hand[stream_wave][k & (kRing - 1)] : &sh.dummy_d[lane];
+  volatile int *ctr = real ? &sh.cnt[stream_wave] : &sh.dummy_i[lane];
+  *lds(slot) = static_cast<double>(value);
+  *lds(ctr) = k + 1;
 }
 
-template <typename T>
-__device__ __forceinline__ void backward_send(T value, int wave, int lane, int j0, int ny, SweepShared &sh) {
-  if (wave > 0 && j0 >= 0 && j0 < ny) {
-    const int k = ny - 1 - j0;
-    wait_at_least(&sh.ack[wave - 1], k + 1 - kRing, sh);
-    if (lane == 0) {
-      sh.hand[wave - 1][k % kRing] = static_cast<double>(value);
-      sh.cnt[wave - 1] = k + 1;
-    }
-  }
+// Number of cells on levels < d of an nx x ny grid (0 <= d <= nx + ny - 1).
+__device__ __forceinline__ int level_offset(int d, int nx, int ny) {
+  const int a = nx < ny ? nx : ny, b = nx < ny ? ny : nx;
+  if (d <= a) return d * (d + 1) / 2;
+  if (d <= b) return a * (a + 1) / 2 + (d - a) * a;
+  const int m = nx + ny - 1 - d;
+  return nx * ny - m * (m + 1) / 2;
+}
+
+// off(d) - lo(d): cell (i, d - i) sits at level_base(d) + i. Branch-free closed form, so with a
+// wave-uniform d the whole computation stays on the scalar unit.
+__device__ __forceinline__ int level_base(int d, int nx, int ny) {
+  const int a = nx < ny ? nx : ny, b = nx < ny ? ny : nx;
+  const int d1 = min(d, a), d2 = min(max(d - a, 0), b - a), d3 = max(d - b, 0);
+  const int off = d1 * (d1 + 1) / 2 + d2 * a + d3 * (a - 1) - d3 * (d3 - 1) / 2;
+  return off - max(0, d - ny + 1);
+}
+
+__device__ __forceinline__ int grid_pos(int i, int j, int nx, int ny) {
+  const int d = i + j;
+  return level_offset(d, nx, ny) + i - level_lo(d, ny);
 }
 
 }  // namespace pp

@@ -90,10 +90,11 @@ __device__ void assemble_cell(const SysArgs &a, int q, int i, int j) {
   }
   double diag = 0.0;
   double b = a.split ? -(split_explicit(a, i, j) + a.rho_ref * divergence(a, i, j) / a.dt) : -divergence(a, i, j) / a.dt;
-  auto boundary = [&](int side, double coeff, double spacing) {
-    int type = a.bc_type[side];
-    const double value = type == 0 ? 0.0 : a.bc_value[side];
-    if (type == 0) type = 2;
+  auto boundary = [&](int type, double value, double coeff, double spacing) {
+    if (type == 0) {  // unset edges are homogeneous Neumann (operators.cpp:36-60)
+      type = 2;
+      value = 0.0;
+    }
     if (type == 1) {
       diag += 2.0 * coeff;
       b += 2.0 * coeff * value;
@@ -101,208 +102,604 @@ __device__ void assemble_cell(const SysArgs &a, int q, int i, int j) {
       b += coeff * value * spacing;
     }
   };
-  if (has_e) diag += ce; else boundary(1, ce, dx);
-  if (has_w) diag += cw; else boundary(0, cw, dx);
-  if (has_n) diag += cn; else boundary(3, cn, dy);
-  if (has_s) diag += cs; else boundary(2, cs, dy);
+  if (has_e) diag += ce; else boundary(a.bc_type[1], a.bc_value[1], ce, dx);
+  if (has_w) diag += cw; else boundary(a.bc_type[0], a.bc_value[0], cw, dx);
+  if (has_n) diag += cn; else boundary(a.bc_type[3], a.bc_value[3], cn, dy);
+  if (has_s) diag += cs; else boundary(a.bc_type[2], a.bc_value[2], cs, dy);
   a.aP[q] = 0.0 + diag;
   a.aW[q] = has_w ? drop_tiny(0.0 + -cw) : 0.0;
   a.aS[q] = has_s ? drop_tiny(0.0 + -cs) : 0.0;
   a.b[q] = b;
 }
 
-// ---- factorization: forward wavefront over levels ------------------------------------------
+// ---- wavefront sweeps --------------------------------------------------------------------------
+//
+// Wave w (w < S) owns rows 64w .. 64w+63 and walks its levels d = 64w .. 64w+63+ny-1 (ascending in
+// the forward direction). Per level the position of every lane is base(d) + i with a wave-uniform
+// base, operands of level d+P are loaded while level d is computed (register ring of P slots), the
+// west/east neighbour comes from the adjacent lane by DPP and, for the boundary lane, from the
+// adjacent wave in batches of P hand-off values. Lanes outside the grid compute on padding values;
+// only their stores are masked, and their results only ever feed other inactive lanes.
+constexpr int kPrefetch = 8;
+
+struct FactorOps {
+  double ap, aw, as;
+};
+
 template <bool kIldlt>
-__device__ void factor_sweep(const SysArgs &a, SweepShared &sh, double floor) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+__device__ __forceinline__ void factor_sweep(const SysArgs &a, SweepShared &sh, double floor) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
   const int S = (nx + 63) >> 6;
   if (wave >= S) return;
   const int i = wave * 64 + lane;
-  const int i_last = min(wave * 64 + 63, nx - 1);
-  const int d_beg = wave * 64, d_end = i_last + ny - 1;
+  const int d_beg = wave * 64, d_end = min(wave * 64 + 63, nx - 1) + ny - 1, d_last = nx + ny - 2;
+  const bool px = a.periodic_x != 0, py = a.periodic_y != 0;
+  const bool row_ok = i < nx, has_w = i > 0, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
+  const global_ptr<const double> aP = as_global<const double>(a.aP);
+  const global_ptr<const double> aW = as_global<const double>(a.aW);
+  const global_ptr<const double> aS = as_global<const double>(a.aS);
+  const global_ptr<double> Lw = as_global(a.Lw);
+  const global_ptr<double> Ls = as_global(a.Ls);
+  const global_ptr<double> dg = as_global(a.dg);
+  const global_ptr<double> LX = as_global(a.LX);
+  const global_ptr<double> LY = as_global(a.LY);
   const double floor_sq = floor * floor;
+  const double ay = (py && row_ok) ? aS[level_base(i, nx, ny) + i] : 0.0;  // entry (i,ny-1)->(i,0)
   double prev = 0.0, first = 0.0;
   bool failed = false;
-  for (int d = d_beg; d <= d_end; ++d) {
-    const int j = d - i;
-    const bool act = i < nx && j >= 0 && j < ny;
-    double west = __shfl_up(prev, 1, 64);
-    west = forward_receive(west, wave, lane, d - wave * 64, ny, sh);
-    double cur = 0.0;
-    if (act) {
-      const int q = sh.off[d] + i - level_lo(d, ny);
-      double dd = a.aP[q];
-      if (a.periodic_x && i == nx - 1) {  // entry (nx-1, j) -> (0, j), first in column order
-        const double dc = sh.wrap[j];
-        const double l = a.aW[cell_pos(sh, 0, j, ny)] / floor_max(dc, floor);
-        a.LX[j] = l;
-        dd = kIldlt ? dd - (l * l) * dc : dd - l * l;
-      }
-      if (i > 0) {
-        const double l = a.aW[q] / floor_max(west, floor);
-        a.Lw[q] = l;
-        dd = kIldlt ? dd - (l * l) * west : dd - l * l;
-      } else {
-        a.Lw[q] = 0.0;
-      }
-      if (a.periodic_y && j == ny - 1) {  // entry (i, ny-1) -> (i, 0)
-        const double l = a.aS[cell_pos(sh, i, 0, ny)] / floor_max(first, floor);
-        a.LY[i] = l;
-        dd = kIldlt ? dd - (l * l) * first : dd - l * l;
-      }
-      if (j > 0) {
-        const double l = a.aS[q] / floor_max(prev, floor);
-        a.Ls[q] = l;
-        dd = kIldlt ? dd - (l * l) * prev : dd - l * l;
-      } else {
-        a.Ls[q] = 0.0;
-      }
-      if (kIldlt) {
-        if (!isfinite(dd) || dd <= floor) failed = true;
-        cur = dd;
-      } else {
-        if (!isfinite(dd) || dd <= floor_sq) failed = true;
-        cur = sqrt(dd);
-      }
-      a.dg[q] = cur;
-      if (j == 0) first = cur;
-      if (a.periodic_x && i == 0) sh.wrap[j] = cur;
-    }
-    prev = cur;
-    forward_send(cur, wave, lane, S, d - (wave * 64 + 63), ny, sh);
+  FactorOps buf[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) {
+    const int q = level_base(min(d_beg + k, d_last), nx, ny) + i;
+    buf[k] = {aP[q], aW[q], aS[q]};
   }
-  if (failed) sh.status = kFactorFail;
+  for (int d0 = d_beg; d0 <= d_end; d0 += kPrefetch) {
+    const double hv = wave > 0 ? receive_batch<double, kPrefetch>(wave, lane, d0 - wave * 64, ny, sh) : 0.0;
+    if (wave + 1 < S) reserve_batch(wave + 1, min(d0 + kPrefetch - 1 - (wave * 64 + 63), ny - 1), sh);
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const int d = d0 + k;
+      const FactorOps o = buf[k];
+      {
+        const int qp = level_base(min(d + kPrefetch, d_last), nx, ny) + i;
+        buf[k] = {aP[qp], aW[qp], aS[qp]};
+      }
+      if (d <= d_end) {
+        const int j = d - i;
+        const bool act = row_ok && j >= 0 && j < ny;
+        double west = wave_shr1(prev);
+        if (wave > 0) {
+          const double h = lane_value(hv, k);
+          west = lane == 0 ? h : west;
+        }
+        double dd = o.ap;
+        if (wrap_row && act) {  // entry (nx-1, j) -> (0, j), first in column order
+          const double dc = *lds(&sh.wrap[j]);
+          const double l = aW[level_base(j, nx, ny)] / floor_max(dc, floor);
+          LX[j] = l;
+          dd = kIldlt ? dd - (l * l) * dc : dd - l * l;
+        }
+        const double lw = o.aw / floor_max(west, floor);
+        dd = has_w ? (kIldlt ? dd - (lw * lw) * west : dd - lw * lw) : dd;
+        if (py && j == ny - 1) {  // entry (i, ny-1) -> (i, 0)
+          const double l = ay / floor_max(first, floor);
+          if (act) LY[i] = l;
+          dd = kIldlt ? dd - (l * l) * first : dd - l * l;
+        }
+        const double ls = o.as / floor_max(prev, floor);
+        dd = j > 0 ? (kIldlt ? dd - (ls * ls) * prev : dd - ls * ls) : dd;
+        double cur;
+        if (kIldlt) {
+          failed = failed || (act && (!isfinite(dd) || dd <= floor));
+          cur = dd;
+        } else {
+          failed = failed || (act && (!isfinite(dd) || dd <= floor_sq));
+          cur = sqrt(dd);
+        }
+        if (act) {
+          const int q = level_base(d, nx, ny) + i;
+          Lw[q] = has_w ? lw : 0.0;
+          Ls[q] = j > 0 ? ls : 0.0;
+          dg[q] = cur;
+        }
+        first = j == 0 ? cur : first;
+        if (wrap_src && act) *lds(&sh.wrap[j]) = cur;
+        prev = cur;
+        if (wave + 1 < S) publish(wave + 1, lane, 63, d - (wave * 64 + 63), ny, cur, sh);
+      }
+    }
+  }
+  if (failed) *lds(&sh.status) = kFactorFail;
 }
 
-// ---- z = M^{-1} (r - m_r): forward then backward wavefront ----------------------------------
+struct ForwardOps {
+  double r, lw, ls, dg;
+};
+
+// Sweep loop shape shared by the solves: whole chunks of kPrefetch levels (levels past the wave's
+// range are inactive for every lane), every vector-memory operation unconditional (inactive lanes
+// store into the padding), so the compiler can keep the prefetch ring in flight across levels.
+__device__ __forceinline__ int chunk_end(int d_beg, int d_end) {
+  return d_beg + ((d_end - d_beg + kPrefetch) / kPrefetch) * kPrefetch - 1;
+}
+
+// y = L^{-1} (r - m_r) (ICC: divided by the pivot; ILDLT: unit lower, stored divided by D).
+// Lower entries in column order: periodic-x wrap (0, j), west (i-1, j), periodic-y wrap (i, 0),
+// south (i, j-1).
 template <bool kIldlt>
-__device__ void forward_solve(const SysArgs &a, SweepShared &sh, double m_r, double floor) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+__device__ __forceinline__ void forward_solve(const SysArgs &a, SweepShared &sh, double m_r, double floor) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
   const int S = (nx + 63) >> 6;
   if (wave >= S) return;
   const int i = wave * 64 + lane;
-  const int i_last = min(wave * 64 + 63, nx - 1);
-  const int d_beg = wave * 64, d_end = i_last + ny - 1;
+  const int d_beg = wave * 64, d_end = min(wave * 64 + 63, nx - 1) + ny - 1, d_last = nx + ny - 2;
+  const int d_stop = chunk_end(d_beg, d_end);
+  const bool px = a.periodic_x != 0, py = a.periodic_y != 0;
+  const bool row_ok = i < nx, has_w = i > 0, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
+  const global_ptr<const double> r = as_global<const double>(a.r);
+  const global_ptr<const double> Lw = as_global<const double>(a.Lw);
+  const global_ptr<const double> Ls = as_global<const double>(a.Ls);
+  const global_ptr<const double> dg = as_global<const double>(a.dg);
+  const global_ptr<double> y = as_global(a.y);
+  const int sink = a.n + kPad - 64 + lane;  // store target of inactive lanes
+  const double ly = (py && row_ok) ? as_global(a.LY)[i] : 0.0;
   double prev = 0.0, first = 0.0;
-  for (int d = d_beg; d <= d_end; ++d) {
-    const int j = d - i;
-    const bool act = i < nx && j >= 0 && j < ny;
-    double west = __shfl_up(prev, 1, 64);
-    west = forward_receive(west, wave, lane, d - wave * 64, ny, sh);
-    double cur = 0.0;
-    if (act) {
-      const int q = sh.off[d] + i - level_lo(d, ny);
-      double s = a.r[q] - m_r;
-      if (a.periodic_x && i == nx - 1) s = s - a.LX[j] * sh.wrap[j];
-      if (i > 0) s = s - a.Lw[q] * west;
-      if (a.periodic_y && j == ny - 1) s = s - a.LY[i] * first;
-      if (j > 0) s = s - a.Ls[q] * prev;
+  ForwardOps buf[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) {
+    const int q = level_base(min(d_beg + k, d_last), nx, ny) + i;
+    buf[k] = {r[q], Lw[q], Ls[q], dg[q]};
+  }
+  for (int d0 = d_beg; d0 <= d_stop; d0 += kPrefetch) {
+    const double hv = wave > 0 ? receive_batch<double, kPrefetch>(wave, lane, d0 - wave * 64, ny, sh) : 0.0;
+    if (wave + 1 < S) reserve_batch(wave + 1, min(d0 + kPrefetch - 1 - (wave * 64 + 63), ny - 1), sh);
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const int d = d0 + k;
+      const ForwardOps o = buf[k];
+      {
+        const int qp = level_base(min(d + kPrefetch, d_last), nx, ny) + i;
+        buf[k] = {r[qp], Lw[qp], Ls[qp], dg[qp]};
+      }
+      const int j = d - i;
+      const bool act = row_ok && j >= 0 && j < ny;
+      double west = wave_shr1(prev);
+      if (wave > 0) {
+        const double h = lane_value(hv, k);
+        west = lane == 0 ? h : west;
+      }
+      double s = o.r - m_r;
+      if (px) {
+        const int jc = min(max(j, 0), ny - 1);
+        const double wx = *lds(&sh.wrap[jc]);
+        s = (wrap_row && act) ? s - sh.lx[jc] * wx : s;
+      }
+      s = has_w ? s - o.lw * west : s;
+      if (py) s = j == ny - 1 ? s - ly * first : s;
+      s = j > 0 ? s - o.ls * prev : s;
+      const double den = floor_max(o.dg, floor);
+      double cur, out;
       if (kIldlt) {
         cur = s;
-        a.y[q] = s / floor_max(a.dg[q], floor);
+        out = s / den;
       } else {
-        cur = s / floor_max(a.dg[q], floor);
-        a.y[q] = cur;
+        cur = s / den;
+        out = cur;
       }
-      if (j == 0) first = cur;
-      if (a.periodic_x && i == 0) sh.wrap[j] = cur;
+      y[act ? level_base(d, nx, ny) + i : sink] = out;
+      first = j == 0 ? cur : first;
+      if (px && wrap_src && act) *lds(&sh.wrap[j]) = cur;
+      prev = cur;
+      if (wave + 1 < S) publish(wave + 1, lane, 63, d - (wave * 64 + 63), ny, cur, sh);
     }
-    prev = cur;
-    forward_send(cur, wave, lane, S, d - (wave * 64 + 63), ny, sh);
   }
 }
 
+struct BackwardOps {
+  double y, dg, ls_n, lw_e;
+};
+
+// z = L^{-T} y, levels descending. Children of (i, j) in ascending row order: (i, j+1), the
+// periodic-y wrap (i, ny-1) when j = 0, (i+1, j), the periodic-x wrap (nx-1, j) when i = 0.
 template <bool kIldlt>
-__device__ void backward_solve(const SysArgs &a, SweepShared &sh, double floor) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+__device__ __forceinline__ void backward_solve(const SysArgs &a, SweepShared &sh, double floor) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
   const int S = (nx + 63) >> 6;
   if (wave >= S) return;
   const int i = wave * 64 + lane;
-  const int i_last = min(wave * 64 + 63, nx - 1);
-  const int d_beg = wave * 64, d_end = i_last + ny - 1;
+  const int d_beg = wave * 64, d_end = min(wave * 64 + 63, nx - 1) + ny - 1, d_last = nx + ny - 2;
+  const bool px = a.periodic_x != 0, py = a.periodic_y != 0;
+  const bool row_ok = i < nx, has_e = i + 1 < nx, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
+  const global_ptr<const double> yv = as_global<const double>(a.y);
+  const global_ptr<const double> Lw = as_global<const double>(a.Lw);
+  const global_ptr<const double> Ls = as_global<const double>(a.Ls);
+  const global_ptr<const double> dg = as_global<const double>(a.dg);
+  const global_ptr<double> z = as_global(a.z);
+  const int sink = a.n + kPad - 64 + lane;
+  const double ly = (py && row_ok) ? as_global(a.LY)[i] : 0.0;
+  auto fetch = [&](int d) {
+    const int dc = min(max(d, 0), d_last);
+    const int q = level_base(dc, nx, ny) + i;
+    const int qn = level_base(min(dc + 1, d_last), nx, ny) + i;  // (i, j+1); (i+1, j) follows it
+    return BackwardOps{yv[q], dg[q], Ls[qn], Lw[qn + 1]};
+  };
+  // levels d_end down to d_low (whole chunks; levels below d_beg are inactive for every lane)
+  const int d_low = d_end - ((d_end - d_beg + kPrefetch) / kPrefetch) * kPrefetch + 1;
   double prev = 0.0, first = 0.0;
-  for (int d = d_end; d >= d_beg; --d) {
-    const int j = d - i;
-    const bool act = i < nx && j >= 0 && j < ny;
-    double east = __shfl_down(prev, 1, 64);
-    east = backward_receive(east, wave, lane, S, d - (wave * 64 + 63), ny, sh);
-    double cur = 0.0;
-    if (act) {
-      const int q = sh.off[d] + i - level_lo(d, ny);
-      double s = a.y[q];
-      if (j + 1 < ny) {
-        const int qn = sh.off[d + 1] + i - level_lo(d + 1, ny);
-        s = s - a.Ls[qn] * prev;
+  BackwardOps buf[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) buf[k] = fetch(d_end - k);
+  for (int d0 = d_end; d0 >= d_low; d0 -= kPrefetch) {
+    const int j63_first = d0 - (wave * 64 + 63);  // column of lane 63 at level d0
+    const double hv = wave + 1 < S ? receive_batch<double, kPrefetch>(wave, lane, ny - 1 - j63_first, ny, sh) : 0.0;
+    if (wave > 0) reserve_batch(wave - 1, min(ny - 1 - (d0 - kPrefetch + 1 - wave * 64), ny - 1), sh);
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const int d = d0 - k;
+      const BackwardOps o = buf[k];
+      buf[k] = fetch(d - kPrefetch);
+      const int j = d - i;
+      const bool act = row_ok && j >= 0 && j < ny;
+      double east = wave_shl1(prev);
+      if (wave + 1 < S) {
+        const double h = lane_value(hv, k);
+        east = lane == 63 ? h : east;
       }
-      if (a.periodic_y && j == 0) s = s - a.LY[i] * first;
-      if (i + 1 < nx) {
-        const int qe = sh.off[d + 1] + i + 1 - level_lo(d + 1, ny);
-        s = s - a.Lw[qe] * east;
+      double s = o.y;
+      s = j + 1 < ny ? s - o.ls_n * prev : s;
+      if (py) s = j == 0 ? s - ly * first : s;
+      s = has_e ? s - o.lw_e * east : s;
+      if (px) {
+        const int jc = min(max(j, 0), ny - 1);
+        const double wx = *lds(&sh.wrap[jc]);
+        s = (wrap_src && act) ? s - sh.lx[jc] * wx : s;
       }
-      if (a.periodic_x && i == 0) s = s - a.LX[j] * sh.wrap[j];
-      cur = kIldlt ? s : s / floor_max(a.dg[q], floor);
-      a.z[q] = cur;
-      if (j == ny - 1) first = cur;
-      if (a.periodic_x && i == nx - 1) sh.wrap[j] = cur;
+      const double cur = kIldlt ? s : s / floor_max(o.dg, floor);
+      z[act ? level_base(d, nx, ny) + i : sink] = cur;
+      first = j == ny - 1 ? cur : first;
+      if (px && wrap_row && act) *lds(&sh.wrap[j]) = cur;
+      prev = cur;
+      if (wave > 0) publish(wave - 1, lane, 0, ny - 1 - (d - wave * 64), ny, cur, sh);
     }
-    prev = cur;
-    backward_send(cur, wave, lane, d - wave * 64, ny, sh);
   }
 }
 
-// (A p)_q in CSR column order, p read through the lazy projection p - m_p.
-__device__ __forceinline__ double spmv_row(const SysArgs &a, const SweepShared &sh, const double *p, double m_p,
-                                           int q, int i, int j) {
+// ---- LDS-staged sweeps ---------------------------------------------------------------------------
+//
+// With S <= kMaxStagedWaves sweep waves, the remaining waves of the block are loaders: they copy the
+// four operands of every (sweep wave, level) into an LDS ring, kStageUnit levels at a time, so the
+// sweep waves never wait on global memory. Ring slot of step t is t mod R; ready[w][slot] = t + 1
+// once filled (flag written after the data, LDS operations of a wave complete in order), used[w] =
+// steps sweep wave w has finished, which bounds how far loaders may run ahead.
+
+__device__ __forceinline__ int stage_ring(int S) { return S <= 4 ? 16 : 8; }
+
+// Steps every sweep wave walks: its 64 rows span 63 + ny levels, rounded up to whole units.
+__device__ __forceinline__ int stage_steps(int ny) { return ((63 + ny + kStageUnit - 1) / kStageUnit) * kStageUnit; }
+
+template <bool kBackward>
+__device__ __forceinline__ int step_level(int w, int t, int T) {
+  return kBackward ? w * 64 + T - 1 - t : w * 64 + t;
+}
+
+__device__ __forceinline__ volatile __attribute__((address_space(3))) double *stage_slot(SweepShared &sh, int w, int R,
+                                                                                        int slot, int lane) {
+  return lds(reinterpret_cast<volatile double *>(&sh.stage[((w * R + slot) * 4) * 64 + lane]));
+}
+
+// Loaders take units (sweep wave w, first step t0) in order of when they are needed: a forward
+// sweep wave w starts 64 w steps after wave 0, a backward one 64 (S-1-w) steps after wave S-1, so the
+// key is that offset plus t0. Everything a unit waits for (ring space of its wave, which in turn needs
+// hand-offs from the upstream wave) has a smaller key, so loaders processing their share of units in
+// key order cannot wait on each other in a cycle. Unit k of that order goes to loader k mod L.
+template <bool kBackward>
+__device__ __forceinline__ void stage_loader(const SysArgs &a, SweepShared &sh, int S) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int nx = a.nx, ny = a.ny, d_last = nx + ny - 2;
+  const int R = stage_ring(S), T = stage_steps(ny);
+  const int U = R / 2;  // levels per unit: two units of lookahead per ring
+  const int L = kWaves - S, me = wave - S;
+  const global_ptr<const double> f0 = as_global<const double>(kBackward ? a.y : a.r);
+  const global_ptr<const double> dg = as_global<const double>(a.dg);
+  const global_ptr<const double> Lw = as_global<const double>(a.Lw);
+  const global_ptr<const double> Ls = as_global<const double>(a.Ls);
+  int k_unit = 0;
+  for (int tau = 0; tau < 64 * (S - 1) + T; tau += U) {
+    for (int w = 0; w < S; ++w) {
+      const int t0 = tau - 64 * (kBackward ? (S - 1 - w) : w);
+      if (t0 < 0 || t0 >= T) continue;
+      if ((k_unit++) % L != me) continue;
+      wait_at_least(&sh.used[w], t0 + U - R, sh);
+      const int i = w * 64 + lane;
+      double v[kStageUnit][4];
+#pragma unroll
+      for (int k = 0; k < kStageUnit; ++k) {
+        if (k < U) {
+          const int d = min(max(step_level<kBackward>(w, t0 + k, T), 0), d_last);
+          const int q = level_base(d, nx, ny) + i;
+          if (kBackward) {
+            const int qn = level_base(min(d + 1, d_last), nx, ny) + i;  // (i, j+1); (i+1, j) follows it
+            v[k][0] = f0[q];
+            v[k][1] = dg[q];
+            v[k][2] = Ls[qn];
+            v[k][3] = Lw[qn + 1];
+          } else {
+            v[k][0] = f0[q];
+            v[k][1] = Lw[q];
+            v[k][2] = Ls[q];
+            v[k][3] = dg[q];
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kStageUnit; ++k) {
+        if (k < U) {
+          volatile __attribute__((address_space(3))) double *slot = stage_slot(sh, w, R, (t0 + k) & (R - 1), lane);
+#pragma unroll
+          for (int f = 0; f < 4; ++f) slot[f * 64] = v[k][f];
+        }
+      }
+      if (lane == 0) {
+        for (int k = 0; k < U; ++k) *lds(&sh.ready[w * kMaxRing + ((t0 + k) & (R - 1))]) = t0 + k + 1;
+      }
+    }
+  }
+}
+
+// Sweep side of the staging: per chunk of kPrefetch levels, wait once for the chunk's units, read
+// all of its operands from LDS, then run the levels with no further LDS waits except the hand-off
+// batch. used[w] advances once per chunk.
+__device__ __forceinline__ void await_chunk(SweepShared &sh, int w, int R, int t0) {
+  const int U = R / 2;
+  for (int u = t0; u < t0 + kPrefetch; u += U) {
+    const int last = u + U - 1;
+    wait_at_least(&sh.ready[w * kMaxRing + (last & (R - 1))], last + 1, sh);
+  }
+}
+
+struct ChunkOps {
+  double f[kPrefetch][4];
+};
+
+__device__ __forceinline__ void read_chunk(SweepShared &sh, int w, int R, int t0, int lane, ChunkOps &c) {
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) {
+    volatile __attribute__((address_space(3))) double *p = stage_slot(sh, w, R, (t0 + k) & (R - 1), lane);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) c.f[k][f] = p[f * 64];
+  }
+}
+
+template <bool kIldlt, bool kPeriodic>
+__device__ __forceinline__ void staged_forward(const SysArgs &a, SweepShared &sh, double m_r, double floor, int S) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
+  const int R = stage_ring(S), T = stage_steps(ny);
+  const int i = wave * 64 + lane;
+  const bool px = kPeriodic && a.periodic_x != 0, py = kPeriodic && a.periodic_y != 0;
+  const bool row_ok = i < nx, has_w = i > 0, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
+  const global_ptr<double> y = as_global(a.y);
+  const int sink = a.n + kPad - 64 + lane;
+  const double ly = (py && row_ok) ? as_global(a.LY)[i] : 0.0;
+  double prev = 0.0, first = 0.0;
+  for (int t0 = 0; t0 < T; t0 += kPrefetch) {
+    const int d0 = wave * 64 + t0;
+    await_chunk(sh, wave, R, t0);
+    ChunkOps c;
+    read_chunk(sh, wave, R, t0, lane, c);
+    const double hv = wave > 0 ? receive_batch<double, kPrefetch>(wave, lane, t0, ny, sh) : 0.0;
+    if (wave + 1 < S) reserve_batch(wave + 1, min(d0 + kPrefetch - 1 - (wave * 64 + 63), ny - 1), sh);
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const int d = d0 + k;
+      const int j = d - i;
+      const bool act = row_ok && j >= 0 && j < ny;
+      double west = wave_shr1(prev);
+      if (wave > 0) {
+        const double h = lane_value(hv, k);
+        west = lane == 0 ? h : west;
+      }
+      double s = c.f[k][0] - m_r;
+      if (kPeriodic && px) {
+        const int jc = min(max(j, 0), ny - 1);
+        const double wx = *lds(&sh.wrap[jc]);
+        s = (wrap_row && act) ? s - sh.lx[jc] * wx : s;
+      }
+      s = has_w ? s - c.f[k][1] * west : s;
+      if (kPeriodic && py) s = j == ny - 1 ? s - ly * first : s;
+      s = j > 0 ? s - c.f[k][2] * prev : s;
+      const double den = floor_max(c.f[k][3], floor);
+      double cur, out;
+      if (kIldlt) {
+        cur = s;
+        out = s / den;
+      } else {
+        cur = s / den;
+        out = cur;
+      }
+      y[act ? level_base(d, nx, ny) + i : sink] = out;
+      if (kPeriodic) {
+        first = j == 0 ? cur : first;
+        if (px && wrap_src && act) *lds(&sh.wrap[j]) = cur;
+      }
+      prev = cur;
+      if (wave + 1 < S) publish_nb(wave + 1, lane, 63, d - (wave * 64 + 63), ny, cur, sh);
+    }
+    if (lane == 0) *lds(&sh.used[wave]) = t0 + kPrefetch;
+  }
+}
+
+template <bool kIldlt, bool kPeriodic>
+__device__ __forceinline__ void staged_backward(const SysArgs &a, SweepShared &sh, double floor, int S) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int nx = a.nx, ny = a.ny;
+  const int R = stage_ring(S), T = stage_steps(ny);
+  const int i = wave * 64 + lane;
+  const bool px = kPeriodic && a.periodic_x != 0, py = kPeriodic && a.periodic_y != 0;
+  const bool row_ok = i < nx, has_e = i + 1 < nx, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
+  const global_ptr<double> z = as_global(a.z);
+  const int sink = a.n + kPad - 64 + lane;
+  const double ly = (py && row_ok) ? as_global(a.LY)[i] : 0.0;
+  double prev = 0.0, first = 0.0;
+  for (int t0 = 0; t0 < T; t0 += kPrefetch) {
+    const int d0 = wave * 64 + T - 1 - t0;        // levels descend
+    const int j63_first = d0 - (wave * 64 + 63);  // column of lane 63 at level d0
+    await_chunk(sh, wave, R, t0);
+    ChunkOps c;
+    read_chunk(sh, wave, R, t0, lane, c);
+    const double hv = wave + 1 < S ? receive_batch<double, kPrefetch>(wave, lane, ny - 1 - j63_first, ny, sh) : 0.0;
+    if (wave > 0) reserve_batch(wave - 1, min(ny - 1 - (d0 - kPrefetch + 1 - wave * 64), ny - 1), sh);
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      const int d = d0 - k;
+      const int j = d - i;
+      const bool act = row_ok && j >= 0 && j < ny;
+      double east = wave_shl1(prev);
+      if (wave + 1 < S) {
+        const double h = lane_value(hv, k);
+        east = lane == 63 ? h : east;
+      }
+      double s = c.f[k][0];
+      s = j + 1 < ny ? s - c.f[k][2] * prev : s;
+      if (kPeriodic && py) s = j == 0 ? s - ly * first : s;
+      s = has_e ? s - c.f[k][3] * east : s;
+      if (kPeriodic && px) {
+        const int jc = min(max(j, 0), ny - 1);
+        const double wx = *lds(&sh.wrap[jc]);
+        s = (wrap_src && act) ? s - sh.lx[jc] * wx : s;
+      }
+      const double cur = kIldlt ? s : s / floor_max(c.f[k][1], floor);
+      z[act ? level_base(d, nx, ny) + i : sink] = cur;
+      if (kPeriodic) {
+        first = j == ny - 1 ? cur : first;
+        if (px && wrap_row && act) *lds(&sh.wrap[j]) = cur;
+      }
+      prev = cur;
+      if (wave > 0) publish_nb(wave - 1, lane, 0, ny - 1 - (d - wave * 64), ny, cur, sh);
+    }
+    if (lane == 0) *lds(&sh.used[wave]) = t0 + kPrefetch;
+  }
+}
+
+__device__ __forceinline__ void reset_stage(SweepShared &sh) {
+  for (int k = threadIdx.x; k < kWaves * kMaxRing; k += kThreads) *lds(&sh.ready[k]) = 0;
+  if (threadIdx.x < kWaves) *lds(&sh.used[threadIdx.x]) = 0;
+}
+
+// Operator fields the SpMV needs, read once per solve.
+struct Stencil {
+  int nx, ny;
+  bool px, py;
+  global_ptr<const double> aP, aW, aS;
+};
+
+// (A p)_q in CSR column order, p read through the lazy projection p - m_p.
+__device__ __forceinline__ double spmv_row(const Stencil &A, const global_ptr<const double> p, double m_p, int q,
+                                           int i, int j) {
+  const int nx = A.nx, ny = A.ny;
+  const int d = i + j;
+  const int offm = d > 0 ? level_offset(d - 1, nx, ny) - level_lo(d - 1, ny) : 0;  // + i' gives level d-1
+  const int offp = level_offset(d + 1, nx, ny) - level_lo(d + 1, ny);                // + i' gives level d+1
   double s = 0.0;
-  if (a.periodic_x && i == nx - 1) {
-    const int qw = cell_pos(sh, 0, j, ny);
-    s += a.aW[qw] * (p[qw] - m_p);
+  if (A.px && i == nx - 1) {
+    const int qw = grid_pos(0, j, nx, ny);
+    s += A.aW[qw] * (p[qw] - m_p);
   }
-  if (i >= 1) s += a.aW[q] * (p[cell_pos(sh, i - 1, j, ny)] - m_p);
-  if (a.periodic_y && j == ny - 1) {
-    const int qs = cell_pos(sh, i, 0, ny);
-    s += a.aS[qs] * (p[qs] - m_p);
+  if (i >= 1) s += A.aW[q] * (p[offm + i - 1] - m_p);
+  if (A.py && j == ny - 1) {
+    const int qs = grid_pos(i, 0, nx, ny);
+    s += A.aS[qs] * (p[qs] - m_p);
   }
-  if (j >= 1) s += a.aS[q] * (p[cell_pos(sh, i, j - 1, ny)] - m_p);
-  s += a.aP[q] * (p[q] - m_p);
+  if (j >= 1) s += A.aS[q] * (p[offm + i] - m_p);
+  s += A.aP[q] * (p[q] - m_p);
   if (j <= ny - 2) {
-    const int qn = cell_pos(sh, i, j + 1, ny);
-    s += a.aS[qn] * (p[qn] - m_p);
+    const int qn = offp + i;
+    s += A.aS[qn] * (p[qn] - m_p);
   }
-  if (a.periodic_y && j == 0) s += a.aS[q] * (p[cell_pos(sh, i, ny - 1, ny)] - m_p);
+  if (A.py && j == 0) s += A.aS[q] * (p[grid_pos(i, ny - 1, nx, ny)] - m_p);
   if (i <= nx - 2) {
-    const int qe = cell_pos(sh, i + 1, j, ny);
-    s += a.aW[qe] * (p[qe] - m_p);
+    const int qe = offp + i + 1;
+    s += A.aW[qe] * (p[qe] - m_p);
   }
-  if (a.periodic_x && i == 0) s += a.aW[q] * (p[cell_pos(sh, nx - 1, j, ny)] - m_p);
+  if (A.px && i == 0) s += A.aW[q] * (p[grid_pos(nx - 1, j, nx, ny)] - m_p);
   return s;
 }
 
 template <bool kIldlt>
-__device__ void precondition(const SysArgs &a, SweepShared &sh, double m_r, double floor) {
+__device__ __forceinline__ void precondition(const SysArgs &a, SweepShared &sh, double m_r, double floor,
+                                             unsigned long long *ticks, unsigned long long &t) {
+  const int S = (a.nx + 63) >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool staged = S <= kMaxStagedWaves;
   reset_handoff(sh);
+  reset_stage(sh);
   __syncthreads();
-  forward_solve<kIldlt>(a, sh, m_r, floor);
+  if (!staged)
+    forward_solve<kIldlt>(a, sh, m_r, floor);
+  else if (wave < S) {
+    if (a.periodic_x || a.periodic_y)
+      staged_forward<kIldlt, true>(a, sh, m_r, floor, S);
+    else
+      staged_forward<kIldlt, false>(a, sh, m_r, floor, S);
+  }
+  else
+    stage_loader<false>(a, sh, S);
   __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long now = device_ticks();
+    ticks[4] += now - t;
+    t = now;
+  }
   reset_handoff(sh);
+  reset_stage(sh);
   __syncthreads();
-  backward_solve<kIldlt>(a, sh, floor);
+  if (!staged)
+    backward_solve<kIldlt>(a, sh, floor);
+  else if (wave < S) {
+    if (a.periodic_x || a.periodic_y)
+      staged_backward<kIldlt, true>(a, sh, floor, S);
+    else
+      staged_backward<kIldlt, false>(a, sh, floor, S);
+  }
+  else
+    stage_loader<true>(a, sh, S);
   __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long now = device_ticks();
+    ticks[5] += now - t;
+    t = now;
+  }
 }
 
+// Streaming passes visit positions tid, tid + T, tid + 2T, ... in ascending order; kUnroll of
+// them are loaded before any is used so each thread keeps several loads in flight.
+constexpr int kUnroll = 8;
+
 template <bool kIldlt>
-__device__ void solve_system(const SysArgs &a, SweepShared &sh) {
+__device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) {
   const int tid = threadIdx.x;
   const int n = a.n, nx = a.nx, ny = a.ny;
   int parity = 0;
-  for (int d = tid; d <= a.nlev; d += kThreads) sh.off[d] = a.lev_off[d];
+  unsigned long long ticks[kPhases];
+  for (int k = 0; k < kPhases; ++k) ticks[k] = 0;
+  unsigned long long t = device_ticks();
+  auto mark = [&](int phase) {
+    if (tid == 0) {
+      const unsigned long long now = device_ticks();
+      ticks[phase] += now - t;
+      t = now;
+    }
+  };
+  auto finish = [&]() {
+    if (tid == 0)
+      for (int k = 0; k < kPhases; ++k) a.rep->phase_ticks[k] = ticks[k];
+  };
   if (tid == 0) {
-    sh.spin_error = 0;
-    sh.status = kOk;
+    *lds(&sh.spin_error) = 0;
+    *lds(&sh.status) = kOk;
   }
   __syncthreads();
 
@@ -318,10 +715,13 @@ __device__ void solve_system(const SysArgs &a, SweepShared &sh) {
     dmax = dmax < aa ? aa : dmax;
     bsum = bsum + a.b[q];
   }
-  if (nodiag) sh.status = kNoDiag;
+  if (nodiag) *lds(&sh.status) = kNoDiag;
   const double total = block_sum1(bsum, sh, parity);
   const double max_diag = block_max1(dmax, sh, parity);
-  if (sh.status != kOk) return;
+  if (*lds(&sh.status) != kOk) {
+    finish();
+    return;
+  }
   const double nd = static_cast<double>(n) < 1.0 ? 1.0 : static_cast<double>(n);
   const double b_mean = a.projected ? total / nd : 0.0;
 
@@ -332,26 +732,42 @@ __device__ void solve_system(const SysArgs &a, SweepShared &sh) {
   __syncthreads();
   factor_sweep<kIldlt>(a, sh, floor);
   __syncthreads();
-  if (sh.spin_error) sh.status = kSweepFail;
-  if (sh.status != kOk) return;
+  if (a.periodic_x)
+    for (int j = tid; j < ny; j += kThreads) sh.lx[j] = as_global<const double>(a.LX)[j];
+  if (*lds(&sh.spin_error)) *lds(&sh.status) = kSweepFail;
+  if (*lds(&sh.status) != kOk) {
+    finish();
+    return;
+  }
+
+  const global_ptr<double> xv = as_global(a.x);
+  const global_ptr<double> rv = as_global(a.r);
+  const global_ptr<double> zv = as_global(a.z);
+  const global_ptr<double> bv = as_global(a.b);
+  const global_ptr<double> apv = as_global(a.y);  // Ap shares storage with the sweep intermediate
+  const global_ptr<const uint32_t> cellv = as_global<const uint32_t>(a.cell);
+  const Stencil stencil{nx, ny, a.periodic_x != 0, a.periodic_y != 0, as_global<const double>(a.aP),
+                        as_global<const double>(a.aW), as_global<const double>(a.aS)};
+  const bool projected = a.projected != 0;
 
   // r = b - A*0 (= b), x = 0; projections and norms
   double acc[2] = {0.0, 0.0};
   for (int q = tid; q < n; q += kThreads) {
-    const double bq = a.projected ? a.b[q] - b_mean : a.b[q];
-    a.b[q] = bq;
-    a.r[q] = bq - 0.0;
-    a.x[q] = 0.0;
+    const double bq = projected ? bv[q] - b_mean : bv[q];
+    const double rq = bq - 0.0;
+    bv[q] = bq;
+    rv[q] = rq;
+    xv[q] = 0.0;
     acc[0] = acc[0] + bq * bq;
-    acc[1] = acc[1] + a.r[q];
+    acc[1] = acc[1] + rq;
   }
   block_sum<2>(acc, sh, parity);
   const double bnorm = sqrt(acc[0] < 0.0 ? 0.0 : acc[0]);
-  double m_r = a.projected ? acc[1] / nd : 0.0;
+  double m_r = projected ? acc[1] / nd : 0.0;
   double rr = 0.0;
   for (int q = tid; q < n; q += kThreads) {
-    const double t = a.r[q] - m_r;
-    rr = rr + t * t;
+    const double tq = rv[q] - m_r;
+    rr = rr + tq * tq;
   }
   rr = block_sum1(rr, sh, parity);
   double abs_res = sqrt(rr < 0.0 ? 0.0 : rr);
@@ -360,111 +776,200 @@ __device__ void solve_system(const SysArgs &a, SweepShared &sh) {
   int iterations = 0;
   double m_x = 0.0, m_p = 0.0;
   const double tol = a.tolerance;
-  double *p_raw = a.pa, *p_prj = a.pb;
+  const global_ptr<double> p_raw = as_global(a.pa);
+  const global_ptr<double> p_prj = as_global(a.pb);
+  mark(0);
 
   if (!(rel_res < tol || abs_res < tol)) {
-    precondition<kIldlt>(a, sh, m_r, floor);
+    precondition<kIldlt>(a, sh, m_r, floor, ticks, t);
     double zs = 0.0;
-    for (int q = tid; q < n; q += kThreads) zs = zs + a.z[q];
+    for (int q = tid; q < n; q += kThreads) zs = zs + zv[q];
     zs = block_sum1(zs, sh, parity);
-    double m_z = a.projected ? zs / nd : 0.0;
+    double m_z = projected ? zs / nd : 0.0;
     double rz = 0.0;
     for (int q = tid; q < n; q += kThreads) {
-      const double zq = a.z[q] - m_z;
-      rz = rz + (a.r[q] - m_r) * zq;
+      const double zq = zv[q] - m_z;
+      rz = rz + (rv[q] - m_r) * zq;
       p_raw[q] = zq;
     }
     rz = block_sum1(rz, sh, parity);
     if (!isfinite(rz) || rz <= 0.0) {
-      if (tid == 0) sh.status = kRz0Fail;
+      if (tid == 0) *lds(&sh.status) = kRz0Fail;
+      finish();
       return;
     }
+    mark(0);
     const int limit = a.max_iterations < 1 ? 1 : a.max_iterations;
     for (int it = 0; it < limit; ++it) {
       // Ap and p^T A p
       double pap = 0.0;
-      for (int q = tid; q < n; q += kThreads) {
-        const uint32_t c = a.cell[q];
-        const double apq = spmv_row(a, sh, p_raw, m_p, q, static_cast<int>(c >> 16), static_cast<int>(c & 0xffffu));
-        const double pq = p_raw[q] - m_p;
-        a.y[q] = apq;  // Ap shares storage with the sweep intermediate
-        p_prj[q] = pq;
-        pap = pap + pq * apq;
+      for (int base = tid; base < n; base += kThreads * 4) {
+        double apq[4], pq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int q = base + u * kThreads;
+          const int qq = q < n ? q : 0;
+          const uint32_t c = cellv[qq];
+          apq[u] = spmv_row(stencil, p_raw, m_p, qq, static_cast<int>(c >> 16), static_cast<int>(c & 0xffffu));
+          pq[u] = p_raw[qq] - m_p;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int q = base + u * kThreads;
+          if (q < n) {
+            apv[q] = apq[u];
+            p_prj[q] = pq[u];
+            pap = pap + pq[u] * apq[u];
+          }
+        }
       }
       pap = block_sum1(pap, sh, parity);
+      mark(1);
       if (!isfinite(pap) || fabs(pap) < 1.0e-30) {
-        if (tid == 0) sh.status = kPapFail;
+        if (tid == 0) *lds(&sh.status) = kPapFail;
+        finish();
         return;
       }
       const double alpha = rz / pap;
       // x += alpha p, r -= alpha Ap (on projected x, r), with the update / iterate norms
       double up[4] = {0.0, 0.0, 0.0, 0.0};
-      for (int q = tid; q < n; q += kThreads) {
-        const double delta = alpha * p_prj[q];
-        const double xq = (a.x[q] - m_x) + delta;
-        const double rq = (a.r[q] - m_r) - alpha * a.y[q];
-        a.x[q] = xq;
-        a.r[q] = rq;
-        up[0] = up[0] + xq;
-        up[1] = up[1] + rq;
-        up[2] = up[2] + delta * delta;
-        up[3] = up[3] + xq * xq;
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        double xq[kUnroll], rq[kUnroll], pq[kUnroll], aq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          const int qq = q < n ? q : 0;
+          xq[u] = xv[qq];
+          rq[u] = rv[qq];
+          pq[u] = p_prj[qq];
+          aq[u] = apv[qq];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          if (q < n) {
+            const double delta = alpha * pq[u];
+            const double xn = (xq[u] - m_x) + delta;
+            const double rn = (rq[u] - m_r) - alpha * aq[u];
+            xv[q] = xn;
+            rv[q] = rn;
+            up[0] = up[0] + xn;
+            up[1] = up[1] + rn;
+            up[2] = up[2] + delta * delta;
+            up[3] = up[3] + xn * xn;
+          }
+        }
       }
       block_sum<4>(up, sh, parity);
-      m_x = a.projected ? up[0] / nd : 0.0;
-      m_r = a.projected ? up[1] / nd : 0.0;
+      mark(2);
+      m_x = projected ? up[0] / nd : 0.0;
+      m_r = projected ? up[1] / nd : 0.0;
       rr = 0.0;
-      for (int q = tid; q < n; q += kThreads) {
-        const double t = a.r[q] - m_r;
-        rr = rr + t * t;
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        double rq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          rq[u] = rv[q < n ? q : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          if (base + u * kThreads < n) {
+            const double tq = rq[u] - m_r;
+            rr = rr + tq * tq;
+          }
+        }
       }
       rr = block_sum1(rr, sh, parity);
+      mark(3);
       iterations = it + 1;
       iter_res = sqrt(up[2] / (up[3] < 1.0e-30 ? 1.0e-30 : up[3]));
       abs_res = sqrt(rr < 0.0 ? 0.0 : rr);
       rel_res = abs_res / (bnorm < 1.0e-30 ? 1.0e-30 : bnorm);
       if (rel_res < tol || abs_res < tol) break;
 
-      precondition<kIldlt>(a, sh, m_r, floor);
+      precondition<kIldlt>(a, sh, m_r, floor, ticks, t);
       zs = 0.0;
-      for (int q = tid; q < n; q += kThreads) zs = zs + a.z[q];
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        double zq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          zq[u] = zv[q < n ? q : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+          if (base + u * kThreads < n) zs = zs + zq[u];
+      }
       zs = block_sum1(zs, sh, parity);
-      m_z = a.projected ? zs / nd : 0.0;
+      mark(6);
+      m_z = projected ? zs / nd : 0.0;
       double rzn = 0.0;
-      for (int q = tid; q < n; q += kThreads) rzn = rzn + (a.r[q] - m_r) * (a.z[q] - m_z);
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        double zq[kUnroll], rq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          const int qq = q < n ? q : 0;
+          zq[u] = zv[qq];
+          rq[u] = rv[qq];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+          if (base + u * kThreads < n) rzn = rzn + (rq[u] - m_r) * (zq[u] - m_z);
+      }
       rzn = block_sum1(rzn, sh, parity);
+      mark(7);
       if (!isfinite(rzn) || rzn <= 0.0) {
-        if (tid == 0) sh.status = kRzFail;
+        if (tid == 0) *lds(&sh.status) = kRzFail;
+        finish();
         return;
       }
       const double beta = rzn / rz;
       double ps = 0.0;
-      for (int q = tid; q < n; q += kThreads) {
-        const double pn = (a.z[q] - m_z) + beta * p_prj[q];
-        p_raw[q] = pn;
-        ps = ps + pn;
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        double zq[kUnroll], pq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          const int qq = q < n ? q : 0;
+          zq[u] = zv[qq];
+          pq[u] = p_prj[qq];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          if (q < n) {
+            const double pn = (zq[u] - m_z) + beta * pq[u];
+            p_raw[q] = pn;
+            ps = ps + pn;
+          }
+        }
       }
       ps = block_sum1(ps, sh, parity);
-      m_p = a.projected ? ps / nd : 0.0;
+      mark(8);
+      m_p = projected ? ps / nd : 0.0;
       rz = rzn;
     }
   }
-  if (sh.spin_error) {
-    if (tid == 0) sh.status = kSweepFail;
+  if (*lds(&sh.spin_error)) {
+    if (tid == 0) *lds(&sh.status) = kSweepFail;
+    finish();
     return;
   }
 
   // scatter the projected x, subtract the interior mean when the system is singular
   double xs = 0.0;
-  for (int q = tid; q < n; q += kThreads) xs = xs + (a.x[q] - m_x);
+  for (int q = tid; q < n; q += kThreads) xs = xs + (xv[q] - m_x);
   xs = block_sum1(xs, sh, parity);
-  const double mean = a.projected ? xs / static_cast<double>(nx * ny) : 0.0;
+  const double mean = projected ? xs / static_cast<double>(nx * ny) : 0.0;
   for (int q = tid; q < n; q += kThreads) {
     const uint32_t c = a.cell[q];
     const int i = static_cast<int>(c >> 16), j = static_cast<int>(c & 0xffffu);
-    const double xq = a.x[q] - m_x;
-    a.p_out[i * ny + j] = a.projected ? xq - mean : xq;
+    const double xq = xv[q] - m_x;
+    a.p_out[i * ny + j] = projected ? xq - mean : xq;
   }
+  mark(9);
   if (tid == 0) {
     a.rep->iterations = iterations;
     a.rep->outer_cycles = 1;
@@ -472,17 +977,18 @@ __device__ void solve_system(const SysArgs &a, SweepShared &sh) {
     a.rep->abs = abs_res;
     a.rep->iter_res = iter_res;
   }
+  finish();
 }
 
 }  // namespace
 
 template <bool kIldlt>
-__global__ void __launch_bounds__(kThreads) fp64_solve_kernel(const SysArgs *systems) {
+__global__ void __launch_bounds__(kThreads) fp64_solve_kernel(const SysArgs *__restrict__ systems) {
   __shared__ SweepShared sh;
   const SysArgs &a = systems[blockIdx.x];
   solve_system<kIldlt>(a, sh);
   __syncthreads();
-  if (threadIdx.x == 0) a.rep->status = sh.status;
+  if (threadIdx.x == 0) a.rep->status = *lds(&sh.status);
 }
 
 hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream) {

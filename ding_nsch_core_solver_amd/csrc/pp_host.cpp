@@ -38,6 +38,7 @@ struct pp_ctx {
   pp::SysArgs *d_args = nullptr;
   pp::DevReport *d_rep = nullptr;
   pp::DevReport *h_rep = nullptr;  // pinned
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 };
 
 namespace {
@@ -87,8 +88,8 @@ const char *status_text(int status, int scheme) {
 int validate(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
   if (!ctx) return fail(ctx, report, PP_ERR_ARG, "null context");
   if (!pb) return fail(ctx, report, PP_ERR_ARG, "null problem");
-  if (pb->nx < 1 || pb->ny < 1 || pb->nx > pp::kMaxSide || pb->ny > pp::kMaxSide)
-    return fail(ctx, report, PP_ERR_ARG, "grid size outside 1..1024 per direction");
+  if (pb->nx < 1 || pb->ny < 1 || pb->nx > pp::kMaxRows || pb->ny > pp::kMaxSide)
+    return fail(ctx, report, PP_ERR_ARG, "grid size outside nx 1..512, ny 1..1024");
   if ((pb->periodic_x && pb->nx < 4) || (pb->periodic_y && pb->ny < 4))
     return fail(ctx, report, PP_ERR_ARG, "periodic direction needs at least 4 cells");
   if (pb->scheme < PP_ICPCG || pb->scheme > PP_SPLIT_ILDLT_PCG) return fail(ctx, report, PP_ERR_ARG, "unknown scheme");
@@ -140,8 +141,12 @@ int ensure_work(pp_ctx *ctx, size_t n, pp_report *report) {
   ctx->work = nullptr;
   ctx->fwork = nullptr;
   ctx->cap = 0;
-  PP_HIP_CHECK(hipMalloc(&ctx->work, n * kF64Arrays * sizeof(double)));
-  PP_HIP_CHECK(hipMalloc(&ctx->fwork, n * kF32Arrays * sizeof(float)));
+  const size_t stride = n + pp::kPad;
+  PP_HIP_CHECK(hipMalloc(&ctx->work, stride * kF64Arrays * sizeof(double)));
+  PP_HIP_CHECK(hipMalloc(&ctx->fwork, stride * kF32Arrays * sizeof(float)));
+  // padding is read by lanes outside the grid; keep it finite
+  PP_HIP_CHECK(hipMemset(ctx->work, 0, stride * kF64Arrays * sizeof(double)));
+  PP_HIP_CHECK(hipMemset(ctx->fwork, 0, stride * kF32Arrays * sizeof(float)));
   ctx->cap = n;
   return PP_OK;
 }
@@ -172,7 +177,7 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
   a.inner_tolerance = pb->inner_tolerance;
   a.lev_off = L.off;
   a.cell = L.cell;
-  const size_t n = ctx->cap;
+  const size_t n = ctx->cap + pp::kPad;  // array stride
   double *w = ctx->work;
   double **f64[kF64Arrays] = {&a.aP, &a.aW, &a.aS, &a.Lw, &a.Ls, &a.dg, &a.b, &a.x, &a.r, &a.pa, &a.pb, &a.z, &a.y};
   for (int k = 0; k < kF64Arrays; ++k) *f64[k] = w + static_cast<size_t>(k) * n;
@@ -188,6 +193,7 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
 
 int run_solve(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u, const double *d_v,
               const double *d_pn, const double *d_pnm1, double *d_out, pp_report *report, hipStream_t stream) {
+  if (pb->precision != PP_FP64) return fail(ctx, report, PP_ERR_ARG, "mixed precision is not available in this build");
   pp_ctx::Layout *L = nullptr;
   if (int rc = ensure_layout(ctx, pb->nx, pb->ny, &L, report)) return rc;
   if (int rc = ensure_work(ctx, static_cast<size_t>(pb->nx) * pb->ny, report)) return rc;
@@ -201,9 +207,13 @@ int run_solve(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const doub
   a.p_out = d_out;
   PP_HIP_CHECK(hipMemcpyAsync(ctx->d_args, &a, sizeof(a), hipMemcpyHostToDevice, stream));
   PP_HIP_CHECK(hipMemsetAsync(ctx->d_rep, 0, sizeof(pp::DevReport), stream));
+  PP_HIP_CHECK(hipEventRecord(ctx->ev_start, stream));
   PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_args, 1, a.ildlt != 0, stream));
+  PP_HIP_CHECK(hipEventRecord(ctx->ev_stop, stream));
   PP_HIP_CHECK(hipMemcpyAsync(ctx->h_rep, ctx->d_rep, sizeof(pp::DevReport), hipMemcpyDeviceToHost, stream));
   PP_HIP_CHECK(hipStreamSynchronize(stream));
+  float kernel_ms = 0.0f;
+  PP_HIP_CHECK(hipEventElapsedTime(&kernel_ms, ctx->ev_start, ctx->ev_stop));
   const pp::DevReport &r = *ctx->h_rep;
   if (report) {
     std::memset(report, 0, sizeof(*report));
@@ -212,6 +222,8 @@ int run_solve(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const doub
     report->relative_residual = r.rel;
     report->absolute_residual = r.abs;
     report->iterate_residual = r.iter_res;
+    report->kernel_ms = kernel_ms;
+    for (int k = 0; k < pp::kPhases; ++k) report->phase_ms[k] = static_cast<double>(r.phase_ticks[k]) * 1.0e-5;
     report->status = PP_OK;
   }
   if (r.status != pp::kOk) return fail(ctx, report, r.status, status_text(r.status, pb->scheme));
@@ -236,6 +248,8 @@ int pp_create(pp_ctx **out, int device) {
   PP_HIP_CHECK(hipMalloc(&ctx->d_args, sizeof(pp::SysArgs)));
   PP_HIP_CHECK(hipMalloc(&ctx->d_rep, sizeof(pp::DevReport)));
   PP_HIP_CHECK(hipHostMalloc(&ctx->h_rep, sizeof(pp::DevReport), hipHostMallocDefault));
+  PP_HIP_CHECK(hipEventCreate(&ctx->ev_start));
+  PP_HIP_CHECK(hipEventCreate(&ctx->ev_stop));
   *out = ctx;
   return PP_OK;
 }
@@ -256,6 +270,8 @@ void pp_destroy(pp_ctx *ctx) {
   if (ctx->d_args) (void)hipFree(ctx->d_args);
   if (ctx->d_rep) (void)hipFree(ctx->d_rep);
   if (ctx->h_rep) (void)hipHostFree(ctx->h_rep);
+  if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+  if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

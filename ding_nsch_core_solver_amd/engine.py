@@ -29,6 +29,9 @@ SCHEME_CODES = {
 }
 
 
+PHASES = ("setup", "spmv", "update", "rnorm", "forward", "backward", "zmean", "rz", "pupdate", "output")
+
+
 class PPProblem(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int), ("ny", ctypes.c_int),
@@ -46,7 +49,8 @@ class PPReport(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int), ("outer_cycles", ctypes.c_int),
         ("relative_residual", ctypes.c_double), ("absolute_residual", ctypes.c_double),
-        ("iterate_residual", ctypes.c_double), ("status", ctypes.c_int),
+        ("iterate_residual", ctypes.c_double), ("kernel_ms", ctypes.c_double),
+        ("phase_ms", ctypes.c_double * 10), ("status", ctypes.c_int),
         ("message", ctypes.c_char * 256),
     ]
 
@@ -120,7 +124,8 @@ def _report_dict(rep: PPReport) -> dict:
     return {
         "iterations": rep.iterations, "outer_cycles": rep.outer_cycles,
         "relative_residual": rep.relative_residual, "absolute_residual": rep.absolute_residual,
-        "iterate_residual": rep.iterate_residual, "status": rep.status,
+        "iterate_residual": rep.iterate_residual, "kernel_ms": rep.kernel_ms,
+        "phase_ms": dict(zip(PHASES, list(rep.phase_ms))), "status": rep.status,
         "message": rep.message.decode(errors="replace"),
     }
 

@@ -47,7 +47,7 @@ enum pp_precision {
 enum pp_bc_type { PP_BC_UNSET = 0, PP_BC_DIRICHLET = 1, PP_BC_NEUMANN = 2 };
 
 typedef struct pp_problem {
-  int nx, ny;             /* cells; 1..1024 each, >= 4 along a periodic direction              */
+  int nx, ny;             /* cells; nx 1..512, ny 1..1024, >= 4 along a periodic direction     */
   double dx, dy;          /* the Solver's dx_, dy_ (lx / nx, ly / ny)                            */
   double dt;              /* cfg_.dt                                                              */
   int periodic_x, periodic_y;
@@ -69,6 +69,9 @@ typedef struct pp_report {
   double relative_residual;  /* the backend's return value                                        */
   double absolute_residual;
   double iterate_residual;   /* LinearSolveReport::iterate_residual                              */
+  double kernel_ms;          /* device time of the solve kernel(s), HIP events on the launch stream */
+  double phase_ms[10];       /* device time per phase (setup, SpMV, x/r update, |r|, forward sweep,
+                                backward sweep, mean(z), r.z, p update, output)                   */
   int status;                /* 0 ok; see pp_status                                               */
   char message[256];
 } pp_report;

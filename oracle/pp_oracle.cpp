@@ -55,9 +55,9 @@ enum Reduction { kSequential = 0, kPairwise = 1, kCanonical = 2 };
 // Summation orders. kSequential is std::accumulate / dot_product (the reference). kPairwise is a
 // recursive halving tree over natural order. kCanonical is the HIP engine's block reduction:
 // positions of the wavefront layout (level d = i + j ascending, i ascending within a level) are
-// dealt round-robin to 1024 threads, each thread sums its positions in order from 0.0, lanes of a
-// 64-wide wave combine with offsets 32..1 (v[l] += v[l + off]) and the 16 wave sums combine with
-// offsets 8..1.
+// dealt round-robin to 512 threads, each thread sums its positions in order from 0.0, lanes of a
+// 64-wide wave combine with offsets 32..1 (v[l] += v[l + off]) and the 8 wave sums combine with
+// offsets 4..1.
 struct Reducer {
   int mode = kSequential;
   std::vector<int> perm;  // kCanonical: natural index of each wavefront position
@@ -88,7 +88,7 @@ struct Reducer {
       }
       return n ? t[0] : 0.0;
     }
-    constexpr std::size_t T = 1024, W = 16;
+    constexpr std::size_t T = 512, W = 8;
     double part[T];
     for (std::size_t t = 0; t < T; ++t) {
       double s = 0.0;
