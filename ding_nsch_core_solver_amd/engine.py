@@ -6,6 +6,7 @@ path: every call goes to the HIP engine and fails loudly when it is missing.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
 
 import numpy as np
@@ -71,11 +72,30 @@ def library_available() -> bool:
     return os.path.exists(LIB_PATH)
 
 
+def _bind_shared_hip_runtime():
+    """Make the engine use the same HIP runtime as PyTorch in this process.
+
+    PyTorch-ROCm wheels bundle their own libamdhip64 (soname libamdhip64.so.7) and load it by
+    the file name libamdhip64.so. If the engine were loaded first it would map the system
+    runtime under that soname, and PyTorch would then map a second runtime that sees no GPU
+    (and whose stream handles the engine could not use). Loading PyTorch's copy first, by path,
+    makes the engine's libamdhip64.so.7 dependency resolve to it. Without PyTorch installed the
+    engine links the system runtime as usual."""
+    spec = importlib.util.find_spec("torch")
+    for root in (spec.submodule_search_locations or []) if spec else []:
+        path = os.path.join(root, "lib", "libamdhip64.so")
+        if os.path.exists(path):
+            ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+            return path
+    return None
+
+
 def lib():
     global _lib
     if _lib is None:
         if not library_available():
             raise FileNotFoundError(f"HIP engine not built: {LIB_PATH} (python -m ding_nsch_core_solver_amd.build)")
+        _bind_shared_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         dp = ctypes.POINTER(ctypes.c_double)
         vp = ctypes.c_void_p

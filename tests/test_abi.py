@@ -1,0 +1,73 @@
+"""The C ABI library builds for gfx950, loads on the host and exports every declared entry point.
+
+No compute call is made here (no GPU in the CPU suite)."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "pp_solver.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pp_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for required in ("pp_create", "pp_destroy", "pp_last_error", "pp_solve", "pp_solve_device"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    from ding_nsch_core_solver_amd import engine
+    if not engine.library_available():
+        pytest.skip("engine not built (python -m ding_nsch_core_solver_amd.build)")
+    lib = ctypes.CDLL(engine.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), f"{name} missing from {engine.LIB_PATH}"
+    assert set(engine.EXPORTS) <= set(declared_functions())
+
+
+def test_library_is_built_for_gfx950():
+    """The offload bundle embedded by hipcc names its target triple and processor."""
+    from ding_nsch_core_solver_amd import engine
+    if not engine.library_available():
+        pytest.skip("engine not built")
+    data = open(engine.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in data
+
+
+def test_one_hip_runtime_when_engine_loads_before_torch():
+    """Loading the engine and then importing torch must map a single libamdhip64/libhsa-runtime64:
+    a second runtime in the process sees no GPU and cannot use torch's streams."""
+    pytest.importorskip("torch")
+    from ding_nsch_core_solver_amd import engine
+    if not engine.library_available():
+        pytest.skip("engine not built")
+    code = ("from ding_nsch_core_solver_amd import engine as E; E.lib(); import torch; "
+            "m = open('/proc/self/maps').read().splitlines(); "
+            "print('\\n'.join(sorted({l.split()[-1] for l in m if 'libamdhip64' in l or 'libhsa-runtime64' in l})))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=REPO, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    libs = out.stdout.split()
+    assert sum("libamdhip64" in p for p in libs) == 1, libs
+    assert sum("libhsa-runtime64" in p for p in libs) == 1, libs
+
+
+def test_report_struct_layout_matches_header():
+    """The ctypes mirror of pp_report must have the header's field order and sizes."""
+    from ding_nsch_core_solver_amd import engine
+    names = [f[0] for f in engine.PPReport._fields_]
+    assert names == ["iterations", "outer_cycles", "relative_residual", "absolute_residual", "iterate_residual",
+                     "kernel_ms", "phase_ms", "status", "message"]
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct pp_report"):text.index("} pp_report;")]
+    for n in names:
+        assert re.search(rf"\b{n}\b", body), n
