@@ -71,6 +71,18 @@ __device__ double split_explicit(const SysArgs &a, int i, int j) {
 
 __device__ __forceinline__ double drop_tiny(double value) { return fabs(value) < 1.0e-18 ? 0.0 : value; }
 
+// Coefficient mode: inverse density of x-face i / y-face j; along a periodic direction face nx
+// (ny) is face 0.
+__device__ __forceinline__ double k_xface(const SysArgs &a, int i, int j) {
+  if (a.periodic_x && i == a.nx) i = 0;
+  return a.ku[i * a.ny + j];
+}
+
+__device__ __forceinline__ double k_yface(const SysArgs &a, int i, int j) {
+  if (a.periodic_y && j == a.ny) j = 0;
+  return a.kv[i * (a.ny + 1) + j];
+}
+
 // One row of pressure.cpp:133-195 in the wavefront layout. Returns the diagonal.
 __device__ void assemble_cell(const SysArgs &a, int q, int i, int j) {
   const bool has_e = a.periodic_x || i < a.nx - 1;
@@ -79,7 +91,12 @@ __device__ void assemble_cell(const SysArgs &a, int q, int i, int j) {
   const bool has_s = a.periodic_y || j > 0;
   const double dx = a.dx, dy = a.dy;
   double ce, cw, cn, cs;
-  if (a.split) {
+  if (a.coeff_mode) {
+    ce = k_xface(a, i + 1, j) / (dx * dx);
+    cw = k_xface(a, i, j) / (dx * dx);
+    cn = k_yface(a, i, j + 1) / (dy * dy);
+    cs = k_yface(a, i, j) / (dy * dy);
+  } else if (a.split) {
     ce = cw = 1.0 / (dx * dx);
     cn = cs = 1.0 / (dy * dy);
   } else {
@@ -89,7 +106,10 @@ __device__ void assemble_cell(const SysArgs &a, int q, int i, int j) {
     cs = 1.0 / (rho_yface(a, i, j) * dy * dy);
   }
   double diag = 0.0;
-  double b = a.split ? -(split_explicit(a, i, j) + a.rho_ref * divergence(a, i, j) / a.dt) : -divergence(a, i, j) / a.dt;
+  double b = a.coeff_mode ? a.src[i * a.ny + j]
+             : a.split    ? -(split_explicit(a, i, j) + a.rho_ref * divergence(a, i, j) / a.dt)
+                          : -divergence(a, i, j) / a.dt;
+  const bool rhs_bc = !a.coeff_mode || a.rhs_bc;
   auto boundary = [&](int type, double value, double coeff, double spacing) {
     if (type == 0) {  // unset edges are homogeneous Neumann (operators.cpp:36-60)
       type = 2;
@@ -97,8 +117,8 @@ __device__ void assemble_cell(const SysArgs &a, int q, int i, int j) {
     }
     if (type == 1) {
       diag += 2.0 * coeff;
-      b += 2.0 * coeff * value;
-    } else {
+      if (rhs_bc) b += 2.0 * coeff * value;
+    } else if (rhs_bc) {
       b += coeff * value * spacing;
     }
   };

@@ -191,20 +191,17 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
   a.rep = ctx->d_rep;
 }
 
-int run_solve(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u, const double *d_v,
-              const double *d_pn, const double *d_pnm1, double *d_out, pp_report *report, hipStream_t stream) {
+// Fills the geometry/work part of `a` for problem pb; field pointers are the caller's.
+int prepare(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *report) {
   if (pb->precision != PP_FP64) return fail(ctx, report, PP_ERR_ARG, "mixed precision is not available in this build");
   pp_ctx::Layout *L = nullptr;
   if (int rc = ensure_layout(ctx, pb->nx, pb->ny, &L, report)) return rc;
   if (int rc = ensure_work(ctx, static_cast<size_t>(pb->nx) * pb->ny, report)) return rc;
-  pp::SysArgs a;
   fill_args(ctx, pb, *L, a);
-  a.rho = d_rho;
-  a.u = d_u;
-  a.v = d_v;
-  a.p_n = d_pn;
-  a.p_nm1 = d_pnm1;
-  a.p_out = d_out;
+  return PP_OK;
+}
+
+int run_solve(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *report, hipStream_t stream) {
   PP_HIP_CHECK(hipMemcpyAsync(ctx->d_args, &a, sizeof(a), hipMemcpyHostToDevice, stream));
   PP_HIP_CHECK(hipMemsetAsync(ctx->d_rep, 0, sizeof(pp::DevReport), stream));
   PP_HIP_CHECK(hipEventRecord(ctx->ev_start, stream));
@@ -228,6 +225,62 @@ int run_solve(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const doub
   }
   if (r.status != pp::kOk) return fail(ctx, report, r.status, status_text(r.status, pb->scheme));
   ctx->last_error.clear();
+  return PP_OK;
+}
+
+int ensure_stage(pp_ctx *ctx, size_t need, pp_report *report) {
+  if (need <= ctx->stage_cap) return PP_OK;
+  if (ctx->stage) (void)hipFree(ctx->stage);
+  ctx->stage = nullptr;
+  ctx->stage_cap = 0;
+  PP_HIP_CHECK(hipMalloc(&ctx->stage, need * sizeof(double)));
+  ctx->stage_cap = need;
+  return PP_OK;
+}
+
+// Coefficient-level solves and the adjoint are defined on the non-split operator.
+int validate_coeffs(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
+  if (int rc = validate(ctx, pb, report)) return rc;
+  if (pb->scheme != PP_ICPCG && pb->scheme != PP_ILDLT_PCG)
+    return fail(ctx, report, PP_ERR_ARG, "coefficient-level solves and the adjoint take PP_ICPCG or PP_ILDLT_PCG");
+  return PP_OK;
+}
+
+int run_coeff_solve(pp_ctx *ctx, const pp_problem *pb, const double *d_ku, const double *d_kv, const double *d_src,
+                    int rhs_bc, double *d_out, pp_report *report, hipStream_t s) {
+  pp::SysArgs a;
+  if (int rc = prepare(ctx, pb, a, report)) return rc;
+  a.coeff_mode = 1;
+  a.rhs_bc = rhs_bc;
+  a.ku = d_ku;
+  a.kv = d_kv;
+  a.src = d_src;
+  a.p_out = d_out;
+  return run_solve(ctx, pb, a, report, s);
+}
+
+int run_adjoint(pp_ctx *ctx, const pp_problem *pb, const double *d_ku, const double *d_kv, const double *d_p,
+                const double *d_g, double *d_lam, double *d_dku, double *d_dkv, pp_report *report, hipStream_t s) {
+  // lam = A^+ P g: the same solve with g as the source and no boundary terms on the right-hand side
+  if (int rc = run_coeff_solve(ctx, pb, d_ku, d_kv, d_g, 0, d_lam, report, s)) return rc;
+  pp::GradArgs g;
+  std::memset(&g, 0, sizeof(g));
+  g.nx = pb->nx;
+  g.ny = pb->ny;
+  g.dx = pb->dx;
+  g.dy = pb->dy;
+  g.periodic_x = pb->periodic_x ? 1 : 0;
+  g.periodic_y = pb->periodic_y ? 1 : 0;
+  for (int k = 0; k < 4; ++k) {
+    g.bc_type[k] = pb->bc_type[k];
+    g.bc_value[k] = pb->bc_value[k];
+  }
+  g.p = d_p;
+  g.lam = d_lam;
+  g.dku = d_dku;
+  g.dkv = d_dkv;
+  PP_HIP_CHECK(pp::launch_face_gradient(g, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
   return PP_OK;
 }
 
@@ -286,7 +339,15 @@ int pp_solve_device(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, cons
   if (!d_u || !d_v || !d_p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
   PP_HIP_CHECK(hipSetDevice(ctx->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  return run_solve(ctx, pb, d_rho, d_u, d_v, d_p_n, d_p_nm1, d_p_out, report, s);
+  pp::SysArgs a;
+  if (int rc = prepare(ctx, pb, a, report)) return rc;
+  a.rho = d_rho;
+  a.u = d_u;
+  a.v = d_v;
+  a.p_n = d_p_n;
+  a.p_nm1 = d_p_nm1;
+  a.p_out = d_p_out;
+  return run_solve(ctx, pb, a, report, s);
 }
 
 int pp_solve(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, const double *u, int ld_u,
@@ -300,13 +361,7 @@ int pp_solve(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, c
   PP_HIP_CHECK(hipSetDevice(ctx->device));
   const size_t n = static_cast<size_t>(nx) * ny;
   const size_t need = 4 * n + static_cast<size_t>(nx + 1) * ny + static_cast<size_t>(nx) * (ny + 1);
-  if (need > ctx->stage_cap) {
-    if (ctx->stage) (void)hipFree(ctx->stage);
-    ctx->stage = nullptr;
-    ctx->stage_cap = 0;
-    PP_HIP_CHECK(hipMalloc(&ctx->stage, need * sizeof(double)));
-    ctx->stage_cap = need;
-  }
+  if (int rc = ensure_stage(ctx, need, report)) return rc;
   double *d_rho = ctx->stage;
   double *d_u = d_rho + n;
   double *d_v = d_u + static_cast<size_t>(nx + 1) * ny;
@@ -323,11 +378,80 @@ int pp_solve(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, c
   PP_HIP_CHECK(up(d_v, v, ld_v, nx, ny + 1));
   if (p_n) PP_HIP_CHECK(up(d_pn, p_n, ld_pn, nx, ny));
   if (p_nm1) PP_HIP_CHECK(up(d_pnm1, p_nm1, ld_pnm1, nx, ny));
-  const int rc = run_solve(ctx, pb, d_rho, d_u, d_v, p_n ? d_pn : nullptr, p_nm1 ? d_pnm1 : nullptr, d_out, report, s);
-  if (rc != PP_OK) return rc;
+  pp::SysArgs a;
+  if (int rc = prepare(ctx, pb, a, report)) return rc;
+  a.rho = d_rho;
+  a.u = d_u;
+  a.v = d_v;
+  a.p_n = p_n ? d_pn : nullptr;
+  a.p_nm1 = p_nm1 ? d_pnm1 : nullptr;
+  a.p_out = d_out;
+  if (int rc = run_solve(ctx, pb, a, report, s)) return rc;
   PP_HIP_CHECK(hipMemcpy2DAsync(p_out, static_cast<size_t>(ld_out) * sizeof(double), d_out,
                                 static_cast<size_t>(ny) * sizeof(double), static_cast<size_t>(ny) * sizeof(double),
                                 static_cast<size_t>(nx), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  return PP_OK;
+}
+
+int pp_solve_coeffs_device(pp_ctx *ctx, const pp_problem *pb, const double *d_ku, const double *d_kv,
+                           const double *d_src, double *d_p_out, pp_report *report, void *stream) {
+  if (int rc = validate_coeffs(ctx, pb, report)) return rc;
+  if (!d_ku || !d_kv || !d_src || !d_p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return run_coeff_solve(ctx, pb, d_ku, d_kv, d_src, 1, d_p_out, report, s);
+}
+
+int pp_adjoint_device(pp_ctx *ctx, const pp_problem *pb, const double *d_ku, const double *d_kv, const double *d_p,
+                      const double *d_dLdp, double *d_dLdsrc, double *d_dLdku, double *d_dLdkv, pp_report *report,
+                      void *stream) {
+  if (int rc = validate_coeffs(ctx, pb, report)) return rc;
+  if (!d_ku || !d_kv || !d_p || !d_dLdp || !d_dLdsrc || !d_dLdku || !d_dLdkv)
+    return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return run_adjoint(ctx, pb, d_ku, d_kv, d_p, d_dLdp, d_dLdsrc, d_dLdku, d_dLdkv, report, s);
+}
+
+int pp_solve_coeffs(pp_ctx *ctx, const pp_problem *pb, const double *ku, const double *kv, const double *src,
+                    double *p_out, pp_report *report) {
+  if (int rc = validate_coeffs(ctx, pb, report)) return rc;
+  if (!ku || !kv || !src || !p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  const size_t n = static_cast<size_t>(pb->nx) * pb->ny;
+  const size_t nu = static_cast<size_t>(pb->nx + 1) * pb->ny, nv = static_cast<size_t>(pb->nx) * (pb->ny + 1);
+  if (int rc = ensure_stage(ctx, nu + nv + 2 * n, report)) return rc;
+  double *d_ku = ctx->stage, *d_kv = d_ku + nu, *d_src = d_kv + nv, *d_out = d_src + n;
+  hipStream_t s = ctx->stream;
+  PP_HIP_CHECK(hipMemcpyAsync(d_ku, ku, nu * sizeof(double), hipMemcpyHostToDevice, s));
+  PP_HIP_CHECK(hipMemcpyAsync(d_kv, kv, nv * sizeof(double), hipMemcpyHostToDevice, s));
+  PP_HIP_CHECK(hipMemcpyAsync(d_src, src, n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (int rc = run_coeff_solve(ctx, pb, d_ku, d_kv, d_src, 1, d_out, report, s)) return rc;
+  PP_HIP_CHECK(hipMemcpyAsync(p_out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  return PP_OK;
+}
+
+int pp_adjoint(pp_ctx *ctx, const pp_problem *pb, const double *ku, const double *kv, const double *p,
+               const double *dLdp, double *dLdsrc, double *dLdku, double *dLdkv, pp_report *report) {
+  if (int rc = validate_coeffs(ctx, pb, report)) return rc;
+  if (!ku || !kv || !p || !dLdp || !dLdsrc || !dLdku || !dLdkv) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  const size_t n = static_cast<size_t>(pb->nx) * pb->ny;
+  const size_t nu = static_cast<size_t>(pb->nx + 1) * pb->ny, nv = static_cast<size_t>(pb->nx) * (pb->ny + 1);
+  if (int rc = ensure_stage(ctx, 2 * (nu + nv) + 3 * n, report)) return rc;
+  double *d_ku = ctx->stage, *d_kv = d_ku + nu, *d_p = d_kv + nv, *d_g = d_p + n, *d_lam = d_g + n;
+  double *d_dku = d_lam + n, *d_dkv = d_dku + nu;
+  hipStream_t s = ctx->stream;
+  PP_HIP_CHECK(hipMemcpyAsync(d_ku, ku, nu * sizeof(double), hipMemcpyHostToDevice, s));
+  PP_HIP_CHECK(hipMemcpyAsync(d_kv, kv, nv * sizeof(double), hipMemcpyHostToDevice, s));
+  PP_HIP_CHECK(hipMemcpyAsync(d_p, p, n * sizeof(double), hipMemcpyHostToDevice, s));
+  PP_HIP_CHECK(hipMemcpyAsync(d_g, dLdp, n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (int rc = run_adjoint(ctx, pb, d_ku, d_kv, d_p, d_g, d_lam, d_dku, d_dkv, report, s)) return rc;
+  PP_HIP_CHECK(hipMemcpyAsync(dLdsrc, d_lam, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipMemcpyAsync(dLdku, d_dku, nu * sizeof(double), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipMemcpyAsync(dLdkv, d_dkv, nv * sizeof(double), hipMemcpyDeviceToHost, s));
   PP_HIP_CHECK(hipStreamSynchronize(s));
   return PP_OK;
 }

@@ -30,6 +30,11 @@ struct SysArgs {
   // fields (natural layout)
   const double *rho, *u, *v, *p_n, *p_nm1;
   double *p_out;
+  // coefficient mode (coeff_mode = 1): face inverse densities k_u ((nx+1) x ny), k_v (nx x (ny+1))
+  // and a source term src (nx x ny) replace rho, u and v; c_f = k_f / h^2, b = src (+ the boundary
+  // terms when rhs_bc = 1)
+  int coeff_mode, rhs_bc;
+  const double *ku, *kv, *src;
   // FP64 work arrays (wavefront layout, n each)
   double *aP, *aW, *aS;    // CSR diagonal, west entry, south entry (0 when absent or dropped)
   double *Lw, *Ls, *dg;    // factor: west/south lower entries, pivot (ICC d or ILDLT D)
@@ -43,5 +48,19 @@ struct SysArgs {
 };
 
 hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream);
+
+// Face gradients of the coefficient-mode adjoint: given the solution p and the adjoint solution lam
+// (both nx x ny, natural layout), writes dL/dk_u ((nx+1) x ny) and dL/dk_v (nx x (ny+1)).
+struct GradArgs {
+  int nx, ny;
+  double dx, dy;
+  int periodic_x, periodic_y;
+  int bc_type[4];
+  double bc_value[4];
+  const double *p, *lam;
+  double *dku, *dkv;
+};
+
+hipError_t launch_face_gradient(const GradArgs &g, hipStream_t stream);
 
 }  // namespace pp

@@ -65,7 +65,8 @@ class EngineError(RuntimeError):
 _lib = None
 
 # Every entry point declared in include/pp_solver.h.
-EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_solve", "pp_solve_device")
+EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_solve", "pp_solve_device", "pp_solve_coeffs",
+           "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device")
 
 
 def library_available() -> bool:
@@ -111,6 +112,15 @@ def lib():
         L.pp_solve_device.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(PPReport), vp]
         L.pp_solve_device.restype = ctypes.c_int
+        L.pp_solve_coeffs.argtypes = [vp, ctypes.POINTER(PPProblem), dp, dp, dp, dp, ctypes.POINTER(PPReport)]
+        L.pp_solve_coeffs.restype = ctypes.c_int
+        L.pp_solve_coeffs_device.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, ctypes.POINTER(PPReport), vp]
+        L.pp_solve_coeffs_device.restype = ctypes.c_int
+        L.pp_adjoint.argtypes = [vp, ctypes.POINTER(PPProblem), dp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(PPReport)]
+        L.pp_adjoint.restype = ctypes.c_int
+        L.pp_adjoint_device.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, vp, vp, vp,
+                                        ctypes.POINTER(PPReport), vp]
+        L.pp_adjoint_device.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -203,6 +213,49 @@ class Engine:
         if rc != 0:
             raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
         return _report_dict(rep)
+
+    def _check(self, rc, rep):
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return _report_dict(rep)
+
+    def solve_coeffs(self, case, ku, kv, src, scheme: int = PP_ICPCG, **kw):
+        """Coefficient-level solve on host arrays (k_u (nx+1)*ny, k_v nx*(ny+1), src nx*ny).
+        Returns (p (nx*ny,), report dict)."""
+        pb = make_problem(case, scheme, PP_FP64, **kw)
+        ku, kv, src = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (ku, kv, src))
+        assert ku.size == (case.nx + 1) * case.ny and kv.size == case.nx * (case.ny + 1) and src.size == case.n
+        out = np.zeros(case.n)
+        rep = PPReport()
+        rc = self._lib.pp_solve_coeffs(self.h, ctypes.byref(pb), _dp(ku), _dp(kv), _dp(src), _dp(out), ctypes.byref(rep))
+        return out, self._check(rc, rep)
+
+    def adjoint(self, case, ku, kv, p, dLdp, scheme: int = PP_ICPCG, **kw):
+        """Adjoint of solve_coeffs on host arrays. Returns (dL/dsrc, dL/dk_u, dL/dk_v, report dict)."""
+        pb = make_problem(case, scheme, PP_FP64, **kw)
+        ku, kv, p, dLdp = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (ku, kv, p, dLdp))
+        lam = np.zeros(case.n)
+        dku = np.zeros((case.nx + 1) * case.ny)
+        dkv = np.zeros(case.nx * (case.ny + 1))
+        rep = PPReport()
+        rc = self._lib.pp_adjoint(self.h, ctypes.byref(pb), _dp(ku), _dp(kv), _dp(p), _dp(dLdp), _dp(lam), _dp(dku),
+                                  _dp(dkv), ctypes.byref(rep))
+        return lam, dku, dkv, self._check(rc, rep)
+
+    def solve_coeffs_device(self, case, d_ku, d_kv, d_src, d_out, scheme: int = PP_ICPCG, stream=None, **kw):
+        pb = make_problem(case, scheme, PP_FP64, **kw)
+        rep = PPReport()
+        rc = self._lib.pp_solve_coeffs_device(self.h, ctypes.byref(pb), d_ku, d_kv, d_src, d_out, ctypes.byref(rep),
+                                              stream)
+        return self._check(rc, rep)
+
+    def adjoint_device(self, case, d_ku, d_kv, d_p, d_g, d_lam, d_dku, d_dkv, scheme: int = PP_ICPCG, stream=None,
+                       **kw):
+        pb = make_problem(case, scheme, PP_FP64, **kw)
+        rep = PPReport()
+        rc = self._lib.pp_adjoint_device(self.h, ctypes.byref(pb), d_ku, d_kv, d_p, d_g, d_lam, d_dku, d_dkv,
+                                         ctypes.byref(rep), stream)
+        return self._check(rc, rep)
 
     def last_error(self) -> str:
         return self._lib.pp_last_error(self.h).decode(errors="replace")

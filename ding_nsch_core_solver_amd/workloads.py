@@ -121,3 +121,33 @@ def batched_cases(count=256, seed=0, tolerance=1.0e-8):
         cases.append(droplet_case(nx, ny, seed=1000 + k, density_ratio=dr, radius=radius, cx=cx, cy=cy,
                                   tolerance=tolerance))
     return cases
+
+
+def face_inverse_density(case):
+    """k_f = 1/rho_f on x-faces ((nx+1) x ny) and y-faces (nx x (ny+1)) with the reference's face
+    averages: mean of the two cells, the adjacent cell on a wall, the wrap mean on a periodic edge
+    (operators.cpp:330-350)."""
+    nx, ny = case.nx, case.ny
+    r = np.asarray(case.rho, dtype=np.float64).reshape(nx, ny)
+    ru = np.empty((nx + 1, ny))
+    ru[1:nx] = 0.5 * (r[:-1] + r[1:])
+    if case.periodic_x:
+        ru[0] = ru[nx] = 0.5 * (r[nx - 1] + r[0])
+    else:
+        ru[0], ru[nx] = r[0], r[nx - 1]
+    rv = np.empty((nx, ny + 1))
+    rv[:, 1:ny] = 0.5 * (r[:, :-1] + r[:, 1:])
+    if case.periodic_y:
+        rv[:, 0] = rv[:, ny] = 0.5 * (r[:, ny - 1] + r[:, 0])
+    else:
+        rv[:, 0], rv[:, ny] = r[:, 0], r[:, ny - 1]
+    return np.ascontiguousarray(1.0 / ru), np.ascontiguousarray(1.0 / rv)
+
+
+def divergence_source(case):
+    """-div(u*)/dt per cell (pressure.cpp:150 with operators.cpp:388-390), nx x ny."""
+    nx, ny = case.nx, case.ny
+    u = np.asarray(case.u, dtype=np.float64).reshape(nx + 1, ny)
+    v = np.asarray(case.v, dtype=np.float64).reshape(nx, ny + 1)
+    div = (u[1:] - u[:-1]) / case.dx + (v[:, 1:] - v[:, :-1]) / case.dy
+    return np.ascontiguousarray(-div / case.dt)

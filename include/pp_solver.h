@@ -118,6 +118,38 @@ int pp_solve_device(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, cons
                     const double *d_v, const double *d_p_n, const double *d_p_nm1, double *d_p_out,
                     pp_report *report, void *stream);
 
+/*
+ * Coefficient-level solve (no counterpart in the reference; the differentiable form of the
+ * non-split operator of pressure.cpp:123-196). Inputs are face inverse densities and a source:
+ *   k_u  (nx+1) x ny   k_f = 1/rho_f on x-faces (face i lies between cells i-1 and i)
+ *   k_v  nx x (ny+1)   on y-faces
+ *   src  nx x ny       right-hand side before boundary terms (the reference's -div/dt)
+ * The operator has c_f = k_f / h^2 on every face; wall faces contribute the reference's boundary
+ * terms (Dirichlet 2c to the diagonal and 2c*value to b, Neumann c*value*h to b). Along a periodic
+ * direction face nx (ny) is face 0 and its entry is not read. The rest is the FP64 path: regularize,
+ * factor (scheme PP_ICPCG or PP_ILDLT_PCG), PCG from 0, mean subtraction. Arrays contiguous.
+ */
+int pp_solve_coeffs(pp_ctx *ctx, const pp_problem *pb, const double *k_u, const double *k_v, const double *src,
+                    double *p_out, pp_report *report);
+int pp_solve_coeffs_device(pp_ctx *ctx, const pp_problem *pb, const double *d_k_u, const double *d_k_v,
+                           const double *d_src, double *d_p_out, pp_report *report, void *stream);
+
+/*
+ * Adjoint of pp_solve_coeffs (no counterpart in the reference). Given the solution p and dL/dp,
+ * solves A lam = P dL/dp with the same PCG (P the zero-mean projection when no edge is Dirichlet)
+ * and returns
+ *   dL/dsrc = lam                                                        (nx x ny)
+ *   dL/dk_f = -(lam_a - lam_b)(p_a - p_b) / h^2 on interior and periodic faces,
+ *             -2 lam_c (p_c - value) / h^2 on Dirichlet walls, lam_c value h / h^2 on Neumann walls
+ *   into dL/dk_u ((nx+1) x ny) and dL/dk_v (nx x (ny+1)); the duplicate periodic face gets 0.
+ * report describes the adjoint solve.
+ */
+int pp_adjoint(pp_ctx *ctx, const pp_problem *pb, const double *k_u, const double *k_v, const double *p,
+               const double *dLdp, double *dLdsrc, double *dLdk_u, double *dLdk_v, pp_report *report);
+int pp_adjoint_device(pp_ctx *ctx, const pp_problem *pb, const double *d_k_u, const double *d_k_v,
+                      const double *d_p, const double *d_dLdp, double *d_dLdsrc, double *d_dLdk_u,
+                      double *d_dLdk_v, pp_report *report, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

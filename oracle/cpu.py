@@ -62,6 +62,12 @@ def lib():
                                  ctypes.c_char_p, ctypes.c_int]
         L.or_sys_pcg.restype = ctypes.c_int
         L.or_apply_pressure_bc.argtypes = [ctypes.POINTER(OrProblem), dp]
+        L.or_solve_coeffs.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), dp, dp, dp, ctypes.c_int, dp, ip, dp,
+                                      ctypes.c_char_p, ctypes.c_int]
+        L.or_solve_coeffs.restype = ctypes.c_int
+        L.or_adjoint.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), dp, dp, dp, dp, ctypes.c_int, dp, dp, dp,
+                                 ip, dp, ctypes.c_char_p, ctypes.c_int]
+        L.or_adjoint.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -101,6 +107,40 @@ def pressure_solve(scheme: str, case, reduction: int = SEQUENTIAL):
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return p, it.value, res.value
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+
+
+def solve_coeffs(scheme: str, case, ku, kv, src, reduction: int = SEQUENTIAL):
+    """Coefficient-level solve: c_f = k_f / h^2 on faces, right-hand side src (+ boundary terms).
+    Returns (p (nx*ny,), iterations, relative residual)."""
+    ku, kv, src = _f64(ku), _f64(kv), _f64(src)
+    assert ku.size == (case.nx + 1) * case.ny and kv.size == case.nx * (case.ny + 1) and src.size == case.n
+    p = np.zeros(case.n)
+    it, res = ctypes.c_int(0), ctypes.c_double(0.0)
+    err = ctypes.create_string_buffer(512)
+    rc = lib().or_solve_coeffs(SCHEMES[scheme], ctypes.byref(problem(case)), _dp(ku), _dp(kv), _dp(src), reduction,
+                               _dp(p), ctypes.byref(it), ctypes.byref(res), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return p, it.value, res.value
+
+
+def adjoint(scheme: str, case, ku, kv, p, dLdp, reduction: int = SEQUENTIAL):
+    """FP64 adjoint of solve_coeffs. Returns (dL/dsrc, dL/dk_u, dL/dk_v, iterations, residual)."""
+    ku, kv, p, dLdp = _f64(ku), _f64(kv), _f64(p), _f64(dLdp)
+    lam = np.zeros(case.n)
+    dku = np.zeros((case.nx + 1) * case.ny)
+    dkv = np.zeros(case.nx * (case.ny + 1))
+    it, res = ctypes.c_int(0), ctypes.c_double(0.0)
+    err = ctypes.create_string_buffer(512)
+    rc = lib().or_adjoint(SCHEMES[scheme], ctypes.byref(problem(case)), _dp(ku), _dp(kv), _dp(p), _dp(dLdp), reduction,
+                          _dp(lam), _dp(dku), _dp(dkv), ctypes.byref(it), ctypes.byref(res), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return lam, dku, dkv, it.value, res.value
 
 
 def apply_pressure_bc(case, field_ghosted: np.ndarray) -> np.ndarray:

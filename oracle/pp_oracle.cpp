@@ -203,11 +203,28 @@ struct System {
   double assembly_max_diag = 0.0;
 };
 
+// Coefficient mode (no reference counterpart): face inverse densities k_u ((nx+1) x ny), k_v
+// (nx x (ny+1)) give c_f = k_f / h^2, src gives the right-hand side before boundary terms, and the
+// boundary right-hand-side terms are added only when rhs_bc is set. Along a periodic direction the
+// last face is face 0.
+struct Coeffs {
+  const double *ku, *kv, *src;
+  bool rhs_bc;
+  double kx(const OrProblem &pb, int i, int j) const {
+    if (pb.periodic_x && i == pb.nx) i = 0;
+    return ku[static_cast<std::size_t>(i) * pb.ny + j];
+  }
+  double ky(const OrProblem &pb, int i, int j) const {
+    if (pb.periodic_y && j == pb.ny) j = 0;
+    return kv[static_cast<std::size_t>(i) * (pb.ny + 1) + j];
+  }
+};
+
 // Assembly + regularization + CSR conversion (pressure.cpp:123-212, ch_sparse_krylov.hpp:105-133).
 // Each row gathers its entries in the reference's insertion order (east, west, north, south,
 // centre), accumulating repeated columns in that order, then sorts by column as std::map would.
 System assemble(const OrProblem &pb, const double *rho_in, const double *u, const double *v, const double *p_n,
-                const double *p_nm1, bool use_split, const Reducer &R) {
+                const double *p_nm1, bool use_split, const Reducer &R, const Coeffs *cf = nullptr) {
   Grid g(pb);
   const int nx = pb.nx, ny = pb.ny, n = nx * ny;
   const CellView rho{rho_in, ny};
@@ -237,14 +254,24 @@ System assemble(const OrProblem &pb, const double *rho_in, const double *u, cons
       const bool has_w = pb.periodic_x || i > 0;
       const bool has_n = pb.periodic_y || j < ny - 1;
       const bool has_s = pb.periodic_y || j > 0;
-      const double ce = use_split ? 1.0 / (g.dx * g.dx) : 1.0 / (rho_xface(g, rho, i + 1, j) * g.dx * g.dx);
-      const double cw = use_split ? 1.0 / (g.dx * g.dx) : 1.0 / (rho_xface(g, rho, i, j) * g.dx * g.dx);
-      const double cn = use_split ? 1.0 / (g.dy * g.dy) : 1.0 / (rho_yface(g, rho, i, j + 1) * g.dy * g.dy);
-      const double cs = use_split ? 1.0 / (g.dy * g.dy) : 1.0 / (rho_yface(g, rho, i, j) * g.dy * g.dy);
+      double ce, cw, cn, cs, b;
+      if (cf) {
+        ce = cf->kx(pb, i + 1, j) / (g.dx * g.dx);
+        cw = cf->kx(pb, i, j) / (g.dx * g.dx);
+        cn = cf->ky(pb, i, j + 1) / (g.dy * g.dy);
+        cs = cf->ky(pb, i, j) / (g.dy * g.dy);
+        b = cf->src[static_cast<std::size_t>(row)];
+      } else {
+        ce = use_split ? 1.0 / (g.dx * g.dx) : 1.0 / (rho_xface(g, rho, i + 1, j) * g.dx * g.dx);
+        cw = use_split ? 1.0 / (g.dx * g.dx) : 1.0 / (rho_xface(g, rho, i, j) * g.dx * g.dx);
+        cn = use_split ? 1.0 / (g.dy * g.dy) : 1.0 / (rho_yface(g, rho, i, j + 1) * g.dy * g.dy);
+        cs = use_split ? 1.0 / (g.dy * g.dy) : 1.0 / (rho_yface(g, rho, i, j) * g.dy * g.dy);
+        b = use_split ? -(split_explicit_divergence(g, rho, pext, rho_ref, i, j) +
+                          rho_ref * cell_divergence(g, u, v, i, j) / pb.dt)
+                      : -cell_divergence(g, u, v, i, j) / pb.dt;
+      }
+      const bool rhs_bc = !cf || cf->rhs_bc;
       double diag = 0.0;
-      double b = use_split ? -(split_explicit_divergence(g, rho, pext, rho_ref, i, j) +
-                               rho_ref * cell_divergence(g, u, v, i, j) / pb.dt)
-                           : -cell_divergence(g, u, v, i, j) / pb.dt;
 
       int cols[5];
       double vals[5];
@@ -264,8 +291,8 @@ System assemble(const OrProblem &pb, const double *rho_in, const double *u, cons
         const Bc bc = g.bc(side);
         if (bc.type == kDirichlet) {
           diag += 2.0 * coeff;
-          b += 2.0 * coeff * bc.value;
-        } else {
+          if (rhs_bc) b += 2.0 * coeff * bc.value;
+        } else if (rhs_bc) {
           b += coeff * bc.value * spacing;
         }
       };
@@ -539,21 +566,15 @@ void set_err(char *err, int len, const char *msg) {
   if (err && len > 0) std::snprintf(err, static_cast<std::size_t>(len), "%s", msg);
 }
 
-}  // namespace
 
-extern "C" {
-
-// Scheme codes shared with the engine: 1 icpcg, 2 ildlt_pcg, 3 split_icpcg, 4 split_ildlt_pcg.
-// Full pressure correction: assemble, factor, PCG, scatter, subtract the interior mean (when no
-// Dirichlet edge). p_out receives the interior (i*ny+j). Returns 0 or 1 (message in err).
-int or_pressure_solve(int scheme, const OrProblem *pb, const double *rho, const double *u, const double *v,
-                      const double *p_n, const double *p_nm1, int reduction, double *p_out, int *iterations,
-                      double *residual, char *err, int errlen) {
+int solve_common(int scheme, const OrProblem *pb, const double *rho, const double *u, const double *v,
+                 const double *p_n, const double *p_nm1, const Coeffs *cf, int reduction, double *p_out,
+                 int *iterations, double *residual, char *err, int errlen) {
   try {
     const bool split = scheme == 3 || scheme == 4;
     const int kind = (scheme == 2 || scheme == 4) ? 2 : 1;
     const Reducer R(reduction, pb->nx, pb->ny);
-    System sys = assemble(*pb, rho, u, v, p_n, p_nm1, split, R);
+    System sys = assemble(*pb, rho, u, v, p_n, p_nm1, split, R, cf);
     Factor F;
     if (!factorize(sys.A, kind, F))
       throw std::runtime_error(kind == 2 ? "ILDLT preconditioner construction failed"
@@ -575,6 +596,90 @@ int or_pressure_solve(int scheme, const OrProblem *pb, const double *rho, const 
     set_err(err, errlen, e.what());
     return 1;
   }
+}
+
+double wall_gradient(const Grid &g, int side, double lam, double p, double h) {
+  const Bc bc = g.bc(side);
+  if (bc.type == kDirichlet) return (-2.0 * lam * (p - bc.value)) / (h * h);
+  return (lam * bc.value * h) / (h * h);
+}
+
+double pair_gradient(const double *lam, const double *p, std::size_t a, std::size_t b, double h) {
+  return -((lam[b] - lam[a]) * (p[b] - p[a])) / (h * h);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Scheme codes shared with the engine: 1 icpcg, 2 ildlt_pcg, 3 split_icpcg, 4 split_ildlt_pcg.
+// Full pressure correction: assemble, factor, PCG, scatter, subtract the interior mean (when no
+// Dirichlet edge). p_out receives the interior (i*ny+j). Returns 0 or 1 (message in err).
+int or_pressure_solve(int scheme, const OrProblem *pb, const double *rho, const double *u, const double *v,
+                      const double *p_n, const double *p_nm1, int reduction, double *p_out, int *iterations,
+                      double *residual, char *err, int errlen) {
+  return solve_common(scheme, pb, rho, u, v, p_n, p_nm1, nullptr, reduction, p_out, iterations, residual, err, errlen);
+}
+
+// Coefficient-level solve (no reference counterpart): scheme 1 or 2 on c_f = k_f / h^2 with
+// right-hand side src plus the reference's boundary terms.
+int or_solve_coeffs(int scheme, const OrProblem *pb, const double *ku, const double *kv, const double *src,
+                    int reduction, double *p_out, int *iterations, double *residual, char *err, int errlen) {
+  if (scheme != 1 && scheme != 2) {
+    set_err(err, errlen, "coefficient-level solves and the adjoint take PP_ICPCG or PP_ILDLT_PCG");
+    return 1;
+  }
+  const Coeffs cf{ku, kv, src, true};
+  return solve_common(scheme, pb, nullptr, nullptr, nullptr, nullptr, nullptr, &cf, reduction, p_out, iterations,
+                      residual, err, errlen);
+}
+
+// FP64 adjoint of or_solve_coeffs for L(p) with gradient dLdp: lam = A^+ P dLdp by the same PCG
+// (source dLdp, no boundary right-hand-side terms), then per face
+//   dL/dk = -(lam_a - lam_b)(p_a - p_b) / h^2 (interior and periodic faces, reported on face 0),
+//           -2 lam_c (p_c - value) / h^2 (Dirichlet wall), lam_c value h / h^2 (Neumann wall).
+int or_adjoint(int scheme, const OrProblem *pb, const double *ku, const double *kv, const double *p,
+               const double *dLdp, int reduction, double *dLdsrc, double *dLdku, double *dLdkv, int *iterations,
+               double *residual, char *err, int errlen) {
+  if (scheme != 1 && scheme != 2) {
+    set_err(err, errlen, "coefficient-level solves and the adjoint take PP_ICPCG or PP_ILDLT_PCG");
+    return 1;
+  }
+  const Coeffs cf{ku, kv, dLdp, false};
+  if (int rc = solve_common(scheme, pb, nullptr, nullptr, nullptr, nullptr, nullptr, &cf, reduction, dLdsrc,
+                            iterations, residual, err, errlen))
+    return rc;
+  const Grid g(*pb);
+  const int nx = pb->nx, ny = pb->ny;
+  const double *lam = dLdsrc;
+  auto cell = [&](int i, int j) { return static_cast<std::size_t>(i) * ny + j; };
+  for (int i = 0; i <= nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      double out;
+      if (i >= 1 && i <= nx - 1)
+        out = pair_gradient(lam, p, cell(i - 1, j), cell(i, j), g.dx);
+      else if (pb->periodic_x)
+        out = i == 0 ? pair_gradient(lam, p, cell(nx - 1, j), cell(0, j), g.dx) : 0.0;
+      else if (i == 0)
+        out = wall_gradient(g, kLeft, lam[cell(0, j)], p[cell(0, j)], g.dx);
+      else
+        out = wall_gradient(g, kRight, lam[cell(nx - 1, j)], p[cell(nx - 1, j)], g.dx);
+      dLdku[static_cast<std::size_t>(i) * ny + j] = out;
+    }
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j <= ny; ++j) {
+      double out;
+      if (j >= 1 && j <= ny - 1)
+        out = pair_gradient(lam, p, cell(i, j - 1), cell(i, j), g.dy);
+      else if (pb->periodic_y)
+        out = j == 0 ? pair_gradient(lam, p, cell(i, ny - 1), cell(i, 0), g.dy) : 0.0;
+      else if (j == 0)
+        out = wall_gradient(g, kBottom, lam[cell(i, 0)], p[cell(i, 0)], g.dy);
+      else
+        out = wall_gradient(g, kTop, lam[cell(i, ny - 1)], p[cell(i, ny - 1)], g.dy);
+      dLdkv[static_cast<std::size_t>(i) * (ny + 1) + j] = out;
+    }
+  return 0;
 }
 
 // ---- system-level access for tests ------------------------------------------------------------
