@@ -32,7 +32,7 @@ constexpr int kMaxLevels = 2 * kMaxSide - 1;
 constexpr int kRing = 64;                 // hand-off ring depth between sweep waves
 constexpr int kPad = 1024;                // padding after every work array (lanes outside the grid)
 constexpr int kStageDoubles = 16384;      // 128 KB of LDS for the sweep operand ring
-constexpr int kMaxRing = 16;              // ring depth in levels (per sweep wave)
+constexpr int kMaxRing = 64;              // deepest operand ring in levels (per sweep wave)
 constexpr int kStageUnit = 8;             // levels a loader moves per step
 constexpr int kMaxStagedWaves = 6;        // more sweep waves leave too few loader waves
 constexpr int kSpinLimit = 1 << 24;       // bounded waits: a broken hand-off ends as an error

@@ -395,7 +395,10 @@ __device__ __forceinline__ void backward_solve(const SysArgs &a, SweepShared &sh
 // once filled (flag written after the data, LDS operations of a wave complete in order), used[w] =
 // steps sweep wave w has finished, which bounds how far loaders may run ahead.
 
-__device__ __forceinline__ int stage_ring(int S) { return S <= 4 ? 16 : 8; }
+// Ring depth R (levels per sweep wave, a power of two with S * R <= 64) and loader unit U: loaders
+// may run R - U levels ahead of the sweep.
+__device__ __forceinline__ int stage_ring(const SysArgs &a, int S) { return a.ring > 0 ? a.ring : (S <= 4 ? 16 : 8); }
+__device__ __forceinline__ int stage_unit(int R) { return R / 2 < kStageUnit ? R / 2 : kStageUnit; }
 
 // Steps every sweep wave walks: its 64 rows span 63 + ny levels, rounded up to whole units.
 __device__ __forceinline__ int stage_steps(int ny) { return ((63 + ny + kStageUnit - 1) / kStageUnit) * kStageUnit; }
@@ -419,8 +422,8 @@ template <bool kBackward>
 __device__ __forceinline__ void stage_loader(const SysArgs &a, SweepShared &sh, int S) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny, d_last = nx + ny - 2;
-  const int R = stage_ring(S), T = stage_steps(ny);
-  const int U = R / 2;  // levels per unit: two units of lookahead per ring
+  const int R = stage_ring(a, S), T = stage_steps(ny);
+  const int U = stage_unit(R);
   const int L = kWaves - S, me = wave - S;
   const global_ptr<const double> f0 = as_global<const double>(kBackward ? a.y : a.r);
   const global_ptr<const double> dg = as_global<const double>(a.dg);
@@ -473,7 +476,7 @@ __device__ __forceinline__ void stage_loader(const SysArgs &a, SweepShared &sh, 
 // all of its operands from LDS, then run the levels with no further LDS waits except the hand-off
 // batch. used[w] advances once per chunk.
 __device__ __forceinline__ void await_chunk(SweepShared &sh, int w, int R, int t0) {
-  const int U = R / 2;
+  const int U = stage_unit(R);
   for (int u = t0; u < t0 + kPrefetch; u += U) {
     const int last = u + U - 1;
     wait_at_least(&sh.ready[w * kMaxRing + (last & (R - 1))], last + 1, sh);
@@ -497,7 +500,7 @@ template <bool kIldlt, bool kPeriodic>
 __device__ __forceinline__ void staged_forward(const SysArgs &a, SweepShared &sh, double m_r, double floor, int S) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
-  const int R = stage_ring(S), T = stage_steps(ny);
+  const int R = stage_ring(a, S), T = stage_steps(ny);
   const int i = wave * 64 + lane;
   const bool px = kPeriodic && a.periodic_x != 0, py = kPeriodic && a.periodic_y != 0;
   const bool row_ok = i < nx, has_w = i > 0, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
@@ -556,7 +559,7 @@ template <bool kIldlt, bool kPeriodic>
 __device__ __forceinline__ void staged_backward(const SysArgs &a, SweepShared &sh, double floor, int S) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
-  const int R = stage_ring(S), T = stage_steps(ny);
+  const int R = stage_ring(a, S), T = stage_steps(ny);
   const int i = wave * 64 + lane;
   const bool px = kPeriodic && a.periodic_x != 0, py = kPeriodic && a.periodic_y != 0;
   const bool row_ok = i < nx, has_e = i + 1 < nx, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;

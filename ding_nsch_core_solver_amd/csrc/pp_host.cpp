@@ -39,6 +39,18 @@ struct pp_ctx {
   pp::DevReport *d_rep = nullptr;
   pp::DevReport *h_rep = nullptr;  // pinned
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+
+  // sweep operand ring depth per grid shape (pp_set_tuning)
+  std::map<std::pair<int, int>, int> tuning;
+
+  // batched solves: per-system work slab, wrap lines, arguments, reports, order and counters
+  size_t batch_cells = 0;       // capacity of batch_work in cells (sum of n + kPad)
+  int batch_cap = 0;            // capacity in systems
+  double *batch_work = nullptr;
+  double *batch_wrap = nullptr;
+  pp::SysArgs *d_batch_args = nullptr;
+  pp::DevReport *d_batch_rep = nullptr;
+  pp::DevReport *h_batch_rep = nullptr;  // pinned
 };
 
 namespace {
@@ -102,6 +114,8 @@ int validate(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
     return fail(ctx, report, PP_ERR_ARG, "pressure_bc_left cannot be set when the direction is periodic");
   if (pb->periodic_y && (pb->bc_type[2] != PP_BC_UNSET || pb->bc_type[3] != PP_BC_UNSET))
     return fail(ctx, report, PP_ERR_ARG, "pressure_bc_bottom cannot be set when the direction is periodic");
+  if (pb->ring_depth != 0 && !pp::valid_ring((pb->nx + 63) / 64, pb->ring_depth))
+    return fail(ctx, report, PP_ERR_ARG, "ring_depth must be a power of two >= 8 with ceil(nx/64) * ring_depth <= 64");
   return PP_OK;
 }
 
@@ -151,6 +165,12 @@ int ensure_work(pp_ctx *ctx, size_t n, pp_report *report) {
   return PP_OK;
 }
 
+// FP64 work arrays of one system: kF64Arrays arrays of `stride` doubles starting at w.
+void set_work(pp::SysArgs &a, double *w, size_t stride) {
+  double **f64[kF64Arrays] = {&a.aP, &a.aW, &a.aS, &a.Lw, &a.Ls, &a.dg, &a.b, &a.x, &a.r, &a.pa, &a.pb, &a.z, &a.y};
+  for (int k = 0; k < kF64Arrays; ++k) *f64[k] = w + static_cast<size_t>(k) * stride;
+}
+
 void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::SysArgs &a) {
   std::memset(&a, 0, sizeof(a));
   a.nx = pb->nx;
@@ -175,12 +195,15 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
   a.step_index = pb->step_index;
   a.inner_max_iterations = pb->inner_max_iterations;
   a.inner_tolerance = pb->inner_tolerance;
+  a.ring = pb->ring_depth;
+  if (a.ring == 0) {
+    auto t = ctx->tuning.find(std::make_pair(pb->nx, pb->ny));
+    if (t != ctx->tuning.end()) a.ring = t->second;
+  }
   a.lev_off = L.off;
   a.cell = L.cell;
+  set_work(a, ctx->work, ctx->cap + pp::kPad);
   const size_t n = ctx->cap + pp::kPad;  // array stride
-  double *w = ctx->work;
-  double **f64[kF64Arrays] = {&a.aP, &a.aW, &a.aS, &a.Lw, &a.Ls, &a.dg, &a.b, &a.x, &a.r, &a.pa, &a.pb, &a.z, &a.y};
-  for (int k = 0; k < kF64Arrays; ++k) *f64[k] = w + static_cast<size_t>(k) * n;
   float *fw = ctx->fwork;
   float **f32[kF32Arrays] = {&a.fP, &a.fW, &a.fS, &a.fLw, &a.fLs, &a.fRd, &a.fx, &a.fr, &a.fpa, &a.fpb, &a.fz, &a.fy};
   for (int k = 0; k < kF32Arrays; ++k) *f32[k] = fw + static_cast<size_t>(k) * n;
@@ -201,6 +224,19 @@ int prepare(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *report
   return PP_OK;
 }
 
+void copy_report(const pp::DevReport &r, double kernel_ms, pp_report *report) {
+  if (!report) return;
+  std::memset(report, 0, sizeof(*report));
+  report->iterations = r.iterations;
+  report->outer_cycles = r.outer_cycles;
+  report->relative_residual = r.rel;
+  report->absolute_residual = r.abs;
+  report->iterate_residual = r.iter_res;
+  report->kernel_ms = kernel_ms;
+  for (int k = 0; k < pp::kPhases; ++k) report->phase_ms[k] = static_cast<double>(r.phase_ticks[k]) * 1.0e-5;
+  report->status = PP_OK;
+}
+
 int run_solve(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *report, hipStream_t stream) {
   PP_HIP_CHECK(hipMemcpyAsync(ctx->d_args, &a, sizeof(a), hipMemcpyHostToDevice, stream));
   PP_HIP_CHECK(hipMemsetAsync(ctx->d_rep, 0, sizeof(pp::DevReport), stream));
@@ -212,20 +248,46 @@ int run_solve(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *repo
   float kernel_ms = 0.0f;
   PP_HIP_CHECK(hipEventElapsedTime(&kernel_ms, ctx->ev_start, ctx->ev_stop));
   const pp::DevReport &r = *ctx->h_rep;
-  if (report) {
-    std::memset(report, 0, sizeof(*report));
-    report->iterations = r.iterations;
-    report->outer_cycles = r.outer_cycles;
-    report->relative_residual = r.rel;
-    report->absolute_residual = r.abs;
-    report->iterate_residual = r.iter_res;
-    report->kernel_ms = kernel_ms;
-    for (int k = 0; k < pp::kPhases; ++k) report->phase_ms[k] = static_cast<double>(r.phase_ticks[k]) * 1.0e-5;
-    report->status = PP_OK;
-  }
+  copy_report(r, kernel_ms, report);
   if (r.status != pp::kOk) return fail(ctx, report, r.status, status_text(r.status, pb->scheme));
   ctx->last_error.clear();
   return PP_OK;
+}
+
+void free_batch(pp_ctx *ctx) {
+  if (ctx->batch_work) (void)hipFree(ctx->batch_work);
+  if (ctx->batch_wrap) (void)hipFree(ctx->batch_wrap);
+  if (ctx->d_batch_args) (void)hipFree(ctx->d_batch_args);
+  if (ctx->d_batch_rep) (void)hipFree(ctx->d_batch_rep);
+  if (ctx->h_batch_rep) (void)hipHostFree(ctx->h_batch_rep);
+  ctx->batch_work = ctx->batch_wrap = nullptr;
+  ctx->d_batch_args = nullptr;
+  ctx->d_batch_rep = ctx->h_batch_rep = nullptr;
+  ctx->batch_cells = 0;
+  ctx->batch_cap = 0;
+}
+
+int ensure_batch(pp_ctx *ctx, int count, size_t cells, pp_report *report) {
+  if (count <= ctx->batch_cap && cells <= ctx->batch_cells) return PP_OK;
+  const int cap = std::max(count, ctx->batch_cap);
+  const size_t cap_cells = std::max(cells, ctx->batch_cells);
+  free_batch(ctx);
+  PP_HIP_CHECK(hipMalloc(&ctx->batch_work, cap_cells * kF64Arrays * sizeof(double)));
+  PP_HIP_CHECK(hipMemset(ctx->batch_work, 0, cap_cells * kF64Arrays * sizeof(double)));  // finite padding
+  PP_HIP_CHECK(hipMalloc(&ctx->batch_wrap, static_cast<size_t>(cap) * 2 * pp::kMaxSide * sizeof(double)));
+  PP_HIP_CHECK(hipMalloc(&ctx->d_batch_args, static_cast<size_t>(cap) * sizeof(pp::SysArgs)));
+  PP_HIP_CHECK(hipMalloc(&ctx->d_batch_rep, static_cast<size_t>(cap) * sizeof(pp::DevReport)));
+  PP_HIP_CHECK(hipHostMalloc(&ctx->h_batch_rep, static_cast<size_t>(cap) * sizeof(pp::DevReport), hipHostMallocDefault));
+  ctx->batch_cap = cap;
+  ctx->batch_cells = cap_cells;
+  return PP_OK;
+}
+
+// Relative cost of one system for largest-first scheduling: iterations grow with the grid extent;
+// an iteration costs two level-sequential sweeps plus streaming passes over the n cells.
+double batch_cost(const pp_problem &pb) {
+  const double levels = pb.nx + pb.ny - 1, n = static_cast<double>(pb.nx) * pb.ny;
+  return (pb.nx + pb.ny) * (0.63 * levels + 0.0035 * n);
 }
 
 int ensure_stage(pp_ctx *ctx, size_t need, pp_report *report) {
@@ -323,6 +385,7 @@ void pp_destroy(pp_ctx *ctx) {
   if (ctx->d_args) (void)hipFree(ctx->d_args);
   if (ctx->d_rep) (void)hipFree(ctx->d_rep);
   if (ctx->h_rep) (void)hipHostFree(ctx->h_rep);
+  free_batch(ctx);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -454,6 +517,113 @@ int pp_adjoint(pp_ctx *ctx, const pp_problem *pb, const double *ku, const double
   PP_HIP_CHECK(hipMemcpyAsync(dLdkv, d_dkv, nv * sizeof(double), hipMemcpyDeviceToHost, s));
   PP_HIP_CHECK(hipStreamSynchronize(s));
   return PP_OK;
+}
+
+int pp_set_tuning(pp_ctx *ctx, int count, const int *nx, const int *ny, const int *ring_depth) {
+  if (!ctx) return PP_ERR_ARG;
+  pp_report *report = nullptr;
+  if (count < 0 || (count > 0 && (!nx || !ny || !ring_depth))) return fail(ctx, report, PP_ERR_ARG, "bad tuning table");
+  std::map<std::pair<int, int>, int> table;
+  for (int k = 0; k < count; ++k) {
+    if (nx[k] < 1 || ny[k] < 1 || nx[k] > pp::kMaxRows || ny[k] > pp::kMaxSide ||
+        !pp::valid_ring((nx[k] + 63) / 64, ring_depth[k]))
+      return fail(ctx, report, PP_ERR_ARG, "invalid tuning entry");
+    table[std::make_pair(nx[k], ny[k])] = ring_depth[k];
+  }
+  ctx->tuning.swap(table);
+  return PP_OK;
+}
+
+int pp_solve_batched_device(pp_ctx *ctx, int count, const pp_problem *problems, const double *const *d_rho,
+                            const double *const *d_u, const double *const *d_v, const double *const *d_p_n,
+                            const double *const *d_p_nm1, double *const *d_p_out, pp_report *reports,
+                            void *stream) {
+  pp_report *report = nullptr;
+  if (!ctx) return PP_ERR_ARG;
+  if (count < 0) return fail(ctx, report, PP_ERR_ARG, "negative batch size");
+  if (count == 0) return PP_OK;
+  if (!problems || !d_rho || !d_u || !d_v || !d_p_out) return fail(ctx, report, PP_ERR_ARG, "null batch array");
+  for (int k = 0; k < count; ++k) {
+    pp_report *rk = reports ? reports + k : nullptr;
+    const pp_problem *pb = problems + k;
+    if (int rc = validate(ctx, pb, rk)) return rc;
+    if (pb->precision != PP_FP64) return fail(ctx, rk, PP_ERR_ARG, "batched solves run in FP64");
+    const bool split = pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG;
+    if ((!d_rho[k] && !split) || !d_u[k] || !d_v[k] || !d_p_out[k]) return fail(ctx, rk, PP_ERR_ARG, "null field pointer");
+  }
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  size_t cells = 0;
+  std::vector<pp_ctx::Layout *> layouts(static_cast<size_t>(count));
+  for (int k = 0; k < count; ++k) {
+    if (int rc = ensure_layout(ctx, problems[k].nx, problems[k].ny, &layouts[static_cast<size_t>(k)], report)) return rc;
+    cells += static_cast<size_t>(problems[k].nx) * problems[k].ny + pp::kPad;
+  }
+  if (int rc = ensure_batch(ctx, count, cells, report)) return rc;
+
+  std::vector<pp::SysArgs> args(static_cast<size_t>(count));
+  size_t at = 0;
+  for (int k = 0; k < count; ++k) {
+    const pp_problem *pb = problems + k;
+    pp::SysArgs &a = args[static_cast<size_t>(k)];
+    fill_args(ctx, pb, *layouts[static_cast<size_t>(k)], a);
+    const size_t stride = static_cast<size_t>(pb->nx) * pb->ny + pp::kPad;
+    set_work(a, ctx->batch_work + at * kF64Arrays, stride);
+    at += stride;
+    a.fP = a.fW = a.fS = a.fLw = a.fLs = a.fRd = nullptr;
+    a.fx = a.fr = a.fpa = a.fpb = a.fz = a.fy = nullptr;
+    a.fLX = a.fLY = nullptr;
+    a.LX = ctx->batch_wrap + static_cast<size_t>(k) * 2 * pp::kMaxSide;
+    a.LY = a.LX + pp::kMaxSide;
+    a.rep = ctx->d_batch_rep + k;
+    a.rho = d_rho[k];
+    a.u = d_u[k];
+    a.v = d_v[k];
+    a.p_n = d_p_n ? d_p_n[k] : nullptr;
+    a.p_nm1 = d_p_nm1 ? d_p_nm1[k] : nullptr;
+    a.p_out = d_p_out[k];
+  }
+  // ICC systems then ILDLT systems, each largest first; the device copy of the arguments is in
+  // that order, reports stay indexed by system
+  std::vector<int> order(static_cast<size_t>(count));
+  for (int k = 0; k < count; ++k) order[static_cast<size_t>(k)] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const int fx = args[static_cast<size_t>(x)].ildlt, fy = args[static_cast<size_t>(y)].ildlt;
+    if (fx != fy) return fx < fy;
+    return batch_cost(problems[x]) > batch_cost(problems[y]);
+  });
+  std::vector<pp::SysArgs> sorted(static_cast<size_t>(count));
+  int n_icc = 0;
+  for (int k = 0; k < count; ++k) {
+    sorted[static_cast<size_t>(k)] = args[static_cast<size_t>(order[static_cast<size_t>(k)])];
+    n_icc += sorted[static_cast<size_t>(k)].ildlt ? 0 : 1;
+  }
+  const int n_ildlt = count - n_icc;
+
+  PP_HIP_CHECK(hipMemcpyAsync(ctx->d_batch_args, sorted.data(), sorted.size() * sizeof(pp::SysArgs),
+                              hipMemcpyHostToDevice, s));
+  PP_HIP_CHECK(hipMemsetAsync(ctx->d_batch_rep, 0, static_cast<size_t>(count) * sizeof(pp::DevReport), s));
+  PP_HIP_CHECK(hipEventRecord(ctx->ev_start, s));
+  if (n_icc > 0) PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_batch_args, n_icc, false, s));
+  if (n_ildlt > 0) PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_batch_args + n_icc, n_ildlt, true, s));
+  PP_HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
+  PP_HIP_CHECK(hipMemcpyAsync(ctx->h_batch_rep, ctx->d_batch_rep, static_cast<size_t>(count) * sizeof(pp::DevReport),
+                              hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  float kernel_ms = 0.0f;
+  PP_HIP_CHECK(hipEventElapsedTime(&kernel_ms, ctx->ev_start, ctx->ev_stop));
+  int first = PP_OK;
+  for (int k = 0; k < count; ++k) {
+    const pp::DevReport &r = ctx->h_batch_rep[k];
+    pp_report *rk = reports ? reports + k : nullptr;
+    copy_report(r, kernel_ms, rk);
+    if (r.status != pp::kOk) {
+      const int rc = fail(ctx, rk, r.status, status_text(r.status, problems[k].scheme));
+      if (first == PP_OK) first = rc;
+    }
+  }
+  if (first == PP_OK) ctx->last_error.clear();
+  return first;
 }
 
 }  // extern "C"

@@ -21,6 +21,7 @@ struct SysArgs {
   double tolerance;
   double rho_ref;
   int step_index;
+  int ring;                // sweep operand ring depth in levels (0: default for the wave count)
   // mixed precision controls
   int inner_max_iterations;
   double inner_tolerance;
@@ -47,7 +48,18 @@ struct SysArgs {
   DevReport *rep;
 };
 
+// One workgroup per system: workgroup k solves d_systems[k]. With more systems than CUs the
+// dispatcher starts workgroup k as soon as a CU is free, in index order, so an array sorted by
+// decreasing cost is scheduled largest first.
 hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream);
+
+// Default and valid operand ring depths for S sweep waves: a power of two, at least one 8-level
+// sweep chunk (the sweep releases ring space once per chunk, so a shallower ring would stall its
+// own loaders), and S * ring * 256 doubles within the LDS stage.
+inline int default_ring(int S) { return S <= 4 ? 16 : 8; }
+inline bool valid_ring(int S, int ring) {
+  return ring >= 8 && ring <= 64 && (ring & (ring - 1)) == 0 && S * ring <= 64;
+}
 
 // Face gradients of the coefficient-mode adjoint: given the solution p and the adjoint solution lam
 // (both nx x ny, natural layout), writes dL/dk_u ((nx+1) x ny) and dL/dk_v (nx x (ny+1)).

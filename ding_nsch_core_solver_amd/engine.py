@@ -43,6 +43,7 @@ class PPProblem(ctypes.Structure):
         ("max_iterations", ctypes.c_int), ("tolerance", ctypes.c_double),
         ("density_ratio", ctypes.c_double), ("step_index", ctypes.c_int),
         ("inner_max_iterations", ctypes.c_int), ("inner_tolerance", ctypes.c_double),
+        ("ring_depth", ctypes.c_int),
     ]
 
 
@@ -66,7 +67,7 @@ _lib = None
 
 # Every entry point declared in include/pp_solver.h.
 EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_solve", "pp_solve_device", "pp_solve_coeffs",
-           "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device")
+           "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device", "pp_set_tuning")
 
 
 def library_available() -> bool:
@@ -121,12 +122,19 @@ def lib():
         L.pp_adjoint_device.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, vp, vp, vp,
                                         ctypes.POINTER(PPReport), vp]
         L.pp_adjoint_device.restype = ctypes.c_int
+        pv = ctypes.POINTER(ctypes.c_void_p)
+        L.pp_solve_batched_device.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PPProblem), pv, pv, pv, pv, pv, pv,
+                                              ctypes.POINTER(PPReport), vp]
+        L.pp_solve_batched_device.restype = ctypes.c_int
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.pp_set_tuning.argtypes = [vp, ctypes.c_int, ip, ip, ip]
+        L.pp_set_tuning.restype = ctypes.c_int
         _lib = L
     return _lib
 
 
 def make_problem(case, scheme: int = PP_ICPCG, precision: int = PP_FP64, inner_max_iterations: int = 0,
-                 inner_tolerance: float = 0.0) -> PPProblem:
+                 inner_tolerance: float = 0.0, ring_depth: int = 0) -> PPProblem:
     pb = PPProblem()
     pb.nx, pb.ny = case.nx, case.ny
     pb.dx, pb.dy, pb.dt = case.dx, case.dy, case.dt
@@ -141,6 +149,7 @@ def make_problem(case, scheme: int = PP_ICPCG, precision: int = PP_FP64, inner_m
     pb.step_index = int(case.step_index)
     pb.inner_max_iterations = int(inner_max_iterations)
     pb.inner_tolerance = float(inner_tolerance)
+    pb.ring_depth = int(ring_depth)
     return pb
 
 
@@ -160,10 +169,24 @@ def _report_dict(rep: PPReport) -> dict:
     }
 
 
-class Engine:
-    """One engine context on one device (pp_create / pp_destroy)."""
+TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_gfx950.json")
 
-    def __init__(self, device: int = 0):
+
+def load_tuning(path: str = TUNING_FILE) -> dict:
+    """{(nx, ny): ring_depth} from the autotuner's table (empty if there is none)."""
+    import json
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        data = json.load(f)
+    return {(int(e["nx"]), int(e["ny"])): int(e["ring_depth"]) for e in data.get("entries", [])}
+
+
+class Engine:
+    """One engine context on one device (pp_create / pp_destroy). The autotuner's table
+    (tuning_gfx950.json beside this module) is installed when present, unless tuning=False."""
+
+    def __init__(self, device: int = 0, tuning: bool = True):
         self._lib = lib()
         h = ctypes.c_void_p()
         rc = self._lib.pp_create(ctypes.byref(h), device)
@@ -171,6 +194,17 @@ class Engine:
             raise EngineError(rc, f"pp_create failed ({rc})")
         self.h = h
         self.device = device
+        if tuning:
+            self.set_tuning(load_tuning())
+
+    def set_tuning(self, table: dict):
+        """Install {(nx, ny): ring_depth} (replaces the previous table)."""
+        keys = sorted(table)
+        arr = lambda vals: (ctypes.c_int * max(len(vals), 1))(*vals)  # noqa: E731
+        rc = self._lib.pp_set_tuning(self.h, len(keys), arr([k[0] for k in keys]), arr([k[1] for k in keys]),
+                                     arr([table[k] for k in keys]))
+        if rc != 0:
+            raise EngineError(rc, self.last_error())
 
     def close(self):
         if getattr(self, "h", None):
@@ -256,6 +290,28 @@ class Engine:
         rc = self._lib.pp_adjoint_device(self.h, ctypes.byref(pb), d_ku, d_kv, d_p, d_g, d_lam, d_dku, d_dkv,
                                          ctypes.byref(rep), stream)
         return self._check(rc, rep)
+
+    def solve_batched_device(self, cases, d_rho, d_u, d_v, d_out, d_pn=None, d_pnm1=None, scheme: int = PP_ICPCG,
+                             stream=None, **kw):
+        """Batched FP64 solve; d_* are lists of device pointers (ints). Returns the report dicts.
+        Raises EngineError for the first failing system (reports still describe every system)."""
+        count = len(cases)
+        schemes = scheme if isinstance(scheme, (list, tuple)) else [scheme] * count
+        pbs = (PPProblem * count)(*[make_problem(c, s, PP_FP64, **kw) for c, s in zip(cases, schemes)])
+        P = ctypes.c_void_p * count
+        ptrs = lambda xs: None if xs is None else P(*[x if x else None for x in xs])  # noqa: E731
+        reps = (PPReport * count)()
+        rc = self._lib.pp_solve_batched_device(self.h, count, pbs, ptrs(d_rho), ptrs(d_u), ptrs(d_v), ptrs(d_pn),
+                                               ptrs(d_pnm1), ptrs(d_out), reps, stream)
+        out = [_report_dict(r) for r in reps]
+        if rc != 0:
+            bad = next((k for k, r in enumerate(out) if r["status"] != 0), None)
+            msg = out[bad]["message"] if bad is not None else self.last_error()
+            err = EngineError(rc, msg)
+            err.reports = out
+            err.index = bad
+            raise err
+        return out
 
     def last_error(self) -> str:
         return self._lib.pp_last_error(self.h).decode(errors="replace")

@@ -61,6 +61,9 @@ typedef struct pp_problem {
   int step_index;         /* split schemes: 0 -> p_ext = p^n, else 2 p^n - p^{n-1} (operators.cpp:413-422) */
   int inner_max_iterations; /* PP_MIXED: cap per inner FP32 solve (<= 0: engine default)         */
   double inner_tolerance;   /* PP_MIXED: inner relative reduction (<= 0: engine default)         */
+  int ring_depth;           /* sweep operand ring depth in levels: 0 = the context's tuning table
+                               (pp_set_tuning) or the built-in default; else a power of two >= 8
+                               with ceil(nx/64) * ring_depth <= 64. Does not change results.     */
 } pp_problem;
 
 typedef struct pp_report {
@@ -149,6 +152,29 @@ int pp_adjoint(pp_ctx *ctx, const pp_problem *pb, const double *k_u, const doubl
 int pp_adjoint_device(pp_ctx *ctx, const pp_problem *pb, const double *d_k_u, const double *d_k_v,
                       const double *d_p, const double *d_dLdp, double *d_dLdsrc, double *d_dLdk_u,
                       double *d_dLdk_v, pp_report *report, void *stream);
+
+/*
+ * Batched solve (no counterpart in the reference, whose solver handles one system per call):
+ * `count` independent pressure corrections of any sizes, one launch per factor kind (ICC schemes,
+ * ILDLT schemes) with one workgroup per system, dispatched largest first so that CUs freed by small
+ * systems pick up the remaining ones. Device arrays are contiguous as for pp_solve_device; the
+ * pointer arrays are host arrays of `count` device pointers (d_p_n / d_p_nm1 may be NULL, or hold
+ * NULL entries). reports[k] gets system k's report; kernel_ms is the duration of the whole batch.
+ * Returns PP_OK, or the status of
+ * the first system (in index order) that failed.
+ */
+int pp_solve_batched_device(pp_ctx *ctx, int count, const pp_problem *problems, const double *const *d_rho,
+                            const double *const *d_u, const double *const *d_v, const double *const *d_p_n,
+                            const double *const *d_p_nm1, double *const *d_p_out, pp_report *reports,
+                            void *stream);
+
+/*
+ * Tuning table of the context (written by the project's autotuner, ding_nsch_core_solver_amd/
+ * autotune.py): the sweep operand ring depth to use for grids of exactly nx[k] x ny[k] when
+ * pp_problem.ring_depth is 0. Replaces the previous table; entries with an invalid depth are
+ * rejected. count = 0 clears it.
+ */
+int pp_set_tuning(pp_ctx *ctx, int count, const int *nx, const int *ny, const int *ring_depth);
 
 #ifdef __cplusplus
 }
