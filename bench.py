@@ -29,6 +29,7 @@ from ding_nsch_core_solver_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "fp64_solve_192x192_dr1000_traffic.txt")
+BATCH_TRAFFIC_FILE = os.path.join(REPO, "profiles", "batched_256_uneven_traffic.txt")
 
 
 def log(msg):
@@ -45,10 +46,11 @@ def iteration_bytes(n, fp32_inner=False):
     return (100 if fp32_inner else 200) * n
 
 
-def read_traffic():
-    if not os.path.exists(TRAFFIC_FILE):
+def read_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (profiles/)."""
+    if not os.path.exists(path):
         return None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         for line in f:
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
@@ -101,6 +103,61 @@ def cpu_baseline(case, seconds):
     }, result
 
 
+def batched_leg(eng, torch, dev, repeats):
+    """The 256-system uneven batch (grids from 64x64 to 256x256, density ratios 1..1000, FP64
+    icpcg to 1e-8), inputs resident in HBM; one pp_solve_batched_device call per repeat."""
+    cases = W.batched_cases(256, seed=0)
+    tens = []
+    for c in cases:
+        tens.append({k: torch.from_numpy(np.ascontiguousarray(getattr(c, k)).reshape(-1)).to(dev)
+                     for k in ("rho", "u", "v")})
+        tens[-1]["out"] = torch.zeros(c.n, dtype=torch.float64, device=dev)
+    ptr = lambda k: [t[k].data_ptr() for t in tens]  # noqa: E731
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def run():
+        return eng.solve_batched_device(cases, ptr("rho"), ptr("u"), ptr("v"), ptr("out"), scheme=E.PP_ICPCG,
+                                        stream=stream)
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms = []
+    for _ in range(repeats):
+        reps = run()
+        kernel_ms.append(reps[0]["kernel_ms"])
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t0) / repeats
+    kms = float(np.mean(kernel_ms))
+    cells = sum(c.n for c in cases)
+    alg = sum(setup_bytes(c.n) + r["iterations"] * iteration_bytes(c.n) for c, r in zip(cases, reps))
+    achieved = alg / (kms * 1e-3) / 1e9
+    out = {"systems": len(cases), "cells": cells, "solves_per_s": len(cases) / elapsed, "ms_per_batch": elapsed * 1e3,
+           "kernel_ms": kms, "iterations_total": int(sum(r["iterations"] for r in reps)),
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(BATCH_TRAFFIC_FILE),
+                        "algorithmic_bytes_per_launch": alg}}
+    return out, cases
+
+
+def batched_cpu_baseline(cases, threads):
+    """Reference icpcg on every system of the batch, one system per thread (the reference is
+    single-threaded; independent solves run concurrently), wall time of the whole batch."""
+    from concurrent.futures import ThreadPoolExecutor
+    try:
+        from oracle import ref as impl
+        if not impl.available():
+            return None
+    except ImportError:
+        return None
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda c: impl.pressure_solve("icpcg", c), cases))
+    wall = time.perf_counter() - t0
+    return {"value": len(cases) / wall, "unit": "solves/s", "cores": threads, "kind": "reference",
+            "sample": f"all {len(cases)} systems of the batch, {threads} threads, wall {wall:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +166,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-system batch leg")
+    ap.add_argument("--batched-repeats", type=int, default=3)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -175,6 +234,13 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
 
+    batch_cases = None
+    if not args.no_batched:
+        log("[bench] 256-system batch")
+        line["batched"], batch_cases = batched_leg(eng, torch, dev, args.batched_repeats)
+        log(f"[bench] batch: {line['batched']['solves_per_s']:.1f} solves/s, kernel {line['batched']['kernel_ms']:.1f} ms, "
+            f"{line['batched']['roofline']['achieved']:.0f} GB/s algorithmic")
+
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline (~{args.cpu_seconds:.0f} s on one core)")
         base, ref_result = cpu_baseline(case, args.cpu_seconds)
@@ -184,6 +250,13 @@ def main():
             line["parity"] = {"iterations_gpu": iters, "iterations_cpu": it_ref,
                               "max_abs_diff_over_max_abs": float(np.abs(p_gpu - p_ref).max() / np.abs(p_ref).max())}
             line["speedup_vs_cpu"] = line["value"] / base["value"]
+        if batch_cases is not None:
+            threads = min(16, len(os.sched_getaffinity(0)))
+            log(f"[bench] CPU baseline of the batch ({threads} threads)")
+            bb = batched_cpu_baseline(batch_cases, threads)
+            if bb is not None:
+                line["batched"]["cpu_baseline"] = bb
+                line["batched"]["speedup_vs_cpu"] = line["batched"]["solves_per_s"] / bb["value"]
     print(json.dumps(line), flush=True)
     eng.close()
 
