@@ -1,0 +1,80 @@
+// C++ host mirror of the reference's pressure-backend interface over the engine's C ABI.
+//
+// The reference selects a pressure backend by the string cfg_.pressure_scheme in
+// Solver::solve_pressure_correction (src/core/coupled/pressure.cpp:1031-1089); each backend
+// (pressure.cpp:302-365, 470-491) zero-fills pressure_correction_, solves, subtracts the mean when no
+// pressure edge is Dirichlet, fills ghosts with apply_pressure_bc (operators.cpp:143-181), sets
+// last_pressure_iterations_ and returns the relative residual; failures throw std::runtime_error
+// with the backend's message. PressureBackend::solve does the same for the hip_* schemes, with
+// fields given as ghosted Field2D-like views (data[(i+g)(ny+2g)+(j+g)], field.hpp:9-36).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+
+#include "pp_solver.h"
+
+namespace ppamd {
+
+// A Field2D-shaped view: nx x ny interior cells with `ghost` layers, i-major, j contiguous.
+struct FieldView {
+  double *data = nullptr;
+  int nx = 0, ny = 0, ghost = 0;
+  int ld() const { return ny + 2 * ghost; }
+  double &operator()(int i, int j) const { return data[static_cast<long>(i + ghost) * ld() + (j + ghost)]; }
+  double *interior() const { return &(*this)(0, 0); }
+};
+
+struct BoundarySpec {
+  int type = PP_BC_UNSET;  // PP_BC_UNSET, PP_BC_DIRICHLET, PP_BC_NEUMANN
+  double value = 0.0;
+};
+
+// The configuration values the pressure backends read (config.hpp pressure fields).
+struct PressureSettings {
+  double lx = 1.0, ly = 1.0, dt = 1.0e-3;
+  bool periodic_x = false, periodic_y = false;
+  BoundarySpec left, right, bottom, top;
+  int poisson_iterations = 400;
+  double pressure_tolerance = 1.0e-8;
+  double density_ratio = 1.0;
+};
+
+// Scheme strings served by the engine, and their engine codes; 0 when not a hip_* scheme.
+int hip_scheme_code(const std::string &scheme);
+// True for the split hip_* schemes: callers must also treat them as split in their velocity
+// correction and analysis lists (operators.cpp:400-409, pressure.cpp:1078-1085).
+bool hip_scheme_is_split(const std::string &scheme);
+
+// Effective boundary of one edge: unset edges are homogeneous Neumann (operators.cpp:36-60).
+BoundarySpec effective_bc(const BoundarySpec &b);
+// pressure_has_dirichlet_boundary (operators.cpp:114-128).
+bool has_dirichlet(const PressureSettings &s);
+
+// Ghost fill of a pressure-like field (periodic copy, Dirichlet 2v - f, Neumann f + 2(g-1/2) v h).
+void apply_pressure_bc(const PressureSettings &s, const FieldView &f);
+
+class PressureBackend {
+ public:
+  explicit PressureBackend(int device = 0);
+  ~PressureBackend();
+  PressureBackend(const PressureBackend &) = delete;
+  PressureBackend &operator=(const PressureBackend &) = delete;
+
+  // One pressure correction. rho_mid: nx x ny; u_star: (nx+1) x ny; v_star: nx x (ny+1) (all with
+  // the same ghost count); pressure / pressure_previous are read by split schemes (may have null
+  // data). Writes pressure_correction (interior + ghosts) and returns the relative residual.
+  double solve(const std::string &scheme, const PressureSettings &s, const FieldView &rho_mid,
+               const FieldView &u_star, const FieldView &v_star, const FieldView &pressure,
+               const FieldView &pressure_previous, int step_index, const FieldView &pressure_correction);
+
+  int last_iterations() const { return last_iterations_; }
+  const pp_report &last_report() const { return report_; }
+
+ private:
+  pp_ctx *ctx_ = nullptr;
+  int last_iterations_ = 0;
+  pp_report report_{};
+};
+
+}  // namespace ppamd

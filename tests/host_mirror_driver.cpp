@@ -1,0 +1,115 @@
+// Test driver for the C++ host mirror (ding_nsch_core_solver_amd/host/pp_backend.*), built by
+// tests/test_host_mirror.py with g++ against libppsolve.so.
+//
+//   driver schemes                         scheme string -> code / split, one per line
+//   driver ghost  <cfg> <field.bin>        ghost-fill a (nx+2g)(ny+2g) field in place
+//   driver solve  <cfg> <scheme> <in.bin> <out.bin>
+//        in.bin: rho (nx*ny), u ((nx+1)*ny), v (nx*(ny+1)), p_n, p_nm1 (nx*ny each), doubles
+//        out.bin: pressure_correction with 3 ghost layers; prints "iterations residual"
+// <cfg> = "nx ny lx ly dt px py t0 v0 t1 v1 t2 v2 t3 v3 maxit tol dr step" as one argument.
+#include <cstdio>
+#include <cstdlib>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "pp_backend.hpp"
+
+using ppamd::FieldView;
+using ppamd::PressureSettings;
+
+namespace {
+
+struct Cfg {
+  int nx = 0, ny = 0, max_it = 0, step = 0;
+  PressureSettings s;
+};
+
+Cfg parse(const char *text) {
+  std::istringstream in(text);
+  Cfg c;
+  int px = 0, py = 0;
+  in >> c.nx >> c.ny >> c.s.lx >> c.s.ly >> c.s.dt >> px >> py;
+  ppamd::BoundarySpec *e[4] = {&c.s.left, &c.s.right, &c.s.bottom, &c.s.top};
+  for (auto *b : e) in >> b->type >> b->value;
+  in >> c.max_it >> c.s.pressure_tolerance >> c.s.density_ratio >> c.step;
+  c.s.periodic_x = px != 0;
+  c.s.periodic_y = py != 0;
+  c.s.poisson_iterations = c.max_it;
+  if (!in) {
+    std::fprintf(stderr, "bad cfg\n");
+    std::exit(2);
+  }
+  return c;
+}
+
+std::vector<double> read_all(const char *path) {
+  std::vector<double> v;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) std::exit(3);
+  double x;
+  while (std::fread(&x, sizeof(double), 1, f) == 1) v.push_back(x);
+  std::fclose(f);
+  return v;
+}
+
+void write_all(const char *path, const std::vector<double> &v) {
+  FILE *f = std::fopen(path, "wb");
+  if (!f) std::exit(3);
+  std::fwrite(v.data(), sizeof(double), v.size(), f);
+  std::fclose(f);
+}
+
+// A ghosted field of rows x cols interior cells filled from a contiguous interior array.
+struct Owned {
+  std::vector<double> buf;
+  FieldView view;
+  Owned(int rows, int cols, int g, const double *interior) : buf(static_cast<size_t>(rows + 2 * g) * (cols + 2 * g), 0.0) {
+    view = FieldView{buf.data(), rows, cols, g};
+    if (interior)
+      for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < cols; ++j) view(i, j) = interior[static_cast<size_t>(i) * cols + j];
+  }
+};
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 2) return 2;
+  const std::string mode = argv[1];
+  if (mode == "schemes") {
+    const char *names[] = {"hip_icpcg", "hip_ildlt_pcg", "hip_split_icpcg", "hip_liu_split_icpcg",
+                           "hip_paper_split_icpcg", "hip_split_ildlt_pcg", "hip_liu_split_ildlt_pcg",
+                           "hip_paper_split_ildlt_pcg", "icpcg", "petsc_pcg"};
+    for (const char *n : names) std::printf("%s %d %d\n", n, ppamd::hip_scheme_code(n), ppamd::hip_scheme_is_split(n) ? 1 : 0);
+    return 0;
+  }
+  if (mode == "ghost" && argc == 4) {
+    const Cfg c = parse(argv[2]);
+    std::vector<double> f = read_all(argv[3]);
+    ppamd::apply_pressure_bc(c.s, FieldView{f.data(), c.nx, c.ny, 3});
+    write_all(argv[3], f);
+    return 0;
+  }
+  if (mode == "solve" && argc == 6) {
+    const Cfg c = parse(argv[2]);
+    const int nx = c.nx, ny = c.ny, g = 3;
+    const std::vector<double> in = read_all(argv[4]);
+    const size_t n = static_cast<size_t>(nx) * ny;
+    const double *rho = in.data(), *u = rho + n, *v = u + static_cast<size_t>(nx + 1) * ny;
+    const double *pn = v + static_cast<size_t>(nx) * (ny + 1), *pm = pn + n;
+    Owned R(nx, ny, g, rho), U(nx + 1, ny, g, u), V(nx, ny + 1, g, v), P(nx, ny, g, pn), PM(nx, ny, g, pm);
+    Owned out(nx, ny, g, nullptr);
+    try {
+      ppamd::PressureBackend backend(0);
+      const double res = backend.solve(argv[3], c.s, R.view, U.view, V.view, P.view, PM.view, c.step, out.view);
+      write_all(argv[5], out.buf);
+      std::printf("%d %.17g\n", backend.last_iterations(), res);
+    } catch (const std::exception &e) {
+      std::printf("error %s\n", e.what());
+      return 1;
+    }
+    return 0;
+  }
+  return 2;
+}

@@ -1,0 +1,109 @@
+"""The C++ host mirror of the reference backends (ding_nsch_core_solver_amd/host/pp_backend.*):
+scheme mapping and ghost filling on the CPU (ghosts bit-for-bit against the compiled reference's
+apply_pressure_bc, or the oracle where the reference is absent), a full solve through
+PressureBackend on the GPU (bit-for-bit against the Python binding of the same engine)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from ding_nsch_core_solver_amd import engine as E
+from ding_nsch_core_solver_amd import workloads as W
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "ding_nsch_core_solver_amd")
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    if not E.library_available():
+        pytest.skip("engine not built")
+    out = str(tmp_path_factory.mktemp("mirror") / "host_mirror_driver")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"), "-I", os.path.join(PKG, "host"),
+           os.path.join(REPO, "tests", "host_mirror_driver.cpp"), os.path.join(PKG, "host", "pp_backend.cpp"),
+           "-L", PKG, "-lppsolve", f"-Wl,-rpath,{PKG}", "-o", out]
+    subprocess.run(cmd, check=True)
+    return out
+
+
+def _cfg(case, max_it=None):
+    vals = [case.nx, case.ny, case.lx, case.ly, case.dt, int(case.periodic_x), int(case.periodic_y)]
+    for t, v in zip(case.bc_type, case.bc_value):
+        vals += [int(t), float(v)]
+    vals += [max_it or case.max_iterations, case.tolerance, case.density_ratio, case.step_index]
+    return " ".join(repr(v) if isinstance(v, float) else str(v) for v in vals)
+
+
+BC = {
+    "unset": dict(),
+    "dirichlet_left": dict(bc_type=(1, 0, 0, 0), bc_value=(3.0, 0.0, 0.0, 0.0)),
+    "mixed": dict(bc_type=(2, 1, 2, 1), bc_value=(0.5, -1.0, 2.0, 0.25)),
+    "periodic_x": dict(periodic_x=True),
+    "periodic_y_dirichlet": dict(periodic_y=True, bc_type=(0, 1, 0, 0), bc_value=(0.0, 1.5, 0.0, 0.0)),
+    "periodic_xy": dict(periodic_x=True, periodic_y=True),
+}
+
+
+def test_scheme_mapping(driver):
+    out = subprocess.run([driver, "schemes"], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {line.split()[0]: (int(line.split()[1]), int(line.split()[2])) for line in out if line}
+    for name, code in E.SCHEME_CODES.items():
+        assert got[name][0] == code
+        assert got[name][1] == (1 if code in (E.PP_SPLIT_ICPCG, E.PP_SPLIT_ILDLT_PCG) else 0)
+    assert got["icpcg"] == (0, 0) and got["petsc_pcg"] == (0, 0)
+
+
+@pytest.mark.parametrize("name", sorted(BC))
+def test_ghost_fill_matches_reference(driver, tmp_path, name):
+    case = W.droplet_case(9, 7, seed=1, lx=1.3, ly=0.7, **BC[name])
+    field = np.random.default_rng(4).standard_normal((case.nx + 6) * (case.ny + 6))
+    path = tmp_path / "f.bin"
+    field.tofile(path)
+    subprocess.run([driver, "ghost", _cfg(case), str(path)], check=True)
+    mine = np.fromfile(path)
+    try:
+        from oracle import ref
+        if not ref.available():
+            raise ImportError
+        expect = field.copy()
+        ref.lib().ref_apply_pressure_bc(ctypes.byref(ref.problem(case)), ref._dp(expect))
+    except ImportError:
+        from oracle import cpu
+        expect = cpu.apply_pressure_bc(case, field.copy())
+    assert np.array_equal(mine, expect)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme,code,kw", [
+    ("hip_icpcg", E.PP_ICPCG, dict()),
+    ("hip_ildlt_pcg", E.PP_ILDLT_PCG, dict(bc_type=(0, 0, 0, 1), bc_value=(0, 0, 0, 2.0))),
+    ("hip_liu_split_icpcg", E.PP_SPLIT_ICPCG, dict(periodic_x=True)),
+    ("hip_paper_split_ildlt_pcg", E.PP_SPLIT_ILDLT_PCG, dict(periodic_y=True)),
+])
+def test_backend_solve_through_cpp_mirror(driver, engine, oracle_lib, tmp_path, scheme, code, kw):
+    case = W.droplet_case(70, 44, seed=3, density_ratio=200.0, **kw)
+    if code in (E.PP_SPLIT_ICPCG, E.PP_SPLIT_ILDLT_PCG):
+        case.step_index = 2
+        case.p_n = np.random.default_rng(5).standard_normal(case.n) * 10.0
+        case.p_nm1 = np.random.default_rng(6).standard_normal(case.n) * 10.0
+    pn = case.p_n if case.p_n is not None else np.zeros(case.n)
+    pm = case.p_nm1 if case.p_nm1 is not None else np.zeros(case.n)
+    inp = tmp_path / "in.bin"
+    outp = tmp_path / "out.bin"
+    np.concatenate([case.rho.reshape(-1), case.u.reshape(-1), case.v.reshape(-1), pn.reshape(-1),
+                    pm.reshape(-1)]).tofile(inp)
+    res = subprocess.run([driver, "solve", _cfg(case), scheme, str(inp), str(outp)], check=True, capture_output=True,
+                         text=True)
+    it, rel = res.stdout.split()
+    p_ref, rep = engine.solve(case, code)
+    assert int(it) == rep["iterations"] and float(rel) == rep["relative_residual"]
+    full = np.fromfile(outp)
+    g = 3
+    interior = full.reshape(case.nx + 2 * g, case.ny + 2 * g)[g:-g, g:-g].reshape(-1)
+    assert np.array_equal(interior, p_ref)
+    expect = np.zeros_like(full)
+    expect.reshape(case.nx + 2 * g, case.ny + 2 * g)[g:-g, g:-g] = p_ref.reshape(case.nx, case.ny)
+    oracle_lib.apply_pressure_bc(case, expect)
+    assert np.array_equal(full, expect)
