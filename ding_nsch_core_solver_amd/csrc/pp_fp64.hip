@@ -233,8 +233,32 @@ __device__ __forceinline__ void factor_sweep(const SysArgs &a, SweepShared &sh, 
   if (failed) *lds(&sh.status) = kFactorFail;
 }
 
+// ---- triangular sweeps ---------------------------------------------------------------------------
+//
+// The sweeps run in the element type T of the solve. double: the FP64 path, dividing by the floored
+// pivot with IEEE division exactly as the reference. float: the inner solves of mixed precision,
+// multiplying by the reciprocal of the floored pivot (computed in FP64, rounded once).
+
+template <typename T>
+struct SweepArrays {
+  const T *in;        // forward: r (read as r - m_r); backward: y
+  const T *Lw, *Ls;   // lower entries west / south
+  const T *dg;        // double: pivot; float: reciprocal of the floored pivot
+  T *out;             // forward: y; backward: z
+  const double *LY;   // periodic-y wrap entries (FP64 factor)
+};
+
+template <typename T>
+__device__ __forceinline__ T pivot_apply(T s, T d, T floor) {
+  if constexpr (sizeof(T) == 8)
+    return s / floor_max(d, floor);
+  else
+    return s * d;
+}
+
+template <typename T>
 struct ForwardOps {
-  double r, lw, ls, dg;
+  T r, lw, ls, dg;
 };
 
 // Sweep loop shape shared by the solves: whole chunks of kPrefetch levels (levels past the wave's
@@ -247,8 +271,9 @@ __device__ __forceinline__ int chunk_end(int d_beg, int d_end) {
 // y = L^{-1} (r - m_r) (ICC: divided by the pivot; ILDLT: unit lower, stored divided by D).
 // Lower entries in column order: periodic-x wrap (0, j), west (i-1, j), periodic-y wrap (i, 0),
 // south (i, j-1).
-template <bool kIldlt>
-__device__ __forceinline__ void forward_solve(const SysArgs &a, SweepShared &sh, double m_r, double floor) {
+template <typename T, bool kIldlt>
+__device__ __forceinline__ void forward_solve(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, T m_r,
+                                              T floor) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
   const int S = (nx + 63) >> 6;
@@ -258,73 +283,73 @@ __device__ __forceinline__ void forward_solve(const SysArgs &a, SweepShared &sh,
   const int d_stop = chunk_end(d_beg, d_end);
   const bool px = a.periodic_x != 0, py = a.periodic_y != 0;
   const bool row_ok = i < nx, has_w = i > 0, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
-  const global_ptr<const double> r = as_global<const double>(a.r);
-  const global_ptr<const double> Lw = as_global<const double>(a.Lw);
-  const global_ptr<const double> Ls = as_global<const double>(a.Ls);
-  const global_ptr<const double> dg = as_global<const double>(a.dg);
-  const global_ptr<double> y = as_global(a.y);
+  const global_ptr<const T> r = as_global<const T>(io.in);
+  const global_ptr<const T> Lw = as_global<const T>(io.Lw);
+  const global_ptr<const T> Ls = as_global<const T>(io.Ls);
+  const global_ptr<const T> dg = as_global<const T>(io.dg);
+  const global_ptr<T> y = as_global(io.out);
   const int sink = a.n + kPad - 64 + lane;  // store target of inactive lanes
-  const double ly = (py && row_ok) ? as_global(a.LY)[i] : 0.0;
-  double prev = 0.0, first = 0.0;
-  ForwardOps buf[kPrefetch];
+  const T ly = (py && row_ok) ? static_cast<T>(as_global<const double>(io.LY)[i]) : T(0);
+  T prev = T(0), first = T(0);
+  ForwardOps<T> buf[kPrefetch];
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k) {
     const int q = level_base(min(d_beg + k, d_last), nx, ny) + i;
     buf[k] = {r[q], Lw[q], Ls[q], dg[q]};
   }
   for (int d0 = d_beg; d0 <= d_stop; d0 += kPrefetch) {
-    const double hv = wave > 0 ? receive_batch<double, kPrefetch>(wave, lane, d0 - wave * 64, ny, sh) : 0.0;
+    const T hv = wave > 0 ? receive_batch<T, kPrefetch>(wave, lane, d0 - wave * 64, ny, sh) : T(0);
     if (wave + 1 < S) reserve_batch(wave + 1, min(d0 + kPrefetch - 1 - (wave * 64 + 63), ny - 1), sh);
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) {
       const int d = d0 + k;
-      const ForwardOps o = buf[k];
+      const ForwardOps<T> o = buf[k];
       {
         const int qp = level_base(min(d + kPrefetch, d_last), nx, ny) + i;
         buf[k] = {r[qp], Lw[qp], Ls[qp], dg[qp]};
       }
       const int j = d - i;
       const bool act = row_ok && j >= 0 && j < ny;
-      double west = wave_shr1(prev);
+      T west = wave_shr1(prev);
       if (wave > 0) {
-        const double h = lane_value(hv, k);
+        const T h = lane_value(hv, k);
         west = lane == 0 ? h : west;
       }
-      double s = o.r - m_r;
+      T s = o.r - m_r;
       if (px) {
         const int jc = min(max(j, 0), ny - 1);
-        const double wx = *lds(&sh.wrap[jc]);
-        s = (wrap_row && act) ? s - sh.lx[jc] * wx : s;
+        const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
+        s = (wrap_row && act) ? s - static_cast<T>(sh.lx[jc]) * wx : s;
       }
       s = has_w ? s - o.lw * west : s;
       if (py) s = j == ny - 1 ? s - ly * first : s;
       s = j > 0 ? s - o.ls * prev : s;
-      const double den = floor_max(o.dg, floor);
-      double cur, out;
+      T cur, out;
       if (kIldlt) {
         cur = s;
-        out = s / den;
+        out = pivot_apply(s, o.dg, floor);
       } else {
-        cur = s / den;
+        cur = pivot_apply(s, o.dg, floor);
         out = cur;
       }
       y[act ? level_base(d, nx, ny) + i : sink] = out;
       first = j == 0 ? cur : first;
-      if (px && wrap_src && act) *lds(&sh.wrap[j]) = cur;
+      if (px && wrap_src && act) *lds(&sh.wrap[j]) = static_cast<double>(cur);
       prev = cur;
       if (wave + 1 < S) publish(wave + 1, lane, 63, d - (wave * 64 + 63), ny, cur, sh);
     }
   }
 }
 
+template <typename T>
 struct BackwardOps {
-  double y, dg, ls_n, lw_e;
+  T y, dg, ls_n, lw_e;
 };
 
 // z = L^{-T} y, levels descending. Children of (i, j) in ascending row order: (i, j+1), the
 // periodic-y wrap (i, ny-1) when j = 0, (i+1, j), the periodic-x wrap (nx-1, j) when i = 0.
-template <bool kIldlt>
-__device__ __forceinline__ void backward_solve(const SysArgs &a, SweepShared &sh, double floor) {
+template <typename T, bool kIldlt>
+__device__ __forceinline__ void backward_solve(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, T floor) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
   const int S = (nx + 63) >> 6;
@@ -333,54 +358,54 @@ __device__ __forceinline__ void backward_solve(const SysArgs &a, SweepShared &sh
   const int d_beg = wave * 64, d_end = min(wave * 64 + 63, nx - 1) + ny - 1, d_last = nx + ny - 2;
   const bool px = a.periodic_x != 0, py = a.periodic_y != 0;
   const bool row_ok = i < nx, has_e = i + 1 < nx, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
-  const global_ptr<const double> yv = as_global<const double>(a.y);
-  const global_ptr<const double> Lw = as_global<const double>(a.Lw);
-  const global_ptr<const double> Ls = as_global<const double>(a.Ls);
-  const global_ptr<const double> dg = as_global<const double>(a.dg);
-  const global_ptr<double> z = as_global(a.z);
+  const global_ptr<const T> yv = as_global<const T>(io.in);
+  const global_ptr<const T> Lw = as_global<const T>(io.Lw);
+  const global_ptr<const T> Ls = as_global<const T>(io.Ls);
+  const global_ptr<const T> dg = as_global<const T>(io.dg);
+  const global_ptr<T> z = as_global(io.out);
   const int sink = a.n + kPad - 64 + lane;
-  const double ly = (py && row_ok) ? as_global(a.LY)[i] : 0.0;
+  const T ly = (py && row_ok) ? static_cast<T>(as_global<const double>(io.LY)[i]) : T(0);
   auto fetch = [&](int d) {
     const int dc = min(max(d, 0), d_last);
     const int q = level_base(dc, nx, ny) + i;
     const int qn = level_base(min(dc + 1, d_last), nx, ny) + i;  // (i, j+1); (i+1, j) follows it
-    return BackwardOps{yv[q], dg[q], Ls[qn], Lw[qn + 1]};
+    return BackwardOps<T>{yv[q], dg[q], Ls[qn], Lw[qn + 1]};
   };
   // levels d_end down to d_low (whole chunks; levels below d_beg are inactive for every lane)
   const int d_low = d_end - ((d_end - d_beg + kPrefetch) / kPrefetch) * kPrefetch + 1;
-  double prev = 0.0, first = 0.0;
-  BackwardOps buf[kPrefetch];
+  T prev = T(0), first = T(0);
+  BackwardOps<T> buf[kPrefetch];
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k) buf[k] = fetch(d_end - k);
   for (int d0 = d_end; d0 >= d_low; d0 -= kPrefetch) {
     const int j63_first = d0 - (wave * 64 + 63);  // column of lane 63 at level d0
-    const double hv = wave + 1 < S ? receive_batch<double, kPrefetch>(wave, lane, ny - 1 - j63_first, ny, sh) : 0.0;
+    const T hv = wave + 1 < S ? receive_batch<T, kPrefetch>(wave, lane, ny - 1 - j63_first, ny, sh) : T(0);
     if (wave > 0) reserve_batch(wave - 1, min(ny - 1 - (d0 - kPrefetch + 1 - wave * 64), ny - 1), sh);
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) {
       const int d = d0 - k;
-      const BackwardOps o = buf[k];
+      const BackwardOps<T> o = buf[k];
       buf[k] = fetch(d - kPrefetch);
       const int j = d - i;
       const bool act = row_ok && j >= 0 && j < ny;
-      double east = wave_shl1(prev);
+      T east = wave_shl1(prev);
       if (wave + 1 < S) {
-        const double h = lane_value(hv, k);
+        const T h = lane_value(hv, k);
         east = lane == 63 ? h : east;
       }
-      double s = o.y;
+      T s = o.y;
       s = j + 1 < ny ? s - o.ls_n * prev : s;
       if (py) s = j == 0 ? s - ly * first : s;
       s = has_e ? s - o.lw_e * east : s;
       if (px) {
         const int jc = min(max(j, 0), ny - 1);
-        const double wx = *lds(&sh.wrap[jc]);
-        s = (wrap_src && act) ? s - sh.lx[jc] * wx : s;
+        const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
+        s = (wrap_src && act) ? s - static_cast<T>(sh.lx[jc]) * wx : s;
       }
-      const double cur = kIldlt ? s : s / floor_max(o.dg, floor);
+      const T cur = kIldlt ? s : pivot_apply(s, o.dg, floor);
       z[act ? level_base(d, nx, ny) + i : sink] = cur;
       first = j == ny - 1 ? cur : first;
-      if (px && wrap_row && act) *lds(&sh.wrap[j]) = cur;
+      if (px && wrap_row && act) *lds(&sh.wrap[j]) = static_cast<double>(cur);
       prev = cur;
       if (wave > 0) publish(wave - 1, lane, 0, ny - 1 - (d - wave * 64), ny, cur, sh);
     }
@@ -395,9 +420,16 @@ __device__ __forceinline__ void backward_solve(const SysArgs &a, SweepShared &sh
 // once filled (flag written after the data, LDS operations of a wave complete in order), used[w] =
 // steps sweep wave w has finished, which bounds how far loaders may run ahead.
 
-// Ring depth R (levels per sweep wave, a power of two with S * R <= 64) and loader unit U: loaders
-// may run R - U levels ahead of the sweep.
-__device__ __forceinline__ int stage_ring(const SysArgs &a, int S) { return a.ring > 0 ? a.ring : (S <= 4 ? 16 : 8); }
+// Ring depth R (levels per sweep wave, a power of two, S * R slots of 4 x 64 elements within the
+// 128 KB stage) and loader unit U: loaders may run R - U levels ahead of the sweep. FP64 rings are
+// tunable (SysArgs::ring); FP32 slots are half the size, so their default is twice as deep.
+template <typename T>
+__device__ __forceinline__ int stage_ring(const SysArgs &a, int S) {
+  if constexpr (sizeof(T) == 8)
+    return a.ring > 0 ? a.ring : (S <= 4 ? 16 : 8);
+  else
+    return S <= 2 ? 64 : (S <= 4 ? 32 : 16);
+}
 __device__ __forceinline__ int stage_unit(int R) { return R / 2 < kStageUnit ? R / 2 : kStageUnit; }
 
 // Steps every sweep wave walks: its 64 rows span 63 + ny levels, rounded up to whole units.
@@ -408,9 +440,10 @@ __device__ __forceinline__ int step_level(int w, int t, int T) {
   return kBackward ? w * 64 + T - 1 - t : w * 64 + t;
 }
 
-__device__ __forceinline__ volatile __attribute__((address_space(3))) double *stage_slot(SweepShared &sh, int w, int R,
-                                                                                        int slot, int lane) {
-  return lds(reinterpret_cast<volatile double *>(&sh.stage[((w * R + slot) * 4) * 64 + lane]));
+template <typename T>
+__device__ __forceinline__ volatile __attribute__((address_space(3))) T *stage_slot(SweepShared &sh, int w, int R,
+                                                                                   int slot, int lane) {
+  return lds(reinterpret_cast<volatile T *>(&sh.stage[0]) + ((w * R + slot) * 4) * 64 + lane);
 }
 
 // Loaders take units (sweep wave w, first step t0) in order of when they are needed: a forward
@@ -418,30 +451,30 @@ __device__ __forceinline__ volatile __attribute__((address_space(3))) double *st
 // key is that offset plus t0. Everything a unit waits for (ring space of its wave, which in turn needs
 // hand-offs from the upstream wave) has a smaller key, so loaders processing their share of units in
 // key order cannot wait on each other in a cycle. Unit k of that order goes to loader k mod L.
-template <bool kBackward>
-__device__ __forceinline__ void stage_loader(const SysArgs &a, SweepShared &sh, int S) {
+template <typename T, bool kBackward>
+__device__ __forceinline__ void stage_loader(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, int S) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny, d_last = nx + ny - 2;
-  const int R = stage_ring(a, S), T = stage_steps(ny);
+  const int R = stage_ring<T>(a, S), Tn = stage_steps(ny);
   const int U = stage_unit(R);
   const int L = kWaves - S, me = wave - S;
-  const global_ptr<const double> f0 = as_global<const double>(kBackward ? a.y : a.r);
-  const global_ptr<const double> dg = as_global<const double>(a.dg);
-  const global_ptr<const double> Lw = as_global<const double>(a.Lw);
-  const global_ptr<const double> Ls = as_global<const double>(a.Ls);
+  const global_ptr<const T> f0 = as_global<const T>(io.in);
+  const global_ptr<const T> dg = as_global<const T>(io.dg);
+  const global_ptr<const T> Lw = as_global<const T>(io.Lw);
+  const global_ptr<const T> Ls = as_global<const T>(io.Ls);
   int k_unit = 0;
-  for (int tau = 0; tau < 64 * (S - 1) + T; tau += U) {
+  for (int tau = 0; tau < 64 * (S - 1) + Tn; tau += U) {
     for (int w = 0; w < S; ++w) {
       const int t0 = tau - 64 * (kBackward ? (S - 1 - w) : w);
-      if (t0 < 0 || t0 >= T) continue;
+      if (t0 < 0 || t0 >= Tn) continue;
       if ((k_unit++) % L != me) continue;
       wait_at_least(&sh.used[w], t0 + U - R, sh);
       const int i = w * 64 + lane;
-      double v[kStageUnit][4];
+      T v[kStageUnit][4];
 #pragma unroll
       for (int k = 0; k < kStageUnit; ++k) {
         if (k < U) {
-          const int d = min(max(step_level<kBackward>(w, t0 + k, T), 0), d_last);
+          const int d = min(max(step_level<kBackward>(w, t0 + k, Tn), 0), d_last);
           const int q = level_base(d, nx, ny) + i;
           if (kBackward) {
             const int qn = level_base(min(d + 1, d_last), nx, ny) + i;  // (i, j+1); (i+1, j) follows it
@@ -460,7 +493,7 @@ __device__ __forceinline__ void stage_loader(const SysArgs &a, SweepShared &sh, 
 #pragma unroll
       for (int k = 0; k < kStageUnit; ++k) {
         if (k < U) {
-          volatile __attribute__((address_space(3))) double *slot = stage_slot(sh, w, R, (t0 + k) & (R - 1), lane);
+          volatile __attribute__((address_space(3))) T *slot = stage_slot<T>(sh, w, R, (t0 + k) & (R - 1), lane);
 #pragma unroll
           for (int f = 0; f < 4; ++f) slot[f * 64] = v[k][f];
         }
@@ -483,70 +516,72 @@ __device__ __forceinline__ void await_chunk(SweepShared &sh, int w, int R, int t
   }
 }
 
+template <typename T>
 struct ChunkOps {
-  double f[kPrefetch][4];
+  T f[kPrefetch][4];
 };
 
-__device__ __forceinline__ void read_chunk(SweepShared &sh, int w, int R, int t0, int lane, ChunkOps &c) {
+template <typename T>
+__device__ __forceinline__ void read_chunk(SweepShared &sh, int w, int R, int t0, int lane, ChunkOps<T> &c) {
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k) {
-    volatile __attribute__((address_space(3))) double *p = stage_slot(sh, w, R, (t0 + k) & (R - 1), lane);
+    volatile __attribute__((address_space(3))) T *p = stage_slot<T>(sh, w, R, (t0 + k) & (R - 1), lane);
 #pragma unroll
     for (int f = 0; f < 4; ++f) c.f[k][f] = p[f * 64];
   }
 }
 
-template <bool kIldlt, bool kPeriodic>
-__device__ __forceinline__ void staged_forward(const SysArgs &a, SweepShared &sh, double m_r, double floor, int S) {
+template <typename T, bool kIldlt, bool kPeriodic>
+__device__ __forceinline__ void staged_forward(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, T m_r,
+                                               T floor, int S) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
-  const int R = stage_ring(a, S), T = stage_steps(ny);
+  const int R = stage_ring<T>(a, S), Tn = stage_steps(ny);
   const int i = wave * 64 + lane;
   const bool px = kPeriodic && a.periodic_x != 0, py = kPeriodic && a.periodic_y != 0;
   const bool row_ok = i < nx, has_w = i > 0, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
-  const global_ptr<double> y = as_global(a.y);
+  const global_ptr<T> y = as_global(io.out);
   const int sink = a.n + kPad - 64 + lane;
-  const double ly = (py && row_ok) ? as_global(a.LY)[i] : 0.0;
-  double prev = 0.0, first = 0.0;
-  for (int t0 = 0; t0 < T; t0 += kPrefetch) {
+  const T ly = (py && row_ok) ? static_cast<T>(as_global<const double>(io.LY)[i]) : T(0);
+  T prev = T(0), first = T(0);
+  for (int t0 = 0; t0 < Tn; t0 += kPrefetch) {
     const int d0 = wave * 64 + t0;
     await_chunk(sh, wave, R, t0);
-    ChunkOps c;
-    read_chunk(sh, wave, R, t0, lane, c);
-    const double hv = wave > 0 ? receive_batch<double, kPrefetch>(wave, lane, t0, ny, sh) : 0.0;
+    ChunkOps<T> c;
+    read_chunk<T>(sh, wave, R, t0, lane, c);
+    const T hv = wave > 0 ? receive_batch<T, kPrefetch>(wave, lane, t0, ny, sh) : T(0);
     if (wave + 1 < S) reserve_batch(wave + 1, min(d0 + kPrefetch - 1 - (wave * 64 + 63), ny - 1), sh);
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) {
       const int d = d0 + k;
       const int j = d - i;
       const bool act = row_ok && j >= 0 && j < ny;
-      double west = wave_shr1(prev);
+      T west = wave_shr1(prev);
       if (wave > 0) {
-        const double h = lane_value(hv, k);
+        const T h = lane_value(hv, k);
         west = lane == 0 ? h : west;
       }
-      double s = c.f[k][0] - m_r;
+      T s = c.f[k][0] - m_r;
       if (kPeriodic && px) {
         const int jc = min(max(j, 0), ny - 1);
-        const double wx = *lds(&sh.wrap[jc]);
-        s = (wrap_row && act) ? s - sh.lx[jc] * wx : s;
+        const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
+        s = (wrap_row && act) ? s - static_cast<T>(sh.lx[jc]) * wx : s;
       }
       s = has_w ? s - c.f[k][1] * west : s;
       if (kPeriodic && py) s = j == ny - 1 ? s - ly * first : s;
       s = j > 0 ? s - c.f[k][2] * prev : s;
-      const double den = floor_max(c.f[k][3], floor);
-      double cur, out;
+      T cur, out;
       if (kIldlt) {
         cur = s;
-        out = s / den;
+        out = pivot_apply(s, c.f[k][3], floor);
       } else {
-        cur = s / den;
+        cur = pivot_apply(s, c.f[k][3], floor);
         out = cur;
       }
       y[act ? level_base(d, nx, ny) + i : sink] = out;
       if (kPeriodic) {
         first = j == 0 ? cur : first;
-        if (px && wrap_src && act) *lds(&sh.wrap[j]) = cur;
+        if (px && wrap_src && act) *lds(&sh.wrap[j]) = static_cast<double>(cur);
       }
       prev = cur;
       if (wave + 1 < S) publish_nb(wave + 1, lane, 63, d - (wave * 64 + 63), ny, cur, sh);
@@ -555,50 +590,51 @@ __device__ __forceinline__ void staged_forward(const SysArgs &a, SweepShared &sh
   }
 }
 
-template <bool kIldlt, bool kPeriodic>
-__device__ __forceinline__ void staged_backward(const SysArgs &a, SweepShared &sh, double floor, int S) {
+template <typename T, bool kIldlt, bool kPeriodic>
+__device__ __forceinline__ void staged_backward(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, T floor,
+                                                int S) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny;
-  const int R = stage_ring(a, S), T = stage_steps(ny);
+  const int R = stage_ring<T>(a, S), Tn = stage_steps(ny);
   const int i = wave * 64 + lane;
   const bool px = kPeriodic && a.periodic_x != 0, py = kPeriodic && a.periodic_y != 0;
   const bool row_ok = i < nx, has_e = i + 1 < nx, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
-  const global_ptr<double> z = as_global(a.z);
+  const global_ptr<T> z = as_global(io.out);
   const int sink = a.n + kPad - 64 + lane;
-  const double ly = (py && row_ok) ? as_global(a.LY)[i] : 0.0;
-  double prev = 0.0, first = 0.0;
-  for (int t0 = 0; t0 < T; t0 += kPrefetch) {
-    const int d0 = wave * 64 + T - 1 - t0;        // levels descend
+  const T ly = (py && row_ok) ? static_cast<T>(as_global<const double>(io.LY)[i]) : T(0);
+  T prev = T(0), first = T(0);
+  for (int t0 = 0; t0 < Tn; t0 += kPrefetch) {
+    const int d0 = wave * 64 + Tn - 1 - t0;        // levels descend
     const int j63_first = d0 - (wave * 64 + 63);  // column of lane 63 at level d0
     await_chunk(sh, wave, R, t0);
-    ChunkOps c;
-    read_chunk(sh, wave, R, t0, lane, c);
-    const double hv = wave + 1 < S ? receive_batch<double, kPrefetch>(wave, lane, ny - 1 - j63_first, ny, sh) : 0.0;
+    ChunkOps<T> c;
+    read_chunk<T>(sh, wave, R, t0, lane, c);
+    const T hv = wave + 1 < S ? receive_batch<T, kPrefetch>(wave, lane, ny - 1 - j63_first, ny, sh) : T(0);
     if (wave > 0) reserve_batch(wave - 1, min(ny - 1 - (d0 - kPrefetch + 1 - wave * 64), ny - 1), sh);
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) {
       const int d = d0 - k;
       const int j = d - i;
       const bool act = row_ok && j >= 0 && j < ny;
-      double east = wave_shl1(prev);
+      T east = wave_shl1(prev);
       if (wave + 1 < S) {
-        const double h = lane_value(hv, k);
+        const T h = lane_value(hv, k);
         east = lane == 63 ? h : east;
       }
-      double s = c.f[k][0];
+      T s = c.f[k][0];
       s = j + 1 < ny ? s - c.f[k][2] * prev : s;
       if (kPeriodic && py) s = j == 0 ? s - ly * first : s;
       s = has_e ? s - c.f[k][3] * east : s;
       if (kPeriodic && px) {
         const int jc = min(max(j, 0), ny - 1);
-        const double wx = *lds(&sh.wrap[jc]);
-        s = (wrap_src && act) ? s - sh.lx[jc] * wx : s;
+        const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
+        s = (wrap_src && act) ? s - static_cast<T>(sh.lx[jc]) * wx : s;
       }
-      const double cur = kIldlt ? s : s / floor_max(c.f[k][1], floor);
+      const T cur = kIldlt ? s : pivot_apply(s, c.f[k][1], floor);
       z[act ? level_base(d, nx, ny) + i : sink] = cur;
       if (kPeriodic) {
         first = j == ny - 1 ? cur : first;
-        if (px && wrap_row && act) *lds(&sh.wrap[j]) = cur;
+        if (px && wrap_row && act) *lds(&sh.wrap[j]) = static_cast<double>(cur);
       }
       prev = cur;
       if (wave > 0) publish_nb(wave - 1, lane, 0, ny - 1 - (d - wave * 64), ny, cur, sh);
@@ -613,20 +649,21 @@ __device__ __forceinline__ void reset_stage(SweepShared &sh) {
 }
 
 // Operator fields the SpMV needs, read once per solve.
+template <typename T>
 struct Stencil {
   int nx, ny;
   bool px, py;
-  global_ptr<const double> aP, aW, aS;
+  global_ptr<const T> aP, aW, aS;
 };
 
 // (A p)_q in CSR column order, p read through the lazy projection p - m_p.
-__device__ __forceinline__ double spmv_row(const Stencil &A, const global_ptr<const double> p, double m_p, int q,
-                                           int i, int j) {
+template <typename T>
+__device__ __forceinline__ T spmv_row(const Stencil<T> &A, const global_ptr<const T> p, T m_p, int q, int i, int j) {
   const int nx = A.nx, ny = A.ny;
   const int d = i + j;
   const int offm = d > 0 ? level_offset(d - 1, nx, ny) - level_lo(d - 1, ny) : 0;  // + i' gives level d-1
   const int offp = level_offset(d + 1, nx, ny) - level_lo(d + 1, ny);                // + i' gives level d+1
-  double s = 0.0;
+  T s = T(0);
   if (A.px && i == nx - 1) {
     const int qw = grid_pos(0, j, nx, ny);
     s += A.aW[qw] * (p[qw] - m_p);
@@ -651,9 +688,11 @@ __device__ __forceinline__ double spmv_row(const Stencil &A, const global_ptr<co
   return s;
 }
 
-template <bool kIldlt>
-__device__ __forceinline__ void precondition(const SysArgs &a, SweepShared &sh, double m_r, double floor,
-                                             unsigned long long *ticks, unsigned long long &t) {
+// z = M^{-1} (r - m_r): forward then backward sweep, staged through LDS when S <= kMaxStagedWaves.
+template <typename T, bool kIldlt>
+__device__ __forceinline__ void precondition(const SysArgs &a, const SweepArrays<T> &fwd, const SweepArrays<T> &bwd,
+                                             SweepShared &sh, T m_r, T floor, unsigned long long *ticks,
+                                             unsigned long long &t) {
   const int S = (a.nx + 63) >> 6;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool staged = S <= kMaxStagedWaves;
@@ -661,15 +700,15 @@ __device__ __forceinline__ void precondition(const SysArgs &a, SweepShared &sh, 
   reset_stage(sh);
   __syncthreads();
   if (!staged)
-    forward_solve<kIldlt>(a, sh, m_r, floor);
+    forward_solve<T, kIldlt>(a, fwd, sh, m_r, floor);
   else if (wave < S) {
     if (a.periodic_x || a.periodic_y)
-      staged_forward<kIldlt, true>(a, sh, m_r, floor, S);
+      staged_forward<T, kIldlt, true>(a, fwd, sh, m_r, floor, S);
     else
-      staged_forward<kIldlt, false>(a, sh, m_r, floor, S);
+      staged_forward<T, kIldlt, false>(a, fwd, sh, m_r, floor, S);
   }
   else
-    stage_loader<false>(a, sh, S);
+    stage_loader<T, false>(a, fwd, sh, S);
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned long long now = device_ticks();
@@ -680,15 +719,15 @@ __device__ __forceinline__ void precondition(const SysArgs &a, SweepShared &sh, 
   reset_stage(sh);
   __syncthreads();
   if (!staged)
-    backward_solve<kIldlt>(a, sh, floor);
+    backward_solve<T, kIldlt>(a, bwd, sh, floor);
   else if (wave < S) {
     if (a.periodic_x || a.periodic_y)
-      staged_backward<kIldlt, true>(a, sh, floor, S);
+      staged_backward<T, kIldlt, true>(a, bwd, sh, floor, S);
     else
-      staged_backward<kIldlt, false>(a, sh, floor, S);
+      staged_backward<T, kIldlt, false>(a, bwd, sh, floor, S);
   }
   else
-    stage_loader<true>(a, sh, S);
+    stage_loader<T, true>(a, bwd, sh, S);
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned long long now = device_ticks();
@@ -701,32 +740,44 @@ __device__ __forceinline__ void precondition(const SysArgs &a, SweepShared &sh, 
 // them are loaded before any is used so each thread keeps several loads in flight.
 constexpr int kUnroll = 8;
 
-template <bool kIldlt>
-__device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) {
-  const int tid = threadIdx.x;
-  const int n = a.n, nx = a.nx, ny = a.ny;
-  int parity = 0;
+// Per-phase device timers (thread 0 keeps them; reported in pp_report.phase_ms).
+struct PhaseClock {
   unsigned long long ticks[kPhases];
-  for (int k = 0; k < kPhases; ++k) ticks[k] = 0;
-  unsigned long long t = device_ticks();
-  auto mark = [&](int phase) {
-    if (tid == 0) {
+  unsigned long long t;
+  __device__ __forceinline__ PhaseClock() : t(device_ticks()) {
+    for (int k = 0; k < kPhases; ++k) ticks[k] = 0;
+  }
+  __device__ __forceinline__ void mark(int phase) {
+    if (threadIdx.x == 0) {
       const unsigned long long now = device_ticks();
       ticks[phase] += now - t;
       t = now;
     }
-  };
-  auto finish = [&]() {
-    if (tid == 0)
-      for (int k = 0; k < kPhases; ++k) a.rep->phase_ticks[k] = ticks[k];
-  };
+  }
+  __device__ __forceinline__ void finish(DevReport *rep) const {
+    if (threadIdx.x == 0)
+      for (int k = 0; k < kPhases; ++k) rep->phase_ticks[k] = ticks[k];
+  }
+};
+
+struct Setup {
+  double nd;      // max(1, n)
+  double b_mean;  // mean of the assembled right-hand side (0 when the system is not projected)
+  double floor;   // pivot floor of the factorization
+};
+
+// Assembly (+ missing-diagonal check), the right-hand-side mean and the factorization, shared by
+// every solve. Returns false, with the status in LDS, when the system cannot be solved.
+template <bool kIldlt>
+__device__ __forceinline__ bool setup_system(const SysArgs &a, SweepShared &sh, int &parity, Setup &out) {
+  const int tid = threadIdx.x;
+  const int n = a.n, ny = a.ny;
   if (tid == 0) {
     *lds(&sh.spin_error) = 0;
     *lds(&sh.status) = kOk;
   }
   __syncthreads();
 
-  // assembly (+ missing-diagonal check), rhs regularization
   double bsum = 0.0, dmax = 0.0;
   int nodiag = 0;
   for (int q = tid; q < n; q += kThreads) {
@@ -741,27 +792,36 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
   if (nodiag) *lds(&sh.status) = kNoDiag;
   const double total = block_sum1(bsum, sh, parity);
   const double max_diag = block_max1(dmax, sh, parity);
-  if (*lds(&sh.status) != kOk) {
-    finish();
-    return;
-  }
-  const double nd = static_cast<double>(n) < 1.0 ? 1.0 : static_cast<double>(n);
-  const double b_mean = a.projected ? total / nd : 0.0;
+  if (*lds(&sh.status) != kOk) return false;
+  out.nd = static_cast<double>(n) < 1.0 ? 1.0 : static_cast<double>(n);
+  out.b_mean = a.projected ? total / out.nd : 0.0;
 
-  // factorization
   const double base = 1.0 < max_diag ? max_diag : 1.0;
-  const double floor = kIldlt ? base * 1.0e-12 : sqrt(base * 1.0e-12);
+  out.floor = kIldlt ? base * 1.0e-12 : sqrt(base * 1.0e-12);
   reset_handoff(sh);
   __syncthreads();
-  factor_sweep<kIldlt>(a, sh, floor);
+  factor_sweep<kIldlt>(a, sh, out.floor);
   __syncthreads();
   if (a.periodic_x)
     for (int j = tid; j < ny; j += kThreads) sh.lx[j] = as_global<const double>(a.LX)[j];
   if (*lds(&sh.spin_error)) *lds(&sh.status) = kSweepFail;
-  if (*lds(&sh.status) != kOk) {
+  return *lds(&sh.status) == kOk;
+}
+
+template <bool kIldlt>
+__device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) {
+  const int tid = threadIdx.x;
+  const int n = a.n, nx = a.nx, ny = a.ny;
+  int parity = 0;
+  PhaseClock clk;
+  auto mark = [&](int phase) { clk.mark(phase); };
+  auto finish = [&]() { clk.finish(a.rep); };
+  Setup su;
+  if (!setup_system<kIldlt>(a, sh, parity, su)) {
     finish();
     return;
   }
+  const double nd = su.nd, b_mean = su.b_mean, floor = su.floor;
 
   const global_ptr<double> xv = as_global(a.x);
   const global_ptr<double> rv = as_global(a.r);
@@ -769,7 +829,9 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
   const global_ptr<double> bv = as_global(a.b);
   const global_ptr<double> apv = as_global(a.y);  // Ap shares storage with the sweep intermediate
   const global_ptr<const uint32_t> cellv = as_global<const uint32_t>(a.cell);
-  const Stencil stencil{nx, ny, a.periodic_x != 0, a.periodic_y != 0, as_global<const double>(a.aP),
+  const SweepArrays<double> fwd{a.r, a.Lw, a.Ls, a.dg, a.y, a.LY};
+  const SweepArrays<double> bwd{a.y, a.Lw, a.Ls, a.dg, a.z, a.LY};
+  const Stencil<double> stencil{nx, ny, a.periodic_x != 0, a.periodic_y != 0, as_global<const double>(a.aP),
                         as_global<const double>(a.aW), as_global<const double>(a.aS)};
   const bool projected = a.projected != 0;
 
@@ -804,7 +866,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
   mark(0);
 
   if (!(rel_res < tol || abs_res < tol)) {
-    precondition<kIldlt>(a, sh, m_r, floor, ticks, t);
+    precondition<double, kIldlt>(a, fwd, bwd, sh, m_r, floor, clk.ticks, clk.t);
     double zs = 0.0;
     for (int q = tid; q < n; q += kThreads) zs = zs + zv[q];
     zs = block_sum1(zs, sh, parity);
@@ -911,7 +973,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
       rel_res = abs_res / (bnorm < 1.0e-30 ? 1.0e-30 : bnorm);
       if (rel_res < tol || abs_res < tol) break;
 
-      precondition<kIldlt>(a, sh, m_r, floor, ticks, t);
+      precondition<double, kIldlt>(a, fwd, bwd, sh, m_r, floor, clk.ticks, clk.t);
       zs = 0.0;
       for (int base = tid; base < n; base += kThreads * kUnroll) {
         double zq[kUnroll];
@@ -1003,6 +1065,224 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
   finish();
 }
 
+// ---- mixed precision: FP32 inner PCG inside FP64 iterative refinement ---------------------------
+//
+// No counterpart in the reference. Setup (assembly, regularization, ICC/ILDLT factorization) is the
+// FP64 path's. Each refinement cycle computes the true residual r = b - A x in FP64 (projected like
+// the FP64 solve), stops when |r| / |b| or |r| is below the tolerance, and otherwise solves
+// A d = r / |r| approximately with PCG in FP32: FP32 copies of the operator and factor (reciprocal of
+// the floored pivot instead of division), FP32 vectors, dot products and means accumulated in FP64.
+// The inner solve stops at relative reduction inner_tolerance; x += |r| d in FP64.
+constexpr int kMaxOuterCycles = 64;
+
+template <bool kIldlt>
+__device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
+  const int tid = threadIdx.x;
+  const int n = a.n, nx = a.nx, ny = a.ny;
+  int parity = 0;
+  PhaseClock clk;
+  Setup su;
+  if (!setup_system<kIldlt>(a, sh, parity, su)) {
+    clk.finish(a.rep);
+    return;
+  }
+  const double nd = su.nd, floor = su.floor;
+  const bool projected = a.projected != 0;
+  const global_ptr<const uint32_t> cellv = as_global<const uint32_t>(a.cell);
+  const global_ptr<double> xv = as_global(a.x), rv = as_global(a.r), bv = as_global(a.b);
+  const global_ptr<float> fP = as_global(a.fP), fW = as_global(a.fW), fS = as_global(a.fS);
+  const global_ptr<float> fLw = as_global(a.fLw), fLs = as_global(a.fLs), fRd = as_global(a.fRd);
+  const global_ptr<float> dv = as_global(a.fx), r32 = as_global(a.fr), p32 = as_global(a.fpa);
+  const global_ptr<float> z32 = as_global(a.fz), ap32 = as_global(a.fy);
+
+  // FP32 operator and factor, regularized FP64 right-hand side, x = 0
+  double bb = 0.0;
+  for (int q = tid; q < n; q += kThreads) {
+    fP[q] = static_cast<float>(a.aP[q]);
+    fW[q] = static_cast<float>(a.aW[q]);
+    fS[q] = static_cast<float>(a.aS[q]);
+    fLw[q] = static_cast<float>(a.Lw[q]);
+    fLs[q] = static_cast<float>(a.Ls[q]);
+    fRd[q] = static_cast<float>(1.0 / floor_max(a.dg[q], floor));
+    const double bq = projected ? bv[q] - su.b_mean : bv[q];
+    bv[q] = bq;
+    xv[q] = 0.0;
+    bb = bb + bq * bq;
+  }
+  bb = block_sum1(bb, sh, parity);
+  const double bnorm = sqrt(bb < 0.0 ? 0.0 : bb);
+  const Stencil<double> A64{nx, ny, a.periodic_x != 0, a.periodic_y != 0, as_global<const double>(a.aP),
+                            as_global<const double>(a.aW), as_global<const double>(a.aS)};
+  const Stencil<float> A32{nx, ny, a.periodic_x != 0, a.periodic_y != 0, fP, fW, fS};
+  const SweepArrays<float> fwd{a.fr, a.fLw, a.fLs, a.fRd, a.fy, a.LY};
+  const SweepArrays<float> bwd{a.fy, a.fLw, a.fLs, a.fRd, a.fz, a.LY};
+  const double tol = a.tolerance;
+  const double inner_tol = a.inner_tolerance > 0.0 ? a.inner_tolerance : 1.0e-4;
+  const int limit = a.max_iterations < 1 ? 1 : a.max_iterations;
+  const int inner_cap = a.inner_max_iterations > 0 ? a.inner_max_iterations : limit;
+  clk.mark(0);
+
+  double m_x = 0.0, rel_res = 0.0, abs_res = 0.0, iter_res = 0.0;
+  int total = 0, cycles = 0;
+  for (;;) {
+    // true residual r = b - A (x - m_x) in FP64, projected
+    double rs = 0.0;
+    for (int q = tid; q < n; q += kThreads) {
+      const uint32_t c = cellv[q];
+      const double rq = bv[q] - spmv_row<double>(A64, xv, m_x, q, static_cast<int>(c >> 16),
+                                                 static_cast<int>(c & 0xffffu));
+      rv[q] = rq;
+      rs = rs + rq;
+    }
+    rs = block_sum1(rs, sh, parity);
+    const double m_r64 = projected ? rs / nd : 0.0;
+    double rr = 0.0;
+    for (int q = tid; q < n; q += kThreads) {
+      const double tq = rv[q] - m_r64;
+      rr = rr + tq * tq;
+    }
+    rr = block_sum1(rr, sh, parity);
+    clk.mark(1);
+    abs_res = sqrt(rr < 0.0 ? 0.0 : rr);
+    rel_res = abs_res / (bnorm < 1.0e-30 ? 1.0e-30 : bnorm);
+    if (rel_res < tol || abs_res < tol || total >= limit || cycles >= kMaxOuterCycles) break;
+
+    // inner FP32 PCG on A d = (r - m_r) / |r|
+    const double scale = 1.0 / abs_res;
+    double r0s = 0.0;
+    for (int q = tid; q < n; q += kThreads) {
+      const float rq = static_cast<float>((rv[q] - m_r64) * scale);
+      r32[q] = rq;
+      dv[q] = 0.0f;
+      r0s = r0s + static_cast<double>(rq);
+    }
+    r0s = block_sum1(r0s, sh, parity);
+    float m_r = projected ? static_cast<float>(r0s / nd) : 0.0f;
+    float m_d = 0.0f;
+    double rr0 = 0.0;
+    for (int q = tid; q < n; q += kThreads) {
+      const double tq = static_cast<double>(r32[q] - m_r);
+      rr0 = rr0 + tq * tq;
+    }
+    rr0 = block_sum1(rr0, sh, parity);
+    const double rn0 = sqrt(rr0 < 0.0 ? 0.0 : rr0);
+    clk.mark(2);
+    int inner = 0;
+    precondition<float, kIldlt>(a, fwd, bwd, sh, m_r, 0.0f, clk.ticks, clk.t);
+    double zs = 0.0;
+    for (int q = tid; q < n; q += kThreads) zs = zs + static_cast<double>(z32[q]);
+    zs = block_sum1(zs, sh, parity);
+    float m_z = projected ? static_cast<float>(zs / nd) : 0.0f;
+    double rz = 0.0;
+    for (int q = tid; q < n; q += kThreads) {
+      const float zq = z32[q] - m_z;
+      rz = rz + static_cast<double>(r32[q] - m_r) * static_cast<double>(zq);
+      p32[q] = zq;
+    }
+    rz = block_sum1(rz, sh, parity);
+    float m_p = 0.0f;
+    clk.mark(7);
+    while (isfinite(rz) && rz > 0.0 && inner < inner_cap && total < limit) {
+      double pap = 0.0;
+      for (int q = tid; q < n; q += kThreads) {
+        const uint32_t c = cellv[q];
+        const float apq = spmv_row<float>(A32, p32, m_p, q, static_cast<int>(c >> 16), static_cast<int>(c & 0xffffu));
+        ap32[q] = apq;
+        pap = pap + static_cast<double>(p32[q] - m_p) * static_cast<double>(apq);
+      }
+      pap = block_sum1(pap, sh, parity);
+      clk.mark(1);
+      if (!isfinite(pap) || fabs(pap) < 1.0e-30) break;
+      const float alpha = static_cast<float>(rz / pap);
+      double up[2] = {0.0, 0.0};
+      for (int q = tid; q < n; q += kThreads) {
+        const float dn = (dv[q] - m_d) + alpha * (p32[q] - m_p);
+        const float rn = (r32[q] - m_r) - alpha * ap32[q];
+        dv[q] = dn;
+        r32[q] = rn;
+        up[0] = up[0] + static_cast<double>(dn);
+        up[1] = up[1] + static_cast<double>(rn);
+      }
+      block_sum<2>(up, sh, parity);
+      m_d = projected ? static_cast<float>(up[0] / nd) : 0.0f;
+      m_r = projected ? static_cast<float>(up[1] / nd) : 0.0f;
+      double rri = 0.0;
+      for (int q = tid; q < n; q += kThreads) {
+        const double tq = static_cast<double>(r32[q] - m_r);
+        rri = rri + tq * tq;
+      }
+      rri = block_sum1(rri, sh, parity);
+      clk.mark(3);
+      ++inner;
+      ++total;
+      if (sqrt(rri < 0.0 ? 0.0 : rri) < inner_tol * rn0 || inner >= inner_cap || total >= limit) break;
+      precondition<float, kIldlt>(a, fwd, bwd, sh, m_r, 0.0f, clk.ticks, clk.t);
+      zs = 0.0;
+      for (int q = tid; q < n; q += kThreads) zs = zs + static_cast<double>(z32[q]);
+      zs = block_sum1(zs, sh, parity);
+      m_z = projected ? static_cast<float>(zs / nd) : 0.0f;
+      double rzn = 0.0;
+      for (int q = tid; q < n; q += kThreads)
+        rzn = rzn + static_cast<double>(r32[q] - m_r) * static_cast<double>(z32[q] - m_z);
+      rzn = block_sum1(rzn, sh, parity);
+      clk.mark(7);
+      if (!isfinite(rzn) || rzn <= 0.0) break;
+      const float beta = static_cast<float>(rzn / rz);
+      double ps = 0.0;
+      for (int q = tid; q < n; q += kThreads) {
+        const float pn = (z32[q] - m_z) + beta * (p32[q] - m_p);
+        p32[q] = pn;
+        ps = ps + static_cast<double>(pn);
+      }
+      ps = block_sum1(ps, sh, parity);
+      m_p = projected ? static_cast<float>(ps / nd) : 0.0f;
+      rz = rzn;
+      clk.mark(8);
+    }
+    if (inner == 0) break;  // the inner solve cannot make progress: stop like a silent non-convergence
+    // x += |r| d in FP64 (x kept unprojected, its mean tracked for the lazy projection)
+    double xs = 0.0, dd = 0.0, xx = 0.0;
+    for (int q = tid; q < n; q += kThreads) {
+      const double delta = abs_res * static_cast<double>(dv[q] - m_d);
+      const double xn = (xv[q] - m_x) + delta;
+      xv[q] = xn;
+      xs = xs + xn;
+      dd = dd + delta * delta;
+      xx = xx + xn * xn;
+    }
+    double acc[3] = {xs, dd, xx};
+    block_sum<3>(acc, sh, parity);
+    m_x = projected ? acc[0] / nd : 0.0;
+    iter_res = sqrt(acc[1] / (acc[2] < 1.0e-30 ? 1.0e-30 : acc[2]));
+    ++cycles;
+    clk.mark(2);
+  }
+  if (*lds(&sh.spin_error)) {
+    if (tid == 0) *lds(&sh.status) = kSweepFail;
+    clk.finish(a.rep);
+    return;
+  }
+  double xs = 0.0;
+  for (int q = tid; q < n; q += kThreads) xs = xs + (xv[q] - m_x);
+  xs = block_sum1(xs, sh, parity);
+  const double mean = projected ? xs / static_cast<double>(nx * ny) : 0.0;
+  for (int q = tid; q < n; q += kThreads) {
+    const uint32_t c = a.cell[q];
+    const int i = static_cast<int>(c >> 16), j = static_cast<int>(c & 0xffffu);
+    const double xq = xv[q] - m_x;
+    a.p_out[i * ny + j] = projected ? xq - mean : xq;
+  }
+  clk.mark(9);
+  if (tid == 0) {
+    a.rep->iterations = total;
+    a.rep->outer_cycles = cycles;
+    a.rep->rel = rel_res;
+    a.rep->abs = abs_res;
+    a.rep->iter_res = iter_res;
+  }
+  clk.finish(a.rep);
+}
+
 }  // namespace
 
 template <bool kIldlt>
@@ -1012,6 +1292,23 @@ __global__ void __launch_bounds__(kThreads) fp64_solve_kernel(const SysArgs *__r
   solve_system<kIldlt>(a, sh);
   __syncthreads();
   if (threadIdx.x == 0) a.rep->status = *lds(&sh.status);
+}
+
+template <bool kIldlt>
+__global__ void __launch_bounds__(kThreads) mixed_solve_kernel(const SysArgs *__restrict__ systems) {
+  __shared__ SweepShared sh;
+  const SysArgs &a = systems[blockIdx.x];
+  mixed_solve<kIldlt>(a, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) a.rep->status = *lds(&sh.status);
+}
+
+hipError_t launch_mixed_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream) {
+  if (ildlt)
+    hipLaunchKernelGGL(mixed_solve_kernel<true>, dim3(count), dim3(kThreads), 0, stream, d_systems);
+  else
+    hipLaunchKernelGGL(mixed_solve_kernel<false>, dim3(count), dim3(kThreads), 0, stream, d_systems);
+  return hipGetLastError();
 }
 
 hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream) {

@@ -216,7 +216,6 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
 
 // Fills the geometry/work part of `a` for problem pb; field pointers are the caller's.
 int prepare(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *report) {
-  if (pb->precision != PP_FP64) return fail(ctx, report, PP_ERR_ARG, "mixed precision is not available in this build");
   pp_ctx::Layout *L = nullptr;
   if (int rc = ensure_layout(ctx, pb->nx, pb->ny, &L, report)) return rc;
   if (int rc = ensure_work(ctx, static_cast<size_t>(pb->nx) * pb->ny, report)) return rc;
@@ -241,7 +240,10 @@ int run_solve(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *repo
   PP_HIP_CHECK(hipMemcpyAsync(ctx->d_args, &a, sizeof(a), hipMemcpyHostToDevice, stream));
   PP_HIP_CHECK(hipMemsetAsync(ctx->d_rep, 0, sizeof(pp::DevReport), stream));
   PP_HIP_CHECK(hipEventRecord(ctx->ev_start, stream));
-  PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_args, 1, a.ildlt != 0, stream));
+  if (pb->precision == PP_MIXED)
+    PP_HIP_CHECK(pp::launch_mixed_solve(ctx->d_args, 1, a.ildlt != 0, stream));
+  else
+    PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_args, 1, a.ildlt != 0, stream));
   PP_HIP_CHECK(hipEventRecord(ctx->ev_stop, stream));
   PP_HIP_CHECK(hipMemcpyAsync(ctx->h_rep, ctx->d_rep, sizeof(pp::DevReport), hipMemcpyDeviceToHost, stream));
   PP_HIP_CHECK(hipStreamSynchronize(stream));

@@ -53,6 +53,9 @@ struct SysArgs {
 // decreasing cost is scheduled largest first.
 hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream);
 
+// Mixed precision (FP32 inner PCG inside FP64 refinement), one workgroup per system.
+hipError_t launch_mixed_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream);
+
 // Default and valid operand ring depths for S sweep waves: a power of two, at least one 8-level
 // sweep chunk (the sweep releases ring space once per chunk, so a shallower ring would stall its
 // own loaders), and S * ring * 256 doubles within the LDS stage.

@@ -64,6 +64,7 @@ def large_cases():
     out.append(("200x72_periodic_x_dirichlet_top", W.droplet_case(200, 72, seed=6, density_ratio=50.0,
                                                                   periodic_x=True, bc_type=(U, U, U, D),
                                                                   bc_value=(0.0, 0.0, 0.0, 2.0))))
+    out.append(("420x40_seven_sweep_waves", W.droplet_case(420, 40, seed=3, density_ratio=100.0)))
     split = W.droplet_case(192, 192, seed=1)
     split.step_index = 1
     split.p_n = np.zeros(split.n)

@@ -1,0 +1,91 @@
+"""Mixed precision (FP32 inner PCG inside FP64 refinement; no reference counterpart), pinned by the
+FP64 oracle:
+* the true FP64 residual of the returned pressure, recomputed with the oracle's own operator, meets
+  the tolerance;
+* its error against a tightly converged oracle solution (relative residual 1e-13) is comparable to
+  the FP64 path's error at the same tolerance ("equal error").
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from ding_nsch_core_solver_amd import engine as E
+from ding_nsch_core_solver_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CODES = {"icpcg": E.PP_ICPCG, "ildlt_pcg": E.PP_ILDLT_PCG, "split_icpcg": E.PP_SPLIT_ICPCG,
+         "split_ildlt_pcg": E.PP_SPLIT_ILDLT_PCG}
+
+
+def _cases():
+    out = [("192x192_dr1000_seed0", W.droplet_case(192, 192, seed=0), "icpcg"),
+           ("192x192_dr1000_seed1_ildlt", W.droplet_case(192, 192, seed=1), "ildlt_pcg"),
+           ("64x64_periodic_dr1", W.droplet_case(64, 64, seed=0, density_ratio=1.0, periodic_x=True, periodic_y=True),
+            "icpcg"),
+           ("96x192_dr300", W.droplet_case(96, 192, seed=5, density_ratio=300.0, radius=0.15), "icpcg"),
+           ("200x72_periodic_x_dirichlet_top", W.droplet_case(200, 72, seed=6, density_ratio=50.0, periodic_x=True,
+                                                              bc_type=(0, 0, 0, 1), bc_value=(0, 0, 0, 2.0)),
+            "ildlt_pcg"),
+           ("130x12_periodic_x_split", W.droplet_case(130, 12, seed=12, density_ratio=30.0, periodic_x=True),
+            "split_icpcg"),
+           ("256x256_dr100", W.droplet_case(256, 256, seed=0, density_ratio=100.0), "icpcg"),
+           ("420x40_fallback_sweeps", W.droplet_case(420, 40, seed=3, density_ratio=100.0), "icpcg")]
+    return out
+
+
+CASES = _cases()
+
+
+def _true_residual(oracle_lib, case, scheme, p):
+    """|P(b - A p)| / |b| with the oracle's assembled (regularized) FP64 system."""
+    s = oracle_lib.OracleSystem(case, use_split=scheme.startswith("split"))
+    r = s.rhs - s.apply_matrix(p)
+    if not s.has_dirichlet:
+        r = r - r.mean()
+    return np.linalg.norm(r) / np.linalg.norm(s.rhs)
+
+
+def _err(p, ref):
+    return float(np.abs(p - ref).max() / np.abs(ref).max())
+
+
+@pytest.mark.parametrize("name,case,scheme", CASES, ids=[c[0] for c in CASES])
+def test_mixed_meets_tolerance_with_fp64_error(engine, oracle_lib, name, case, scheme):
+    p_mix, rep = engine.solve(case, CODES[scheme], precision=E.PP_MIXED)
+    assert rep["status"] == 0
+    assert rep["relative_residual"] < case.tolerance
+    assert rep["outer_cycles"] >= 1
+    assert _true_residual(oracle_lib, case, scheme, p_mix) < 2.0 * case.tolerance
+    p64, rep64 = engine.solve(case, CODES[scheme])
+    tight = dataclasses.replace(case, tolerance=1e-13, max_iterations=20000)
+    p_ex, _, _ = oracle_lib.pressure_solve(scheme, tight, oracle_lib.SEQUENTIAL)
+    e_mix, e64 = _err(p_mix, p_ex), _err(p64, p_ex)
+    print(f"[mixed] {name}: iterations {rep['iterations']} ({rep['outer_cycles']} cycles) vs FP64 "
+          f"{rep64['iterations']}; error vs 1e-13 solution: mixed {e_mix:.2e}, FP64 {e64:.2e}; "
+          f"kernel {rep['kernel_ms']:.2f} ms vs {rep64['kernel_ms']:.2f} ms")
+    assert e_mix <= 10.0 * max(e64, 1e-12)
+
+
+@pytest.mark.parametrize("inner_tol", [1e-2, 1e-3, 1e-5])
+def test_inner_tolerance_settings_converge(engine, oracle_lib, inner_tol):
+    case = W.droplet_case(128, 128, seed=2, density_ratio=500.0)
+    p, rep = engine.solve(case, E.PP_ICPCG, precision=E.PP_MIXED, inner_tolerance=inner_tol)
+    assert rep["relative_residual"] < case.tolerance
+    assert _true_residual(oracle_lib, case, "icpcg", p) < 2.0 * case.tolerance
+
+
+def test_mixed_silent_nonconvergence(engine):
+    case = W.droplet_case(64, 64, seed=0, density_ratio=100.0, max_iterations=5)
+    p, rep = engine.solve(case, E.PP_ICPCG, precision=E.PP_MIXED)
+    assert rep["iterations"] <= 5 and rep["relative_residual"] > case.tolerance
+    assert np.all(np.isfinite(p))
+
+
+def test_mixed_factor_failure_message(engine):
+    case = W.droplet_case(8, 8, seed=0, density_ratio=10.0)
+    case.rho = case.rho.copy()
+    case.rho[:3, :3] = -5.0
+    with pytest.raises(E.EngineError, match="ICC preconditioner construction failed"):
+        engine.solve(case, E.PP_ICPCG, precision=E.PP_MIXED)
