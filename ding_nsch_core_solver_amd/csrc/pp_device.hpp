@@ -36,6 +36,7 @@ constexpr int kMaxRing = 64;              // deepest operand ring in levels (per
 constexpr int kStageUnit = 8;             // levels a loader moves per step
 constexpr int kMaxStagedWaves = 6;        // more sweep waves leave too few loader waves
 constexpr int kSpinLimit = 1 << 24;       // bounded waits: a broken hand-off ends as an error
+constexpr int kLevPad = 80;               // level-base table margin: sweeps index up to 71 levels past the grid
 
 enum DevStatus : int {
   kOk = 0, kFactorFail = 1, kPapFail = 2, kRz0Fail = 3, kRzFail = 4, kNoDiag = 5, kSweepFail = 6
@@ -175,26 +176,27 @@ __device__ __forceinline__ void reset_handoff(SweepShared &sh) {
 
 // ---- lane shifts and hand-offs between sweep waves ---------------------------------------------
 
-// Lane l receives lane l-1's value (lane 0 receives 0): one DPP wave_shr per 32-bit half.
+// Lane l receives lane l-1's value (lane 0 receives +0): one DPP wave_shr per 32-bit half, with
+// bound_ctrl so the lane without a source reads zero.
 __device__ __forceinline__ double wave_shr1(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
-// Lane l receives lane l+1's value (lane 63 receives 0).
+// Lane l receives lane l+1's value (lane 63 receives +0).
 __device__ __forceinline__ double wave_shl1(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
 __device__ __forceinline__ float wave_shr1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ float wave_shl1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 
 // Value held by lane k (compile-time k), broadcast through SGPRs.
@@ -206,6 +208,13 @@ __device__ __forceinline__ double lane_value(double v, int k) {
 
 __device__ __forceinline__ float lane_value(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// Lane k of the result holds v (wave-uniform, e.g. from lane_value); other lanes keep `into`.
+// The lane comparison is loop-invariant, so this is one select per 32-bit half.
+template <typename T>
+__device__ __forceinline__ T write_lane(T into, T v, int k) {
+  return (threadIdx.x & 63) == static_cast<unsigned>(k) ? v : into;
 }
 
 // Hand-off protocol. Ring hand[w] carries boundary values into wave w; cnt[w] counts values
@@ -252,6 +261,16 @@ __device__ __forceinline__ void publish_nb(int stream_wave, int lane, int src_la
   volatile int *ctr = real ? &sh.cnt[stream_wave] : &sh.dummy_i[lane];
   *lds(slot) = static_cast<double>(value);
   *lds(ctr) = k + 1;
+}
+
+// Chunk publish: lane kk (< kChunk) holds stream value k0 + kk; the values in range go into the ring,
+// then lane 0 advances the counter past the last of them (one LDS write per lane, one counter write).
+template <int kChunk, typename T>
+__device__ __forceinline__ void publish_chunk(int stream_wave, int lane, int k0, int count, T pub, SweepShared &sh) {
+  const int k = k0 + lane;
+  if (lane < kChunk && k >= 0 && k < count) *lds(&sh.hand[stream_wave][k & (kRing - 1)]) = static_cast<double>(pub);
+  const int last = (k0 + kChunk - 1) < (count - 1) ? (k0 + kChunk - 1) : (count - 1);
+  if (lane == 0 && last >= 0) *lds(&sh.cnt[stream_wave]) = last + 1;
 }
 
 // Number of cells on levels < d of an nx x ny grid (0 <= d <= nx + ny - 1).

@@ -451,8 +451,27 @@ __device__ __forceinline__ volatile __attribute__((address_space(3))) T *stage_s
 // key is that offset plus t0. Everything a unit waits for (ring space of its wave, which in turn needs
 // hand-offs from the upstream wave) has a smaller key, so loaders processing their share of units in
 // key order cannot wait on each other in a cycle. Unit k of that order goes to loader k mod L.
+// Staged operand values: the forward input is r - m_r and, in FP64, the pivot is already floored,
+// so the sweep waves do neither (the same operations, moved off the sweep's chain).
+template <typename T>
+__device__ __forceinline__ T staged_pivot(T d, T floor) {
+  if constexpr (sizeof(T) == 8)
+    return floor_max(d, floor);
+  else
+    return d;
+}
+
+template <typename T>
+__device__ __forceinline__ T staged_apply(T s, T d) {
+  if constexpr (sizeof(T) == 8)
+    return s / d;
+  else
+    return s * d;
+}
+
 template <typename T, bool kBackward>
-__device__ __forceinline__ void stage_loader(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, int S) {
+__device__ __forceinline__ void stage_loader(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, int S,
+                                             T m_r, T floor) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny, d_last = nx + ny - 2;
   const int R = stage_ring<T>(a, S), Tn = stage_steps(ny);
@@ -462,6 +481,7 @@ __device__ __forceinline__ void stage_loader(const SysArgs &a, const SweepArrays
   const global_ptr<const T> dg = as_global<const T>(io.dg);
   const global_ptr<const T> Lw = as_global<const T>(io.Lw);
   const global_ptr<const T> Ls = as_global<const T>(io.Ls);
+  const __attribute__((address_space(4))) int *lb = (const __attribute__((address_space(4))) int *)a.lev_base;
   int k_unit = 0;
   for (int tau = 0; tau < 64 * (S - 1) + Tn; tau += U) {
     for (int w = 0; w < S; ++w) {
@@ -475,18 +495,18 @@ __device__ __forceinline__ void stage_loader(const SysArgs &a, const SweepArrays
       for (int k = 0; k < kStageUnit; ++k) {
         if (k < U) {
           const int d = min(max(step_level<kBackward>(w, t0 + k, Tn), 0), d_last);
-          const int q = level_base(d, nx, ny) + i;
+          const int q = lb[d] + i;
           if (kBackward) {
-            const int qn = level_base(min(d + 1, d_last), nx, ny) + i;  // (i, j+1); (i+1, j) follows it
+            const int qn = lb[min(d + 1, d_last)] + i;  // (i, j+1); (i+1, j) follows it
             v[k][0] = f0[q];
-            v[k][1] = dg[q];
+            v[k][1] = staged_pivot(dg[q], floor);
             v[k][2] = Ls[qn];
             v[k][3] = Lw[qn + 1];
           } else {
-            v[k][0] = f0[q];
+            v[k][0] = f0[q] - m_r;
             v[k][1] = Lw[q];
             v[k][2] = Ls[q];
-            v[k][3] = dg[q];
+            v[k][3] = staged_pivot(dg[q], floor);
           }
         }
       }
@@ -539,8 +559,9 @@ __device__ __forceinline__ void staged_forward(const SysArgs &a, const SweepArra
   const int R = stage_ring<T>(a, S), Tn = stage_steps(ny);
   const int i = wave * 64 + lane;
   const bool px = kPeriodic && a.periodic_x != 0, py = kPeriodic && a.periodic_y != 0;
-  const bool row_ok = i < nx, has_w = i > 0, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
+  const bool row_ok = i < nx, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
   const global_ptr<T> y = as_global(io.out);
+  const __attribute__((address_space(4))) int *lev_base = (const __attribute__((address_space(4))) int *)a.lev_base;
   const int sink = a.n + kPad - 64 + lane;
   const T ly = (py && row_ok) ? static_cast<T>(as_global<const double>(io.LY)[i]) : T(0);
   T prev = T(0), first = T(0);
@@ -551,41 +572,44 @@ __device__ __forceinline__ void staged_forward(const SysArgs &a, const SweepArra
     read_chunk<T>(sh, wave, R, t0, lane, c);
     const T hv = wave > 0 ? receive_batch<T, kPrefetch>(wave, lane, t0, ny, sh) : T(0);
     if (wave + 1 < S) reserve_batch(wave + 1, min(d0 + kPrefetch - 1 - (wave * 64 + 63), ny - 1), sh);
+    int base[kPrefetch];  // level bases of the chunk, scalar loads from the constant table
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) base[k] = lev_base[d0 + k];
+    T pub = T(0);  // lane k: this wave's row 64w+63 value at level d0 + k (wave w+1's west input)
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) {
       const int d = d0 + k;
       const int j = d - i;
-      const bool act = row_ok && j >= 0 && j < ny;
-      T west = wave_shr1(prev);
-      if (wave > 0) {
-        const T h = lane_value(hv, k);
-        west = lane == 0 ? h : west;
-      }
-      T s = c.f[k][0] - m_r;
+      const bool act = row_ok && static_cast<unsigned>(j) < static_cast<unsigned>(ny);
+      // west neighbour: lane l-1 of this wave, lane 0 from wave w-1. Row 0 (lane 0 of wave 0) has no
+      // west entry: its Lw is +0 and its west is +0, so s - Lw * west = s exactly, no select needed.
+      const T west = write_lane(wave_shr1(prev), lane_value(hv, k), 0);
+      T s = c.f[k][0];  // r - m_r
       if (kPeriodic && px) {
         const int jc = min(max(j, 0), ny - 1);
         const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
         s = (wrap_row && act) ? s - static_cast<T>(sh.lx[jc]) * wx : s;
       }
-      s = has_w ? s - c.f[k][1] * west : s;
+      s = s - c.f[k][1] * west;
       if (kPeriodic && py) s = j == ny - 1 ? s - ly * first : s;
       s = j > 0 ? s - c.f[k][2] * prev : s;
       T cur, out;
       if (kIldlt) {
         cur = s;
-        out = pivot_apply(s, c.f[k][3], floor);
+        out = staged_apply(s, c.f[k][3]);
       } else {
-        cur = pivot_apply(s, c.f[k][3], floor);
+        cur = staged_apply(s, c.f[k][3]);
         out = cur;
       }
-      y[act ? level_base(d, nx, ny) + i : sink] = out;
+      y[static_cast<unsigned>(act ? base[k] + i : sink)] = out;
       if (kPeriodic) {
         first = j == 0 ? cur : first;
         if (px && wrap_src && act) *lds(&sh.wrap[j]) = static_cast<double>(cur);
       }
       prev = cur;
-      if (wave + 1 < S) publish_nb(wave + 1, lane, 63, d - (wave * 64 + 63), ny, cur, sh);
+      pub = write_lane(pub, lane_value(cur, 63), k);
     }
+    if (wave + 1 < S) publish_chunk<kPrefetch>(wave + 1, lane, d0 - (wave * 64 + 63), ny, pub, sh);
     if (lane == 0) *lds(&sh.used[wave]) = t0 + kPrefetch;
   }
 }
@@ -600,6 +624,7 @@ __device__ __forceinline__ void staged_backward(const SysArgs &a, const SweepArr
   const bool px = kPeriodic && a.periodic_x != 0, py = kPeriodic && a.periodic_y != 0;
   const bool row_ok = i < nx, has_e = i + 1 < nx, wrap_row = px && i == nx - 1, wrap_src = px && i == 0;
   const global_ptr<T> z = as_global(io.out);
+  const __attribute__((address_space(4))) int *lev_base = (const __attribute__((address_space(4))) int *)a.lev_base;
   const int sink = a.n + kPad - 64 + lane;
   const T ly = (py && row_ok) ? static_cast<T>(as_global<const double>(io.LY)[i]) : T(0);
   T prev = T(0), first = T(0);
@@ -611,16 +636,17 @@ __device__ __forceinline__ void staged_backward(const SysArgs &a, const SweepArr
     read_chunk<T>(sh, wave, R, t0, lane, c);
     const T hv = wave + 1 < S ? receive_batch<T, kPrefetch>(wave, lane, ny - 1 - j63_first, ny, sh) : T(0);
     if (wave > 0) reserve_batch(wave - 1, min(ny - 1 - (d0 - kPrefetch + 1 - wave * 64), ny - 1), sh);
+    int base[kPrefetch];  // level bases of the chunk, scalar loads from the constant table
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) base[k] = lev_base[d0 - k];
+    T pub = T(0);  // lane k: this wave's row 64w value at level d0 - k (wave w-1's east input)
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) {
       const int d = d0 - k;
       const int j = d - i;
-      const bool act = row_ok && j >= 0 && j < ny;
-      T east = wave_shl1(prev);
-      if (wave + 1 < S) {
-        const T h = lane_value(hv, k);
-        east = lane == 63 ? h : east;
-      }
+      const bool act = row_ok && static_cast<unsigned>(j) < static_cast<unsigned>(ny);
+      // east neighbour: lane l+1 of this wave, lane 63 from wave w+1 (0 for the last wave)
+      const T east = write_lane(wave_shl1(prev), lane_value(hv, k), 63);
       T s = c.f[k][0];
       s = j + 1 < ny ? s - c.f[k][2] * prev : s;
       if (kPeriodic && py) s = j == 0 ? s - ly * first : s;
@@ -630,15 +656,16 @@ __device__ __forceinline__ void staged_backward(const SysArgs &a, const SweepArr
         const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
         s = (wrap_src && act) ? s - static_cast<T>(sh.lx[jc]) * wx : s;
       }
-      const T cur = kIldlt ? s : pivot_apply(s, c.f[k][1], floor);
-      z[act ? level_base(d, nx, ny) + i : sink] = cur;
+      const T cur = kIldlt ? s : staged_apply(s, c.f[k][1]);
+      z[static_cast<unsigned>(act ? base[k] + i : sink)] = cur;
       if (kPeriodic) {
         first = j == ny - 1 ? cur : first;
         if (px && wrap_row && act) *lds(&sh.wrap[j]) = static_cast<double>(cur);
       }
       prev = cur;
-      if (wave > 0) publish_nb(wave - 1, lane, 0, ny - 1 - (d - wave * 64), ny, cur, sh);
+      pub = write_lane(pub, lane_value(cur, 0), k);
     }
+    if (wave > 0) publish_chunk<kPrefetch>(wave - 1, lane, ny - 1 - (d0 - wave * 64), ny, pub, sh);
     if (lane == 0) *lds(&sh.used[wave]) = t0 + kPrefetch;
   }
 }
@@ -656,36 +683,127 @@ struct Stencil {
   global_ptr<const T> aP, aW, aS;
 };
 
-// (A p)_q in CSR column order, p read through the lazy projection p - m_p.
+// Level-major SpMV: wave w takes levels w, w + kWaves, w + 2 kWaves, ...; its lanes walk the
+// level's cells (contiguous positions base(d) + i), so every neighbour address is a scalar level
+// base plus the lane's row and no index array is read. Row sums follow the CSR column order
+// (ch_sparse_krylov.hpp:242-248): wrap (0, j), west, wrap (i, 0), south, centre, north,
+// wrap (i, ny-1), east, wrap (nx-1, j). epi(q, s) consumes row q's result and returns its
+// contribution to the reduction; thread (w, lane) adds its contributions in (level, row) order,
+// which the oracle replays for p^T A p (Reducer::sum_levels).
+// One row, branch-free: every operand is loaded from a valid address and an absent entry gets a
+// +0 coefficient by a select, so all loads of a row (and of neighbouring rows) can be in flight at
+// once. The terms are added in the CSR column order; an absent term adds a signed zero to a sum
+// that started at +0.0 and therefore never holds -0, which leaves it bit-identical to skipping it.
 template <typename T>
-__device__ __forceinline__ T spmv_row(const Stencil<T> &A, const global_ptr<const T> p, T m_p, int q, int i, int j) {
+__device__ __forceinline__ T spmv_cell(const Stencil<T> &A, const global_ptr<const T> p, T m_p, int i, int j, int bm,
+                                       int b0, int bp, T &pq) {
   const int nx = A.nx, ny = A.ny;
-  const int d = i + j;
-  const int offm = d > 0 ? level_offset(d - 1, nx, ny) - level_lo(d - 1, ny) : 0;  // + i' gives level d-1
-  const int offp = level_offset(d + 1, nx, ny) - level_lo(d + 1, ny);                // + i' gives level d+1
+  const int q = b0 + i;
+  const bool has_w = i >= 1, has_s = j >= 1, has_n = j <= ny - 2, has_e = i <= nx - 2;
+  const int qw = has_w ? bm + i - 1 : q, qs = has_s ? bm + i : q;
+  const int qn = bp + i, qe = bp + i + 1;
+  const T cw = has_w ? A.aW[q] : T(0), cs = has_s ? A.aS[q] : T(0);
+  const T cn = has_n ? A.aS[qn] : T(0), ce = has_e ? A.aW[qe] : T(0);
+  const T pw = p[qw], ps = p[qs], pc = p[q], pn = p[qn], pe = p[qe], cp = A.aP[q];
   T s = T(0);
-  if (A.px && i == nx - 1) {
-    const int qw = grid_pos(0, j, nx, ny);
-    s += A.aW[qw] * (p[qw] - m_p);
+  // wrap terms only for cells inside the grid (the lane may hold a cell past its level)
+  const bool in_i = static_cast<unsigned>(i) < static_cast<unsigned>(nx);
+  const bool in_j = static_cast<unsigned>(j) < static_cast<unsigned>(ny);
+  if (A.px) {  // wrap (0, j) of row (nx-1, j)
+    const bool on = i == nx - 1 && in_j;
+    const int qx = on ? level_base(j, nx, ny) : q;
+    s += (on ? A.aW[qx] : T(0)) * (p[qx] - m_p);
   }
-  if (i >= 1) s += A.aW[q] * (p[offm + i - 1] - m_p);
-  if (A.py && j == ny - 1) {
-    const int qs = grid_pos(i, 0, nx, ny);
-    s += A.aS[qs] * (p[qs] - m_p);
+  s += cw * (pw - m_p);
+  if (A.py) {  // wrap (i, 0) of row (i, ny-1)
+    const bool on = j == ny - 1 && in_i;
+    const int qy = on ? level_base(i, nx, ny) + i : q;
+    s += (on ? A.aS[qy] : T(0)) * (p[qy] - m_p);
   }
-  if (j >= 1) s += A.aS[q] * (p[offm + i] - m_p);
-  s += A.aP[q] * (p[q] - m_p);
-  if (j <= ny - 2) {
-    const int qn = offp + i;
-    s += A.aS[qn] * (p[qn] - m_p);
+  s += cs * (ps - m_p);
+  s += cp * (pc - m_p);
+  s += cn * (pn - m_p);
+  if (A.py) {  // wrap (i, ny-1) of row (i, 0)
+    const bool on = j == 0 && in_i;
+    const int qy = on ? level_base(i + ny - 1, nx, ny) + i : q;
+    s += (on ? A.aS[q] : T(0)) * (p[qy] - m_p);
   }
-  if (A.py && j == 0) s += A.aS[q] * (p[grid_pos(i, ny - 1, nx, ny)] - m_p);
-  if (i <= nx - 2) {
-    const int qe = offp + i + 1;
-    s += A.aW[qe] * (p[qe] - m_p);
+  s += ce * (pe - m_p);
+  pq = pc - m_p;
+  if (A.px) {  // wrap (nx-1, j) of row (0, j)
+    const bool on = i == 0 && in_j;
+    const int qx = on ? level_base(nx - 1 + j, nx, ny) + nx - 1 : q;
+    s += (on ? A.aW[q] : T(0)) * (p[qx] - m_p);
   }
-  if (A.px && i == 0) s += A.aW[q] * (p[grid_pos(nx - 1, j, nx, ny)] - m_p);
   return s;
+}
+
+constexpr int kSpmvChunks = 4;  // 64-cell chunks per level computed before any result is stored
+
+// epi(q, s, pq, ex) gets row q's result s, pq = p[q] - m_p and ex = extra[q] (0 when extra is null);
+// every load of a pass happens before its epilogues store anything.
+template <typename T, typename F>
+__device__ __forceinline__ double spmv_levels(const SysArgs &a, const Stencil<T> &A, const global_ptr<const T> p, T m_p,
+                                              const global_ptr<const T> extra, F &&epi) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int nx = A.nx, ny = A.ny, nlev = nx + ny - 1;
+  const __attribute__((address_space(4))) int *lb = (const __attribute__((address_space(4))) int *)a.lev_base;
+  const int max_cells = nx < ny ? nx : ny;
+  const int chunks = (max_cells + 63) >> 6;  // chunks per level that can hold cells (uniform)
+  double acc = 0.0;
+  if (chunks <= kSpmvChunks) {
+    // two levels per pass, every chunk of both loaded first; contributions added in level order
+    // (an absent cell adds +0.0, exact: the running sum is never -0)
+    for (int d0 = wave; d0 < nlev; d0 += 2 * kWaves) {
+      T sv[2][kSpmvChunks], pv[2][kSpmvChunks], xv[2][kSpmvChunks];
+      int qv[2][kSpmvChunks];
+      bool ok[2][kSpmvChunks];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int d = d0 + u * kWaves;
+        const int lo = level_lo(d, ny), hi = d < nx - 1 ? d : nx - 1;
+        const int bm = lb[d - 1], b0 = lb[d], bp = lb[d + 1];
+#pragma unroll
+        for (int c = 0; c < kSpmvChunks; ++c) {
+          if (c < chunks) {
+            const int i = lo + 64 * c + lane;
+            ok[u][c] = d < nlev && i <= hi;
+            qv[u][c] = b0 + i;
+            sv[u][c] = spmv_cell<T>(A, p, m_p, i, d - i, bm, b0, bp, pv[u][c]);  // in-bounds loads even when !ok
+            xv[u][c] = extra ? extra[b0 + i] : T(0);
+          } else {
+            ok[u][c] = false;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int c = 0; c < kSpmvChunks; ++c)
+          if (c < chunks) acc = acc + (ok[u][c] ? epi(qv[u][c], sv[u][c], pv[u][c], xv[u][c]) : 0.0);
+    }
+  } else {
+    // wide levels: one level at a time, its chunks in groups
+    for (int d = wave; d < nlev; d += kWaves) {
+      const int lo = level_lo(d, ny), hi = d < nx - 1 ? d : nx - 1;
+      const int bm = lb[d - 1], b0 = lb[d], bp = lb[d + 1];
+      for (int i0 = lo; i0 <= hi; i0 += 64 * kSpmvChunks) {
+        T sv[kSpmvChunks], pv[kSpmvChunks], xv[kSpmvChunks];
+        bool ok[kSpmvChunks];
+#pragma unroll
+        for (int c = 0; c < kSpmvChunks; ++c) {
+          const int i = i0 + 64 * c + lane;
+          ok[c] = i <= hi;
+          sv[c] = spmv_cell<T>(A, p, m_p, i, d - i, bm, b0, bp, pv[c]);  // in-bounds loads even when !ok
+          xv[c] = extra ? extra[b0 + i] : T(0);
+        }
+#pragma unroll
+        for (int c = 0; c < kSpmvChunks; ++c)
+          acc = acc + (ok[c] ? epi(b0 + i0 + 64 * c + lane, sv[c], pv[c], xv[c]) : 0.0);
+      }
+    }
+  }
+  return acc;
 }
 
 // z = M^{-1} (r - m_r): forward then backward sweep, staged through LDS when S <= kMaxStagedWaves.
@@ -699,16 +817,20 @@ __device__ __forceinline__ void precondition(const SysArgs &a, const SweepArrays
   reset_handoff(sh);
   reset_stage(sh);
   __syncthreads();
+  // sweep waves carry the dependency chain: they win instruction issue over loader waves on the
+  // same SIMD (wave priority, restored after the sweep)
   if (!staged)
     forward_solve<T, kIldlt>(a, fwd, sh, m_r, floor);
   else if (wave < S) {
+    __builtin_amdgcn_s_setprio(3);
     if (a.periodic_x || a.periodic_y)
       staged_forward<T, kIldlt, true>(a, fwd, sh, m_r, floor, S);
     else
       staged_forward<T, kIldlt, false>(a, fwd, sh, m_r, floor, S);
+    __builtin_amdgcn_s_setprio(0);
   }
   else
-    stage_loader<T, false>(a, fwd, sh, S);
+    stage_loader<T, false>(a, fwd, sh, S, m_r, floor);
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned long long now = device_ticks();
@@ -721,13 +843,15 @@ __device__ __forceinline__ void precondition(const SysArgs &a, const SweepArrays
   if (!staged)
     backward_solve<T, kIldlt>(a, bwd, sh, floor);
   else if (wave < S) {
+    __builtin_amdgcn_s_setprio(3);
     if (a.periodic_x || a.periodic_y)
       staged_backward<T, kIldlt, true>(a, bwd, sh, floor, S);
     else
       staged_backward<T, kIldlt, false>(a, bwd, sh, floor, S);
+    __builtin_amdgcn_s_setprio(0);
   }
   else
-    stage_loader<T, true>(a, bwd, sh, S);
+    stage_loader<T, true>(a, bwd, sh, S, T(0), floor);
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned long long now = device_ticks();
@@ -828,7 +952,6 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
   const global_ptr<double> zv = as_global(a.z);
   const global_ptr<double> bv = as_global(a.b);
   const global_ptr<double> apv = as_global(a.y);  // Ap shares storage with the sweep intermediate
-  const global_ptr<const uint32_t> cellv = as_global<const uint32_t>(a.cell);
   const SweepArrays<double> fwd{a.r, a.Lw, a.Ls, a.dg, a.y, a.LY};
   const SweepArrays<double> bwd{a.y, a.Lw, a.Ls, a.dg, a.z, a.LY};
   const Stencil<double> stencil{nx, ny, a.periodic_x != 0, a.periodic_y != 0, as_global<const double>(a.aP),
@@ -886,28 +1009,12 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
     mark(0);
     const int limit = a.max_iterations < 1 ? 1 : a.max_iterations;
     for (int it = 0; it < limit; ++it) {
-      // Ap and p^T A p
-      double pap = 0.0;
-      for (int base = tid; base < n; base += kThreads * 4) {
-        double apq[4], pq[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int q = base + u * kThreads;
-          const int qq = q < n ? q : 0;
-          const uint32_t c = cellv[qq];
-          apq[u] = spmv_row(stencil, p_raw, m_p, qq, static_cast<int>(c >> 16), static_cast<int>(c & 0xffffu));
-          pq[u] = p_raw[qq] - m_p;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int q = base + u * kThreads;
-          if (q < n) {
-            apv[q] = apq[u];
-            p_prj[q] = pq[u];
-            pap = pap + pq[u] * apq[u];
-          }
-        }
-      }
+      // Ap and p^T A p (level-major order)
+      double pap = spmv_levels<double>(a, stencil, p_raw, m_p, nullptr, [&](int q, double apq, double pq, double) {
+        apv[q] = apq;
+        p_prj[q] = pq;
+        return pq * apq;
+      });
       pap = block_sum1(pap, sh, parity);
       mark(1);
       if (!isfinite(pap) || fabs(pap) < 1.0e-30) {
@@ -1088,7 +1195,6 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
   }
   const double nd = su.nd, floor = su.floor;
   const bool projected = a.projected != 0;
-  const global_ptr<const uint32_t> cellv = as_global<const uint32_t>(a.cell);
   const global_ptr<double> xv = as_global(a.x), rv = as_global(a.r), bv = as_global(a.b);
   const global_ptr<float> fP = as_global(a.fP), fW = as_global(a.fW), fS = as_global(a.fS);
   const global_ptr<float> fLw = as_global(a.fLw), fLs = as_global(a.fLs), fRd = as_global(a.fRd);
@@ -1117,7 +1223,10 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
   const SweepArrays<float> fwd{a.fr, a.fLw, a.fLs, a.fRd, a.fy, a.LY};
   const SweepArrays<float> bwd{a.fy, a.fLw, a.fLs, a.fRd, a.fz, a.LY};
   const double tol = a.tolerance;
-  const double inner_tol = a.inner_tolerance > 0.0 ? a.inner_tolerance : 1.0e-4;
+  // default inner reduction 0.5 sqrt(tol) (at least 1e-5, about what FP32 PCG attains): two cycles
+  // reach tol; each cycle asks no more than the remaining reduction 0.5 tol |b| / |r|
+  const double inner_tol = a.inner_tolerance > 0.0 ? a.inner_tolerance
+                                                    : (0.5 * sqrt(a.tolerance) > 1.0e-5 ? 0.5 * sqrt(a.tolerance) : 1.0e-5);
   const int limit = a.max_iterations < 1 ? 1 : a.max_iterations;
   const int inner_cap = a.inner_max_iterations > 0 ? a.inner_max_iterations : limit;
   clk.mark(0);
@@ -1126,14 +1235,11 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
   int total = 0, cycles = 0;
   for (;;) {
     // true residual r = b - A (x - m_x) in FP64, projected
-    double rs = 0.0;
-    for (int q = tid; q < n; q += kThreads) {
-      const uint32_t c = cellv[q];
-      const double rq = bv[q] - spmv_row<double>(A64, xv, m_x, q, static_cast<int>(c >> 16),
-                                                 static_cast<int>(c & 0xffffu));
+    double rs = spmv_levels<double>(a, A64, xv, m_x, bv, [&](int q, double axq, double, double bq) {
+      const double rq = bq - axq;
       rv[q] = rq;
-      rs = rs + rq;
-    }
+      return rq;
+    });
     rs = block_sum1(rs, sh, parity);
     const double m_r64 = projected ? rs / nd : 0.0;
     double rr = 0.0;
@@ -1149,6 +1255,8 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
 
     // inner FP32 PCG on A d = (r - m_r) / |r|
     const double scale = 1.0 / abs_res;
+    const double needed = 0.5 * tol * (bnorm < 1.0e-30 ? 1.0e-30 : bnorm) / abs_res;
+    const double cycle_tol = needed > inner_tol ? needed : inner_tol;
     double r0s = 0.0;
     for (int q = tid; q < n; q += kThreads) {
       const float rq = static_cast<float>((rv[q] - m_r64) * scale);
@@ -1183,56 +1291,116 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
     float m_p = 0.0f;
     clk.mark(7);
     while (isfinite(rz) && rz > 0.0 && inner < inner_cap && total < limit) {
-      double pap = 0.0;
-      for (int q = tid; q < n; q += kThreads) {
-        const uint32_t c = cellv[q];
-        const float apq = spmv_row<float>(A32, p32, m_p, q, static_cast<int>(c >> 16), static_cast<int>(c & 0xffffu));
+      double pap = spmv_levels<float>(a, A32, p32, m_p, nullptr, [&](int q, float apq, float pq, float) {
         ap32[q] = apq;
-        pap = pap + static_cast<double>(p32[q] - m_p) * static_cast<double>(apq);
-      }
+        return static_cast<double>(pq) * static_cast<double>(apq);
+      });
       pap = block_sum1(pap, sh, parity);
       clk.mark(1);
       if (!isfinite(pap) || fabs(pap) < 1.0e-30) break;
       const float alpha = static_cast<float>(rz / pap);
       double up[2] = {0.0, 0.0};
-      for (int q = tid; q < n; q += kThreads) {
-        const float dn = (dv[q] - m_d) + alpha * (p32[q] - m_p);
-        const float rn = (r32[q] - m_r) - alpha * ap32[q];
-        dv[q] = dn;
-        r32[q] = rn;
-        up[0] = up[0] + static_cast<double>(dn);
-        up[1] = up[1] + static_cast<double>(rn);
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        float dq[kUnroll], rq[kUnroll], pq[kUnroll], aq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          const int qq = q < n ? q : 0;
+          dq[u] = dv[qq];
+          rq[u] = r32[qq];
+          pq[u] = p32[qq];
+          aq[u] = ap32[qq];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          if (q < n) {
+            const float dn = (dq[u] - m_d) + alpha * (pq[u] - m_p);
+            const float rn = (rq[u] - m_r) - alpha * aq[u];
+            dv[q] = dn;
+            r32[q] = rn;
+            up[0] = up[0] + static_cast<double>(dn);
+            up[1] = up[1] + static_cast<double>(rn);
+          }
+        }
       }
       block_sum<2>(up, sh, parity);
       m_d = projected ? static_cast<float>(up[0] / nd) : 0.0f;
       m_r = projected ? static_cast<float>(up[1] / nd) : 0.0f;
       double rri = 0.0;
-      for (int q = tid; q < n; q += kThreads) {
-        const double tq = static_cast<double>(r32[q] - m_r);
-        rri = rri + tq * tq;
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        float rq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          rq[u] = r32[q < n ? q : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          if (base + u * kThreads < n) {
+            const double tq = static_cast<double>(rq[u] - m_r);
+            rri = rri + tq * tq;
+          }
+        }
       }
       rri = block_sum1(rri, sh, parity);
       clk.mark(3);
       ++inner;
       ++total;
-      if (sqrt(rri < 0.0 ? 0.0 : rri) < inner_tol * rn0 || inner >= inner_cap || total >= limit) break;
+      if (sqrt(rri < 0.0 ? 0.0 : rri) < cycle_tol * rn0 || inner >= inner_cap || total >= limit) break;
       precondition<float, kIldlt>(a, fwd, bwd, sh, m_r, 0.0f, clk.ticks, clk.t);
       zs = 0.0;
-      for (int q = tid; q < n; q += kThreads) zs = zs + static_cast<double>(z32[q]);
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        float zq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          zq[u] = z32[q < n ? q : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+          if (base + u * kThreads < n) zs = zs + static_cast<double>(zq[u]);
+      }
       zs = block_sum1(zs, sh, parity);
       m_z = projected ? static_cast<float>(zs / nd) : 0.0f;
       double rzn = 0.0;
-      for (int q = tid; q < n; q += kThreads)
-        rzn = rzn + static_cast<double>(r32[q] - m_r) * static_cast<double>(z32[q] - m_z);
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        float zq[kUnroll], rq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          const int qq = q < n ? q : 0;
+          zq[u] = z32[qq];
+          rq[u] = r32[qq];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+          if (base + u * kThreads < n)
+            rzn = rzn + static_cast<double>(rq[u] - m_r) * static_cast<double>(zq[u] - m_z);
+      }
       rzn = block_sum1(rzn, sh, parity);
       clk.mark(7);
       if (!isfinite(rzn) || rzn <= 0.0) break;
       const float beta = static_cast<float>(rzn / rz);
       double ps = 0.0;
-      for (int q = tid; q < n; q += kThreads) {
-        const float pn = (z32[q] - m_z) + beta * (p32[q] - m_p);
-        p32[q] = pn;
-        ps = ps + static_cast<double>(pn);
+      for (int base = tid; base < n; base += kThreads * kUnroll) {
+        float zq[kUnroll], pq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          const int qq = q < n ? q : 0;
+          zq[u] = z32[qq];
+          pq[u] = p32[qq];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int q = base + u * kThreads;
+          if (q < n) {
+            const float pn = (zq[u] - m_z) + beta * (pq[u] - m_p);
+            p32[q] = pn;
+            ps = ps + static_cast<double>(pn);
+          }
+        }
       }
       ps = block_sum1(ps, sh, parity);
       m_p = projected ? static_cast<float>(ps / nd) : 0.0f;

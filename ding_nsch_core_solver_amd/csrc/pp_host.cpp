@@ -21,7 +21,7 @@ struct pp_ctx {
 
   struct Layout {
     int nlev = 0;
-    int *off = nullptr;
+    int *base = nullptr;   // level-base table with kLevPad entries of margin on both sides
     uint32_t *cell = nullptr;
   };
   std::map<std::pair<int, int>, Layout> layouts;
@@ -71,16 +71,17 @@ int fail(pp_ctx *ctx, pp_report *rep, int status, const std::string &msg) {
       return fail(ctx, report, PP_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));    \
   } while (0)
 
-// Wavefront layout tables: level offsets and the (i, j) of every position.
-void build_layout(int nx, int ny, std::vector<int> &off, std::vector<uint32_t> &cell) {
+// Wavefront layout tables: the position of cell (i, d - i) is base[d] + i, and the (i, j) of every
+// position. base has kLevPad entries of margin on each side (value 0: those levels hold no cell).
+void build_layout(int nx, int ny, std::vector<int> &base, std::vector<uint32_t> &cell) {
   const int nlev = nx + ny - 1;
-  off.assign(static_cast<size_t>(nlev) + 1, 0);
+  base.assign(static_cast<size_t>(nlev) + 2 * pp::kLevPad, 0);
   cell.clear();
   cell.reserve(static_cast<size_t>(nx) * ny);
   for (int d = 0; d < nlev; ++d) {
     const int lo = std::max(0, d - ny + 1), hi = std::min(d, nx - 1);
+    base[static_cast<size_t>(d + pp::kLevPad)] = static_cast<int>(cell.size()) - lo;
     for (int i = lo; i <= hi; ++i) cell.push_back((static_cast<uint32_t>(i) << 16) | static_cast<uint32_t>(d - i));
-    off[static_cast<size_t>(d) + 1] = static_cast<int>(cell.size());
   }
 }
 
@@ -130,14 +131,14 @@ int ensure_layout(pp_ctx *ctx, int nx, int ny, pp_ctx::Layout **out, pp_report *
   auto key = std::make_pair(nx, ny);
   auto it = ctx->layouts.find(key);
   if (it == ctx->layouts.end()) {
-    std::vector<int> off;
+    std::vector<int> base;
     std::vector<uint32_t> cell;
-    build_layout(nx, ny, off, cell);
+    build_layout(nx, ny, base, cell);
     pp_ctx::Layout L;
     L.nlev = nx + ny - 1;
-    PP_HIP_CHECK(hipMalloc(&L.off, off.size() * sizeof(int)));
+    PP_HIP_CHECK(hipMalloc(&L.base, base.size() * sizeof(int)));
     PP_HIP_CHECK(hipMalloc(&L.cell, cell.size() * sizeof(uint32_t)));
-    PP_HIP_CHECK(hipMemcpy(L.off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    PP_HIP_CHECK(hipMemcpy(L.base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
     PP_HIP_CHECK(hipMemcpy(L.cell, cell.data(), cell.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     it = ctx->layouts.emplace(key, L).first;
   }
@@ -200,7 +201,7 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
     auto t = ctx->tuning.find(std::make_pair(pb->nx, pb->ny));
     if (t != ctx->tuning.end()) a.ring = t->second;
   }
-  a.lev_off = L.off;
+  a.lev_base = L.base + pp::kLevPad;
   a.cell = L.cell;
   set_work(a, ctx->work, ctx->cap + pp::kPad);
   const size_t n = ctx->cap + pp::kPad;  // array stride
@@ -376,7 +377,7 @@ void pp_destroy(pp_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto &kv : ctx->layouts) {
-    (void)hipFree(kv.second.off);
+    (void)hipFree(kv.second.base);
     (void)hipFree(kv.second.cell);
   }
   if (ctx->work) (void)hipFree(ctx->work);

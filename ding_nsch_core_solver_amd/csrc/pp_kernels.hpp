@@ -26,7 +26,7 @@ struct SysArgs {
   int inner_max_iterations;
   double inner_tolerance;
   // layout tables
-  const int *lev_off;      // nlev + 1
+  const int *lev_base;     // level_base(d) for d in [-kLevPad, nlev + kLevPad) (0 outside the grid)
   const uint32_t *cell;    // n, (i << 16) | j of each wavefront position
   // fields (natural layout)
   const double *rho, *u, *v, *p_n, *p_nm1;

@@ -61,8 +61,10 @@ enum Reduction { kSequential = 0, kPairwise = 1, kCanonical = 2 };
 struct Reducer {
   int mode = kSequential;
   std::vector<int> perm;  // kCanonical: natural index of each wavefront position
+  int nx = 0, ny = 0;
 
-  Reducer(int m, int nx, int ny) : mode(m) {
+  Reducer(int m, int nx_, int ny_) : mode(m), nx(nx_), ny(ny_) {
+    const int nx = nx_, ny = ny_;
     if (mode == kCanonical) {
       perm.reserve(static_cast<std::size_t>(nx) * ny);
       for (int d = 0; d < nx + ny - 1; ++d)
@@ -94,6 +96,36 @@ struct Reducer {
       double s = 0.0;
       for (std::size_t q = t; q < n; q += T) s = s + f(static_cast<std::size_t>(perm[q]));
       part[t] = s;
+    }
+    double wsum[W];
+    for (std::size_t w = 0; w < W; ++w) {
+      double *v = part + 64 * w;
+      for (std::size_t off = 32; off >= 1; off >>= 1)
+        for (std::size_t l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+      wsum[w] = v[0];
+    }
+    for (std::size_t off = W / 2; off >= 1; off >>= 1)
+      for (std::size_t t = 0; t < off; ++t) wsum[t] = wsum[t] + wsum[t + off];
+    return wsum[0];
+  }
+
+  // The engine's level-major order (its SpMV pass, used for p^T A p): wave w of 8 takes levels
+  // d = w, w + 8, ...; within a level, cell i goes to lane (i - lo(d)) mod 64; each of the 512
+  // threads sums its cells in (level, row) order from 0.0; then the same lane and wave trees.
+  // Other modes: the same as sum().
+  template <class F>
+  double sum_levels(std::size_t n, F f) const {
+    if (mode != kCanonical) return sum(n, f);
+    constexpr std::size_t T = 512, W = 8;
+    double part[T];
+    for (std::size_t t = 0; t < T; ++t) part[t] = 0.0;
+    for (int d = 0; d < nx + ny - 1; ++d) {
+      const int lo = std::max(0, d - ny + 1), hi = std::min(d, nx - 1);
+      const std::size_t w = static_cast<std::size_t>(d) % W;
+      for (int i = lo; i <= hi; ++i) {
+        const std::size_t t = 64 * w + static_cast<std::size_t>(i - lo) % 64;
+        part[t] = part[t] + f(static_cast<std::size_t>(i) * static_cast<std::size_t>(ny) + static_cast<std::size_t>(d - i));
+      }
     }
     double wsum[W];
     for (std::size_t w = 0; w < W; ++w) {
@@ -518,7 +550,7 @@ Report pcg(const Csr &A, const Factor &F, const std::vector<double> &b, int max_
   const int limit = std::max(1, max_iterations);
   for (int it = 0; it < limit; ++it) {
     spmv(A, p.data(), ap.data());
-    const double pap = dot(p, ap);
+    const double pap = R.sum_levels(static_cast<std::size_t>(n), [&](std::size_t k) { return p[k] * ap[k]; });
     if (!std::isfinite(pap) || std::abs(pap) < 1.0e-30) throw std::runtime_error("pcg breakdown: p^T A p is invalid");
     const double alpha = rz / pap;
     double upd = 0.0, xsq = 0.0;
