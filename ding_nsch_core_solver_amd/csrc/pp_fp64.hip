@@ -461,6 +461,15 @@ __device__ __forceinline__ T staged_pivot(T d, T floor) {
     return d;
 }
 
+// s - c * x: separately rounded in FP64 (the reference's arithmetic), fused in FP32.
+template <typename T>
+__device__ __forceinline__ T sub_mul(T s, T c, T x) {
+  if constexpr (sizeof(T) == 8)
+    return s - c * x;
+  else
+    return __builtin_fmaf(-c, x, s);
+}
+
 template <typename T>
 __device__ __forceinline__ T staged_apply(T s, T d) {
   if constexpr (sizeof(T) == 8)
@@ -590,9 +599,9 @@ __device__ __forceinline__ void staged_forward(const SysArgs &a, const SweepArra
         const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
         s = (wrap_row && act) ? s - static_cast<T>(sh.lx[jc]) * wx : s;
       }
-      s = s - c.f[k][1] * west;
+      s = sub_mul(s, c.f[k][1], west);
       if (kPeriodic && py) s = j == ny - 1 ? s - ly * first : s;
-      s = j > 0 ? s - c.f[k][2] * prev : s;
+      s = j > 0 ? sub_mul(s, c.f[k][2], prev) : s;
       T cur, out;
       if (kIldlt) {
         cur = s;
@@ -648,9 +657,9 @@ __device__ __forceinline__ void staged_backward(const SysArgs &a, const SweepArr
       // east neighbour: lane l+1 of this wave, lane 63 from wave w+1 (0 for the last wave)
       const T east = write_lane(wave_shl1(prev), lane_value(hv, k), 63);
       T s = c.f[k][0];
-      s = j + 1 < ny ? s - c.f[k][2] * prev : s;
+      s = j + 1 < ny ? sub_mul(s, c.f[k][2], prev) : s;
       if (kPeriodic && py) s = j == 0 ? s - ly * first : s;
-      s = has_e ? s - c.f[k][3] * east : s;
+      s = has_e ? sub_mul(s, c.f[k][3], east) : s;
       if (kPeriodic && px) {
         const int jc = min(max(j, 0), ny - 1);
         const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
@@ -1223,10 +1232,11 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
   const SweepArrays<float> fwd{a.fr, a.fLw, a.fLs, a.fRd, a.fy, a.LY};
   const SweepArrays<float> bwd{a.fy, a.fLw, a.fLs, a.fRd, a.fz, a.LY};
   const double tol = a.tolerance;
-  // default inner reduction 0.5 sqrt(tol) (at least 1e-5, about what FP32 PCG attains): two cycles
-  // reach tol; each cycle asks no more than the remaining reduction 0.5 tol |b| / |r|
+  // default inner reduction 3 sqrt(tol), within [1e-5, 1e-2] (measured best at tol 1e-8: about three
+  // cycles, fewest inner iterations); each cycle asks no more than the remaining 0.5 tol |b| / |r|
+  const double inner_guess = 3.0 * sqrt(a.tolerance);
   const double inner_tol = a.inner_tolerance > 0.0 ? a.inner_tolerance
-                                                    : (0.5 * sqrt(a.tolerance) > 1.0e-5 ? 0.5 * sqrt(a.tolerance) : 1.0e-5);
+                                                    : (inner_guess < 1.0e-5 ? 1.0e-5 : (inner_guess > 1.0e-2 ? 1.0e-2 : inner_guess));
   const int limit = a.max_iterations < 1 ? 1 : a.max_iterations;
   const int inner_cap = a.inner_max_iterations > 0 ? a.inner_max_iterations : limit;
   clk.mark(0);
