@@ -478,6 +478,33 @@ __device__ __forceinline__ T staged_apply(T s, T d) {
     return s * d;
 }
 
+// Walks the units in key order and returns the next one owned by loader `me` of L.
+struct UnitCursor {
+  int tau, w, k_unit;
+};
+
+template <bool kBackward>
+__device__ __forceinline__ bool next_unit(UnitCursor &c, int S, int Tn, int U, int L, int me, int &w_out, int &t0_out) {
+  const int tau_end = 64 * (S - 1) + Tn;
+  while (c.tau < tau_end) {
+    const int w = c.w;
+    const int t0 = c.tau - 64 * (kBackward ? (S - 1 - w) : w);
+    if (++c.w == S) {
+      c.w = 0;
+      c.tau += U;
+    }
+    if (t0 < 0 || t0 >= Tn) continue;
+    if ((c.k_unit++) % L != me) continue;
+    w_out = w;
+    t0_out = t0;
+    return true;
+  }
+  return false;
+}
+
+// Loaders are software-pipelined over two register buffers: the global loads of a loader's next
+// unit are in flight while it waits for ring space and writes the current one to LDS. Only the LDS
+// writes are gated by ring space, in the same order as before, so the deadlock argument is unchanged.
 template <typename T, bool kBackward>
 __device__ __forceinline__ void stage_loader(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, int S,
                                              T m_r, T floor) {
@@ -491,46 +518,64 @@ __device__ __forceinline__ void stage_loader(const SysArgs &a, const SweepArrays
   const global_ptr<const T> Lw = as_global<const T>(io.Lw);
   const global_ptr<const T> Ls = as_global<const T>(io.Ls);
   const __attribute__((address_space(4))) int *lb = (const __attribute__((address_space(4))) int *)a.lev_base;
-  int k_unit = 0;
-  for (int tau = 0; tau < 64 * (S - 1) + Tn; tau += U) {
-    for (int w = 0; w < S; ++w) {
-      const int t0 = tau - 64 * (kBackward ? (S - 1 - w) : w);
-      if (t0 < 0 || t0 >= Tn) continue;
-      if ((k_unit++) % L != me) continue;
-      wait_at_least(&sh.used[w], t0 + U - R, sh);
-      const int i = w * 64 + lane;
-      T v[kStageUnit][4];
+  auto load = [&](int w, int t0, T (&v)[kStageUnit][4]) {
+    const int i = w * 64 + lane;
 #pragma unroll
-      for (int k = 0; k < kStageUnit; ++k) {
-        if (k < U) {
-          const int d = min(max(step_level<kBackward>(w, t0 + k, Tn), 0), d_last);
-          const int q = lb[d] + i;
-          if (kBackward) {
-            const int qn = lb[min(d + 1, d_last)] + i;  // (i, j+1); (i+1, j) follows it
-            v[k][0] = f0[q];
-            v[k][1] = staged_pivot(dg[q], floor);
-            v[k][2] = Ls[qn];
-            v[k][3] = Lw[qn + 1];
-          } else {
-            v[k][0] = f0[q] - m_r;
-            v[k][1] = Lw[q];
-            v[k][2] = Ls[q];
-            v[k][3] = staged_pivot(dg[q], floor);
-          }
+    for (int k = 0; k < kStageUnit; ++k) {
+      if (k < U) {
+        const int d = min(max(step_level<kBackward>(w, t0 + k, Tn), 0), d_last);
+        const int q = lb[d] + i;
+        if (kBackward) {
+          const int qn = lb[min(d + 1, d_last)] + i;  // (i, j+1); (i+1, j) follows it
+          v[k][0] = f0[q];
+          v[k][1] = dg[q];
+          v[k][2] = Ls[qn];
+          v[k][3] = Lw[qn + 1];
+        } else {
+          v[k][0] = f0[q];
+          v[k][1] = Lw[q];
+          v[k][2] = Ls[q];
+          v[k][3] = dg[q];
         }
-      }
-#pragma unroll
-      for (int k = 0; k < kStageUnit; ++k) {
-        if (k < U) {
-          volatile __attribute__((address_space(3))) T *slot = stage_slot<T>(sh, w, R, (t0 + k) & (R - 1), lane);
-#pragma unroll
-          for (int f = 0; f < 4; ++f) slot[f * 64] = v[k][f];
-        }
-      }
-      if (lane == 0) {
-        for (int k = 0; k < U; ++k) *lds(&sh.ready[w * kMaxRing + ((t0 + k) & (R - 1))]) = t0 + k + 1;
       }
     }
+  };
+  auto store = [&](int w, int t0, const T (&v)[kStageUnit][4]) {
+    wait_at_least(&sh.used[w], t0 + U - R, sh);
+#pragma unroll
+    for (int k = 0; k < kStageUnit; ++k) {
+      if (k < U) {
+        volatile __attribute__((address_space(3))) T *slot = stage_slot<T>(sh, w, R, (t0 + k) & (R - 1), lane);
+        if (kBackward) {
+          slot[0] = v[k][0];
+          slot[64] = staged_pivot(v[k][1], floor);
+          slot[128] = v[k][2];
+          slot[192] = v[k][3];
+        } else {
+          slot[0] = v[k][0] - m_r;
+          slot[64] = v[k][1];
+          slot[128] = v[k][2];
+          slot[192] = staged_pivot(v[k][3], floor);
+        }
+      }
+    }
+    if (lane == 0) {
+      for (int k = 0; k < U; ++k) *lds(&sh.ready[w * kMaxRing + ((t0 + k) & (R - 1))]) = t0 + k + 1;
+    }
+  };
+  UnitCursor cur{0, 0, 0};
+  int wA = 0, tA = 0, wB = 0, tB = 0;
+  T bufA[kStageUnit][4], bufB[kStageUnit][4];
+  bool hasA = next_unit<kBackward>(cur, S, Tn, U, L, me, wA, tA);
+  if (hasA) load(wA, tA, bufA);
+  while (hasA) {
+    const bool hasB = next_unit<kBackward>(cur, S, Tn, U, L, me, wB, tB);
+    if (hasB) load(wB, tB, bufB);
+    store(wA, tA, bufA);
+    if (!hasB) break;
+    hasA = next_unit<kBackward>(cur, S, Tn, U, L, me, wA, tA);
+    if (hasA) load(wA, tA, bufA);
+    store(wB, tB, bufB);
   }
 }
 
