@@ -29,7 +29,9 @@ from ding_nsch_core_solver_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "fp64_solve_192x192_dr1000_traffic.txt")
-BATCH_TRAFFIC_FILE = os.path.join(REPO, "profiles", "batched_256_uneven_traffic.txt")
+MIXED_TRAFFIC_FILE = os.path.join(REPO, "profiles", "mixed_solve_192x192_dr1000_traffic.txt")
+BATCH_TRAFFIC_FILE = os.path.join(REPO, "profiles", "batched_256_uneven_mixed_traffic.txt")
+BATCH64_TRAFFIC_FILE = os.path.join(REPO, "profiles", "batched_256_uneven_fp64_traffic.txt")
 
 
 def log(msg):
@@ -103,9 +105,16 @@ def cpu_baseline(case, seconds):
     }, result
 
 
+def mixed_bytes(n, iterations, cycles):
+    """Algorithmic bytes of one mixed-precision solve: FP64 setup, FP32 operator/factor copies, FP32
+    inner iterations, FP64 outer residual and correction once per cycle (and the final check)."""
+    return setup_bytes(n) + 72 * n + iterations * iteration_bytes(n, fp32_inner=True) + (cycles + 1) * 56 * n
+
+
 def batched_leg(eng, torch, dev, repeats):
-    """The 256-system uneven batch (grids from 64x64 to 256x256, density ratios 1..1000, FP64
-    icpcg to 1e-8), inputs resident in HBM; one pp_solve_batched_device call per repeat."""
+    """The 256-system uneven batch (grids from 64x64 to 256x256, density ratios 1..1000, icpcg to
+    1e-8 with FP32 inner PCG inside FP64 refinement), inputs resident in HBM; one
+    pp_solve_batched_device call per repeat. The same batch in plain FP64 is timed beside it."""
     cases = W.batched_cases(256, seed=0)
     tens = []
     for c in cases:
@@ -115,28 +124,43 @@ def batched_leg(eng, torch, dev, repeats):
     ptr = lambda k: [t[k].data_ptr() for t in tens]  # noqa: E731
     stream = torch.cuda.current_stream(dev).cuda_stream
 
-    def run():
-        return eng.solve_batched_device(cases, ptr("rho"), ptr("u"), ptr("v"), ptr("out"), scheme=E.PP_ICPCG,
-                                        stream=stream)
+    def timed(precision):
+        def run():
+            return eng.solve_batched_device(cases, ptr("rho"), ptr("u"), ptr("v"), ptr("out"), scheme=E.PP_ICPCG,
+                                            precision=precision, stream=stream)
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kernel_ms = []
+        for _ in range(repeats):
+            reps = run()
+            kernel_ms.append(reps[0]["kernel_ms"])
+        torch.cuda.synchronize()
+        return reps, (time.perf_counter() - t0) / repeats, float(np.mean(kernel_ms))
 
-    run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    kernel_ms = []
-    for _ in range(repeats):
-        reps = run()
-        kernel_ms.append(reps[0]["kernel_ms"])
-    torch.cuda.synchronize()
-    elapsed = (time.perf_counter() - t0) / repeats
-    kms = float(np.mean(kernel_ms))
     cells = sum(c.n for c in cases)
-    alg = sum(setup_bytes(c.n) + r["iterations"] * iteration_bytes(c.n) for c, r in zip(cases, reps))
+    reps64, elapsed64, kms64 = timed(E.PP_FP64)
+    p64 = [t["out"].cpu().numpy() for t in tens]
+    alg64 = sum(setup_bytes(c.n) + r["iterations"] * iteration_bytes(c.n) for c, r in zip(cases, reps64))
+    reps, elapsed, kms = timed(E.PP_MIXED)
+    diff = max(float(np.abs(t["out"].cpu().numpy() - q).max() / np.abs(q).max()) for t, q in zip(tens, p64))
+    alg = sum(mixed_bytes(c.n, r["iterations"], r["outer_cycles"]) for c, r in zip(cases, reps))
     achieved = alg / (kms * 1e-3) / 1e9
-    out = {"systems": len(cases), "cells": cells, "solves_per_s": len(cases) / elapsed, "ms_per_batch": elapsed * 1e3,
-           "kernel_ms": kms, "iterations_total": int(sum(r["iterations"] for r in reps)),
+    out = {"systems": len(cases), "cells": cells, "precision": "fp32 inner / fp64 refinement",
+           "solves_per_s": len(cases) / elapsed, "ms_per_batch": elapsed * 1e3, "kernel_ms": kms,
+           "inner_iterations_total": int(sum(r["iterations"] for r in reps)),
+           "outer_cycles_total": int(sum(r["outer_cycles"] for r in reps)),
+           "max_relative_residual": float(max(r["relative_residual"] for r in reps)),
+           "max_abs_diff_vs_fp64_batch_over_max_abs": diff,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(BATCH_TRAFFIC_FILE),
-                        "algorithmic_bytes_per_launch": alg}}
+                        "algorithmic_bytes_per_launch": alg},
+           "fp64": {"solves_per_s": len(cases) / elapsed64, "ms_per_batch": elapsed64 * 1e3, "kernel_ms": kms64,
+                    "iterations_total": int(sum(r["iterations"] for r in reps64)),
+                    "roofline": {"bound": "hbm", "achieved": alg64 / (kms64 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": alg64 / (kms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                 "traffic": read_traffic(BATCH64_TRAFFIC_FILE),
+                                 "algorithmic_bytes_per_launch": alg64}}}
     return out, cases
 
 
@@ -234,6 +258,30 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
 
+    # mixed precision on the same workload (FP32 inner PCG inside FP64 refinement, same tolerance)
+    log(f"[bench] timing {args.steps} mixed-precision steps")
+    for _ in range(args.warmup):
+        step(E.PP_MIXED)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mreps = [step(E.PP_MIXED) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    m_elapsed = time.perf_counter() - t0
+    p_mixed = d_out.cpu().numpy()
+    m_kernel = float(np.mean([r["kernel_ms"] for r in mreps]))
+    m_it, m_cyc = mreps[-1]["iterations"], mreps[-1]["outer_cycles"]
+    m_bytes = mixed_bytes(n, m_it, m_cyc)
+    line["mixed"] = {"value": args.steps / m_elapsed, "unit": "solves/s", "ms_per_step": m_elapsed / args.steps * 1e3,
+                     "kernel_ms": m_kernel, "inner_iterations": m_it, "outer_cycles": m_cyc,
+                     "relative_residual": mreps[-1]["relative_residual"],
+                     "speedup_vs_fp64": (args.steps / m_elapsed) / line["value"],
+                     "max_abs_diff_vs_fp64_over_max_abs": float(np.abs(p_mixed - p_gpu).max() / np.abs(p_gpu).max()),
+                     "roofline": {"bound": "hbm", "achieved": m_bytes / (m_kernel * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": m_bytes / (m_kernel * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "traffic": read_traffic(MIXED_TRAFFIC_FILE), "algorithmic_bytes_per_launch": m_bytes}}
+    log(f"[bench] mixed: {line['mixed']['value']:.2f} solves/s, kernel {m_kernel:.2f} ms, {m_it} inner iterations in "
+        f"{m_cyc} cycles")
+
     batch_cases = None
     if not args.no_batched:
         log("[bench] 256-system batch")
@@ -249,6 +297,7 @@ def main():
             p_ref, it_ref, res_ref = ref_result
             line["parity"] = {"iterations_gpu": iters, "iterations_cpu": it_ref,
                               "max_abs_diff_over_max_abs": float(np.abs(p_gpu - p_ref).max() / np.abs(p_ref).max())}
+            line["mixed"]["max_abs_diff_vs_cpu_over_max_abs"] = float(np.abs(p_mixed - p_ref).max() / np.abs(p_ref).max())
             line["speedup_vs_cpu"] = line["value"] / base["value"]
         if batch_cases is not None:
             threads = min(16, len(os.sched_getaffinity(0)))

@@ -4,6 +4,9 @@ pp_adjoint_device (one adjoint PCG solve plus the face-gradient kernel).
     p = pressure_solve(k_u, k_v, src, case)      # float64 CUDA tensors, p of shape (nx*ny,)
     loss = 0.5 * ((p - p_target) ** 2).sum(); loss.backward()
 
+precision="mixed" runs both the forward and the adjoint solve with FP32 inner PCG inside FP64
+iterative refinement (same tolerance, FP64 results).
+
 k_u ((nx+1)*ny) and k_v (nx*(ny+1)) are face inverse densities 1/rho_f, src (nx*ny) the source
 term (the reference's -div(u*)/dt); geometry, boundary conditions and solver settings come from
 `case` (a PressureCase; its rho/u/v are not used). Gradients flow to all three tensors;
@@ -35,7 +38,7 @@ def _checked(t: torch.Tensor, numel: int, name: str) -> torch.Tensor:
 
 class PressureSolve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, k_u, k_v, src, case, scheme):
+    def forward(ctx, k_u, k_v, src, case, scheme, precision):
         nx, ny = case.nx, case.ny
         k_u = _checked(k_u, (nx + 1) * ny, "k_u")
         k_v = _checked(k_v, nx * (ny + 1), "k_v")
@@ -44,8 +47,8 @@ class PressureSolve(torch.autograd.Function):
         p = torch.empty(nx * ny, dtype=torch.float64, device=k_u.device)
         stream = torch.cuda.current_stream(k_u.device).cuda_stream
         ctx.report = eng.solve_coeffs_device(case, k_u.data_ptr(), k_v.data_ptr(), src.data_ptr(), p.data_ptr(),
-                                             scheme=scheme, stream=stream)
-        ctx.case, ctx.scheme = case, scheme
+                                             scheme=scheme, stream=stream, precision=precision)
+        ctx.case, ctx.scheme, ctx.precision = case, scheme, precision
         ctx.save_for_backward(k_u, k_v, p)
         return p
 
@@ -60,13 +63,17 @@ class PressureSolve(torch.autograd.Function):
         stream = torch.cuda.current_stream(p.device).cuda_stream
         _engine(p.device).adjoint_device(case, k_u.data_ptr(), k_v.data_ptr(), p.data_ptr(), g.data_ptr(),
                                          lam.data_ptr(), dku.data_ptr(), dkv.data_ptr(), scheme=ctx.scheme,
-                                         stream=stream)
-        return dku, dkv, lam, None, None
+                                         stream=stream, precision=ctx.precision)
+        return dku, dkv, lam, None, None, None
 
 
-def pressure_solve(k_u, k_v, src, case, scheme: str | int = "hip_icpcg"):
+_PRECISIONS = {"fp64": E.PP_FP64, "mixed": E.PP_MIXED}
+
+
+def pressure_solve(k_u, k_v, src, case, scheme: str | int = "hip_icpcg", precision: str | int = "fp64"):
     code = E.SCHEME_CODES[scheme] if isinstance(scheme, str) else int(scheme)
-    return PressureSolve.apply(k_u, k_v, src, case, code)
+    prec = _PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    return PressureSolve.apply(k_u, k_v, src, case, code, prec)
 
 
 def face_inverse_density(rho: torch.Tensor, case) -> tuple[torch.Tensor, torch.Tensor]:

@@ -116,7 +116,7 @@ __device__ __forceinline__ void block_sum(double (&v)[K], SweepShared &sh, int &
   }
   __syncthreads();
   // wave tree with offsets 8, 4, 2, 1; the first level is folded into the loads
-#pragma unroll 1
+#pragma unroll
   for (int k = 0; k < K; ++k) {
     const double *src = slot + k * kWaves;
     double w[kWaves / 2];

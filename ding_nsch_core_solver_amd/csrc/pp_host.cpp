@@ -45,8 +45,10 @@ struct pp_ctx {
 
   // batched solves: per-system work slab, wrap lines, arguments, reports, order and counters
   size_t batch_cells = 0;       // capacity of batch_work in cells (sum of n + kPad)
+  size_t batch_fcells = 0;      // capacity of batch_fwork in cells (mixed-precision systems only)
   int batch_cap = 0;            // capacity in systems
   double *batch_work = nullptr;
+  float *batch_fwork = nullptr;
   double *batch_wrap = nullptr;
   pp::SysArgs *d_batch_args = nullptr;
   pp::DevReport *d_batch_rep = nullptr;
@@ -172,6 +174,12 @@ void set_work(pp::SysArgs &a, double *w, size_t stride) {
   for (int k = 0; k < kF64Arrays; ++k) *f64[k] = w + static_cast<size_t>(k) * stride;
 }
 
+// FP32 work arrays of one mixed-precision system: kF32Arrays arrays of `stride` floats at w.
+void set_fwork(pp::SysArgs &a, float *w, size_t stride) {
+  float **f32[kF32Arrays] = {&a.fP, &a.fW, &a.fS, &a.fLw, &a.fLs, &a.fRd, &a.fx, &a.fr, &a.fpa, &a.fpb, &a.fz, &a.fy};
+  for (int k = 0; k < kF32Arrays; ++k) *f32[k] = w + static_cast<size_t>(k) * stride;
+}
+
 void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::SysArgs &a) {
   std::memset(&a, 0, sizeof(a));
   a.nx = pb->nx;
@@ -204,10 +212,7 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
   a.lev_base = L.base + pp::kLevPad;
   a.cell = L.cell;
   set_work(a, ctx->work, ctx->cap + pp::kPad);
-  const size_t n = ctx->cap + pp::kPad;  // array stride
-  float *fw = ctx->fwork;
-  float **f32[kF32Arrays] = {&a.fP, &a.fW, &a.fS, &a.fLw, &a.fLs, &a.fRd, &a.fx, &a.fr, &a.fpa, &a.fpb, &a.fz, &a.fy};
-  for (int k = 0; k < kF32Arrays; ++k) *f32[k] = fw + static_cast<size_t>(k) * n;
+  set_fwork(a, ctx->fwork, ctx->cap + pp::kPad);
   a.LX = ctx->wrap64;
   a.LY = ctx->wrap64 + pp::kMaxSide;
   a.fLX = ctx->wrap32;
@@ -259,30 +264,40 @@ int run_solve(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *repo
 
 void free_batch(pp_ctx *ctx) {
   if (ctx->batch_work) (void)hipFree(ctx->batch_work);
+  if (ctx->batch_fwork) (void)hipFree(ctx->batch_fwork);
   if (ctx->batch_wrap) (void)hipFree(ctx->batch_wrap);
   if (ctx->d_batch_args) (void)hipFree(ctx->d_batch_args);
   if (ctx->d_batch_rep) (void)hipFree(ctx->d_batch_rep);
   if (ctx->h_batch_rep) (void)hipHostFree(ctx->h_batch_rep);
   ctx->batch_work = ctx->batch_wrap = nullptr;
+  ctx->batch_fwork = nullptr;
   ctx->d_batch_args = nullptr;
   ctx->d_batch_rep = ctx->h_batch_rep = nullptr;
-  ctx->batch_cells = 0;
+  ctx->batch_cells = ctx->batch_fcells = 0;
   ctx->batch_cap = 0;
 }
 
-int ensure_batch(pp_ctx *ctx, int count, size_t cells, pp_report *report) {
-  if (count <= ctx->batch_cap && cells <= ctx->batch_cells) return PP_OK;
+// Batch buffers for `count` systems with `cells` FP64 work cells in total, `fcells` of them also
+// needing FP32 work arrays (mixed precision).
+int ensure_batch(pp_ctx *ctx, int count, size_t cells, size_t fcells, pp_report *report) {
+  if (count <= ctx->batch_cap && cells <= ctx->batch_cells && fcells <= ctx->batch_fcells) return PP_OK;
   const int cap = std::max(count, ctx->batch_cap);
   const size_t cap_cells = std::max(cells, ctx->batch_cells);
+  const size_t cap_fcells = std::max(fcells, ctx->batch_fcells);
   free_batch(ctx);
   PP_HIP_CHECK(hipMalloc(&ctx->batch_work, cap_cells * kF64Arrays * sizeof(double)));
   PP_HIP_CHECK(hipMemset(ctx->batch_work, 0, cap_cells * kF64Arrays * sizeof(double)));  // finite padding
+  if (cap_fcells > 0) {
+    PP_HIP_CHECK(hipMalloc(&ctx->batch_fwork, cap_fcells * kF32Arrays * sizeof(float)));
+    PP_HIP_CHECK(hipMemset(ctx->batch_fwork, 0, cap_fcells * kF32Arrays * sizeof(float)));
+  }
   PP_HIP_CHECK(hipMalloc(&ctx->batch_wrap, static_cast<size_t>(cap) * 2 * pp::kMaxSide * sizeof(double)));
   PP_HIP_CHECK(hipMalloc(&ctx->d_batch_args, static_cast<size_t>(cap) * sizeof(pp::SysArgs)));
   PP_HIP_CHECK(hipMalloc(&ctx->d_batch_rep, static_cast<size_t>(cap) * sizeof(pp::DevReport)));
   PP_HIP_CHECK(hipHostMalloc(&ctx->h_batch_rep, static_cast<size_t>(cap) * sizeof(pp::DevReport), hipHostMallocDefault));
   ctx->batch_cap = cap;
   ctx->batch_cells = cap_cells;
+  ctx->batch_fcells = cap_fcells;
   return PP_OK;
 }
 
@@ -550,22 +565,23 @@ int pp_solve_batched_device(pp_ctx *ctx, int count, const pp_problem *problems, 
     pp_report *rk = reports ? reports + k : nullptr;
     const pp_problem *pb = problems + k;
     if (int rc = validate(ctx, pb, rk)) return rc;
-    if (pb->precision != PP_FP64) return fail(ctx, rk, PP_ERR_ARG, "batched solves run in FP64");
     const bool split = pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG;
     if ((!d_rho[k] && !split) || !d_u[k] || !d_v[k] || !d_p_out[k]) return fail(ctx, rk, PP_ERR_ARG, "null field pointer");
   }
   PP_HIP_CHECK(hipSetDevice(ctx->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  size_t cells = 0;
+  size_t cells = 0, fcells = 0;
   std::vector<pp_ctx::Layout *> layouts(static_cast<size_t>(count));
   for (int k = 0; k < count; ++k) {
     if (int rc = ensure_layout(ctx, problems[k].nx, problems[k].ny, &layouts[static_cast<size_t>(k)], report)) return rc;
-    cells += static_cast<size_t>(problems[k].nx) * problems[k].ny + pp::kPad;
+    const size_t stride = static_cast<size_t>(problems[k].nx) * problems[k].ny + pp::kPad;
+    cells += stride;
+    if (problems[k].precision == PP_MIXED) fcells += stride;
   }
-  if (int rc = ensure_batch(ctx, count, cells, report)) return rc;
+  if (int rc = ensure_batch(ctx, count, cells, fcells, report)) return rc;
 
   std::vector<pp::SysArgs> args(static_cast<size_t>(count));
-  size_t at = 0;
+  size_t at = 0, fat = 0;
   for (int k = 0; k < count; ++k) {
     const pp_problem *pb = problems + k;
     pp::SysArgs &a = args[static_cast<size_t>(k)];
@@ -575,6 +591,10 @@ int pp_solve_batched_device(pp_ctx *ctx, int count, const pp_problem *problems, 
     at += stride;
     a.fP = a.fW = a.fS = a.fLw = a.fLs = a.fRd = nullptr;
     a.fx = a.fr = a.fpa = a.fpb = a.fz = a.fy = nullptr;
+    if (pb->precision == PP_MIXED) {
+      set_fwork(a, ctx->batch_fwork + fat * kF32Arrays, stride);
+      fat += stride;
+    }
     a.fLX = a.fLY = nullptr;
     a.LX = ctx->batch_wrap + static_cast<size_t>(k) * 2 * pp::kMaxSide;
     a.LY = a.LX + pp::kMaxSide;
@@ -586,29 +606,40 @@ int pp_solve_batched_device(pp_ctx *ctx, int count, const pp_problem *problems, 
     a.p_nm1 = d_p_nm1 ? d_p_nm1[k] : nullptr;
     a.p_out = d_p_out[k];
   }
-  // ICC systems then ILDLT systems, each largest first; the device copy of the arguments is in
-  // that order, reports stay indexed by system
+  // Systems grouped into kernel families (FP64 ICC, FP64 ILDLT, mixed ICC, mixed ILDLT), each
+  // family largest first; the device copy of the arguments is in that order, reports stay indexed
+  // by system. One launch per non-empty family, one workgroup per system.
+  auto family = [&](int k) {
+    return (problems[k].precision == PP_MIXED ? 2 : 0) + (args[static_cast<size_t>(k)].ildlt ? 1 : 0);
+  };
   std::vector<int> order(static_cast<size_t>(count));
   for (int k = 0; k < count; ++k) order[static_cast<size_t>(k)] = k;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    const int fx = args[static_cast<size_t>(x)].ildlt, fy = args[static_cast<size_t>(y)].ildlt;
+    const int fx = family(x), fy = family(y);
     if (fx != fy) return fx < fy;
     return batch_cost(problems[x]) > batch_cost(problems[y]);
   });
   std::vector<pp::SysArgs> sorted(static_cast<size_t>(count));
-  int n_icc = 0;
+  int family_count[4] = {0, 0, 0, 0};
   for (int k = 0; k < count; ++k) {
     sorted[static_cast<size_t>(k)] = args[static_cast<size_t>(order[static_cast<size_t>(k)])];
-    n_icc += sorted[static_cast<size_t>(k)].ildlt ? 0 : 1;
+    ++family_count[family(order[static_cast<size_t>(k)])];
   }
-  const int n_ildlt = count - n_icc;
 
   PP_HIP_CHECK(hipMemcpyAsync(ctx->d_batch_args, sorted.data(), sorted.size() * sizeof(pp::SysArgs),
                               hipMemcpyHostToDevice, s));
   PP_HIP_CHECK(hipMemsetAsync(ctx->d_batch_rep, 0, static_cast<size_t>(count) * sizeof(pp::DevReport), s));
   PP_HIP_CHECK(hipEventRecord(ctx->ev_start, s));
-  if (n_icc > 0) PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_batch_args, n_icc, false, s));
-  if (n_ildlt > 0) PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_batch_args + n_icc, n_ildlt, true, s));
+  int first_sys = 0;
+  for (int f = 0; f < 4; ++f) {
+    if (family_count[f] == 0) continue;
+    const pp::SysArgs *d_first = ctx->d_batch_args + first_sys;
+    if (f >= 2)
+      PP_HIP_CHECK(pp::launch_mixed_solve(d_first, family_count[f], (f & 1) != 0, s));
+    else
+      PP_HIP_CHECK(pp::launch_fp64_solve(d_first, family_count[f], (f & 1) != 0, s));
+    first_sys += family_count[f];
+  }
   PP_HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
   PP_HIP_CHECK(hipMemcpyAsync(ctx->h_batch_rep, ctx->d_batch_rep, static_cast<size_t>(count) * sizeof(pp::DevReport),
                               hipMemcpyDeviceToHost, s));

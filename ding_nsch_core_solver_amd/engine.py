@@ -253,10 +253,10 @@ class Engine:
             raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
         return _report_dict(rep)
 
-    def solve_coeffs(self, case, ku, kv, src, scheme: int = PP_ICPCG, **kw):
+    def solve_coeffs(self, case, ku, kv, src, scheme: int = PP_ICPCG, precision: int = PP_FP64, **kw):
         """Coefficient-level solve on host arrays (k_u (nx+1)*ny, k_v nx*(ny+1), src nx*ny).
         Returns (p (nx*ny,), report dict)."""
-        pb = make_problem(case, scheme, PP_FP64, **kw)
+        pb = make_problem(case, scheme, precision, **kw)
         ku, kv, src = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (ku, kv, src))
         assert ku.size == (case.nx + 1) * case.ny and kv.size == case.nx * (case.ny + 1) and src.size == case.n
         out = np.zeros(case.n)
@@ -264,9 +264,9 @@ class Engine:
         rc = self._lib.pp_solve_coeffs(self.h, ctypes.byref(pb), _dp(ku), _dp(kv), _dp(src), _dp(out), ctypes.byref(rep))
         return out, self._check(rc, rep)
 
-    def adjoint(self, case, ku, kv, p, dLdp, scheme: int = PP_ICPCG, **kw):
+    def adjoint(self, case, ku, kv, p, dLdp, scheme: int = PP_ICPCG, precision: int = PP_FP64, **kw):
         """Adjoint of solve_coeffs on host arrays. Returns (dL/dsrc, dL/dk_u, dL/dk_v, report dict)."""
-        pb = make_problem(case, scheme, PP_FP64, **kw)
+        pb = make_problem(case, scheme, precision, **kw)
         ku, kv, p, dLdp = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (ku, kv, p, dLdp))
         lam = np.zeros(case.n)
         dku = np.zeros((case.nx + 1) * case.ny)
@@ -276,28 +276,31 @@ class Engine:
                                   _dp(dkv), ctypes.byref(rep))
         return lam, dku, dkv, self._check(rc, rep)
 
-    def solve_coeffs_device(self, case, d_ku, d_kv, d_src, d_out, scheme: int = PP_ICPCG, stream=None, **kw):
-        pb = make_problem(case, scheme, PP_FP64, **kw)
+    def solve_coeffs_device(self, case, d_ku, d_kv, d_src, d_out, scheme: int = PP_ICPCG, stream=None,
+                            precision: int = PP_FP64, **kw):
+        pb = make_problem(case, scheme, precision, **kw)
         rep = PPReport()
         rc = self._lib.pp_solve_coeffs_device(self.h, ctypes.byref(pb), d_ku, d_kv, d_src, d_out, ctypes.byref(rep),
                                               stream)
         return self._check(rc, rep)
 
     def adjoint_device(self, case, d_ku, d_kv, d_p, d_g, d_lam, d_dku, d_dkv, scheme: int = PP_ICPCG, stream=None,
-                       **kw):
-        pb = make_problem(case, scheme, PP_FP64, **kw)
+                       precision: int = PP_FP64, **kw):
+        pb = make_problem(case, scheme, precision, **kw)
         rep = PPReport()
         rc = self._lib.pp_adjoint_device(self.h, ctypes.byref(pb), d_ku, d_kv, d_p, d_g, d_lam, d_dku, d_dkv,
                                          ctypes.byref(rep), stream)
         return self._check(rc, rep)
 
     def solve_batched_device(self, cases, d_rho, d_u, d_v, d_out, d_pn=None, d_pnm1=None, scheme: int = PP_ICPCG,
-                             stream=None, **kw):
-        """Batched FP64 solve; d_* are lists of device pointers (ints). Returns the report dicts.
-        Raises EngineError for the first failing system (reports still describe every system)."""
+                             stream=None, precision=PP_FP64, **kw):
+        """Batched solve; d_* are lists of device pointers (ints); scheme and precision are one value
+        or one per system. Returns the report dicts. Raises EngineError for the first failing system
+        (reports still describe every system)."""
         count = len(cases)
         schemes = scheme if isinstance(scheme, (list, tuple)) else [scheme] * count
-        pbs = (PPProblem * count)(*[make_problem(c, s, PP_FP64, **kw) for c, s in zip(cases, schemes)])
+        precs = precision if isinstance(precision, (list, tuple)) else [precision] * count
+        pbs = (PPProblem * count)(*[make_problem(c, s, pr, **kw) for c, s, pr in zip(cases, schemes, precs)])
         P = ctypes.c_void_p * count
         ptrs = lambda xs: None if xs is None else P(*[x if x else None for x in xs])  # noqa: E731
         reps = (PPReport * count)()

@@ -155,9 +155,11 @@ int pp_adjoint_device(pp_ctx *ctx, const pp_problem *pb, const double *d_k_u, co
 
 /*
  * Batched solve (no counterpart in the reference, whose solver handles one system per call):
- * `count` independent pressure corrections of any sizes, one launch per factor kind (ICC schemes,
- * ILDLT schemes) with one workgroup per system, dispatched largest first so that CUs freed by small
- * systems pick up the remaining ones. Device arrays are contiguous as for pp_solve_device; the
+ * `count` independent pressure corrections of any sizes, schemes and precisions (problems[k].
+ * precision: PP_FP64 or PP_MIXED), one launch per kernel family (FP64 ICC, FP64 ILDLT, mixed ICC,
+ * mixed ILDLT) with one workgroup per system, dispatched largest first so that CUs freed by small
+ * systems pick up the remaining ones. Each system's result is the one pp_solve_device gives for
+ * it. Device arrays are contiguous as for pp_solve_device; the
  * pointer arrays are host arrays of `count` device pointers (d_p_n / d_p_nm1 may be NULL, or hold
  * NULL entries). reports[k] gets system k's report; kernel_ms is the duration of the whole batch.
  * Returns PP_OK, or the status of

@@ -145,3 +145,76 @@ def test_autograd_chain_to_density():
             rm[c] -= h
             fd = (float(loss_of(rp)) - float(loss_of(rm))) / (2 * h)
             assert abs(fd - g[c]) <= 1e-5 * np.abs(g).max() + 1e-4 * abs(g[c]), (int(c), fd, g[c])
+
+
+def test_mixed_precision_adjoint_close_to_fp64_and_meets_tolerance(engine, oracle_lib):
+    """FP32-inner adjoint: the adjoint solve meets its tolerance in FP64 and its face gradients match
+    the FP64 adjoint's on the 256x256 density-ratio-100 workload (both at relative residual 1e-10)."""
+    case = _case(256, 256, seed=0, dr=100.0, tol=1e-10)
+    target_case = _case(256, 256, seed=1, dr=100.0, tol=1e-10)
+    ku, kv = W.face_inverse_density(case)
+    src = W.divergence_source(case)
+    p_target, _ = engine.solve_coeffs(target_case, ku, kv, W.divergence_source(target_case))
+    p64, _ = engine.solve_coeffs(case, ku, kv, src)
+    pm, prep = engine.solve_coeffs(case, ku, kv, src, precision=E.PP_MIXED)
+    assert prep["outer_cycles"] >= 1 and prep["relative_residual"] < 1e-10
+    assert np.abs(pm - p64).max() <= 1e-6 * np.abs(p64).max()
+    lam64, dku64, dkv64, r64 = engine.adjoint(case, ku, kv, p64, p64 - p_target)
+    lam, dku, dkv, rep = engine.adjoint(case, ku, kv, pm, pm - p_target, precision=E.PP_MIXED)
+    assert rep["outer_cycles"] >= 1 and rep["relative_residual"] < 1e-10
+    g64 = np.concatenate([dku64, dkv64])
+    g = np.concatenate([dku, dkv])
+    err_g = np.abs(g - g64).max() / np.abs(g64).max()
+    err_l = np.abs(lam - lam64).max() / np.abs(lam64).max()
+    print(f"[adjoint mixed] iterations {rep['iterations']} ({rep['outer_cycles']} cycles) vs FP64 {r64['iterations']}; "
+          f"max grad diff / max grad {err_g:.2e}, max lambda diff / max lambda {err_l:.2e}")
+    assert err_g < 1e-5 and err_l < 1e-5
+
+
+def test_mixed_precision_finite_differences_at_64_faces_256x256(engine):
+    """The FP32-inner adjoint gradient against central differences of the FP64 loss (64 faces, seed 2)."""
+    case = _case(256, 256, seed=0, dr=100.0, tol=1e-12)
+    target_case = _case(256, 256, seed=1, dr=100.0, tol=1e-12)
+    ku, kv = W.face_inverse_density(case)
+    src = W.divergence_source(case)
+    p_target, _ = engine.solve_coeffs(target_case, ku, kv, W.divergence_source(target_case))
+    p, _ = engine.solve_coeffs(case, ku, kv, src, precision=E.PP_MIXED)
+    _, dku, dkv, _ = engine.adjoint(case, ku, kv, p, p - p_target, precision=E.PP_MIXED)
+    grad = np.concatenate([dku, dkv])
+    k = np.concatenate([ku.reshape(-1), kv.reshape(-1)])
+    nu = ku.size
+    faces = np.random.default_rng(2).choice(k.size, 64, replace=False)
+
+    def loss(kk):
+        pp, _ = engine.solve_coeffs(case, kk[:nu], kk[nu:], src)
+        return 0.5 * float(np.sum((pp - p_target) ** 2))
+
+    scale = np.abs(grad).max()
+    worst = 0.0
+    for f in faces:
+        h = 1e-6 * abs(k[f])
+        kp, km = k.copy(), k.copy()
+        kp[f] += h
+        km[f] -= h
+        fd = (loss(kp) - loss(km)) / (2 * h)
+        err = abs(fd - grad[f])
+        worst = max(worst, err / scale)
+        assert err <= 1e-6 * scale + 1e-4 * abs(grad[f]), (int(f), fd, grad[f])
+    print(f"[fd mixed] 64 faces, max |fd - adjoint| / max|grad| = {worst:.3e}")
+
+
+def test_autograd_mixed_precision():
+    torch = pytest.importorskip("torch")
+    from ding_nsch_core_solver_amd import autograd as A
+    case = _case(64, 48, dr=50.0, tol=1e-10)
+    ku, kv = W.face_inverse_density(case)
+    src = W.divergence_source(case)
+    dev = torch.device("cuda:0")
+    grads = {}
+    for prec in ("fp64", "mixed"):
+        t = [torch.tensor(a.reshape(-1), device=dev, requires_grad=True) for a in (ku, kv, src)]
+        p = A.pressure_solve(*t, case, precision=prec)
+        (0.5 * (p ** 2).sum()).backward()
+        grads[prec] = [x.grad.cpu().numpy() for x in t]
+    for g64, gm in zip(grads["fp64"], grads["mixed"]):
+        assert np.abs(gm - g64).max() <= 1e-5 * np.abs(g64).max()

@@ -27,11 +27,12 @@ def _upload(cases):
     return t
 
 
-def _batch(engine, cases, schemes):
+def _batch(engine, cases, schemes, precision=E.PP_FP64):
     t = _upload(cases)
     ptr = lambda k: [d[k].data_ptr() if d[k] is not None else None for d in t]  # noqa: E731
     reps = engine.solve_batched_device(cases, ptr("rho"), ptr("u"), ptr("v"), ptr("out"), ptr("p_n"), ptr("p_nm1"),
-                                       scheme=schemes, stream=torch.cuda.current_stream(DEV).cuda_stream)
+                                       scheme=schemes, precision=precision,
+                                       stream=torch.cuda.current_stream(DEV).cuda_stream)
     return [d["out"].cpu().numpy() for d in t], reps
 
 
@@ -141,3 +142,54 @@ def test_batched_256_uneven_systems(engine, oracle_lib):
         p_can, it_can, _ = oracle_lib.pressure_solve("icpcg", cases[k], oracle_lib.CANONICAL)
         assert reps[k]["iterations"] == it_can
         assert np.array_equal(outs[k], p_can)
+
+
+def test_batch_of_both_precisions_matches_single_solves(engine):
+    """FP64 and mixed-precision systems of every scheme in one batch (four kernel families): each
+    equals its single solve in the same precision bit for bit, outer cycles included."""
+    cases, schemes, precs = [], [], []
+    picks = ["8x8_neumann_dr1000", "10x14_mixed_values", "16x12_periodic_x", "12x16_periodic_y_dirichlet",
+             "70x20_two_sweep_waves", "130x12_periodic_x_three_waves", "split_step3_70x9_px"]
+    small = dict(small_cases())
+    for k, name in enumerate(picks):
+        for j, s in enumerate(SCHEMES):
+            cases.append(small[name])
+            schemes.append(CODES[s])
+            precs.append(E.PP_MIXED if (k + j) % 2 == 0 else E.PP_FP64)
+    for name, c in large_cases()[:2]:
+        for pr in (E.PP_FP64, E.PP_MIXED):
+            cases.append(c)
+            schemes.append(E.PP_ICPCG)
+            precs.append(pr)
+    outs, reps = _batch(engine, cases, schemes, precs)
+    for k, (c, s, pr) in enumerate(zip(cases, schemes, precs)):
+        p, rep = engine.solve(c, s, precision=pr)
+        assert reps[k]["iterations"] == rep["iterations"], k
+        assert reps[k]["outer_cycles"] == rep["outer_cycles"], k
+        assert reps[k]["relative_residual"] == rep["relative_residual"], k
+        assert np.array_equal(outs[k], p), k
+
+
+def test_batched_256_uneven_systems_mixed_precision(engine, oracle_lib):
+    """The 256-system uneven batch with FP32-inner / FP64-refinement PCG: every system meets the
+    tolerance in its true FP64 residual (recomputed with the oracle's operator for a sample),
+    equals its single mixed solve, and stays close to the FP64 batch."""
+    cases = W.batched_cases(256, seed=0)
+    outs, reps = _batch(engine, cases, E.PP_ICPCG, E.PP_MIXED)
+    assert all(r["status"] == 0 for r in reps)
+    assert all(r["relative_residual"] < 1e-8 for r in reps)
+    outs64, _ = _batch(engine, cases, E.PP_ICPCG)
+    rng = np.random.default_rng(1)
+    sample = rng.choice(256, 16, replace=False)
+    worst = 0.0
+    for k in sample:
+        p, rep = engine.solve(cases[k], E.PP_ICPCG, precision=E.PP_MIXED)
+        assert reps[k]["iterations"] == rep["iterations"] and reps[k]["outer_cycles"] == rep["outer_cycles"]
+        assert np.array_equal(outs[k], p), k
+        s = oracle_lib.OracleSystem(cases[k])
+        r = s.rhs - s.apply_matrix(outs[k])
+        r = r - r.mean()
+        assert np.linalg.norm(r) / np.linalg.norm(s.rhs) < 2e-8, k
+        worst = max(worst, float(np.abs(outs[k] - outs64[k]).max() / np.abs(outs64[k]).max()))
+    print(f"[batch mixed] max |p_mixed - p_fp64| / max |p_fp64| over the sample: {worst:.2e}")
+    assert worst < 1e-5

@@ -31,7 +31,11 @@ def _cases():
            ("130x12_periodic_x_split", W.droplet_case(130, 12, seed=12, density_ratio=30.0, periodic_x=True),
             "split_icpcg"),
            ("256x256_dr100", W.droplet_case(256, 256, seed=0, density_ratio=100.0), "icpcg"),
-           ("420x40_fallback_sweeps", W.droplet_case(420, 40, seed=3, density_ratio=100.0), "icpcg")]
+           ("420x40_fallback_sweeps", W.droplet_case(420, 40, seed=3, density_ratio=100.0), "icpcg"),
+           # the rising-bubble geometry: [0,1] x [0,2], bubble R = 0.25 at (0.5, 0.5), ratio 1000, dt 5e-5,
+           # refinement to 1e-10
+           ("192x384_bubble_dr1000_tol1e-10", W.droplet_case(192, 384, seed=3, density_ratio=1000.0, radius=0.25,
+                                                              ly=2.0, dt=5e-5, tolerance=1e-10), "icpcg")]
     return out
 
 
