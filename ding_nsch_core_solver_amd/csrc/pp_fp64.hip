@@ -505,78 +505,98 @@ __device__ __forceinline__ bool next_unit(UnitCursor &c, int S, int Tn, int U, i
 // Loaders are software-pipelined over two register buffers: the global loads of a loader's next
 // unit are in flight while it waits for ring space and writes the current one to LDS. Only the LDS
 // writes are gated by ring space, in the same order as before, so the deadlock argument is unchanged.
-template <typename T, bool kBackward>
-__device__ __forceinline__ void stage_loader(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, int S,
-                                             T m_r, T floor) {
+// The unit size is a template constant and the next unit's loads are issued unconditionally (the
+// last step reloads the current unit, which is never written), so the code between a unit's loads
+// and its LDS writes has no branch that holds a load: the compiler then waits only for that unit's
+// loads (vmcnt = the other unit's count) instead of draining the queue before every level.
+// Level bases are read without clamping: the level-base table's margin (0 outside the grid) covers
+// every level a sweep wave steps through, and the positions it yields stay below n + kPad.
+template <typename T, bool kBackward, int U>
+__device__ __forceinline__ void stage_loader_units(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, int S,
+                                                   T m_r, T floor) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int nx = a.nx, ny = a.ny, d_last = nx + ny - 2;
+  const int ny = a.ny;
   const int R = stage_ring<T>(a, S), Tn = stage_steps(ny);
-  const int U = stage_unit(R);
   const int L = kWaves - S, me = wave - S;
   const global_ptr<const T> f0 = as_global<const T>(io.in);
   const global_ptr<const T> dg = as_global<const T>(io.dg);
   const global_ptr<const T> Lw = as_global<const T>(io.Lw);
   const global_ptr<const T> Ls = as_global<const T>(io.Ls);
   const __attribute__((address_space(4))) int *lb = (const __attribute__((address_space(4))) int *)a.lev_base;
-  auto load = [&](int w, int t0, T (&v)[kStageUnit][4]) {
+  auto load = [&](int w, int t0, T (&v)[U][4]) {
     const int i = w * 64 + lane;
 #pragma unroll
-    for (int k = 0; k < kStageUnit; ++k) {
-      if (k < U) {
-        const int d = min(max(step_level<kBackward>(w, t0 + k, Tn), 0), d_last);
-        const int q = lb[d] + i;
-        if (kBackward) {
-          const int qn = lb[min(d + 1, d_last)] + i;  // (i, j+1); (i+1, j) follows it
-          v[k][0] = f0[q];
-          v[k][1] = dg[q];
-          v[k][2] = Ls[qn];
-          v[k][3] = Lw[qn + 1];
-        } else {
-          v[k][0] = f0[q];
-          v[k][1] = Lw[q];
-          v[k][2] = Ls[q];
-          v[k][3] = dg[q];
-        }
+    for (int k = 0; k < U; ++k) {
+      const int d = step_level<kBackward>(w, t0 + k, Tn);
+      const unsigned q = static_cast<unsigned>(lb[d] + i);
+      if (kBackward) {
+        const unsigned qn = static_cast<unsigned>(lb[d + 1] + i);  // (i, j+1); (i+1, j) follows it
+        v[k][0] = f0[q];
+        v[k][1] = dg[q];
+        v[k][2] = Ls[qn];
+        v[k][3] = Lw[qn + 1];
+      } else {
+        v[k][0] = f0[q];
+        v[k][1] = Lw[q];
+        v[k][2] = Ls[q];
+        v[k][3] = dg[q];
       }
     }
   };
-  auto store = [&](int w, int t0, const T (&v)[kStageUnit][4]) {
+  auto store = [&](int w, int t0, const T (&v)[U][4]) {
     wait_at_least(&sh.used[w], t0 + U - R, sh);
 #pragma unroll
-    for (int k = 0; k < kStageUnit; ++k) {
-      if (k < U) {
-        volatile __attribute__((address_space(3))) T *slot = stage_slot<T>(sh, w, R, (t0 + k) & (R - 1), lane);
-        if (kBackward) {
-          slot[0] = v[k][0];
-          slot[64] = staged_pivot(v[k][1], floor);
-          slot[128] = v[k][2];
-          slot[192] = v[k][3];
-        } else {
-          slot[0] = v[k][0] - m_r;
-          slot[64] = v[k][1];
-          slot[128] = v[k][2];
-          slot[192] = staged_pivot(v[k][3], floor);
-        }
+    for (int k = 0; k < U; ++k) {
+      volatile __attribute__((address_space(3))) T *slot = stage_slot<T>(sh, w, R, (t0 + k) & (R - 1), lane);
+      if (kBackward) {
+        slot[0] = v[k][0];
+        slot[64] = staged_pivot(v[k][1], floor);
+        slot[128] = v[k][2];
+        slot[192] = v[k][3];
+      } else {
+        slot[0] = v[k][0] - m_r;
+        slot[64] = v[k][1];
+        slot[128] = v[k][2];
+        slot[192] = staged_pivot(v[k][3], floor);
       }
     }
     if (lane == 0) {
+#pragma unroll
       for (int k = 0; k < U; ++k) *lds(&sh.ready[w * kMaxRing + ((t0 + k) & (R - 1))]) = t0 + k + 1;
     }
   };
   UnitCursor cur{0, 0, 0};
   int wA = 0, tA = 0, wB = 0, tB = 0;
-  T bufA[kStageUnit][4], bufB[kStageUnit][4];
-  bool hasA = next_unit<kBackward>(cur, S, Tn, U, L, me, wA, tA);
-  if (hasA) load(wA, tA, bufA);
-  while (hasA) {
+  T bufA[U][4], bufB[U][4];
+  if (!next_unit<kBackward>(cur, S, Tn, U, L, me, wA, tA)) return;
+  load(wA, tA, bufA);
+  for (;;) {
     const bool hasB = next_unit<kBackward>(cur, S, Tn, U, L, me, wB, tB);
-    if (hasB) load(wB, tB, bufB);
+    if (!hasB) {
+      wB = wA;
+      tB = tA;
+    }
+    load(wB, tB, bufB);
     store(wA, tA, bufA);
     if (!hasB) break;
-    hasA = next_unit<kBackward>(cur, S, Tn, U, L, me, wA, tA);
-    if (hasA) load(wA, tA, bufA);
+    const bool hasA = next_unit<kBackward>(cur, S, Tn, U, L, me, wA, tA);
+    if (!hasA) {
+      wA = wB;
+      tA = tB;
+    }
+    load(wA, tA, bufA);
     store(wB, tB, bufB);
+    if (!hasA) break;
   }
+}
+
+template <typename T, bool kBackward>
+__device__ __forceinline__ void stage_loader(const SysArgs &a, const SweepArrays<T> &io, SweepShared &sh, int S, T m_r,
+                                             T floor) {
+  if (stage_unit(stage_ring<T>(a, S)) == kStageUnit)
+    stage_loader_units<T, kBackward, kStageUnit>(a, io, sh, S, m_r, floor);
+  else
+    stage_loader_units<T, kBackward, kStageUnit / 2>(a, io, sh, S, m_r, floor);
 }
 
 // Sweep side of the staging: per chunk of kPrefetch levels, wait once for the chunk's units, read
@@ -860,6 +880,171 @@ __device__ __forceinline__ double spmv_levels(const SysArgs &a, const Stencil<T>
   return acc;
 }
 
+// ---- windowed SpMV (nx <= 64 kWinChunks) ----------------------------------------------------------
+//
+// Wave w walks the levels [spmv_split[w], spmv_split[w+1]) in ascending order (contiguous ranges
+// balanced by cell count, set by the host); lane l of row chunk c holds row i = 64 c + l of every
+// level, as in the sweeps. A level's p, aW, aS and aP are loaded once: a cell's south and north
+// neighbours are the same lane in the previous and next level, west and east the neighbouring
+// lanes (a DPP shift; the chunk-boundary lane takes the adjacent chunk's value), so a cell reads
+// four values where the level-major pass reads eleven. Loads run two levels ahead of the
+// arithmetic. Row sums are the ones spmv_cell forms, term for term; thread (w, l) adds its
+// contributions in (level, chunk) order.
+constexpr int kWinChunks = 4;
+static_assert(sizeof(SysArgs::spmv_split) == (kWaves + 1) * sizeof(int), "one SpMV range per wave");
+
+template <typename T>
+struct WinLevel {
+  T p[kWinChunks], w[kWinChunks], s[kWinChunks], c[kWinChunks];
+};
+
+// Position of row 64 c + lane in level d, clamped into the level's rows so every lane reads a real
+// cell. Branch-free: for the levels just outside the grid (d = -1 and up to six past the last),
+// the level-base table's margin holds 0 and the clamp lands on a position below n + kPad.
+__device__ __forceinline__ int win_pos(const __attribute__((address_space(4))) int *lb, int d, int c, int lane,
+                                       int nx, int ny) {
+  const int lo = level_lo(d, ny), hi = d < nx - 1 ? d : nx - 1;
+  const int top = hi > lo ? hi : lo;
+  const int i = 64 * c + lane;
+  return lb[d] + (i < lo ? lo : (i > top ? top : i));
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void win_load(const Stencil<T> &A, const global_ptr<const T> p,
+                                         const __attribute__((address_space(4))) int *lb, int d, int lane,
+                                         WinLevel<T> &L) {
+#pragma unroll
+  for (int c = 0; c < S; ++c) {
+    const int q = win_pos(lb, d, c, lane, A.nx, A.ny);
+    L.p[c] = p[q];
+    L.w[c] = A.aW[q];
+    L.s[c] = A.aS[q];
+    L.c[c] = A.aP[q];
+  }
+}
+
+// Level d of the window: rows of every chunk from (prev, cur, next) = levels (d-1, d, d+1).
+template <typename T, int S, bool kPeriodic, bool kExtra, typename F>
+__device__ __forceinline__ double win_level(const Stencil<T> &A, const global_ptr<const T> p, T m_p,
+                                            const global_ptr<const T> extra,
+                                            const __attribute__((address_space(4))) int *lb, int d, int lane,
+                                            const WinLevel<T> &prev, const WinLevel<T> &cur, const WinLevel<T> &next,
+                                            double acc, F &epi) {
+  const int nx = A.nx, ny = A.ny;
+  const int b0 = lb[d];
+#pragma unroll
+  for (int c = 0; c < S; ++c) {
+    const int i = 64 * c + lane, j = d - i;
+    const bool in_i = i < nx;
+    const bool in_j = static_cast<unsigned>(j) < static_cast<unsigned>(ny);
+    const bool act = in_i && in_j;
+    const bool has_w = i >= 1, has_s = j >= 1, has_n = j <= ny - 2, has_e = i <= nx - 2;
+    T pw = wave_shr1(prev.p[c]);
+    if (c > 0) pw = write_lane(pw, lane_value(prev.p[c - 1], 63), 0);
+    T pe = wave_shl1(next.p[c]), ew = wave_shl1(next.w[c]);
+    if (c + 1 < S) {
+      pe = write_lane(pe, lane_value(next.p[c + 1], 0), 63);
+      ew = write_lane(ew, lane_value(next.w[c + 1], 0), 63);
+    }
+    const T cw = has_w ? cur.w[c] : T(0), cs = has_s ? cur.s[c] : T(0);
+    const T cn = has_n ? next.s[c] : T(0), ce = has_e ? ew : T(0);
+    const T pc = cur.p[c];
+    const int q = act ? b0 + i : win_pos(lb, d, c, lane, nx, ny);
+    T s = T(0);
+    if (kPeriodic && A.px) {  // wrap (0, j) of row (nx-1, j)
+      const bool on = i == nx - 1 && in_j;
+      const int qx = on ? level_base(j, nx, ny) : q;
+      s += (on ? A.aW[qx] : T(0)) * (p[qx] - m_p);
+    }
+    s += cw * (pw - m_p);
+    if (kPeriodic && A.py) {  // wrap (i, 0) of row (i, ny-1)
+      const bool on = j == ny - 1 && in_i;
+      const int qy = on ? level_base(i, nx, ny) + i : q;
+      s += (on ? A.aS[qy] : T(0)) * (p[qy] - m_p);
+    }
+    s += cs * (prev.p[c] - m_p);
+    s += cur.c[c] * (pc - m_p);
+    s += cn * (next.p[c] - m_p);
+    if (kPeriodic && A.py) {  // wrap (i, ny-1) of row (i, 0)
+      const bool on = j == 0 && in_i;
+      const int qy = on ? level_base(i + ny - 1, nx, ny) + i : q;
+      s += (on ? A.aS[q] : T(0)) * (p[qy] - m_p);
+    }
+    s += ce * (pe - m_p);
+    if (kPeriodic && A.px) {  // wrap (nx-1, j) of row (0, j)
+      const bool on = i == 0 && in_j;
+      const int qx = on ? level_base(nx - 1 + j, nx, ny) + nx - 1 : q;
+      s += (on ? A.aW[q] : T(0)) * (p[qx] - m_p);
+    }
+    const T ex = kExtra ? extra[q] : T(0);
+    acc = acc + (act ? epi(q, s, pc - m_p, ex) : 0.0);
+  }
+  return acc;
+}
+
+template <typename T, int S, bool kPeriodic, bool kExtra, typename F>
+__device__ __forceinline__ double spmv_window(const SysArgs &a, const Stencil<T> &A, const global_ptr<const T> p, T m_p,
+                                              const global_ptr<const T> extra, F &epi) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const __attribute__((address_space(4))) int *lb = (const __attribute__((address_space(4))) int *)a.lev_base;
+  const int d_begin = a.spmv_split[wave], d_end = a.spmv_split[wave + 1];
+  double acc = 0.0;
+  if (d_begin >= d_end) return acc;
+  WinLevel<T> B0, B1, B2, B3;
+  win_load<T, S>(A, p, lb, d_begin - 1, lane, B0);
+  win_load<T, S>(A, p, lb, d_begin, lane, B1);
+  win_load<T, S>(A, p, lb, d_begin + 1, lane, B2);
+  // four rotating level buffers, unrolled so no buffer is copied: step k computes level d + k from
+  // (d+k-1, d+k, d+k+1) after starting the loads of level d + k + 2 into the free buffer
+  int d = d_begin;
+  for (; d + 4 <= d_end; d += 4) {
+    win_load<T, S>(A, p, lb, d + 2, lane, B3);
+    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d, lane, B0, B1, B2, acc, epi);
+    win_load<T, S>(A, p, lb, d + 3, lane, B0);
+    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 1, lane, B1, B2, B3, acc, epi);
+    win_load<T, S>(A, p, lb, d + 4, lane, B1);
+    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 2, lane, B2, B3, B0, acc, epi);
+    win_load<T, S>(A, p, lb, d + 5, lane, B2);
+    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 3, lane, B3, B0, B1, acc, epi);
+  }
+  // remaining 0..3 levels: B0, B1, B2 hold d-1, d, d+1
+  if (d < d_end) {
+    win_load<T, S>(A, p, lb, d + 2, lane, B3);
+    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d, lane, B0, B1, B2, acc, epi);
+    if (d + 1 < d_end) {
+      win_load<T, S>(A, p, lb, d + 3, lane, B0);
+      acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 1, lane, B1, B2, B3, acc, epi);
+      if (d + 2 < d_end) acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 2, lane, B2, B3, B0, acc, epi);
+    }
+  }
+  return acc;
+}
+
+template <typename T, bool kPeriodic, bool kExtra, typename F>
+__device__ __forceinline__ double spmv_window_rows(const SysArgs &a, const Stencil<T> &A, const global_ptr<const T> p,
+                                                   T m_p, const global_ptr<const T> extra, F &epi) {
+  switch ((A.nx + 63) >> 6) {
+    case 1: return spmv_window<T, 1, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
+    case 2: return spmv_window<T, 2, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
+    case 3: return spmv_window<T, 3, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
+    default: return spmv_window<T, 4, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
+  }
+}
+
+// A p (plus the epilogue) with the windowed pass where the rows fit its registers, else level-major.
+template <typename T, typename F>
+__device__ __forceinline__ double spmv_pass(const SysArgs &a, const Stencil<T> &A, const global_ptr<const T> p, T m_p,
+                                            const global_ptr<const T> extra, F &&epi) {
+  if (A.nx > 64 * kWinChunks) return spmv_levels<T>(a, A, p, m_p, extra, epi);
+  const bool periodic = A.px || A.py;
+  if (extra) {
+    return periodic ? spmv_window_rows<T, true, true>(a, A, p, m_p, extra, epi)
+                    : spmv_window_rows<T, false, true>(a, A, p, m_p, extra, epi);
+  }
+  return periodic ? spmv_window_rows<T, true, false>(a, A, p, m_p, extra, epi)
+                  : spmv_window_rows<T, false, false>(a, A, p, m_p, extra, epi);
+}
+
 // z = M^{-1} (r - m_r): forward then backward sweep, staged through LDS when S <= kMaxStagedWaves.
 template <typename T, bool kIldlt>
 __device__ __forceinline__ void precondition(const SysArgs &a, const SweepArrays<T> &fwd, const SweepArrays<T> &bwd,
@@ -1064,7 +1249,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
     const int limit = a.max_iterations < 1 ? 1 : a.max_iterations;
     for (int it = 0; it < limit; ++it) {
       // Ap and p^T A p (level-major order)
-      double pap = spmv_levels<double>(a, stencil, p_raw, m_p, nullptr, [&](int q, double apq, double pq, double) {
+      double pap = spmv_pass<double>(a, stencil, p_raw, m_p, nullptr, [&](int q, double apq, double pq, double) {
         apv[q] = apq;
         p_prj[q] = pq;
         return pq * apq;
@@ -1290,7 +1475,7 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
   int total = 0, cycles = 0;
   for (;;) {
     // true residual r = b - A (x - m_x) in FP64, projected
-    double rs = spmv_levels<double>(a, A64, xv, m_x, bv, [&](int q, double axq, double, double bq) {
+    double rs = spmv_pass<double>(a, A64, xv, m_x, bv, [&](int q, double axq, double, double bq) {
       const double rq = bq - axq;
       rv[q] = rq;
       return rq;
@@ -1346,7 +1531,7 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
     float m_p = 0.0f;
     clk.mark(7);
     while (isfinite(rz) && rz > 0.0 && inner < inner_cap && total < limit) {
-      double pap = spmv_levels<float>(a, A32, p32, m_p, nullptr, [&](int q, float apq, float pq, float) {
+      double pap = spmv_pass<float>(a, A32, p32, m_p, nullptr, [&](int q, float apq, float pq, float) {
         ap32[q] = apq;
         return static_cast<double>(pq) * static_cast<double>(apq);
       });

@@ -23,6 +23,7 @@ struct pp_ctx {
     int nlev = 0;
     int *base = nullptr;   // level-base table with kLevPad entries of margin on both sides
     uint32_t *cell = nullptr;
+    int split[pp::kWaves + 1] = {};  // SpMV level ranges per wave
   };
   std::map<std::pair<int, int>, Layout> layouts;
 
@@ -87,6 +88,21 @@ void build_layout(int nx, int ny, std::vector<int> &base, std::vector<uint32_t> 
   }
 }
 
+// SpMV level ranges: wave w takes levels [split[w], split[w+1]), the boundaries at the first levels
+// where the running cell count reaches w n / kWaves (ranges may be empty on small grids).
+void spmv_split(int nx, int ny, int *split) {
+  const int nlev = nx + ny - 1;
+  const long long n = static_cast<long long>(nx) * ny;
+  long long cum = 0;
+  int w = 1;
+  split[0] = 0;
+  for (int d = 0; d < nlev && w < pp::kWaves; ++d) {
+    cum += std::min(d, nx - 1) - std::max(0, d - ny + 1) + 1;
+    while (w < pp::kWaves && cum * pp::kWaves >= w * n) split[w++] = d + 1;
+  }
+  while (w <= pp::kWaves) split[w++] = nlev;
+}
+
 const char *status_text(int status, int scheme) {
   const bool ildlt = scheme == PP_ILDLT_PCG || scheme == PP_SPLIT_ILDLT_PCG;
   switch (status) {
@@ -138,6 +154,7 @@ int ensure_layout(pp_ctx *ctx, int nx, int ny, pp_ctx::Layout **out, pp_report *
     build_layout(nx, ny, base, cell);
     pp_ctx::Layout L;
     L.nlev = nx + ny - 1;
+    spmv_split(nx, ny, L.split);
     PP_HIP_CHECK(hipMalloc(&L.base, base.size() * sizeof(int)));
     PP_HIP_CHECK(hipMalloc(&L.cell, cell.size() * sizeof(uint32_t)));
     PP_HIP_CHECK(hipMemcpy(L.base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -210,6 +227,7 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
     if (t != ctx->tuning.end()) a.ring = t->second;
   }
   a.lev_base = L.base + pp::kLevPad;
+  for (int w = 0; w <= pp::kWaves; ++w) a.spmv_split[w] = L.split[w];
   a.cell = L.cell;
   set_work(a, ctx->work, ctx->cap + pp::kPad);
   set_fwork(a, ctx->fwork, ctx->cap + pp::kPad);

@@ -27,6 +27,7 @@ struct SysArgs {
   double inner_tolerance;
   // layout tables
   const int *lev_base;     // level_base(d) for d in [-kLevPad, nlev + kLevPad) (0 outside the grid)
+  int spmv_split[9];       // SpMV level ranges of the 8 waves: wave w takes [spmv_split[w], spmv_split[w+1])
   const uint32_t *cell;    // n, (i << 16) | j of each wavefront position
   // fields (natural layout)
   const double *rho, *u, *v, *p_n, *p_nm1;

@@ -109,7 +109,10 @@ struct Reducer {
     return wsum[0];
   }
 
-  // The engine's level-major order (its SpMV pass, used for p^T A p): wave w of 8 takes levels
+  // The engine's SpMV order for p^T A p. Grids with nx <= 256 use its windowed pass: wave w of 8
+  // takes the levels [split[w], split[w+1]) (window_split below), lane l of chunk c holds row
+  // 64 c + l, and each of the 512 threads sums its cells in (level, chunk) order from 0.0. Wider
+  // grids use the level-major pass: wave w of 8 takes levels
   // d = w, w + 8, ...; within a level, cell i goes to lane (i - lo(d)) mod 64; each of the 512
   // threads sums its cells in (level, row) order from 0.0; then the same lane and wave trees.
   // Other modes: the same as sum().
@@ -119,6 +122,23 @@ struct Reducer {
     constexpr std::size_t T = 512, W = 8;
     double part[T];
     for (std::size_t t = 0; t < T; ++t) part[t] = 0.0;
+    if (nx <= 256) {
+      int split[W + 1];
+      window_split(nx, ny, split);
+      for (std::size_t w = 0; w < W; ++w)
+        for (int d = split[w]; d < split[w + 1]; ++d) {
+          const int lo = std::max(0, d - ny + 1), hi = std::min(d, nx - 1);
+          for (int c = 0; 64 * c < nx; ++c)
+            for (int l = 0; l < 64; ++l) {
+              const int i = 64 * c + l;
+              if (i < lo || i > hi) continue;
+              const std::size_t t = 64 * w + static_cast<std::size_t>(l);
+              part[t] = part[t] + f(static_cast<std::size_t>(i) * static_cast<std::size_t>(ny) +
+                                    static_cast<std::size_t>(d - i));
+            }
+        }
+      return tree(part);
+    }
     for (int d = 0; d < nx + ny - 1; ++d) {
       const int lo = std::max(0, d - ny + 1), hi = std::min(d, nx - 1);
       const std::size_t w = static_cast<std::size_t>(d) % W;
@@ -127,6 +147,12 @@ struct Reducer {
         part[t] = part[t] + f(static_cast<std::size_t>(i) * static_cast<std::size_t>(ny) + static_cast<std::size_t>(d - i));
       }
     }
+    return tree(part);
+  }
+
+  // Lane tree (offsets 32 .. 1) inside each wave, then the wave tree (offsets 4, 2, 1).
+  static double tree(double *part) {
+    constexpr std::size_t W = 8;
     double wsum[W];
     for (std::size_t w = 0; w < W; ++w) {
       double *v = part + 64 * w;
@@ -137,6 +163,22 @@ struct Reducer {
     for (std::size_t off = W / 2; off >= 1; off >>= 1)
       for (std::size_t t = 0; t < off; ++t) wsum[t] = wsum[t] + wsum[t + off];
     return wsum[0];
+  }
+
+  // The engine's SpMV level ranges: boundary w at the first level where the running cell count
+  // reaches w n / 8 (empty ranges on small grids).
+  static void window_split(int nx, int ny, int *split) {
+    constexpr int W = 8;
+    const int nlev = nx + ny - 1;
+    const long long n = static_cast<long long>(nx) * ny;
+    long long cum = 0;
+    int w = 1;
+    split[0] = 0;
+    for (int d = 0; d < nlev && w < W; ++d) {
+      cum += std::min(d, nx - 1) - std::max(0, d - ny + 1) + 1;
+      while (w < W && cum * W >= w * n) split[w++] = d + 1;
+    }
+    while (w <= W) split[w++] = nlev;
   }
 };
 
