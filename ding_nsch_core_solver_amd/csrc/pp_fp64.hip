@@ -565,13 +565,33 @@ __device__ __forceinline__ void stage_loader_units(const SysArgs &a, const Sweep
       for (int k = 0; k < U; ++k) *lds(&sh.ready[w * kMaxRing + ((t0 + k) & (R - 1))]) = t0 + k + 1;
     }
   };
+  // With at least as many loaders as sweep waves, each loader serves one wave only (its units in
+  // step order, interleaved with the wave's other loaders), so a loader waiting for ring space never
+  // holds back another wave; spare loaders go to the waves that start last (S-1 forward, 0
+  // backward), which finish the sweep. Otherwise loaders take all units in the order of need.
+  const bool dedicated = S <= L;
+  const int rank = me % S;
+  const int w_own = kBackward ? rank : S - 1 - rank;
+  const int n_own = (L - 1 - rank) / S + 1, k_own = me / S;
+  int m_own = 0;
   UnitCursor cur{0, 0, 0};
+  auto next = [&](int &w_out, int &t_out) -> bool {
+    if (dedicated) {
+      const int t0 = U * (k_own + n_own * m_own);
+      if (t0 >= Tn) return false;
+      ++m_own;
+      w_out = w_own;
+      t_out = t0;
+      return true;
+    }
+    return next_unit<kBackward>(cur, S, Tn, U, L, me, w_out, t_out);
+  };
   int wA = 0, tA = 0, wB = 0, tB = 0;
   T bufA[U][4], bufB[U][4];
-  if (!next_unit<kBackward>(cur, S, Tn, U, L, me, wA, tA)) return;
+  if (!next(wA, tA)) return;
   load(wA, tA, bufA);
   for (;;) {
-    const bool hasB = next_unit<kBackward>(cur, S, Tn, U, L, me, wB, tB);
+    const bool hasB = next(wB, tB);
     if (!hasB) {
       wB = wA;
       tB = tA;
@@ -579,7 +599,7 @@ __device__ __forceinline__ void stage_loader_units(const SysArgs &a, const Sweep
     load(wB, tB, bufB);
     store(wA, tA, bufA);
     if (!hasB) break;
-    const bool hasA = next_unit<kBackward>(cur, S, Tn, U, L, me, wA, tA);
+    const bool hasA = next(wA, tA);
     if (!hasA) {
       wA = wB;
       tA = tB;
