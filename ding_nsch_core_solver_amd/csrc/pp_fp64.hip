@@ -781,7 +781,7 @@ struct Stencil {
 // level's cells (contiguous positions base(d) + i), so every neighbour address is a scalar level
 // base plus the lane's row and no index array is read. Row sums follow the CSR column order
 // (ch_sparse_krylov.hpp:242-248): wrap (0, j), west, wrap (i, 0), south, centre, north,
-// wrap (i, ny-1), east, wrap (nx-1, j). epi(q, s) consumes row q's result and returns its
+// wrap (i, ny-1), east, wrap (nx-1, j). epi consumes the row's result and returns its
 // contribution to the reduction; thread (w, lane) adds its contributions in (level, row) order,
 // which the oracle replays for p^T A p (Reducer::sum_levels).
 // One row, branch-free: every operand is loaded from a valid address and an absent entry gets a
@@ -834,7 +834,8 @@ __device__ __forceinline__ T spmv_cell(const Stencil<T> &A, const global_ptr<con
 
 constexpr int kSpmvChunks = 4;  // 64-cell chunks per level computed before any result is stored
 
-// epi(q, s, pq, ex) gets row q's result s, pq = p[q] - m_p and ex = extra[q] (0 when extra is null);
+// epi(base, i, s, pq, ex) gets the result s of the row at position q = base + i (base: the level
+// base, i: the row), pq = p[q] - m_p and ex = extra[q] (0 when extra is null);
 // every load of a pass happens before its epilogues store anything.
 template <typename T, typename F>
 __device__ __forceinline__ double spmv_levels(const SysArgs &a, const Stencil<T> &A, const global_ptr<const T> p, T m_p,
@@ -850,19 +851,20 @@ __device__ __forceinline__ double spmv_levels(const SysArgs &a, const Stencil<T>
     // (an absent cell adds +0.0, exact: the running sum is never -0)
     for (int d0 = wave; d0 < nlev; d0 += 2 * kWaves) {
       T sv[2][kSpmvChunks], pv[2][kSpmvChunks], xv[2][kSpmvChunks];
-      int qv[2][kSpmvChunks];
+      int qv[2][kSpmvChunks], bv[2];
       bool ok[2][kSpmvChunks];
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int d = d0 + u * kWaves;
         const int lo = level_lo(d, ny), hi = d < nx - 1 ? d : nx - 1;
         const int bm = lb[d - 1], b0 = lb[d], bp = lb[d + 1];
+        bv[u] = b0;
 #pragma unroll
         for (int c = 0; c < kSpmvChunks; ++c) {
           if (c < chunks) {
             const int i = lo + 64 * c + lane;
             ok[u][c] = d < nlev && i <= hi;
-            qv[u][c] = b0 + i;
+            qv[u][c] = i;
             sv[u][c] = spmv_cell<T>(A, p, m_p, i, d - i, bm, b0, bp, pv[u][c]);  // in-bounds loads even when !ok
             xv[u][c] = extra ? extra[b0 + i] : T(0);
           } else {
@@ -874,7 +876,7 @@ __device__ __forceinline__ double spmv_levels(const SysArgs &a, const Stencil<T>
       for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int c = 0; c < kSpmvChunks; ++c)
-          if (c < chunks) acc = acc + (ok[u][c] ? epi(qv[u][c], sv[u][c], pv[u][c], xv[u][c]) : 0.0);
+          if (c < chunks) acc = acc + (ok[u][c] ? epi(bv[u], qv[u][c], sv[u][c], pv[u][c], xv[u][c]) : 0.0);
     }
   } else {
     // wide levels: one level at a time, its chunks in groups
@@ -893,12 +895,42 @@ __device__ __forceinline__ double spmv_levels(const SysArgs &a, const Stencil<T>
         }
 #pragma unroll
         for (int c = 0; c < kSpmvChunks; ++c)
-          acc = acc + (ok[c] ? epi(b0 + i0 + 64 * c + lane, sv[c], pv[c], xv[c]) : 0.0);
+          acc = acc + (ok[c] ? epi(b0, i0 + 64 * c + lane, sv[c], pv[c], xv[c]) : 0.0);
       }
     }
   }
   return acc;
 }
+
+// A work array addressed as (level base, row): raw buffer accesses with the level base as the
+// scalar offset and the row as the vector offset, so a lane's address costs one 32-bit value and
+// no 64-bit arithmetic. Accesses past n + kPad are discarded (stores) or read 0 (loads) by the
+// buffer range check.
+template <typename T>
+struct LevelArray {
+  __amdgpu_buffer_rsrc_t rsrc;
+  __device__ __forceinline__ LevelArray(const T *ptr, int n)
+      : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(ptr), 0, static_cast<int>((n + kPad) * sizeof(T)),
+                                               0x00020000)) {}
+  __device__ __forceinline__ LevelArray(const __attribute__((address_space(1))) T *ptr, int n)
+      : LevelArray((const T *)ptr, n) {}
+  __device__ __forceinline__ T get(int base, unsigned voff) const {
+    if constexpr (sizeof(T) == 8) {
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, base * 8, 0));
+    } else {
+      return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, base * 4, 0));
+    }
+  }
+  __device__ __forceinline__ void put(int base, int row, T v) const {
+    const unsigned voff = static_cast<unsigned>(row) * sizeof(T);
+    if constexpr (sizeof(T) == 8) {
+      typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rsrc, voff, base * 8, 0);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rsrc, voff, base * 4, 0);
+    }
+  }
+};
 
 // ---- windowed SpMV (nx <= 64 kWinChunks) ----------------------------------------------------------
 //
@@ -911,6 +943,10 @@ __device__ __forceinline__ double spmv_levels(const SysArgs &a, const Stencil<T>
 // arithmetic. Row sums are the ones spmv_cell forms, term for term; thread (w, l) adds its
 // contributions in (level, chunk) order.
 constexpr int kWinChunks = 4;
+#ifndef PP_WIN_BUFFERS
+#define PP_WIN_BUFFERS 4
+#endif
+constexpr int kWinBuffers = PP_WIN_BUFFERS;  // rotating level buffers of the windowed SpMV
 static_assert(sizeof(SysArgs::spmv_split) == (kWaves + 1) * sizeof(int), "one SpMV range per wave");
 
 template <typename T>
@@ -930,23 +966,41 @@ __device__ __forceinline__ int win_pos(const __attribute__((address_space(4))) i
 }
 
 template <typename T, int S>
-__device__ __forceinline__ void win_load(const Stencil<T> &A, const global_ptr<const T> p,
-                                         const __attribute__((address_space(4))) int *lb, int d, int lane,
-                                         WinLevel<T> &L) {
+__device__ __forceinline__ void win_shift(WinLevel<T> &L, T m_p) {
+#pragma unroll
+  for (int c = 0; c < S; ++c) L.p[c] = L.p[c] - m_p;
+}
+
+template <typename T>
+struct WinArrays {
+  LevelArray<T> p, w, s, c;
+};
+
+// One level's operands: positions lb[d] + clamp(row, lo, top), the clamp keeping every lane on a
+// real cell of the level (levels just outside the grid read position 0 or a pad position).
+template <typename T, int S>
+__device__ __forceinline__ void win_load(const WinArrays<T> &V, const __attribute__((address_space(4))) int *lb, int d,
+                                         int lane, int nx, int ny, WinLevel<T> &L) {
+  const int lo = level_lo(d, ny), hi = d < nx - 1 ? d : nx - 1;
+  const int top = hi > lo ? hi : lo;
+  const int base = lb[d];
 #pragma unroll
   for (int c = 0; c < S; ++c) {
-    const int q = win_pos(lb, d, c, lane, A.nx, A.ny);
-    L.p[c] = p[q];
-    L.w[c] = A.aW[q];
-    L.s[c] = A.aS[q];
-    L.c[c] = A.aP[q];
+    const int i = 64 * c + lane;
+    const unsigned voff = static_cast<unsigned>(i < lo ? lo : (i > top ? top : i)) * sizeof(T);
+    L.p[c] = V.p.get(base, voff);
+    L.w[c] = V.w.get(base, voff);
+    L.s[c] = V.s.get(base, voff);
+    L.c[c] = V.c.get(base, voff);
   }
 }
 
-// Level d of the window: rows of every chunk from (prev, cur, next) = levels (d-1, d, d+1).
+// Level d of the window: rows of every chunk from (prev, cur, next) = levels (d-1, d, d+1), whose p
+// values already hold p - m_p (win_shift: each level's value serves three rows' sums, and the
+// subtraction is the one a row-by-row SpMV repeats per use).
 template <typename T, int S, bool kPeriodic, bool kExtra, typename F>
 __device__ __forceinline__ double win_level(const Stencil<T> &A, const global_ptr<const T> p, T m_p,
-                                            const global_ptr<const T> extra,
+                                            const LevelArray<T> &extra,
                                             const __attribute__((address_space(4))) int *lb, int d, int lane,
                                             const WinLevel<T> &prev, const WinLevel<T> &cur, const WinLevel<T> &next,
                                             double acc, F &epi) {
@@ -959,7 +1013,7 @@ __device__ __forceinline__ double win_level(const Stencil<T> &A, const global_pt
     const bool in_j = static_cast<unsigned>(j) < static_cast<unsigned>(ny);
     const bool act = in_i && in_j;
     const bool has_w = i >= 1, has_s = j >= 1, has_n = j <= ny - 2, has_e = i <= nx - 2;
-    T pw = wave_shr1(prev.p[c]);
+    T pw = wave_shr1(prev.p[c]);  // p - m_p of the west neighbour
     if (c > 0) pw = write_lane(pw, lane_value(prev.p[c - 1], 63), 0);
     T pe = wave_shl1(next.p[c]), ew = wave_shl1(next.w[c]);
     if (c + 1 < S) {
@@ -969,72 +1023,68 @@ __device__ __forceinline__ double win_level(const Stencil<T> &A, const global_pt
     const T cw = has_w ? cur.w[c] : T(0), cs = has_s ? cur.s[c] : T(0);
     const T cn = has_n ? next.s[c] : T(0), ce = has_e ? ew : T(0);
     const T pc = cur.p[c];
-    const int q = act ? b0 + i : win_pos(lb, d, c, lane, nx, ny);
     T s = T(0);
+    const int q = kPeriodic ? (act ? b0 + i : win_pos(lb, d, c, lane, nx, ny)) : 0;
     if (kPeriodic && A.px) {  // wrap (0, j) of row (nx-1, j)
       const bool on = i == nx - 1 && in_j;
       const int qx = on ? level_base(j, nx, ny) : q;
       s += (on ? A.aW[qx] : T(0)) * (p[qx] - m_p);
     }
-    s += cw * (pw - m_p);
+    s += cw * pw;
     if (kPeriodic && A.py) {  // wrap (i, 0) of row (i, ny-1)
       const bool on = j == ny - 1 && in_i;
       const int qy = on ? level_base(i, nx, ny) + i : q;
       s += (on ? A.aS[qy] : T(0)) * (p[qy] - m_p);
     }
-    s += cs * (prev.p[c] - m_p);
-    s += cur.c[c] * (pc - m_p);
-    s += cn * (next.p[c] - m_p);
+    s += cs * prev.p[c];
+    s += cur.c[c] * pc;
+    s += cn * next.p[c];
     if (kPeriodic && A.py) {  // wrap (i, ny-1) of row (i, 0)
       const bool on = j == 0 && in_i;
       const int qy = on ? level_base(i + ny - 1, nx, ny) + i : q;
       s += (on ? A.aS[q] : T(0)) * (p[qy] - m_p);
     }
-    s += ce * (pe - m_p);
+    s += ce * pe;
     if (kPeriodic && A.px) {  // wrap (nx-1, j) of row (0, j)
       const bool on = i == 0 && in_j;
       const int qx = on ? level_base(nx - 1 + j, nx, ny) + nx - 1 : q;
       s += (on ? A.aW[q] : T(0)) * (p[qx] - m_p);
     }
-    const T ex = kExtra ? extra[q] : T(0);
-    acc = acc + (act ? epi(q, s, pc - m_p, ex) : 0.0);
+    const T ex = kExtra ? extra.get(b0, static_cast<unsigned>(i) * sizeof(T)) : T(0);
+    acc = acc + (act ? epi(b0, i, s, pc, ex) : 0.0);
   }
   return acc;
 }
 
-template <typename T, int S, bool kPeriodic, bool kExtra, typename F>
+// K rotating level buffers: the buffer of level L is (L - d_begin + 1) mod K. Step d computes level
+// d from (d-1, d, d+1) after starting the loads of level d + K - 2, so a level's loads are issued
+// K - 3 steps before it is first read (as "next"). The loop is unrolled K times, so every buffer
+// index is a compile-time constant and no buffer is copied.
+template <typename T, int S, int K, bool kPeriodic, bool kExtra, typename F>
 __device__ __forceinline__ double spmv_window(const SysArgs &a, const Stencil<T> &A, const global_ptr<const T> p, T m_p,
-                                              const global_ptr<const T> extra, F &epi) {
+                                              const global_ptr<const T> extra_ptr, F &epi) {
+  static_assert(K >= 4, "prev, cur, next and one buffer in flight");
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const __attribute__((address_space(4))) int *lb = (const __attribute__((address_space(4))) int *)a.lev_base;
   const int d_begin = a.spmv_split[wave], d_end = a.spmv_split[wave + 1];
   double acc = 0.0;
   if (d_begin >= d_end) return acc;
-  WinLevel<T> B0, B1, B2, B3;
-  win_load<T, S>(A, p, lb, d_begin - 1, lane, B0);
-  win_load<T, S>(A, p, lb, d_begin, lane, B1);
-  win_load<T, S>(A, p, lb, d_begin + 1, lane, B2);
-  // four rotating level buffers, unrolled so no buffer is copied: step k computes level d + k from
-  // (d+k-1, d+k, d+k+1) after starting the loads of level d + k + 2 into the free buffer
-  int d = d_begin;
-  for (; d + 4 <= d_end; d += 4) {
-    win_load<T, S>(A, p, lb, d + 2, lane, B3);
-    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d, lane, B0, B1, B2, acc, epi);
-    win_load<T, S>(A, p, lb, d + 3, lane, B0);
-    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 1, lane, B1, B2, B3, acc, epi);
-    win_load<T, S>(A, p, lb, d + 4, lane, B1);
-    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 2, lane, B2, B3, B0, acc, epi);
-    win_load<T, S>(A, p, lb, d + 5, lane, B2);
-    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 3, lane, B3, B0, B1, acc, epi);
-  }
-  // remaining 0..3 levels: B0, B1, B2 hold d-1, d, d+1
-  if (d < d_end) {
-    win_load<T, S>(A, p, lb, d + 2, lane, B3);
-    acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d, lane, B0, B1, B2, acc, epi);
-    if (d + 1 < d_end) {
-      win_load<T, S>(A, p, lb, d + 3, lane, B0);
-      acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 1, lane, B1, B2, B3, acc, epi);
-      if (d + 2 < d_end) acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + 2, lane, B2, B3, B0, acc, epi);
+  const int nx = A.nx, ny = A.ny, n = a.n;
+  const WinArrays<T> V{LevelArray<T>(p, n), LevelArray<T>(A.aW, n), LevelArray<T>(A.aS, n), LevelArray<T>(A.aP, n)};
+  const LevelArray<T> extra(kExtra ? extra_ptr : p, n);
+  WinLevel<T> B[K];
+#pragma unroll
+  for (int b = 0; b < K - 1; ++b) win_load<T, S>(V, lb, d_begin - 1 + b, lane, nx, ny, B[b]);
+  win_shift<T, S>(B[0], m_p);
+  win_shift<T, S>(B[1], m_p);
+  for (int d = d_begin; d < d_end; d += K) {
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      if (d + r >= d_end) return acc;
+      win_load<T, S>(V, lb, d + r + K - 2, lane, nx, ny, B[(r + K - 1) % K]);
+      win_shift<T, S>(B[(r + 2) % K], m_p);  // level d + r + 1, first read now (as next)
+      acc = win_level<T, S, kPeriodic, kExtra>(A, p, m_p, extra, lb, d + r, lane, B[r % K], B[(r + 1) % K],
+                                                B[(r + 2) % K], acc, epi);
     }
   }
   return acc;
@@ -1044,10 +1094,10 @@ template <typename T, bool kPeriodic, bool kExtra, typename F>
 __device__ __forceinline__ double spmv_window_rows(const SysArgs &a, const Stencil<T> &A, const global_ptr<const T> p,
                                                    T m_p, const global_ptr<const T> extra, F &epi) {
   switch ((A.nx + 63) >> 6) {
-    case 1: return spmv_window<T, 1, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
-    case 2: return spmv_window<T, 2, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
-    case 3: return spmv_window<T, 3, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
-    default: return spmv_window<T, 4, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
+    case 1: return spmv_window<T, 1, kWinBuffers, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
+    case 2: return spmv_window<T, 2, kWinBuffers, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
+    case 3: return spmv_window<T, 3, kWinBuffers, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
+    default: return spmv_window<T, 4, kWinBuffers, kPeriodic, kExtra>(a, A, p, m_p, extra, epi);
   }
 }
 
@@ -1244,7 +1294,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
   double m_x = 0.0, m_p = 0.0;
   const double tol = a.tolerance;
   const global_ptr<double> p_raw = as_global(a.pa);
-  const global_ptr<double> p_prj = as_global(a.pb);
+  const LevelArray<double> apv_rows(a.y, n);
   mark(0);
 
   if (!(rel_res < tol || abs_res < tol)) {
@@ -1269,9 +1319,8 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
     const int limit = a.max_iterations < 1 ? 1 : a.max_iterations;
     for (int it = 0; it < limit; ++it) {
       // Ap and p^T A p (level-major order)
-      double pap = spmv_pass<double>(a, stencil, p_raw, m_p, nullptr, [&](int q, double apq, double pq, double) {
-        apv[q] = apq;
-        p_prj[q] = pq;
+      double pap = spmv_pass<double>(a, stencil, p_raw, m_p, nullptr, [&](int base, int i, double apq, double pq, double) {
+        apv_rows.put(base, i, apq);
         return pq * apq;
       });
       pap = block_sum1(pap, sh, parity);
@@ -1292,7 +1341,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
           const int qq = q < n ? q : 0;
           xq[u] = xv[qq];
           rq[u] = rv[qq];
-          pq[u] = p_prj[qq];
+          pq[u] = p_raw[qq] - m_p;  // the projected p the SpMV used
           aq[u] = apv[qq];
         }
 #pragma unroll
@@ -1385,7 +1434,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
           const int q = base + u * kThreads;
           const int qq = q < n ? q : 0;
           zq[u] = zv[qq];
-          pq[u] = p_prj[qq];
+          pq[u] = p_raw[qq] - m_p;  // the projected p the SpMV used
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
@@ -1459,6 +1508,8 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
   const global_ptr<float> fLw = as_global(a.fLw), fLs = as_global(a.fLs), fRd = as_global(a.fRd);
   const global_ptr<float> dv = as_global(a.fx), r32 = as_global(a.fr), p32 = as_global(a.fpa);
   const global_ptr<float> z32 = as_global(a.fz), ap32 = as_global(a.fy);
+  const LevelArray<double> r_rows(a.r, n);
+  const LevelArray<float> ap_rows(a.fy, n);
 
   // FP32 operator and factor, regularized FP64 right-hand side, x = 0
   double bb = 0.0;
@@ -1495,9 +1546,9 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
   int total = 0, cycles = 0;
   for (;;) {
     // true residual r = b - A (x - m_x) in FP64, projected
-    double rs = spmv_pass<double>(a, A64, xv, m_x, bv, [&](int q, double axq, double, double bq) {
+    double rs = spmv_pass<double>(a, A64, xv, m_x, bv, [&](int base, int i, double axq, double, double bq) {
       const double rq = bq - axq;
-      rv[q] = rq;
+      r_rows.put(base, i, rq);
       return rq;
     });
     rs = block_sum1(rs, sh, parity);
@@ -1551,8 +1602,8 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
     float m_p = 0.0f;
     clk.mark(7);
     while (isfinite(rz) && rz > 0.0 && inner < inner_cap && total < limit) {
-      double pap = spmv_pass<float>(a, A32, p32, m_p, nullptr, [&](int q, float apq, float pq, float) {
-        ap32[q] = apq;
+      double pap = spmv_pass<float>(a, A32, p32, m_p, nullptr, [&](int base, int i, float apq, float pq, float) {
+        ap_rows.put(base, i, apq);
         return static_cast<double>(pq) * static_cast<double>(apq);
       });
       pap = block_sum1(pap, sh, parity);
