@@ -54,6 +54,13 @@ struct pp_ctx {
   pp::SysArgs *d_batch_args = nullptr;
   pp::DevReport *d_batch_rep = nullptr;
   pp::DevReport *h_batch_rep = nullptr;  // pinned
+
+  // velocity correction: argument block, split-extrapolation field and speed pair
+  pp::CorrArgs *d_corr_args = nullptr;
+  size_t corr_cap = 0;          // capacity of corr_work in doubles
+  double *corr_work = nullptr;  // 2 speeds, then the ghosted extrapolation field
+  size_t field_cap = 0;         // host entry: staging capacity in doubles
+  double *field_stage = nullptr;
 };
 
 namespace {
@@ -382,9 +389,149 @@ int run_adjoint(pp_ctx *ctx, const pp_problem *pb, const double *d_ku, const dou
   return PP_OK;
 }
 
+// Velocity correction (operators.cpp:480-534): resolved boundary conditions and the launch.
+int run_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost,
+                            const double *d_rho_mid, const double *d_u_star, const double *d_v_star,
+                            const double *d_p_corr, double *d_u, double *d_v, double *d_pressure,
+                            double *d_pressure_prev, double *speeds, hipStream_t s) {
+  pp_report *report = nullptr;
+  const int nx = pb->nx, ny = pb->ny, G = ghost;
+  const size_t cells = static_cast<size_t>(nx + 2 * G) * (ny + 2 * G);
+  if (2 + cells > ctx->corr_cap) {
+    if (ctx->corr_work) (void)hipFree(ctx->corr_work);
+    ctx->corr_work = nullptr;
+    ctx->corr_cap = 0;
+    PP_HIP_CHECK(hipMalloc(&ctx->corr_work, (2 + cells) * sizeof(double)));
+    ctx->corr_cap = 2 + cells;
+  }
+  if (!ctx->d_corr_args) PP_HIP_CHECK(hipMalloc(&ctx->d_corr_args, sizeof(pp::CorrArgs)));
+  pp::CorrArgs c;
+  std::memset(&c, 0, sizeof(c));
+  c.nx = nx;
+  c.ny = ny;
+  c.G = G;
+  c.dx = pb->dx;
+  c.dy = pb->dy;
+  c.dt = pb->dt;
+  c.periodic_x = pb->periodic_x ? 1 : 0;
+  c.periodic_y = pb->periodic_y ? 1 : 0;
+  for (int k = 0; k < 4; ++k) {
+    // effective_pressure_bc: unset -> Neumann 0 (operators.cpp:36-60)
+    const bool p_unset = pb->bc_type[k] == PP_BC_UNSET;
+    c.p_type[k] = p_unset ? PP_BC_NEUMANN : pb->bc_type[k];
+    c.p_value[k] = p_unset ? 0.0 : pb->bc_value[k];
+    // effective_u_bc / effective_v_bc: unset -> no-slip, u bottom/top at the wall speed (62-112)
+    const bool u_unset = vbc->u_type[k] == PP_BC_UNSET;
+    const double u_default = k == 2 ? vbc->bottom_wall_velocity_x : (k == 3 ? vbc->top_wall_velocity_x : 0.0);
+    c.u_type[k] = u_unset ? PP_BC_DIRICHLET : vbc->u_type[k];
+    c.u_value[k] = u_unset ? u_default : vbc->u_value[k];
+    const bool v_unset = vbc->v_type[k] == PP_BC_UNSET;
+    c.v_type[k] = v_unset ? PP_BC_DIRICHLET : vbc->v_type[k];
+    c.v_value[k] = v_unset ? 0.0 : vbc->v_value[k];
+  }
+  c.split = (pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG) ? 1 : 0;
+  c.step_index = pb->step_index;
+  c.has_dirichlet = has_dirichlet(pb) ? 1 : 0;
+  c.rho_ref = c.split ? std::min(1.0, pb->density_ratio) : 1.0;
+  c.rho_mid = d_rho_mid;
+  c.u_star = d_u_star;
+  c.v_star = d_v_star;
+  c.p_corr = d_p_corr;
+  c.u = d_u;
+  c.v = d_v;
+  c.pressure = d_pressure;
+  c.pressure_prev = d_pressure_prev;
+  c.speeds = ctx->corr_work;
+  c.p_ext = ctx->corr_work + 2;
+  PP_HIP_CHECK(hipMemcpyAsync(ctx->d_corr_args, &c, sizeof(c), hipMemcpyHostToDevice, s));
+  PP_HIP_CHECK(pp::launch_velocity_correction(ctx->d_corr_args, s));
+  double sp[2] = {0.0, 0.0};
+  PP_HIP_CHECK(hipMemcpyAsync(sp, ctx->corr_work, sizeof(sp), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  if (speeds) {
+    speeds[0] = sp[0];
+    speeds[1] = sp[1];
+  }
+  ctx->last_error.clear();
+  return PP_OK;
+}
+
+int validate_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost) {
+  pp_report *report = nullptr;
+  if (int rc = validate(ctx, pb, report)) return rc;
+  if (!vbc) return fail(ctx, report, PP_ERR_ARG, "null velocity boundary conditions");
+  if (ghost < 1 || ghost > 8) return fail(ctx, report, PP_ERR_ARG, "ghost must be 1..8");
+  for (int k = 0; k < 4; ++k)
+    if (vbc->u_type[k] < PP_BC_UNSET || vbc->u_type[k] > PP_BC_NEUMANN || vbc->v_type[k] < PP_BC_UNSET ||
+        vbc->v_type[k] > PP_BC_NEUMANN)
+      return fail(ctx, report, PP_ERR_ARG, "unknown velocity boundary type");
+  return PP_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int pp_velocity_correction_device(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost,
+                                  const double *d_rho_mid, const double *d_u_star, const double *d_v_star,
+                                  const double *d_p_corr, double *d_u, double *d_v, double *d_pressure,
+                                  double *d_pressure_prev, double *speeds, void *stream) {
+  if (int rc = validate_correction(ctx, pb, vbc, ghost)) return rc;
+  pp_report *report = nullptr;
+  if (!d_rho_mid || !d_u_star || !d_v_star || !d_p_corr || !d_u || !d_v || !d_pressure || !d_pressure_prev)
+    return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return run_velocity_correction(ctx, pb, vbc, ghost, d_rho_mid, d_u_star, d_v_star, d_p_corr, d_u, d_v, d_pressure,
+                                 d_pressure_prev, speeds, s);
+}
+
+int pp_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost,
+                           const double *rho_mid, const double *u_star, const double *v_star, const double *p_corr,
+                           double *u, double *v, double *pressure, double *pressure_prev, double *speeds) {
+  if (int rc = validate_correction(ctx, pb, vbc, ghost)) return rc;
+  pp_report *report = nullptr;
+  if (!rho_mid || !u_star || !v_star || !p_corr || !u || !v || !pressure || !pressure_prev)
+    return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  const int nx = pb->nx, ny = pb->ny, G = ghost;
+  const size_t nc = static_cast<size_t>(nx + 2 * G) * (ny + 2 * G);
+  const size_t nu = static_cast<size_t>(nx + 1 + 2 * G) * (ny + 2 * G);
+  const size_t nv = static_cast<size_t>(nx + 2 * G) * (ny + 1 + 2 * G);
+  const size_t need = 5 * nc + 2 * nu + 2 * nv;
+  if (need > ctx->field_cap) {
+    if (ctx->field_stage) (void)hipFree(ctx->field_stage);
+    ctx->field_stage = nullptr;
+    ctx->field_cap = 0;
+    PP_HIP_CHECK(hipMalloc(&ctx->field_stage, need * sizeof(double)));
+    ctx->field_cap = need;
+  }
+  double *d_rho = ctx->field_stage, *d_pc = d_rho + nc, *d_p = d_pc + nc, *d_pp = d_p + nc;
+  double *d_us = d_pp + nc, *d_u = d_us + nu, *d_vs = d_u + nu, *d_v = d_vs + nv;
+  hipStream_t s = ctx->stream;
+  auto up = [&](double *dst, const double *src, size_t count) {
+    return hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, s);
+  };
+  PP_HIP_CHECK(up(d_rho, rho_mid, nc));
+  PP_HIP_CHECK(up(d_pc, p_corr, nc));
+  PP_HIP_CHECK(up(d_p, pressure, nc));
+  PP_HIP_CHECK(up(d_pp, pressure_prev, nc));
+  PP_HIP_CHECK(up(d_us, u_star, nu));
+  PP_HIP_CHECK(up(d_u, u, nu));
+  PP_HIP_CHECK(up(d_vs, v_star, nv));
+  PP_HIP_CHECK(up(d_v, v, nv));
+  if (int rc = run_velocity_correction(ctx, pb, vbc, ghost, d_rho, d_us, d_vs, d_pc, d_u, d_v, d_p, d_pp, speeds, s))
+    return rc;
+  auto down = [&](double *dst, const double *src, size_t count) {
+    return hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, s);
+  };
+  PP_HIP_CHECK(down(u, d_u, nu));
+  PP_HIP_CHECK(down(v, d_v, nv));
+  PP_HIP_CHECK(down(pressure, d_p, nc));
+  PP_HIP_CHECK(down(pressure_prev, d_pp, nc));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  return PP_OK;
+}
 
 int pp_create(pp_ctx **out, int device) {
   if (!out) return PP_ERR_ARG;
@@ -422,6 +569,9 @@ void pp_destroy(pp_ctx *ctx) {
   if (ctx->d_rep) (void)hipFree(ctx->d_rep);
   if (ctx->h_rep) (void)hipHostFree(ctx->h_rep);
   free_batch(ctx);
+  if (ctx->d_corr_args) (void)hipFree(ctx->d_corr_args);
+  if (ctx->corr_work) (void)hipFree(ctx->corr_work);
+  if (ctx->field_stage) (void)hipFree(ctx->field_stage);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -79,4 +79,27 @@ struct GradArgs {
 
 hipError_t launch_face_gradient(const GradArgs &g, hipStream_t stream);
 
+// Velocity correction and pressure-history shift on ghosted Field2D buffers (G ghost layers).
+// Boundary conditions arrive resolved: unset pressure edges as Neumann 0, unset velocity edges as
+// their no-slip / wall-speed Dirichlet defaults.
+struct CorrArgs {
+  int nx, ny, G;
+  double dx, dy, dt;
+  int periodic_x, periodic_y;
+  int p_type[4];
+  double p_value[4];
+  int u_type[4];
+  double u_value[4];
+  int v_type[4];
+  double v_value[4];
+  int split, step_index, has_dirichlet;
+  double rho_ref;
+  const double *rho_mid, *u_star, *v_star, *p_corr;
+  double *u, *v, *pressure, *pressure_prev;
+  double *p_ext;   // scratch: ghosted nx x ny field for the split extrapolation
+  double *speeds;  // [2]: boundary speed before / after
+};
+
+hipError_t launch_velocity_correction(const CorrArgs *d_args, hipStream_t stream);
+
 }  // namespace pp

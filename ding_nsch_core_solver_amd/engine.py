@@ -47,6 +47,25 @@ class PPProblem(ctypes.Structure):
     ]
 
 
+class PPVelocityBC(ctypes.Structure):
+    """pp_velocity_bc: velocity boundary conditions of the correction step."""
+    _fields_ = [
+        ("u_type", ctypes.c_int * 4), ("u_value", ctypes.c_double * 4),
+        ("v_type", ctypes.c_int * 4), ("v_value", ctypes.c_double * 4),
+        ("bottom_wall_velocity_x", ctypes.c_double), ("top_wall_velocity_x", ctypes.c_double),
+    ]
+
+
+def make_velocity_bc(vbc) -> PPVelocityBC:
+    out = PPVelocityBC()
+    for k in range(4):
+        out.u_type[k], out.u_value[k] = int(vbc.u_type[k]), float(vbc.u_value[k])
+        out.v_type[k], out.v_value[k] = int(vbc.v_type[k]), float(vbc.v_value[k])
+    out.bottom_wall_velocity_x = float(vbc.bottom_wall_velocity_x)
+    out.top_wall_velocity_x = float(vbc.top_wall_velocity_x)
+    return out
+
+
 class PPReport(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int), ("outer_cycles", ctypes.c_int),
@@ -67,7 +86,8 @@ _lib = None
 
 # Every entry point declared in include/pp_solver.h.
 EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_solve", "pp_solve_device", "pp_solve_coeffs",
-           "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device", "pp_set_tuning")
+           "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device", "pp_set_tuning",
+           "pp_velocity_correction", "pp_velocity_correction_device")
 
 
 def library_available() -> bool:
@@ -129,6 +149,13 @@ def lib():
         ip = ctypes.POINTER(ctypes.c_int)
         L.pp_set_tuning.argtypes = [vp, ctypes.c_int, ip, ip, ip]
         L.pp_set_tuning.restype = ctypes.c_int
+        vb = ctypes.POINTER(PPVelocityBC)
+        L.pp_velocity_correction_device.argtypes = [vp, ctypes.POINTER(PPProblem), vb, ctypes.c_int, vp, vp, vp, vp,
+                                                    vp, vp, vp, vp, dp, vp]
+        L.pp_velocity_correction_device.restype = ctypes.c_int
+        L.pp_velocity_correction.argtypes = [vp, ctypes.POINTER(PPProblem), vb, ctypes.c_int, dp, dp, dp, dp, dp, dp,
+                                             dp, dp, dp]
+        L.pp_velocity_correction.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -315,6 +342,33 @@ class Engine:
             err.index = bad
             raise err
         return out
+
+    def velocity_correction(self, case, vbc, rho_mid, u_star, v_star, p_corr, u, v, pressure, pressure_prev,
+                            scheme: int = PP_ICPCG, ghost: int = 3):
+        """Velocity correction and pressure-history shift on host Field2D buffers (ghost layers
+        included). Returns new (u, v, pressure, pressure_prev, (speed_pre, speed_post)); the inputs
+        are not modified."""
+        pb = make_problem(case, scheme, PP_FP64)
+        ins = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (rho_mid, u_star, v_star, p_corr)]
+        outs = [np.array(a, dtype=np.float64, copy=True).reshape(-1) for a in (u, v, pressure, pressure_prev)]
+        speeds = np.zeros(2)
+        rc = self._lib.pp_velocity_correction(self.h, ctypes.byref(pb), ctypes.byref(make_velocity_bc(vbc)), ghost,
+                                              *[_dp(a) for a in ins], *[_dp(a) for a in outs], _dp(speeds))
+        if rc != 0:
+            raise EngineError(rc, self.last_error())
+        return outs[0], outs[1], outs[2], outs[3], (float(speeds[0]), float(speeds[1]))
+
+    def velocity_correction_device(self, case, vbc, d_rho_mid, d_u_star, d_v_star, d_p_corr, d_u, d_v, d_pressure,
+                                   d_pressure_prev, scheme: int = PP_ICPCG, ghost: int = 3, stream=None):
+        """Device-pointer form; returns (speed_pre, speed_post)."""
+        pb = make_problem(case, scheme, PP_FP64)
+        speeds = np.zeros(2)
+        rc = self._lib.pp_velocity_correction_device(self.h, ctypes.byref(pb), ctypes.byref(make_velocity_bc(vbc)),
+                                                     ghost, d_rho_mid, d_u_star, d_v_star, d_p_corr, d_u, d_v,
+                                                     d_pressure, d_pressure_prev, _dp(speeds), stream)
+        if rc != 0:
+            raise EngineError(rc, self.last_error())
+        return float(speeds[0]), float(speeds[1])
 
     def last_error(self) -> str:
         return self._lib.pp_last_error(self.h).decode(errors="replace")

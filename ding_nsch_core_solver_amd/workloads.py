@@ -151,3 +151,20 @@ def divergence_source(case):
     v = np.asarray(case.v, dtype=np.float64).reshape(nx, ny + 1)
     div = (u[1:] - u[:-1]) / case.dx + (v[:, 1:] - v[:, :-1]) / case.dy
     return np.ascontiguousarray(-div / case.dt)
+
+
+@dataclasses.dataclass
+class VelocityBC:
+    """Velocity boundary conditions of the correction step (the reference Config's u_bc_* / v_bc_*
+    per edge in left, right, bottom, top order, and the wall speeds of unset u bottom/top edges)."""
+    u_type: tuple = (BC_UNSET, BC_UNSET, BC_UNSET, BC_UNSET)
+    u_value: tuple = (0.0, 0.0, 0.0, 0.0)
+    v_type: tuple = (BC_UNSET, BC_UNSET, BC_UNSET, BC_UNSET)
+    v_value: tuple = (0.0, 0.0, 0.0, 0.0)
+    bottom_wall_velocity_x: float = 0.0
+    top_wall_velocity_x: float = 0.0
+
+
+def ghosted_shape(rows: int, cols: int, ghost: int = 3) -> tuple:
+    """Shape of a Field2D buffer (rows x cols interior, `ghost` layers on every side)."""
+    return (rows + 2 * ghost, cols + 2 * ghost)

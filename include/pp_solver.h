@@ -171,6 +171,46 @@ int pp_solve_batched_device(pp_ctx *ctx, int count, const pp_problem *problems, 
                             void *stream);
 
 /*
+ * Velocity boundary conditions of the correction step: the reference Config's u_bc_* / v_bc_*
+ * (config.hpp:116-123) per edge in left, right, bottom, top order, and the wall speeds an unset u
+ * bottom / top edge takes (bottom_wall_velocity_x, top_wall_velocity_x, config.hpp:105-106).
+ * Unset edges behave as in the reference (operators.cpp:62-112): no-slip Dirichlet 0, u bottom/top
+ * at the wall speed.
+ */
+typedef struct pp_velocity_bc {
+  int u_type[4];
+  double u_value[4];
+  int v_type[4];
+  double v_value[4];
+  double bottom_wall_velocity_x;
+  double top_wall_velocity_x;
+} pp_velocity_bc;
+
+/*
+ * Velocity correction and pressure-history shift after a solve: replaces
+ * Solver::apply_pressure_velocity_correction (operators.cpp:480-534), the step solver.cpp:272 runs
+ * after solve_pressure_correction. All fields are whole Field2D buffers with `ghost` layers
+ * (element (i, j) of an a x b field at (i + ghost) * (b + 2 ghost) + j + ghost):
+ *   read:       rho_mid, p_corr (nx x ny; p_corr = pressure_correction_ with its ghosts filled, as
+ *               pp_solve's caller leaves it), u_star ((nx+1) x ny), v_star (nx x (ny+1));
+ *   read/write: u ((nx+1) x ny), v (nx x (ny+1)), pressure (pressure_), pressure_prev
+ *               (pressure_previous_step_): on entry the current fields (the reference reads old
+ *               ghost cells of u, v and pressure, and the split extrapolation reads both
+ *               pressures), on return u^{n+1}, v^{n+1}, the new pressure and the previous one.
+ * pb supplies the grid, dt, periodicity, pressure BCs, scheme (split schemes add the explicit
+ * term with rho_ref = min(1, density_ratio)) and step_index. speeds[0] / speeds[1] receive
+ * last_boundary_speed_pre_correction_ / _post_correction_ (operators.cpp:463-478). Results are
+ * the reference's bit for bit.
+ */
+int pp_velocity_correction_device(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost,
+                                  const double *d_rho_mid, const double *d_u_star, const double *d_v_star,
+                                  const double *d_p_corr, double *d_u, double *d_v, double *d_pressure,
+                                  double *d_pressure_prev, double *speeds, void *stream);
+int pp_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost,
+                           const double *rho_mid, const double *u_star, const double *v_star, const double *p_corr,
+                           double *u, double *v, double *pressure, double *pressure_prev, double *speeds);
+
+/*
  * Tuning table of the context (written by the project's autotuner, ding_nsch_core_solver_amd/
  * autotune.py): the sweep operand ring depth to use for grids of exactly nx[k] x ny[k] when
  * pp_problem.ring_depth is 0. Replaces the previous table; entries with an invalid depth are

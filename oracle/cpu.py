@@ -28,6 +28,14 @@ class OrProblem(ctypes.Structure):
     ]
 
 
+class OrVelocityBc(ctypes.Structure):
+    _fields_ = [
+        ("u_type", ctypes.c_int * 4), ("u_value", ctypes.c_double * 4),
+        ("v_type", ctypes.c_int * 4), ("v_value", ctypes.c_double * 4),
+        ("bottom_wall_velocity_x", ctypes.c_double), ("top_wall_velocity_x", ctypes.c_double),
+    ]
+
+
 def available() -> bool:
     return os.path.exists(LIB_PATH)
 
@@ -68,6 +76,9 @@ def lib():
         L.or_adjoint.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), dp, dp, dp, dp, ctypes.c_int, dp, dp, dp,
                                  ip, dp, ctypes.c_char_p, ctypes.c_int]
         L.or_adjoint.restype = ctypes.c_int
+        L.or_velocity_correction.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), ctypes.POINTER(OrVelocityBc),
+                                             ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.or_velocity_correction.restype = None
         _lib = L
     return _lib
 
@@ -209,3 +220,26 @@ class OracleSystem:
         if getattr(self, "h", None):
             lib().or_sys_free(self.h)
             self.h = None
+
+
+def velocity_bc(vbc) -> OrVelocityBc:
+    out = OrVelocityBc()
+    for k in range(4):
+        out.u_type[k], out.u_value[k] = int(vbc.u_type[k]), float(vbc.u_value[k])
+        out.v_type[k], out.v_value[k] = int(vbc.v_type[k]), float(vbc.v_value[k])
+    out.bottom_wall_velocity_x = float(vbc.bottom_wall_velocity_x)
+    out.top_wall_velocity_x = float(vbc.top_wall_velocity_x)
+    return out
+
+
+def velocity_correction(case, vbc, split: bool, ghost: int, rho_mid, u_star, v_star, p_corr, u, v, pressure,
+                        pressure_prev):
+    """Solver::apply_pressure_velocity_correction on ghosted Field2D buffers (see pp_oracle.cpp).
+    Returns the new (u, v, pressure, pressure_prev, (speed_pre, speed_post)); the inputs are not
+    modified."""
+    u, v, pressure, prev = (np.array(a, dtype=np.float64, copy=True) for a in (u, v, pressure, pressure_prev))
+    speeds = np.zeros(2)
+    args = [_f64(a) for a in (rho_mid, u_star, v_star, p_corr)]
+    lib().or_velocity_correction(int(split), ctypes.byref(problem(case)), ctypes.byref(velocity_bc(vbc)), int(ghost),
+                                 *[_dp(a) for a in args], _dp(u), _dp(v), _dp(pressure), _dp(prev), _dp(speeds))
+    return u, v, pressure, prev, (float(speeds[0]), float(speeds[1]))

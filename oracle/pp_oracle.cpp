@@ -44,6 +44,17 @@ struct OrProblem {
   int step_index;
 };
 
+// Velocity boundary conditions of the correction step (config.hpp:105-123): u and v per edge
+// (left, right, bottom, top; 0 unset, 1 dirichlet, 2 neumann) and the wall speeds that unset u
+// bottom/top edges take.
+struct OrVelocityBc {
+  int u_type[4];
+  double u_value[4];
+  int v_type[4];
+  double v_value[4];
+  double bottom_wall_velocity_x, top_wall_velocity_x;
+};
+
 }  // extern "C"
 
 namespace {
@@ -682,6 +693,38 @@ double pair_gradient(const double *lam, const double *p, std::size_t a, std::siz
   return -((lam[b] - lam[a]) * (p[b] - p[a])) / (h * h);
 }
 
+// apply_pressure_bc (operators.cpp:143-181) on a ghosted nx x ny field with G ghost layers.
+void fill_pressure_ghosts(const OrProblem &pb, int G, double *f) {
+  const Grid g(pb);
+  const int nx = pb.nx, ny = pb.ny, ld = ny + 2 * G;
+  auto at = [&](int i, int j) -> double & { return f[static_cast<std::size_t>(i + G) * ld + (j + G)]; };
+  const Bc L = g.bc(kLeft), Rb = g.bc(kRight), B = g.bc(kBottom), T = g.bc(kTop);
+  for (int k = 1; k <= G; ++k) {
+    for (int j = 0; j < ny; ++j) {
+      if (pb.periodic_x) {
+        at(-k, j) = at(nx - k, j);
+        at(nx - 1 + k, j) = at(k - 1, j);
+      } else {
+        const double lo = 2.0 * (static_cast<double>(k) - 0.5) * L.value * g.dx;
+        const double ro = 2.0 * (static_cast<double>(k) - 0.5) * Rb.value * g.dx;
+        at(-k, j) = (L.type == kDirichlet) ? 2.0 * L.value - at(k - 1, j) : at(k - 1, j) + lo;
+        at(nx - 1 + k, j) = (Rb.type == kDirichlet) ? 2.0 * Rb.value - at(nx - k, j) : at(nx - k, j) + ro;
+      }
+    }
+    for (int i = -G; i < nx + G; ++i) {
+      if (pb.periodic_y) {
+        at(i, -k) = at(i, ny - k);
+        at(i, ny - 1 + k) = at(i, k - 1);
+      } else {
+        const double bo = 2.0 * (static_cast<double>(k) - 0.5) * B.value * g.dy;
+        const double to = 2.0 * (static_cast<double>(k) - 0.5) * T.value * g.dy;
+        at(i, -k) = (B.type == kDirichlet) ? 2.0 * B.value - at(i, k - 1) : at(i, k - 1) + bo;
+        at(i, ny - 1 + k) = (T.type == kDirichlet) ? 2.0 * T.value - at(i, ny - k) : at(i, ny - k) + to;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -826,36 +869,193 @@ int or_sys_pcg(void *h, int max_iterations, double tolerance, double *x, int *it
   }
 }
 
-// Ghost fill of a pressure-like field with 3 ghost layers (operators.cpp:143-181).
-void or_apply_pressure_bc(const OrProblem *pb, double *f) {
+// Velocity correction and pressure-history shift after a pressure solve
+// (Solver::apply_pressure_velocity_correction, operators.cpp:480-534), on ghosted fields of G
+// ghost layers in Field2D layout (element (i, j) of an a x b field at (i + G) * (b + 2G) + j + G):
+// rho_mid, p_corr, pressure, pressure_prev are nx x ny; u_star, u are (nx+1) x ny; v_star, v are
+// nx x (ny+1). u, v and pressure are updated in place (their ghost cells take part, as in the
+// reference, where the y pass of a layer reads x-ghost cells of deeper layers not yet rewritten);
+// pressure_prev holds the previous step's pressure on entry (the split extrapolation reads it) and
+// the pressure before this correction, ghosts included, on return. speeds[0] / speeds[1] get the
+// periodic boundary speed before / after (operators.cpp:463-478).
+void or_velocity_correction(int split, const OrProblem *pb, const OrVelocityBc *vb, int G, const double *rho_mid,
+                            const double *u_star, const double *v_star, const double *p_corr, double *u, double *v,
+                            double *pressure, double *pressure_prev, double *speeds) {
   const Grid g(*pb);
-  const int nx = pb->nx, ny = pb->ny, G = 3, ld = ny + 2 * G;
-  auto at = [&](int i, int j) -> double & { return f[static_cast<std::size_t>(i + G) * ld + (j + G)]; };
-  const Bc L = g.bc(kLeft), Rb = g.bc(kRight), B = g.bc(kBottom), T = g.bc(kTop);
-  for (int k = 1; k <= G; ++k) {
+  const int nx = pb->nx, ny = pb->ny;
+  const int ld_c = ny + 2 * G, ld_u = ny + 2 * G, ld_v = ny + 1 + 2 * G;
+  auto cell = [&](const double *f, int i, int j) -> double { return f[static_cast<std::size_t>(i + G) * ld_c + (j + G)]; };
+  auto cref = [&](double *f, int i, int j) -> double & { return f[static_cast<std::size_t>(i + G) * ld_c + (j + G)]; };
+  auto uref = [&](double *f, int i, int j) -> double & { return f[static_cast<std::size_t>(i + G) * ld_u + (j + G)]; };
+  auto uval = [&](const double *f, int i, int j) -> double { return f[static_cast<std::size_t>(i + G) * ld_u + (j + G)]; };
+  auto vref = [&](double *f, int i, int j) -> double & { return f[static_cast<std::size_t>(i + G) * ld_v + (j + G)]; };
+  auto vval = [&](const double *f, int i, int j) -> double { return f[static_cast<std::size_t>(i + G) * ld_v + (j + G)]; };
+  const std::size_t n_cell = static_cast<std::size_t>(nx + 2 * G) * ld_c;
+
+  // periodic_boundary_speed_stat (operators.cpp:463-478)
+  auto speed = [&](const double *uf, const double *vf) {
+    double m = 0.0;
+    if (pb->periodic_x)
+      for (int j = 0; j < ny; ++j) {
+        m = std::max(m, std::abs(uval(uf, 0, j)));
+        m = std::max(m, std::abs(uval(uf, nx, j)));
+      }
+    if (pb->periodic_y)
+      for (int i = 0; i < nx; ++i) {
+        m = std::max(m, std::abs(vval(vf, i, 0)));
+        m = std::max(m, std::abs(vval(vf, i, ny)));
+      }
+    return m;
+  };
+  speeds[0] = speed(u_star, v_star);
+
+  // rho_u_face / rho_v_face (operators.cpp:330-350)
+  auto rho_u = [&](int i, int j) {
+    if (i <= 0) return pb->periodic_x ? 0.5 * (cell(rho_mid, nx - 1, j) + cell(rho_mid, 0, j)) : cell(rho_mid, 0, j);
+    if (i >= nx) return pb->periodic_x ? 0.5 * (cell(rho_mid, nx - 1, j) + cell(rho_mid, 0, j)) : cell(rho_mid, nx - 1, j);
+    return 0.5 * (cell(rho_mid, i - 1, j) + cell(rho_mid, i, j));
+  };
+  auto rho_v = [&](int i, int j) {
+    if (j <= 0) return pb->periodic_y ? 0.5 * (cell(rho_mid, i, ny - 1) + cell(rho_mid, i, 0)) : cell(rho_mid, i, 0);
+    if (j >= ny) return pb->periodic_y ? 0.5 * (cell(rho_mid, i, ny - 1) + cell(rho_mid, i, 0)) : cell(rho_mid, i, ny - 1);
+    return 0.5 * (cell(rho_mid, i, j - 1) + cell(rho_mid, i, j));
+  };
+  // split extrapolation, ghost-filled on a zeroed field (operators.cpp:413-422)
+  const double rho_ref = split ? std::min(1.0, pb->density_ratio) : 1.0;
+  std::vector<double> p_ext(n_cell, 0.0);
+  if (split) {
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        cref(p_ext.data(), i, j) = (pb->step_index == 0) ? cell(pressure, i, j)
+                                                          : (2.0 * cell(pressure, i, j) - cell(pressure_prev, i, j));
+    fill_pressure_ghosts(*pb, G, p_ext.data());
+  }
+  // u = u* - dt (implicit + explicit), implicit = grad p / rho_f (non-split) or grad p / rho_ref,
+  // explicit = (1/rho_f - 1/rho_ref) grad p_ext (split) or 0 (operators.cpp:490-519)
+  for (int i = 0; i <= nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const double ex = split ? (1.0 / rho_u(i, j) - 1.0 / rho_ref) *
+                                    ((cell(p_ext.data(), i, j) - cell(p_ext.data(), i - 1, j)) / g.dx)
+                              : 0.0;
+      const double gp = (cell(p_corr, i, j) - cell(p_corr, i - 1, j)) / g.dx;
+      const double im = split ? gp / rho_ref : gp / rho_u(i, j);
+      uref(u, i, j) = uval(u_star, i, j) - pb->dt * (im + ex);
+    }
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j <= ny; ++j) {
+      const double ex = split ? (1.0 / rho_v(i, j) - 1.0 / rho_ref) *
+                                    ((cell(p_ext.data(), i, j) - cell(p_ext.data(), i, j - 1)) / g.dy)
+                              : 0.0;
+      const double gp = (cell(p_corr, i, j) - cell(p_corr, i, j - 1)) / g.dy;
+      const double im = split ? gp / rho_ref : gp / rho_v(i, j);
+      vref(v, i, j) = vval(v_star, i, j) - pb->dt * (im + ex);
+    }
+
+  // apply_u_bc (operators.cpp:182-233); unset edges: no-slip, bottom/top at the wall speed (62-86)
+  auto ubc = [&](int s) -> Bc {
+    if (vb->u_type[s] != kUnset) return {vb->u_type[s], vb->u_value[s]};
+    if (s == kBottom) return {kDirichlet, vb->bottom_wall_velocity_x};
+    if (s == kTop) return {kDirichlet, vb->top_wall_velocity_x};
+    return {kDirichlet, 0.0};
+  };
+  {
+    const Bc L = ubc(kLeft), Rb = ubc(kRight), B = ubc(kBottom), T = ubc(kTop);
+    const int fnx = nx + 1;
     for (int j = 0; j < ny; ++j) {
       if (pb->periodic_x) {
-        at(-k, j) = at(nx - k, j);
-        at(nx - 1 + k, j) = at(k - 1, j);
+        uref(u, fnx - 1, j) = uval(u, 0, j);
       } else {
-        const double lo = 2.0 * (static_cast<double>(k) - 0.5) * L.value * g.dx;
-        const double ro = 2.0 * (static_cast<double>(k) - 0.5) * Rb.value * g.dx;
-        at(-k, j) = (L.type == kDirichlet) ? 2.0 * L.value - at(k - 1, j) : at(k - 1, j) + lo;
-        at(nx - 1 + k, j) = (Rb.type == kDirichlet) ? 2.0 * Rb.value - at(nx - k, j) : at(nx - k, j) + ro;
+        uref(u, 0, j) = (L.type == kDirichlet) ? L.value : uval(u, 1, j) + L.value * g.dx;
+        uref(u, fnx - 1, j) = (Rb.type == kDirichlet) ? Rb.value : uval(u, fnx - 2, j) + Rb.value * g.dx;
       }
     }
-    for (int i = -G; i < nx + G; ++i) {
-      if (pb->periodic_y) {
-        at(i, -k) = at(i, ny - k);
-        at(i, ny - 1 + k) = at(i, k - 1);
-      } else {
-        const double bo = 2.0 * (static_cast<double>(k) - 0.5) * B.value * g.dy;
-        const double to = 2.0 * (static_cast<double>(k) - 0.5) * T.value * g.dy;
-        at(i, -k) = (B.type == kDirichlet) ? 2.0 * B.value - at(i, k - 1) : at(i, k - 1) + bo;
-        at(i, ny - 1 + k) = (T.type == kDirichlet) ? 2.0 * T.value - at(i, ny - k) : at(i, ny - k) + to;
+    for (int k = 1; k <= G; ++k) {
+      for (int j = 0; j < ny; ++j) {
+        if (pb->periodic_x) {
+          uref(u, -k, j) = uval(u, fnx - k - 1, j);
+          uref(u, fnx - 1 + k, j) = uval(u, k, j);
+        } else {
+          const double lo = 2.0 * static_cast<double>(k) * L.value * g.dx;
+          const double ro = 2.0 * static_cast<double>(k) * Rb.value * g.dx;
+          uref(u, -k, j) = (L.type == kDirichlet) ? 2.0 * L.value - uval(u, k, j) : uval(u, k, j) + lo;
+          uref(u, fnx - 1 + k, j) = (Rb.type == kDirichlet) ? 2.0 * Rb.value - uval(u, fnx - 1 - k, j)
+                                                            : uval(u, fnx - 1 - k, j) + ro;
+        }
+      }
+      for (int i = -G; i < fnx + G; ++i) {
+        if (pb->periodic_y) {
+          uref(u, i, -k) = uval(u, i, ny - k);
+          uref(u, i, ny - 1 + k) = uval(u, i, k - 1);
+        } else {
+          const double bo = 2.0 * (static_cast<double>(k) - 0.5) * B.value * g.dy;
+          const double to = 2.0 * (static_cast<double>(k) - 0.5) * T.value * g.dy;
+          uref(u, i, -k) = (B.type == kDirichlet) ? 2.0 * B.value - uval(u, i, k - 1) : uval(u, i, k - 1) + bo;
+          uref(u, i, ny - 1 + k) = (T.type == kDirichlet) ? 2.0 * T.value - uval(u, i, ny - k) : uval(u, i, ny - k) + to;
+        }
       }
     }
   }
+  // apply_v_bc (operators.cpp:239-283); unset edges: no-slip (88-112)
+  auto vbc = [&](int s) -> Bc {
+    if (vb->v_type[s] != kUnset) return {vb->v_type[s], vb->v_value[s]};
+    return {kDirichlet, 0.0};
+  };
+  {
+    const Bc L = vbc(kLeft), Rb = vbc(kRight), B = vbc(kBottom), T = vbc(kTop);
+    const int fny = ny + 1;
+    for (int i = 0; i < nx; ++i) {
+      if (pb->periodic_y) {
+        vref(v, i, fny - 1) = vval(v, i, 0);
+      } else {
+        vref(v, i, 0) = (B.type == kDirichlet) ? B.value : vval(v, i, 1) + B.value * g.dy;
+        vref(v, i, fny - 1) = (T.type == kDirichlet) ? T.value : vval(v, i, fny - 2) + T.value * g.dy;
+      }
+    }
+    for (int k = 1; k <= G; ++k) {
+      for (int j = 0; j < fny; ++j) {
+        if (pb->periodic_x) {
+          vref(v, -k, j) = vval(v, nx - k, j);
+          vref(v, nx - 1 + k, j) = vval(v, k - 1, j);
+        } else {
+          const double lo = 2.0 * (static_cast<double>(k) - 0.5) * L.value * g.dx;
+          const double ro = 2.0 * (static_cast<double>(k) - 0.5) * Rb.value * g.dx;
+          vref(v, -k, j) = (L.type == kDirichlet) ? 2.0 * L.value - vval(v, k - 1, j) : vval(v, k - 1, j) + lo;
+          vref(v, nx - 1 + k, j) = (Rb.type == kDirichlet) ? 2.0 * Rb.value - vval(v, nx - k, j) : vval(v, nx - k, j) + ro;
+        }
+      }
+      for (int i = -G; i < nx + G; ++i) {
+        if (pb->periodic_y) {
+          vref(v, i, -k) = vval(v, i, fny - k - 1);
+          vref(v, i, fny - 1 + k) = vval(v, i, k);
+        } else {
+          const double bo = 2.0 * static_cast<double>(k) * B.value * g.dy;
+          const double to = 2.0 * static_cast<double>(k) * T.value * g.dy;
+          vref(v, i, -k) = (B.type == kDirichlet) ? 2.0 * B.value - vval(v, i, k) : vval(v, i, k) + bo;
+          vref(v, i, fny - 1 + k) = (T.type == kDirichlet) ? 2.0 * T.value - vval(v, i, fny - 1 - k)
+                                                           : vval(v, i, fny - 1 - k) + to;
+        }
+      }
+    }
+  }
+  speeds[1] = speed(u, v);
+
+  // pressure history (operators.cpp:524-533): previous <- pressure; pressure <- p_corr (interior),
+  // mean removed when no edge is Dirichlet (subtract_mean, 447-460), ghosts refilled
+  std::copy(pressure, pressure + n_cell, pressure_prev);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) cref(pressure, i, j) = cell(p_corr, i, j);
+  if (!g.has_dirichlet()) {
+    double mean = 0.0;
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) mean += cell(pressure, i, j);
+    mean /= static_cast<double>(nx * ny);
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) cref(pressure, i, j) -= mean;
+  }
+  fill_pressure_ghosts(*pb, G, pressure);
 }
+
+// Ghost fill of a pressure-like field with 3 ghost layers (operators.cpp:143-181).
+void or_apply_pressure_bc(const OrProblem *pb, double *f) { fill_pressure_ghosts(*pb, 3, f); }
 
 }  // extern "C"

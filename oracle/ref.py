@@ -27,6 +27,14 @@ class RefProblem(ctypes.Structure):
     ]
 
 
+class RefVelocityBc(ctypes.Structure):
+    _fields_ = [
+        ("u_type", ctypes.c_int * 4), ("u_value", ctypes.c_double * 4),
+        ("v_type", ctypes.c_int * 4), ("v_value", ctypes.c_double * 4),
+        ("bottom_wall_velocity_x", ctypes.c_double), ("top_wall_velocity_x", ctypes.c_double),
+    ]
+
+
 def available() -> bool:
     return os.path.exists(LIB_PATH)
 
@@ -56,6 +64,10 @@ def lib():
         L.ref_sys_pcg.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, dp, ip, dp, dp,
                                   ctypes.c_char_p, ctypes.c_int]
         L.ref_sys_pcg.restype = ctypes.c_int
+        L.ref_velocity_correction.argtypes = [ctypes.c_char_p, ctypes.POINTER(RefProblem), ctypes.POINTER(RefVelocityBc),
+                                              ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp, ctypes.c_char_p,
+                                              ctypes.c_int]
+        L.ref_velocity_correction.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -157,3 +169,25 @@ class RefSystem:
         if getattr(self, "h", None):
             lib().ref_sys_free(self.h)
             self.h = None
+
+
+def velocity_correction(scheme: str, case, vbc, ghost: int, rho_mid, u_star, v_star, p_corr, u, v, pressure,
+                        pressure_prev):
+    """The reference's apply_pressure_velocity_correction on ghosted Field2D buffers.
+    Returns the new (u, v, pressure, pressure_prev, (speed_pre, speed_post))."""
+    b = RefVelocityBc()
+    for k in range(4):
+        b.u_type[k], b.u_value[k] = int(vbc.u_type[k]), float(vbc.u_value[k])
+        b.v_type[k], b.v_value[k] = int(vbc.v_type[k]), float(vbc.v_value[k])
+    b.bottom_wall_velocity_x = float(vbc.bottom_wall_velocity_x)
+    b.top_wall_velocity_x = float(vbc.top_wall_velocity_x)
+    u, v, pressure, prev = (np.array(a, dtype=np.float64, copy=True) for a in (u, v, pressure, pressure_prev))
+    speeds = np.zeros(2)
+    ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho_mid, u_star, v_star, p_corr)]
+    err = ctypes.create_string_buffer(512)
+    rc = lib().ref_velocity_correction(scheme.encode(), ctypes.byref(problem(case)), ctypes.byref(b), int(ghost),
+                                       *[_dp(a) for a in ins], _dp(u), _dp(v), _dp(pressure), _dp(prev), _dp(speeds),
+                                       err, 512)
+    if rc:
+        raise RuntimeError(err.value.decode())
+    return u, v, pressure, prev, (float(speeds[0]), float(speeds[1]))

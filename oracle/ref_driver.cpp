@@ -141,6 +141,65 @@ int ref_pressure_solve(const char *scheme, const RefProblem *pb, const double *r
   }
 }
 
+struct RefVelocityBc {
+  int u_type[4];      // left, right, bottom, top; 0 unset, 1 dirichlet, 2 neumann
+  double u_value[4];
+  int v_type[4];
+  double v_value[4];
+  double bottom_wall_velocity_x, top_wall_velocity_x;
+};
+
+// One call of Solver::apply_pressure_velocity_correction() (operators.cpp:480-534) on whole
+// Field2D buffers with G ghost layers: rho_mid, u_star, v_star, p_corr are read; u, v, pressure
+// and pressure_prev (pressure_previous_step_) are read and written.
+// speeds[0] / speeds[1] receive the periodic boundary speed before / after the correction.
+int ref_velocity_correction(const char *scheme, const RefProblem *pb, const RefVelocityBc *vb, int G,
+                            const double *rho_mid, const double *u_star, const double *v_star, const double *p_corr,
+                            double *u, double *v, double *pressure, double *pressure_prev, double *speeds, char *err,
+                            int errlen) {
+  try {
+    ding::Config cfg = make_config(*pb, scheme);
+    cfg.ghost = G;
+    ding::BoundaryConditionSpec *ub[4] = {&cfg.u_bc_left, &cfg.u_bc_right, &cfg.u_bc_bottom, &cfg.u_bc_top};
+    ding::BoundaryConditionSpec *vbs[4] = {&cfg.v_bc_left, &cfg.v_bc_right, &cfg.v_bc_bottom, &cfg.v_bc_top};
+    for (int k = 0; k < 4; ++k) {
+      ub[k]->type = static_cast<ding::BoundaryConditionType>(vb->u_type[k]);
+      ub[k]->value = vb->u_value[k];
+      vbs[k]->type = static_cast<ding::BoundaryConditionType>(vb->v_type[k]);
+      vbs[k]->value = vb->v_value[k];
+    }
+    cfg.bottom_wall_velocity_x = vb->bottom_wall_velocity_x;
+    cfg.top_wall_velocity_x = vb->top_wall_velocity_x;
+    auto s = std::make_unique<ding::Solver>(cfg);
+    auto in = [](ding::Field2D &f, const double *src) {
+      std::memcpy(f.data.data(), src, f.data.size() * sizeof(double));
+    };
+    auto out = [](const ding::Field2D &f, double *dst) {
+      std::memcpy(dst, f.data.data(), f.data.size() * sizeof(double));
+    };
+    in(s->rho_mid_, rho_mid);
+    in(s->u_star_, u_star);
+    in(s->v_star_, v_star);
+    in(s->pressure_correction_, p_corr);
+    in(s->u_, u);
+    in(s->v_, v);
+    in(s->pressure_, pressure);
+    in(s->pressure_previous_step_, pressure_prev);
+    s->current_step_index_ = pb->step_index;
+    s->apply_pressure_velocity_correction();
+    out(s->u_, u);
+    out(s->v_, v);
+    out(s->pressure_, pressure);
+    out(s->pressure_previous_step_, pressure_prev);
+    speeds[0] = s->last_boundary_speed_pre_correction_;
+    speeds[1] = s->last_boundary_speed_post_correction_;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
 // Ghost fill of a pressure-like field (Field2D buffer with 3 ghost layers) by apply_pressure_bc.
 int ref_apply_pressure_bc(const RefProblem *pb, double *field_ghosted) {
   auto s = std::make_unique<ding::Solver>(make_config(*pb, "icpcg"));

@@ -71,3 +71,13 @@ def test_report_struct_layout_matches_header():
     body = text[text.index("typedef struct pp_report"):text.index("} pp_report;")]
     for n in names:
         assert re.search(rf"\b{n}\b", body), n
+
+
+def test_velocity_bc_struct_layout_matches_header():
+    from ding_nsch_core_solver_amd import engine
+    names = [f[0] for f in engine.PPVelocityBC._fields_]
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct pp_velocity_bc"):text.index("} pp_velocity_bc;")]
+    fields = re.findall(r"\b(?:int|double)\s+([a-z_]+)", body)
+    assert names == fields
+    assert ctypes.sizeof(engine.PPVelocityBC) == 4 * 4 + 4 * 8 + 4 * 4 + 4 * 8 + 2 * 8

@@ -1,0 +1,98 @@
+"""HIP velocity correction and pressure-history shift (pp_velocity_correction[_device]) against the
+oracle's restatement of Solver::apply_pressure_velocity_correction (operators.cpp:480-534), which
+tests/test_velocity_oracle.py pins bit for bit to the compiled reference. Same inputs, bitwise
+equal outputs: u, v, pressure, pressure_prev (ghost cells included) and the boundary speeds."""
+import zlib
+
+import numpy as np
+import pytest
+
+from ding_nsch_core_solver_amd import engine as E
+from ding_nsch_core_solver_amd import workloads as W
+from tests.test_velocity_oracle import CASES, G, PRESSURE_BC, VELOCITY_BC, _case, fields
+
+pytestmark = pytest.mark.gpu
+
+CODES = {"icpcg": E.PP_ICPCG, "split_icpcg": E.PP_SPLIT_ICPCG}
+
+
+def _check(got, expect):
+    for name, a, b in zip(("u", "v", "pressure", "pressure_prev"), got[:4], expect[:4]):
+        assert np.array_equal(a, b), name
+    assert got[4] == expect[4]
+
+
+@pytest.mark.parametrize("pname,vname,scheme_step", CASES, ids=[f"{p}-{v}-{s}{k}" for p, v, (s, k) in CASES])
+def test_engine_matches_oracle(engine, oracle_lib, pname, vname, scheme_step):
+    scheme, step = scheme_step
+    case = _case(pname, step=step)
+    f = fields(case, seed=zlib.crc32(f"{pname}/{vname}/{scheme}/{step}".encode()))
+    vbc = VELOCITY_BC[vname]
+    expect = oracle_lib.velocity_correction(case, vbc, scheme.startswith("split"), G, **f)
+    got = engine.velocity_correction(case, vbc, scheme=CODES[scheme], ghost=G, **f)
+    _check(got, expect)
+
+
+@pytest.mark.parametrize("shape", [(1, 5), (4, 4), (70, 20), (130, 12), (33, 200)])
+def test_engine_matches_oracle_shapes(engine, oracle_lib, shape):
+    case = W.droplet_case(*shape, seed=1, density_ratio=7.0, periodic_x=shape[0] >= 4, step_index=1)
+    f = fields(case, seed=shape[0] * 1000 + shape[1])
+    vbc = VELOCITY_BC["neumann_outflow"]
+    for scheme in ("icpcg", "split_icpcg"):
+        expect = oracle_lib.velocity_correction(case, vbc, scheme == "split_icpcg", G, **f)
+        got = engine.velocity_correction(case, vbc, scheme=CODES[scheme], ghost=G, **f)
+        _check(got, expect)
+
+
+def test_ghost_depth_two(engine, oracle_lib):
+    case = W.droplet_case(12, 9, seed=2, density_ratio=5.0, periodic_y=True, step_index=2)
+    g = 2
+    rng = np.random.default_rng(11)
+    cell = W.ghosted_shape(12, 9, g)
+    f = dict(rho_mid=(1.0 + rng.random(cell)).reshape(-1), u_star=rng.standard_normal(W.ghosted_shape(13, 9, g)).reshape(-1),
+             v_star=rng.standard_normal(W.ghosted_shape(12, 10, g)).reshape(-1),
+             p_corr=rng.standard_normal(cell).reshape(-1), u=rng.standard_normal(W.ghosted_shape(13, 9, g)).reshape(-1),
+             v=rng.standard_normal(W.ghosted_shape(12, 10, g)).reshape(-1), pressure=rng.standard_normal(cell).reshape(-1),
+             pressure_prev=rng.standard_normal(cell).reshape(-1))
+    vbc = VELOCITY_BC["dirichlet_values"]
+    _check(engine.velocity_correction(case, vbc, scheme=E.PP_SPLIT_ICPCG, ghost=g, **f),
+           oracle_lib.velocity_correction(case, vbc, True, g, **f))
+
+
+def test_device_entry_after_a_solve(engine, oracle_lib, ref_lib):
+    """The timestep's pair on the 192x192 density-ratio-1000 workload: the solve's pressure (ghosts
+    filled as the reference backend leaves them) feeds the device correction, all buffers resident;
+    equal to the oracle on the same inputs, and to the compiled reference."""
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    case = W.droplet_case(192, 192, seed=0)
+    p, _ = engine.solve(case, E.PP_ICPCG)
+    nx, ny = case.nx, case.ny
+    pc = np.zeros(W.ghosted_shape(nx, ny, G))
+    pc[G:-G, G:-G] = p.reshape(nx, ny)
+    oracle_lib.apply_pressure_bc(case, pc.reshape(-1))
+    f = fields(case, seed=5)
+    f["p_corr"] = pc.reshape(-1)
+    rho = np.zeros(W.ghosted_shape(nx, ny, G))
+    rho[G:-G, G:-G] = case.rho.reshape(nx, ny)
+    f["rho_mid"] = rho.reshape(-1)
+    vbc = W.VelocityBC()
+    t = {k: torch.from_numpy(np.ascontiguousarray(a)).to(dev) for k, a in f.items()}
+    speeds = engine.velocity_correction_device(case, vbc, t["rho_mid"].data_ptr(), t["u_star"].data_ptr(),
+                                               t["v_star"].data_ptr(), t["p_corr"].data_ptr(), t["u"].data_ptr(),
+                                               t["v"].data_ptr(), t["pressure"].data_ptr(),
+                                               t["pressure_prev"].data_ptr(), scheme=E.PP_ICPCG, ghost=G,
+                                               stream=torch.cuda.current_stream(dev).cuda_stream)
+    got = (t["u"].cpu().numpy(), t["v"].cpu().numpy(), t["pressure"].cpu().numpy(), t["pressure_prev"].cpu().numpy(),
+           speeds)
+    _check(got, oracle_lib.velocity_correction(case, vbc, False, G, **f))
+    _check(got, ref_lib.velocity_correction("icpcg", case, vbc, G, **f))
+
+
+def test_bad_arguments_rejected(engine):
+    case = _case("unset")
+    f = fields(case, seed=0)
+    with pytest.raises(E.EngineError, match="ghost"):
+        engine.velocity_correction(case, W.VelocityBC(), ghost=0, **f)
+    with pytest.raises(E.EngineError, match="velocity boundary type"):
+        engine.velocity_correction(case, W.VelocityBC(u_type=(7, 0, 0, 0)), ghost=G, **f)
