@@ -32,6 +32,7 @@ TRAFFIC_FILE = os.path.join(REPO, "profiles", "fp64_solve_192x192_dr1000_traffic
 MIXED_TRAFFIC_FILE = os.path.join(REPO, "profiles", "mixed_solve_192x192_dr1000_traffic.txt")
 BATCH_TRAFFIC_FILE = os.path.join(REPO, "profiles", "batched_256_uneven_mixed_traffic.txt")
 BATCH64_TRAFFIC_FILE = os.path.join(REPO, "profiles", "batched_256_uneven_fp64_traffic.txt")
+CORR_TRAFFIC_FILE = os.path.join(REPO, "profiles", "velocity_correction_192x192_traffic.txt")
 
 
 def log(msg):
@@ -164,6 +165,83 @@ def batched_leg(eng, torch, dev, repeats):
     return out, cases
 
 
+def correction_bytes(n, split=False):
+    """Algorithmic bytes of one velocity correction on n cells: rho, p_corr, u*, v* and the old
+    pressure read once; u, v, the new pressure and the previous pressure written once (the split
+    scheme also reads the previous pressure)."""
+    return (80 if split else 72) * n
+
+
+def correction_leg(eng, torch, dev, case, p_solve, steps, cpu_seconds, cpu=True):
+    """The step after the solve on the same workload: velocity correction and pressure-history shift
+    (Solver::apply_pressure_velocity_correction) with 3 ghost layers, the solve's pressure (ghosts
+    mirrored as Neumann-0 walls leave them), no-slip velocity walls, non-split icpcg. All fields
+    resident in HBM; one pp_velocity_correction_device call per step."""
+    nx, ny, G = case.nx, case.ny, 3
+    rng = np.random.default_rng(7)
+    pad = lambda a, shape: np.pad(np.asarray(a, dtype=np.float64).reshape(shape), G, mode="symmetric")  # noqa: E731
+    host = {"rho_mid": pad(case.rho, (nx, ny)), "u_star": pad(case.u, (nx + 1, ny)),
+            "v_star": pad(case.v, (nx, ny + 1)), "p_corr": pad(p_solve, (nx, ny)),
+            "u": np.zeros(W.ghosted_shape(nx + 1, ny, G)), "v": np.zeros(W.ghosted_shape(nx, ny + 1, G)),
+            "pressure": rng.standard_normal(W.ghosted_shape(nx, ny, G)),
+            "pressure_prev": rng.standard_normal(W.ghosted_shape(nx, ny, G))}
+    host = {k: np.ascontiguousarray(a).reshape(-1) for k, a in host.items()}
+    dt = {k: torch.from_numpy(a.copy()).to(dev) for k, a in host.items()}
+    order = ("rho_mid", "u_star", "v_star", "p_corr", "u", "v", "pressure", "pressure_prev")
+    vbc = W.VelocityBC()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def run():
+        return eng.velocity_correction_device(case, vbc, *(dt[k].data_ptr() for k in order), scheme=E.PP_ICPCG,
+                                              ghost=G, stream=stream)
+
+    speeds = run()
+    first = {k: dt[k].cpu().numpy() for k in ("u", "v", "pressure", "pressure_prev")}
+    for _ in range(3):
+        run()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+        kernel_ms.append(eng.last_kernel_ms())
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t0) / steps
+    kms = float(np.mean(kernel_ms))
+    alg = correction_bytes(case.n)
+    out = {"value": 1.0 / elapsed, "unit": "corrections/s", "ms_per_step": elapsed * 1e3, "kernel_ms": kms,
+           "kernels": "correction_faces_kernel + correction_history_kernel + correction_finish_kernel",
+           "mean_chain_terms": case.n,
+           "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": read_traffic(CORR_TRAFFIC_FILE),
+                        "algorithmic_bytes_per_launch": alg,
+                        "note": "latency-bound: the mean is one dependent FP64 add per cell in the reference's order"}}
+    if cpu:
+        try:
+            from oracle import ref
+            if not ref.available():
+                raise FileNotFoundError
+        except (ImportError, FileNotFoundError):
+            return out
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+        try:
+            expect = ref.velocity_correction("icpcg", case, vbc, G, **host)
+            reps = max(10, min(2000, int(cpu_seconds / 1e-3)))
+            timing = []
+            ref.velocity_correction("icpcg", case, vbc, G, **host, repeats=reps, timing=timing)
+        finally:
+            os.sched_setaffinity(0, old)
+        out["parity"] = {"bitwise_equal_to_reference": bool(
+            all(np.array_equal(first[k], e) for k, e in zip(("u", "v", "pressure", "pressure_prev"), expect[:4]))
+            and tuple(speeds) == tuple(expect[4]))}
+        out["cpu_baseline"] = {"value": 1.0 / timing[0], "unit": "corrections/s", "cores": 1, "kind": "reference",
+                               "sample": f"{reps} calls of apply_pressure_velocity_correction on the same inputs, "
+                                         f"median {timing[0] * 1e6:.0f} us (solver set-up excluded)"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def batched_cpu_baseline(cases, threads):
     """Reference icpcg on every system of the batch, one system per thread (the reference is
     single-threaded; independent solves run concurrently), wall time of the whole batch."""
@@ -281,6 +359,12 @@ def main():
                                   "traffic": read_traffic(MIXED_TRAFFIC_FILE), "algorithmic_bytes_per_launch": m_bytes}}
     log(f"[bench] mixed: {line['mixed']['value']:.2f} solves/s, kernel {m_kernel:.2f} ms, {m_it} inner iterations in "
         f"{m_cyc} cycles")
+
+    log("[bench] velocity correction after the solve")
+    line["velocity_correction"] = correction_leg(eng, torch, dev, case, p_gpu, args.steps * 5, 2.0,
+                                                 cpu=not args.no_cpu_baseline)
+    log(f"[bench] correction: {line['velocity_correction']['value']:.0f} /s, kernels "
+        f"{line['velocity_correction']['kernel_ms'] * 1e3:.0f} us")
 
     batch_cases = None
     if not args.no_batched:

@@ -2,13 +2,17 @@
 // (Solver::apply_pressure_velocity_correction, operators.cpp:480-534), on the device, on ghosted
 // Field2D buffers.
 //
-// One workgroup runs the whole step: the face updates are parallel over faces, the boundary-face
-// rules and the ghost layers follow the reference's pass order (layer by layer: x ghosts, then y
-// ghosts over the full ghosted x range). The y pass of a layer reads x-ghost cells of deeper layers
-// that the reference has not rewritten yet, i.e. the caller's old values: the buffers are updated
-// in place, as the reference updates its fields. The mean removed from the new pressure is summed
-// by one lane in the reference's i-outer, j-inner order, so the result is the reference's bit for
-// bit.
+// Three launches on the caller's stream:
+//   faces    one thread per u and v face (the split extrapolation evaluated in place), and, in
+//            block 0, the mean of the correction summed by one lane in the reference's i-outer,
+//            j-inner order (terms staged through LDS by the block's other waves), so it is the
+//            reference's value bit for bit;
+//   history  one thread per cell: pressure_prev <- pressure, new pressure interior;
+//   finish   one block: boundary speeds, boundary faces and ghost layers in the reference's pass
+//            order (layer by layer: x ghosts, then y ghosts over the full ghosted x range).
+// The y pass of a layer reads x-ghost cells of deeper layers that the reference has not rewritten
+// yet, i.e. the caller's old values: the buffers are updated in place, as the reference updates
+// its fields.
 #include <hip/hip_runtime.h>
 
 #include "pp_device.hpp"
@@ -19,18 +23,27 @@ namespace pp {
 
 namespace {
 
-// Field2D element (i, j) of a field with `cols` interior columns and G ghost layers.
+constexpr int kSumChunk = 1024;  // terms per LDS stage of the sequential mean (two stages, 16 KB)
+constexpr int kSumGroup = 16;   // terms the summing lane loads per step (kSumChunk: a multiple of 2x)
+constexpr int kWaveSize = 64;
+constexpr int kFaceBlock = 256;  // threads per block of the face and history kernels
+constexpr int kMeanSlot = 2;     // work[0], work[1]: boundary speeds; work[2]: the pressure mean
+
+// Field2D element (i, j) of a field with `cols` interior columns and G ghost layers (global
+// address space, so the accesses are global_load / global_store rather than flat ones).
 struct Ghosted {
-  double *f;
+  global_ptr<double> f;
   int cols, G;
-  __device__ __forceinline__ double &operator()(int i, int j) const {
+  __device__ Ghosted(double *p, int c, int g) : f(as_global(p)), cols(c), G(g) {}
+  __device__ __forceinline__ __attribute__((address_space(1))) double &operator()(int i, int j) const {
     return f[static_cast<size_t>(i + G) * (cols + 2 * G) + (j + G)];
   }
 };
 
 struct GhostedConst {
-  const double *f;
+  global_ptr<const double> f;
   int cols, G;
+  __device__ GhostedConst(const double *p, int c, int g) : f(as_global(p)), cols(c), G(g) {}
   __device__ __forceinline__ double operator()(int i, int j) const {
     return f[static_cast<size_t>(i + G) * (cols + 2 * G) + (j + G)];
   }
@@ -195,93 +208,195 @@ __device__ __forceinline__ double rho_v_face(const CorrArgs &a, const GhostedCon
   return 0.5 * (r(i, j - 1) + r(i, j));
 }
 
-}  // namespace
+// The split extrapolation p_ext (operators.cpp:413-422) at (i, j), read where the face updates
+// need it: interior cells (p^n on the first step, else 2 p^n - p^{n-1}) and the first ghost layer
+// off the corners, which apply_pressure_bc fills from the interior (periodic copy, Dirichlet
+// 2v - f, Neumann f + v h), evaluated here with the same expressions as pressure_ghosts.
+__device__ __forceinline__ double p_ext_interior(const CorrArgs &a, const GhostedConst &p, const GhostedConst &pm,
+                                                 int i, int j) {
+  return a.step_index == 0 ? p(i, j) : 2.0 * p(i, j) - pm(i, j);
+}
 
-__global__ void __launch_bounds__(kThreads) velocity_correction_kernel(const CorrArgs *__restrict__ args) {
-  __shared__ double red[kWaves];
-  __shared__ double mean_sh;
-  const CorrArgs &a = *args;
-  const int nx = a.nx, ny = a.ny, G = a.G, tid = threadIdx.x;
-  const GhostedConst rho{a.rho_mid, ny, G}, p_corr{a.p_corr, ny, G};
-  const GhostedConst u_star{a.u_star, ny, G}, v_star{a.v_star, ny + 1, G};
-  const Ghosted u{a.u, ny, G}, v{a.v, ny + 1, G}, pres{a.pressure, ny, G}, prev{a.pressure_prev, ny, G};
-  const Ghosted p_ext{a.p_ext, ny, G};
+__device__ double p_ext_at(const CorrArgs &a, const GhostedConst &p, const GhostedConst &pm, int i, int j) {
+  const int nx = a.nx, ny = a.ny;
+  const double layer = 2.0 * (1.0 - 0.5);
+  if (i < 0 || i >= nx) {
+    const int side = i < 0 ? 0 : 1;
+    if (a.periodic_x) return p_ext_interior(a, p, pm, i < 0 ? nx - 1 : 0, j);
+    const double f = p_ext_interior(a, p, pm, i < 0 ? 0 : nx - 1, j);
+    return a.p_type[side] == PP_BC_DIRICHLET ? 2.0 * a.p_value[side] - f : f + layer * a.p_value[side] * a.dx;
+  }
+  if (j < 0 || j >= ny) {
+    const int side = j < 0 ? 2 : 3;
+    if (a.periodic_y) return p_ext_interior(a, p, pm, i, j < 0 ? ny - 1 : 0);
+    const double f = p_ext_interior(a, p, pm, i, j < 0 ? 0 : ny - 1);
+    return a.p_type[side] == PP_BC_DIRICHLET ? 2.0 * a.p_value[side] - f : f + layer * a.p_value[side] * a.dy;
+  }
+  return p_ext_interior(a, p, pm, i, j);
+}
 
-  const double speed_pre = boundary_speed(a, u_star, v_star, red);
-
-  // split extrapolation p_ext = p^n (first step) or 2 p^n - p^{n-1}, ghost-filled on a zeroed field
-  // (operators.cpp:413-422)
-  if (a.split) {
-    const size_t cells = static_cast<size_t>(nx + 2 * G) * (ny + 2 * G);
-    for (size_t q = tid; q < cells; q += kThreads) a.p_ext[q] = 0.0;
-    __syncthreads();
-    for (int q = tid; q < nx * ny; q += kThreads) {
-      const int i = q / ny, j = q - i * ny;
-      p_ext(i, j) = a.step_index == 0 ? pres(i, j) : 2.0 * pres(i, j) - prev(i, j);
+// Sum of the nx x ny interior of f in i-outer, j-inner order with one rounding per term, as a
+// plain sequential loop computes it (the reference's mean, operators.cpp:526-532). The additions
+// form one dependent chain, so the chain runs in one lane; waves 1..7 stage the next chunk of
+// terms into LDS while lane 0 adds the current one. The tail of the last chunk is padded with
+// +0.0, which leaves the running sum unchanged bit for bit (it starts at +0.0 and round-to-nearest
+// never produces -0.0 from a sum of a nonzero and its negation). Runs in a kFaceBlock-thread
+// block; returns the sum in thread 0.
+__device__ double sequential_sum(const GhostedConst &f, int nx, int ny, double (*stage)[kSumChunk]) {
+  const int tid = threadIdx.x, n = nx * ny;
+  const int chunks = (n + kSumChunk - 1) / kSumChunk;
+  constexpr int kStagers = kFaceBlock - kWaveSize;
+  constexpr int kPerStager = (kSumChunk + kStagers - 1) / kStagers;
+  // every load of a chunk is issued before the first LDS store (term indices clamped into the grid)
+  auto fill = [&](int c) {
+    const int base = c * kSumChunk, m = min(kSumChunk, n - base);
+    const int padded = (m + 2 * kSumGroup - 1) / (2 * kSumGroup) * (2 * kSumGroup);
+    double *dst = stage[c & 1];
+    double val[kPerStager];
+#pragma unroll
+    for (int k = 0; k < kPerStager; ++k) {
+      const int e = min(base + tid - kWaveSize + k * kStagers, n - 1), i = e / ny, j = e - i * ny;
+      val[k] = f(i, j);
+    }
+#pragma unroll
+    for (int k = 0; k < kPerStager; ++k) {
+      const int q = tid - kWaveSize + k * kStagers;
+      if (q < padded) dst[q] = q < m ? val[k] : 0.0;
+    }
+  };
+  if (tid >= kWaveSize) fill(0);
+  __syncthreads();
+  double acc = 0.0;
+  for (int c = 0; c < chunks; ++c) {
+    if (tid >= kWaveSize) {
+      if (c + 1 < chunks) fill(c + 1);
+    } else if (tid == 0) {
+      // groups A and B of kSumGroup terms alternate: B is loaded while A is added, then the next
+      // A while B is added. Volatile loads and scheduling barriers keep the loads one group ahead
+      // of the additions (plain loads get folded into a load at their use).
+      typedef double pair_t __attribute__((ext_vector_type(2)));
+      const volatile __attribute__((address_space(3))) pair_t *src =
+          reinterpret_cast<const volatile __attribute__((address_space(3))) pair_t *>(lds<double>(stage[c & 1]));
+      constexpr int kVec = kSumGroup / 2;
+      const int pairs = (min(kSumChunk, n - c * kSumChunk) + 2 * kSumGroup - 1) / (2 * kSumGroup);
+      pair_t A[kVec], B[kVec];
+#pragma unroll
+      for (int r = 0; r < kVec; ++r) A[r] = src[r];
+      for (int p = 0; p < pairs; ++p) {
+#pragma unroll
+        for (int r = 0; r < kVec; ++r) B[r] = src[2 * kVec * p + kVec + r];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < kVec; ++r) {
+          acc += A[r].x;
+          acc += A[r].y;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const int next = p + 1 < pairs ? p + 1 : p;
+#pragma unroll
+        for (int r = 0; r < kVec; ++r) A[r] = src[2 * kVec * next + r];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < kVec; ++r) {
+          acc += B[r].x;
+          acc += B[r].y;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
     __syncthreads();
-    pressure_ghosts(a, p_ext);
   }
-  const GhostedConst pe{a.p_ext, ny, G};
+  return acc;
+}
 
-  // u = u* - dt (implicit + explicit) (operators.cpp:490-519)
-  const int n_u = (nx + 1) * ny;
-  for (int q = tid; q < n_u; q += kThreads) {
+}  // namespace
+
+// Block 0: the sequential mean of the correction (when no pressure edge is Dirichlet) into
+// work[kMeanSlot]. Other blocks: one face each, u = u* - dt (implicit + explicit)
+// (operators.cpp:490-519), every face of the (nx+1) x ny and nx x (ny+1) fields including the
+// boundary faces, which the finish kernel then overwrites as the reference does.
+__global__ void __launch_bounds__(kFaceBlock) correction_faces_kernel(const CorrArgs *__restrict__ args) {
+  __shared__ __align__(16) double stage[2][kSumChunk];
+  const CorrArgs &a = *args;
+  const int nx = a.nx, ny = a.ny, G = a.G;
+  const GhostedConst rho{a.rho_mid, ny, G}, p_corr{a.p_corr, ny, G};
+  if (blockIdx.x == 0) {
+    if (!a.has_dirichlet) {
+      const double sum = sequential_sum(p_corr, nx, ny, stage);
+      if (threadIdx.x == 0) a.work[kMeanSlot] = sum / static_cast<double>(nx * ny);
+    }
+    return;
+  }
+  const GhostedConst pres{a.pressure, ny, G}, prev{a.pressure_prev, ny, G};
+  const int q = (blockIdx.x - 1) * kFaceBlock + static_cast<int>(threadIdx.x);
+  const int n_u = (nx + 1) * ny, n_v = nx * (ny + 1);
+  if (q < n_u) {
+    const GhostedConst u_star{a.u_star, ny, G};
     const int i = q / ny, j = q - i * ny;
-    const double ex = a.split ? (1.0 / rho_u_face(a, rho, i, j) - 1.0 / a.rho_ref) * ((pe(i, j) - pe(i - 1, j)) / a.dx)
+    const double ex = a.split ? (1.0 / rho_u_face(a, rho, i, j) - 1.0 / a.rho_ref) *
+                                    ((p_ext_at(a, pres, prev, i, j) - p_ext_at(a, pres, prev, i - 1, j)) / a.dx)
                               : 0.0;
     const double gp = (p_corr(i, j) - p_corr(i - 1, j)) / a.dx;
     const double im = a.split ? gp / a.rho_ref : gp / rho_u_face(a, rho, i, j);
-    u(i, j) = u_star(i, j) - a.dt * (im + ex);
-  }
-  const int n_v = nx * (ny + 1);
-  for (int q = tid; q < n_v; q += kThreads) {
-    const int i = q / (ny + 1), j = q - i * (ny + 1);
-    const double ex = a.split ? (1.0 / rho_v_face(a, rho, i, j) - 1.0 / a.rho_ref) * ((pe(i, j) - pe(i, j - 1)) / a.dy)
+    Ghosted{a.u, ny, G}(i, j) = u_star(i, j) - a.dt * (im + ex);
+  } else if (q < n_u + n_v) {
+    const GhostedConst v_star{a.v_star, ny + 1, G};
+    const int r = q - n_u, i = r / (ny + 1), j = r - i * (ny + 1);
+    const double ex = a.split ? (1.0 / rho_v_face(a, rho, i, j) - 1.0 / a.rho_ref) *
+                                    ((p_ext_at(a, pres, prev, i, j) - p_ext_at(a, pres, prev, i, j - 1)) / a.dy)
                               : 0.0;
     const double gp = (p_corr(i, j) - p_corr(i, j - 1)) / a.dy;
     const double im = a.split ? gp / a.rho_ref : gp / rho_v_face(a, rho, i, j);
-    v(i, j) = v_star(i, j) - a.dt * (im + ex);
-  }
-  __syncthreads();
-  u_bc(a, u);
-  v_bc(a, v);
-  const double speed_post = boundary_speed(a, GhostedConst{a.u, ny, G}, GhostedConst{a.v, ny + 1, G}, red);
-
-  // pressure history (operators.cpp:524-533): previous <- pressure (whole buffer), pressure <-
-  // the correction's interior, its mean removed when no edge is Dirichlet, ghosts refilled
-  const size_t cells = static_cast<size_t>(nx + 2 * G) * (ny + 2 * G);
-  for (size_t q = tid; q < cells; q += kThreads) a.pressure_prev[q] = a.pressure[q];
-  __syncthreads();
-  for (int q = tid; q < nx * ny; q += kThreads) {
-    const int i = q / ny, j = q - i * ny;
-    pres(i, j) = p_corr(i, j);
-  }
-  __syncthreads();
-  if (!a.has_dirichlet) {
-    if (tid == 0) {
-      double mean = 0.0;
-      for (int i = 0; i < nx; ++i)
-        for (int j = 0; j < ny; ++j) mean += pres(i, j);
-      mean_sh = mean / static_cast<double>(nx * ny);
-    }
-    __syncthreads();
-    const double mean = mean_sh;
-    for (int q = tid; q < nx * ny; q += kThreads) {
-      const int i = q / ny, j = q - i * ny;
-      pres(i, j) -= mean;
-    }
-    __syncthreads();
-  }
-  pressure_ghosts(a, pres);
-  if (tid == 0) {
-    a.speeds[0] = speed_pre;
-    a.speeds[1] = speed_post;
+    Ghosted{a.v, ny + 1, G}(i, j) = v_star(i, j) - a.dt * (im + ex);
   }
 }
 
-hipError_t launch_velocity_correction(const CorrArgs *d_args, hipStream_t stream) {
-  hipLaunchKernelGGL(velocity_correction_kernel, dim3(1), dim3(kThreads), 0, stream, d_args);
+// Pressure history (operators.cpp:524-533): previous <- pressure over the whole ghosted buffer,
+// the new pressure interior <- the correction minus the mean (when no edge is Dirichlet). Each
+// thread owns one cell of both buffers, so the old value is read before it is replaced; the
+// pressure ghost cells keep their old values for the finish kernel's ghost passes.
+__global__ void __launch_bounds__(kFaceBlock) correction_history_kernel(const CorrArgs *__restrict__ args) {
+  const CorrArgs &a = *args;
+  const int ld = a.ny + 2 * a.G;
+  const int cells = (a.nx + 2 * a.G) * ld;
+  const int q = blockIdx.x * kFaceBlock + static_cast<int>(threadIdx.x);
+  if (q >= cells) return;
+  const global_ptr<double> pressure = as_global(a.pressure);
+  const double old = pressure[q];
+  as_global(a.pressure_prev)[q] = old;
+  const int i = q / ld - a.G, j = q % ld - a.G;
+  if (i >= 0 && i < a.nx && j >= 0 && j < a.ny) {
+    const double c = as_global(a.p_corr)[q];
+    pressure[q] = a.has_dirichlet ? c : c - as_global(a.work)[kMeanSlot];
+  }
+}
+
+// The ordered part in one block: boundary speeds before and after (operators.cpp:463-478),
+// apply_u_bc / apply_v_bc on the corrected faces and apply_pressure_bc on the new pressure.
+__global__ void __launch_bounds__(kThreads) correction_finish_kernel(const CorrArgs *__restrict__ args) {
+  __shared__ double red[kWaves];
+  const CorrArgs &a = *args;
+  const int ny = a.ny, G = a.G;
+  const double speed_pre =
+      boundary_speed(a, GhostedConst{a.u_star, ny, G}, GhostedConst{a.v_star, ny + 1, G}, red);
+  u_bc(a, Ghosted{a.u, ny, G});
+  v_bc(a, Ghosted{a.v, ny + 1, G});
+  const double speed_post = boundary_speed(a, GhostedConst{a.u, ny, G}, GhostedConst{a.v, ny + 1, G}, red);
+  pressure_ghosts(a, Ghosted{a.pressure, ny, G});
+  if (threadIdx.x == 0) {
+    a.work[0] = speed_pre;
+    a.work[1] = speed_post;
+  }
+}
+
+hipError_t launch_velocity_correction(const CorrArgs *d_args, int nx, int ny, int ghost, hipStream_t stream) {
+  const int faces = (nx + 1) * ny + nx * (ny + 1);
+  const int cells = (nx + 2 * ghost) * (ny + 2 * ghost);
+  hipLaunchKernelGGL(correction_faces_kernel, dim3(1 + (faces + kFaceBlock - 1) / kFaceBlock), dim3(kFaceBlock), 0,
+                     stream, d_args);
+  hipLaunchKernelGGL(correction_history_kernel, dim3((cells + kFaceBlock - 1) / kFaceBlock), dim3(kFaceBlock), 0,
+                     stream, d_args);
+  hipLaunchKernelGGL(correction_finish_kernel, dim3(1), dim3(kThreads), 0, stream, d_args);
   return hipGetLastError();
 }
 

@@ -40,6 +40,7 @@ struct pp_ctx {
   pp::DevReport *d_rep = nullptr;
   pp::DevReport *h_rep = nullptr;  // pinned
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  double last_kernel_ms = 0.0;  // event time of the last call's kernels (pp_last_kernel_ms)
 
   // sweep operand ring depth per grid shape (pp_set_tuning)
   std::map<std::pair<int, int>, int> tuning;
@@ -57,8 +58,7 @@ struct pp_ctx {
 
   // velocity correction: argument block, split-extrapolation field and speed pair
   pp::CorrArgs *d_corr_args = nullptr;
-  size_t corr_cap = 0;          // capacity of corr_work in doubles
-  double *corr_work = nullptr;  // 2 speeds, then the ghosted extrapolation field
+  double *corr_work = nullptr;  // 4 doubles: the two boundary speeds, the pressure mean
   size_t field_cap = 0;         // host entry: staging capacity in doubles
   double *field_stage = nullptr;
 };
@@ -280,6 +280,7 @@ int run_solve(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *repo
   PP_HIP_CHECK(hipStreamSynchronize(stream));
   float kernel_ms = 0.0f;
   PP_HIP_CHECK(hipEventElapsedTime(&kernel_ms, ctx->ev_start, ctx->ev_stop));
+  ctx->last_kernel_ms = kernel_ms;
   const pp::DevReport &r = *ctx->h_rep;
   copy_report(r, kernel_ms, report);
   if (r.status != pp::kOk) return fail(ctx, report, r.status, status_text(r.status, pb->scheme));
@@ -396,14 +397,7 @@ int run_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity
                             double *d_pressure_prev, double *speeds, hipStream_t s) {
   pp_report *report = nullptr;
   const int nx = pb->nx, ny = pb->ny, G = ghost;
-  const size_t cells = static_cast<size_t>(nx + 2 * G) * (ny + 2 * G);
-  if (2 + cells > ctx->corr_cap) {
-    if (ctx->corr_work) (void)hipFree(ctx->corr_work);
-    ctx->corr_work = nullptr;
-    ctx->corr_cap = 0;
-    PP_HIP_CHECK(hipMalloc(&ctx->corr_work, (2 + cells) * sizeof(double)));
-    ctx->corr_cap = 2 + cells;
-  }
+  if (!ctx->corr_work) PP_HIP_CHECK(hipMalloc(&ctx->corr_work, 4 * sizeof(double)));
   if (!ctx->d_corr_args) PP_HIP_CHECK(hipMalloc(&ctx->d_corr_args, sizeof(pp::CorrArgs)));
   pp::CorrArgs c;
   std::memset(&c, 0, sizeof(c));
@@ -441,13 +435,17 @@ int run_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity
   c.v = d_v;
   c.pressure = d_pressure;
   c.pressure_prev = d_pressure_prev;
-  c.speeds = ctx->corr_work;
-  c.p_ext = ctx->corr_work + 2;
+  c.work = ctx->corr_work;
   PP_HIP_CHECK(hipMemcpyAsync(ctx->d_corr_args, &c, sizeof(c), hipMemcpyHostToDevice, s));
-  PP_HIP_CHECK(pp::launch_velocity_correction(ctx->d_corr_args, s));
+  PP_HIP_CHECK(hipEventRecord(ctx->ev_start, s));
+  PP_HIP_CHECK(pp::launch_velocity_correction(ctx->d_corr_args, nx, ny, G, s));
+  PP_HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
   double sp[2] = {0.0, 0.0};
   PP_HIP_CHECK(hipMemcpyAsync(sp, ctx->corr_work, sizeof(sp), hipMemcpyDeviceToHost, s));
   PP_HIP_CHECK(hipStreamSynchronize(s));
+  float kernel_ms = 0.0f;
+  PP_HIP_CHECK(hipEventElapsedTime(&kernel_ms, ctx->ev_start, ctx->ev_stop));
+  ctx->last_kernel_ms = kernel_ms;
   if (speeds) {
     speeds[0] = sp[0];
     speeds[1] = sp[1];
@@ -579,6 +577,8 @@ void pp_destroy(pp_ctx *ctx) {
 }
 
 const char *pp_last_error(const pp_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+double pp_last_kernel_ms(const pp_ctx *ctx) { return ctx ? ctx->last_kernel_ms : 0.0; }
 
 int pp_solve_device(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u, const double *d_v,
                     const double *d_p_n, const double *d_p_nm1, double *d_p_out, pp_report *report, void *stream) {
@@ -814,6 +814,7 @@ int pp_solve_batched_device(pp_ctx *ctx, int count, const pp_problem *problems, 
   PP_HIP_CHECK(hipStreamSynchronize(s));
   float kernel_ms = 0.0f;
   PP_HIP_CHECK(hipEventElapsedTime(&kernel_ms, ctx->ev_start, ctx->ev_stop));
+  ctx->last_kernel_ms = kernel_ms;
   int first = PP_OK;
   for (int k = 0; k < count; ++k) {
     const pp::DevReport &r = ctx->h_batch_rep[k];

@@ -96,10 +96,11 @@ struct CorrArgs {
   double rho_ref;
   const double *rho_mid, *u_star, *v_star, *p_corr;
   double *u, *v, *pressure, *pressure_prev;
-  double *p_ext;   // scratch: ghosted nx x ny field for the split extrapolation
-  double *speeds;  // [2]: boundary speed before / after
+  double *work;    // [4]: boundary speed before / after, the pressure mean, unused
 };
 
-hipError_t launch_velocity_correction(const CorrArgs *d_args, hipStream_t stream);
+// Faces, history and finish kernels of the velocity correction on `stream` (nx, ny, ghost as in
+// the argument block, for the grid sizes).
+hipError_t launch_velocity_correction(const CorrArgs *d_args, int nx, int ny, int ghost, hipStream_t stream);
 
 }  // namespace pp

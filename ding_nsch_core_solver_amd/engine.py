@@ -85,9 +85,9 @@ class EngineError(RuntimeError):
 _lib = None
 
 # Every entry point declared in include/pp_solver.h.
-EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_solve", "pp_solve_device", "pp_solve_coeffs",
-           "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device", "pp_set_tuning",
-           "pp_velocity_correction", "pp_velocity_correction_device")
+EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_solve", "pp_solve_device",
+           "pp_solve_coeffs", "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device",
+           "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device")
 
 
 def library_available() -> bool:
@@ -127,6 +127,8 @@ def lib():
         L.pp_destroy.restype = None
         L.pp_last_error.argtypes = [vp]
         L.pp_last_error.restype = ctypes.c_char_p
+        L.pp_last_kernel_ms.argtypes = [vp]
+        L.pp_last_kernel_ms.restype = ctypes.c_double
         L.pp_solve.argtypes = [vp, ctypes.POINTER(PPProblem), dp, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int,
                                dp, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(PPReport)]
         L.pp_solve.restype = ctypes.c_int
@@ -372,3 +374,7 @@ class Engine:
 
     def last_error(self) -> str:
         return self._lib.pp_last_error(self.h).decode(errors="replace")
+
+    def last_kernel_ms(self) -> float:
+        """Device time of the kernels the last call ran (HIP events on that call's stream)."""
+        return float(self._lib.pp_last_kernel_ms(self.h))

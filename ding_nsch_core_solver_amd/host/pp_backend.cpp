@@ -24,6 +24,30 @@ constexpr SchemeName kSchemes[] = {
     {"hip_paper_split_ildlt_pcg", PP_SPLIT_ILDLT_PCG},
 };
 
+pp_problem make_problem(const PressureSettings &s, int code, int nx, int ny, int step_index) {
+  pp_problem pb;
+  std::memset(&pb, 0, sizeof(pb));
+  pb.nx = nx;
+  pb.ny = ny;
+  pb.dx = s.lx / nx;
+  pb.dy = s.ly / ny;
+  pb.dt = s.dt;
+  pb.periodic_x = s.periodic_x ? 1 : 0;
+  pb.periodic_y = s.periodic_y ? 1 : 0;
+  const BoundarySpec *edges[4] = {&s.left, &s.right, &s.bottom, &s.top};
+  for (int k = 0; k < 4; ++k) {
+    pb.bc_type[k] = edges[k]->type;
+    pb.bc_value[k] = edges[k]->value;
+  }
+  pb.scheme = code;
+  pb.precision = PP_FP64;
+  pb.max_iterations = s.poisson_iterations;
+  pb.tolerance = s.pressure_tolerance;
+  pb.density_ratio = s.density_ratio;
+  pb.step_index = step_index;
+  return pb;
+}
+
 }  // namespace
 
 int hip_scheme_code(const std::string &scheme) {
@@ -92,26 +116,7 @@ double PressureBackend::solve(const std::string &scheme, const PressureSettings 
   const int code = hip_scheme_code(scheme);
   if (code == 0) throw std::runtime_error("unsupported pressure scheme: " + scheme);
   const int nx = pressure_correction.nx, ny = pressure_correction.ny;
-  pp_problem pb;
-  std::memset(&pb, 0, sizeof(pb));
-  pb.nx = nx;
-  pb.ny = ny;
-  pb.dx = s.lx / nx;
-  pb.dy = s.ly / ny;
-  pb.dt = s.dt;
-  pb.periodic_x = s.periodic_x ? 1 : 0;
-  pb.periodic_y = s.periodic_y ? 1 : 0;
-  const BoundarySpec *edges[4] = {&s.left, &s.right, &s.bottom, &s.top};
-  for (int k = 0; k < 4; ++k) {
-    pb.bc_type[k] = edges[k]->type;
-    pb.bc_value[k] = edges[k]->value;
-  }
-  pb.scheme = code;
-  pb.precision = PP_FP64;
-  pb.max_iterations = s.poisson_iterations;
-  pb.tolerance = s.pressure_tolerance;
-  pb.density_ratio = s.density_ratio;
-  pb.step_index = step_index;
+  const pp_problem pb = make_problem(s, code, nx, ny, step_index);
 
   // zero-fill as the reference backends do before solving, so a failed solve leaves zeros
   const long total = static_cast<long>(nx + 2 * pressure_correction.ghost) * pressure_correction.ld();
@@ -129,6 +134,40 @@ double PressureBackend::solve(const std::string &scheme, const PressureSettings 
   last_iterations_ = report_.iterations;
   apply_pressure_bc(s, pressure_correction);
   return report_.relative_residual;
+}
+
+void PressureBackend::correct_velocity(const std::string &scheme, const PressureSettings &s,
+                                       const VelocitySettings &vel, const FieldView &rho_mid,
+                                       const FieldView &u_star, const FieldView &v_star,
+                                       const FieldView &pressure_correction, int step_index, const FieldView &u,
+                                       const FieldView &v, const FieldView &pressure,
+                                       const FieldView &pressure_previous) {
+  const int code = hip_scheme_code(scheme);
+  if (code == 0) throw std::runtime_error("unsupported pressure scheme: " + scheme);
+  const int nx = pressure.nx, ny = pressure.ny, g = pressure.ghost;
+  auto same = [&](const FieldView &f, int rows, int cols) {
+    return f.data && f.nx == rows && f.ny == cols && f.ghost == g;
+  };
+  if (!same(rho_mid, nx, ny) || !same(pressure_correction, nx, ny) || !same(pressure_previous, nx, ny) ||
+      !same(u_star, nx + 1, ny) || !same(u, nx + 1, ny) || !same(v_star, nx, ny + 1) || !same(v, nx, ny + 1))
+    throw std::runtime_error("velocity correction: field shapes or ghost counts do not match the pressure grid");
+  const pp_problem pb = make_problem(s, code, nx, ny, step_index);
+  pp_velocity_bc vb;
+  std::memset(&vb, 0, sizeof(vb));
+  const BoundarySpec *ue[4] = {&vel.u_left, &vel.u_right, &vel.u_bottom, &vel.u_top};
+  const BoundarySpec *ve[4] = {&vel.v_left, &vel.v_right, &vel.v_bottom, &vel.v_top};
+  for (int k = 0; k < 4; ++k) {
+    vb.u_type[k] = ue[k]->type;
+    vb.u_value[k] = ue[k]->value;
+    vb.v_type[k] = ve[k]->type;
+    vb.v_value[k] = ve[k]->value;
+  }
+  vb.bottom_wall_velocity_x = vel.bottom_wall_velocity_x;
+  vb.top_wall_velocity_x = vel.top_wall_velocity_x;
+  const int rc = pp_velocity_correction(ctx_, &pb, &vb, g, rho_mid.data, u_star.data, v_star.data,
+                                        pressure_correction.data, u.data, v.data, pressure.data,
+                                        pressure_previous.data, speeds_);
+  if (rc != PP_OK) throw std::runtime_error(pp_last_error(ctx_));
 }
 
 }  // namespace ppamd

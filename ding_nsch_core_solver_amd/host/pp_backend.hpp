@@ -40,6 +40,14 @@ struct PressureSettings {
   double density_ratio = 1.0;
 };
 
+// The velocity boundary values the correction step reads (config.hpp:105-106, 116-123); unset
+// edges are no-slip, with the wall speeds on the u bottom / top edges (operators.cpp:62-112).
+struct VelocitySettings {
+  BoundarySpec u_left, u_right, u_bottom, u_top;
+  BoundarySpec v_left, v_right, v_bottom, v_top;
+  double bottom_wall_velocity_x = 0.0, top_wall_velocity_x = 0.0;
+};
+
 // Scheme strings served by the engine, and their engine codes; 0 when not a hip_* scheme.
 int hip_scheme_code(const std::string &scheme);
 // True for the split hip_* schemes: callers must also treat them as split in their velocity
@@ -68,13 +76,27 @@ class PressureBackend {
                const FieldView &u_star, const FieldView &v_star, const FieldView &pressure,
                const FieldView &pressure_previous, int step_index, const FieldView &pressure_correction);
 
+  // The step after the solve, Solver::apply_pressure_velocity_correction (operators.cpp:480-534):
+  // u, v get the rho-weighted gradient of pressure_correction (plus the split schemes' explicit
+  // term) and their ghost cells, pressure_previous takes pressure, and pressure becomes the new
+  // pressure with ghosts. All fields share one ghost count; u, v, pressure and pressure_previous
+  // are read and written. Failures throw std::runtime_error.
+  void correct_velocity(const std::string &scheme, const PressureSettings &s, const VelocitySettings &vel,
+                        const FieldView &rho_mid, const FieldView &u_star, const FieldView &v_star,
+                        const FieldView &pressure_correction, int step_index, const FieldView &u,
+                        const FieldView &v, const FieldView &pressure, const FieldView &pressure_previous);
+
   int last_iterations() const { return last_iterations_; }
   const pp_report &last_report() const { return report_; }
+  // last_boundary_speed_pre_correction_ / _post_correction_ of the last correct_velocity
+  double boundary_speed_pre_correction() const { return speeds_[0]; }
+  double boundary_speed_post_correction() const { return speeds_[1]; }
 
  private:
   pp_ctx *ctx_ = nullptr;
   int last_iterations_ = 0;
   pp_report report_{};
+  double speeds_[2] = {0.0, 0.0};
 };
 
 }  // namespace ppamd

@@ -98,6 +98,11 @@ int pp_create(pp_ctx **ctx, int device);
 void pp_destroy(pp_ctx *ctx);
 /* Text of the last error on this context ("" if none). */
 const char *pp_last_error(const pp_ctx *ctx);
+/* Device time in ms of the kernels the last call on this context ran (HIP events on that call's
+ * stream): the solve kernel(s), the whole batch, or the velocity correction kernel. The solve
+ * entries also report it in pp_report.kernel_ms; the reference has no counterpart (its backends
+ * run on the host). */
+double pp_last_kernel_ms(const pp_ctx *ctx);
 
 /*
  * One pressure correction from host fields (replaces pressure.cpp:302-365, 470-491 between the

@@ -65,8 +65,8 @@ def lib():
                                   ctypes.c_char_p, ctypes.c_int]
         L.ref_sys_pcg.restype = ctypes.c_int
         L.ref_velocity_correction.argtypes = [ctypes.c_char_p, ctypes.POINTER(RefProblem), ctypes.POINTER(RefVelocityBc),
-                                              ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp, ctypes.c_char_p,
-                                              ctypes.c_int]
+                                              ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp, ctypes.c_int, dp,
+                                              ctypes.c_char_p, ctypes.c_int]
         L.ref_velocity_correction.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -172,9 +172,11 @@ class RefSystem:
 
 
 def velocity_correction(scheme: str, case, vbc, ghost: int, rho_mid, u_star, v_star, p_corr, u, v, pressure,
-                        pressure_prev):
+                        pressure_prev, repeats: int = 1, timing: list | None = None):
     """The reference's apply_pressure_velocity_correction on ghosted Field2D buffers.
-    Returns the new (u, v, pressure, pressure_prev, (speed_pre, speed_post))."""
+    Returns the new (u, v, pressure, pressure_prev, (speed_pre, speed_post)). With repeats > 1 the
+    call runs that many times on the same state (outputs of the last call) and, when `timing` is a
+    list, the median seconds of one call (set-up excluded) is appended to it."""
     b = RefVelocityBc()
     for k in range(4):
         b.u_type[k], b.u_value[k] = int(vbc.u_type[k]), float(vbc.u_value[k])
@@ -183,11 +185,14 @@ def velocity_correction(scheme: str, case, vbc, ghost: int, rho_mid, u_star, v_s
     b.top_wall_velocity_x = float(vbc.top_wall_velocity_x)
     u, v, pressure, prev = (np.array(a, dtype=np.float64, copy=True) for a in (u, v, pressure, pressure_prev))
     speeds = np.zeros(2)
+    seconds = np.zeros(1)
     ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho_mid, u_star, v_star, p_corr)]
     err = ctypes.create_string_buffer(512)
     rc = lib().ref_velocity_correction(scheme.encode(), ctypes.byref(problem(case)), ctypes.byref(b), int(ghost),
                                        *[_dp(a) for a in ins], _dp(u), _dp(v), _dp(pressure), _dp(prev), _dp(speeds),
-                                       err, 512)
+                                       int(repeats), _dp(seconds), err, 512)
     if rc:
         raise RuntimeError(err.value.decode())
+    if timing is not None:
+        timing.append(float(seconds[0]))
     return u, v, pressure, prev, (float(speeds[0]), float(speeds[1]))

@@ -11,6 +11,7 @@
 //                        preconditioner application and one PCG solve, taken from the reference's
 //                        own functions (pressure.cpp:123-248, ch_sparse_krylov.hpp:105-660).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -149,14 +150,16 @@ struct RefVelocityBc {
   double bottom_wall_velocity_x, top_wall_velocity_x;
 };
 
-// One call of Solver::apply_pressure_velocity_correction() (operators.cpp:480-534) on whole
-// Field2D buffers with G ghost layers: rho_mid, u_star, v_star, p_corr are read; u, v, pressure
-// and pressure_prev (pressure_previous_step_) are read and written.
-// speeds[0] / speeds[1] receive the periodic boundary speed before / after the correction.
+// Solver::apply_pressure_velocity_correction() (operators.cpp:480-534) on whole Field2D buffers
+// with G ghost layers: rho_mid, u_star, v_star, p_corr are read; u, v, pressure and pressure_prev
+// (pressure_previous_step_) are read and written. speeds[0] / speeds[1] receive the periodic
+// boundary speed before / after the correction. With repeats > 1 the call runs that many times on
+// the same solver state (for timing; the outputs are those of the last call) and *seconds (if not
+// null) receives the median wall time of one call, solver set-up and copies excluded.
 int ref_velocity_correction(const char *scheme, const RefProblem *pb, const RefVelocityBc *vb, int G,
                             const double *rho_mid, const double *u_star, const double *v_star, const double *p_corr,
-                            double *u, double *v, double *pressure, double *pressure_prev, double *speeds, char *err,
-                            int errlen) {
+                            double *u, double *v, double *pressure, double *pressure_prev, double *speeds,
+                            int repeats, double *seconds, char *err, int errlen) {
   try {
     ding::Config cfg = make_config(*pb, scheme);
     cfg.ghost = G;
@@ -186,7 +189,16 @@ int ref_velocity_correction(const char *scheme, const RefProblem *pb, const RefV
     in(s->pressure_, pressure);
     in(s->pressure_previous_step_, pressure_prev);
     s->current_step_index_ = pb->step_index;
-    s->apply_pressure_velocity_correction();
+    std::vector<double> times;
+    for (int r = 0; r < std::max(1, repeats); ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+      s->apply_pressure_velocity_correction();
+      times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    if (seconds) {
+      std::sort(times.begin(), times.end());
+      *seconds = times[times.size() / 2];
+    }
     out(s->u_, u);
     out(s->v_, v);
     out(s->pressure_, pressure);

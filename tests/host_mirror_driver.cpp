@@ -6,7 +6,11 @@
 //   driver solve  <cfg> <scheme> <in.bin> <out.bin>
 //        in.bin: rho (nx*ny), u ((nx+1)*ny), v (nx*(ny+1)), p_n, p_nm1 (nx*ny each), doubles
 //        out.bin: pressure_correction with 3 ghost layers; prints "iterations residual"
-// <cfg> = "nx ny lx ly dt px py t0 v0 t1 v1 t2 v2 t3 v3 maxit tol dr step" as one argument.
+//   driver correct <cfg> <vel> <scheme> <in.bin> <out.bin>
+//        in.bin: whole ghosted buffers (3 layers) rho_mid, u_star, v_star, p_corr, u, v, pressure,
+//        pressure_prev; out.bin: u, v, pressure, pressure_prev; prints "speed_pre speed_post"
+// <cfg> = "nx ny lx ly dt px py t0 v0 t1 v1 t2 v2 t3 v3 maxit tol dr step" as one argument;
+// <vel> = "ut0 uv0 .. ut3 uv3 vt0 vv0 .. vt3 vv3 bottom_wall top_wall" as one argument.
 #include <cstdio>
 #include <cstdlib>
 #include <sstream>
@@ -41,6 +45,20 @@ Cfg parse(const char *text) {
     std::exit(2);
   }
   return c;
+}
+
+ppamd::VelocitySettings parse_vel(const char *text) {
+  std::istringstream in(text);
+  ppamd::VelocitySettings v;
+  ppamd::BoundarySpec *e[8] = {&v.u_left, &v.u_right, &v.u_bottom, &v.u_top,
+                               &v.v_left, &v.v_right, &v.v_bottom, &v.v_top};
+  for (auto *b : e) in >> b->type >> b->value;
+  in >> v.bottom_wall_velocity_x >> v.top_wall_velocity_x;
+  if (!in) {
+    std::fprintf(stderr, "bad velocity cfg\n");
+    std::exit(2);
+  }
+  return v;
 }
 
 std::vector<double> read_all(const char *path) {
@@ -105,6 +123,39 @@ int main(int argc, char **argv) {
       const double res = backend.solve(argv[3], c.s, R.view, U.view, V.view, P.view, PM.view, c.step, out.view);
       write_all(argv[5], out.buf);
       std::printf("%d %.17g\n", backend.last_iterations(), res);
+    } catch (const std::exception &e) {
+      std::printf("error %s\n", e.what());
+      return 1;
+    }
+    return 0;
+  }
+  if (mode == "correct" && argc == 7) {
+    const Cfg c = parse(argv[2]);
+    const ppamd::VelocitySettings vel = parse_vel(argv[3]);
+    const int nx = c.nx, ny = c.ny, g = 3;
+    std::vector<double> in = read_all(argv[5]);
+    const size_t cell = static_cast<size_t>(nx + 2 * g) * (ny + 2 * g);
+    const size_t uf = static_cast<size_t>(nx + 1 + 2 * g) * (ny + 2 * g);
+    const size_t vf = static_cast<size_t>(nx + 2 * g) * (ny + 1 + 2 * g);
+    if (in.size() != 4 * cell + 2 * uf + 2 * vf) return 3;
+    double *at = in.data();
+    auto take = [&](size_t n) {
+      double *p = at;
+      at += n;
+      return p;
+    };
+    const FieldView rho{take(cell), nx, ny, g}, us{take(uf), nx + 1, ny, g}, vs{take(vf), nx, ny + 1, g};
+    const FieldView pc{take(cell), nx, ny, g}, u{take(uf), nx + 1, ny, g}, v{take(vf), nx, ny + 1, g};
+    const FieldView p{take(cell), nx, ny, g}, pm{take(cell), nx, ny, g};
+    try {
+      ppamd::PressureBackend backend(0);
+      backend.correct_velocity(argv[4], c.s, vel, rho, us, vs, pc, c.step, u, v, p, pm);
+      std::vector<double> out(u.data, u.data + uf);
+      out.insert(out.end(), v.data, v.data + vf);
+      out.insert(out.end(), p.data, p.data + cell);
+      out.insert(out.end(), pm.data, pm.data + cell);
+      write_all(argv[6], out);
+      std::printf("%.17g %.17g\n", backend.boundary_speed_pre_correction(), backend.boundary_speed_post_correction());
     } catch (const std::exception &e) {
       std::printf("error %s\n", e.what());
       return 1;

@@ -1,7 +1,8 @@
 """The C++ host mirror of the reference backends (ding_nsch_core_solver_amd/host/pp_backend.*):
 scheme mapping and ghost filling on the CPU (ghosts bit-for-bit against the compiled reference's
 apply_pressure_bc, or the oracle where the reference is absent), a full solve through
-PressureBackend on the GPU (bit-for-bit against the Python binding of the same engine)."""
+PressureBackend on the GPU (bit-for-bit against the Python binding of the same engine), and the
+velocity correction through PressureBackend::correct_velocity (bit-for-bit against the oracle)."""
 import ctypes
 import os
 import subprocess
@@ -107,3 +108,38 @@ def test_backend_solve_through_cpp_mirror(driver, engine, oracle_lib, tmp_path, 
     expect.reshape(case.nx + 2 * g, case.ny + 2 * g)[g:-g, g:-g] = p_ref.reshape(case.nx, case.ny)
     oracle_lib.apply_pressure_bc(case, expect)
     assert np.array_equal(full, expect)
+
+
+def _vel_cfg(vbc):
+    vals = []
+    for types, values in ((vbc.u_type, vbc.u_value), (vbc.v_type, vbc.v_value)):
+        for t, v in zip(types, values):
+            vals += [str(int(t)), repr(float(v))]
+    vals += [repr(float(vbc.bottom_wall_velocity_x)), repr(float(vbc.top_wall_velocity_x))]
+    return " ".join(vals)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme,vname,kw,step", [
+    ("hip_icpcg", "lid_driven", dict(), 0),
+    ("hip_ildlt_pcg", "dirichlet_values", dict(bc_type=(0, 0, 0, 1), bc_value=(0, 0, 0, 2.0)), 0),
+    ("hip_liu_split_icpcg", "neumann_outflow", dict(periodic_x=True), 3),
+    ("hip_paper_split_ildlt_pcg", "no_slip_defaults", dict(periodic_y=True), 0),
+])
+def test_velocity_correction_through_cpp_mirror(driver, oracle_lib, tmp_path, scheme, vname, kw, step):
+    """PressureBackend::correct_velocity against the oracle's restatement of
+    Solver::apply_pressure_velocity_correction (operators.cpp:480-534), bit for bit."""
+    from tests.test_velocity_oracle import G, VELOCITY_BC, fields
+    case = W.droplet_case(21, 13, seed=4, density_ratio=30.0, lx=0.8, ly=0.6, step_index=step, **kw)
+    f = fields(case, seed=17 + step)
+    vbc = VELOCITY_BC[vname]
+    order = ("rho_mid", "u_star", "v_star", "p_corr", "u", "v", "pressure", "pressure_prev")
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([f[k] for k in order]).tofile(inp)
+    res = subprocess.run([driver, "correct", _cfg(case), _vel_cfg(vbc), scheme, str(inp), str(outp)], check=True,
+                         capture_output=True, text=True)
+    pre, post = (float(x) for x in res.stdout.split())
+    split = E.SCHEME_CODES[scheme] in (E.PP_SPLIT_ICPCG, E.PP_SPLIT_ILDLT_PCG)
+    u, v, p, prev, speeds = oracle_lib.velocity_correction(case, vbc, split, G, **f)
+    assert np.array_equal(np.fromfile(outp), np.concatenate([u, v, p, prev]))
+    assert (pre, post) == tuple(speeds)
