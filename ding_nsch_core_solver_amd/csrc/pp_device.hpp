@@ -46,7 +46,7 @@ struct DevReport {
   int status;
   int iterations;
   int outer_cycles;
-  int pad;
+  int monitor_calls;  // PCG monitor calls made (records written up to the log's capacity)
   double rel, abs, iter_res;
   unsigned long long phase_ticks[10];
 };

@@ -1241,14 +1241,35 @@ __device__ __forceinline__ bool setup_system(const SysArgs &a, SweepShared &sh, 
   return *lds(&sh.status) == kOk;
 }
 
-template <bool kIldlt>
+template <bool kIldlt, bool kMonitor>
 __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) {
   const int tid = threadIdx.x;
   const int n = a.n, nx = a.nx, ny = a.ny;
   int parity = 0;
   PhaseClock clk;
   auto mark = [&](int phase) { clk.mark(phase); };
-  auto finish = [&]() { clk.finish(a.rep); };
+  // PCG monitor: one record per call of the reference's CGMonitor (ch_sparse_krylov.hpp:547-648)
+  int monitor_calls = 0;
+  auto monitor = [&](double alpha, double beta, double abs_r, double rel_r, double iter_r, double prec_r) {
+    if (!kMonitor) return;
+    if (tid == 0 && monitor_calls < a.monitor_cap) {
+      MonitorRecord m;
+      m.iteration = monitor_calls;
+      m.pad = 0;
+      m.alpha = alpha;
+      m.beta = beta;
+      m.abs_res = abs_r;
+      m.rel_res = rel_r;
+      m.iter_res = iter_r;
+      m.prec_res = prec_r;
+      a.monitor[monitor_calls] = m;
+    }
+    ++monitor_calls;
+  };
+  auto finish = [&]() {
+    if (kMonitor && tid == 0) a.rep->monitor_calls = monitor_calls;
+    clk.finish(a.rep);
+  };
   Setup su;
   if (!setup_system<kIldlt>(a, sh, parity, su)) {
     finish();
@@ -1297,6 +1318,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
   const LevelArray<double> apv_rows(a.y, n);
   mark(0);
 
+  if (rel_res < tol || abs_res < tol) monitor(0.0, 0.0, abs_res, rel_res, 0.0, 0.0);
   if (!(rel_res < tol || abs_res < tol)) {
     precondition<double, kIldlt>(a, fwd, bwd, sh, m_r, floor, clk.ticks, clk.t);
     double zs = 0.0;
@@ -1315,6 +1337,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
       finish();
       return;
     }
+    monitor(0.0, 0.0, abs_res, rel_res, 0.0, sqrt(rz < 0.0 ? 0.0 : rz));
     mark(0);
     const int limit = a.max_iterations < 1 ? 1 : a.max_iterations;
     for (int it = 0; it < limit; ++it) {
@@ -1386,7 +1409,10 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
       iter_res = sqrt(up[2] / (up[3] < 1.0e-30 ? 1.0e-30 : up[3]));
       abs_res = sqrt(rr < 0.0 ? 0.0 : rr);
       rel_res = abs_res / (bnorm < 1.0e-30 ? 1.0e-30 : bnorm);
-      if (rel_res < tol || abs_res < tol) break;
+      if (rel_res < tol || abs_res < tol) {
+        monitor(alpha, 0.0, abs_res, rel_res, iter_res, 0.0);
+        break;
+      }
 
       precondition<double, kIldlt>(a, fwd, bwd, sh, m_r, floor, clk.ticks, clk.t);
       zs = 0.0;
@@ -1426,6 +1452,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
         return;
       }
       const double beta = rzn / rz;
+      monitor(alpha, beta, abs_res, rel_res, iter_res, sqrt(rzn < 0.0 ? 0.0 : rzn));
       double ps = 0.0;
       for (int base = tid; base < n; base += kThreads * kUnroll) {
         double zq[kUnroll], pq[kUnroll];
@@ -1764,11 +1791,11 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
 
 }  // namespace
 
-template <bool kIldlt>
+template <bool kIldlt, bool kMonitor>
 __global__ void __launch_bounds__(kThreads) fp64_solve_kernel(const SysArgs *__restrict__ systems) {
   __shared__ SweepShared sh;
   const SysArgs &a = systems[blockIdx.x];
-  solve_system<kIldlt>(a, sh);
+  solve_system<kIldlt, kMonitor>(a, sh);
   __syncthreads();
   if (threadIdx.x == 0) a.rep->status = *lds(&sh.status);
 }
@@ -1790,11 +1817,18 @@ hipError_t launch_mixed_solve(const SysArgs *d_systems, int count, bool ildlt, h
   return hipGetLastError();
 }
 
-hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream) {
-  if (ildlt)
-    hipLaunchKernelGGL(fp64_solve_kernel<true>, dim3(count), dim3(kThreads), 0, stream, d_systems);
-  else
-    hipLaunchKernelGGL(fp64_solve_kernel<false>, dim3(count), dim3(kThreads), 0, stream, d_systems);
+hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream, bool monitor) {
+  if (monitor) {
+    if (ildlt)
+      hipLaunchKernelGGL((fp64_solve_kernel<true, true>), dim3(count), dim3(kThreads), 0, stream, d_systems);
+    else
+      hipLaunchKernelGGL((fp64_solve_kernel<false, true>), dim3(count), dim3(kThreads), 0, stream, d_systems);
+  } else {
+    if (ildlt)
+      hipLaunchKernelGGL((fp64_solve_kernel<true, false>), dim3(count), dim3(kThreads), 0, stream, d_systems);
+    else
+      hipLaunchKernelGGL((fp64_solve_kernel<false, false>), dim3(count), dim3(kThreads), 0, stream, d_systems);
+  }
   return hipGetLastError();
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -59,6 +60,10 @@ struct pp_ctx {
   // velocity correction: argument block, split-extrapolation field and speed pair
   pp::CorrArgs *d_corr_args = nullptr;
   double *corr_work = nullptr;  // 4 doubles: the two boundary speeds, the pressure mean
+
+  // PCG monitor log of the host entry point (pp_solve_monitored)
+  pp::MonitorRecord *monitor_log = nullptr;
+  size_t monitor_cap = 0;       // capacity in records
   size_t field_cap = 0;         // host entry: staging capacity in doubles
   double *field_stage = nullptr;
 };
@@ -267,14 +272,16 @@ void copy_report(const pp::DevReport &r, double kernel_ms, pp_report *report) {
   report->status = PP_OK;
 }
 
-int run_solve(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *report, hipStream_t stream) {
+// monitored: launch the kernel variant that writes the PCG monitor log and counts its calls
+int run_solve(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *report, hipStream_t stream,
+              bool monitored = false) {
   PP_HIP_CHECK(hipMemcpyAsync(ctx->d_args, &a, sizeof(a), hipMemcpyHostToDevice, stream));
   PP_HIP_CHECK(hipMemsetAsync(ctx->d_rep, 0, sizeof(pp::DevReport), stream));
   PP_HIP_CHECK(hipEventRecord(ctx->ev_start, stream));
   if (pb->precision == PP_MIXED)
     PP_HIP_CHECK(pp::launch_mixed_solve(ctx->d_args, 1, a.ildlt != 0, stream));
   else
-    PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_args, 1, a.ildlt != 0, stream));
+    PP_HIP_CHECK(pp::launch_fp64_solve(ctx->d_args, 1, a.ildlt != 0, stream, monitored));
   PP_HIP_CHECK(hipEventRecord(ctx->ev_stop, stream));
   PP_HIP_CHECK(hipMemcpyAsync(ctx->h_rep, ctx->d_rep, sizeof(pp::DevReport), hipMemcpyDeviceToHost, stream));
   PP_HIP_CHECK(hipStreamSynchronize(stream));
@@ -287,6 +294,47 @@ int run_solve(pp_ctx *ctx, const pp_problem *pb, pp::SysArgs &a, pp_report *repo
   ctx->last_error.clear();
   return PP_OK;
 }
+
+static_assert(sizeof(pp::MonitorRecord) == sizeof(pp_monitor_record), "monitor record layout");
+static_assert(offsetof(pp::MonitorRecord, prec_res) == offsetof(pp_monitor_record, preconditioned_residual),
+              "monitor record layout");
+
+int validate_monitor(pp_ctx *ctx, const pp_problem *pb, const void *records, int capacity, pp_report *report) {
+  if (int rc = validate(ctx, pb, report)) return rc;
+  if (pb->precision == PP_MIXED) return fail(ctx, report, PP_ERR_ARG, "the PCG monitor log covers FP64 solves");
+  if (capacity < 0 || (capacity > 0 && !records)) return fail(ctx, report, PP_ERR_ARG, "bad monitor log buffer");
+  return PP_OK;
+}
+
+// pp_solve_device with an optional monitor log (device records, capacity, number of calls)
+int solve_device(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u, const double *d_v,
+                 const double *d_p_n, const double *d_p_nm1, double *d_p_out, pp::MonitorRecord *d_records,
+                 int capacity, int *count, pp_report *report, void *stream) {
+  if (int rc = validate(ctx, pb, report)) return rc;
+  if (!d_rho && !(pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG))
+    return fail(ctx, report, PP_ERR_ARG, "rho is required");
+  if (!d_u || !d_v || !d_p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  pp::SysArgs a;
+  if (int rc = prepare(ctx, pb, a, report)) return rc;
+  a.rho = d_rho;
+  a.u = d_u;
+  a.v = d_v;
+  a.p_n = d_p_n;
+  a.p_nm1 = d_p_nm1;
+  a.p_out = d_p_out;
+  const bool monitored = d_records != nullptr || count != nullptr;
+  a.monitor = d_records;
+  a.monitor_cap = d_records ? capacity : 0;
+  const int rc = run_solve(ctx, pb, a, report, s, monitored);
+  if (count) *count = monitored ? ctx->h_rep->monitor_calls : 0;
+  return rc;
+}
+
+int solve_host(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, const double *u, int ld_u,
+               const double *v, int ld_v, const double *p_n, int ld_pn, const double *p_nm1, int ld_pnm1,
+               double *p_out, int ld_out, pp_monitor_record *records, int capacity, int *count, pp_report *report);
 
 void free_batch(pp_ctx *ctx) {
   if (ctx->batch_work) (void)hipFree(ctx->batch_work);
@@ -569,6 +617,7 @@ void pp_destroy(pp_ctx *ctx) {
   free_batch(ctx);
   if (ctx->d_corr_args) (void)hipFree(ctx->d_corr_args);
   if (ctx->corr_work) (void)hipFree(ctx->corr_work);
+  if (ctx->monitor_log) (void)hipFree(ctx->monitor_log);
   if (ctx->field_stage) (void)hipFree(ctx->field_stage);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
@@ -582,26 +631,41 @@ double pp_last_kernel_ms(const pp_ctx *ctx) { return ctx ? ctx->last_kernel_ms :
 
 int pp_solve_device(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u, const double *d_v,
                     const double *d_p_n, const double *d_p_nm1, double *d_p_out, pp_report *report, void *stream) {
-  if (int rc = validate(ctx, pb, report)) return rc;
-  if (!d_rho && !(pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG))
-    return fail(ctx, report, PP_ERR_ARG, "rho is required");
-  if (!d_u || !d_v || !d_p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
-  PP_HIP_CHECK(hipSetDevice(ctx->device));
-  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  pp::SysArgs a;
-  if (int rc = prepare(ctx, pb, a, report)) return rc;
-  a.rho = d_rho;
-  a.u = d_u;
-  a.v = d_v;
-  a.p_n = d_p_n;
-  a.p_nm1 = d_p_nm1;
-  a.p_out = d_p_out;
-  return run_solve(ctx, pb, a, report, s);
+  return solve_device(ctx, pb, d_rho, d_u, d_v, d_p_n, d_p_nm1, d_p_out, nullptr, 0, nullptr, report, stream);
+}
+
+int pp_solve_device_monitored(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u,
+                              const double *d_v, const double *d_p_n, const double *d_p_nm1, double *d_p_out,
+                              pp_monitor_record *d_records, int capacity, int *count, pp_report *report,
+                              void *stream) {
+  if (int rc = validate_monitor(ctx, pb, d_records, capacity, report)) return rc;
+  return solve_device(ctx, pb, d_rho, d_u, d_v, d_p_n, d_p_nm1, d_p_out, reinterpret_cast<pp::MonitorRecord *>(d_records),
+                      capacity, count, report, stream);
 }
 
 int pp_solve(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, const double *u, int ld_u,
              const double *v, int ld_v, const double *p_n, int ld_pn, const double *p_nm1, int ld_pnm1,
              double *p_out, int ld_out, pp_report *report) {
+  return solve_host(ctx, pb, rho, ld_rho, u, ld_u, v, ld_v, p_n, ld_pn, p_nm1, ld_pnm1, p_out, ld_out, nullptr, 0,
+                    nullptr, report);
+}
+
+int pp_solve_monitored(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, const double *u, int ld_u,
+                       const double *v, int ld_v, const double *p_n, int ld_pn, const double *p_nm1, int ld_pnm1,
+                       double *p_out, int ld_out, pp_monitor_record *records, int capacity, int *count,
+                       pp_report *report) {
+  if (int rc = validate_monitor(ctx, pb, records, capacity, report)) return rc;
+  return solve_host(ctx, pb, rho, ld_rho, u, ld_u, v, ld_v, p_n, ld_pn, p_nm1, ld_pnm1, p_out, ld_out, records,
+                    capacity, count, report);
+}
+
+}  // extern "C"
+
+namespace {
+
+int solve_host(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, const double *u, int ld_u,
+               const double *v, int ld_v, const double *p_n, int ld_pn, const double *p_nm1, int ld_pnm1,
+               double *p_out, int ld_out, pp_monitor_record *records, int capacity, int *count, pp_report *report) {
   if (int rc = validate(ctx, pb, report)) return rc;
   const int nx = pb->nx, ny = pb->ny;
   if (!rho || !u || !v || !p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
@@ -635,13 +699,36 @@ int pp_solve(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, c
   a.p_n = p_n ? d_pn : nullptr;
   a.p_nm1 = p_nm1 ? d_pnm1 : nullptr;
   a.p_out = d_out;
-  if (int rc = run_solve(ctx, pb, a, report, s)) return rc;
-  PP_HIP_CHECK(hipMemcpy2DAsync(p_out, static_cast<size_t>(ld_out) * sizeof(double), d_out,
-                                static_cast<size_t>(ny) * sizeof(double), static_cast<size_t>(ny) * sizeof(double),
-                                static_cast<size_t>(nx), hipMemcpyDeviceToHost, s));
+  if (records && capacity > 0) {
+    if (static_cast<size_t>(capacity) > ctx->monitor_cap) {
+      if (ctx->monitor_log) (void)hipFree(ctx->monitor_log);
+      ctx->monitor_log = nullptr;
+      ctx->monitor_cap = 0;
+      PP_HIP_CHECK(hipMalloc(&ctx->monitor_log, static_cast<size_t>(capacity) * sizeof(pp::MonitorRecord)));
+      ctx->monitor_cap = static_cast<size_t>(capacity);
+    }
+    a.monitor = ctx->monitor_log;
+    a.monitor_cap = capacity;
+  }
+  const bool monitored = records != nullptr || count != nullptr;
+  const int rc = run_solve(ctx, pb, a, report, s, monitored);
+  const int calls = monitored ? ctx->h_rep->monitor_calls : 0;
+  if (count) *count = calls;
+  if (a.monitor && calls > 0)
+    PP_HIP_CHECK(hipMemcpyAsync(records, ctx->monitor_log,
+                                static_cast<size_t>(std::min(calls, capacity)) * sizeof(pp::MonitorRecord),
+                                hipMemcpyDeviceToHost, s));
+  if (rc == PP_OK)
+    PP_HIP_CHECK(hipMemcpy2DAsync(p_out, static_cast<size_t>(ld_out) * sizeof(double), d_out,
+                                  static_cast<size_t>(ny) * sizeof(double), static_cast<size_t>(ny) * sizeof(double),
+                                  static_cast<size_t>(nx), hipMemcpyDeviceToHost, s));
   PP_HIP_CHECK(hipStreamSynchronize(s));
-  return PP_OK;
+  return rc;
 }
+
+}  // namespace
+
+extern "C" {
 
 int pp_solve_coeffs_device(pp_ctx *ctx, const pp_problem *pb, const double *d_ku, const double *d_kv,
                            const double *d_src, double *d_p_out, pp_report *report, void *stream) {

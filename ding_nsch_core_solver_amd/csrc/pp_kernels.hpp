@@ -8,6 +8,13 @@ namespace pp {
 
 struct DevReport;
 
+// One PCG monitor record: the scalar fields of one call of the reference's CGMonitor (layout of
+// pp_monitor_record in pp_solver.h).
+struct MonitorRecord {
+  int iteration, pad;
+  double alpha, beta, abs_res, rel_res, iter_res, prec_res;
+};
+
 // One pressure system as the kernels see it. Field inputs/outputs are in the natural i-major
 // layout; every work array is in the wavefront layout described in pp_device.hpp.
 struct SysArgs {
@@ -47,12 +54,17 @@ struct SysArgs {
   float *fx, *fr, *fpa, *fpb, *fz, *fy;
   float *fLX, *fLY;
   DevReport *rep;
+  // PCG monitor log (FP64 solve): record k written for monitor call k while k < monitor_cap
+  MonitorRecord *monitor;
+  int monitor_cap;
 };
 
 // One workgroup per system: workgroup k solves d_systems[k]. With more systems than CUs the
 // dispatcher starts workgroup k as soon as a CU is free, in index order, so an array sorted by
 // decreasing cost is scheduled largest first.
-hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream);
+// monitor: the variant that writes the PCG monitor log (SysArgs::monitor) of every system.
+hipError_t launch_fp64_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream,
+                             bool monitor = false);
 
 // Mixed precision (FP32 inner PCG inside FP64 refinement), one workgroup per system.
 hipError_t launch_mixed_solve(const SysArgs *d_systems, int count, bool ildlt, hipStream_t stream);

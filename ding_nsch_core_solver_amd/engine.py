@@ -87,7 +87,13 @@ _lib = None
 # Every entry point declared in include/pp_solver.h.
 EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_solve", "pp_solve_device",
            "pp_solve_coeffs", "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device",
-           "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device")
+           "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device", "pp_solve_monitored",
+           "pp_solve_device_monitored")
+
+# One PCG monitor record (pp_monitor_record): the scalar fields of one CGMonitor call.
+MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
+                          ("absolute_residual", "<f8"), ("relative_residual", "<f8"), ("iterate_residual", "<f8"),
+                          ("preconditioned_residual", "<f8")])
 
 
 def library_available() -> bool:
@@ -135,6 +141,14 @@ def lib():
         L.pp_solve_device.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(PPReport), vp]
         L.pp_solve_device.restype = ctypes.c_int
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.pp_solve_monitored.argtypes = [vp, ctypes.POINTER(PPProblem), dp, ctypes.c_int, dp, ctypes.c_int, dp,
+                                         ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, vp,
+                                         ctypes.c_int, ip, ctypes.POINTER(PPReport)]
+        L.pp_solve_monitored.restype = ctypes.c_int
+        L.pp_solve_device_monitored.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, vp, vp, vp,
+                                                ctypes.c_int, ip, ctypes.POINTER(PPReport), vp]
+        L.pp_solve_device_monitored.restype = ctypes.c_int
         L.pp_solve_coeffs.argtypes = [vp, ctypes.POINTER(PPProblem), dp, dp, dp, dp, ctypes.POINTER(PPReport)]
         L.pp_solve_coeffs.restype = ctypes.c_int
         L.pp_solve_coeffs_device.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, ctypes.POINTER(PPReport), vp]
@@ -265,6 +279,48 @@ class Engine:
         if rc != 0:
             raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
         return out.reshape(-1), _report_dict(rep)
+
+    def solve_monitored(self, case, scheme: int = PP_ICPCG, capacity: int | None = None, precision: int = PP_FP64,
+                        **kw):
+        """Host-array solve with the PCG monitor log (FP64 only; the library rejects PP_MIXED).
+        Returns (p, report dict, records): one MONITOR_DTYPE record per monitor call, up to
+        `capacity` (default max_iterations + 1); the report's "monitor_calls" is the number of calls."""
+        pb = make_problem(case, scheme, precision, **kw)
+        nx, ny = case.nx, case.ny
+        rho = np.ascontiguousarray(case.rho, dtype=np.float64).reshape(nx, ny)
+        u = np.ascontiguousarray(case.u, dtype=np.float64).reshape(nx + 1, ny)
+        v = np.ascontiguousarray(case.v, dtype=np.float64).reshape(nx, ny + 1)
+        pn = None if case.p_n is None else np.ascontiguousarray(case.p_n, dtype=np.float64).reshape(nx, ny)
+        pm = None if case.p_nm1 is None else np.ascontiguousarray(case.p_nm1, dtype=np.float64).reshape(nx, ny)
+        cap = pb.max_iterations + 1 if capacity is None else int(capacity)
+        records = np.zeros(max(cap, 1), dtype=MONITOR_DTYPE)
+        count = ctypes.c_int(0)
+        out = np.zeros((nx, ny))
+        rep = PPReport()
+        rc = self._lib.pp_solve_monitored(self.h, ctypes.byref(pb), _dp(rho), ny, _dp(u), ny, _dp(v), ny + 1,
+                                          _dp(pn), ny, _dp(pm), ny, _dp(out), ny, records.ctypes.data, cap,
+                                          ctypes.byref(count), ctypes.byref(rep))
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        report = _report_dict(rep)
+        report["monitor_calls"] = count.value
+        return out.reshape(-1), report, records[:min(count.value, cap)]
+
+    def solve_device_monitored(self, case, d_rho, d_u, d_v, d_pn, d_pnm1, d_out, d_records, capacity: int,
+                               scheme: int = PP_ICPCG, stream=None, **kw):
+        """Device-pointer FP64 solve with the PCG monitor log written to d_records (device memory
+        for `capacity` records). Returns the report dict with "monitor_calls"."""
+        pb = make_problem(case, scheme, PP_FP64, **kw)
+        rep = PPReport()
+        count = ctypes.c_int(0)
+        rc = self._lib.pp_solve_device_monitored(self.h, ctypes.byref(pb), d_rho, d_u, d_v, d_pn, d_pnm1, d_out,
+                                                 d_records, int(capacity), ctypes.byref(count), ctypes.byref(rep),
+                                                 stream)
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        report = _report_dict(rep)
+        report["monitor_calls"] = count.value
+        return report
 
     def solve_device(self, case, d_rho, d_u, d_v, d_pn, d_pnm1, d_out, scheme: int = PP_ICPCG,
                      precision: int = PP_FP64, stream=None, **kw):

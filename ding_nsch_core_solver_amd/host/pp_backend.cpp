@@ -1,7 +1,9 @@
 // C++ host mirror of the reference pressure backends over the engine's C ABI (see pp_backend.hpp).
 #include "pp_backend.hpp"
 
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 namespace ppamd {
 
@@ -113,6 +115,14 @@ double PressureBackend::solve(const std::string &scheme, const PressureSettings 
                               const FieldView &u_star, const FieldView &v_star, const FieldView &pressure,
                               const FieldView &pressure_previous, int step_index,
                               const FieldView &pressure_correction) {
+  return solve(scheme, s, rho_mid, u_star, v_star, pressure, pressure_previous, step_index, pressure_correction,
+               CGMonitor());
+}
+
+double PressureBackend::solve(const std::string &scheme, const PressureSettings &s, const FieldView &rho_mid,
+                              const FieldView &u_star, const FieldView &v_star, const FieldView &pressure,
+                              const FieldView &pressure_previous, int step_index,
+                              const FieldView &pressure_correction, const CGMonitor &monitor) {
   const int code = hip_scheme_code(scheme);
   if (code == 0) throw std::runtime_error("unsupported pressure scheme: " + scheme);
   const int nx = pressure_correction.nx, ny = pressure_correction.ny;
@@ -123,10 +133,22 @@ double PressureBackend::solve(const std::string &scheme, const PressureSettings 
   for (long k = 0; k < total; ++k) pressure_correction.data[k] = 0.0;
   last_iterations_ = 0;
   const bool split = code == PP_SPLIT_ICPCG || code == PP_SPLIT_ILDLT_PCG;
-  const int rc = pp_solve(ctx_, &pb, rho_mid.interior(), rho_mid.ld(), u_star.interior(), u_star.ld(),
-                          v_star.interior(), v_star.ld(), split && pressure.data ? pressure.interior() : nullptr,
-                          pressure.ld(), split && pressure_previous.data ? pressure_previous.interior() : nullptr,
-                          pressure_previous.ld(), pressure_correction.interior(), pressure_correction.ld(), &report_);
+  const double *pn = split && pressure.data ? pressure.interior() : nullptr;
+  const double *pm = split && pressure_previous.data ? pressure_previous.interior() : nullptr;
+  int rc = PP_OK;
+  if (monitor) {
+    std::vector<pp_monitor_record> log(static_cast<size_t>(std::max(1, s.poisson_iterations)) + 1);
+    int calls = 0;
+    rc = pp_solve_monitored(ctx_, &pb, rho_mid.interior(), rho_mid.ld(), u_star.interior(), u_star.ld(),
+                            v_star.interior(), v_star.ld(), pn, pressure.ld(), pm, pressure_previous.ld(),
+                            pressure_correction.interior(), pressure_correction.ld(), log.data(),
+                            static_cast<int>(log.size()), &calls, &report_);
+    for (int k = 0; k < std::min(calls, static_cast<int>(log.size())); ++k) monitor(log[static_cast<size_t>(k)]);
+  } else {
+    rc = pp_solve(ctx_, &pb, rho_mid.interior(), rho_mid.ld(), u_star.interior(), u_star.ld(), v_star.interior(),
+                  v_star.ld(), pn, pressure.ld(), pm, pressure_previous.ld(), pressure_correction.interior(),
+                  pressure_correction.ld(), &report_);
+  }
   if (rc != PP_OK) {
     apply_pressure_bc(s, pressure_correction);
     throw std::runtime_error(report_.message[0] ? report_.message : pp_last_error(ctx_));

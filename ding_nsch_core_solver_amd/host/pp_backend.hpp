@@ -9,6 +9,7 @@
 // fields given as ghosted Field2D-like views (data[(i+g)(ny+2g)+(j+g)], field.hpp:9-36).
 #pragma once
 
+#include <functional>
 #include <stdexcept>
 #include <string>
 
@@ -62,6 +63,10 @@ bool has_dirichlet(const PressureSettings &s);
 // Ghost fill of a pressure-like field (periodic copy, Dirichlet 2v - f, Neumann f + 2(g-1/2) v h).
 void apply_pressure_bc(const PressureSettings &s, const FieldView &f);
 
+// Per-iteration PCG monitor: called once per record of the solve's log, in order, after the solve
+// (the scalar part of the reference's CGMonitor, ch_sparse_krylov.hpp:47-61).
+using CGMonitor = std::function<void(const pp_monitor_record &)>;
+
 class PressureBackend {
  public:
   explicit PressureBackend(int device = 0);
@@ -75,6 +80,11 @@ class PressureBackend {
   double solve(const std::string &scheme, const PressureSettings &s, const FieldView &rho_mid,
                const FieldView &u_star, const FieldView &v_star, const FieldView &pressure,
                const FieldView &pressure_previous, int step_index, const FieldView &pressure_correction);
+  // The same solve with the PCG monitor log replayed through `monitor` (FP64 path).
+  double solve(const std::string &scheme, const PressureSettings &s, const FieldView &rho_mid,
+               const FieldView &u_star, const FieldView &v_star, const FieldView &pressure,
+               const FieldView &pressure_previous, int step_index, const FieldView &pressure_correction,
+               const CGMonitor &monitor);
 
   // The step after the solve, Solver::apply_pressure_velocity_correction (operators.cpp:480-534):
   // u, v get the rho-weighted gradient of pressure_correction (plus the split schemes' explicit

@@ -127,6 +127,42 @@ int pp_solve_device(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, cons
                     pp_report *report, void *stream);
 
 /*
+ * Per-iteration PCG log: the scalar fields of one call of the reference's CGMonitor
+ * (CGMonitorData, ch_sparse_krylov.hpp:47-58; the calls at 547-555, 569-579, 612-621 and
+ * 636-648), the vectors left out. Record k belongs to iteration k: record 0 is the initial
+ * residual (preconditioned_residual = sqrt(r.z) once z exists), record k >= 1 follows PCG
+ * iteration k (beta and preconditioned_residual stay 0 on the converged last iteration, as in the
+ * reference). Fields the reference leaves at their defaults are 0.
+ */
+typedef struct pp_monitor_record {
+  int iteration;
+  int pad;
+  double alpha;
+  double beta;
+  double absolute_residual;
+  double relative_residual;
+  double iterate_residual;
+  double preconditioned_residual;
+} pp_monitor_record;
+
+/*
+ * pp_solve / pp_solve_device with the PCG log: the kernel writes one record per monitor call into
+ * `records` (host memory for pp_solve_monitored, device memory for pp_solve_device_monitored), up
+ * to `capacity` records; *count receives the number of monitor calls the solve made (report
+ * iterations + 1 after a normal end), which may exceed capacity. Replaces passing a CGMonitor to
+ * solve_preconditioned_cg (the online analysis hook, pressure.cpp:1078-1087; the frozen replay,
+ * pressure_analysis.cpp:255-294) for the scalar part. FP64 solves only (PP_ERR_ARG for PP_MIXED).
+ */
+int pp_solve_monitored(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, const double *u, int ld_u,
+                       const double *v, int ld_v, const double *p_n, int ld_pn, const double *p_nm1, int ld_pnm1,
+                       double *p_out, int ld_out, pp_monitor_record *records, int capacity, int *count,
+                       pp_report *report);
+int pp_solve_device_monitored(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, const double *d_u,
+                              const double *d_v, const double *d_p_n, const double *d_p_nm1, double *d_p_out,
+                              pp_monitor_record *d_records, int capacity, int *count, pp_report *report,
+                              void *stream);
+
+/*
  * Coefficient-level solve (no counterpart in the reference; the differentiable form of the
  * non-split operator of pressure.cpp:123-196). Inputs are face inverse densities and a source:
  *   k_u  (nx+1) x ny   k_f = 1/rho_f on x-faces (face i lies between cells i-1 and i)

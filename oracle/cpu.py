@@ -16,6 +16,11 @@ _lib = None
 SCHEMES = {"icpcg": 1, "ildlt_pcg": 2, "split_icpcg": 3, "split_ildlt_pcg": 4}
 SEQUENTIAL, PAIRWISE, CANONICAL = 0, 1, 2
 
+# One PCG monitor record (the scalar fields of CGMonitorData, ch_sparse_krylov.hpp:47-58).
+MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
+                          ("absolute_residual", "<f8"), ("relative_residual", "<f8"), ("iterate_residual", "<f8"),
+                          ("preconditioned_residual", "<f8")])
+
 
 class OrProblem(ctypes.Structure):
     _fields_ = [
@@ -56,6 +61,13 @@ def lib():
         L.or_pressure_solve.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), dp, dp, dp, dp, dp, ctypes.c_int,
                                         dp, ip, dp, ctypes.c_char_p, ctypes.c_int]
         L.or_pressure_solve.restype = ctypes.c_int
+        L.or_pressure_solve_monitored.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), dp, dp, dp, dp, dp,
+                                                  ctypes.c_int, dp, ip, dp, ctypes.c_void_p, ctypes.c_int, ip,
+                                                  ctypes.c_char_p, ctypes.c_int]
+        L.or_pressure_solve_monitored.restype = ctypes.c_int
+        L.or_sys_pcg_monitored.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, dp, ip, dp, dp,
+                                           ctypes.c_void_p, ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_int]
+        L.or_sys_pcg_monitored.restype = ctypes.c_int
         L.or_sys_build.argtypes = [ctypes.POINTER(OrProblem), ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_int,
                                    ctypes.c_char_p, ctypes.c_int]
         L.or_sys_build.restype = ctypes.c_void_p
@@ -118,6 +130,23 @@ def pressure_solve(scheme: str, case, reduction: int = SEQUENTIAL):
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return p, it.value, res.value
+
+
+def pressure_solve_monitored(scheme: str, case, reduction: int = SEQUENTIAL, capacity: int | None = None):
+    """pressure_solve with the PCG monitor log. Returns (p, iterations, relative residual, records)
+    where records holds one MONITOR_DTYPE entry per monitor call (up to capacity)."""
+    cap = case.max_iterations + 1 if capacity is None else capacity
+    records = np.zeros(max(cap, 1), dtype=MONITOR_DTYPE)
+    p = np.zeros(case.n)
+    it, res, count = ctypes.c_int(0), ctypes.c_double(0.0), ctypes.c_int(0)
+    err = ctypes.create_string_buffer(512)
+    rc = lib().or_pressure_solve_monitored(SCHEMES[scheme], ctypes.byref(problem(case)), _dp(case.rho), _dp(case.u),
+                                           _dp(case.v), _dp(case.p_n), _dp(case.p_nm1), reduction, _dp(p),
+                                           ctypes.byref(it), ctypes.byref(res), records.ctypes.data, cap,
+                                           ctypes.byref(count), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return p, it.value, res.value, records[:min(count.value, cap)]
 
 
 def _f64(a):
@@ -215,6 +244,19 @@ class OracleSystem:
         if rc != 0:
             raise RuntimeError(err.value.decode())
         return x, it.value, rel.value, ab.value
+
+    def pcg_monitored(self, max_iterations, tolerance, capacity=None):
+        """pcg with the monitor log; returns (x, iterations, rel, abs, records, error text or None):
+        a breakdown is returned as its message with the records logged before it."""
+        cap = max_iterations + 1 if capacity is None else capacity
+        records = np.zeros(max(cap, 1), dtype=MONITOR_DTYPE)
+        x = np.zeros(self.n)
+        it, rel, ab, count = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        err = ctypes.create_string_buffer(512)
+        rc = lib().or_sys_pcg_monitored(self.h, max_iterations, tolerance, _dp(x), ctypes.byref(it), ctypes.byref(rel),
+                                        ctypes.byref(ab), records.ctypes.data, cap, ctypes.byref(count), err, 512)
+        return (x, it.value, rel.value, ab.value, records[:min(count.value, cap)],
+                err.value.decode() if rc != 0 else None)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -55,6 +55,13 @@ struct OrVelocityBc {
   double bottom_wall_velocity_x, top_wall_velocity_x;
 };
 
+// The scalar fields of one CGMonitor call (CGMonitorData, ch_sparse_krylov.hpp:47-58); record k
+// belongs to monitor call k (iteration k).
+struct OrMonitorRecord {
+  int iteration, pad;
+  double alpha, beta, absolute_residual, relative_residual, iterate_residual, preconditioned_residual;
+};
+
 }  // extern "C"
 
 namespace {
@@ -575,8 +582,15 @@ void project(std::vector<double> &w, const Reducer &R) {
 }
 
 // PCG from x0 = 0 (ch_sparse_krylov.hpp:515-660, pressure.cpp:50-57).
+// log (optional) receives one record per CGMonitor call at the reference's call sites
+// (ch_sparse_krylov.hpp:547-555, 569-579, 612-621, 636-648).
 Report pcg(const Csr &A, const Factor &F, const std::vector<double> &b, int max_iterations, double tol, bool projected,
-           const Reducer &R, std::vector<double> &x) {
+           const Reducer &R, std::vector<double> &x, std::vector<OrMonitorRecord> *log = nullptr) {
+  auto monitor = [&](double alpha, double beta, double abs_r, double rel_r, double iter_r, double prec_r) {
+    if (!log) return;
+    OrMonitorRecord m{static_cast<int>(log->size()), 0, alpha, beta, abs_r, rel_r, iter_r, prec_r};
+    log->push_back(m);
+  };
   const int n = A.n;
   if (n <= 0) throw std::runtime_error("cannot solve an empty linear system");
   auto dot = [&](const std::vector<double> &a, const std::vector<double> &c) {
@@ -592,13 +606,17 @@ Report pcg(const Csr &A, const Factor &F, const std::vector<double> &b, int max_
   Report rep;
   rep.abs_res = std::sqrt(std::max(dot(r, r), 0.0));
   rep.rel_res = rep.abs_res / std::max(bnorm, 1.0e-30);
-  if (rep.rel_res < tol || rep.abs_res < tol) return rep;
+  if (rep.rel_res < tol || rep.abs_res < tol) {
+    monitor(0.0, 0.0, rep.abs_res, rep.rel_res, rep.iterate_res, 0.0);
+    return rep;
+  }
 
   std::vector<double> z(static_cast<std::size_t>(n));
   precondition(A, F, r.data(), z.data());
   if (projected) project(z, R);
   double rz = dot(r, z);
   if (!std::isfinite(rz) || rz <= 0.0) throw std::runtime_error("pcg preconditioner breakdown: non-positive r^T M^{-1} r");
+  monitor(0.0, 0.0, rep.abs_res, rep.rel_res, rep.iterate_res, std::sqrt(std::max(rz, 0.0)));
   std::vector<double> p(z), ap(static_cast<std::size_t>(n));
   const int limit = std::max(1, max_iterations);
   for (int it = 0; it < limit; ++it) {
@@ -630,13 +648,17 @@ Report pcg(const Csr &A, const Factor &F, const std::vector<double> &b, int max_
     rep.iterate_res = std::sqrt(upd / std::max(xsq, 1.0e-30));
     rep.abs_res = std::sqrt(std::max(dot(r, r), 0.0));
     rep.rel_res = rep.abs_res / std::max(bnorm, 1.0e-30);
-    if (rep.rel_res < tol || rep.abs_res < tol) break;
+    if (rep.rel_res < tol || rep.abs_res < tol) {
+      monitor(alpha, 0.0, rep.abs_res, rep.rel_res, rep.iterate_res, 0.0);
+      break;
+    }
     precondition(A, F, r.data(), z.data());
     if (projected) project(z, R);
     const double rz_new = dot(r, z);
     if (!std::isfinite(rz_new) || rz_new <= 0.0)
       throw std::runtime_error("pcg preconditioner breakdown: non-positive new r^T M^{-1} r");
     const double beta = rz_new / rz;
+    monitor(alpha, beta, rep.abs_res, rep.rel_res, rep.iterate_res, std::sqrt(std::max(rz_new, 0.0)));
     for (int k = 0; k < n; ++k) {
       const std::size_t s = static_cast<std::size_t>(k);
       p[s] = z[s] + beta * p[s];
@@ -647,6 +669,12 @@ Report pcg(const Csr &A, const Factor &F, const std::vector<double> &b, int max_
   return rep;
 }
 
+void copy_log(const std::vector<OrMonitorRecord> &log, OrMonitorRecord *records, int capacity, int *count) {
+  const std::size_t m = std::min(log.size(), static_cast<std::size_t>(std::max(capacity, 0)));
+  if (records) std::copy(log.begin(), log.begin() + static_cast<std::ptrdiff_t>(m), records);
+  if (count) *count = static_cast<int>(log.size());
+}
+
 void set_err(char *err, int len, const char *msg) {
   if (err && len > 0) std::snprintf(err, static_cast<std::size_t>(len), "%s", msg);
 }
@@ -654,7 +682,8 @@ void set_err(char *err, int len, const char *msg) {
 
 int solve_common(int scheme, const OrProblem *pb, const double *rho, const double *u, const double *v,
                  const double *p_n, const double *p_nm1, const Coeffs *cf, int reduction, double *p_out,
-                 int *iterations, double *residual, char *err, int errlen) {
+                 int *iterations, double *residual, char *err, int errlen,
+                 std::vector<OrMonitorRecord> *log = nullptr) {
   try {
     const bool split = scheme == 3 || scheme == 4;
     const int kind = (scheme == 2 || scheme == 4) ? 2 : 1;
@@ -665,7 +694,7 @@ int solve_common(int scheme, const OrProblem *pb, const double *rho, const doubl
       throw std::runtime_error(kind == 2 ? "ILDLT preconditioner construction failed"
                                          : "ICC preconditioner construction failed");
     std::vector<double> x;
-    Report rep = pcg(sys.A, F, sys.rhs, pb->max_iterations, pb->tolerance, !sys.has_dirichlet, R, x);
+    Report rep = pcg(sys.A, F, sys.rhs, pb->max_iterations, pb->tolerance, !sys.has_dirichlet, R, x, log);
     const int n = pb->nx * pb->ny;
     for (int k = 0; k < n; ++k) p_out[k] = x[static_cast<std::size_t>(k)];
     if (!sys.has_dirichlet) {
@@ -736,6 +765,19 @@ int or_pressure_solve(int scheme, const OrProblem *pb, const double *rho, const 
                       const double *p_n, const double *p_nm1, int reduction, double *p_out, int *iterations,
                       double *residual, char *err, int errlen) {
   return solve_common(scheme, pb, rho, u, v, p_n, p_nm1, nullptr, reduction, p_out, iterations, residual, err, errlen);
+}
+
+// or_pressure_solve with the PCG monitor log: up to `capacity` records into `records`, *count =
+// the number of monitor calls (also on failure: the calls made before the breakdown).
+int or_pressure_solve_monitored(int scheme, const OrProblem *pb, const double *rho, const double *u, const double *v,
+                                const double *p_n, const double *p_nm1, int reduction, double *p_out, int *iterations,
+                                double *residual, OrMonitorRecord *records, int capacity, int *count, char *err,
+                                int errlen) {
+  std::vector<OrMonitorRecord> log;
+  const int rc = solve_common(scheme, pb, rho, u, v, p_n, p_nm1, nullptr, reduction, p_out, iterations, residual, err,
+                              errlen, &log);
+  copy_log(log, records, capacity, count);
+  return rc;
 }
 
 // Coefficient-level solve (no reference counterpart): scheme 1 or 2 on c_f = k_f / h^2 with
@@ -852,12 +894,31 @@ void or_sys_apply_precond(void *h, const double *r, double *z) {
   precondition(o->sys.A, o->F, r, z);
 }
 
+int or_sys_pcg_monitored(void *h, int max_iterations, double tolerance, double *x, int *iterations, double *rel,
+                         double *abs_res, OrMonitorRecord *records, int capacity, int *count, char *err, int errlen);
+
 int or_sys_pcg(void *h, int max_iterations, double tolerance, double *x, int *iterations, double *rel, double *abs_res,
                char *err, int errlen) {
+  return or_sys_pcg_monitored(h, max_iterations, tolerance, x, iterations, rel, abs_res, nullptr, 0, nullptr, err,
+                              errlen);
+}
+
+// or_sys_pcg with the PCG monitor log (see or_pressure_solve_monitored).
+int or_sys_pcg_monitored(void *h, int max_iterations, double tolerance, double *x, int *iterations, double *rel,
+                         double *abs_res, OrMonitorRecord *records, int capacity, int *count, char *err, int errlen) {
   auto *o = static_cast<OrSystem *>(h);
+  std::vector<OrMonitorRecord> log;
+  struct Flush {
+    std::vector<OrMonitorRecord> &log;
+    OrMonitorRecord *records;
+    int capacity;
+    int *count;
+    ~Flush() { copy_log(log, records, capacity, count); }
+  } flush{log, records, capacity, count};
   try {
     std::vector<double> xv;
-    const Report rep = pcg(o->sys.A, o->F, o->sys.rhs, max_iterations, tolerance, !o->sys.has_dirichlet, o->R, xv);
+    const Report rep = pcg(o->sys.A, o->F, o->sys.rhs, max_iterations, tolerance, !o->sys.has_dirichlet, o->R, xv,
+                           &log);
     std::copy(xv.begin(), xv.end(), x);
     *iterations = rep.iterations;
     *rel = rep.rel_res;

@@ -64,6 +64,9 @@ def lib():
         L.ref_sys_pcg.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, dp, ip, dp, dp,
                                   ctypes.c_char_p, ctypes.c_int]
         L.ref_sys_pcg.restype = ctypes.c_int
+        L.ref_sys_pcg_monitored.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, dp, ip, dp, dp,
+                                            ctypes.c_void_p, ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_int]
+        L.ref_sys_pcg_monitored.restype = ctypes.c_int
         L.ref_velocity_correction.argtypes = [ctypes.c_char_p, ctypes.POINTER(RefProblem), ctypes.POINTER(RefVelocityBc),
                                               ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp, ctypes.c_int, dp,
                                               ctypes.c_char_p, ctypes.c_int]
@@ -164,6 +167,21 @@ class RefSystem:
         if rc != 0:
             raise RuntimeError(err.value.decode())
         return x, it.value, rel.value, ab.value
+
+    def pcg_monitored(self, max_iterations, tolerance, capacity=None):
+        """The reference's PCG with a CGMonitor recording every call; returns (x, iterations, rel,
+        abs, records, error text or None) like the oracle's pcg_monitored."""
+        from oracle.cpu import MONITOR_DTYPE
+        cap = max_iterations + 1 if capacity is None else capacity
+        records = np.zeros(max(cap, 1), dtype=MONITOR_DTYPE)
+        x = np.zeros(self.n)
+        it, rel, ab, count = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        err = ctypes.create_string_buffer(512)
+        rc = lib().ref_sys_pcg_monitored(self.h, max_iterations, tolerance, _dp(x), ctypes.byref(it),
+                                         ctypes.byref(rel), ctypes.byref(ab), records.ctypes.data, cap,
+                                         ctypes.byref(count), err, 512)
+        return (x, it.value, rel.value, ab.value, records[:min(count.value, cap)],
+                err.value.decode() if rc != 0 else None)
 
     def __del__(self):
         if getattr(self, "h", None):

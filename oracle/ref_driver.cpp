@@ -288,14 +288,47 @@ void ref_sys_apply_precond(void *h, const double *r, double *z) {
 }
 
 // PCG from x0 = 0 with the reference's nullspace projection (pressure.cpp:50-57, 225-239).
+// The scalar fields of one CGMonitor call (CGMonitorData, ch_sparse_krylov.hpp:47-58).
+struct RefMonitorRecord {
+  int iteration, pad;
+  double alpha, beta, absolute_residual, relative_residual, iterate_residual, preconditioned_residual;
+};
+
+int ref_sys_pcg_monitored(void *h, int max_iterations, double tolerance, double *x, int *iterations, double *rel,
+                          double *abs_res, RefMonitorRecord *records, int capacity, int *count, char *err,
+                          int errlen);
+
 int ref_sys_pcg(void *h, int max_iterations, double tolerance, double *x, int *iterations, double *rel,
                 double *abs_res, char *err, int errlen) {
+  return ref_sys_pcg_monitored(h, max_iterations, tolerance, x, iterations, rel, abs_res, nullptr, 0, nullptr, err,
+                               errlen);
+}
+
+// The reference's solve_preconditioned_cg with a CGMonitor that records the scalar fields of every
+// call: up to `capacity` records, *count = the number of calls (also when the PCG throws).
+int ref_sys_pcg_monitored(void *h, int max_iterations, double tolerance, double *x, int *iterations, double *rel,
+                          double *abs_res, RefMonitorRecord *records, int capacity, int *count, char *err,
+                          int errlen) {
   auto *sys = static_cast<RefSystem *>(h);
+  int calls = 0;
+  const ding::ch_sparse_krylov::CGMonitor monitor = [&](const ding::ch_sparse_krylov::CGMonitorData &d) {
+    if (records && calls < capacity)
+      records[calls] = RefMonitorRecord{d.iteration, 0, d.alpha, d.beta, d.absolute_residual, d.relative_residual,
+                                        d.iterate_residual, d.preconditioned_residual};
+    ++calls;
+  };
+  struct Count {
+    int &calls;
+    int *count;
+    ~Count() {
+      if (count) *count = calls;
+    }
+  } counted{calls, count};
   try {
     std::vector<double> xv(static_cast<std::size_t>(sys->matrix.n), 0.0);
     const auto report = ding::ch_sparse_krylov::solve_preconditioned_cg(
         sys->matrix, sys->precond, sys->system.rhs, max_iterations, tolerance, xv,
-        sys->solver->pressure_nullspace_projection());
+        sys->solver->pressure_nullspace_projection(), monitor);
     std::copy(xv.begin(), xv.end(), x);
     *iterations = report.iterations;
     *rel = report.relative_residual;

@@ -6,6 +6,9 @@
 //   driver solve  <cfg> <scheme> <in.bin> <out.bin>
 //        in.bin: rho (nx*ny), u ((nx+1)*ny), v (nx*(ny+1)), p_n, p_nm1 (nx*ny each), doubles
 //        out.bin: pressure_correction with 3 ghost layers; prints "iterations residual"
+//   driver monitor <cfg> <scheme> <in.bin> <out.bin>
+//        as solve, with the PCG monitor: one line per record "k alpha beta abs rel iterate precond"
+//        after the "iterations residual" line
 //   driver correct <cfg> <vel> <scheme> <in.bin> <out.bin>
 //        in.bin: whole ghosted buffers (3 layers) rho_mid, u_star, v_star, p_corr, u, v, pressure,
 //        pressure_prev; out.bin: u, v, pressure, pressure_prev; prints "speed_pre speed_post"
@@ -109,7 +112,7 @@ int main(int argc, char **argv) {
     write_all(argv[3], f);
     return 0;
   }
-  if (mode == "solve" && argc == 6) {
+  if ((mode == "solve" || mode == "monitor") && argc == 6) {
     const Cfg c = parse(argv[2]);
     const int nx = c.nx, ny = c.ny, g = 3;
     const std::vector<double> in = read_all(argv[4]);
@@ -120,9 +123,17 @@ int main(int argc, char **argv) {
     Owned out(nx, ny, g, nullptr);
     try {
       ppamd::PressureBackend backend(0);
-      const double res = backend.solve(argv[3], c.s, R.view, U.view, V.view, P.view, PM.view, c.step, out.view);
+      std::vector<pp_monitor_record> log;
+      const double res =
+          mode == "monitor"
+              ? backend.solve(argv[3], c.s, R.view, U.view, V.view, P.view, PM.view, c.step, out.view,
+                              [&](const pp_monitor_record &m) { log.push_back(m); })
+              : backend.solve(argv[3], c.s, R.view, U.view, V.view, P.view, PM.view, c.step, out.view);
       write_all(argv[5], out.buf);
       std::printf("%d %.17g\n", backend.last_iterations(), res);
+      for (const pp_monitor_record &m : log)
+        std::printf("%d %.17g %.17g %.17g %.17g %.17g %.17g\n", m.iteration, m.alpha, m.beta, m.absolute_residual,
+                    m.relative_residual, m.iterate_residual, m.preconditioned_residual);
     } catch (const std::exception &e) {
       std::printf("error %s\n", e.what());
       return 1;

@@ -81,3 +81,17 @@ def test_velocity_bc_struct_layout_matches_header():
     fields = re.findall(r"\b(?:int|double)\s+([a-z_]+)", body)
     assert names == fields
     assert ctypes.sizeof(engine.PPVelocityBC) == 4 * 4 + 4 * 8 + 4 * 4 + 4 * 8 + 2 * 8
+
+
+def test_monitor_record_layout_matches_header():
+    """MONITOR_DTYPE (engine and oracle) has pp_monitor_record's fields in order: 56 bytes, no padding
+    beyond the explicit pad after the iteration index."""
+    from ding_nsch_core_solver_amd import engine
+    from oracle import cpu
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct pp_monitor_record"):text.index("} pp_monitor_record;")]
+    fields = re.findall(r"\b(?:int|double)\s+([a-z_]+)", body)
+    assert list(engine.MONITOR_DTYPE.names) == fields
+    assert engine.MONITOR_DTYPE == cpu.MONITOR_DTYPE
+    assert engine.MONITOR_DTYPE.itemsize == 2 * 4 + 6 * 8
+    assert engine.MONITOR_DTYPE.fields["alpha"][1] == 8

@@ -143,3 +143,25 @@ def test_velocity_correction_through_cpp_mirror(driver, oracle_lib, tmp_path, sc
     u, v, p, prev, speeds = oracle_lib.velocity_correction(case, vbc, split, G, **f)
     assert np.array_equal(np.fromfile(outp), np.concatenate([u, v, p, prev]))
     assert (pre, post) == tuple(speeds)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme,code", [("hip_icpcg", E.PP_ICPCG), ("hip_split_ildlt_pcg", E.PP_SPLIT_ILDLT_PCG)])
+def test_monitor_through_cpp_mirror(driver, engine, tmp_path, scheme, code):
+    """PressureBackend::solve with a CGMonitor replays the engine's PCG log in order, bit for bit."""
+    case = W.droplet_case(40, 36, seed=8, density_ratio=300.0, step_index=1)
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    z = np.zeros(case.n)
+    np.concatenate([case.rho.reshape(-1), case.u.reshape(-1), case.v.reshape(-1), z, z]).tofile(inp)
+    res = subprocess.run([driver, "monitor", _cfg(case), scheme, str(inp), str(outp)], check=True,
+                         capture_output=True, text=True)
+    lines = res.stdout.split("\n")
+    it = int(lines[0].split()[0])
+    rows = [line.split() for line in lines[1:] if line]
+    case.p_n, case.p_nm1 = z, z
+    p, rep, rec = engine.solve_monitored(case, code)
+    assert it == rep["iterations"] and len(rows) == len(rec) == it + 1
+    for row, r in zip(rows, rec):
+        assert int(row[0]) == r["iteration"]
+        assert [float(x) for x in row[1:]] == [r["alpha"], r["beta"], r["absolute_residual"], r["relative_residual"],
+                                               r["iterate_residual"], r["preconditioned_residual"]]
