@@ -389,6 +389,17 @@ __global__ void __launch_bounds__(kThreads) correction_finish_kernel(const CorrA
   }
 }
 
+// apply_pressure_bc (operators.cpp:143-181) on one ghosted nx x ny field in place, one block.
+__global__ void __launch_bounds__(kThreads) pressure_ghost_kernel(const CorrArgs *__restrict__ args, double *field) {
+  const CorrArgs &a = *args;
+  pressure_ghosts(a, Ghosted{field, a.ny, a.G});
+}
+
+hipError_t launch_pressure_ghosts(const CorrArgs *d_args, double *d_field, hipStream_t stream) {
+  hipLaunchKernelGGL(pressure_ghost_kernel, dim3(1), dim3(kThreads), 0, stream, d_args, d_field);
+  return hipGetLastError();
+}
+
 hipError_t launch_velocity_correction(const CorrArgs *d_args, int nx, int ny, int ghost, hipStream_t stream) {
   const int faces = (nx + 1) * ny + nx * (ny + 1);
   const int cells = (nx + 2 * ghost) * (ny + 2 * ghost);

@@ -238,6 +238,8 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
     auto t = ctx->tuning.find(std::make_pair(pb->nx, pb->ny));
     if (t != ctx->tuning.end()) a.ring = t->second;
   }
+  a.ld_rho = a.ld_u = a.ld_pn = a.ld_pnm1 = a.ld_out = pb->ny;
+  a.ld_v = pb->ny + 1;
   a.lev_base = L.base + pp::kLevPad;
   for (int w = 0; w <= pp::kWaves; ++w) a.spmv_split[w] = L.split[w];
   a.cell = L.cell;
@@ -438,19 +440,14 @@ int run_adjoint(pp_ctx *ctx, const pp_problem *pb, const double *d_ku, const dou
   return PP_OK;
 }
 
-// Velocity correction (operators.cpp:480-534): resolved boundary conditions and the launch.
-int run_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost,
-                            const double *d_rho_mid, const double *d_u_star, const double *d_v_star,
-                            const double *d_p_corr, double *d_u, double *d_v, double *d_pressure,
-                            double *d_pressure_prev, double *speeds, hipStream_t s) {
-  pp_report *report = nullptr;
-  const int nx = pb->nx, ny = pb->ny, G = ghost;
-  if (!ctx->corr_work) PP_HIP_CHECK(hipMalloc(&ctx->corr_work, 4 * sizeof(double)));
-  if (!ctx->d_corr_args) PP_HIP_CHECK(hipMalloc(&ctx->d_corr_args, sizeof(pp::CorrArgs)));
+// Argument block of the correction / ghost-fill kernels: grid, effective pressure BCs
+// (operators.cpp:36-60: unset -> Neumann 0) and, when vbc is given, effective velocity BCs
+// (62-112: unset -> no-slip, u bottom/top at the wall speed). Field pointers are the caller's.
+pp::CorrArgs corr_args(const pp_problem *pb, const pp_velocity_bc *vbc, int G) {
   pp::CorrArgs c;
   std::memset(&c, 0, sizeof(c));
-  c.nx = nx;
-  c.ny = ny;
+  c.nx = pb->nx;
+  c.ny = pb->ny;
   c.G = G;
   c.dx = pb->dx;
   c.dy = pb->dy;
@@ -458,11 +455,10 @@ int run_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity
   c.periodic_x = pb->periodic_x ? 1 : 0;
   c.periodic_y = pb->periodic_y ? 1 : 0;
   for (int k = 0; k < 4; ++k) {
-    // effective_pressure_bc: unset -> Neumann 0 (operators.cpp:36-60)
     const bool p_unset = pb->bc_type[k] == PP_BC_UNSET;
     c.p_type[k] = p_unset ? PP_BC_NEUMANN : pb->bc_type[k];
     c.p_value[k] = p_unset ? 0.0 : pb->bc_value[k];
-    // effective_u_bc / effective_v_bc: unset -> no-slip, u bottom/top at the wall speed (62-112)
+    if (!vbc) continue;
     const bool u_unset = vbc->u_type[k] == PP_BC_UNSET;
     const double u_default = k == 2 ? vbc->bottom_wall_velocity_x : (k == 3 ? vbc->top_wall_velocity_x : 0.0);
     c.u_type[k] = u_unset ? PP_BC_DIRICHLET : vbc->u_type[k];
@@ -475,6 +471,25 @@ int run_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity
   c.step_index = pb->step_index;
   c.has_dirichlet = has_dirichlet(pb) ? 1 : 0;
   c.rho_ref = c.split ? std::min(1.0, pb->density_ratio) : 1.0;
+  return c;
+}
+
+int ensure_corr(pp_ctx *ctx) {
+  pp_report *report = nullptr;
+  if (!ctx->corr_work) PP_HIP_CHECK(hipMalloc(&ctx->corr_work, 4 * sizeof(double)));
+  if (!ctx->d_corr_args) PP_HIP_CHECK(hipMalloc(&ctx->d_corr_args, sizeof(pp::CorrArgs)));
+  return PP_OK;
+}
+
+// Velocity correction (operators.cpp:480-534): resolved boundary conditions and the launch.
+int run_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost,
+                            const double *d_rho_mid, const double *d_u_star, const double *d_v_star,
+                            const double *d_p_corr, double *d_u, double *d_v, double *d_pressure,
+                            double *d_pressure_prev, double *speeds, hipStream_t s) {
+  pp_report *report = nullptr;
+  const int nx = pb->nx, ny = pb->ny, G = ghost;
+  if (int rc = ensure_corr(ctx)) return rc;
+  pp::CorrArgs c = corr_args(pb, vbc, G);
   c.rho_mid = d_rho_mid;
   c.u_star = d_u_star;
   c.v_star = d_v_star;
@@ -641,6 +656,41 @@ int pp_solve_device_monitored(pp_ctx *ctx, const pp_problem *pb, const double *d
   if (int rc = validate_monitor(ctx, pb, d_records, capacity, report)) return rc;
   return solve_device(ctx, pb, d_rho, d_u, d_v, d_p_n, d_p_nm1, d_p_out, reinterpret_cast<pp::MonitorRecord *>(d_records),
                       capacity, count, report, stream);
+}
+
+int pp_solve_device_field(pp_ctx *ctx, const pp_problem *pb, int ghost, const double *d_rho_mid,
+                          const double *d_u_star, const double *d_v_star, const double *d_pressure,
+                          const double *d_pressure_prev, double *d_p_corr, pp_report *report, void *stream) {
+  if (int rc = validate(ctx, pb, report)) return rc;
+  if (ghost < 1 || ghost > 8) return fail(ctx, report, PP_ERR_ARG, "ghost must be 1..8");
+  const bool split = pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG;
+  if (!d_rho_mid && !split) return fail(ctx, report, PP_ERR_ARG, "rho is required");
+  if (!d_u_star || !d_v_star || !d_p_corr) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  const int G = ghost, nx = pb->nx, ny = pb->ny;
+  const int lc = ny + 2 * G, lv = ny + 1 + 2 * G;
+  auto at00 = [&](const double *f, int ld) { return f ? f + static_cast<size_t>(G) * ld + G : nullptr; };
+  pp::SysArgs a;
+  if (int rc = prepare(ctx, pb, a, report)) return rc;
+  a.rho = at00(d_rho_mid, lc);
+  a.u = at00(d_u_star, lc);
+  a.v = at00(d_v_star, lv);
+  a.p_n = split ? at00(d_pressure, lc) : nullptr;
+  a.p_nm1 = split ? at00(d_pressure_prev, lc) : nullptr;
+  a.p_out = d_p_corr + static_cast<size_t>(G) * lc + G;
+  a.ld_rho = a.ld_u = a.ld_pn = a.ld_pnm1 = a.ld_out = lc;
+  a.ld_v = lv;
+  // the reference zero-fills pressure_correction_ first: the ghost passes read deeper ghost cells
+  // of that zero field before rewriting them, and a failed solve leaves zeros
+  PP_HIP_CHECK(hipMemsetAsync(d_p_corr, 0, static_cast<size_t>(nx + 2 * G) * lc * sizeof(double), s));
+  if (int rc = run_solve(ctx, pb, a, report, s)) return rc;
+  if (int rc = ensure_corr(ctx)) return rc;
+  const pp::CorrArgs c = corr_args(pb, nullptr, G);
+  PP_HIP_CHECK(hipMemcpyAsync(ctx->d_corr_args, &c, sizeof(c), hipMemcpyHostToDevice, s));
+  PP_HIP_CHECK(pp::launch_pressure_ghosts(ctx->d_corr_args, d_p_corr, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  return PP_OK;
 }
 
 int pp_solve(pp_ctx *ctx, const pp_problem *pb, const double *rho, int ld_rho, const double *u, int ld_u,

@@ -39,6 +39,9 @@ struct SysArgs {
   // fields (natural layout)
   const double *rho, *u, *v, *p_n, *p_nm1;
   double *p_out;
+  // row pitches of the fields in doubles (natural: ny, ny, ny + 1, ny, ny, ny); a ghosted Field2D
+  // buffer is passed as &field(0, 0) with its own pitch
+  int ld_rho, ld_u, ld_v, ld_pn, ld_pnm1, ld_out;
   // coefficient mode (coeff_mode = 1): face inverse densities k_u ((nx+1) x ny), k_v (nx x (ny+1))
   // and a source term src (nx x ny) replace rho, u and v; c_f = k_f / h^2, b = src (+ the boundary
   // terms when rhs_bc = 1)
@@ -114,5 +117,9 @@ struct CorrArgs {
 // Faces, history and finish kernels of the velocity correction on `stream` (nx, ny, ghost as in
 // the argument block, for the grid sizes).
 hipError_t launch_velocity_correction(const CorrArgs *d_args, int nx, int ny, int ghost, hipStream_t stream);
+
+// apply_pressure_bc on d_field (nx x ny with G ghost layers, Field2D layout) per the pressure BCs
+// and periodicity of the argument block; one block on `stream`.
+hipError_t launch_pressure_ghosts(const CorrArgs *d_args, double *d_field, hipStream_t stream);
 
 }  // namespace pp

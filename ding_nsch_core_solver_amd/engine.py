@@ -88,7 +88,7 @@ _lib = None
 EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_solve", "pp_solve_device",
            "pp_solve_coeffs", "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device",
            "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device", "pp_solve_monitored",
-           "pp_solve_device_monitored")
+           "pp_solve_device_monitored", "pp_solve_device_field")
 
 # One PCG monitor record (pp_monitor_record): the scalar fields of one CGMonitor call.
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
@@ -149,6 +149,9 @@ def lib():
         L.pp_solve_device_monitored.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, vp, vp, vp,
                                                 ctypes.c_int, ip, ctypes.POINTER(PPReport), vp]
         L.pp_solve_device_monitored.restype = ctypes.c_int
+        L.pp_solve_device_field.argtypes = [vp, ctypes.POINTER(PPProblem), ctypes.c_int, vp, vp, vp, vp, vp, vp,
+                                            ctypes.POINTER(PPReport), vp]
+        L.pp_solve_device_field.restype = ctypes.c_int
         L.pp_solve_coeffs.argtypes = [vp, ctypes.POINTER(PPProblem), dp, dp, dp, dp, ctypes.POINTER(PPReport)]
         L.pp_solve_coeffs.restype = ctypes.c_int
         L.pp_solve_coeffs_device.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, ctypes.POINTER(PPReport), vp]
@@ -321,6 +324,18 @@ class Engine:
         report = _report_dict(rep)
         report["monitor_calls"] = count.value
         return report
+
+    def solve_device_field(self, case, d_rho_mid, d_u_star, d_v_star, d_pressure, d_pressure_prev, d_p_corr,
+                           scheme: int = PP_ICPCG, precision: int = PP_FP64, ghost: int = 3, stream=None, **kw):
+        """The backend contract on ghosted Field2D device buffers (see pp_solve_device_field):
+        d_p_corr receives pressure_correction_ with its ghosts. Returns the report dict."""
+        pb = make_problem(case, scheme, precision, **kw)
+        rep = PPReport()
+        rc = self._lib.pp_solve_device_field(self.h, ctypes.byref(pb), int(ghost), d_rho_mid, d_u_star, d_v_star,
+                                             d_pressure, d_pressure_prev, d_p_corr, ctypes.byref(rep), stream)
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return _report_dict(rep)
 
     def solve_device(self, case, d_rho, d_u, d_v, d_pn, d_pnm1, d_out, scheme: int = PP_ICPCG,
                      precision: int = PP_FP64, stream=None, **kw):

@@ -127,6 +127,20 @@ int pp_solve_device(pp_ctx *ctx, const pp_problem *pb, const double *d_rho, cons
                     pp_report *report, void *stream);
 
 /*
+ * The reference backend's whole contract on device-resident Field2D buffers (pressure.cpp:302-365,
+ * 470-491 from the zero-fill of pressure_correction_ through apply_pressure_bc,
+ * operators.cpp:143-181). Every field is a whole ghosted buffer with `ghost` layers, element (i, j)
+ * of an a x b field at (i + ghost) * (b + 2 ghost) + j + ghost: rho_mid (nx x ny), u_star
+ * ((nx+1) x ny), v_star (nx x (ny+1)), pressure / pressure_prev (nx x ny, read by split schemes,
+ * may be NULL otherwise). d_p_corr (nx x ny) receives pressure_correction_: zero-filled, the
+ * interior solved, ghosts filled as apply_pressure_bc fills them (all zeros after a failure). The
+ * output feeds pp_velocity_correction_device directly. Synchronous on `stream`.
+ */
+int pp_solve_device_field(pp_ctx *ctx, const pp_problem *pb, int ghost, const double *d_rho_mid,
+                          const double *d_u_star, const double *d_v_star, const double *d_pressure,
+                          const double *d_pressure_prev, double *d_p_corr, pp_report *report, void *stream);
+
+/*
  * Per-iteration PCG log: the scalar fields of one call of the reference's CGMonitor
  * (CGMonitorData, ch_sparse_krylov.hpp:47-58; the calls at 547-555, 569-579, 612-621 and
  * 636-648), the vectors left out. Record k belongs to iteration k: record 0 is the initial

@@ -82,6 +82,7 @@ def lib():
                                  ctypes.c_char_p, ctypes.c_int]
         L.or_sys_pcg.restype = ctypes.c_int
         L.or_apply_pressure_bc.argtypes = [ctypes.POINTER(OrProblem), dp]
+        L.or_pressure_ghosts.argtypes = [ctypes.POINTER(OrProblem), ctypes.c_int, dp]
         L.or_solve_coeffs.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), dp, dp, dp, ctypes.c_int, dp, ip, dp,
                                       ctypes.c_char_p, ctypes.c_int]
         L.or_solve_coeffs.restype = ctypes.c_int
@@ -187,6 +188,13 @@ def apply_pressure_bc(case, field_ghosted: np.ndarray) -> np.ndarray:
     """Ghost fill (3 layers) of a pressure-like Field2D buffer, in place; returns it."""
     assert field_ghosted.size == (case.nx + 6) * (case.ny + 6)
     lib().or_apply_pressure_bc(ctypes.byref(problem(case)), _dp(field_ghosted))
+    return field_ghosted
+
+
+def pressure_ghosts(case, field_ghosted: np.ndarray, ghost: int) -> np.ndarray:
+    """Ghost fill of a pressure-like Field2D buffer with `ghost` layers, in place; returns it."""
+    assert field_ghosted.size == (case.nx + 2 * ghost) * (case.ny + 2 * ghost)
+    lib().or_pressure_ghosts(ctypes.byref(problem(case)), int(ghost), _dp(field_ghosted))
     return field_ghosted
 
 

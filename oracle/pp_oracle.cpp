@@ -1119,4 +1119,7 @@ void or_velocity_correction(int split, const OrProblem *pb, const OrVelocityBc *
 // Ghost fill of a pressure-like field with 3 ghost layers (operators.cpp:143-181).
 void or_apply_pressure_bc(const OrProblem *pb, double *f) { fill_pressure_ghosts(*pb, 3, f); }
 
+// apply_pressure_bc (operators.cpp:143-181) on a ghosted nx x ny field with G ghost layers.
+void or_pressure_ghosts(const OrProblem *pb, int G, double *f) { fill_pressure_ghosts(*pb, G, f); }
+
 }  // extern "C"

@@ -67,6 +67,8 @@ def lib():
         L.ref_sys_pcg_monitored.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, dp, ip, dp, dp,
                                             ctypes.c_void_p, ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_int]
         L.ref_sys_pcg_monitored.restype = ctypes.c_int
+        L.ref_pressure_ghosts.argtypes = [ctypes.POINTER(RefProblem), ctypes.c_int, dp]
+        L.ref_pressure_ghosts.restype = ctypes.c_int
         L.ref_velocity_correction.argtypes = [ctypes.c_char_p, ctypes.POINTER(RefProblem), ctypes.POINTER(RefVelocityBc),
                                               ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp, ctypes.c_int, dp,
                                               ctypes.c_char_p, ctypes.c_int]
@@ -214,3 +216,10 @@ def velocity_correction(scheme: str, case, vbc, ghost: int, rho_mid, u_star, v_s
     if timing is not None:
         timing.append(float(seconds[0]))
     return u, v, pressure, prev, (float(speeds[0]), float(speeds[1]))
+
+
+def pressure_ghosts(case, field_ghosted: np.ndarray, ghost: int) -> np.ndarray:
+    """The reference's apply_pressure_bc on a Field2D buffer with `ghost` layers, in place."""
+    assert field_ghosted.size == (case.nx + 2 * ghost) * (case.ny + 2 * ghost)
+    lib().ref_pressure_ghosts(ctypes.byref(problem(case)), int(ghost), _dp(field_ghosted))
+    return field_ghosted

@@ -222,6 +222,16 @@ int ref_apply_pressure_bc(const RefProblem *pb, double *field_ghosted) {
   return 0;
 }
 
+// apply_pressure_bc on a Field2D buffer with G ghost layers (the loop runs over field.ghost).
+int ref_pressure_ghosts(const RefProblem *pb, int G, double *field_ghosted) {
+  auto s = std::make_unique<ding::Solver>(make_config(*pb, "icpcg"));
+  ding::Field2D f(pb->nx, pb->ny, G, 0.0);
+  std::memcpy(f.data.data(), field_ghosted, f.data.size() * sizeof(double));
+  s->apply_pressure_bc(f);
+  std::memcpy(field_ghosted, f.data.data(), f.data.size() * sizeof(double));
+  return 0;
+}
+
 // Assemble (split or not), regularize and convert to CSR with the reference's own functions.
 void *ref_sys_build(const RefProblem *pb, int use_split, const double *rho, const double *u, const double *v,
                     const double *p_n, const double *p_nm1, char *err, int errlen) {

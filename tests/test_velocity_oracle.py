@@ -102,3 +102,15 @@ def test_oracle_invariants(oracle_lib):
     u2, *_ = oracle_lib.velocity_correction(case, vbc, False, G, **flat)
     us = f["u_star"].reshape(W.ghosted_shape(nx + 1, ny, G))
     assert np.array_equal(u2.reshape(ug.shape)[G + 1:G + nx, G:-G], us[G + 1:G + nx, G:-G])
+
+
+@pytest.mark.parametrize("ghost", [1, 2, 3, 4])
+@pytest.mark.parametrize("pname", sorted(PRESSURE_BC))
+def test_pressure_ghosts_match_reference(ref_lib, oracle_lib, pname, ghost):
+    """apply_pressure_bc (operators.cpp:143-181) for any ghost depth, on a field whose ghost cells
+    hold random values beforehand (the y passes read deeper x-ghost cells before rewriting them)."""
+    case = _case(pname, shape=(7, 5))
+    f = np.random.default_rng(zlib.crc32(f"{pname}/{ghost}".encode())).standard_normal(W.ghosted_shape(7, 5, ghost))
+    got = oracle_lib.pressure_ghosts(case, f.reshape(-1).copy(), ghost)
+    expect = ref_lib.pressure_ghosts(case, f.reshape(-1).copy(), ghost)
+    assert np.array_equal(got, expect)
