@@ -5,11 +5,16 @@ Run in a container where the reference sources exist (``make -C oracle ref`` fir
 Each fixture is an .npz with the case inputs, the reference's CSR operator and regularized RHS
 (pressure.cpp:123-212 + ch_sparse_krylov.hpp:105-133), its ICC(0)/ILDLT(0) factor
 (ch_sparse_krylov.hpp:252-419), one SpMV and one preconditioner application on a seeded probe
-vector (235-250, 444-513), and the full Solver::solve_pressure_correction result (pressure.cpp:1031).
+vector (235-250, 444-513), the scalar CGMonitor log of the reference's PCG on that system
+(ch_sparse_krylov.hpp:47-58, 515-660) and the full Solver::solve_pressure_correction result
+(pressure.cpp:1031). The velcorr_*.npz fixtures hold the inputs and outputs of
+Solver::apply_pressure_velocity_correction (operators.cpp:480-534) on whole ghosted buffers, and
+of apply_pressure_bc (143-181) on a random ghosted field.
 The files hold data only (no code) and load with numpy.load(allow_pickle=False).
 """
 import os
 import sys
+import zlib
 
 import numpy as np
 
@@ -54,6 +59,7 @@ def write(name, scheme, case):
     ok, lower, diag, fl = rs.factor(kind)
     probe = np.random.default_rng(2024).standard_normal(rs.n)
     p, it, res = ref.pressure_solve(scheme, case)
+    monitor = rs.pcg_monitored(case.max_iterations, case.tolerance)[4] if ok else np.zeros(0)
     empty = np.zeros(0)
     np.savez_compressed(
         os.path.join(HERE, name + ".npz"),
@@ -66,8 +72,53 @@ def write(name, scheme, case):
         row_ptr=rs.row_ptr, col_idx=rs.col_idx, values=rs.values, rhs=rs.rhs,
         factor_ok=int(ok), floor=fl, lower=lower, diag=diag,
         probe=probe, probe_Ax=rs.apply_matrix(probe), probe_Mx=rs.apply_precond(probe),
-        iterations=it, residual=res, pressure=p)
-    print(f"{name}: n={rs.n} nnz={rs.nnz} iterations={it} residual={res:.3e}")
+        iterations=it, residual=res, pressure=p, monitor=monitor)
+    print(f"{name}: n={rs.n} nnz={rs.nnz} iterations={it} residual={res:.3e} monitor records={len(monitor)}")
+
+
+def velcorr_cases():
+    """(name, case, velocity BCs, scheme, ghost) of the correction fixtures."""
+    out = []
+    out.append(("velcorr_11x7_neumann_lid", W.droplet_case(11, 7, seed=31, density_ratio=80.0, lx=1.1, ly=0.7),
+                W.VelocityBC(top_wall_velocity_x=1.0), "icpcg", 3))
+    out.append(("velcorr_9x10_split_periodic_x",
+                W.droplet_case(9, 10, seed=32, density_ratio=300.0, periodic_x=True, step_index=4,
+                               bc_type=(U, U, D, N), bc_value=(0.0, 0.0, 1.25, -0.5)),
+                W.VelocityBC(u_type=(U, U, N, D), u_value=(0.0, 0.0, 0.2, 0.6), v_type=(U, U, D, N),
+                             v_value=(0.0, 0.0, -0.1, 0.3)), "split_icpcg", 3))
+    out.append(("velcorr_8x8_dirichlet_g2_periodic_y",
+                W.droplet_case(8, 8, seed=33, density_ratio=20.0, periodic_y=True, bc_type=(D, N, U, U),
+                               bc_value=(2.0, 0.75, 0.0, 0.0)),
+                W.VelocityBC(u_type=(D, D, U, U), u_value=(0.4, -0.3, 0.0, 0.0), v_type=(D, N, U, U),
+                             v_value=(0.1, 0.05, 0.0, 0.0)), "split_icpcg", 2))
+    return out
+
+
+def write_velcorr(name, case, vbc, scheme, g):
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    nx, ny = case.nx, case.ny
+    cell, ush, vsh = W.ghosted_shape(nx, ny, g), W.ghosted_shape(nx + 1, ny, g), W.ghosted_shape(nx, ny + 1, g)
+    f = dict(rho_mid=(1.0 + 60.0 * rng.random(cell)).reshape(-1), u_star=rng.standard_normal(ush).reshape(-1),
+             v_star=rng.standard_normal(vsh).reshape(-1), p_corr=(rng.standard_normal(cell) * 1e2).reshape(-1),
+             u=rng.standard_normal(ush).reshape(-1), v=rng.standard_normal(vsh).reshape(-1),
+             pressure=(rng.standard_normal(cell) * 1e2).reshape(-1),
+             pressure_prev=(rng.standard_normal(cell) * 1e2).reshape(-1))
+    u, v, pr, prev, speeds = ref.velocity_correction(scheme, case, vbc, g, **f)
+    ghost_in = rng.standard_normal(cell).reshape(-1)
+    ghost_out = ref.pressure_ghosts(case, ghost_in.copy(), g)
+    np.savez_compressed(
+        os.path.join(HERE, name + ".npz"),
+        scheme=np.array(scheme), ghost=g, nx=nx, ny=ny, lx=case.lx, ly=case.ly, dt=case.dt,
+        periodic_x=int(case.periodic_x), periodic_y=int(case.periodic_y),
+        bc_type=np.array(case.bc_type, dtype=np.int32), bc_value=np.array(case.bc_value, dtype=np.float64),
+        density_ratio=case.density_ratio, step_index=case.step_index,
+        u_type=np.array(vbc.u_type, dtype=np.int32), u_value=np.array(vbc.u_value, dtype=np.float64),
+        v_type=np.array(vbc.v_type, dtype=np.int32), v_value=np.array(vbc.v_value, dtype=np.float64),
+        bottom_wall_velocity_x=vbc.bottom_wall_velocity_x, top_wall_velocity_x=vbc.top_wall_velocity_x,
+        **{"in_" + k: a for k, a in f.items()},
+        out_u=u, out_v=v, out_pressure=pr, out_pressure_prev=prev, out_speeds=np.array(speeds),
+        ghost_in=ghost_in, ghost_out=ghost_out)
+    print(f"{name}: speeds={speeds}")
 
 
 if __name__ == "__main__":
@@ -75,3 +126,5 @@ if __name__ == "__main__":
         sys.exit("build the reference checker first: make -C oracle ref")
     for name, scheme, case in cases():
         write(name, scheme, case)
+    for name, case, vbc, scheme, g in velcorr_cases():
+        write_velcorr(name, case, vbc, scheme, g)

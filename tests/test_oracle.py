@@ -66,7 +66,8 @@ def test_oracle_operators_match_reference_bitwise(oracle_lib, ref_lib, name, cas
 
 
 def _golden_files():
-    return sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+    """The pressure-solve fixtures (velcorr_*.npz hold the correction step; test_velocity_oracle.py)."""
+    return sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz")) if not os.path.basename(p).startswith("velcorr_"))
 
 
 def _case_from_npz(z):
@@ -108,6 +109,9 @@ def test_oracle_matches_golden_vectors(oracle_lib, path):
         assert it == int(z["iterations"])
         assert res == float(z["residual"])
         assert np.array_equal(p, z["pressure"])
+        # the reference's CGMonitor log of its PCG on this system
+        log = os_.pcg_monitored(case.max_iterations, case.tolerance)[4]
+        assert log.tobytes() == z["monitor"].tobytes()
 
 
 @pytest.mark.parametrize("use_split", [False, True])

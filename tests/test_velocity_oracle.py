@@ -4,7 +4,9 @@ every periodicity, pressure and velocity boundary kind (unset defaults, Dirichle
 speeds), split and non-split schemes at step 0 and later steps. Every input buffer, ghost cells
 included, is random: the reference reads the old ghost cells of u, v and pressure, so they are part
 of the contract."""
+import glob
 import itertools
+import os
 import zlib
 
 import numpy as np
@@ -114,3 +116,34 @@ def test_pressure_ghosts_match_reference(ref_lib, oracle_lib, pname, ghost):
     got = oracle_lib.pressure_ghosts(case, f.reshape(-1).copy(), ghost)
     expect = ref_lib.pressure_ghosts(case, f.reshape(-1).copy(), ghost)
     assert np.array_equal(got, expect)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+VELCORR = sorted(glob.glob(os.path.join(GOLDEN, "velcorr_*.npz")))
+
+
+def test_velcorr_fixtures_exist():
+    assert len(VELCORR) >= 3, "correction fixtures missing: run tests/golden/make_golden.py"
+
+
+@pytest.mark.parametrize("path", VELCORR, ids=[os.path.basename(p) for p in VELCORR])
+def test_oracle_matches_velcorr_golden(oracle_lib, path):
+    """Fixtures written from the compiled reference (tests/golden/make_golden.py): the oracle's
+    correction and ghost fill reproduce them bit for bit without the reference present."""
+    with np.load(path, allow_pickle=False) as z:
+        case = W.PressureCase(nx=int(z["nx"]), ny=int(z["ny"]), lx=float(z["lx"]), ly=float(z["ly"]),
+                              dt=float(z["dt"]), periodic_x=bool(z["periodic_x"]), periodic_y=bool(z["periodic_y"]),
+                              bc_type=tuple(int(t) for t in z["bc_type"]),
+                              bc_value=tuple(float(t) for t in z["bc_value"]), density_ratio=float(z["density_ratio"]),
+                              step_index=int(z["step_index"]), rho=np.zeros(0), u=np.zeros(0), v=np.zeros(0))
+        vbc = W.VelocityBC(u_type=tuple(int(t) for t in z["u_type"]), u_value=tuple(float(t) for t in z["u_value"]),
+                           v_type=tuple(int(t) for t in z["v_type"]), v_value=tuple(float(t) for t in z["v_value"]),
+                           bottom_wall_velocity_x=float(z["bottom_wall_velocity_x"]),
+                           top_wall_velocity_x=float(z["top_wall_velocity_x"]))
+        g = int(z["ghost"])
+        f = {k: z["in_" + k] for k in ("rho_mid", "u_star", "v_star", "p_corr", "u", "v", "pressure", "pressure_prev")}
+        u, v, p, prev, speeds = oracle_lib.velocity_correction(case, vbc, str(z["scheme"]).startswith("split"), g, **f)
+        for name, a in (("u", u), ("v", v), ("pressure", p), ("pressure_prev", prev)):
+            assert np.array_equal(a, z["out_" + name]), name
+        assert tuple(speeds) == tuple(z["out_speeds"])
+        assert np.array_equal(oracle_lib.pressure_ghosts(case, z["ghost_in"].copy(), g), z["ghost_out"])
