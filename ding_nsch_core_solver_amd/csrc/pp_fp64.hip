@@ -21,25 +21,24 @@ namespace pp {
 namespace {
 
 __device__ __forceinline__ double rho_xface(const SysArgs &a, int i, int j) {
-  const int nx = a.nx;
+  const int nx = a.nx, ny = a.ny;
   const double *rho = a.rho;
-  const int ld = a.ld_rho;
-  if (i <= 0) return a.periodic_x ? 0.5 * (rho[(nx - 1) * ld + j] + rho[j]) : rho[j];
-  if (i >= nx) return a.periodic_x ? 0.5 * (rho[(nx - 1) * ld + j] + rho[j]) : rho[(nx - 1) * ld + j];
-  return 0.5 * (rho[(i - 1) * ld + j] + rho[i * ld + j]);
+  if (i <= 0) return a.periodic_x ? 0.5 * (rho[(nx - 1) * ny + j] + rho[j]) : rho[j];
+  if (i >= nx) return a.periodic_x ? 0.5 * (rho[(nx - 1) * ny + j] + rho[j]) : rho[(nx - 1) * ny + j];
+  return 0.5 * (rho[(i - 1) * ny + j] + rho[i * ny + j]);
 }
 
 __device__ __forceinline__ double rho_yface(const SysArgs &a, int i, int j) {
   const int ny = a.ny;
-  const double *rho = a.rho + static_cast<size_t>(i) * a.ld_rho;
-  if (j <= 0) return a.periodic_y ? 0.5 * (rho[ny - 1] + rho[0]) : rho[0];
-  if (j >= ny) return a.periodic_y ? 0.5 * (rho[ny - 1] + rho[0]) : rho[ny - 1];
-  return 0.5 * (rho[j - 1] + rho[j]);
+  const double *rho = a.rho;
+  if (j <= 0) return a.periodic_y ? 0.5 * (rho[i * ny + ny - 1] + rho[i * ny]) : rho[i * ny];
+  if (j >= ny) return a.periodic_y ? 0.5 * (rho[i * ny + ny - 1] + rho[i * ny]) : rho[i * ny + ny - 1];
+  return 0.5 * (rho[i * ny + j - 1] + rho[i * ny + j]);
 }
 
 __device__ __forceinline__ double divergence(const SysArgs &a, int i, int j) {
-  const int lu = a.ld_u, lv = a.ld_v;
-  return (a.u[(i + 1) * lu + j] - a.u[i * lu + j]) / a.dx + (a.v[i * lv + j + 1] - a.v[i * lv + j]) / a.dy;
+  const int ny = a.ny;
+  return (a.u[(i + 1) * ny + j] - a.u[i * ny + j]) / a.dx + (a.v[i * (ny + 1) + j + 1] - a.v[i * (ny + 1) + j]) / a.dy;
 }
 
 // p_ext = p^n (first step) or 2 p^n - p^{n-1}, with periodic wrap (operators.cpp:413-422).
@@ -48,9 +47,10 @@ __device__ __forceinline__ double p_ext(const SysArgs &a, int i, int j) {
   if (i >= a.nx) i -= a.nx;
   if (j < 0) j += a.ny;
   if (j >= a.ny) j -= a.ny;
-  const double pn = a.p_n ? a.p_n[i * a.ld_pn + j] : 0.0;
+  const int k = i * a.ny + j;
+  const double pn = a.p_n ? a.p_n[k] : 0.0;
   if (a.step_index == 0) return pn;
-  const double pm = a.p_nm1 ? a.p_nm1[i * a.ld_pnm1 + j] : 0.0;
+  const double pm = a.p_nm1 ? a.p_nm1[k] : 0.0;
   return 2.0 * pn - pm;
 }
 
@@ -1494,7 +1494,7 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
     const uint32_t c = a.cell[q];
     const int i = static_cast<int>(c >> 16), j = static_cast<int>(c & 0xffffu);
     const double xq = xv[q] - m_x;
-    a.p_out[i * a.ld_out + j] = projected ? xq - mean : xq;
+    a.p_out[i * ny + j] = projected ? xq - mean : xq;
   }
   mark(9);
   if (tid == 0) {
@@ -1776,7 +1776,7 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
     const uint32_t c = a.cell[q];
     const int i = static_cast<int>(c >> 16), j = static_cast<int>(c & 0xffffu);
     const double xq = xv[q] - m_x;
-    a.p_out[i * a.ld_out + j] = projected ? xq - mean : xq;
+    a.p_out[i * ny + j] = projected ? xq - mean : xq;
   }
   clk.mark(9);
   if (tid == 0) {

@@ -238,8 +238,6 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
     auto t = ctx->tuning.find(std::make_pair(pb->nx, pb->ny));
     if (t != ctx->tuning.end()) a.ring = t->second;
   }
-  a.ld_rho = a.ld_u = a.ld_pn = a.ld_pnm1 = a.ld_out = pb->ny;
-  a.ld_v = pb->ny + 1;
   a.lev_base = L.base + pp::kLevPad;
   for (int w = 0; w <= pp::kWaves; ++w) a.spmv_split[w] = L.split[w];
   a.cell = L.cell;
@@ -670,21 +668,45 @@ int pp_solve_device_field(pp_ctx *ctx, const pp_problem *pb, int ghost, const do
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   const int G = ghost, nx = pb->nx, ny = pb->ny;
   const int lc = ny + 2 * G, lv = ny + 1 + 2 * G;
-  auto at00 = [&](const double *f, int ld) { return f ? f + static_cast<size_t>(G) * ld + G : nullptr; };
+  // the solve kernel reads contiguous fields: pack the interiors of the Field2D buffers into the
+  // context's staging area with strided device-to-device copies, and unpack the result after
+  const size_t n = static_cast<size_t>(nx) * ny;
+  const size_t need = 4 * n + static_cast<size_t>(nx + 1) * ny + static_cast<size_t>(nx) * (ny + 1);
+  if (int rc = ensure_stage(ctx, need, report)) return rc;
+  double *d_rho = ctx->stage;
+  double *d_u = d_rho + n;
+  double *d_v = d_u + static_cast<size_t>(nx + 1) * ny;
+  double *d_pn = d_v + static_cast<size_t>(nx) * (ny + 1);
+  double *d_pnm1 = d_pn + n;
+  double *d_out = d_pnm1 + n;
+  auto at00 = [&](const double *f, int ld) { return f + static_cast<size_t>(G) * ld + G; };
+  auto pack = [&](double *dst, const double *field, int rows, int cols, int ld) {
+    return hipMemcpy2DAsync(dst, static_cast<size_t>(cols) * sizeof(double), at00(field, ld),
+                            static_cast<size_t>(ld) * sizeof(double), static_cast<size_t>(cols) * sizeof(double),
+                            static_cast<size_t>(rows), hipMemcpyDeviceToDevice, s);
+  };
+  const bool has_pn = split && d_pressure, has_pnm1 = split && d_pressure_prev;
+  if (d_rho_mid) PP_HIP_CHECK(pack(d_rho, d_rho_mid, nx, ny, lc));
+  PP_HIP_CHECK(pack(d_u, d_u_star, nx + 1, ny, lc));
+  PP_HIP_CHECK(pack(d_v, d_v_star, nx, ny + 1, lv));
+  if (has_pn) PP_HIP_CHECK(pack(d_pn, d_pressure, nx, ny, lc));
+  if (has_pnm1) PP_HIP_CHECK(pack(d_pnm1, d_pressure_prev, nx, ny, lc));
   pp::SysArgs a;
   if (int rc = prepare(ctx, pb, a, report)) return rc;
-  a.rho = at00(d_rho_mid, lc);
-  a.u = at00(d_u_star, lc);
-  a.v = at00(d_v_star, lv);
-  a.p_n = split ? at00(d_pressure, lc) : nullptr;
-  a.p_nm1 = split ? at00(d_pressure_prev, lc) : nullptr;
-  a.p_out = d_p_corr + static_cast<size_t>(G) * lc + G;
-  a.ld_rho = a.ld_u = a.ld_pn = a.ld_pnm1 = a.ld_out = lc;
-  a.ld_v = lv;
+  a.rho = d_rho_mid ? d_rho : nullptr;
+  a.u = d_u;
+  a.v = d_v;
+  a.p_n = has_pn ? d_pn : nullptr;
+  a.p_nm1 = has_pnm1 ? d_pnm1 : nullptr;
+  a.p_out = d_out;
   // the reference zero-fills pressure_correction_ first: the ghost passes read deeper ghost cells
   // of that zero field before rewriting them, and a failed solve leaves zeros
   PP_HIP_CHECK(hipMemsetAsync(d_p_corr, 0, static_cast<size_t>(nx + 2 * G) * lc * sizeof(double), s));
   if (int rc = run_solve(ctx, pb, a, report, s)) return rc;
+  double *out00 = d_p_corr + static_cast<size_t>(G) * lc + G;
+  PP_HIP_CHECK(hipMemcpy2DAsync(out00, static_cast<size_t>(lc) * sizeof(double), d_out,
+                                static_cast<size_t>(ny) * sizeof(double), static_cast<size_t>(ny) * sizeof(double),
+                                static_cast<size_t>(nx), hipMemcpyDeviceToDevice, s));
   if (int rc = ensure_corr(ctx)) return rc;
   const pp::CorrArgs c = corr_args(pb, nullptr, G);
   PP_HIP_CHECK(hipMemcpyAsync(ctx->d_corr_args, &c, sizeof(c), hipMemcpyHostToDevice, s));

@@ -39,9 +39,6 @@ struct SysArgs {
   // fields (natural layout)
   const double *rho, *u, *v, *p_n, *p_nm1;
   double *p_out;
-  // row pitches of the fields in doubles (natural: ny, ny, ny + 1, ny, ny, ny); a ghosted Field2D
-  // buffer is passed as &field(0, 0) with its own pitch
-  int ld_rho, ld_u, ld_v, ld_pn, ld_pnm1, ld_out;
   // coefficient mode (coeff_mode = 1): face inverse densities k_u ((nx+1) x ny), k_v (nx x (ny+1))
   // and a source term src (nx x ny) replace rho, u and v; c_f = k_f / h^2, b = src (+ the boundary
   // terms when rhs_bc = 1)
