@@ -479,6 +479,18 @@ int ensure_corr(pp_ctx *ctx) {
   return PP_OK;
 }
 
+// Device staging of whole ghosted Field2D buffers for the host-pointer correction / step entries.
+int ensure_fields(pp_ctx *ctx, size_t need) {
+  pp_report *report = nullptr;
+  if (need <= ctx->field_cap) return PP_OK;
+  if (ctx->field_stage) (void)hipFree(ctx->field_stage);
+  ctx->field_stage = nullptr;
+  ctx->field_cap = 0;
+  PP_HIP_CHECK(hipMalloc(&ctx->field_stage, need * sizeof(double)));
+  ctx->field_cap = need;
+  return PP_OK;
+}
+
 // Velocity correction (operators.cpp:480-534): resolved boundary conditions and the launch.
 int run_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost,
                             const double *d_rho_mid, const double *d_u_star, const double *d_v_star,
@@ -557,14 +569,7 @@ int pp_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_
   const size_t nc = static_cast<size_t>(nx + 2 * G) * (ny + 2 * G);
   const size_t nu = static_cast<size_t>(nx + 1 + 2 * G) * (ny + 2 * G);
   const size_t nv = static_cast<size_t>(nx + 2 * G) * (ny + 1 + 2 * G);
-  const size_t need = 5 * nc + 2 * nu + 2 * nv;
-  if (need > ctx->field_cap) {
-    if (ctx->field_stage) (void)hipFree(ctx->field_stage);
-    ctx->field_stage = nullptr;
-    ctx->field_cap = 0;
-    PP_HIP_CHECK(hipMalloc(&ctx->field_stage, need * sizeof(double)));
-    ctx->field_cap = need;
-  }
+  if (int rc = ensure_fields(ctx, 5 * nc + 2 * nu + 2 * nv)) return rc;
   double *d_rho = ctx->field_stage, *d_pc = d_rho + nc, *d_p = d_pc + nc, *d_pp = d_p + nc;
   double *d_us = d_pp + nc, *d_u = d_us + nu, *d_vs = d_u + nu, *d_v = d_vs + nv;
   hipStream_t s = ctx->stream;
@@ -590,6 +595,50 @@ int pp_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_
   PP_HIP_CHECK(down(pressure_prev, d_pp, nc));
   PP_HIP_CHECK(hipStreamSynchronize(s));
   return PP_OK;
+}
+
+int pp_pressure_step(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost, const double *rho_mid,
+                     const double *u_star, const double *v_star, double *p_corr, double *u, double *v,
+                     double *pressure, double *pressure_prev, double *speeds, pp_report *report) {
+  if (int rc = validate_correction(ctx, pb, vbc, ghost)) return rc;
+  if (!rho_mid || !u_star || !v_star || !p_corr || !u || !v || !pressure || !pressure_prev)
+    return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  const int nx = pb->nx, ny = pb->ny, G = ghost;
+  const size_t nc = static_cast<size_t>(nx + 2 * G) * (ny + 2 * G);
+  const size_t nu = static_cast<size_t>(nx + 1 + 2 * G) * (ny + 2 * G);
+  const size_t nv = static_cast<size_t>(nx + 2 * G) * (ny + 1 + 2 * G);
+  if (int rc = ensure_fields(ctx, 5 * nc + 2 * nu + 2 * nv)) return rc;
+  double *d_rho = ctx->field_stage, *d_pc = d_rho + nc, *d_p = d_pc + nc, *d_pp = d_p + nc;
+  double *d_us = d_pp + nc, *d_u = d_us + nu, *d_vs = d_u + nu, *d_v = d_vs + nv;
+  hipStream_t s = ctx->stream;
+  auto up = [&](double *dst, const double *src, size_t count) {
+    return hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, s);
+  };
+  auto down = [&](double *dst, const double *src, size_t count) {
+    return hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, s);
+  };
+  PP_HIP_CHECK(up(d_rho, rho_mid, nc));
+  PP_HIP_CHECK(up(d_p, pressure, nc));
+  PP_HIP_CHECK(up(d_pp, pressure_prev, nc));
+  PP_HIP_CHECK(up(d_us, u_star, nu));
+  PP_HIP_CHECK(up(d_u, u, nu));
+  PP_HIP_CHECK(up(d_vs, v_star, nv));
+  PP_HIP_CHECK(up(d_v, v, nv));
+  const int rc = pp_solve_device_field(ctx, pb, ghost, d_rho, d_us, d_vs, d_p, d_pp, d_pc, report, s);
+  if (rc == PP_OK) {
+    if (int rc2 = run_velocity_correction(ctx, pb, vbc, ghost, d_rho, d_us, d_vs, d_pc, d_u, d_v, d_p, d_pp, speeds,
+                                          s))
+      return rc2;
+    PP_HIP_CHECK(down(u, d_u, nu));
+    PP_HIP_CHECK(down(v, d_v, nv));
+    PP_HIP_CHECK(down(pressure, d_p, nc));
+    PP_HIP_CHECK(down(pressure_prev, d_pp, nc));
+  }
+  // pressure_correction_ as the reference leaves it: solved with ghosts, or all zeros after a failure
+  PP_HIP_CHECK(down(p_corr, d_pc, nc));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  return rc;
 }
 
 int pp_create(pp_ctx **out, int device) {

@@ -88,7 +88,7 @@ _lib = None
 EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_solve", "pp_solve_device",
            "pp_solve_coeffs", "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device",
            "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device", "pp_solve_monitored",
-           "pp_solve_device_monitored", "pp_solve_device_field")
+           "pp_solve_device_monitored", "pp_solve_device_field", "pp_pressure_step")
 
 # One PCG monitor record (pp_monitor_record): the scalar fields of one CGMonitor call.
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
@@ -152,6 +152,9 @@ def lib():
         L.pp_solve_device_field.argtypes = [vp, ctypes.POINTER(PPProblem), ctypes.c_int, vp, vp, vp, vp, vp, vp,
                                             ctypes.POINTER(PPReport), vp]
         L.pp_solve_device_field.restype = ctypes.c_int
+        L.pp_pressure_step.argtypes = [vp, ctypes.POINTER(PPProblem), ctypes.POINTER(PPVelocityBC), ctypes.c_int, dp,
+                                       dp, dp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(PPReport)]
+        L.pp_pressure_step.restype = ctypes.c_int
         L.pp_solve_coeffs.argtypes = [vp, ctypes.POINTER(PPProblem), dp, dp, dp, dp, ctypes.POINTER(PPReport)]
         L.pp_solve_coeffs.restype = ctypes.c_int
         L.pp_solve_coeffs_device.argtypes = [vp, ctypes.POINTER(PPProblem), vp, vp, vp, vp, ctypes.POINTER(PPReport), vp]
@@ -430,6 +433,25 @@ class Engine:
         if rc != 0:
             raise EngineError(rc, self.last_error())
         return outs[0], outs[1], outs[2], outs[3], (float(speeds[0]), float(speeds[1]))
+
+    def pressure_step(self, case, vbc, rho_mid, u_star, v_star, u, v, pressure, pressure_prev,
+                      scheme: int = PP_ICPCG, precision: int = PP_FP64, ghost: int = 3, **kw):
+        """The pressure part of a timestep on host Field2D buffers (pp_pressure_step). Returns
+        (p_corr, u, v, pressure, pressure_prev, (speed_pre, speed_post), report); inputs unchanged."""
+        pb = make_problem(case, scheme, precision, **kw)
+        f64 = lambda a: np.array(a, dtype=np.float64, copy=True).reshape(-1)  # noqa: E731
+        rho_mid, u_star, v_star = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                                   for a in (rho_mid, u_star, v_star))
+        u, v, pressure, prev = f64(u), f64(v), f64(pressure), f64(pressure_prev)
+        p_corr = np.zeros_like(pressure)
+        speeds = np.zeros(2)
+        rep = PPReport()
+        rc = self._lib.pp_pressure_step(self.h, ctypes.byref(pb), ctypes.byref(make_velocity_bc(vbc)), int(ghost),
+                                        _dp(rho_mid), _dp(u_star), _dp(v_star), _dp(p_corr), _dp(u), _dp(v),
+                                        _dp(pressure), _dp(prev), _dp(speeds), ctypes.byref(rep))
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return p_corr, u, v, pressure, prev, (float(speeds[0]), float(speeds[1])), _report_dict(rep)
 
     def velocity_correction_device(self, case, vbc, d_rho_mid, d_u_star, d_v_star, d_p_corr, d_u, d_v, d_pressure,
                                    d_pressure_prev, scheme: int = PP_ICPCG, ghost: int = 3, stream=None):

@@ -50,6 +50,33 @@ pp_problem make_problem(const PressureSettings &s, int code, int nx, int ny, int
   return pb;
 }
 
+pp_velocity_bc velocity_bc(const VelocitySettings &vel) {
+  pp_velocity_bc vb;
+  std::memset(&vb, 0, sizeof(vb));
+  const BoundarySpec *ue[4] = {&vel.u_left, &vel.u_right, &vel.u_bottom, &vel.u_top};
+  const BoundarySpec *ve[4] = {&vel.v_left, &vel.v_right, &vel.v_bottom, &vel.v_top};
+  for (int k = 0; k < 4; ++k) {
+    vb.u_type[k] = ue[k]->type;
+    vb.u_value[k] = ue[k]->value;
+    vb.v_type[k] = ve[k]->type;
+    vb.v_value[k] = ve[k]->value;
+  }
+  vb.bottom_wall_velocity_x = vel.bottom_wall_velocity_x;
+  vb.top_wall_velocity_x = vel.top_wall_velocity_x;
+  return vb;
+}
+
+// All fields of a pressure step share the pressure grid's ghost count and the MAC shapes.
+bool same_grid(const FieldView &pressure, const FieldView &rho_mid, const FieldView &u_star, const FieldView &v_star,
+               const FieldView &p_corr, const FieldView &u, const FieldView &v, const FieldView &pressure_previous) {
+  const int nx = pressure.nx, ny = pressure.ny, g = pressure.ghost;
+  auto same = [&](const FieldView &f, int rows, int cols) {
+    return f.data && f.nx == rows && f.ny == cols && f.ghost == g;
+  };
+  return same(pressure, nx, ny) && same(rho_mid, nx, ny) && same(p_corr, nx, ny) && same(pressure_previous, nx, ny) &&
+         same(u_star, nx + 1, ny) && same(u, nx + 1, ny) && same(v_star, nx, ny + 1) && same(v, nx, ny + 1);
+}
+
 }  // namespace
 
 int hip_scheme_code(const std::string &scheme) {
@@ -167,29 +194,34 @@ void PressureBackend::correct_velocity(const std::string &scheme, const Pressure
   const int code = hip_scheme_code(scheme);
   if (code == 0) throw std::runtime_error("unsupported pressure scheme: " + scheme);
   const int nx = pressure.nx, ny = pressure.ny, g = pressure.ghost;
-  auto same = [&](const FieldView &f, int rows, int cols) {
-    return f.data && f.nx == rows && f.ny == cols && f.ghost == g;
-  };
-  if (!same(rho_mid, nx, ny) || !same(pressure_correction, nx, ny) || !same(pressure_previous, nx, ny) ||
-      !same(u_star, nx + 1, ny) || !same(u, nx + 1, ny) || !same(v_star, nx, ny + 1) || !same(v, nx, ny + 1))
+  if (!same_grid(pressure, rho_mid, u_star, v_star, pressure_correction, u, v, pressure_previous))
     throw std::runtime_error("velocity correction: field shapes or ghost counts do not match the pressure grid");
   const pp_problem pb = make_problem(s, code, nx, ny, step_index);
-  pp_velocity_bc vb;
-  std::memset(&vb, 0, sizeof(vb));
-  const BoundarySpec *ue[4] = {&vel.u_left, &vel.u_right, &vel.u_bottom, &vel.u_top};
-  const BoundarySpec *ve[4] = {&vel.v_left, &vel.v_right, &vel.v_bottom, &vel.v_top};
-  for (int k = 0; k < 4; ++k) {
-    vb.u_type[k] = ue[k]->type;
-    vb.u_value[k] = ue[k]->value;
-    vb.v_type[k] = ve[k]->type;
-    vb.v_value[k] = ve[k]->value;
-  }
-  vb.bottom_wall_velocity_x = vel.bottom_wall_velocity_x;
-  vb.top_wall_velocity_x = vel.top_wall_velocity_x;
+  const pp_velocity_bc vb = velocity_bc(vel);
   const int rc = pp_velocity_correction(ctx_, &pb, &vb, g, rho_mid.data, u_star.data, v_star.data,
                                         pressure_correction.data, u.data, v.data, pressure.data,
                                         pressure_previous.data, speeds_);
   if (rc != PP_OK) throw std::runtime_error(pp_last_error(ctx_));
+}
+
+double PressureBackend::pressure_step(const std::string &scheme, const PressureSettings &s,
+                                     const VelocitySettings &vel, const FieldView &rho_mid, const FieldView &u_star,
+                                     const FieldView &v_star, int step_index, const FieldView &pressure_correction,
+                                     const FieldView &u, const FieldView &v, const FieldView &pressure,
+                                     const FieldView &pressure_previous) {
+  const int code = hip_scheme_code(scheme);
+  if (code == 0) throw std::runtime_error("unsupported pressure scheme: " + scheme);
+  if (!same_grid(pressure, rho_mid, u_star, v_star, pressure_correction, u, v, pressure_previous))
+    throw std::runtime_error("pressure step: field shapes or ghost counts do not match the pressure grid");
+  const pp_problem pb = make_problem(s, code, pressure.nx, pressure.ny, step_index);
+  const pp_velocity_bc vb = velocity_bc(vel);
+  last_iterations_ = 0;
+  const int rc = pp_pressure_step(ctx_, &pb, &vb, pressure.ghost, rho_mid.data, u_star.data, v_star.data,
+                                  pressure_correction.data, u.data, v.data, pressure.data, pressure_previous.data,
+                                  speeds_, &report_);
+  if (rc != PP_OK) throw std::runtime_error(report_.message[0] ? report_.message : pp_last_error(ctx_));
+  last_iterations_ = report_.iterations;
+  return report_.relative_residual;
 }
 
 }  // namespace ppamd

@@ -96,6 +96,16 @@ class PressureBackend {
                         const FieldView &pressure_correction, int step_index, const FieldView &u,
                         const FieldView &v, const FieldView &pressure, const FieldView &pressure_previous);
 
+  // Both calls of solver.cpp:271-272 in one go (pp_pressure_step): pressure_correction is
+  // written (interior and ghosts), then u, v, pressure and pressure_previous are corrected as by
+  // correct_velocity; the fields cross to the device and back once. Returns the solve's relative
+  // residual; a solve failure throws with the reference's message and leaves pressure_correction
+  // zero and the other fields untouched.
+  double pressure_step(const std::string &scheme, const PressureSettings &s, const VelocitySettings &vel,
+                       const FieldView &rho_mid, const FieldView &u_star, const FieldView &v_star, int step_index,
+                       const FieldView &pressure_correction, const FieldView &u, const FieldView &v,
+                       const FieldView &pressure, const FieldView &pressure_previous);
+
   int last_iterations() const { return last_iterations_; }
   const pp_report &last_report() const { return report_; }
   // last_boundary_speed_pre_correction_ / _post_correction_ of the last correct_velocity

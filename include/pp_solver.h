@@ -266,6 +266,19 @@ int pp_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_
                            double *u, double *v, double *pressure, double *pressure_prev, double *speeds);
 
 /*
+ * The pressure part of one timestep from host Field2D buffers, the two calls solver.cpp:271-272
+ * makes (solve_pressure_correction, then apply_pressure_velocity_correction) with one upload and
+ * one download: the fields go to the device once, pp_solve_device_field and the device correction
+ * run back to back, and p_corr (pressure_correction_ with ghosts), u, v, pressure and
+ * pressure_prev come back. Same buffers and meanings as pp_velocity_correction, with p_corr an
+ * output. On a solve failure the status and report carry the reference's error, p_corr is all
+ * zeros and u, v, pressure, pressure_prev are untouched (the reference throws before correcting).
+ */
+int pp_pressure_step(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost, const double *rho_mid,
+                     const double *u_star, const double *v_star, double *p_corr, double *u, double *v,
+                     double *pressure, double *pressure_prev, double *speeds, pp_report *report);
+
+/*
  * Tuning table of the context (written by the project's autotuner, ding_nsch_core_solver_amd/
  * autotune.py): the sweep operand ring depth to use for grids of exactly nx[k] x ny[k] when
  * pp_problem.ring_depth is 0. Replaces the previous table; entries with an invalid depth are

@@ -9,6 +9,9 @@
 //   driver monitor <cfg> <scheme> <in.bin> <out.bin>
 //        as solve, with the PCG monitor: one line per record "k alpha beta abs rel iterate precond"
 //        after the "iterations residual" line
+//   driver step <cfg> <vel> <scheme> <in.bin> <out.bin>
+//        as correct, through PressureBackend::pressure_step (p_corr in in.bin is ignored); out.bin:
+//        p_corr, u, v, pressure, pressure_prev; prints "iterations residual speed_pre speed_post"
 //   driver correct <cfg> <vel> <scheme> <in.bin> <out.bin>
 //        in.bin: whole ghosted buffers (3 layers) rho_mid, u_star, v_star, p_corr, u, v, pressure,
 //        pressure_prev; out.bin: u, v, pressure, pressure_prev; prints "speed_pre speed_post"
@@ -140,7 +143,7 @@ int main(int argc, char **argv) {
     }
     return 0;
   }
-  if (mode == "correct" && argc == 7) {
+  if ((mode == "correct" || mode == "step") && argc == 7) {
     const Cfg c = parse(argv[2]);
     const ppamd::VelocitySettings vel = parse_vel(argv[3]);
     const int nx = c.nx, ny = c.ny, g = 3;
@@ -160,8 +163,15 @@ int main(int argc, char **argv) {
     const FieldView p{take(cell), nx, ny, g}, pm{take(cell), nx, ny, g};
     try {
       ppamd::PressureBackend backend(0);
-      backend.correct_velocity(argv[4], c.s, vel, rho, us, vs, pc, c.step, u, v, p, pm);
-      std::vector<double> out(u.data, u.data + uf);
+      std::vector<double> out;
+      if (mode == "step") {
+        const double res = backend.pressure_step(argv[4], c.s, vel, rho, us, vs, c.step, pc, u, v, p, pm);
+        std::printf("%d %.17g ", backend.last_iterations(), res);
+        out.insert(out.end(), pc.data, pc.data + cell);
+      } else {
+        backend.correct_velocity(argv[4], c.s, vel, rho, us, vs, pc, c.step, u, v, p, pm);
+      }
+      out.insert(out.end(), u.data, u.data + uf);
       out.insert(out.end(), v.data, v.data + vf);
       out.insert(out.end(), p.data, p.data + cell);
       out.insert(out.end(), pm.data, pm.data + cell);

@@ -101,3 +101,49 @@ def test_four_pressure_steps(engine, oracle_lib, ref_lib, scheme, kw):
             scale = max(np.abs(ref_state[key]).max(), 1e-30)
             assert np.abs(got[key] - ref_state[key]).max() <= 1e-7 * scale, (k, key)
         print(f"[timestep] {scheme} step {k}: {rep['iterations']} iterations, bitwise vs oracle", flush=True)
+
+
+@pytest.mark.parametrize("scheme", ["icpcg", "split_ildlt_pcg"])
+def test_pressure_step_entry_equals_separate_calls(engine, oracle_lib, scheme):
+    """pp_pressure_step (host buffers, one upload and download) against the oracle's chain on the
+    same inputs, bit for bit: pressure_correction with ghosts, u, v, pressure, pressure_prev, speeds."""
+    nx, ny = 40, 36
+    case = W.droplet_case(nx, ny, seed=12, density_ratio=200.0, step_index=1, periodic_x=True)
+    rng = np.random.default_rng(8)
+    state = dict(rho_mid=np.pad(case.rho.reshape(nx, ny), G, mode="edge").reshape(-1),
+                 u=rng.standard_normal(W.ghosted_shape(nx + 1, ny, G)).reshape(-1),
+                 v=rng.standard_normal(W.ghosted_shape(nx, ny + 1, G)).reshape(-1),
+                 pressure=rng.standard_normal(W.ghosted_shape(nx, ny, G)).reshape(-1) * 10.0,
+                 pressure_prev=rng.standard_normal(W.ghosted_shape(nx, ny, G)).reshape(-1) * 10.0)
+    u_star = state["u"] + rng.standard_normal(state["u"].shape)
+    v_star = state["v"] + rng.standard_normal(state["v"].shape)
+    vbc = W.VelocityBC(bottom_wall_velocity_x=-0.25)
+    code = {"icpcg": E.PP_ICPCG, "split_ildlt_pcg": E.PP_SPLIT_ILDLT_PCG}[scheme]
+    p_corr, u, v, p, prev, speeds, rep = engine.pressure_step(case, vbc, state["rho_mid"], u_star, v_star, state["u"],
+                                                              state["v"], state["pressure"], state["pressure_prev"],
+                                                              scheme=code, ghost=G)
+    expect, it_o, sp_o = _oracle_step(oracle_lib, scheme, case, vbc, state, u_star, v_star, oracle_lib.CANONICAL)
+    assert rep["iterations"] == it_o and speeds == tuple(sp_o)
+    for key, a in (("u", u), ("v", v), ("pressure", p), ("pressure_prev", prev)):
+        assert np.array_equal(a, expect[key]), key
+    step = W.PressureCase(**{k: getattr(case, k) for k in ("nx", "ny", "lx", "ly", "dt", "periodic_x", "periodic_y",
+                                                          "bc_type", "bc_value", "density_ratio", "step_index")})
+    pc = np.zeros(W.ghosted_shape(nx, ny, G))
+    step.rho = _interior(state["rho_mid"], nx, ny)
+    step.u, step.v = _interior(u_star, nx + 1, ny), _interior(v_star, nx, ny + 1)
+    step.p_n, step.p_nm1 = _interior(state["pressure"], nx, ny), _interior(state["pressure_prev"], nx, ny)
+    pc[G:-G, G:-G] = oracle_lib.pressure_solve(scheme, step, oracle_lib.CANONICAL)[0].reshape(nx, ny)
+    assert np.array_equal(p_corr, oracle_lib.pressure_ghosts(step, pc.reshape(-1), G))
+
+
+def test_pressure_step_failure_leaves_fields(engine):
+    nx, ny = 8, 8
+    case = W.droplet_case(nx, ny, seed=0, density_ratio=10.0)
+    rho = case.rho.reshape(nx, ny).copy()
+    rho[:3, :3] = -5.0
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(W.ghosted_shape(nx + 1, ny, G)).reshape(-1)
+    v = rng.standard_normal(W.ghosted_shape(nx, ny + 1, G)).reshape(-1)
+    p = rng.standard_normal(W.ghosted_shape(nx, ny, G)).reshape(-1)
+    with pytest.raises(E.EngineError, match="ICC preconditioner construction failed"):
+        engine.pressure_step(case, W.VelocityBC(), np.pad(rho, G, mode="edge").reshape(-1), u, v, u, v, p, p)

@@ -165,3 +165,30 @@ def test_monitor_through_cpp_mirror(driver, engine, tmp_path, scheme, code):
         assert int(row[0]) == r["iteration"]
         assert [float(x) for x in row[1:]] == [r["alpha"], r["beta"], r["absolute_residual"], r["relative_residual"],
                                                r["iterate_residual"], r["preconditioned_residual"]]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme,code,kw,step", [
+    ("hip_icpcg", E.PP_ICPCG, dict(bc_type=(0, 1, 0, 0), bc_value=(0, 0.5, 0, 0)), 0),
+    ("hip_liu_split_icpcg", E.PP_SPLIT_ICPCG, dict(periodic_y=True), 2),
+])
+def test_pressure_step_through_cpp_mirror(driver, engine, tmp_path, scheme, code, kw, step):
+    """PressureBackend::pressure_step (solve + correction, one upload and download) equals the
+    engine's pp_pressure_step bit for bit: pressure_correction with ghosts, u, v, both pressures."""
+    from tests.test_velocity_oracle import G, VELOCITY_BC, fields
+    case = W.droplet_case(30, 26, seed=6, density_ratio=120.0, step_index=step, **kw)
+    f = fields(case, seed=51 + step)
+    f["rho_mid"] = np.pad(case.rho.reshape(case.nx, case.ny), G, mode="edge").reshape(-1)
+    vbc = VELOCITY_BC["lid_driven"]
+    order = ("rho_mid", "u_star", "v_star", "p_corr", "u", "v", "pressure", "pressure_prev")
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([f[k] for k in order]).tofile(inp)
+    res = subprocess.run([driver, "step", _cfg(case), _vel_cfg(vbc), scheme, str(inp), str(outp)], check=True,
+                         capture_output=True, text=True)
+    it, rel, pre, post = res.stdout.split()
+    p_corr, u, v, p, prev, speeds, rep = engine.pressure_step(case, vbc, f["rho_mid"], f["u_star"], f["v_star"],
+                                                              f["u"], f["v"], f["pressure"], f["pressure_prev"],
+                                                              scheme=code, ghost=G)
+    assert int(it) == rep["iterations"] and float(rel) == rep["relative_residual"]
+    assert (float(pre), float(post)) == speeds
+    assert np.array_equal(np.fromfile(outp), np.concatenate([p_corr, u, v, p, prev]))
