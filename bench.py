@@ -9,7 +9,9 @@ Inputs are resident in HBM when the timed region starts; every step runs assembl
 PCG and the output mean subtraction.
 
 Prints one JSON line with solves/s, the roofline of the solve kernel (algorithmic bytes from
-HIP-event kernel time) and the reference CPU path timed on this host's cores.
+HIP-event kernel time) and the reference CPU path timed on this host's cores. Beside it: the same
+workload in mixed precision, the velocity correction that follows the solve (and the whole device
+pressure step), and the 256-system uneven batch, each with its own roofline and CPU baseline.
 """
 from __future__ import annotations
 
