@@ -25,6 +25,7 @@ namespace {
 
 constexpr int kSumChunk = 1024;  // terms per LDS stage of the sequential mean (two stages, 16 KB)
 constexpr int kSumGroup = 16;   // terms the summing lane loads per step (kSumChunk: a multiple of 2x)
+constexpr int kSumStage = kSumChunk + kSumGroup;  // one stage plus the group the last step preloads
 constexpr int kWaveSize = 64;
 constexpr int kFaceBlock = 256;  // threads per block of the face and history kernels
 constexpr int kMeanSlot = 2;     // work[0], work[1]: boundary speeds; work[2]: the pressure mean
@@ -242,7 +243,7 @@ __device__ double p_ext_at(const CorrArgs &a, const GhostedConst &p, const Ghost
 // +0.0, which leaves the running sum unchanged bit for bit (it starts at +0.0 and round-to-nearest
 // never produces -0.0 from a sum of a nonzero and its negation). Runs in a kFaceBlock-thread
 // block; returns the sum in thread 0.
-__device__ double sequential_sum(const GhostedConst &f, int nx, int ny, double (*stage)[kSumChunk]) {
+__device__ double sequential_sum(const GhostedConst &f, int nx, int ny, double (*stage)[kSumStage]) {
   const int tid = threadIdx.x, n = nx * ny;
   const int chunks = (n + kSumChunk - 1) / kSumChunk;
   constexpr int kStagers = kFaceBlock - kWaveSize;
@@ -292,9 +293,9 @@ __device__ double sequential_sum(const GhostedConst &f, int nx, int ny, double (
           acc += A[r].y;
         }
         __builtin_amdgcn_sched_barrier(0);
-        const int next = p + 1 < pairs ? p + 1 : p;
+        // the last pair's "next" group is the stage's padding: read, never added
 #pragma unroll
-        for (int r = 0; r < kVec; ++r) A[r] = src[2 * kVec * next + r];
+        for (int r = 0; r < kVec; ++r) A[r] = src[2 * kVec * (p + 1) + r];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int r = 0; r < kVec; ++r) {
@@ -316,7 +317,7 @@ __device__ double sequential_sum(const GhostedConst &f, int nx, int ny, double (
 // (operators.cpp:490-519), every face of the (nx+1) x ny and nx x (ny+1) fields including the
 // boundary faces, which the finish kernel then overwrites as the reference does.
 __global__ void __launch_bounds__(kFaceBlock) correction_faces_kernel(const CorrArgs *__restrict__ args) {
-  __shared__ __align__(16) double stage[2][kSumChunk];
+  __shared__ __align__(16) double stage[2][kSumStage];
   const CorrArgs &a = *args;
   const int nx = a.nx, ny = a.ny, G = a.G;
   const GhostedConst rho{a.rho_mid, ny, G}, p_corr{a.p_corr, ny, G};
