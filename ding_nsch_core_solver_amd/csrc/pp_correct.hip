@@ -24,7 +24,7 @@ namespace pp {
 namespace {
 
 constexpr int kSumChunk = 1024;  // terms per LDS stage of the sequential mean (two stages, 16 KB)
-constexpr int kSumGroup = 16;   // terms the summing lane loads per step (kSumChunk: a multiple of 2x)
+constexpr int kSumGroup = 32;   // terms the summing lane loads per step (kSumChunk: a multiple of 2x)
 constexpr int kSumStage = kSumChunk + kSumGroup;  // one stage plus the group the last step preloads
 constexpr int kWaveSize = 64;
 constexpr int kFaceBlock = 256;  // threads per block of the face and history kernels
