@@ -419,11 +419,26 @@ class Engine:
             raise err
         return out
 
+    @staticmethod
+    def _check_field_sizes(case, ghost, **fields):
+        """Host Field2D buffers must have exactly their ghosted sizes: the library copies whole
+        buffers of the sizes the grid implies."""
+        nx, ny, g = case.nx, case.ny, int(ghost)
+        shapes = {"u_star": (nx + 1, ny), "u": (nx + 1, ny), "v_star": (nx, ny + 1), "v": (nx, ny + 1)}
+        for name, a in fields.items():
+            rows, cols = shapes.get(name, (nx, ny))
+            want = (rows + 2 * g) * (cols + 2 * g)
+            if np.asarray(a).size != want:
+                raise ValueError(f"{name}: {np.asarray(a).size} values, expected {want} "
+                                 f"({rows} x {cols} with {g} ghost layers)")
+
     def velocity_correction(self, case, vbc, rho_mid, u_star, v_star, p_corr, u, v, pressure, pressure_prev,
                             scheme: int = PP_ICPCG, ghost: int = 3):
         """Velocity correction and pressure-history shift on host Field2D buffers (ghost layers
         included). Returns new (u, v, pressure, pressure_prev, (speed_pre, speed_post)); the inputs
         are not modified."""
+        self._check_field_sizes(case, ghost, rho_mid=rho_mid, u_star=u_star, v_star=v_star, p_corr=p_corr, u=u, v=v,
+                                pressure=pressure, pressure_prev=pressure_prev)
         pb = make_problem(case, scheme, PP_FP64)
         ins = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (rho_mid, u_star, v_star, p_corr)]
         outs = [np.array(a, dtype=np.float64, copy=True).reshape(-1) for a in (u, v, pressure, pressure_prev)]
@@ -438,6 +453,8 @@ class Engine:
                       scheme: int = PP_ICPCG, precision: int = PP_FP64, ghost: int = 3, **kw):
         """The pressure part of a timestep on host Field2D buffers (pp_pressure_step). Returns
         (p_corr, u, v, pressure, pressure_prev, (speed_pre, speed_post), report); inputs unchanged."""
+        self._check_field_sizes(case, ghost, rho_mid=rho_mid, u_star=u_star, v_star=v_star, u=u, v=v,
+                                pressure=pressure, pressure_prev=pressure_prev)
         pb = make_problem(case, scheme, precision, **kw)
         f64 = lambda a: np.array(a, dtype=np.float64, copy=True).reshape(-1)  # noqa: E731
         rho_mid, u_star, v_star = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1)

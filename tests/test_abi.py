@@ -95,3 +95,18 @@ def test_monitor_record_layout_matches_header():
     assert engine.MONITOR_DTYPE == cpu.MONITOR_DTYPE
     assert engine.MONITOR_DTYPE.itemsize == 2 * 4 + 6 * 8
     assert engine.MONITOR_DTYPE.fields["alpha"][1] == 8
+
+
+def test_field_size_check_rejects_wrong_buffers():
+    """The host Field2D wrappers refuse buffers whose size does not match the grid and ghost count
+    (the library would copy past their end)."""
+    import numpy as np
+    from ding_nsch_core_solver_amd import engine, workloads as W
+    case = W.droplet_case(6, 5, seed=0)
+    g = 2
+    ok = dict(rho_mid=np.zeros(10 * 9), u_star=np.zeros(11 * 9), v_star=np.zeros(10 * 10), p_corr=np.zeros(10 * 9),
+              u=np.zeros(11 * 9), v=np.zeros(10 * 10), pressure=np.zeros(10 * 9), pressure_prev=np.zeros(10 * 9))
+    engine.Engine._check_field_sizes(case, g, **ok)
+    bad = dict(ok, v=np.zeros(11 * 9))
+    with pytest.raises(ValueError, match="v: 99 values, expected 100"):
+        engine.Engine._check_field_sizes(case, g, **bad)
