@@ -424,6 +424,8 @@ class Engine:
         """Host Field2D buffers must have exactly their ghosted sizes: the library copies whole
         buffers of the sizes the grid implies."""
         nx, ny, g = case.nx, case.ny, int(ghost)
+        if not 1 <= g <= 8:
+            return  # the library rejects the ghost count itself
         shapes = {"u_star": (nx + 1, ny), "u": (nx + 1, ny), "v_star": (nx, ny + 1), "v": (nx, ny + 1)}
         for name, a in fields.items():
             rows, cols = shapes.get(name, (nx, ny))
