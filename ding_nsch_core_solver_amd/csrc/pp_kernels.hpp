@@ -119,4 +119,21 @@ hipError_t launch_velocity_correction(const CorrArgs *d_args, int nx, int ny, in
 // and periodicity of the argument block; one block on `stream`.
 hipError_t launch_pressure_ghosts(const CorrArgs *d_args, double *d_field, hipStream_t stream);
 
+// BiCGSTAB with the diagonal preconditioner on a CSR system (the momentum predictor's solve).
+// Workgroup c of `chunks` owns rows [kBicgRows c, kBicgRows (c + 1)) and leaves its dot-product
+// partials at partials[k * chunks + c].
+constexpr int kBicgRows = 512;
+struct BicgArgs {
+  int n, chunks;
+  const int *row_ptr, *col_idx;
+  const double *values, *rhs;
+  double *x;
+  double *r, *r_hat, *p, *v, *p_hat, *s, *s_hat, *t, *update, *dinv;  // n each
+  double *partials;  // 4 * chunks
+  int *flag;         // set when a row has no diagonal entry
+};
+enum class BicgStep { kSetup, kDirection, kSpmvV, kHalfStep, kHalfExit, kSpmvT, kFullStep };
+// One pass; s0, s1: beta and omega (kDirection), alpha (kHalfStep), omega (kFullStep).
+hipError_t launch_bicg(BicgStep step, const BicgArgs &a, double s0, double s1, hipStream_t stream);
+
 }  // namespace pp

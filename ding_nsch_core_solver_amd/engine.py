@@ -15,6 +15,10 @@ from .build import LIB_PATH
 
 PP_ICPCG, PP_ILDLT_PCG, PP_SPLIT_ICPCG, PP_SPLIT_ILDLT_PCG = 1, 2, 3, 4
 PP_FP64, PP_MIXED = 0, 1
+# pp_status (include/pp_solver.h)
+PP_OK, PP_ERR_FACTOR, PP_ERR_PAP, PP_ERR_RZ0, PP_ERR_RZ, PP_ERR_NODIAG, PP_ERR_SWEEP = 0, 1, 2, 3, 4, 5, 6
+PP_ERR_BICG_RHO, PP_ERR_BICG_RHAT_V, PP_ERR_BICG_TT, PP_ERR_BICG_OMEGA = 7, 8, 9, 10
+PP_ERR_ARG, PP_ERR_HIP = 100, 101
 
 # New scheme strings (the reference's names prefixed with hip_) and their codes. The split
 # aliases mirror the reference dispatch (pressure.cpp:1048-1058).
@@ -88,7 +92,8 @@ _lib = None
 EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_solve", "pp_solve_device",
            "pp_solve_coeffs", "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device",
            "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device", "pp_solve_monitored",
-           "pp_solve_device_monitored", "pp_solve_device_field", "pp_pressure_step")
+           "pp_solve_device_monitored", "pp_solve_device_field", "pp_pressure_step", "pp_bicgstab",
+           "pp_bicgstab_device")
 
 # One PCG monitor record (pp_monitor_record): the scalar fields of one CGMonitor call.
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
@@ -178,6 +183,12 @@ def lib():
         L.pp_velocity_correction.argtypes = [vp, ctypes.POINTER(PPProblem), vb, ctypes.c_int, dp, dp, dp, dp, dp, dp,
                                              dp, dp, dp]
         L.pp_velocity_correction.restype = ctypes.c_int
+        L.pp_bicgstab.argtypes = [vp, ctypes.c_int, ip, ip, dp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  ctypes.POINTER(PPReport)]
+        L.pp_bicgstab.restype = ctypes.c_int
+        L.pp_bicgstab_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.POINTER(PPReport), vp]
+        L.pp_bicgstab_device.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -483,6 +494,39 @@ class Engine:
         if rc != 0:
             raise EngineError(rc, self.last_error())
         return float(speeds[0]), float(speeds[1])
+
+    def bicgstab(self, system, x0=None, max_iterations: int = 50, tolerance: float = 1.0e-8,
+                 use_relative: bool = False):
+        """The momentum predictor's solve (pp_bicgstab) on a CSR system with row_ptr, col_idx (int32),
+        values and rhs (float64). Returns (x, report); defaults are the reference's
+        momentum_iterations, momentum_tolerance and absolute test (momentum.cpp:469-472)."""
+        rp = np.ascontiguousarray(system.row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(system.col_idx, dtype=np.int32)
+        val = np.ascontiguousarray(system.values, dtype=np.float64)
+        rhs = np.ascontiguousarray(system.rhs, dtype=np.float64)
+        n = len(rp) - 1
+        x = np.zeros(max(n, 0)) if x0 is None else np.array(x0, dtype=np.float64).reshape(-1)
+        if n > 0 and (len(x) != n or len(rhs) != n or len(ci) < rp[n] or len(val) < rp[n]):
+            raise ValueError("CSR, right-hand side and x sizes do not match")
+        ipp = ctypes.POINTER(ctypes.c_int)
+        rep = PPReport()
+        rc = self._lib.pp_bicgstab(self.h, n, rp.ctypes.data_as(ipp), ci.ctypes.data_as(ipp), _dp(val), _dp(rhs),
+                                   _dp(x) if n > 0 else None, int(max_iterations), float(tolerance),
+                                   int(bool(use_relative)), ctypes.byref(rep))
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return x, _report_dict(rep)
+
+    def bicgstab_device(self, n, d_row_ptr, d_col_idx, d_values, d_rhs, d_x, max_iterations: int = 50,
+                        tolerance: float = 1.0e-8, use_relative: bool = False, stream=None) -> dict:
+        """Device-pointer form (pp_bicgstab_device); d_x is solved in place. Returns the report."""
+        rep = PPReport()
+        rc = self._lib.pp_bicgstab_device(self.h, int(n), d_row_ptr, d_col_idx, d_values, d_rhs, d_x,
+                                          int(max_iterations), float(tolerance), int(bool(use_relative)),
+                                          ctypes.byref(rep), stream)
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return _report_dict(rep)
 
     def last_error(self) -> str:
         return self._lib.pp_last_error(self.h).decode(errors="replace")

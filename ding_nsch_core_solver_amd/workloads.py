@@ -168,3 +168,107 @@ class VelocityBC:
 def ghosted_shape(rows: int, cols: int, ghost: int = 3) -> tuple:
     """Shape of a Field2D buffer (rows x cols interior, `ghost` layers on every side)."""
     return (rows + 2 * ghost, cols + 2 * ghost)
+
+
+@dataclasses.dataclass
+class CsrSystem:
+    """A sparse system in the form finalize_row_maps produces (ch_sparse_krylov.hpp:105-133):
+    int32 row_ptr (n + 1) and col_idx with ascending columns per row, one diagonal entry per row,
+    no entry below 1e-18 in magnitude; float64 values and right-hand side."""
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+    values: np.ndarray
+    rhs: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return len(self.row_ptr) - 1
+
+
+def csr_from_coo(n, rows, cols, vals, rhs) -> CsrSystem:
+    """Sums duplicate (row, col) entries, drops |value| < 1e-18 as finalize_row_maps does."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    vals = np.asarray(vals, dtype=np.float64)
+    key = rows * n + cols
+    order = np.argsort(key, kind="stable")
+    key, vals = key[order], vals[order]
+    uniq, start = np.unique(key, return_index=True)
+    summed = np.add.reduceat(vals, start) if len(vals) else np.zeros(0)
+    keep = np.abs(summed) >= 1.0e-18
+    uniq, summed = uniq[keep], summed[keep]
+    r, c = uniq // n, uniq % n
+    row_ptr = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(row_ptr, r + 1, 1)
+    return CsrSystem(np.cumsum(row_ptr).astype(np.int32), c.astype(np.int32), summed,
+                     np.ascontiguousarray(rhs, dtype=np.float64))
+
+
+def momentum_system(nx=192, ny=192, seed=0, density_ratio=1000.0, viscosity_ratio=100.0, dt=1.0e-3, re=100.0,
+                    lx=1.0, ly=1.0) -> CsrSystem:
+    """A synthetic system of the momentum predictor's shape (momentum.cpp:157-472): the u unknowns
+    of the interior x-faces, then the v unknowns of the interior y-faces (walls on every side,
+    numbered face by face in row-major order), each row rho_face u + alpha (full viscous tensor) with
+    a droplet's density and viscosity: 5 entries on its own component and 4 cross entries on the
+    other, alpha = dt / (2 Re). Right-hand side: rho_face times a seeded velocity plus forcing.
+    Not the reference's assembly, the same structure and size (2 nx ny - nx - ny unknowns)."""
+    rng = np.random.default_rng(seed)
+    dx, dy = lx / nx, ly / ny
+    alpha = 0.5 * dt / re
+    rho = droplet_density(nx, ny, lx, ly, rho_in=density_ratio, rho_out=1.0)
+    phase = (rho - rho.min()) / max(rho.max() - rho.min(), 1e-30)
+    eta_c = 1.0 + (viscosity_ratio - 1.0) * phase  # cell centres
+    eta_pad = np.pad(eta_c, 1, mode="edge")
+    eta_n = 0.25 * (eta_pad[:-1, :-1] + eta_pad[1:, :-1] + eta_pad[:-1, 1:] + eta_pad[1:, 1:])  # corners (nx+1, ny+1)
+    u_id = -np.ones((nx + 1, ny), dtype=np.int64)
+    u_id[1:nx, :] = np.arange((nx - 1) * ny).reshape(nx - 1, ny)
+    nu = (nx - 1) * ny
+    v_id = -np.ones((nx, ny + 1), dtype=np.int64)
+    v_id[:, 1:ny] = nu + np.arange(nx * (ny - 1)).reshape(nx, ny - 1)
+    n = nu + nx * (ny - 1)
+    rows, cols, vals = [], [], []
+    rhs = np.zeros(n)
+
+    def add(r, c, v):
+        m = (r >= 0) & (c >= 0)
+        rows.append(r[m]); cols.append(c[m]); vals.append(v[m])
+
+    cxy = 1.0 / (dx * dy)
+    # u rows at faces (i, j), i = 1..nx-1
+    I, J = np.meshgrid(np.arange(1, nx), np.arange(ny), indexing="ij")
+    r = u_id[I, J]
+    rho_f = 0.5 * (rho[I - 1, J] + rho[I, J])
+    ce, cw = 2.0 * eta_c[I, J] / dx**2, 2.0 * eta_c[I - 1, J] / dx**2
+    cn, cs = eta_n[I, J + 1] / dy**2, eta_n[I, J] / dy**2
+    wall = np.where(J == ny - 1, cn, 0.0) + np.where(J == 0, cs, 0.0)  # no-slip walls: reflected ghost
+    add(r, u_id[I - 1, J], -alpha * cw)
+    add(r, np.where(J > 0, u_id[I, np.maximum(J - 1, 0)], -1), -alpha * cs)
+    add(r, r, rho_f + alpha * (ce + cw + cn + cs + wall))
+    add(r, np.where(J < ny - 1, u_id[I, np.minimum(J + 1, ny - 1)], -1), -alpha * cn)
+    add(r, u_id[np.minimum(I + 1, nx), J], -alpha * ce)
+    add(r, v_id[I - 1, J], alpha * eta_n[I, J] * cxy)
+    add(r, v_id[I, J], -alpha * eta_n[I, J] * cxy)
+    add(r, v_id[I - 1, J + 1], -alpha * eta_n[I, J + 1] * cxy)
+    add(r, v_id[I, J + 1], alpha * eta_n[I, J + 1] * cxy)
+    rhs[r.ravel()] = (rho_f * rng.standard_normal(rho_f.shape) + dt * rng.standard_normal(rho_f.shape)).ravel()
+    # v rows at faces (i, j), j = 1..ny-1
+    I, J = np.meshgrid(np.arange(nx), np.arange(1, ny), indexing="ij")
+    r = v_id[I, J]
+    rho_f = 0.5 * (rho[I, J - 1] + rho[I, J])
+    cn, cs = 2.0 * eta_c[I, J] / dy**2, 2.0 * eta_c[I, J - 1] / dy**2
+    ce, cw = eta_n[I + 1, J] / dx**2, eta_n[I, J] / dx**2
+    wall = np.where(I == nx - 1, ce, 0.0) + np.where(I == 0, cw, 0.0)
+    add(r, np.where(I > 0, v_id[np.maximum(I - 1, 0), J], -1), -alpha * cw)
+    add(r, u_id[I, J - 1], alpha * eta_n[I, J] * cxy)
+    add(r, u_id[I, J], -alpha * eta_n[I, J] * cxy)
+    add(r, v_id[I, J - 1], -alpha * cs)
+    add(r, r, rho_f + alpha * (ce + cw + cn + cs + wall))
+    add(r, v_id[I, J + 1], -alpha * cn)
+    add(r, u_id[I + 1, J - 1], -alpha * eta_n[I + 1, J] * cxy)
+    add(r, u_id[I + 1, J], alpha * eta_n[I + 1, J] * cxy)
+    add(r, np.where(I < nx - 1, v_id[np.minimum(I + 1, nx - 1), J], -1), -alpha * ce)
+    rhs[r.ravel()] = (rho_f * rng.standard_normal(rho_f.shape) + dt * rng.standard_normal(rho_f.shape)).ravel()
+    rows = np.concatenate([a.ravel() for a in rows])
+    cols = np.concatenate([a.ravel() for a in cols])
+    vals = np.concatenate([a.ravel() for a in vals])
+    return csr_from_coo(n, rows, cols, vals, rhs)

@@ -87,6 +87,10 @@ enum pp_status {
   PP_ERR_RZ = 4,            /* "pcg preconditioner breakdown: non-positive new r^T M^{-1} r"     */
   PP_ERR_NODIAG = 5,        /* "sparse matrix assembly failed to produce a diagonal entry"       */
   PP_ERR_SWEEP = 6,         /* internal: wavefront hand-off did not complete                     */
+  PP_ERR_BICG_RHO = 7,      /* "bicgstab breakdown: invalid rho"                                 */
+  PP_ERR_BICG_RHAT_V = 8,   /* "bicgstab breakdown: invalid r_hat^T A M^{-1} p"                  */
+  PP_ERR_BICG_TT = 9,       /* "bicgstab breakdown: invalid t^T t"                               */
+  PP_ERR_BICG_OMEGA = 10,   /* "bicgstab breakdown: invalid omega"                               */
   PP_ERR_ARG = 100,         /* invalid argument (sizes, scheme, pointers)                       */
   PP_ERR_HIP = 101          /* HIP runtime failure                                               */
 };
@@ -277,6 +281,26 @@ int pp_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_
 int pp_pressure_step(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vbc, int ghost, const double *rho_mid,
                      const double *u_star, const double *v_star, double *p_corr, double *u, double *v,
                      double *pressure, double *pressure_prev, double *speeds, pp_report *report);
+
+/*
+ * The momentum predictor's linear solve (momentum.cpp:469-472): build_diagonal_preconditioner
+ * (ch_sparse_krylov.hpp:421-431) and solve_preconditioned_bicgstab (ch_sparse_krylov.hpp:839-963)
+ * on an n x n CSR matrix in the form finalize_row_maps produces (ch_sparse_krylov.hpp:105-133):
+ * row_ptr[n + 1], col_idx / values[row_ptr[n]] with ascending columns and one diagonal entry per
+ * row. x is the initial guess on entry (the reference passes zeros) and the solution on return.
+ * use_relative_tolerance as the reference's argument (momentum.cpp passes false). The report's
+ * iterations and absolute / relative / iterate residuals are LinearSolveReport's; kernel_ms is the
+ * device time of the whole solve, host round trips between its passes included. Errors carry the
+ * reference's texts: PP_ERR_ARG "cannot solve an empty linear system" (n <= 0), PP_ERR_NODIAG,
+ * PP_ERR_BICG_*; x then holds the iterate at the breakdown, as the reference's vector does.
+ * Dot products are summed in 512-row chunks (differing from the reference's sequential sums by
+ * rounding only); everything else is the reference's arithmetic term for term.
+ */
+int pp_bicgstab_device(pp_ctx *ctx, int n, const int *d_row_ptr, const int *d_col_idx, const double *d_values,
+                       const double *d_rhs, double *d_x, int max_iterations, double tolerance,
+                       int use_relative_tolerance, pp_report *report, void *stream);
+int pp_bicgstab(pp_ctx *ctx, int n, const int *row_ptr, const int *col_idx, const double *values, const double *rhs,
+                double *x, int max_iterations, double tolerance, int use_relative_tolerance, pp_report *report);
 
 /*
  * Tuning table of the context (written by the project's autotuner, ding_nsch_core_solver_amd/

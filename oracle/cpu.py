@@ -15,6 +15,7 @@ _lib = None
 
 SCHEMES = {"icpcg": 1, "ildlt_pcg": 2, "split_icpcg": 3, "split_ildlt_pcg": 4}
 SEQUENTIAL, PAIRWISE, CANONICAL = 0, 1, 2
+CHUNKED = 3  # BiCGSTAB dot products in the engine's order (512-row chunks, then chunk order)
 
 # One PCG monitor record (the scalar fields of CGMonitorData, ch_sparse_krylov.hpp:47-58).
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
@@ -92,6 +93,9 @@ def lib():
         L.or_velocity_correction.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), ctypes.POINTER(OrVelocityBc),
                                              ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.or_velocity_correction.restype = None
+        L.or_bicgstab.argtypes = [ctypes.c_int, ip, ip, dp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  ctypes.c_int, ip, dp, dp, dp, ctypes.c_char_p, ctypes.c_int]
+        L.or_bicgstab.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -293,3 +297,28 @@ def velocity_correction(case, vbc, split: bool, ghost: int, rho_mid, u_star, v_s
     lib().or_velocity_correction(int(split), ctypes.byref(problem(case)), ctypes.byref(velocity_bc(vbc)), int(ghost),
                                  *[_dp(a) for a in args], _dp(u), _dp(v), _dp(pressure), _dp(prev), _dp(speeds))
     return u, v, pressure, prev, (float(speeds[0]), float(speeds[1]))
+
+
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def bicgstab(system, x0=None, max_iterations: int = 200, tolerance: float = 1e-8, use_relative: bool = False,
+             reduction: int = SEQUENTIAL):
+    """BiCGSTAB with the diagonal preconditioner (ch_sparse_krylov.hpp:421-431, 839-963) on a CSR
+    system (row_ptr, col_idx, values, rhs). Returns (x, iterations, absolute, relative, iterate
+    residual); raises RuntimeError with the reference's text on a breakdown."""
+    L = lib()
+    rp, ci, val, rhs = system.row_ptr, system.col_idx, system.values, system.rhs
+    n = len(rp) - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = ctypes.c_int(0)
+    ab, rel, itr = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+    err = ctypes.create_string_buffer(512)
+    rc = L.or_bicgstab(n, _ip(rp), _ip(ci), _dp(val), _dp(rhs), _dp(x), int(max_iterations), float(tolerance),
+                       int(bool(use_relative)), int(reduction), ctypes.byref(it), ctypes.byref(ab), ctypes.byref(rel),
+                       ctypes.byref(itr), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return x, it.value, ab.value, rel.value, itr.value

@@ -1123,3 +1123,167 @@ void or_apply_pressure_bc(const OrProblem *pb, double *f) { fill_pressure_ghosts
 void or_pressure_ghosts(const OrProblem *pb, int G, double *f) { fill_pressure_ghosts(*pb, G, f); }
 
 }  // extern "C"
+
+// ---- momentum predictor's linear solve: BiCGSTAB with the diagonal preconditioner ---------------
+//
+// build_diagonal_preconditioner (ch_sparse_krylov.hpp:421-431) and solve_preconditioned_bicgstab
+// (ch_sparse_krylov.hpp:839-963), as momentum.cpp:469-472 calls them, on a CSR matrix in the form
+// finalize_row_maps produces (ch_sparse_krylov.hpp:105-133: ascending columns, one diagonal entry
+// per row). SpMV row sums follow apply_matrix (235-250); dot products are either dot_product's
+// sequential sum (84-90, kSequential) or the HIP engine's order (kChunked, bicg_sum below).
+namespace {
+
+constexpr int kChunked = 3;
+
+// The engine's BiCGSTAB dot-product order: rows are cut into chunks of 512; in chunk c, thread t
+// holds the term of row 512 c + t added to 0.0 (0.0 past the last row), the 64-lane trees
+// (offsets 32..1) and the 8-wave tree (offsets 4..1) give the chunk sum, and the chunk sums are
+// added in chunk order from 0.0.
+template <class F>
+double bicg_sum(int mode, std::size_t n, F f) {
+  if (mode == kSequential) {
+    double s = 0.0;
+    for (std::size_t k = 0; k < n; ++k) s += f(k);
+    return s;
+  }
+  constexpr std::size_t T = 512, W = 8;
+  double total = 0.0;
+  for (std::size_t c0 = 0; c0 < n; c0 += T) {
+    double part[T];
+    for (std::size_t t = 0; t < T; ++t) {
+      double s = 0.0;
+      if (c0 + t < n) s = s + f(c0 + t);
+      part[t] = s;
+    }
+    double wsum[W];
+    for (std::size_t w = 0; w < W; ++w) {
+      double *v = part + 64 * w;
+      for (std::size_t off = 32; off >= 1; off >>= 1)
+        for (std::size_t l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+      wsum[w] = v[0];
+    }
+    for (std::size_t off = W / 2; off >= 1; off >>= 1)
+      for (std::size_t t = 0; t < off; ++t) wsum[t] = wsum[t] + wsum[t + off];
+    total = total + wsum[0];
+  }
+  return total;
+}
+
+struct BicgReport {
+  int iterations = 0;
+  double abs_res = 0.0, rel_res = 0.0, iter_res = 0.0;
+};
+
+void csr_apply(int n, const int *rp, const int *ci, const double *val, const std::vector<double> &x,
+               std::vector<double> &y) {
+  y.assign(static_cast<std::size_t>(n), 0.0);
+  for (int row = 0; row < n; ++row) {
+    double sum = 0.0;
+    for (int pos = rp[row]; pos < rp[row + 1]; ++pos) sum += val[pos] * x[static_cast<std::size_t>(ci[pos])];
+    y[static_cast<std::size_t>(row)] = sum;
+  }
+}
+
+BicgReport bicgstab(int mode, int n, const int *rp, const int *ci, const double *val, const double *rhs,
+                    std::vector<double> &x, int max_iterations, double tolerance, bool use_rel) {
+  if (n <= 0) throw std::runtime_error("cannot solve an empty linear system");
+  const std::size_t N = static_cast<std::size_t>(n);
+  // diagonal preconditioner: 1 / max(|a_ii|, 1e-30), the diagonal located as finalize_row_maps does
+  std::vector<double> dinv(N, 1.0);
+  for (int row = 0; row < n; ++row) {
+    int diag = -1;
+    for (int pos = rp[row]; pos < rp[row + 1]; ++pos)
+      if (ci[pos] == row) diag = pos;
+    if (diag < 0) throw std::runtime_error("sparse matrix assembly failed to produce a diagonal entry");
+    dinv[static_cast<std::size_t>(row)] = 1.0 / std::max(std::abs(val[diag]), 1.0e-30);
+  }
+  auto dot = [&](const std::vector<double> &a, const std::vector<double> &b) {
+    return bicg_sum(mode, N, [&](std::size_t k) { return a[k] * b[k]; });
+  };
+  std::vector<double> ax;
+  csr_apply(n, rp, ci, val, x, ax);
+  std::vector<double> r(N), b(rhs, rhs + N);
+  for (std::size_t k = 0; k < N; ++k) r[k] = rhs[k] - ax[k];
+  const double rhs_norm = std::sqrt(std::max(dot(b, b), 0.0));
+  BicgReport rep;
+  rep.abs_res = std::sqrt(std::max(dot(r, r), 0.0));
+  rep.rel_res = rep.abs_res / std::max(rhs_norm, 1.0e-30);
+  if ((use_rel && rep.rel_res < tolerance) || rep.abs_res < tolerance) return rep;
+
+  std::vector<double> r_hat = r, p(N, 0.0), v(N, 0.0), phat(N), s(N, 0.0), shat(N), t(N, 0.0), update(N, 0.0);
+  double rho_old = 1.0, alpha = 1.0, omega = 1.0;
+  const int limit = std::max(1, max_iterations);
+  for (int iter = 0; iter < limit; ++iter) {
+    const double rho_new = dot(r_hat, r);
+    if (!std::isfinite(rho_new) || std::abs(rho_new) < 1.0e-30) throw std::runtime_error("bicgstab breakdown: invalid rho");
+    const double beta = (iter == 0) ? 0.0 : (rho_new / rho_old) * (alpha / omega);
+    for (std::size_t k = 0; k < N; ++k) p[k] = r[k] + beta * (p[k] - omega * v[k]);
+    for (std::size_t k = 0; k < N; ++k) phat[k] = dinv[k] * p[k];
+    csr_apply(n, rp, ci, val, phat, v);
+    const double rhat_v = dot(r_hat, v);
+    if (!std::isfinite(rhat_v) || std::abs(rhat_v) < 1.0e-30)
+      throw std::runtime_error("bicgstab breakdown: invalid r_hat^T A M^{-1} p");
+    alpha = rho_new / rhat_v;
+    for (std::size_t k = 0; k < N; ++k) {
+      s[k] = r[k] - alpha * v[k];
+      update[k] = alpha * phat[k];
+    }
+    const double s_norm = std::sqrt(std::max(dot(s, s), 0.0));
+    if ((use_rel && s_norm / std::max(rhs_norm, 1.0e-30) < tolerance) || s_norm < tolerance) {
+      for (std::size_t k = 0; k < N; ++k) x[k] += update[k];
+      const double update_sq = dot(update, update), x_norm_sq = dot(x, x);
+      rep.iterations = iter + 1;
+      rep.iter_res = std::sqrt(update_sq / std::max(x_norm_sq, 1.0e-30));
+      rep.abs_res = s_norm;
+      rep.rel_res = s_norm / std::max(rhs_norm, 1.0e-30);
+      return rep;
+    }
+    for (std::size_t k = 0; k < N; ++k) shat[k] = dinv[k] * s[k];
+    csr_apply(n, rp, ci, val, shat, t);
+    const double tt = dot(t, t);
+    if (!std::isfinite(tt) || std::abs(tt) < 1.0e-30) throw std::runtime_error("bicgstab breakdown: invalid t^T t");
+    omega = dot(t, s) / tt;
+    if (!std::isfinite(omega) || std::abs(omega) < 1.0e-30) throw std::runtime_error("bicgstab breakdown: invalid omega");
+    for (std::size_t k = 0; k < N; ++k) {
+      update[k] += omega * shat[k];
+      x[k] += update[k];
+      r[k] = s[k] - omega * t[k];
+    }
+    const double update_sq = dot(update, update), x_norm_sq = dot(x, x);
+    rep.iterations = iter + 1;
+    rep.iter_res = std::sqrt(update_sq / std::max(x_norm_sq, 1.0e-30));
+    rep.abs_res = std::sqrt(std::max(dot(r, r), 0.0));
+    rep.rel_res = rep.abs_res / std::max(rhs_norm, 1.0e-30);
+    if ((use_rel && rep.rel_res < tolerance) || rep.abs_res < tolerance) return rep;
+    rho_old = rho_new;
+  }
+  return rep;
+}
+
+}  // namespace
+
+extern "C" {
+
+// BiCGSTAB (diagonal preconditioner) on a CSR system; x holds the initial guess and receives the
+// solution. reduction: 0 the reference's sequential dot products, 3 the engine's chunked order.
+// Returns 0, or 1 with the reference's error text in err.
+int or_bicgstab(int n, const int *row_ptr, const int *col_idx, const double *values, const double *rhs, double *x,
+                int max_iterations, double tolerance, int use_relative, int reduction, int *iterations,
+                double *abs_res, double *rel_res, double *iter_res, char *err, int errlen) {
+  try {
+    std::vector<double> xv(x, x + std::max(n, 0));
+    const BicgReport rep = bicgstab(reduction == kChunked ? kChunked : kSequential, n, row_ptr, col_idx, values, rhs,
+                                    xv, max_iterations, tolerance, use_relative != 0);
+    std::copy(xv.begin(), xv.end(), x);
+    *iterations = rep.iterations;
+    *abs_res = rep.abs_res;
+    *rel_res = rep.rel_res;
+    *iter_res = rep.iter_res;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+}  // extern "C"

@@ -73,6 +73,9 @@ def lib():
                                               ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp, ctypes.c_int, dp,
                                               ctypes.c_char_p, ctypes.c_int]
         L.ref_velocity_correction.restype = ctypes.c_int
+        L.ref_bicgstab.argtypes = [ctypes.c_int, ip, ip, dp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                   ip, dp, dp, dp, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
+        L.ref_bicgstab.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -223,3 +226,29 @@ def pressure_ghosts(case, field_ghosted: np.ndarray, ghost: int) -> np.ndarray:
     assert field_ghosted.size == (case.nx + 2 * ghost) * (case.ny + 2 * ghost)
     lib().ref_pressure_ghosts(ctypes.byref(problem(case)), int(ghost), _dp(field_ghosted))
     return field_ghosted
+
+
+def bicgstab(system, x0=None, max_iterations: int = 200, tolerance: float = 1e-8, use_relative: bool = False,
+             repeats: int = 1, timing: list | None = None):
+    """The reference's build_diagonal_preconditioner + solve_preconditioned_bicgstab on a CSR system
+    (row_ptr, col_idx, values, rhs), after its finalize_row_maps. Returns (x, iterations, absolute,
+    relative, iterate residual); raises RuntimeError with the reference's text. With a timing list,
+    the mean seconds of one solve over `repeats` is appended to it."""
+    L = lib()
+    rp, ci, val, rhs = system.row_ptr, system.col_idx, system.values, system.rhs
+    assert rp.dtype == np.int32 and ci.dtype == np.int32
+    n = len(rp) - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = ctypes.c_int(0)
+    ab, rel, itr = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+    seconds = np.zeros(1)
+    err = ctypes.create_string_buffer(512)
+    ipp = ctypes.POINTER(ctypes.c_int)
+    rc = L.ref_bicgstab(n, rp.ctypes.data_as(ipp), ci.ctypes.data_as(ipp), _dp(val), _dp(rhs), _dp(x),
+                        int(max_iterations), float(tolerance), int(bool(use_relative)), ctypes.byref(it),
+                        ctypes.byref(ab), ctypes.byref(rel), ctypes.byref(itr), int(repeats), _dp(seconds), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    if timing is not None:
+        timing.append(float(seconds[0]))
+    return x, it.value, ab.value, rel.value, itr.value

@@ -351,3 +351,48 @@ int ref_sys_pcg_monitored(void *h, int max_iterations, double tolerance, double 
 }
 
 }  // extern "C"
+
+// ---- the momentum predictor's linear solve -------------------------------------------------------
+
+extern "C" {
+
+// The reference's build_diagonal_preconditioner + solve_preconditioned_bicgstab
+// (ch_sparse_krylov.hpp:421-431, 839-963) on the CSR system given, after the reference's own
+// finalize_row_maps (105-133) rebuilt it from per-row maps, as momentum.cpp:469-472 calls them.
+// repeats > 1 times the solve (from the same x) and returns the mean in *seconds.
+int ref_bicgstab(int n, const int *row_ptr, const int *col_idx, const double *values, const double *rhs, double *x,
+                 int max_iterations, double tolerance, int use_relative, int *iterations, double *abs_res,
+                 double *rel_res, double *iter_res, int repeats, double *seconds, char *err, int errlen) {
+  namespace K = ding::ch_sparse_krylov;
+  try {
+    std::vector<std::map<int, double>> rows(static_cast<std::size_t>(std::max(n, 0)));
+    for (int r = 0; r < n; ++r)
+      for (int pos = row_ptr[r]; pos < row_ptr[r + 1]; ++pos) rows[static_cast<std::size_t>(r)][col_idx[pos]] = values[pos];
+    K::SparseMatrixCSR matrix;
+    K::finalize_row_maps(rows, matrix);
+    const std::vector<double> b(rhs, rhs + std::max(n, 0));
+    const std::vector<double> x0(x, x + std::max(n, 0));
+    std::vector<double> xv;
+    K::LinearSolveReport rep;
+    const int reps = std::max(1, repeats);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < reps; ++k) {
+      xv = x0;
+      const K::KrylovPreconditioner pre = K::build_diagonal_preconditioner(matrix);
+      rep = K::solve_preconditioned_bicgstab(matrix, pre, b, max_iterations, tolerance, xv, use_relative != 0);
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count() / reps;
+    std::copy(xv.begin(), xv.end(), x);
+    *iterations = rep.iterations;
+    *abs_res = rep.absolute_residual;
+    *rel_res = rep.relative_residual;
+    *iter_res = rep.iterate_residual;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+}  // extern "C"

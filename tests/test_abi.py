@@ -110,3 +110,15 @@ def test_field_size_check_rejects_wrong_buffers():
     bad = dict(ok, v=np.zeros(11 * 9))
     with pytest.raises(ValueError, match="v: 99 values, expected 100"):
         engine.Engine._check_field_sizes(case, g, **bad)
+
+
+def test_status_codes_match_header():
+    """engine.py's PP_* status constants are the header's pp_status values."""
+    import re
+    from ding_nsch_core_solver_amd import engine as E
+    text = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                             "pp_solver.h")).read()
+    body = text[text.index("enum pp_status"):]
+    body = body[:body.index("};")]
+    codes = dict(re.findall(r"(PP_\w+)\s*=\s*(\d+)", body))
+    assert codes and all(getattr(E, k) == int(v) for k, v in codes.items()), codes

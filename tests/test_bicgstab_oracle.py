@@ -1,0 +1,136 @@
+"""The momentum predictor's linear solve (build_diagonal_preconditioner + solve_preconditioned_bicgstab,
+ch_sparse_krylov.hpp:421-431, 839-963, as momentum.cpp:469-472 calls it): the oracle's restatement
+against the reference's own functions compiled from its sources, bit for bit with sequential dot
+products, on momentum-shaped systems and on the reference's own assembled pressure systems; the
+same error texts on every breakdown the tests can force; and checks that need no reference (exact
+one-step solves of diagonal systems, the engine's chunked dot order within rounding)."""
+import numpy as np
+import pytest
+
+from ding_nsch_core_solver_amd import workloads as W
+from tests.cases import small_cases
+
+SYSTEMS = [("momentum_12x10", dict(nx=12, ny=10, seed=1)),
+           ("momentum_24x20_dr1000", dict(nx=24, ny=20, seed=2)),
+           ("momentum_33x17_visc1000", dict(nx=33, ny=17, seed=3, viscosity_ratio=1000.0)),
+           ("momentum_40x40_dt1e-2", dict(nx=40, ny=40, seed=4, dt=1.0e-2))]
+
+
+def _system(kw):
+    return W.momentum_system(**kw)
+
+
+def _csr(n, rows):
+    """CSR from a dense row list (entries kept where nonzero)."""
+    rp, ci, val = [0], [], []
+    for r in rows:
+        for c, v in enumerate(r):
+            if v != 0.0:
+                ci.append(c)
+                val.append(float(v))
+        rp.append(len(ci))
+    return np.array(rp, dtype=np.int32), np.array(ci, dtype=np.int32), np.array(val, dtype=np.float64)
+
+
+def _dense_system(a, b):
+    rp, ci, val = _csr(len(b), a)
+    return W.CsrSystem(rp, ci, val, np.array(b, dtype=np.float64))
+
+
+@pytest.mark.parametrize("name,kw", SYSTEMS, ids=[n for n, _ in SYSTEMS])
+@pytest.mark.parametrize("tol,rel", [(1e-8, False), (1e-10, True), (1e-13, False)])
+def test_oracle_matches_reference_bitwise(oracle_lib, ref_lib, name, kw, tol, rel):
+    s = _system(kw)
+    got = oracle_lib.bicgstab(s, max_iterations=50, tolerance=tol, use_relative=rel)
+    expect = ref_lib.bicgstab(s, max_iterations=50, tolerance=tol, use_relative=rel)
+    assert got[1:] == expect[1:]
+    assert np.array_equal(got[0], expect[0])
+
+
+def test_nonzero_initial_guess_and_iteration_limit(oracle_lib, ref_lib):
+    s = _system(dict(nx=30, ny=26, seed=7))
+    x0 = np.random.default_rng(0).standard_normal(s.n)
+    for limit in (1, 2, 5):
+        got = oracle_lib.bicgstab(s, x0, max_iterations=limit, tolerance=1e-15)
+        expect = ref_lib.bicgstab(s, x0, max_iterations=limit, tolerance=1e-15)
+        assert got[1] == limit and got[1:] == expect[1:]
+        assert np.array_equal(got[0], expect[0])
+
+
+@pytest.mark.parametrize("name,case", small_cases()[:8], ids=[n for n, _ in small_cases()[:8]])
+def test_on_the_reference_pressure_systems(oracle_lib, ref_lib, name, case):
+    """The reference's own assembled pressure matrices (pressure.cpp:123-196) as BiCGSTAB input."""
+    r = ref_lib.RefSystem(case)
+    s = W.CsrSystem(r.row_ptr.copy(), r.col_idx.copy(), r.values.copy(), r.rhs.copy())
+    got = oracle_lib.bicgstab(s, max_iterations=200, tolerance=1e-9, use_relative=True)
+    expect = ref_lib.bicgstab(s, max_iterations=200, tolerance=1e-9, use_relative=True)
+    assert got[1:] == expect[1:]
+    assert np.array_equal(got[0], expect[0])
+
+
+def _errors(lib, s, **kw):
+    try:
+        lib.bicgstab(s, **kw)
+    except RuntimeError as e:
+        return str(e)
+    return None
+
+
+@pytest.mark.parametrize("label,a,b,kw,text", [
+    ("nan_rhs", [[2.0, 1.0], [1.0, 3.0]], [np.nan, 1.0], {}, "bicgstab breakdown: invalid rho"),
+    ("rhat_v", [[1.0, 0.0], [0.0, -1.0]], [1.0, 1.0], {}, "bicgstab breakdown: invalid r_hat^T A M^{-1} p"),
+    ("tt", [[1.0, 0.0], [0.0, -1.0]], [1.0, 0.0], dict(tolerance=0.0), "bicgstab breakdown: invalid t^T t"),
+    ("no_diagonal", [[0.0, 1.0], [1.0, 2.0]], [1.0, 1.0], {},
+     "sparse matrix assembly failed to produce a diagonal entry"),
+])
+def test_breakdown_texts_match_reference(oracle_lib, ref_lib, label, a, b, kw, text):
+    s = _dense_system(a, b)
+    assert _errors(oracle_lib, s, **kw) == text
+    assert _errors(ref_lib, s, **kw) == text
+
+
+def test_empty_system(oracle_lib, ref_lib):
+    s = W.CsrSystem(np.zeros(1, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0), np.zeros(0))
+    assert _errors(oracle_lib, s) == "cannot solve an empty linear system"
+    assert _errors(ref_lib, s) == "cannot solve an empty linear system"
+
+
+def test_diagonal_system_solves_in_one_step(oracle_lib):
+    """M = diag(|a_ii|)^-1 is exact for a positive diagonal A: p_hat = x*, s = 0 up to rounding,
+    converged at the half step with x = alpha p_hat, alpha = 1 up to rounding."""
+    d = np.array([4.0, 2.0, 0.5, 8.0])
+    b = np.array([1.0, 3.0, -2.0, 0.25])
+    s = _dense_system(np.diag(d), b)
+    x, it, ab, rel, itr = oracle_lib.bicgstab(s, tolerance=1e-12)
+    assert it == 1 and ab < 1e-14
+    assert np.allclose(x, b / d, rtol=4e-16, atol=0)
+    assert np.isclose(itr, 1.0)  # x0 = 0: the update is the whole solution
+
+
+def test_zero_rhs_returns_before_iterating(oracle_lib, ref_lib):
+    s = _system(dict(nx=10, ny=8, seed=0))
+    s.rhs = np.zeros(s.n)
+    got = oracle_lib.bicgstab(s)
+    assert got[1] == 0 and got[2] == 0.0 and not got[0].any()
+    assert got[1:] == ref_lib.bicgstab(s)[1:]
+
+
+@pytest.mark.parametrize("name,kw", SYSTEMS, ids=[n for n, _ in SYSTEMS])
+def test_chunked_order_within_rounding(oracle_lib, name, kw):
+    """The engine's dot-product order changes the result by rounding only."""
+    s = _system(kw)
+    seq = oracle_lib.bicgstab(s, tolerance=1e-10)
+    chk = oracle_lib.bicgstab(s, tolerance=1e-10, reduction=oracle_lib.CHUNKED)
+    assert abs(chk[1] - seq[1]) <= 1
+    assert np.abs(chk[0] - seq[0]).max() <= 1e-8 * np.abs(seq[0]).max()
+
+
+def test_momentum_solution_satisfies_the_system(oracle_lib):
+    """The converged iterate solves A x = b to the absolute tolerance (checked with a dense SpMV)."""
+    s = _system(dict(nx=16, ny=14, seed=5))
+    x, it, ab, rel, itr = oracle_lib.bicgstab(s, tolerance=1e-10)
+    a = np.zeros((s.n, s.n))
+    for r in range(s.n):
+        a[r, s.col_idx[s.row_ptr[r]:s.row_ptr[r + 1]]] = s.values[s.row_ptr[r]:s.row_ptr[r + 1]]
+    assert np.linalg.norm(a @ x - s.rhs) < 1e-9
+    assert np.allclose(x, np.linalg.solve(a, s.rhs), rtol=0, atol=1e-9 * np.abs(x).max())

@@ -1,0 +1,114 @@
+"""HIP BiCGSTAB for the momentum predictor's system (pp_bicgstab / pp_bicgstab_device) against the
+oracle with the engine's chunked dot-product order, bit for bit: iterations, the three residuals
+and x. The oracle is pinned to the reference's own solve_preconditioned_bicgstab in
+tests/test_bicgstab_oracle.py; against the reference directly the iteration counts agree and x
+agrees to rounding. Breakdowns return the reference's error texts."""
+import numpy as np
+import pytest
+
+from ding_nsch_core_solver_amd import engine as E
+from ding_nsch_core_solver_amd import workloads as W
+from tests.cases import small_cases
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = [("momentum_12x10", dict(nx=12, ny=10, seed=1)),
+           ("momentum_24x20_dr1000", dict(nx=24, ny=20, seed=2)),
+           ("momentum_33x17_visc1000", dict(nx=33, ny=17, seed=3, viscosity_ratio=1000.0)),
+           ("momentum_40x40_dt1e-2", dict(nx=40, ny=40, seed=4, dt=1.0e-2)),
+           ("momentum_96x80", dict(nx=96, ny=80, seed=5))]
+
+
+def _check_equal(got_x, rep, expect):
+    x_o, it_o, ab_o, rel_o, itr_o = expect
+    assert rep["iterations"] == it_o
+    assert rep["absolute_residual"] == ab_o and rep["relative_residual"] == rel_o
+    assert rep["iterate_residual"] == itr_o
+    assert np.array_equal(got_x, x_o)
+
+
+@pytest.mark.parametrize("name,kw", SYSTEMS, ids=[n for n, _ in SYSTEMS])
+@pytest.mark.parametrize("tol,rel", [(1e-8, False), (1e-11, True)])
+def test_matches_oracle_bitwise(engine, oracle_lib, name, kw, tol, rel):
+    s = W.momentum_system(**kw)
+    x, rep = engine.bicgstab(s, max_iterations=50, tolerance=tol, use_relative=rel)
+    _check_equal(x, rep, oracle_lib.bicgstab(s, max_iterations=50, tolerance=tol, use_relative=rel,
+                                             reduction=oracle_lib.CHUNKED))
+
+
+@pytest.mark.parametrize("name,case", small_cases()[:6], ids=[n for n, _ in small_cases()[:6]])
+def test_pressure_matrices(engine, oracle_lib, name, case):
+    """The pressure systems' CSR (SPD, hundreds of iterations) through the same solver."""
+    o = oracle_lib.OracleSystem(case)
+    s = W.CsrSystem(o.row_ptr.copy(), o.col_idx.copy(), o.values.copy(), o.rhs.copy())
+    x, rep = engine.bicgstab(s, max_iterations=300, tolerance=1e-9, use_relative=True)
+    _check_equal(x, rep, oracle_lib.bicgstab(s, max_iterations=300, tolerance=1e-9, use_relative=True,
+                                             reduction=oracle_lib.CHUNKED))
+
+
+def test_initial_guess_limit_and_reference(engine, oracle_lib, ref_lib):
+    s = W.momentum_system(64, 48, seed=6)
+    x0 = np.random.default_rng(3).standard_normal(s.n)
+    for limit in (1, 3):
+        x, rep = engine.bicgstab(s, x0, max_iterations=limit, tolerance=1e-15)
+        _check_equal(x, rep, oracle_lib.bicgstab(s, x0, max_iterations=limit, tolerance=1e-15,
+                                                 reduction=oracle_lib.CHUNKED))
+    x, rep = engine.bicgstab(s)  # the reference's momentum settings: 50 iterations, absolute 1e-8
+    x_r, it_r, ab_r, rel_r, _ = ref_lib.bicgstab(s, max_iterations=50, tolerance=1e-8)
+    assert rep["iterations"] == it_r and ab_r < 1e-8 and rep["absolute_residual"] < 1e-8
+    assert np.abs(x - x_r).max() <= 1e-9 * np.abs(x_r).max()
+
+
+def _dense(a, b):
+    rp, ci, val = [0], [], []
+    for r in a:
+        for c, v in enumerate(r):
+            if v != 0.0:
+                ci.append(c)
+                val.append(float(v))
+        rp.append(len(ci))
+    return W.CsrSystem(np.array(rp, dtype=np.int32), np.array(ci, dtype=np.int32), np.array(val),
+                       np.array(b, dtype=np.float64))
+
+
+@pytest.mark.parametrize("a,b,kw,status,text", [
+    ([[2.0, 1.0], [1.0, 3.0]], [np.nan, 1.0], {}, E.PP_ERR_BICG_RHO, "bicgstab breakdown: invalid rho"),
+    ([[1.0, 0.0], [0.0, -1.0]], [1.0, 1.0], {}, E.PP_ERR_BICG_RHAT_V,
+     "bicgstab breakdown: invalid r_hat^T A M^{-1} p"),
+    ([[1.0, 0.0], [0.0, -1.0]], [1.0, 0.0], dict(tolerance=0.0), E.PP_ERR_BICG_TT,
+     "bicgstab breakdown: invalid t^T t"),
+    ([[0.0, 1.0], [1.0, 2.0]], [1.0, 1.0], {}, E.PP_ERR_NODIAG,
+     "sparse matrix assembly failed to produce a diagonal entry"),
+])
+def test_breakdowns(engine, oracle_lib, a, b, kw, status, text):
+    s = _dense(a, b)
+    with pytest.raises(RuntimeError, match=""):
+        oracle_lib.bicgstab(s, **kw)
+    with pytest.raises(E.EngineError) as e:
+        engine.bicgstab(s, **kw)
+    assert e.value.status == status and str(e.value) == text
+
+
+def test_empty_system(engine):
+    s = W.CsrSystem(np.zeros(1, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0), np.zeros(0))
+    with pytest.raises(E.EngineError, match="cannot solve an empty linear system"):
+        engine.bicgstab(s)
+
+
+def test_device_entry_192(engine, oracle_lib):
+    """The 192x192 momentum-shaped system (73 344 unknowns) on device buffers, twice on one
+    context (the second call reuses its workspace): equal to the oracle both times."""
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    s = W.momentum_system(192, 192, seed=0)
+    expect = oracle_lib.bicgstab(s, max_iterations=50, tolerance=1e-8, reduction=oracle_lib.CHUNKED)
+    t = dict(rp=torch.from_numpy(s.row_ptr).to(dev), ci=torch.from_numpy(s.col_idx).to(dev),
+             val=torch.from_numpy(s.values).to(dev), rhs=torch.from_numpy(s.rhs).to(dev))
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    for _ in range(2):
+        x = torch.zeros(s.n, dtype=torch.float64, device=dev)
+        rep = engine.bicgstab_device(s.n, t["rp"].data_ptr(), t["ci"].data_ptr(), t["val"].data_ptr(),
+                                     t["rhs"].data_ptr(), x.data_ptr(), stream=stream)
+        _check_equal(x.cpu().numpy(), rep, expect)
+    print(f"[bicgstab] 192x192 momentum system: {rep['iterations']} iterations, {rep['kernel_ms']:.3f} ms",
+          flush=True)
