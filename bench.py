@@ -262,6 +262,76 @@ def correction_leg(eng, torch, dev, case, steps, step_repeats, cpu_seconds, cpu=
     return out
 
 
+def bicgstab_bytes(n, nnz):
+    """Algorithmic HBM bytes of one BiCGSTAB iteration (FP64 vectors, int32 CSR indices): the
+    direction pass (reads r, p, v, M^-1; writes p, p_hat), two CSR SpMVs (row_ptr, col_idx, values,
+    the operand once, one dot operand, the result), the half step (reads r, v, p_hat, M^-1; writes
+    s, update, s_hat) and the full step (reads update, s_hat, x, s, t, r_hat; writes update, x, r)."""
+    spmv = 4 * (n + 1) + 12 * nnz + 3 * 8 * n
+    return (6 + 7 + 9) * 8 * n + 2 * spmv
+
+
+def momentum_leg(eng, torch, dev, steps, cpu_seconds, cpu=True):
+    """The momentum predictor's linear solve (build_diagonal_preconditioner +
+    solve_preconditioned_bicgstab with the reference's momentum settings: 50 iterations, absolute
+    tolerance 1e-8, x0 = 0) on a synthetic 192x192 momentum-shaped CSR system (73 344 unknowns),
+    resident in HBM; one pp_bicgstab_device call per step, x zeroed before each."""
+    s = W.momentum_system(192, 192, seed=0)
+    t = {k: torch.from_numpy(getattr(s, k)).to(dev) for k in ("row_ptr", "col_idx", "values", "rhs")}
+    x = torch.zeros(s.n, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def run():
+        x.zero_()
+        return eng.bicgstab_device(s.n, t["row_ptr"].data_ptr(), t["col_idx"].data_ptr(), t["values"].data_ptr(),
+                                   t["rhs"].data_ptr(), x.data_ptr(), stream=stream)
+
+    for _ in range(3):
+        rep = run()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rep = run()
+        kernel_ms.append(rep["kernel_ms"])
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t0) / steps
+    kms = float(np.mean(kernel_ms))
+    nnz = int(s.row_ptr[-1])
+    alg = bicgstab_bytes(s.n, nnz) * rep["iterations"]
+    out = {"value": 1.0 / elapsed, "unit": "solves/s", "ms_per_step": elapsed * 1e3, "device_ms": kms,
+           "iterations": rep["iterations"], "absolute_residual": rep["absolute_residual"], "unknowns": s.n,
+           "nonzeros": nnz, "data": "synthetic momentum-shaped system (droplet density and viscosity)",
+           "kernels": "bicg_direction/spmv_v/half_step/spmv_t/full_step_kernel, host scalar recurrence",
+           "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "algorithmic_bytes_per_solve": alg,
+                        "note": "whole solve, host round trips between passes included"}}
+    if cpu:
+        try:
+            from oracle import ref
+            if not ref.available():
+                raise FileNotFoundError
+        except (ImportError, FileNotFoundError):
+            return out
+        x_gpu = x.cpu().numpy()
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+        try:
+            x_r, it_r, ab_r, _, _ = ref.bicgstab(s, max_iterations=50, tolerance=1e-8)
+            timing = []
+            reps = max(3, min(500, int(cpu_seconds / 0.03)))
+            ref.bicgstab(s, max_iterations=50, tolerance=1e-8, repeats=reps, timing=timing)
+        finally:
+            os.sched_setaffinity(0, old)
+        out["parity"] = {"iterations_gpu": rep["iterations"], "iterations_cpu": it_r,
+                         "max_abs_diff_over_max_abs": float(np.abs(x_gpu - x_r).max() / np.abs(x_r).max())}
+        out["cpu_baseline"] = {"value": 1.0 / timing[0], "unit": "solves/s", "cores": 1, "kind": "reference",
+                               "sample": f"{reps} solves of the same system, mean {timing[0] * 1e3:.2f} ms"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def batched_cpu_baseline(cases, threads):
     """Reference icpcg on every system of the batch, one system per thread (the reference is
     single-threaded; independent solves run concurrently), wall time of the whole batch."""
@@ -385,6 +455,11 @@ def main():
                                                  cpu=not args.no_cpu_baseline)
     log(f"[bench] correction: {line['velocity_correction']['value']:.0f} /s, kernels "
         f"{line['velocity_correction']['kernel_ms'] * 1e3:.0f} us")
+
+    log("[bench] momentum predictor BiCGSTAB")
+    line["momentum_bicgstab"] = momentum_leg(eng, torch, dev, args.steps * 5, 3.0, cpu=not args.no_cpu_baseline)
+    log(f"[bench] bicgstab: {line['momentum_bicgstab']['device_ms']:.3f} ms, "
+        f"{line['momentum_bicgstab']['iterations']} iterations")
 
     batch_cases = None
     if not args.no_batched:

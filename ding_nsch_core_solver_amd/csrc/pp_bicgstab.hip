@@ -5,11 +5,19 @@
 // vector pass is a full-chip kernel: 512-thread workgroups, workgroup c owning rows [512 c, 512 c
 // + 512), one thread per row. SpMV row sums run over the row's CSR entries in order from 0.0
 // (apply_matrix, 235-250), and every vector update is the reference's expression term for term,
-// so only the dot products differ from the reference, by summation order: each kernel leaves one
-// partial per workgroup (the canonical 64-lane and 8-wave trees over its 512 rows) and the host
-// adds the partials in workgroup order. The scalar recurrence (rho, alpha, omega, beta, the
-// breakdown checks and the stopping tests) runs on the host between launches.
+// so only the dot products differ from the reference, by summation order: each pass leaves one
+// partial per workgroup (the canonical 64-lane and 8-wave trees over its 512 rows), and the next
+// pass adds the partials in workgroup order.
+//
+// The scalar recurrence lives on the device. Each pass starts by forming the scalars it needs
+// from the previous pass's partials (every workgroup computes the same values in the same order)
+// and runs the reference's test at that point of the iteration: a stop (convergence or a
+// breakdown) is decided identically by every workgroup, which then write nothing, and workgroup 0
+// records it in the state block. Every later pass sees the state and returns at once, so the host
+// can queue several iterations between its checks.
 #include <hip/hip_runtime.h>
+
+#include <math.h>
 
 #include "pp_kernels.hpp"
 
@@ -47,12 +55,69 @@ __device__ __forceinline__ void chunk_partials(double (&v)[K], double *partials,
   }
 }
 
+// The K dot products of a previous pass: sum k is partials[k * chunks + c] added over c in order
+// from 0.0 by thread k. The partials are staged through LDS in tiles of kTile per sum (one
+// cooperative load per tile) and read back 8 at a time, so the chain waits on LDS, not on memory.
+// Every thread receives all K.
+constexpr int kTile = 512;
+template <int K>
+__device__ __forceinline__ void dot_sums(const double *partials, int chunks, double (&out)[K]) {
+  __shared__ double tile[K * kTile];
+  __shared__ double sums[K];
+  double total = 0.0;
+  __syncthreads();  // an earlier call's tile and sums have been read
+  for (int c0 = 0; c0 < chunks; c0 += kTile) {
+    const int m = chunks - c0 < kTile ? chunks - c0 : kTile;
+    for (int i = threadIdx.x; i < K * kTile; i += kRows) {
+      const int k = i / kTile, c = i - k * kTile;
+      if (c < m) tile[i] = partials[k * chunks + c0 + c];
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+      const double *t = tile + threadIdx.x * kTile;
+      for (int c = 0; c < m; c += 8) {
+        double b[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = c + j < m ? t[c + j] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (c + j < m) total = total + b[j];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < K) sums[threadIdx.x] = total;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = sums[k];
+}
+
 __device__ __forceinline__ double csr_row(const int *__restrict__ rp, const int *__restrict__ ci,
                                           const double *__restrict__ val, const double *__restrict__ x, int row) {
   double sum = 0.0;
   const int end = rp[row + 1];
   for (int pos = rp[row]; pos < end; ++pos) sum += val[pos] * x[ci[pos]];
   return sum;
+}
+
+__device__ __forceinline__ double max0(double v) { return v < 0.0 ? 0.0 : v; }      // std::max(v, 0.0)
+__device__ __forceinline__ double floor30(double v) { return v < 1.0e-30 ? 1.0e-30 : v; }  // std::max(v, 1e-30)
+
+__device__ __forceinline__ bool converged(const BicgArgs &a, double rel, double abs_res) {
+  return (a.use_rel && rel < a.tolerance) || abs_res < a.tolerance;
+}
+
+
+// Workgroup 0 records a stop (status != 0) with the report fields.
+__device__ __forceinline__ void record(const BicgArgs &a, int status, int iterations, double abs_res, double rel,
+                                       double iter_res, bool with_iter_res) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.state->iterations = iterations;
+    a.state->abs_res = abs_res;
+    a.state->rel_res = rel;
+    if (with_iter_res) a.state->iter_res = iter_res;
+    a.state->status = status;
+  }
 }
 
 // Diagonal preconditioner (1 / max(|a_ii|, 1e-30), the diagonal entry located as
@@ -66,10 +131,9 @@ __global__ void __launch_bounds__(kRows) bicg_setup_kernel(BicgArgs a) {
       if (a.col_idx[pos] == q) diag = pos;
     double d = 1.0;
     if (diag < 0) {
-      *a.flag = 1;
+      a.state->status = kBicgNoDiag;
     } else {
-      const double m = fabs(a.values[diag]);
-      d = 1.0 / (m < 1.0e-30 ? 1.0e-30 : m);
+      d = 1.0 / floor30(fabs(a.values[diag]));
     }
     a.dinv[q] = d;
     const double ax = csr_row(a.row_ptr, a.col_idx, a.values, a.x, q);
@@ -82,13 +146,50 @@ __global__ void __launch_bounds__(kRows) bicg_setup_kernel(BicgArgs a) {
     v[0] = v[0] + b * b;
     v[1] = v[1] + r * r;
   }
-  chunk_partials<2>(v, a.partials, a.chunks);
+  chunk_partials<2>(v, a.p_setup, a.chunks);
 }
 
-// p = r + beta (p - omega v); p_hat = M^-1 p
-__global__ void __launch_bounds__(kRows) bicg_direction_kernel(BicgArgs a, double beta, double omega) {
+// Start of iteration `iter` (or the report after the last one, final): the stopping test of the
+// previous iteration (of the initial residual at iter 0), the rho check, then
+// p = r + beta (p - omega v); p_hat = M^-1 p.
+__global__ void __launch_bounds__(kRows) bicg_direction_kernel(BicgArgs a, int iter, int final) {
+  if (a.state->status != kBicgRunning) return;
+  double bn, rho_new;
+  if (iter == 0) {
+    double rr[2];
+    dot_sums<2>(a.p_setup, a.chunks, rr);  // b.b, r.r
+    bn = sqrt(max0(rr[0]));
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.state->rhs_norm = bn;
+    const double abs_res = sqrt(max0(rr[1]));
+    const double rel = abs_res / floor30(bn);
+    if (converged(a, rel, abs_res) || final) {
+      record(a, kBicgDone, 0, abs_res, rel, 0.0, false);
+      return;
+    }
+    rho_new = rr[1];  // r_hat = r: r_hat . r is r . r, summed in the same order
+  } else {
+    bn = a.state->rhs_norm;
+    double d[4];
+    dot_sums<4>(a.p_x, a.chunks, d);  // update.update, x.x, r.r, r_hat.r of iteration iter - 1
+    const double iter_res = sqrt(d[0] / floor30(d[1]));
+    const double abs_res = sqrt(max0(d[2]));
+    const double rel = abs_res / floor30(bn);
+    if (converged(a, rel, abs_res) || final) {
+      record(a, kBicgDone, iter, abs_res, rel, iter_res, true);
+      return;
+    }
+    rho_new = d[3];
+  }
+  if (!isfinite(rho_new) || fabs(rho_new) < 1.0e-30) {
+    record(a, kBicgRho, iter, 0.0, 0.0, 0.0, false);
+    return;
+  }
+  const double rho_old = iter == 0 ? 1.0 : a.state->rho[(iter - 1) & 1];
+  const double beta = (iter == 0) ? 0.0 : (rho_new / rho_old) * (a.state->alpha / a.state->omega);
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.state->rho[iter & 1] = rho_new;
   const int q = blockIdx.x * kRows + threadIdx.x;
   if (q >= a.n) return;
+  const double omega = iter == 0 ? 1.0 : a.state->omega;
   const double p = a.r[q] + beta * (a.p[q] - omega * a.v[q]);
   a.p[q] = p;
   a.p_hat[q] = a.dinv[q] * p;
@@ -96,6 +197,7 @@ __global__ void __launch_bounds__(kRows) bicg_direction_kernel(BicgArgs a, doubl
 
 // v = A p_hat; partial r_hat.v
 __global__ void __launch_bounds__(kRows) bicg_spmv_v_kernel(BicgArgs a) {
+  if (a.state->status != kBicgRunning) return;
   const int q = blockIdx.x * kRows + threadIdx.x;
   double v[1] = {0.0};
   if (q < a.n) {
@@ -103,11 +205,22 @@ __global__ void __launch_bounds__(kRows) bicg_spmv_v_kernel(BicgArgs a) {
     a.v[q] = y;
     v[0] = v[0] + a.r_hat[q] * y;
   }
-  chunk_partials<1>(v, a.partials, a.chunks);
+  chunk_partials<1>(v, a.p_v, a.chunks);
 }
 
-// s = r - alpha v; update = alpha p_hat; s_hat = M^-1 s; partial s.s
-__global__ void __launch_bounds__(kRows) bicg_half_step_kernel(BicgArgs a, double alpha) {
+// alpha = rho / r_hat.v (with its check); s = r - alpha v; update = alpha p_hat; s_hat = M^-1 s;
+// partial s.s
+__global__ void __launch_bounds__(kRows) bicg_half_step_kernel(BicgArgs a, int iter) {
+  if (a.state->status != kBicgRunning) return;
+  double d[1];
+  dot_sums<1>(a.p_v, a.chunks, d);
+  const double rhat_v = d[0];
+  if (!isfinite(rhat_v) || fabs(rhat_v) < 1.0e-30) {
+    record(a, kBicgRhatV, iter, 0.0, 0.0, 0.0, false);
+    return;
+  }
+  const double alpha = a.state->rho[iter & 1] / rhat_v;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.state->alpha = alpha;
   const int q = blockIdx.x * kRows + threadIdx.x;
   double v[1] = {0.0};
   if (q < a.n) {
@@ -117,26 +230,36 @@ __global__ void __launch_bounds__(kRows) bicg_half_step_kernel(BicgArgs a, doubl
     a.s_hat[q] = a.dinv[q] * s;
     v[0] = v[0] + s * s;
   }
-  chunk_partials<1>(v, a.partials, a.chunks);
+  chunk_partials<1>(v, a.p_s, a.chunks);
 }
 
-// converged at the half step: x += update; partials update.update, x.x
-__global__ void __launch_bounds__(kRows) bicg_half_exit_kernel(BicgArgs a) {
+// The half-step test on |s|: converged -> x += update (partials update.update, x.x; workgroup 0
+// marks the half exit and the full-step pass records the report); else t = A s_hat (partials t.t,
+// t.s).
+__global__ void __launch_bounds__(kRows) bicg_spmv_t_kernel(BicgArgs a, int iter) {
+  if (a.state->status != kBicgRunning) return;
+  const double bn = a.state->rhs_norm;
+  double d[1];
+  dot_sums<1>(a.p_s, a.chunks, d);
+  const double s_norm = sqrt(max0(d[0]));
   const int q = blockIdx.x * kRows + threadIdx.x;
-  double v[2] = {0.0, 0.0};
-  if (q < a.n) {
-    const double u = a.update[q];
-    const double x = a.x[q] + u;
-    a.x[q] = x;
-    v[0] = v[0] + u * u;
-    v[1] = v[1] + x * x;
+  if (converged(a, s_norm / floor30(bn), s_norm)) {
+    double v[2] = {0.0, 0.0};
+    if (q < a.n) {
+      const double u = a.update[q];
+      const double x = a.x[q] + u;
+      a.x[q] = x;
+      v[0] = v[0] + u * u;
+      v[1] = v[1] + x * x;
+    }
+    chunk_partials<2>(v, a.p_exit, a.chunks);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // status stays running: every workgroup does its rows
+      a.state->abs_res = s_norm;
+      a.state->rel_res = s_norm / floor30(bn);
+      a.state->half_exit = 1;
+    }
+    return;
   }
-  chunk_partials<2>(v, a.partials, a.chunks);
-}
-
-// t = A s_hat; partials t.t, t.s
-__global__ void __launch_bounds__(kRows) bicg_spmv_t_kernel(BicgArgs a) {
-  const int q = blockIdx.x * kRows + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (q < a.n) {
     const double t = csr_row(a.row_ptr, a.col_idx, a.values, a.s_hat, q);
@@ -144,11 +267,32 @@ __global__ void __launch_bounds__(kRows) bicg_spmv_t_kernel(BicgArgs a) {
     v[0] = v[0] + t * t;
     v[1] = v[1] + t * a.s[q];
   }
-  chunk_partials<2>(v, a.partials, a.chunks);
+  chunk_partials<2>(v, a.p_t, a.chunks);
 }
 
+// The half-step exit's report, or omega = t.s / t.t (with both checks);
 // update += omega s_hat; x += update; r = s - omega t; partials update.update, x.x, r.r, r_hat.r
-__global__ void __launch_bounds__(kRows) bicg_full_step_kernel(BicgArgs a, double omega) {
+__global__ void __launch_bounds__(kRows) bicg_full_step_kernel(BicgArgs a, int iter) {
+  if (a.state->status != kBicgRunning) return;
+  if (a.state->half_exit) {
+    double d[2];
+    dot_sums<2>(a.p_exit, a.chunks, d);
+    record(a, kBicgDone, iter + 1, a.state->abs_res, a.state->rel_res, sqrt(d[0] / floor30(d[1])), true);
+    return;
+  }
+  double d[2];
+  dot_sums<2>(a.p_t, a.chunks, d);
+  const double tt = d[0];
+  if (!isfinite(tt) || fabs(tt) < 1.0e-30) {
+    record(a, kBicgTT, iter, 0.0, 0.0, 0.0, false);
+    return;
+  }
+  const double omega = d[1] / tt;
+  if (!isfinite(omega) || fabs(omega) < 1.0e-30) {
+    record(a, kBicgOmega, iter, 0.0, 0.0, 0.0, false);
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.state->omega = omega;
   const int q = blockIdx.x * kRows + threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (q < a.n) {
@@ -163,22 +307,28 @@ __global__ void __launch_bounds__(kRows) bicg_full_step_kernel(BicgArgs a, doubl
     v[2] = v[2] + r * r;
     v[3] = v[3] + a.r_hat[q] * r;
   }
-  chunk_partials<4>(v, a.partials, a.chunks);
+  chunk_partials<4>(v, a.p_x, a.chunks);
 }
 
 }  // namespace
 
-hipError_t launch_bicg(BicgStep step, const BicgArgs &a, double s0, double s1, hipStream_t stream) {
+hipError_t launch_bicg_setup(const BicgArgs &a, hipStream_t stream) {
+  bicg_setup_kernel<<<dim3(a.chunks), dim3(kRows), 0, stream>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_bicg_iteration(const BicgArgs &a, int iter, hipStream_t stream) {
   const dim3 grid(a.chunks), block(kRows);
-  switch (step) {
-    case BicgStep::kSetup: bicg_setup_kernel<<<grid, block, 0, stream>>>(a); break;
-    case BicgStep::kDirection: bicg_direction_kernel<<<grid, block, 0, stream>>>(a, s0, s1); break;
-    case BicgStep::kSpmvV: bicg_spmv_v_kernel<<<grid, block, 0, stream>>>(a); break;
-    case BicgStep::kHalfStep: bicg_half_step_kernel<<<grid, block, 0, stream>>>(a, s0); break;
-    case BicgStep::kHalfExit: bicg_half_exit_kernel<<<grid, block, 0, stream>>>(a); break;
-    case BicgStep::kSpmvT: bicg_spmv_t_kernel<<<grid, block, 0, stream>>>(a); break;
-    case BicgStep::kFullStep: bicg_full_step_kernel<<<grid, block, 0, stream>>>(a, s0); break;
-  }
+  bicg_direction_kernel<<<grid, block, 0, stream>>>(a, iter, 0);
+  bicg_spmv_v_kernel<<<grid, block, 0, stream>>>(a);
+  bicg_half_step_kernel<<<grid, block, 0, stream>>>(a, iter);
+  bicg_spmv_t_kernel<<<grid, block, 0, stream>>>(a, iter);
+  bicg_full_step_kernel<<<grid, block, 0, stream>>>(a, iter);
+  return hipGetLastError();
+}
+
+hipError_t launch_bicg_report(const BicgArgs &a, int iterations, hipStream_t stream) {
+  bicg_direction_kernel<<<dim3(a.chunks), dim3(kRows), 0, stream>>>(a, iterations, 1);
   return hipGetLastError();
 }
 

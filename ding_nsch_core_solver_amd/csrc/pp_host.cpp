@@ -63,12 +63,12 @@ struct pp_ctx {
   double *corr_work = nullptr;  // 4 doubles: the two boundary speeds, the pressure mean
 
   // BiCGSTAB (pp_bicgstab*): 10 work vectors of bicg_cap rows, per-workgroup partials, the
-  // missing-diagonal flag, pinned partials for the host's sums, and the host entry's CSR staging
+  // device state block and its pinned copy, and the host entry's CSR staging
   size_t bicg_cap = 0;
   double *bicg_work = nullptr;
-  double *bicg_partials = nullptr;
-  int *bicg_flag = nullptr;
-  double *h_bicg = nullptr;  // pinned, 4 * chunks
+  double *bicg_partials = nullptr;     // 12 * chunks
+  pp::BicgState *bicg_state = nullptr;
+  pp::BicgState *h_bicg = nullptr;     // pinned copy of the state
   size_t bicg_stage_rows = 0, bicg_stage_nnz = 0;
   int *bicg_stage_idx = nullptr;     // row_ptr then col_idx
   double *bicg_stage_val = nullptr;  // values, rhs, x
@@ -498,7 +498,7 @@ int ensure_fields(pp_ctx *ctx, size_t need) {
   if (ctx->field_stage) (void)hipFree(ctx->field_stage);
   if (ctx->bicg_work) (void)hipFree(ctx->bicg_work);
   if (ctx->bicg_partials) (void)hipFree(ctx->bicg_partials);
-  if (ctx->bicg_flag) (void)hipFree(ctx->bicg_flag);
+  if (ctx->bicg_state) (void)hipFree(ctx->bicg_state);
   if (ctx->h_bicg) (void)hipHostFree(ctx->h_bicg);
   if (ctx->bicg_stage_idx) (void)hipFree(ctx->bicg_stage_idx);
   if (ctx->bicg_stage_val) (void)hipFree(ctx->bicg_stage_val);
@@ -1062,44 +1062,38 @@ int pp_solve_batched_device(pp_ctx *ctx, int count, const pp_problem *problems, 
 namespace {
 
 int ensure_bicg(pp_ctx *ctx, size_t n, pp_report *report) {
+  if (!ctx->bicg_state) {
+    PP_HIP_CHECK(hipMalloc(&ctx->bicg_state, sizeof(pp::BicgState)));
+    PP_HIP_CHECK(hipHostMalloc(&ctx->h_bicg, sizeof(pp::BicgState), hipHostMallocDefault));
+  }
   if (n <= ctx->bicg_cap) return PP_OK;
   if (ctx->bicg_work) (void)hipFree(ctx->bicg_work);
   if (ctx->bicg_partials) (void)hipFree(ctx->bicg_partials);
-  if (ctx->h_bicg) (void)hipHostFree(ctx->h_bicg);
   ctx->bicg_work = nullptr;
   ctx->bicg_partials = nullptr;
-  ctx->h_bicg = nullptr;
   ctx->bicg_cap = 0;
   const size_t chunks = (n + pp::kBicgRows - 1) / pp::kBicgRows;
   PP_HIP_CHECK(hipMalloc(&ctx->bicg_work, 10 * n * sizeof(double)));
-  PP_HIP_CHECK(hipMalloc(&ctx->bicg_partials, 4 * chunks * sizeof(double)));
-  PP_HIP_CHECK(hipHostMalloc(&ctx->h_bicg, 4 * chunks * sizeof(double), hipHostMallocDefault));
-  if (!ctx->bicg_flag) PP_HIP_CHECK(hipMalloc(&ctx->bicg_flag, sizeof(int)));
+  PP_HIP_CHECK(hipMalloc(&ctx->bicg_partials, 12 * chunks * sizeof(double)));
   ctx->bicg_cap = n;
   return PP_OK;
 }
 
-// Brings the first k partial sums of the last pass to the host and adds each in workgroup order.
-int bicg_sums(pp_ctx *ctx, const pp::BicgArgs &a, int k, double *out, pp_report *report, hipStream_t s) {
-  const size_t chunks = static_cast<size_t>(a.chunks);
-  PP_HIP_CHECK(hipMemcpyAsync(ctx->h_bicg, a.partials, k * chunks * sizeof(double), hipMemcpyDeviceToHost, s));
-  PP_HIP_CHECK(hipStreamSynchronize(s));
-  for (int j = 0; j < k; ++j) {
-    double total = 0.0;
-    for (size_t c = 0; c < chunks; ++c) total = total + ctx->h_bicg[j * chunks + c];
-    out[j] = total;
-  }
-  return PP_OK;
-}
+// Iterations queued between two reads of the device state: a solve that stops at iteration k
+// leaves at most this many iterations of passes that return at once.
+constexpr int kBicgQueue = 4;
 
-// solve_preconditioned_bicgstab (ch_sparse_krylov.hpp:839-963): the scalar recurrence and every
-// test of the reference, in its order, between the device passes.
+// solve_preconditioned_bicgstab (ch_sparse_krylov.hpp:839-963). The device passes run the
+// reference's recurrence and tests; the host queues iterations and reads the state between
+// groups of kBicgQueue.
 int run_bicgstab(pp_ctx *ctx, int n, const int *rp, const int *ci, const double *val, const double *rhs, double *x,
                  int max_iterations, double tolerance, bool use_rel, pp_report *report, hipStream_t s) {
   if (int rc = ensure_bicg(ctx, static_cast<size_t>(n), report)) return rc;
   pp::BicgArgs a{};
   a.n = n;
   a.chunks = (n + pp::kBicgRows - 1) / pp::kBicgRows;
+  a.use_rel = use_rel ? 1 : 0;
+  a.tolerance = tolerance;
   a.row_ptr = rp;
   a.col_idx = ci;
   a.values = val;
@@ -1107,79 +1101,51 @@ int run_bicgstab(pp_ctx *ctx, int n, const int *rp, const int *ci, const double 
   a.x = x;
   double **vec[10] = {&a.r, &a.r_hat, &a.p, &a.v, &a.p_hat, &a.s, &a.s_hat, &a.t, &a.update, &a.dinv};
   for (int k = 0; k < 10; ++k) *vec[k] = ctx->bicg_work + static_cast<size_t>(k) * n;
-  a.partials = ctx->bicg_partials;
-  a.flag = ctx->bicg_flag;
-  pp_report rep{};
-  auto finish = [&](int status, const char *msg) -> int {
-    PP_HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
-    PP_HIP_CHECK(hipEventSynchronize(ctx->ev_stop));
-    float ms = 0.0f;
-    PP_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-    ctx->last_kernel_ms = ms;
-    rep.kernel_ms = ms;
-    rep.outer_cycles = 1;
-    if (report) *report = rep;
-    if (status != PP_OK) return fail(ctx, report, status, msg);
-    ctx->last_error.clear();
-    return PP_OK;
-  };
-  PP_HIP_CHECK(hipMemsetAsync(a.flag, 0, sizeof(int), s));
-  PP_HIP_CHECK(hipEventRecord(ctx->ev_start, s));
-  PP_HIP_CHECK(pp::launch_bicg(pp::BicgStep::kSetup, a, 0.0, 0.0, s));
-  int flag = 0;
-  PP_HIP_CHECK(hipMemcpyAsync(&flag, a.flag, sizeof(int), hipMemcpyDeviceToHost, s));
-  double d[4];
-  if (int rc = bicg_sums(ctx, a, 2, d, report, s)) return rc;
-  if (flag) return finish(PP_ERR_NODIAG, "sparse matrix assembly failed to produce a diagonal entry");
-  const double rhs_norm = std::sqrt(std::max(d[0], 0.0));
-  rep.absolute_residual = std::sqrt(std::max(d[1], 0.0));
-  rep.relative_residual = rep.absolute_residual / std::max(rhs_norm, 1.0e-30);
-  if ((use_rel && rep.relative_residual < tolerance) || rep.absolute_residual < tolerance) return finish(PP_OK, "");
-  double rho_old = 1.0, alpha = 1.0, omega = 1.0;
-  double rho_next = d[1];  // r_hat = r: r_hat . r is r . r, summed in the same order
+  double *part = ctx->bicg_partials;
+  const size_t c = static_cast<size_t>(a.chunks);
+  a.p_setup = part;
+  a.p_v = part + 2 * c;
+  a.p_s = part + 3 * c;
+  a.p_t = part + 4 * c;
+  a.p_x = part + 6 * c;
+  a.p_exit = part + 10 * c;
+  a.state = ctx->bicg_state;
   const int limit = std::max(1, max_iterations);
-  for (int iter = 0; iter < limit; ++iter) {
-    const double rho_new = rho_next;
-    if (!std::isfinite(rho_new) || std::abs(rho_new) < 1.0e-30)
-      return finish(PP_ERR_BICG_RHO, "bicgstab breakdown: invalid rho");
-    const double beta = (iter == 0) ? 0.0 : (rho_new / rho_old) * (alpha / omega);
-    PP_HIP_CHECK(pp::launch_bicg(pp::BicgStep::kDirection, a, beta, omega, s));
-    PP_HIP_CHECK(pp::launch_bicg(pp::BicgStep::kSpmvV, a, 0.0, 0.0, s));
-    if (int rc = bicg_sums(ctx, a, 1, d, report, s)) return rc;
-    const double rhat_v = d[0];
-    if (!std::isfinite(rhat_v) || std::abs(rhat_v) < 1.0e-30)
-      return finish(PP_ERR_BICG_RHAT_V, "bicgstab breakdown: invalid r_hat^T A M^{-1} p");
-    alpha = rho_new / rhat_v;
-    PP_HIP_CHECK(pp::launch_bicg(pp::BicgStep::kHalfStep, a, alpha, 0.0, s));
-    if (int rc = bicg_sums(ctx, a, 1, d, report, s)) return rc;
-    const double s_norm = std::sqrt(std::max(d[0], 0.0));
-    if ((use_rel && s_norm / std::max(rhs_norm, 1.0e-30) < tolerance) || s_norm < tolerance) {
-      PP_HIP_CHECK(pp::launch_bicg(pp::BicgStep::kHalfExit, a, 0.0, 0.0, s));
-      if (int rc = bicg_sums(ctx, a, 2, d, report, s)) return rc;
-      rep.iterations = iter + 1;
-      rep.iterate_residual = std::sqrt(d[0] / std::max(d[1], 1.0e-30));
-      rep.absolute_residual = s_norm;
-      rep.relative_residual = s_norm / std::max(rhs_norm, 1.0e-30);
-      return finish(PP_OK, "");
-    }
-    PP_HIP_CHECK(pp::launch_bicg(pp::BicgStep::kSpmvT, a, 0.0, 0.0, s));
-    if (int rc = bicg_sums(ctx, a, 2, d, report, s)) return rc;
-    const double tt = d[0];
-    if (!std::isfinite(tt) || std::abs(tt) < 1.0e-30) return finish(PP_ERR_BICG_TT, "bicgstab breakdown: invalid t^T t");
-    omega = d[1] / tt;
-    if (!std::isfinite(omega) || std::abs(omega) < 1.0e-30)
-      return finish(PP_ERR_BICG_OMEGA, "bicgstab breakdown: invalid omega");
-    PP_HIP_CHECK(pp::launch_bicg(pp::BicgStep::kFullStep, a, omega, 0.0, s));
-    if (int rc = bicg_sums(ctx, a, 4, d, report, s)) return rc;
-    rep.iterations = iter + 1;
-    rep.iterate_residual = std::sqrt(d[0] / std::max(d[1], 1.0e-30));
-    rep.absolute_residual = std::sqrt(std::max(d[2], 0.0));
-    rep.relative_residual = rep.absolute_residual / std::max(rhs_norm, 1.0e-30);
-    if ((use_rel && rep.relative_residual < tolerance) || rep.absolute_residual < tolerance) return finish(PP_OK, "");
-    rho_old = rho_new;
-    rho_next = d[3];
+  PP_HIP_CHECK(hipMemsetAsync(a.state, 0, sizeof(pp::BicgState), s));
+  PP_HIP_CHECK(hipEventRecord(ctx->ev_start, s));
+  PP_HIP_CHECK(pp::launch_bicg_setup(a, s));
+  pp::BicgState &st = *ctx->h_bicg;
+  st.status = pp::kBicgRunning;
+  for (int it0 = 0; it0 < limit && st.status == pp::kBicgRunning; it0 += kBicgQueue) {
+    for (int iter = it0; iter < std::min(limit, it0 + kBicgQueue); ++iter)
+      PP_HIP_CHECK(pp::launch_bicg_iteration(a, iter, s));
+    PP_HIP_CHECK(hipMemcpyAsync(&st, a.state, sizeof(pp::BicgState), hipMemcpyDeviceToHost, s));
+    PP_HIP_CHECK(hipStreamSynchronize(s));
   }
-  return finish(PP_OK, "");
+  if (st.status == pp::kBicgRunning) PP_HIP_CHECK(pp::launch_bicg_report(a, limit, s));
+  PP_HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
+  PP_HIP_CHECK(hipMemcpyAsync(&st, a.state, sizeof(pp::BicgState), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  PP_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+  ctx->last_kernel_ms = ms;
+  pp_report rep{};
+  rep.iterations = st.iterations;
+  rep.outer_cycles = 1;
+  rep.absolute_residual = st.abs_res;
+  rep.relative_residual = st.rel_res;
+  rep.iterate_residual = st.iter_res;
+  rep.kernel_ms = ms;
+  if (report) *report = rep;
+  switch (st.status) {
+    case pp::kBicgDone: ctx->last_error.clear(); return PP_OK;
+    case pp::kBicgNoDiag: return fail(ctx, report, PP_ERR_NODIAG, "sparse matrix assembly failed to produce a diagonal entry");
+    case pp::kBicgRho: return fail(ctx, report, PP_ERR_BICG_RHO, "bicgstab breakdown: invalid rho");
+    case pp::kBicgRhatV: return fail(ctx, report, PP_ERR_BICG_RHAT_V, "bicgstab breakdown: invalid r_hat^T A M^{-1} p");
+    case pp::kBicgTT: return fail(ctx, report, PP_ERR_BICG_TT, "bicgstab breakdown: invalid t^T t");
+    case pp::kBicgOmega: return fail(ctx, report, PP_ERR_BICG_OMEGA, "bicgstab breakdown: invalid omega");
+    default: return fail(ctx, report, PP_ERR_HIP, "bicgstab: the device state did not reach a stop");
+  }
 }
 
 int validate_bicg(pp_ctx *ctx, int n, const void *rp, const void *ci, const void *val, const void *rhs, const void *x,

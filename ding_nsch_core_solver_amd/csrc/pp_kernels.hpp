@@ -120,20 +120,37 @@ hipError_t launch_velocity_correction(const CorrArgs *d_args, int nx, int ny, in
 hipError_t launch_pressure_ghosts(const CorrArgs *d_args, double *d_field, hipStream_t stream);
 
 // BiCGSTAB with the diagonal preconditioner on a CSR system (the momentum predictor's solve).
-// Workgroup c of `chunks` owns rows [kBicgRows c, kBicgRows (c + 1)) and leaves its dot-product
-// partials at partials[k * chunks + c].
+// Workgroup c of `chunks` owns rows [kBicgRows c, kBicgRows (c + 1)); each pass leaves its dot
+// products as per-workgroup partials (partials[k * chunks + c]) that the next pass sums, and the
+// scalar recurrence and stopping tests run on the device against the state block.
 constexpr int kBicgRows = 512;
+enum BicgStatus : int {
+  kBicgRunning = 0, kBicgDone = 1, kBicgNoDiag = 2, kBicgRho = 3, kBicgRhatV = 4, kBicgTT = 5, kBicgOmega = 6
+};
+struct BicgState {
+  int status;      // BicgStatus
+  int half_exit;   // converged at the half step; the full-step pass finishes the report
+  int iterations;  // report (LinearSolveReport)
+  int pad;
+  double rho[2];   // rho of the iterations of each parity
+  double alpha, omega;
+  double abs_res, rel_res, iter_res;
+  double rhs_norm;  // |b|, from the first iteration's pass on
+};
 struct BicgArgs {
-  int n, chunks;
+  int n, chunks, use_rel;
+  double tolerance;
   const int *row_ptr, *col_idx;
   const double *values, *rhs;
   double *x;
   double *r, *r_hat, *p, *v, *p_hat, *s, *s_hat, *t, *update, *dinv;  // n each
-  double *partials;  // 4 * chunks
-  int *flag;         // set when a row has no diagonal entry
+  double *p_setup, *p_v, *p_s, *p_t, *p_x, *p_exit;                   // 2, 1, 1, 2, 4, 2 x chunks
+  BicgState *state;                                                    // zeroed before the setup pass
 };
-enum class BicgStep { kSetup, kDirection, kSpmvV, kHalfStep, kHalfExit, kSpmvT, kFullStep };
-// One pass; s0, s1: beta and omega (kDirection), alpha (kHalfStep), omega (kFullStep).
-hipError_t launch_bicg(BicgStep step, const BicgArgs &a, double s0, double s1, hipStream_t stream);
+// The setup pass; the five passes of iteration iter; the report after `iterations` iterations
+// when the limit ended the solve.
+hipError_t launch_bicg_setup(const BicgArgs &a, hipStream_t stream);
+hipError_t launch_bicg_iteration(const BicgArgs &a, int iter, hipStream_t stream);
+hipError_t launch_bicg_report(const BicgArgs &a, int iterations, hipStream_t stream);
 
 }  // namespace pp
