@@ -496,12 +496,6 @@ int ensure_fields(pp_ctx *ctx, size_t need) {
   pp_report *report = nullptr;
   if (need <= ctx->field_cap) return PP_OK;
   if (ctx->field_stage) (void)hipFree(ctx->field_stage);
-  if (ctx->bicg_work) (void)hipFree(ctx->bicg_work);
-  if (ctx->bicg_partials) (void)hipFree(ctx->bicg_partials);
-  if (ctx->bicg_state) (void)hipFree(ctx->bicg_state);
-  if (ctx->h_bicg) (void)hipHostFree(ctx->h_bicg);
-  if (ctx->bicg_stage_idx) (void)hipFree(ctx->bicg_stage_idx);
-  if (ctx->bicg_stage_val) (void)hipFree(ctx->bicg_stage_val);
   ctx->field_stage = nullptr;
   ctx->field_cap = 0;
   PP_HIP_CHECK(hipMalloc(&ctx->field_stage, need * sizeof(double)));
@@ -699,6 +693,12 @@ void pp_destroy(pp_ctx *ctx) {
   if (ctx->corr_work) (void)hipFree(ctx->corr_work);
   if (ctx->monitor_log) (void)hipFree(ctx->monitor_log);
   if (ctx->field_stage) (void)hipFree(ctx->field_stage);
+  if (ctx->bicg_work) (void)hipFree(ctx->bicg_work);
+  if (ctx->bicg_partials) (void)hipFree(ctx->bicg_partials);
+  if (ctx->bicg_state) (void)hipFree(ctx->bicg_state);
+  if (ctx->h_bicg) (void)hipHostFree(ctx->h_bicg);
+  if (ctx->bicg_stage_idx) (void)hipFree(ctx->bicg_stage_idx);
+  if (ctx->bicg_stage_val) (void)hipFree(ctx->bicg_stage_val);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

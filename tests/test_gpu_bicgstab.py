@@ -82,8 +82,9 @@ def _dense(a, b):
 ])
 def test_breakdowns(engine, oracle_lib, a, b, kw, status, text):
     s = _dense(a, b)
-    with pytest.raises(RuntimeError, match=""):
+    with pytest.raises(RuntimeError) as eo:
         oracle_lib.bicgstab(s, **kw)
+    assert str(eo.value) == text
     with pytest.raises(E.EngineError) as e:
         engine.bicgstab(s, **kw)
     assert e.value.status == status and str(e.value) == text
@@ -112,3 +113,23 @@ def test_device_entry_192(engine, oracle_lib):
         _check_equal(x.cpu().numpy(), rep, expect)
     print(f"[bicgstab] 192x192 momentum system: {rep['iterations']} iterations, {rep['kernel_ms']:.3f} ms",
           flush=True)
+
+
+def test_workspace_survives_other_entries(oracle_lib):
+    """BiCGSTAB, then the host-buffer velocity correction (which grows the context's field staging),
+    then BiCGSTAB again on one fresh context: both solves equal the oracle (the entries keep their
+    own device buffers)."""
+    from tests.test_velocity_oracle import fields
+    eng = E.Engine(0)
+    try:
+        s = W.momentum_system(40, 36, seed=8)
+        expect = oracle_lib.bicgstab(s, max_iterations=50, tolerance=1e-8, reduction=oracle_lib.CHUNKED)
+        x, rep = eng.bicgstab(s)
+        _check_equal(x, rep, expect)
+        for shape in ((8, 6), (70, 20)):
+            case = W.droplet_case(*shape, seed=1, density_ratio=7.0, step_index=1)
+            eng.velocity_correction(case, W.VelocityBC(), scheme=E.PP_ICPCG, ghost=3, **fields(case, seed=3))
+        x, rep = eng.bicgstab(s)
+        _check_equal(x, rep, expect)
+    finally:
+        eng.close()
