@@ -92,6 +92,21 @@ __device__ __forceinline__ void dot_sums(const double *partials, int chunks, dou
   for (int k = 0; k < K; ++k) out[k] = sums[k];
 }
 
+// The state's status and half-exit flag as one value per workgroup: thread 0 reads them and the
+// barrier hands them to every wave, so a workgroup never splits on a stop another one records.
+struct StateFlags {
+  int status, half_exit;
+};
+__device__ __forceinline__ StateFlags block_flags(const BicgArgs &a) {
+  __shared__ int flags[2];
+  if (threadIdx.x == 0) {
+    flags[0] = a.state->status;
+    flags[1] = a.state->half_exit;
+  }
+  __syncthreads();
+  return {flags[0], flags[1]};
+}
+
 __device__ __forceinline__ double csr_row(const int *__restrict__ rp, const int *__restrict__ ci,
                                           const double *__restrict__ val, const double *__restrict__ x, int row) {
   double sum = 0.0;
@@ -121,7 +136,8 @@ __device__ __forceinline__ void record(const BicgArgs &a, int status, int iterat
 }
 
 // Diagonal preconditioner (1 / max(|a_ii|, 1e-30), the diagonal entry located as
-// finalize_row_maps records it), r = b - A x, r_hat = r, p = v = 0; partials b.b, r.r.
+// finalize_row_maps records it), r = b - A x, r_hat = r, p = v = 0 (both parities); partials b.b,
+// r.r.
 __global__ void __launch_bounds__(kRows) bicg_setup_kernel(BicgArgs a) {
   const int q = blockIdx.x * kRows + threadIdx.x;
   double v[2] = {0.0, 0.0};
@@ -141,21 +157,43 @@ __global__ void __launch_bounds__(kRows) bicg_setup_kernel(BicgArgs a) {
     const double r = b - ax;
     a.r[q] = r;
     a.r_hat[q] = r;
-    a.p[q] = 0.0;
-    a.v[q] = 0.0;
+    a.p[0][q] = 0.0;
+    a.p[1][q] = 0.0;
+    a.v[0][q] = 0.0;
+    a.v[1][q] = 0.0;
     v[0] = v[0] + b * b;
     v[1] = v[1] + r * r;
   }
   chunk_partials<2>(v, a.p_setup, a.chunks);
 }
 
-// Start of iteration `iter` (or the report after the last one, final): the stopping test of the
-// previous iteration (of the initial residual at iter 0), the rho check, then
-// p = r + beta (p - omega v); p_hat = M^-1 p.
-__global__ void __launch_bounds__(kRows) bicg_direction_kernel(BicgArgs a, int iter, int final) {
-  if (a.state->status != kBicgRunning) return;
+// Pass A of iteration k (or the report after the last iteration, final): the stop test of the
+// previous iteration (half-step exit, full-step convergence, or the initial residual at k = 0),
+// the rho check, then p = r + beta (p_old - omega v_old), p_hat = M^-1 p and v = A p_hat; partial
+// r_hat.v. A row's neighbours' p_hat are recomputed from r, p_old, v_old and M^-1 with the owner's
+// expression (same bits), so p and v are double-buffered by iteration parity.
+__global__ void __launch_bounds__(kRows) bicg_pass_a_kernel(BicgArgs a, int k, int final) {
+  // loads that need no scalar are issued first, so their latency overlaps the partial sums
+  const double *__restrict__ p_old = a.p[(k + 1) & 1];
+  const double *__restrict__ v_old = a.v[(k + 1) & 1];
+  const double *__restrict__ r = a.r;
+  const double *__restrict__ dinv = a.dinv;
+  const int q = blockIdx.x * kRows + threadIdx.x;
+  int beg = 0, end = 0;
+  double rq = 0.0, pq_old = 0.0, vq_old = 0.0, dq = 0.0, rhq = 0.0;
+  if (q < a.n) {
+    beg = a.row_ptr[q];
+    end = a.row_ptr[q + 1];
+    rq = r[q];
+    pq_old = p_old[q];
+    vq_old = v_old[q];
+    dq = dinv[q];
+    rhq = a.r_hat[q];
+  }
+  const StateFlags fl = block_flags(a);
+  if (fl.status != kBicgRunning) return;
   double bn, rho_new;
-  if (iter == 0) {
+  if (k == 0) {
     double rr[2];
     dot_sums<2>(a.p_setup, a.chunks, rr);  // b.b, r.r
     bn = sqrt(max0(rr[0]));
@@ -167,87 +205,125 @@ __global__ void __launch_bounds__(kRows) bicg_direction_kernel(BicgArgs a, int i
       return;
     }
     rho_new = rr[1];  // r_hat = r: r_hat . r is r . r, summed in the same order
+  } else if (fl.half_exit) {
+    double d[2];
+    dot_sums<2>(a.p_exit, a.chunks, d);  // update.update, x.x after x += update
+    record(a, kBicgDone, k, a.state->abs_res, a.state->rel_res, sqrt(d[0] / floor30(d[1])), true);
+    return;
   } else {
     bn = a.state->rhs_norm;
     double d[4];
-    dot_sums<4>(a.p_x, a.chunks, d);  // update.update, x.x, r.r, r_hat.r of iteration iter - 1
+    dot_sums<4>(a.p_x, a.chunks, d);  // update.update, x.x, r.r, r_hat.r of iteration k - 1
     const double iter_res = sqrt(d[0] / floor30(d[1]));
     const double abs_res = sqrt(max0(d[2]));
     const double rel = abs_res / floor30(bn);
     if (converged(a, rel, abs_res) || final) {
-      record(a, kBicgDone, iter, abs_res, rel, iter_res, true);
+      record(a, kBicgDone, k, abs_res, rel, iter_res, true);
       return;
     }
     rho_new = d[3];
   }
   if (!isfinite(rho_new) || fabs(rho_new) < 1.0e-30) {
-    record(a, kBicgRho, iter, 0.0, 0.0, 0.0, false);
+    record(a, kBicgRho, k, 0.0, 0.0, 0.0, false);
     return;
   }
-  const double rho_old = iter == 0 ? 1.0 : a.state->rho[(iter - 1) & 1];
-  const double beta = (iter == 0) ? 0.0 : (rho_new / rho_old) * (a.state->alpha / a.state->omega);
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.state->rho[iter & 1] = rho_new;
-  const int q = blockIdx.x * kRows + threadIdx.x;
-  if (q >= a.n) return;
-  const double omega = iter == 0 ? 1.0 : a.state->omega;
-  const double p = a.r[q] + beta * (a.p[q] - omega * a.v[q]);
-  a.p[q] = p;
-  a.p_hat[q] = a.dinv[q] * p;
-}
-
-// v = A p_hat; partial r_hat.v
-__global__ void __launch_bounds__(kRows) bicg_spmv_v_kernel(BicgArgs a) {
-  if (a.state->status != kBicgRunning) return;
-  const int q = blockIdx.x * kRows + threadIdx.x;
+  const double rho_old = k == 0 ? 1.0 : a.state->rho[(k - 1) & 1];
+  const double alpha = k == 0 ? 1.0 : a.state->alpha;
+  const double omega = k == 0 ? 1.0 : a.state->omega;
+  const double beta = (k == 0) ? 0.0 : (rho_new / rho_old) * (alpha / omega);
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.state->rho[k & 1] = rho_new;
   double v[1] = {0.0};
   if (q < a.n) {
-    const double y = csr_row(a.row_ptr, a.col_idx, a.values, a.p_hat, q);
-    a.v[q] = y;
-    v[0] = v[0] + a.r_hat[q] * y;
+    const double pq = rq + beta * (pq_old - omega * vq_old);
+    a.p[k & 1][q] = pq;
+    a.p_hat[q] = dq * pq;
+    double sum = 0.0;
+    for (int pos = beg; pos < end; ++pos) {
+      const int j = a.col_idx[pos];
+      const double pj = r[j] + beta * (p_old[j] - omega * v_old[j]);
+      sum += a.values[pos] * (dinv[j] * pj);
+    }
+    a.v[k & 1][q] = sum;
+    v[0] = v[0] + rhq * sum;
   }
   chunk_partials<1>(v, a.p_v, a.chunks);
 }
 
-// alpha = rho / r_hat.v (with its check); s = r - alpha v; update = alpha p_hat; s_hat = M^-1 s;
-// partial s.s
-__global__ void __launch_bounds__(kRows) bicg_half_step_kernel(BicgArgs a, int iter) {
-  if (a.state->status != kBicgRunning) return;
+// Pass B: alpha = rho / r_hat.v (with its check); s = r - alpha v; update = alpha p_hat;
+// s_hat = M^-1 s; t = A s_hat (neighbours' s_hat recomputed from r, v, M^-1); partials s.s, t.t,
+// t.s. t is formed before the half-step test, which the next pass makes; on a half exit it goes
+// unused.
+__global__ void __launch_bounds__(kRows) bicg_pass_b_kernel(BicgArgs a, int k) {
+  const double *__restrict__ vk = a.v[k & 1];
+  const double *__restrict__ r = a.r;
+  const double *__restrict__ dinv = a.dinv;
+  const int q = blockIdx.x * kRows + threadIdx.x;
+  int beg = 0, end = 0;
+  double rq = 0.0, vq = 0.0, dq = 0.0, phq = 0.0;
+  if (q < a.n) {  // issued before the partial sums, as in pass A
+    beg = a.row_ptr[q];
+    end = a.row_ptr[q + 1];
+    rq = r[q];
+    vq = vk[q];
+    dq = dinv[q];
+    phq = a.p_hat[q];
+  }
+  const StateFlags fl = block_flags(a);
+  if (fl.status != kBicgRunning) return;
   double d[1];
   dot_sums<1>(a.p_v, a.chunks, d);
   const double rhat_v = d[0];
   if (!isfinite(rhat_v) || fabs(rhat_v) < 1.0e-30) {
-    record(a, kBicgRhatV, iter, 0.0, 0.0, 0.0, false);
+    record(a, kBicgRhatV, k, 0.0, 0.0, 0.0, false);
     return;
   }
-  const double alpha = a.state->rho[iter & 1] / rhat_v;
+  const double alpha = a.state->rho[k & 1] / rhat_v;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.state->alpha = alpha;
-  const int q = blockIdx.x * kRows + threadIdx.x;
-  double v[1] = {0.0};
+  double v[3] = {0.0, 0.0, 0.0};
   if (q < a.n) {
-    const double s = a.r[q] - alpha * a.v[q];
-    a.s[q] = s;
-    a.update[q] = alpha * a.p_hat[q];
-    a.s_hat[q] = a.dinv[q] * s;
-    v[0] = v[0] + s * s;
+    const double sq = rq - alpha * vq;
+    a.s[q] = sq;
+    a.update[q] = alpha * phq;
+    a.s_hat[q] = dq * sq;
+    double sum = 0.0;
+    for (int pos = beg; pos < end; ++pos) {
+      const int j = a.col_idx[pos];
+      const double sj = r[j] - alpha * vk[j];
+      sum += a.values[pos] * (dinv[j] * sj);
+    }
+    a.t[q] = sum;
+    v[0] = v[0] + sq * sq;
+    v[1] = v[1] + sum * sum;
+    v[2] = v[2] + sum * sq;
   }
-  chunk_partials<1>(v, a.p_s, a.chunks);
+  chunk_partials<3>(v, a.p_b, a.chunks);
 }
 
-// The half-step test on |s|: converged -> x += update (partials update.update, x.x; workgroup 0
-// marks the half exit and the full-step pass records the report); else t = A s_hat (partials t.t,
-// t.s).
-__global__ void __launch_bounds__(kRows) bicg_spmv_t_kernel(BicgArgs a, int iter) {
-  if (a.state->status != kBicgRunning) return;
-  const double bn = a.state->rhs_norm;
-  double d[1];
-  dot_sums<1>(a.p_s, a.chunks, d);
-  const double s_norm = sqrt(max0(d[0]));
+// Pass C: the half-step test on |s| (converged: x += update, partials update.update, x.x, and
+// workgroup 0 marks the half exit for the next pass A); else omega = t.s / t.t with both checks,
+// update += omega s_hat; x += update; r = s - omega t; partials update.update, x.x, r.r, r_hat.r.
+__global__ void __launch_bounds__(kRows) bicg_pass_c_kernel(BicgArgs a, int k) {
   const int q = blockIdx.x * kRows + threadIdx.x;
+  double uq = 0.0, xq = 0.0, shq = 0.0, sq = 0.0, tq = 0.0, rhq = 0.0;
+  if (q < a.n) {  // issued before the partial sums, as in pass A
+    uq = a.update[q];
+    xq = a.x[q];
+    shq = a.s_hat[q];
+    sq = a.s[q];
+    tq = a.t[q];
+    rhq = a.r_hat[q];
+  }
+  const StateFlags fl = block_flags(a);
+  if (fl.status != kBicgRunning) return;
+  const double bn = a.state->rhs_norm;
+  double d[3];
+  dot_sums<3>(a.p_b, a.chunks, d);  // s.s, t.t, t.s
+  const double s_norm = sqrt(max0(d[0]));
   if (converged(a, s_norm / floor30(bn), s_norm)) {
     double v[2] = {0.0, 0.0};
     if (q < a.n) {
-      const double u = a.update[q];
-      const double x = a.x[q] + u;
+      const double u = uq;
+      const double x = xq + u;
       a.x[q] = x;
       v[0] = v[0] + u * u;
       v[1] = v[1] + x * x;
@@ -260,52 +336,29 @@ __global__ void __launch_bounds__(kRows) bicg_spmv_t_kernel(BicgArgs a, int iter
     }
     return;
   }
-  double v[2] = {0.0, 0.0};
-  if (q < a.n) {
-    const double t = csr_row(a.row_ptr, a.col_idx, a.values, a.s_hat, q);
-    a.t[q] = t;
-    v[0] = v[0] + t * t;
-    v[1] = v[1] + t * a.s[q];
-  }
-  chunk_partials<2>(v, a.p_t, a.chunks);
-}
-
-// The half-step exit's report, or omega = t.s / t.t (with both checks);
-// update += omega s_hat; x += update; r = s - omega t; partials update.update, x.x, r.r, r_hat.r
-__global__ void __launch_bounds__(kRows) bicg_full_step_kernel(BicgArgs a, int iter) {
-  if (a.state->status != kBicgRunning) return;
-  if (a.state->half_exit) {
-    double d[2];
-    dot_sums<2>(a.p_exit, a.chunks, d);
-    record(a, kBicgDone, iter + 1, a.state->abs_res, a.state->rel_res, sqrt(d[0] / floor30(d[1])), true);
-    return;
-  }
-  double d[2];
-  dot_sums<2>(a.p_t, a.chunks, d);
-  const double tt = d[0];
+  const double tt = d[1];
   if (!isfinite(tt) || fabs(tt) < 1.0e-30) {
-    record(a, kBicgTT, iter, 0.0, 0.0, 0.0, false);
+    record(a, kBicgTT, k, 0.0, 0.0, 0.0, false);
     return;
   }
-  const double omega = d[1] / tt;
+  const double omega = d[2] / tt;
   if (!isfinite(omega) || fabs(omega) < 1.0e-30) {
-    record(a, kBicgOmega, iter, 0.0, 0.0, 0.0, false);
+    record(a, kBicgOmega, k, 0.0, 0.0, 0.0, false);
     return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.state->omega = omega;
-  const int q = blockIdx.x * kRows + threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (q < a.n) {
-    const double u = a.update[q] + omega * a.s_hat[q];
+    const double u = uq + omega * shq;
     a.update[q] = u;
-    const double x = a.x[q] + u;
+    const double x = xq + u;
     a.x[q] = x;
-    const double r = a.s[q] - omega * a.t[q];
+    const double r = sq - omega * tq;
     a.r[q] = r;
     v[0] = v[0] + u * u;
     v[1] = v[1] + x * x;
     v[2] = v[2] + r * r;
-    v[3] = v[3] + a.r_hat[q] * r;
+    v[3] = v[3] + rhq * r;
   }
   chunk_partials<4>(v, a.p_x, a.chunks);
 }
@@ -319,16 +372,14 @@ hipError_t launch_bicg_setup(const BicgArgs &a, hipStream_t stream) {
 
 hipError_t launch_bicg_iteration(const BicgArgs &a, int iter, hipStream_t stream) {
   const dim3 grid(a.chunks), block(kRows);
-  bicg_direction_kernel<<<grid, block, 0, stream>>>(a, iter, 0);
-  bicg_spmv_v_kernel<<<grid, block, 0, stream>>>(a);
-  bicg_half_step_kernel<<<grid, block, 0, stream>>>(a, iter);
-  bicg_spmv_t_kernel<<<grid, block, 0, stream>>>(a, iter);
-  bicg_full_step_kernel<<<grid, block, 0, stream>>>(a, iter);
+  bicg_pass_a_kernel<<<grid, block, 0, stream>>>(a, iter, 0);
+  bicg_pass_b_kernel<<<grid, block, 0, stream>>>(a, iter);
+  bicg_pass_c_kernel<<<grid, block, 0, stream>>>(a, iter);
   return hipGetLastError();
 }
 
 hipError_t launch_bicg_report(const BicgArgs &a, int iterations, hipStream_t stream) {
-  bicg_direction_kernel<<<dim3(a.chunks), dim3(kRows), 0, stream>>>(a, iterations, 1);
+  bicg_pass_a_kernel<<<dim3(a.chunks), dim3(kRows), 0, stream>>>(a, iterations, 1);
   return hipGetLastError();
 }
 

@@ -62,7 +62,7 @@ struct pp_ctx {
   pp::CorrArgs *d_corr_args = nullptr;
   double *corr_work = nullptr;  // 4 doubles: the two boundary speeds, the pressure mean
 
-  // BiCGSTAB (pp_bicgstab*): 10 work vectors of bicg_cap rows, per-workgroup partials, the
+  // BiCGSTAB (pp_bicgstab*): 12 work vectors of bicg_cap rows, per-workgroup partials, the
   // device state block and its pinned copy, and the host entry's CSR staging
   size_t bicg_cap = 0;
   double *bicg_work = nullptr;
@@ -1073,14 +1073,14 @@ int ensure_bicg(pp_ctx *ctx, size_t n, pp_report *report) {
   ctx->bicg_partials = nullptr;
   ctx->bicg_cap = 0;
   const size_t chunks = (n + pp::kBicgRows - 1) / pp::kBicgRows;
-  PP_HIP_CHECK(hipMalloc(&ctx->bicg_work, 10 * n * sizeof(double)));
+  PP_HIP_CHECK(hipMalloc(&ctx->bicg_work, 12 * n * sizeof(double)));
   PP_HIP_CHECK(hipMalloc(&ctx->bicg_partials, 12 * chunks * sizeof(double)));
   ctx->bicg_cap = n;
   return PP_OK;
 }
 
 // Iterations queued between two reads of the device state: a solve that stops at iteration k
-// leaves at most this many iterations of passes that return at once.
+// leaves fewer than this many iterations of passes that return at once.
 constexpr int kBicgQueue = 4;
 
 // solve_preconditioned_bicgstab (ch_sparse_krylov.hpp:839-963). The device passes run the
@@ -1099,14 +1099,14 @@ int run_bicgstab(pp_ctx *ctx, int n, const int *rp, const int *ci, const double 
   a.values = val;
   a.rhs = rhs;
   a.x = x;
-  double **vec[10] = {&a.r, &a.r_hat, &a.p, &a.v, &a.p_hat, &a.s, &a.s_hat, &a.t, &a.update, &a.dinv};
-  for (int k = 0; k < 10; ++k) *vec[k] = ctx->bicg_work + static_cast<size_t>(k) * n;
+  double **vec[12] = {&a.r, &a.r_hat, &a.p[0], &a.p[1], &a.v[0], &a.v[1], &a.p_hat, &a.s, &a.s_hat, &a.t,
+                      &a.update, &a.dinv};
+  for (int k = 0; k < 12; ++k) *vec[k] = ctx->bicg_work + static_cast<size_t>(k) * n;
   double *part = ctx->bicg_partials;
   const size_t c = static_cast<size_t>(a.chunks);
   a.p_setup = part;
   a.p_v = part + 2 * c;
-  a.p_s = part + 3 * c;
-  a.p_t = part + 4 * c;
+  a.p_b = part + 3 * c;
   a.p_x = part + 6 * c;
   a.p_exit = part + 10 * c;
   a.state = ctx->bicg_state;
@@ -1173,8 +1173,11 @@ int pp_bicgstab_device(pp_ctx *ctx, int n, const int *d_row_ptr, const int *d_co
 int pp_bicgstab(pp_ctx *ctx, int n, const int *row_ptr, const int *col_idx, const double *values, const double *rhs,
                 double *x, int max_iterations, double tolerance, int use_relative_tolerance, pp_report *report) {
   if (int rc = validate_bicg(ctx, n, row_ptr, col_idx, values, rhs, x, report)) return rc;
-  if (row_ptr[0] != 0 || row_ptr[n] < n)
-    return fail(ctx, report, PP_ERR_ARG, "row_ptr must start at 0 and hold at least one entry per row");
+  if (row_ptr[0] != 0) return fail(ctx, report, PP_ERR_ARG, "row_ptr must start at 0");
+  for (int r = 0; r < n; ++r)
+    if (row_ptr[r + 1] < row_ptr[r]) return fail(ctx, report, PP_ERR_ARG, "row_ptr must not decrease");
+  for (int pos = 0; pos < row_ptr[n]; ++pos)
+    if (col_idx[pos] < 0 || col_idx[pos] >= n) return fail(ctx, report, PP_ERR_ARG, "column index out of range");
   PP_HIP_CHECK(hipSetDevice(ctx->device));
   const size_t N = static_cast<size_t>(n), nnz = static_cast<size_t>(row_ptr[n]);
   if (N > ctx->bicg_stage_rows || nnz > ctx->bicg_stage_nnz) {

@@ -143,11 +143,11 @@ struct BicgArgs {
   const int *row_ptr, *col_idx;
   const double *values, *rhs;
   double *x;
-  double *r, *r_hat, *p, *v, *p_hat, *s, *s_hat, *t, *update, *dinv;  // n each
-  double *p_setup, *p_v, *p_s, *p_t, *p_x, *p_exit;                   // 2, 1, 1, 2, 4, 2 x chunks
+  double *r, *r_hat, *p[2], *v[2], *p_hat, *s, *s_hat, *t, *update, *dinv;  // n each; p, v by parity
+  double *p_setup, *p_v, *p_b, *p_x, *p_exit;                               // 2, 1, 3, 4, 2 x chunks
   BicgState *state;                                                    // zeroed before the setup pass
 };
-// The setup pass; the five passes of iteration iter; the report after `iterations` iterations
+// The setup pass; the three passes of iteration iter; the report after `iterations` iterations
 // when the limit ended the solve.
 hipError_t launch_bicg_setup(const BicgArgs &a, hipStream_t stream);
 hipError_t launch_bicg_iteration(const BicgArgs &a, int iter, hipStream_t stream);

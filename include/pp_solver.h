@@ -294,7 +294,9 @@ int pp_pressure_step(pp_ctx *ctx, const pp_problem *pb, const pp_velocity_bc *vb
  * reference's texts: PP_ERR_ARG "cannot solve an empty linear system" (n <= 0), PP_ERR_NODIAG,
  * PP_ERR_BICG_*; x then holds the iterate at the breakdown, as the reference's vector does.
  * Dot products are summed in 512-row chunks (differing from the reference's sequential sums by
- * rounding only); everything else is the reference's arithmetic term for term.
+ * rounding only); everything else is the reference's arithmetic term for term. pp_bicgstab checks
+ * row_ptr and the column indices (PP_ERR_ARG); pp_bicgstab_device trusts them (every col_idx in
+ * [0, n), row_ptr non-decreasing from 0).
  */
 int pp_bicgstab_device(pp_ctx *ctx, int n, const int *d_row_ptr, const int *d_col_idx, const double *d_values,
                        const double *d_rhs, double *d_x, int max_iterations, double tolerance,

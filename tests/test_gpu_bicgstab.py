@@ -133,3 +133,11 @@ def test_workspace_survives_other_entries(oracle_lib):
         _check_equal(x, rep, expect)
     finally:
         eng.close()
+
+
+def test_host_entry_rejects_bad_csr(engine):
+    s = _dense([[2.0, 1.0], [1.0, 3.0]], [1.0, 1.0])
+    s.col_idx = s.col_idx.copy()
+    s.col_idx[1] = 5
+    with pytest.raises(E.EngineError, match="column index out of range"):
+        engine.bicgstab(s)
