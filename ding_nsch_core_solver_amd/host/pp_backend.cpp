@@ -224,4 +224,26 @@ double PressureBackend::pressure_step(const std::string &scheme, const PressureS
   return report_.relative_residual;
 }
 
+LinearSolveReport PressureBackend::solve_bicgstab(const SparseMatrixCSR &matrix, const std::vector<double> &rhs,
+                                                  int max_iterations, double tolerance, std::vector<double> &x,
+                                                  bool use_relative_tolerance) {
+  if (matrix.n <= 0) throw std::runtime_error("cannot solve an empty linear system");
+  if (static_cast<int>(rhs.size()) != matrix.n) throw std::runtime_error("bicgstab rhs size mismatch");
+  if (static_cast<int>(matrix.row_ptr.size()) != matrix.n + 1 ||
+      matrix.col_idx.size() < static_cast<size_t>(matrix.row_ptr.back()) ||
+      matrix.values.size() < static_cast<size_t>(matrix.row_ptr.back()))
+    throw std::runtime_error("bicgstab: CSR arrays do not match n");
+  if (static_cast<int>(x.size()) != matrix.n) x.assign(static_cast<size_t>(matrix.n), 0.0);
+  const int rc = pp_bicgstab(ctx_, matrix.n, matrix.row_ptr.data(), matrix.col_idx.data(), matrix.values.data(),
+                             rhs.data(), x.data(), max_iterations, tolerance, use_relative_tolerance ? 1 : 0, &report_);
+  if (rc != PP_OK) throw std::runtime_error(report_.message[0] ? report_.message : pp_last_error(ctx_));
+  last_iterations_ = report_.iterations;
+  LinearSolveReport out;
+  out.iterations = report_.iterations;
+  out.absolute_residual = report_.absolute_residual;
+  out.relative_residual = report_.relative_residual;
+  out.iterate_residual = report_.iterate_residual;
+  return out;
+}
+
 }  // namespace ppamd

@@ -12,6 +12,7 @@
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "pp_solver.h"
 
@@ -67,6 +68,22 @@ void apply_pressure_bc(const PressureSettings &s, const FieldView &f);
 // (the scalar part of the reference's CGMonitor, ch_sparse_krylov.hpp:47-61).
 using CGMonitor = std::function<void(const pp_monitor_record &)>;
 
+// The reference's CSR system and solve report as the momentum predictor holds them
+// (ch_sparse_krylov.hpp:12-18, 40-45). diag_pos is not read: the engine locates the diagonal.
+struct SparseMatrixCSR {
+  int n = 0;
+  std::vector<int> row_ptr;
+  std::vector<int> col_idx;
+  std::vector<int> diag_pos;
+  std::vector<double> values;
+};
+struct LinearSolveReport {
+  int iterations = 0;
+  double absolute_residual = 0.0;
+  double relative_residual = 0.0;
+  double iterate_residual = 0.0;
+};
+
 class PressureBackend {
  public:
   explicit PressureBackend(int device = 0);
@@ -105,6 +122,14 @@ class PressureBackend {
                        const FieldView &rho_mid, const FieldView &u_star, const FieldView &v_star, int step_index,
                        const FieldView &pressure_correction, const FieldView &u, const FieldView &v,
                        const FieldView &pressure, const FieldView &pressure_previous);
+
+  // The momentum predictor's solve (momentum.cpp:469-472): build_diagonal_preconditioner +
+  // solve_preconditioned_bicgstab (ch_sparse_krylov.hpp:421-431, 839-963) on the engine, with the
+  // reference's arguments, report and errors (std::runtime_error with its texts). x is the
+  // initial guess, reset to n zeros when its size differs (as the reference does), then the
+  // solution.
+  LinearSolveReport solve_bicgstab(const SparseMatrixCSR &matrix, const std::vector<double> &rhs, int max_iterations,
+                                   double tolerance, std::vector<double> &x, bool use_relative_tolerance = true);
 
   int last_iterations() const { return last_iterations_; }
   const pp_report &last_report() const { return report_; }

@@ -183,5 +183,30 @@ int main(int argc, char **argv) {
     }
     return 0;
   }
+  if (mode == "bicgstab" && argc == 4) {
+    // input: n, nnz, max_iterations, tolerance, use_relative, row_ptr[n+1], col_idx[nnz], values[nnz],
+    // rhs[n], x0[n] (all as doubles); output: x
+    const std::vector<double> in = read_all(argv[2]);
+    const int n = static_cast<int>(in[0]), nnz = static_cast<int>(in[1]);
+    ppamd::SparseMatrixCSR m;
+    m.n = n;
+    size_t o = 5;
+    for (int k = 0; k <= n; ++k) m.row_ptr.push_back(static_cast<int>(in[o++]));
+    for (int k = 0; k < nnz; ++k) m.col_idx.push_back(static_cast<int>(in[o++]));
+    m.values.assign(in.begin() + o, in.begin() + o + nnz);
+    o += nnz;
+    const std::vector<double> rhs(in.begin() + o, in.begin() + o + n);
+    std::vector<double> x(in.begin() + o + n, in.begin() + o + 2 * n);
+    try {
+      ppamd::PressureBackend backend(0);
+      const ppamd::LinearSolveReport r = backend.solve_bicgstab(m, rhs, static_cast<int>(in[2]), in[3], x, in[4] != 0.0);
+      write_all(argv[3], x);
+      std::printf("%d %.17g %.17g %.17g\n", r.iterations, r.absolute_residual, r.relative_residual, r.iterate_residual);
+    } catch (const std::exception &e) {
+      std::printf("error %s\n", e.what());
+      return 1;
+    }
+    return 0;
+  }
   return 2;
 }

@@ -192,3 +192,35 @@ def test_pressure_step_through_cpp_mirror(driver, engine, tmp_path, scheme, code
     assert int(it) == rep["iterations"] and float(rel) == rep["relative_residual"]
     assert (float(pre), float(post)) == speeds
     assert np.array_equal(np.fromfile(outp), np.concatenate([p_corr, u, v, p, prev]))
+
+
+def _bicg_input(path, s, max_it, tol, rel):
+    x0 = np.zeros(s.n)
+    buf = np.concatenate([[s.n, len(s.values), max_it, tol, float(rel)], s.row_ptr, s.col_idx, s.values, s.rhs, x0])
+    buf.astype(np.float64).tofile(path)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rel", [False, True])
+def test_bicgstab_through_cpp_mirror(driver, oracle_lib, tmp_path, rel):
+    """PressureBackend::solve_bicgstab (the reference's solve_preconditioned_bicgstab signature) on
+    a momentum-shaped system: report and x bit for bit equal to the oracle in the engine's order."""
+    s = W.momentum_system(48, 40, seed=3)
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    _bicg_input(inp, s, 50, 1e-9, rel)
+    res = subprocess.run([driver, "bicgstab", str(inp), str(outp)], check=True, capture_output=True, text=True)
+    it, ab, rr, itr = res.stdout.split()
+    xo, ito, abo, relo, itro = oracle_lib.bicgstab(s, max_iterations=50, tolerance=1e-9, use_relative=rel,
+                                                   reduction=oracle_lib.CHUNKED)
+    assert int(it) == ito and float(ab) == abo and float(rr) == relo and float(itr) == itro
+    assert np.array_equal(np.fromfile(outp), xo)
+
+
+@pytest.mark.gpu
+def test_bicgstab_mirror_breakdown_text(driver, tmp_path):
+    s = W.CsrSystem(np.array([0, 2, 4], dtype=np.int32), np.array([0, 1, 0, 1], dtype=np.int32),
+                    np.array([2.0, 1.0, 1.0, 3.0]), np.array([np.nan, 1.0]))
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    _bicg_input(inp, s, 50, 1e-8, False)
+    res = subprocess.run([driver, "bicgstab", str(inp), str(outp)], capture_output=True, text=True)
+    assert res.returncode == 1 and res.stdout.strip() == "error bicgstab breakdown: invalid rho"
