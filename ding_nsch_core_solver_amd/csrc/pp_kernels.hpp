@@ -129,7 +129,7 @@ enum BicgStatus : int {
 };
 struct BicgState {
   int status;      // BicgStatus
-  int half_exit;   // converged at the half step; the full-step pass finishes the report
+  int half_exit;   // converged at the half step; the next iteration's pass A finishes the report
   int iterations;  // report (LinearSolveReport)
   int pad;
   double rho[2];   // rho of the iterations of each parity
