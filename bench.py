@@ -272,12 +272,12 @@ def bicgstab_bytes(n, nnz):
     return (8 + 8 + 9) * 8 * n + 2 * csr
 
 
-def momentum_leg(eng, torch, dev, steps, cpu_seconds, cpu=True):
+def momentum_leg(eng, torch, dev, steps, cpu_seconds, cpu=True, side=192):
     """The momentum predictor's linear solve (build_diagonal_preconditioner +
     solve_preconditioned_bicgstab with the reference's momentum settings: 50 iterations, absolute
-    tolerance 1e-8, x0 = 0) on a synthetic 192x192 momentum-shaped CSR system (73 344 unknowns),
-    resident in HBM; one pp_bicgstab_device call per step, x zeroed before each."""
-    s = W.momentum_system(192, 192, seed=0)
+    tolerance 1e-8, x0 = 0) on a synthetic side x side momentum-shaped CSR system (73 344 unknowns
+    at 192), resident in HBM; one pp_bicgstab_device call per step, x zeroed before each."""
+    s = W.momentum_system(side, side, seed=0)
     t = {k: torch.from_numpy(getattr(s, k)).to(dev) for k in ("row_ptr", "col_idx", "values", "rhs")}
     x = torch.zeros(s.n, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
@@ -308,6 +308,8 @@ def momentum_leg(eng, torch, dev, steps, cpu_seconds, cpu=True):
                         "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": read_traffic(BICG_TRAFFIC_FILE),
                         "algorithmic_bytes_per_solve": alg,
                         "note": "whole solve (all passes and the host's state checks); traffic per solve"}}
+    if side != 192:
+        out["roofline"]["traffic"] = None  # the measured traffic file covers the 192x192 system
     if cpu:
         try:
             from oracle import ref
@@ -321,7 +323,7 @@ def momentum_leg(eng, torch, dev, steps, cpu_seconds, cpu=True):
         try:
             x_r, it_r, ab_r, _, _ = ref.bicgstab(s, max_iterations=50, tolerance=1e-8)
             timing = []
-            reps = max(3, min(500, int(cpu_seconds / 0.03)))
+            reps = max(1, min(500, int(cpu_seconds / (0.03 * (side / 192.0) ** 2))))
             ref.bicgstab(s, max_iterations=50, tolerance=1e-8, repeats=reps, timing=timing)
         finally:
             os.sched_setaffinity(0, old)
@@ -461,6 +463,11 @@ def main():
     line["momentum_bicgstab"] = momentum_leg(eng, torch, dev, args.steps * 5, 3.0, cpu=not args.no_cpu_baseline)
     log(f"[bench] bicgstab: {line['momentum_bicgstab']['device_ms']:.3f} ms, "
         f"{line['momentum_bicgstab']['iterations']} iterations")
+    log("[bench] momentum predictor BiCGSTAB, 1024x1024 (2.1 M unknowns)")
+    line["momentum_bicgstab"]["large_1024"] = momentum_leg(eng, torch, dev, 5, 5.0, cpu=not args.no_cpu_baseline,
+                                                           side=1024)
+    log(f"[bench] bicgstab 1024: {line['momentum_bicgstab']['large_1024']['device_ms']:.3f} ms, "
+        f"{line['momentum_bicgstab']['large_1024']['roofline']['frac'] * 100:.1f}% of HBM")
 
     batch_cases = None
     if not args.no_batched:

@@ -6,8 +6,9 @@
 // + 512), one thread per row. SpMV row sums run over the row's CSR entries in order from 0.0
 // (apply_matrix, 235-250), and every vector update is the reference's expression term for term,
 // so only the dot products differ from the reference, by summation order: each pass leaves one
-// partial per workgroup (the canonical 64-lane and 8-wave trees over its 512 rows), and the next
-// pass adds the partials in workgroup order.
+// partial per workgroup (the canonical 64-lane and 8-wave trees over its 512 rows), and the
+// partials are reduced the same way (thread t adds workgroups t, t + 512, ... in order, then the
+// same trees).
 //
 // The scalar recurrence lives on the device. Each pass starts by forming the scalars it needs
 // from the previous pass's partials (every workgroup computes the same values in the same order)
@@ -55,41 +56,41 @@ __device__ __forceinline__ void chunk_partials(double (&v)[K], double *partials,
   }
 }
 
-// The K dot products of a previous pass: sum k is partials[k * chunks + c] added over c in order
-// from 0.0 by thread k. The partials are staged through LDS in tiles of kTile per sum (one
-// cooperative load per tile) and read back 8 at a time, so the chain waits on LDS, not on memory.
+// The K dot products of a previous pass, reduced like a pass's own rows: thread t adds the
+// partials of workgroups t, t + 512, ... in order from 0.0, then the 64-lane and 8-wave trees.
 // Every thread receives all K.
-constexpr int kTile = 512;
 template <int K>
 __device__ __forceinline__ void dot_sums(const double *partials, int chunks, double (&out)[K]) {
-  __shared__ double tile[K * kTile];
-  __shared__ double sums[K];
-  double total = 0.0;
-  __syncthreads();  // an earlier call's tile and sums have been read
-  for (int c0 = 0; c0 < chunks; c0 += kTile) {
-    const int m = chunks - c0 < kTile ? chunks - c0 : kTile;
-    for (int i = threadIdx.x; i < K * kTile; i += kRows) {
-      const int k = i / kTile, c = i - k * kTile;
-      if (c < m) tile[i] = partials[k * chunks + c0 + c];
-    }
-    __syncthreads();
-    if (threadIdx.x < K) {
-      const double *t = tile + threadIdx.x * kTile;
-      for (int c = 0; c < m; c += 8) {
-        double b[8];
+  __shared__ double red[K * 8];
+  double v[K];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) b[j] = c + j < m ? t[c + j] : 0.0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          if (c + j < m) total = total + b[j];
-      }
-    }
-    __syncthreads();
+  for (int k = 0; k < K; ++k) {
+    const double *p = partials + static_cast<size_t>(k) * chunks;
+    double total = 0.0;
+    for (int c = threadIdx.x; c < chunks; c += kRows) total = total + p[c];
+    v[k] = total;
   }
-  if (threadIdx.x < K) sums[threadIdx.x] = total;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = v[k] + __shfl_down(v[k], off, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();  // an earlier call's red has been read
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k * 8 + wave] = v[k];
+  }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < K; ++k) out[k] = sums[k];
+  for (int k = 0; k < K; ++k) {
+    double w[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w[t] = red[k * 8 + t] + red[k * 8 + t + 4];
+    w[0] = w[0] + w[2];
+    w[1] = w[1] + w[3];
+    out[k] = w[0] + w[1];
+  }
 }
 
 // The state's status and half-exit flag as one value per workgroup: thread 0 reads them and the
@@ -105,6 +106,20 @@ __device__ __forceinline__ StateFlags block_flags(const BicgArgs &a) {
   }
   __syncthreads();
   return {flags[0], flags[1]};
+}
+
+// The K sums of partial slots [slot, slot + K): summed here by every workgroup (small systems),
+// or read from the state where the sum pass left them (a.presummed: systems of more than
+// kBicgPresumChunks workgroups, where every workgroup re-reading every partial would cost
+// chunks^2 loads). Both add in the same order, so the bits do not depend on the choice.
+template <int K>
+__device__ __forceinline__ void slot_sums(const BicgArgs &a, int slot, double (&out)[K]) {
+  if (a.presummed) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = a.state->sums[slot + k];
+  } else {
+    dot_sums<K>(a.partials + static_cast<size_t>(slot) * a.chunks, a.chunks, out);
+  }
 }
 
 __device__ __forceinline__ double csr_row(const int *__restrict__ rp, const int *__restrict__ ci,
@@ -195,7 +210,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_a_kernel(BicgArgs a, int k, i
   double bn, rho_new;
   if (k == 0) {
     double rr[2];
-    dot_sums<2>(a.p_setup, a.chunks, rr);  // b.b, r.r
+    slot_sums<2>(a, kSlotSetup, rr);  // b.b, r.r
     bn = sqrt(max0(rr[0]));
     if (blockIdx.x == 0 && threadIdx.x == 0) a.state->rhs_norm = bn;
     const double abs_res = sqrt(max0(rr[1]));
@@ -207,13 +222,13 @@ __global__ void __launch_bounds__(kRows) bicg_pass_a_kernel(BicgArgs a, int k, i
     rho_new = rr[1];  // r_hat = r: r_hat . r is r . r, summed in the same order
   } else if (fl.half_exit) {
     double d[2];
-    dot_sums<2>(a.p_exit, a.chunks, d);  // update.update, x.x after x += update
+    slot_sums<2>(a, kSlotExit, d);  // update.update, x.x after x += update
     record(a, kBicgDone, k, a.state->abs_res, a.state->rel_res, sqrt(d[0] / floor30(d[1])), true);
     return;
   } else {
     bn = a.state->rhs_norm;
     double d[4];
-    dot_sums<4>(a.p_x, a.chunks, d);  // update.update, x.x, r.r, r_hat.r of iteration k - 1
+    slot_sums<4>(a, kSlotX, d);  // update.update, x.x, r.r, r_hat.r of iteration k - 1
     const double iter_res = sqrt(d[0] / floor30(d[1]));
     const double abs_res = sqrt(max0(d[2]));
     const double rel = abs_res / floor30(bn);
@@ -271,7 +286,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_b_kernel(BicgArgs a, int k) {
   const StateFlags fl = block_flags(a);
   if (fl.status != kBicgRunning) return;
   double d[1];
-  dot_sums<1>(a.p_v, a.chunks, d);
+  slot_sums<1>(a, kSlotV, d);
   const double rhat_v = d[0];
   if (!isfinite(rhat_v) || fabs(rhat_v) < 1.0e-30) {
     record(a, kBicgRhatV, k, 0.0, 0.0, 0.0, false);
@@ -317,7 +332,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_c_kernel(BicgArgs a, int k) {
   if (fl.status != kBicgRunning) return;
   const double bn = a.state->rhs_norm;
   double d[3];
-  dot_sums<3>(a.p_b, a.chunks, d);  // s.s, t.t, t.s
+  slot_sums<3>(a, kSlotB, d);  // s.s, t.t, t.s
   const double s_norm = sqrt(max0(d[0]));
   if (converged(a, s_norm / floor30(bn), s_norm)) {
     double v[2] = {0.0, 0.0};
@@ -363,6 +378,20 @@ __global__ void __launch_bounds__(kRows) bicg_pass_c_kernel(BicgArgs a, int k) {
   chunk_partials<4>(v, a.p_x, a.chunks);
 }
 
+// Sum pass for large systems: one workgroup reduces partial slots [first, first + count) as
+// dot_sums does and leaves the totals in state->sums.
+template <int K>
+__global__ void __launch_bounds__(kRows) bicg_sum_kernel(BicgArgs a, int first) {
+  const StateFlags fl = block_flags(a);
+  if (fl.status != kBicgRunning) return;
+  double d[K];
+  dot_sums<K>(a.partials + static_cast<size_t>(first) * a.chunks, a.chunks, d);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) a.state->sums[first + k] = d[k];
+  }
+}
+
 }  // namespace
 
 hipError_t launch_bicg_setup(const BicgArgs &a, hipStream_t stream) {
@@ -370,15 +399,31 @@ hipError_t launch_bicg_setup(const BicgArgs &a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// Before pass A of iteration k: the sums it may read (the setup's at k = 0; else the full step's
+// and the half exit's, whichever the state selects).
+void presum_a(const BicgArgs &a, int k, hipStream_t stream) {
+  if (!a.presummed) return;
+  if (k == 0) {
+    bicg_sum_kernel<2><<<dim3(1), dim3(kRows), 0, stream>>>(a, kSlotSetup);
+  } else {
+    bicg_sum_kernel<4><<<dim3(1), dim3(kRows), 0, stream>>>(a, kSlotX);
+    bicg_sum_kernel<2><<<dim3(1), dim3(kRows), 0, stream>>>(a, kSlotExit);
+  }
+}
+
 hipError_t launch_bicg_iteration(const BicgArgs &a, int iter, hipStream_t stream) {
   const dim3 grid(a.chunks), block(kRows);
+  presum_a(a, iter, stream);
   bicg_pass_a_kernel<<<grid, block, 0, stream>>>(a, iter, 0);
+  if (a.presummed) bicg_sum_kernel<1><<<dim3(1), block, 0, stream>>>(a, kSlotV);
   bicg_pass_b_kernel<<<grid, block, 0, stream>>>(a, iter);
+  if (a.presummed) bicg_sum_kernel<3><<<dim3(1), block, 0, stream>>>(a, kSlotB);
   bicg_pass_c_kernel<<<grid, block, 0, stream>>>(a, iter);
   return hipGetLastError();
 }
 
 hipError_t launch_bicg_report(const BicgArgs &a, int iterations, hipStream_t stream) {
+  presum_a(a, iterations, stream);
   bicg_pass_a_kernel<<<dim3(a.chunks), dim3(kRows), 0, stream>>>(a, iterations, 1);
   return hipGetLastError();
 }

@@ -1074,7 +1074,7 @@ int ensure_bicg(pp_ctx *ctx, size_t n, pp_report *report) {
   ctx->bicg_cap = 0;
   const size_t chunks = (n + pp::kBicgRows - 1) / pp::kBicgRows;
   PP_HIP_CHECK(hipMalloc(&ctx->bicg_work, 12 * n * sizeof(double)));
-  PP_HIP_CHECK(hipMalloc(&ctx->bicg_partials, 12 * chunks * sizeof(double)));
+  PP_HIP_CHECK(hipMalloc(&ctx->bicg_partials, pp::kBicgSlots * chunks * sizeof(double)));
   ctx->bicg_cap = n;
   return PP_OK;
 }
@@ -1104,11 +1104,13 @@ int run_bicgstab(pp_ctx *ctx, int n, const int *rp, const int *ci, const double 
   for (int k = 0; k < 12; ++k) *vec[k] = ctx->bicg_work + static_cast<size_t>(k) * n;
   double *part = ctx->bicg_partials;
   const size_t c = static_cast<size_t>(a.chunks);
-  a.p_setup = part;
-  a.p_v = part + 2 * c;
-  a.p_b = part + 3 * c;
-  a.p_x = part + 6 * c;
-  a.p_exit = part + 10 * c;
+  a.partials = part;
+  a.p_setup = part + pp::kSlotSetup * c;
+  a.p_v = part + pp::kSlotV * c;
+  a.p_b = part + pp::kSlotB * c;
+  a.p_x = part + pp::kSlotX * c;
+  a.p_exit = part + pp::kSlotExit * c;
+  a.presummed = a.chunks > pp::kBicgPresumChunks ? 1 : 0;
   a.state = ctx->bicg_state;
   const int limit = std::max(1, max_iterations);
   PP_HIP_CHECK(hipMemsetAsync(a.state, 0, sizeof(pp::BicgState), s));

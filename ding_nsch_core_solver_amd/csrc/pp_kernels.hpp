@@ -136,15 +136,23 @@ struct BicgState {
   double alpha, omega;
   double abs_res, rel_res, iter_res;
   double rhs_norm;  // |b|, from the first iteration's pass on
+  double sums[12];  // large systems: the dot products of the partial slots, from the sum pass
 };
+// Partial slots (each `chunks` long, in this order): setup b.b, r.r; pass A r_hat.v; pass B s.s,
+// t.t, t.s; pass C update.update, x.x, r.r, r_hat.r; half exit update.update, x.x.
+constexpr int kSlotSetup = 0, kSlotV = 2, kSlotB = 3, kSlotX = 6, kSlotExit = 10, kBicgSlots = 12;
+// Above this many workgroups the partials are summed once by a one-workgroup pass.
+constexpr int kBicgPresumChunks = 256;
 struct BicgArgs {
   int n, chunks, use_rel;
+  int presummed;  // chunks > kBicgPresumChunks: passes read state->sums
   double tolerance;
   const int *row_ptr, *col_idx;
   const double *values, *rhs;
   double *x;
   double *r, *r_hat, *p[2], *v[2], *p_hat, *s, *s_hat, *t, *update, *dinv;  // n each; p, v by parity
-  double *p_setup, *p_v, *p_b, *p_x, *p_exit;                               // 2, 1, 3, 4, 2 x chunks
+  double *partials;  // kBicgSlots x chunks
+  double *p_setup, *p_v, *p_b, *p_x, *p_exit;  // the slots' starts
   BicgState *state;                                                    // zeroed before the setup pass
 };
 // The setup pass; the three passes of iteration iter; the report after `iterations` iterations

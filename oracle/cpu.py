@@ -15,7 +15,7 @@ _lib = None
 
 SCHEMES = {"icpcg": 1, "ildlt_pcg": 2, "split_icpcg": 3, "split_ildlt_pcg": 4}
 SEQUENTIAL, PAIRWISE, CANONICAL = 0, 1, 2
-CHUNKED = 3  # BiCGSTAB dot products in the engine's order (512-row chunks, then chunk order)
+CHUNKED = 3  # BiCGSTAB dot products in the engine's order (512-row chunk trees, then a tree over the chunks)
 
 # One PCG monitor record (the scalar fields of CGMonitorData, ch_sparse_krylov.hpp:47-58).
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),

@@ -1135,10 +1135,25 @@ namespace {
 
 constexpr int kChunked = 3;
 
-// The engine's BiCGSTAB dot-product order: rows are cut into chunks of 512; in chunk c, thread t
-// holds the term of row 512 c + t added to 0.0 (0.0 past the last row), the 64-lane trees
-// (offsets 32..1) and the 8-wave tree (offsets 4..1) give the chunk sum, and the chunk sums are
-// added in chunk order from 0.0.
+// The engine's BiCGSTAB dot-product order. Rows are cut into chunks of 512: in chunk c, thread t
+// holds the term of row 512 c + t added to 0.0 (0.0 past the last row), and the 64-lane trees
+// (offsets 32..1) and the 8-wave tree (offsets 4..1) give the chunk sum. The chunk sums are then
+// reduced the same way: thread t adds chunk sums t, t + 512, ... in order from 0.0, and the same
+// lane and wave trees give the total.
+double tree512(double *part) {
+  constexpr std::size_t W = 8;
+  double wsum[W];
+  for (std::size_t w = 0; w < W; ++w) {
+    double *v = part + 64 * w;
+    for (std::size_t off = 32; off >= 1; off >>= 1)
+      for (std::size_t l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+    wsum[w] = v[0];
+  }
+  for (std::size_t off = W / 2; off >= 1; off >>= 1)
+    for (std::size_t t = 0; t < off; ++t) wsum[t] = wsum[t] + wsum[t + off];
+  return wsum[0];
+}
+
 template <class F>
 double bicg_sum(int mode, std::size_t n, F f) {
   if (mode == kSequential) {
@@ -1146,8 +1161,8 @@ double bicg_sum(int mode, std::size_t n, F f) {
     for (std::size_t k = 0; k < n; ++k) s += f(k);
     return s;
   }
-  constexpr std::size_t T = 512, W = 8;
-  double total = 0.0;
+  constexpr std::size_t T = 512;
+  std::vector<double> chunk;
   for (std::size_t c0 = 0; c0 < n; c0 += T) {
     double part[T];
     for (std::size_t t = 0; t < T; ++t) {
@@ -1155,18 +1170,15 @@ double bicg_sum(int mode, std::size_t n, F f) {
       if (c0 + t < n) s = s + f(c0 + t);
       part[t] = s;
     }
-    double wsum[W];
-    for (std::size_t w = 0; w < W; ++w) {
-      double *v = part + 64 * w;
-      for (std::size_t off = 32; off >= 1; off >>= 1)
-        for (std::size_t l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
-      wsum[w] = v[0];
-    }
-    for (std::size_t off = W / 2; off >= 1; off >>= 1)
-      for (std::size_t t = 0; t < off; ++t) wsum[t] = wsum[t] + wsum[t + off];
-    total = total + wsum[0];
+    chunk.push_back(tree512(part));
   }
-  return total;
+  double part[T];
+  for (std::size_t t = 0; t < T; ++t) {
+    double s = 0.0;
+    for (std::size_t c = t; c < chunk.size(); c += T) s = s + chunk[c];
+    part[t] = s;
+  }
+  return tree512(part);
 }
 
 struct BicgReport {

@@ -141,3 +141,12 @@ def test_host_entry_rejects_bad_csr(engine):
     s.col_idx[1] = 5
     with pytest.raises(E.EngineError, match="column index out of range"):
         engine.bicgstab(s)
+
+
+def test_512_system_full_chip(engine, oracle_lib):
+    """A 512x512 momentum-shaped system (523 264 unknowns, 1022 workgroups) to the iteration limit:
+    equal to the oracle bit for bit."""
+    s = W.momentum_system(512, 512, seed=2)
+    x, rep = engine.bicgstab(s, max_iterations=30, tolerance=1e-12)
+    _check_equal(x, rep, oracle_lib.bicgstab(s, max_iterations=30, tolerance=1e-12, reduction=oracle_lib.CHUNKED))
+    print(f"[bicgstab] 512x512: {rep['iterations']} iterations, {rep['kernel_ms']:.3f} ms", flush=True)
