@@ -2,13 +2,13 @@
 //
 // The reference solves it on the host with build_diagonal_preconditioner (ch_sparse_krylov.hpp:
 // 421-431) and solve_preconditioned_bicgstab (839-963), called at momentum.cpp:469-472. Here every
-// vector pass is a full-chip kernel: 512-thread workgroups, workgroup c owning rows [512 c, 512 c
-// + 512), one thread per row. SpMV row sums run over the row's CSR entries in order from 0.0
-// (apply_matrix, 235-250), and every vector update is the reference's expression term for term,
-// so only the dot products differ from the reference, by summation order: each pass leaves one
-// partial per workgroup (the canonical 64-lane and 8-wave trees over its 512 rows), and the
-// partials are reduced the same way (thread t adds workgroups t, t + 512, ... in order, then the
-// same trees).
+// vector pass is a full-chip kernel: 512-thread workgroups, each owning one chunk of 512 rows
+// (row_chunk: contiguous chunks per XCD), one thread per row. SpMV row sums run over the row's
+// CSR entries in order from 0.0 (apply_matrix, 235-250), and every vector update is the
+// reference's expression term for term, so only the dot products differ from the reference, by
+// summation order: each pass leaves one partial per chunk (the canonical 64-lane and 8-wave trees
+// over its 512 rows), and the partials are reduced the same way (thread t adds chunks t, t + 512,
+// ... in order, then the same trees).
 //
 // The scalar recurrence lives on the device. Each pass starts by forming the scalars it needs
 // from the previous pass's partials (every workgroup computes the same values in the same order)
@@ -27,8 +27,19 @@ namespace {
 
 constexpr int kRows = kBicgRows;  // rows per workgroup, one per thread
 
+// Row chunk of this workgroup. Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD
+// b mod 8), and each XCD has its own L2, so XCD x is given one contiguous run of chunks: a row's
+// neighbours one grid line away are then mostly in chunks of the same XCD and their lines are
+// shared in its L2. The map is a bijection on [0, chunks) for any dispatch order; partials are
+// indexed by chunk, so the summation order does not depend on it.
+__device__ __forceinline__ int row_chunk(int chunks) {
+  const int b = blockIdx.x, x = b & 7, i = b >> 3;
+  const int per = chunks >> 3, rem = chunks & 7;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
 // K per-thread values summed over the workgroup: lanes with offsets 32..1, then the 8 wave sums
-// with offsets 4, 2, 1. Thread 0 stores sum k at partials[k * chunks + blockIdx.x].
+// with offsets 4, 2, 1. Thread 0 stores sum k at partials[k * chunks + row_chunk(chunks)].
 template <int K>
 __device__ __forceinline__ void chunk_partials(double (&v)[K], double *partials, int chunks) {
   __shared__ double red[K * 8];
@@ -51,7 +62,7 @@ __device__ __forceinline__ void chunk_partials(double (&v)[K], double *partials,
       for (int t = 0; t < 4; ++t) w[t] = red[k * 8 + t] + red[k * 8 + t + 4];
       w[0] = w[0] + w[2];
       w[1] = w[1] + w[3];
-      partials[k * chunks + blockIdx.x] = w[0] + w[1];
+      partials[k * chunks + row_chunk(chunks)] = w[0] + w[1];
     }
   }
 }
@@ -154,7 +165,7 @@ __device__ __forceinline__ void record(const BicgArgs &a, int status, int iterat
 // finalize_row_maps records it), r = b - A x, r_hat = r, p = v = 0 (both parities); partials b.b,
 // r.r.
 __global__ void __launch_bounds__(kRows) bicg_setup_kernel(BicgArgs a) {
-  const int q = blockIdx.x * kRows + threadIdx.x;
+  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (q < a.n) {
     int diag = -1;
@@ -193,7 +204,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_a_kernel(BicgArgs a, int k, i
   const double *__restrict__ v_old = a.v[(k + 1) & 1];
   const double *__restrict__ r = a.r;
   const double *__restrict__ dinv = a.dinv;
-  const int q = blockIdx.x * kRows + threadIdx.x;
+  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
   int beg = 0, end = 0;
   double rq = 0.0, pq_old = 0.0, vq_old = 0.0, dq = 0.0, rhq = 0.0;
   if (q < a.n) {
@@ -272,7 +283,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_b_kernel(BicgArgs a, int k) {
   const double *__restrict__ vk = a.v[k & 1];
   const double *__restrict__ r = a.r;
   const double *__restrict__ dinv = a.dinv;
-  const int q = blockIdx.x * kRows + threadIdx.x;
+  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
   int beg = 0, end = 0;
   double rq = 0.0, vq = 0.0, dq = 0.0, phq = 0.0;
   if (q < a.n) {  // issued before the partial sums, as in pass A
@@ -318,7 +329,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_b_kernel(BicgArgs a, int k) {
 // workgroup 0 marks the half exit for the next pass A); else omega = t.s / t.t with both checks,
 // update += omega s_hat; x += update; r = s - omega t; partials update.update, x.x, r.r, r_hat.r.
 __global__ void __launch_bounds__(kRows) bicg_pass_c_kernel(BicgArgs a, int k) {
-  const int q = blockIdx.x * kRows + threadIdx.x;
+  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
   double uq = 0.0, xq = 0.0, shq = 0.0, sq = 0.0, tq = 0.0, rhq = 0.0;
   if (q < a.n) {  // issued before the partial sums, as in pass A
     uq = a.update[q];
