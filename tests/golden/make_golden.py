@@ -9,7 +9,9 @@ vector (235-250, 444-513), the scalar CGMonitor log of the reference's PCG on th
 (ch_sparse_krylov.hpp:47-58, 515-660) and the full Solver::solve_pressure_correction result
 (pressure.cpp:1031). The velcorr_*.npz fixtures hold the inputs and outputs of
 Solver::apply_pressure_velocity_correction (operators.cpp:480-534) on whole ghosted buffers, and
-of apply_pressure_bc (143-181) on a random ghosted field.
+of apply_pressure_bc (143-181) on a random ghosted field. The bicgstab_*.npz fixtures hold a
+momentum-shaped CSR system with the result of the reference's build_diagonal_preconditioner +
+solve_preconditioned_bicgstab (ch_sparse_krylov.hpp:421-431, 839-963) on it.
 The files hold data only (no code) and load with numpy.load(allow_pickle=False).
 """
 import os
@@ -121,6 +123,23 @@ def write_velcorr(name, case, vbc, scheme, g):
     print(f"{name}: speeds={speeds}")
 
 
+def bicgstab_cases():
+    """(name, momentum_system kwargs, max_iterations, tolerance, use_relative, random x0 seed or None)"""
+    return [("bicgstab_momentum_12x10_abs", dict(nx=12, ny=10, seed=41), 50, 1e-8, False, None),
+            ("bicgstab_momentum_20x16_rel_x0", dict(nx=20, ny=16, seed=42, viscosity_ratio=300.0), 50, 1e-11, True, 7),
+            ("bicgstab_momentum_16x16_limit3", dict(nx=16, ny=16, seed=43, dt=1e-2), 3, 1e-15, False, None)]
+
+
+def write_bicgstab(name, kw, max_it, tol, rel, x0_seed):
+    s = W.momentum_system(**kw)
+    x0 = np.zeros(s.n) if x0_seed is None else np.random.default_rng(x0_seed).standard_normal(s.n)
+    x, it, ab, rr, itr = ref.bicgstab(s, x0, max_iterations=max_it, tolerance=tol, use_relative=rel)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values,
+                        rhs=s.rhs, x0=x0, max_iterations=max_it, tolerance=tol, use_relative=int(rel), out_x=x,
+                        out_iterations=it, out_absolute=ab, out_relative=rr, out_iterate=itr)
+    print(f"{name}: {it} iterations, |r| {ab:.3e}")
+
+
 if __name__ == "__main__":
     if not ref.available():
         sys.exit("build the reference checker first: make -C oracle ref")
@@ -128,3 +147,5 @@ if __name__ == "__main__":
         write(name, scheme, case)
     for name, case, vbc, scheme, g in velcorr_cases():
         write_velcorr(name, case, vbc, scheme, g)
+    for args in bicgstab_cases():
+        write_bicgstab(*args)

@@ -4,6 +4,9 @@ against the reference's own functions compiled from its sources, bit for bit wit
 products, on momentum-shaped systems and on the reference's own assembled pressure systems; the
 same error texts on every breakdown the tests can force; and checks that need no reference (exact
 one-step solves of diagonal systems, the engine's chunked dot order within rounding)."""
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -134,3 +137,27 @@ def test_momentum_solution_satisfies_the_system(oracle_lib):
         a[r, s.col_idx[s.row_ptr[r]:s.row_ptr[r + 1]]] = s.values[s.row_ptr[r]:s.row_ptr[r + 1]]
     assert np.linalg.norm(a @ x - s.rhs) < 1e-9
     assert np.allclose(x, np.linalg.solve(a, s.rhs), rtol=0, atol=1e-9 * np.abs(x).max())
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bicgstab_*.npz")))
+
+
+def golden_system(z):
+    return W.CsrSystem(z["row_ptr"], z["col_idx"], z["values"], z["rhs"])
+
+
+def test_bicgstab_fixtures_exist():
+    assert len(GOLDEN) >= 3, "golden vectors missing: run tests/golden/make_golden.py"
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_oracle_matches_bicgstab_golden(oracle_lib, path):
+    """The reference's own results, committed as data: the oracle reproduces them bit for bit
+    without the reference present."""
+    with np.load(path, allow_pickle=False) as z:
+        s = golden_system(z)
+        x, it, ab, rel, itr = oracle_lib.bicgstab(s, z["x0"], max_iterations=int(z["max_iterations"]),
+                                                  tolerance=float(z["tolerance"]), use_relative=bool(z["use_relative"]))
+        assert it == int(z["out_iterations"])
+        assert (ab, rel, itr) == (float(z["out_absolute"]), float(z["out_relative"]), float(z["out_iterate"]))
+        assert np.array_equal(x, z["out_x"])

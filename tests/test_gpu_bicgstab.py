@@ -150,3 +150,19 @@ def test_512_system_full_chip(engine, oracle_lib):
     x, rep = engine.bicgstab(s, max_iterations=30, tolerance=1e-12)
     _check_equal(x, rep, oracle_lib.bicgstab(s, max_iterations=30, tolerance=1e-12, reduction=oracle_lib.CHUNKED))
     print(f"[bicgstab] 512x512: {rep['iterations']} iterations, {rep['kernel_ms']:.3f} ms", flush=True)
+
+
+@pytest.mark.parametrize("path", __import__("tests.test_bicgstab_oracle", fromlist=["GOLDEN"]).GOLDEN,
+                         ids=lambda p: p.rsplit("/", 1)[-1])
+def test_against_reference_golden(engine, oracle_lib, path):
+    """The reference's committed results: the same iteration count and x to rounding (the GPU sums
+    dot products in its chunked order), and the oracle's chunked replay bit for bit."""
+    from tests.test_bicgstab_oracle import golden_system
+    with np.load(path, allow_pickle=False) as z:
+        s = golden_system(z)
+        kw = dict(max_iterations=int(z["max_iterations"]), tolerance=float(z["tolerance"]),
+                  use_relative=bool(z["use_relative"]))
+        x, rep = engine.bicgstab(s, z["x0"], **kw)
+        assert rep["iterations"] == int(z["out_iterations"])
+        assert np.abs(x - z["out_x"]).max() <= 1e-12 * np.abs(z["out_x"]).max()
+        _check_equal(x, rep, oracle_lib.bicgstab(s, z["x0"], reduction=oracle_lib.CHUNKED, **kw))
