@@ -68,7 +68,8 @@ struct pp_ctx {
   double *bicg_work = nullptr;
   double *bicg_partials = nullptr;     // 12 * chunks
   pp::BicgState *bicg_state = nullptr;
-  pp::BicgState *h_bicg = nullptr;     // pinned copy of the state
+  pp::BicgState *h_bicg = nullptr;     // pinned copies of the state (3 slots)
+  hipEvent_t bicg_ev[2] = {nullptr, nullptr};
   size_t bicg_stage_rows = 0, bicg_stage_nnz = 0;
   int *bicg_stage_idx = nullptr;     // row_ptr then col_idx
   double *bicg_stage_val = nullptr;  // values, rhs, x
@@ -697,6 +698,8 @@ void pp_destroy(pp_ctx *ctx) {
   if (ctx->bicg_partials) (void)hipFree(ctx->bicg_partials);
   if (ctx->bicg_state) (void)hipFree(ctx->bicg_state);
   if (ctx->h_bicg) (void)hipHostFree(ctx->h_bicg);
+  if (ctx->bicg_ev[0]) (void)hipEventDestroy(ctx->bicg_ev[0]);
+  if (ctx->bicg_ev[1]) (void)hipEventDestroy(ctx->bicg_ev[1]);
   if (ctx->bicg_stage_idx) (void)hipFree(ctx->bicg_stage_idx);
   if (ctx->bicg_stage_val) (void)hipFree(ctx->bicg_stage_val);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
@@ -1064,7 +1067,9 @@ namespace {
 int ensure_bicg(pp_ctx *ctx, size_t n, pp_report *report) {
   if (!ctx->bicg_state) {
     PP_HIP_CHECK(hipMalloc(&ctx->bicg_state, sizeof(pp::BicgState)));
-    PP_HIP_CHECK(hipHostMalloc(&ctx->h_bicg, sizeof(pp::BicgState), hipHostMallocDefault));
+    PP_HIP_CHECK(hipHostMalloc(&ctx->h_bicg, 3 * sizeof(pp::BicgState), hipHostMallocDefault));
+    PP_HIP_CHECK(hipEventCreateWithFlags(&ctx->bicg_ev[0], hipEventDisableTiming));
+    PP_HIP_CHECK(hipEventCreateWithFlags(&ctx->bicg_ev[1], hipEventDisableTiming));
   }
   if (n <= ctx->bicg_cap) return PP_OK;
   if (ctx->bicg_work) (void)hipFree(ctx->bicg_work);
@@ -1079,8 +1084,8 @@ int ensure_bicg(pp_ctx *ctx, size_t n, pp_report *report) {
   return PP_OK;
 }
 
-// Iterations queued between two reads of the device state: a solve that stops at iteration k
-// leaves fewer than this many iterations of passes that return at once.
+// Iterations per queued group. The host reads the state of group g while group g + 1 is queued,
+// so a solve that stops at iteration k leaves fewer than two groups of passes that return at once.
 constexpr int kBicgQueue = 4;
 
 // solve_preconditioned_bicgstab (ch_sparse_krylov.hpp:839-963). The device passes run the
@@ -1116,16 +1121,29 @@ int run_bicgstab(pp_ctx *ctx, int n, const int *rp, const int *ci, const double 
   PP_HIP_CHECK(hipMemsetAsync(a.state, 0, sizeof(pp::BicgState), s));
   PP_HIP_CHECK(hipEventRecord(ctx->ev_start, s));
   PP_HIP_CHECK(pp::launch_bicg_setup(a, s));
-  pp::BicgState &st = *ctx->h_bicg;
-  st.status = pp::kBicgRunning;
-  for (int it0 = 0; it0 < limit && st.status == pp::kBicgRunning; it0 += kBicgQueue) {
-    for (int iter = it0; iter < std::min(limit, it0 + kBicgQueue); ++iter)
+  // Group g (kBicgQueue iterations) is followed by a copy of the state into pinned slot g & 1 and
+  // an event. The host waits for group g's copy only after queueing group g + 1, so the device
+  // always has the next group queued while the host reads the state.
+  pp::BicgState *hs = ctx->h_bicg;  // slots 0, 1: groups; 2: the final state
+  const int groups = (limit + kBicgQueue - 1) / kBicgQueue;
+  auto queue_group = [&](int g) -> int {
+    for (int iter = g * kBicgQueue; iter < std::min(limit, (g + 1) * kBicgQueue); ++iter)
       PP_HIP_CHECK(pp::launch_bicg_iteration(a, iter, s));
-    PP_HIP_CHECK(hipMemcpyAsync(&st, a.state, sizeof(pp::BicgState), hipMemcpyDeviceToHost, s));
-    PP_HIP_CHECK(hipStreamSynchronize(s));
+    PP_HIP_CHECK(hipMemcpyAsync(&hs[g & 1], a.state, sizeof(pp::BicgState), hipMemcpyDeviceToHost, s));
+    PP_HIP_CHECK(hipEventRecord(ctx->bicg_ev[g & 1], s));
+    return PP_OK;
+  };
+  if (int rc = queue_group(0)) return rc;
+  bool stopped = false;
+  for (int g = 0; g < groups && !stopped; ++g) {
+    if (g + 1 < groups)
+      if (int rc = queue_group(g + 1)) return rc;
+    PP_HIP_CHECK(hipEventSynchronize(ctx->bicg_ev[g & 1]));
+    stopped = hs[g & 1].status != pp::kBicgRunning;
   }
-  if (st.status == pp::kBicgRunning) PP_HIP_CHECK(pp::launch_bicg_report(a, limit, s));
+  if (!stopped) PP_HIP_CHECK(pp::launch_bicg_report(a, limit, s));
   PP_HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
+  pp::BicgState &st = hs[2];
   PP_HIP_CHECK(hipMemcpyAsync(&st, a.state, sizeof(pp::BicgState), hipMemcpyDeviceToHost, s));
   PP_HIP_CHECK(hipStreamSynchronize(s));
   float ms = 0.0f;
