@@ -85,7 +85,12 @@ def cpu_baseline(case, seconds):
     except (ImportError, FileNotFoundError):
         from oracle import cpu as impl
         if not impl.available():
+            log("[bench] WARNING: no CPU baseline: neither oracle/_ref/libdingref.so (the compiled reference) "
+                "nor oracle/liboracle.so is built")
             return None, None
+        log("[bench] WARNING: oracle/_ref/libdingref.so (the reference compiled from its own sources) is not "
+            "present; the CPU baseline falls back to the repository's single-threaded restatement "
+            "(oracle/liboracle.so, kind \"port\")")
         kind = "port"
         solve = lambda: impl.pressure_solve("icpcg", case, impl.SEQUENTIAL)  # noqa: E731
     old = os.sched_getaffinity(0)
@@ -353,6 +358,51 @@ def batched_cpu_baseline(cases, threads):
             "sample": f"all {len(cases)} systems of the batch, {threads} threads, wall {wall:.1f} s"}
 
 
+NORTH_STAR_TOLERANCE = 1.0e-12  # relative pressure agreement asked of the FP64 drop-in
+
+
+def parity_leg(eng, seeds=(0, 1, 2)):
+    """FP64 drop-in parity on the 192x192 dr=1000 workload: every seed x scheme (icpcg, ildlt_pcg,
+    split_icpcg) solved on the GPU and by the compiled reference (oracle/_ref) on the host; reports the
+    iteration counts and the max-abs relative pressure difference against the 1e-12 target."""
+    try:
+        from oracle import ref
+        if not ref.available():
+            raise FileNotFoundError
+    except (ImportError, FileNotFoundError):
+        log("[bench] WARNING: parity leg skipped: oracle/_ref/libdingref.so is not present")
+        return None
+    rows, worst, same_iters = [], 0.0, True
+    for seed in seeds:
+        case = W.droplet_case(192, 192, seed=seed, density_ratio=1000.0, tolerance=1e-8, max_iterations=20000)
+        for name, code in (("icpcg", E.PP_ICPCG), ("ildlt_pcg", E.PP_ILDLT_PCG), ("split_icpcg", E.PP_SPLIT_ICPCG)):
+            p_gpu, rep = eng.solve(case, code)
+            p_ref, it_ref, _ = ref.pressure_solve(name, case)
+            dev = float(np.abs(p_gpu - p_ref).max() / np.abs(p_ref).max())
+            rows.append({"seed": seed, "scheme": name, "iterations_gpu": rep["iterations"], "iterations_cpu": it_ref,
+                         "max_abs_diff_over_max_abs": dev})
+            worst = max(worst, dev)
+            same_iters = same_iters and rep["iterations"] == it_ref
+    return {"north_star_tolerance": NORTH_STAR_TOLERANCE, "max_deviation": worst,
+            "met": bool(worst <= NORTH_STAR_TOLERANCE and same_iters), "iterations_equal": bool(same_iters),
+            "cases": rows,
+            "note": "the engine sums reductions in a fixed tree order and is bit-identical to the oracle replaying "
+                    "that order; the deviation from the reference's sequential order is the reordering spread of "
+                    "the reference's own arithmetic"}
+
+
+def copy_calibration(nbytes=256 << 20, repeats=3):
+    """A plain device-to-device copy of known size, run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    to calibrate the HBM byte counters: nbytes read and nbytes written per copy."""
+    import torch
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(repeats):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copied {nbytes} bytes x{repeats}")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,12 +413,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-system batch leg")
     ap.add_argument("--batched-repeats", type=int, default=3)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this engine runs one pressure system per "
               "solve on one GPU (independent systems would shard by system); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        copy_calibration()
+        return
 
     import torch
     dev = torch.device("cuda:0")
@@ -484,6 +540,10 @@ def main():
             p_ref, it_ref, res_ref = ref_result
             line["parity"] = {"iterations_gpu": iters, "iterations_cpu": it_ref,
                               "max_abs_diff_over_max_abs": float(np.abs(p_gpu - p_ref).max() / np.abs(p_ref).max())}
+            log("[bench] parity leg: seeds 0-2 x icpcg / ildlt_pcg / split_icpcg against the reference")
+            leg = parity_leg(eng)
+            if leg is not None:
+                line["parity"].update(leg)
             line["mixed"]["max_abs_diff_vs_cpu_over_max_abs"] = float(np.abs(p_mixed - p_ref).max() / np.abs(p_ref).max())
             line["speedup_vs_cpu"] = line["value"] / base["value"]
         if batch_cases is not None:

@@ -91,7 +91,8 @@ struct SweepShared {
   // Write targets for lanes that have nothing to publish, so hand-off stores need no branch.
   volatile double dummy_d[64];
   volatile int dummy_i[64];
-  volatile int spin_error;
+  volatile int spin_error;              // set by the first wait that gives up; every later wait returns at once
+  volatile int spin_limit;              // sleeps a wait may spend before it gives up (kSpinLimit unless lowered)
   volatile int status;                  // DevStatus of the system, uniform after each barrier
 };
 
@@ -155,15 +156,21 @@ __device__ __forceinline__ double block_max1(double v, SweepShared &sh, int &par
   return m;
 }
 
-// Bounded wait on an LDS counter; on timeout records an error and returns (never hangs).
+// Bounded wait on an LDS counter. A wait that spins past spin_limit sleeps records spin_error and
+// returns; once spin_error is set every wait returns at once, so one broken hand-off costs a single
+// wait budget and the sweeps run to their end without further waiting. The solve loops read
+// spin_error at a point where no wave can be inside a wait (after a block-wide barrier that follows
+// the sweeps and precedes the next ones) and leave with kSweepFail, so every wave takes the same
+// path and the kernel exits.
 __device__ __forceinline__ void wait_at_least(volatile int *counter, int target, SweepShared &sh) {
   int guard = 0;
   while (*lds(counter) < target) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++guard > kSpinLimit) {
+    if (*lds(&sh.spin_error)) return;
+    if (++guard > *lds(&sh.spin_limit)) {
       *lds(&sh.spin_error) = 1;
       return;
     }
+    __builtin_amdgcn_s_sleep(1);
   }
 }
 

@@ -1207,6 +1207,7 @@ __device__ __forceinline__ bool setup_system(const SysArgs &a, SweepShared &sh, 
   const int n = a.n, ny = a.ny;
   if (tid == 0) {
     *lds(&sh.spin_error) = 0;
+    *lds(&sh.spin_limit) = a.spin_limit > 0 ? a.spin_limit : kSpinLimit;
     *lds(&sh.status) = kOk;
   }
   __syncthreads();
@@ -1333,14 +1334,15 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
     }
     rz = block_sum1(rz, sh, parity);
     if (!isfinite(rz) || rz <= 0.0) {
-      if (tid == 0) *lds(&sh.status) = kRz0Fail;
+      if (tid == 0) *lds(&sh.status) = *lds(&sh.spin_error) ? kSweepFail : kRz0Fail;  // as for r.z below
       finish();
       return;
     }
     monitor(0.0, 0.0, abs_res, rel_res, 0.0, sqrt(rz < 0.0 ? 0.0 : rz));
     mark(0);
     const int limit = a.max_iterations < 1 ? 1 : a.max_iterations;
-    for (int it = 0; it < limit; ++it) {
+    // the loop head reads spin_error uniformly: the last sweeps ended before a barrier since
+    for (int it = 0; it < limit && !*lds(&sh.spin_error); ++it) {
       // Ap and p^T A p (level-major order)
       double pap = spmv_pass<double>(a, stencil, p_raw, m_p, nullptr, [&](int base, int i, double apq, double pq, double) {
         apv_rows.put(base, i, apq);
@@ -1446,8 +1448,8 @@ __device__ __forceinline__ void solve_system(const SysArgs &a, SweepShared &sh) 
       }
       rzn = block_sum1(rzn, sh, parity);
       mark(7);
-      if (!isfinite(rzn) || rzn <= 0.0) {
-        if (tid == 0) *lds(&sh.status) = kRzFail;
+      if (!isfinite(rzn) || rzn <= 0.0) {  // after a failed hand-off r.z is garbage: report the hand-off
+        if (tid == 0) *lds(&sh.status) = *lds(&sh.spin_error) ? kSweepFail : kRzFail;
         finish();
         return;
       }
@@ -1628,7 +1630,7 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
     rz = block_sum1(rz, sh, parity);
     float m_p = 0.0f;
     clk.mark(7);
-    while (isfinite(rz) && rz > 0.0 && inner < inner_cap && total < limit) {
+    while (isfinite(rz) && rz > 0.0 && inner < inner_cap && total < limit && !*lds(&sh.spin_error)) {
       double pap = spmv_pass<float>(a, A32, p32, m_p, nullptr, [&](int base, int i, float apq, float pq, float) {
         ap_rows.put(base, i, apq);
         return static_cast<double>(pq) * static_cast<double>(apq);
@@ -1745,7 +1747,7 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
       rz = rzn;
       clk.mark(8);
     }
-    if (inner == 0) break;  // the inner solve cannot make progress: stop like a silent non-convergence
+    if (inner == 0 || *lds(&sh.spin_error)) break;  // no progress possible: stop like a silent non-convergence
     // x += |r| d in FP64 (x kept unprojected, its mean tracked for the lazy projection)
     double xs = 0.0, dd = 0.0, xx = 0.0;
     for (int q = tid; q < n; q += kThreads) {

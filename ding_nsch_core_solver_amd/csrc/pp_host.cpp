@@ -46,6 +46,8 @@ struct pp_ctx {
 
   // sweep operand ring depth per grid shape (pp_set_tuning)
   std::map<std::pair<int, int>, int> tuning;
+  // sleeps a hand-off wait may spend before the solve fails (pp_set_wait_limit; 0: built-in)
+  int wait_limit = 0;
 
   // batched solves: per-system work slab, wrap lines, arguments, reports, order and counters
   size_t batch_cells = 0;       // capacity of batch_work in cells (sum of n + kPad)
@@ -153,11 +155,15 @@ int validate(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
   for (int s = 0; s < 4; ++s)
     if (pb->bc_type[s] < PP_BC_UNSET || pb->bc_type[s] > PP_BC_NEUMANN)
       return fail(ctx, report, PP_ERR_ARG, "unknown boundary type");
-  // explicit pressure BCs on periodic axes are rejected by the reference (operators.cpp:11-34)
-  if (pb->periodic_x && (pb->bc_type[0] != PP_BC_UNSET || pb->bc_type[1] != PP_BC_UNSET))
-    return fail(ctx, report, PP_ERR_ARG, "pressure_bc_left cannot be set when the direction is periodic");
-  if (pb->periodic_y && (pb->bc_type[2] != PP_BC_UNSET || pb->bc_type[3] != PP_BC_UNSET))
-    return fail(ctx, report, PP_ERR_ARG, "pressure_bc_bottom cannot be set when the direction is periodic");
+  // explicit pressure BCs on periodic axes are rejected by the reference, which checks left, right,
+  // bottom, top in that order and names the first edge that is set (operators.cpp:11-34)
+  static const char *const kEdgeNames[4] = {"pressure_bc_left", "pressure_bc_right", "pressure_bc_bottom",
+                                            "pressure_bc_top"};
+  for (int s = 0; s < 4; ++s) {
+    const bool periodic = s < 2 ? pb->periodic_x != 0 : pb->periodic_y != 0;
+    if (periodic && pb->bc_type[s] != PP_BC_UNSET)
+      return fail(ctx, report, PP_ERR_ARG, std::string(kEdgeNames[s]) + " cannot be set when the direction is periodic");
+  }
   if (pb->ring_depth != 0 && !pp::valid_ring((pb->nx + 63) / 64, pb->ring_depth))
     return fail(ctx, report, PP_ERR_ARG, "ring_depth must be a power of two >= 8 with ceil(nx/64) * ring_depth <= 64");
   return PP_OK;
@@ -251,6 +257,7 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
     auto t = ctx->tuning.find(std::make_pair(pb->nx, pb->ny));
     if (t != ctx->tuning.end()) a.ring = t->second;
   }
+  a.spin_limit = ctx->wait_limit;
   a.lev_base = L.base + pp::kLevPad;
   for (int w = 0; w <= pp::kWaves; ++w) a.spmv_split[w] = L.split[w];
   a.cell = L.cell;
@@ -946,6 +953,14 @@ int pp_set_tuning(pp_ctx *ctx, int count, const int *nx, const int *ny, const in
     table[std::make_pair(nx[k], ny[k])] = ring_depth[k];
   }
   ctx->tuning.swap(table);
+  return PP_OK;
+}
+
+int pp_set_wait_limit(pp_ctx *ctx, int sleeps) {
+  if (!ctx) return PP_ERR_ARG;
+  pp_report *report = nullptr;
+  if (sleeps < 0) return fail(ctx, report, PP_ERR_ARG, "wait limit must be >= 0");
+  ctx->wait_limit = sleeps;
   return PP_OK;
 }
 

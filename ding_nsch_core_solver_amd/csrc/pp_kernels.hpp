@@ -29,6 +29,7 @@ struct SysArgs {
   double rho_ref;
   int step_index;
   int ring;                // sweep operand ring depth in levels (0: default for the wave count)
+  int spin_limit;          // sleeps a hand-off wait may spend before the solve fails (0: kSpinLimit)
   // mixed precision controls
   int inner_max_iterations;
   double inner_tolerance;

@@ -93,7 +93,7 @@ EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_
            "pp_solve_coeffs", "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device",
            "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device", "pp_solve_monitored",
            "pp_solve_device_monitored", "pp_solve_device_field", "pp_pressure_step", "pp_bicgstab",
-           "pp_bicgstab_device")
+           "pp_bicgstab_device", "pp_set_wait_limit")
 
 # One PCG monitor record (pp_monitor_record): the scalar fields of one CGMonitor call.
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
@@ -176,6 +176,8 @@ def lib():
         ip = ctypes.POINTER(ctypes.c_int)
         L.pp_set_tuning.argtypes = [vp, ctypes.c_int, ip, ip, ip]
         L.pp_set_tuning.restype = ctypes.c_int
+        L.pp_set_wait_limit.argtypes = [vp, ctypes.c_int]
+        L.pp_set_wait_limit.restype = ctypes.c_int
         vb = ctypes.POINTER(PPVelocityBC)
         L.pp_velocity_correction_device.argtypes = [vp, ctypes.POINTER(PPProblem), vb, ctypes.c_int, vp, vp, vp, vp,
                                                     vp, vp, vp, vp, dp, vp]
@@ -263,6 +265,13 @@ class Engine:
         arr = lambda vals: (ctypes.c_int * max(len(vals), 1))(*vals)  # noqa: E731
         rc = self._lib.pp_set_tuning(self.h, len(keys), arr([k[0] for k in keys]), arr([k[1] for k in keys]),
                                      arr([table[k] for k in keys]))
+        if rc != 0:
+            raise EngineError(rc, self.last_error())
+
+    def set_wait_limit(self, sleeps: int = 0):
+        """Sleeps a sweep hand-off wait may spend before the solve fails with PP_ERR_SWEEP
+        (0: the built-in budget)."""
+        rc = self._lib.pp_set_wait_limit(self.h, int(sleeps))
         if rc != 0:
             raise EngineError(rc, self.last_error())
 

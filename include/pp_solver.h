@@ -312,6 +312,18 @@ int pp_bicgstab(pp_ctx *ctx, int n, const int *row_ptr, const int *col_idx, cons
  */
 int pp_set_tuning(pp_ctx *ctx, int count, const int *nx, const int *ny, const int *ring_depth);
 
+/*
+ * Watchdog of the wavefront sweeps (no counterpart in the reference, whose sweeps are sequential).
+ * The waves of a solve hand boundary values and staged operands to each other through LDS counters;
+ * each wait is bounded, and a wait that spins past `sleeps` sleeps fails the solve with
+ * PP_ERR_SWEEP ("wavefront hand-off did not complete") instead of hanging. Once one wait has given
+ * up, every later wait of that launch returns at once and the solve loop leaves at its next
+ * iteration, so a failure costs one wait budget. sleeps = 0 restores the built-in budget (2^24
+ * sleeps, about a second); a small value makes an ordinary solve fail, which is how the error path
+ * is tested. Applies to later calls on the context.
+ */
+int pp_set_wait_limit(pp_ctx *ctx, int sleeps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@ Bar, per case and scheme:
   the reference's reductions moves its own dr=1000 pressure by 1e-11 to 2.5e-8 (max-abs relative),
   so no parallel summation can be tighter than that; the tolerance below covers it with margin.
 """
+import re
 import ctypes
 
 import numpy as np
@@ -51,7 +52,7 @@ def _check(engine, oracle_lib, case, scheme):
     try:
         p_seq, it_seq, res_seq = oracle_lib.pressure_solve(scheme, case, oracle_lib.SEQUENTIAL)
     except RuntimeError as err:
-        with pytest.raises(E.EngineError, match=str(err)):
+        with pytest.raises(E.EngineError, match=re.escape(str(err))):
             engine.solve(case, CODES[scheme])
         return
     p_gpu, rep = engine.solve(case, CODES[scheme])
@@ -109,6 +110,50 @@ def test_argument_errors(engine):
     case = W.droplet_case(3, 8, seed=0, periodic_x=True)
     with pytest.raises(E.EngineError, match="periodic direction"):
         engine.solve(case, E.PP_ICPCG)
+
+
+@pytest.mark.parametrize("edge", [0, 1, 2, 3])
+def test_periodic_edge_error_names_the_edge(engine, edge):
+    """An explicit pressure BC on a periodic axis is rejected with the name of the first edge set, in
+    the reference's check order left, right, bottom, top (operators.cpp:11-34)."""
+    names = ("pressure_bc_left", "pressure_bc_right", "pressure_bc_bottom", "pressure_bc_top")
+    bc = [W.BC_UNSET] * 4
+    bc[edge] = W.BC_NEUMANN
+    case = W.droplet_case(8, 8, seed=0, periodic_x=edge < 2, periodic_y=edge >= 2, bc_type=tuple(bc))
+    with pytest.raises(E.EngineError, match=re.escape(f"{names[edge]} cannot be set when the direction is periodic")):
+        engine.solve(case, E.PP_ICPCG)
+    if edge in (1, 3):  # both edges of the axis set: the first one in the reference's order is named
+        bc[edge - 1] = W.BC_DIRICHLET
+        case = W.droplet_case(8, 8, seed=0, periodic_x=edge < 2, periodic_y=edge >= 2, bc_type=tuple(bc))
+        with pytest.raises(E.EngineError, match=re.escape(names[edge - 1])):
+            engine.solve(case, E.PP_ICPCG)
+
+
+@pytest.mark.parametrize("nx,ny,precision", [(192, 192, E.PP_FP64), (48, 40, E.PP_FP64), (48, 40, E.PP_MIXED),
+                                             (192, 192, E.PP_MIXED)])
+def test_hand_off_watchdog_fails_fast_and_context_recovers(engine, oracle_lib, nx, ny, precision):
+    """A wait limit of one sleep makes the first sweep wait that finds its counter behind give up
+    (192x192: the factorization's hand-off between sweep waves; 48x40, one sweep wave: the first
+    preconditioner application, where the sweep wave waits for its loaders). The solve must come back
+    with PP_ERR_SWEEP quickly, and the next solve on the same context must be exact again."""
+    import time
+    case = W.droplet_case(nx, ny, seed=3, density_ratio=100.0)
+    p0, rep0 = engine.solve(case, E.PP_ICPCG, precision)  # warm: layouts and buffers exist
+    engine.set_wait_limit(1)
+    try:
+        t0 = time.perf_counter()
+        with pytest.raises(E.EngineError, match="wavefront hand-off did not complete") as ei:
+            engine.solve(case, E.PP_ICPCG, precision)
+        elapsed = time.perf_counter() - t0
+    finally:
+        engine.set_wait_limit(0)
+    assert ei.value.status == E.PP_ERR_SWEEP
+    assert elapsed < 0.5, f"failing solve took {elapsed:.3f} s"
+    p1, rep1 = engine.solve(case, E.PP_ICPCG, precision)
+    assert rep1["iterations"] == rep0["iterations"] and np.array_equal(p1, p0)
+    if precision == E.PP_FP64:
+        p_can, it_can, _ = oracle_lib.pressure_solve("icpcg", case, oracle_lib.CANONICAL)
+        assert rep1["iterations"] == it_can and np.array_equal(p1, p_can)
 
 
 def test_pitched_host_buffers_match_contiguous(engine):
