@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -143,8 +144,8 @@ const char *status_text(int status, int scheme) {
   }
 }
 
-int validate(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
-  if (!ctx) return fail(ctx, report, PP_ERR_ARG, "null context");
+// Checks of the problem itself; ctx may be null (the error text then only goes to the report).
+int validate_problem(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
   if (!pb) return fail(ctx, report, PP_ERR_ARG, "null problem");
   if (pb->nx < 1 || pb->ny < 1 || pb->nx > pp::kMaxRows || pb->ny > pp::kMaxSide)
     return fail(ctx, report, PP_ERR_ARG, "grid size outside nx 1..512, ny 1..1024");
@@ -167,6 +168,11 @@ int validate(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
   if (pb->ring_depth != 0 && !pp::valid_ring((pb->nx + 63) / 64, pb->ring_depth))
     return fail(ctx, report, PP_ERR_ARG, "ring_depth must be a power of two >= 8 with ceil(nx/64) * ring_depth <= 64");
   return PP_OK;
+}
+
+int validate(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
+  if (!ctx) return fail(ctx, report, PP_ERR_ARG, "null context");
+  return validate_problem(ctx, pb, report);
 }
 
 bool has_dirichlet(const pp_problem *pb) {
@@ -228,12 +234,13 @@ void set_fwork(pp::SysArgs &a, float *w, size_t stride) {
   for (int k = 0; k < kF32Arrays; ++k) *f32[k] = w + static_cast<size_t>(k) * stride;
 }
 
-void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::SysArgs &a) {
+// The problem's scalar part of the argument block (no buffers, default ring depth and wait limit).
+void fill_problem_args(const pp_problem *pb, pp::SysArgs &a) {
   std::memset(&a, 0, sizeof(a));
   a.nx = pb->nx;
   a.ny = pb->ny;
   a.n = pb->nx * pb->ny;
-  a.nlev = L.nlev;
+  a.nlev = pb->nx + pb->ny - 1;
   a.dx = pb->dx;
   a.dy = pb->dy;
   a.dt = pb->dt;
@@ -253,6 +260,10 @@ void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::S
   a.inner_max_iterations = pb->inner_max_iterations;
   a.inner_tolerance = pb->inner_tolerance;
   a.ring = pb->ring_depth;
+}
+
+void fill_args(pp_ctx *ctx, const pp_problem *pb, const pp_ctx::Layout &L, pp::SysArgs &a) {
+  fill_problem_args(pb, a);
   if (a.ring == 0) {
     auto t = ctx->tuning.find(std::make_pair(pb->nx, pb->ny));
     if (t != ctx->tuning.end()) a.ring = t->second;
@@ -1241,6 +1252,153 @@ int pp_bicgstab(pp_ctx *ctx, int n, const int *row_ptr, const int *col_idx, cons
   PP_HIP_CHECK(hipStreamSynchronize(s));
   ctx->last_error = err;
   return rc;
+}
+
+}  // extern "C"
+
+// ---- pp_solve_device_noalloc: the device solve without any allocation -----------------------------
+//
+// Every buffer the solve kernels need beyond the caller's inputs and output lives in one file-scope
+// device array, `scratch`, of kNoAllocBytes bytes: the argument block, the report, the wavefront
+// layout tables (filled by a kernel on each call), the periodic wrap lines and the FP64 and FP32
+// work arrays, laid out for at most kNoAllocMaxCells cells. Each GPU the library runs on holds its own
+// copy until the process ends. Host code never uses the array's name as an address: the address on
+// each device comes from hipGetSymbolAddress, taken once per device on the first call there.
+
+namespace {
+
+constexpr size_t kNoAllocMaxCells = PP_NOALLOC_MAX_CELLS;
+constexpr size_t kNoAllocStride = kNoAllocMaxCells + pp::kPad;  // work-array stride, as a context's cap + kPad
+constexpr size_t kAlign = 256;
+constexpr size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+constexpr size_t kOffArgs = 0;
+constexpr size_t kOffRep = align_up(kOffArgs + sizeof(pp::SysArgs));
+constexpr size_t kOffBase = align_up(kOffRep + sizeof(pp::DevReport));
+constexpr size_t kBaseEntries = pp::kMaxRows + pp::kMaxSide - 1 + 2 * pp::kLevPad;
+constexpr size_t kOffCell = align_up(kOffBase + kBaseEntries * sizeof(int));
+constexpr size_t kOffWrap64 = align_up(kOffCell + kNoAllocMaxCells * sizeof(uint32_t));
+constexpr size_t kOffWrap32 = align_up(kOffWrap64 + 2 * pp::kMaxSide * sizeof(double));
+constexpr size_t kOffWork = align_up(kOffWrap32 + 2 * pp::kMaxSide * sizeof(float));
+constexpr size_t kOffFwork = align_up(kOffWork + kF64Arrays * kNoAllocStride * sizeof(double));
+constexpr size_t kNoAllocBytes = align_up(kOffFwork + kF32Arrays * kNoAllocStride * sizeof(float));
+
+static_assert(kNoAllocBytes == PP_NOALLOC_SCRATCH_BYTES, "pp_solver.h states the scratch size");
+
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void noalloc_zero_kernel(uint4 *p, size_t words) {
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < words) p[t] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// The device address of `scratch` on the current device, looked up (and the array zeroed) once per
+// device, with that device current.
+int scratch_address(int device, unsigned char **out, pp_report *report) {
+  pp_ctx *ctx = nullptr;
+  if (device < 0 || device >= kMaxDevices) return fail(ctx, report, PP_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    PP_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(ctx, report, PP_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    // the work arrays' padding is read by lanes outside the grid: start the whole array at zero once
+    const size_t words = kNoAllocBytes / sizeof(uint4);
+    hipLaunchKernelGGL(noalloc_zero_kernel, dim3(static_cast<unsigned>((words + 255) / 256)), dim3(256), 0, nullptr,
+                       static_cast<uint4 *>(ptr), words);
+    PP_HIP_CHECK(hipGetLastError());
+    PP_HIP_CHECK(hipDeviceSynchronize());
+    g_scratch_ptr[device] = static_cast<unsigned char *>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return PP_OK;
+}
+
+// Layout tables of an nx x ny grid into the scratch (the values build_layout computes on the host:
+// base[d + kLevPad] = cells on levels < d minus the level's first row, 0 on the margins; cell[pos(i, j)]
+// = (i << 16) | j), and a zeroed report.
+__global__ void noalloc_layout_kernel(int nx, int ny, int *base, uint32_t *cell, pp::DevReport *rep) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nlev = nx + ny - 1, n = nx * ny;
+  if (t < nlev + 2 * pp::kLevPad) {
+    const int d = t - pp::kLevPad;
+    base[t] = (d >= 0 && d < nlev) ? pp::level_offset(d, nx, ny) - pp::level_lo(d, ny) : 0;
+  }
+  if (t < n) {
+    const int i = t / ny, j = t - i * ny;
+    cell[pp::grid_pos(i, j, nx, ny)] = (static_cast<uint32_t>(i) << 16) | static_cast<uint32_t>(j);
+  }
+  if (t == 0) {
+    int *r = reinterpret_cast<int *>(rep);
+    for (size_t k = 0; k < sizeof(pp::DevReport) / sizeof(int); ++k) r[k] = 0;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_solve_device_noalloc(const pp_problem *pb, const double *d_rho, const double *d_u, const double *d_v,
+                            const double *d_p_n, const double *d_p_nm1, double *d_p_out, pp_report *report,
+                            void *stream) {
+  pp_ctx *ctx = nullptr;
+  if (int rc = validate_problem(ctx, pb, report)) return rc;
+  if (static_cast<size_t>(pb->nx) * static_cast<size_t>(pb->ny) > kNoAllocMaxCells)
+    return fail(ctx, report, PP_ERR_ARG,
+                "grid has more cells than the allocation-free solve supports (nx * ny <= " +
+                    std::to_string(kNoAllocMaxCells) + ")");
+  if (!d_rho && !(pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG))
+    return fail(ctx, report, PP_ERR_ARG, "rho is required");
+  if (!d_u || !d_v || !d_p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  int device = 0;
+  PP_HIP_CHECK(hipGetDevice(&device));
+  unsigned char *base = nullptr;
+  if (int rc = scratch_address(device, &base, report)) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+
+  pp::SysArgs a;
+  fill_problem_args(pb, a);
+  spmv_split(pb->nx, pb->ny, a.spmv_split);
+  int *lev = reinterpret_cast<int *>(base + kOffBase);
+  uint32_t *cell = reinterpret_cast<uint32_t *>(base + kOffCell);
+  auto *d_rep = reinterpret_cast<pp::DevReport *>(base + kOffRep);
+  a.lev_base = lev + pp::kLevPad;
+  a.cell = cell;
+  set_work(a, reinterpret_cast<double *>(base + kOffWork), kNoAllocStride);
+  set_fwork(a, reinterpret_cast<float *>(base + kOffFwork), kNoAllocStride);
+  double *w64 = reinterpret_cast<double *>(base + kOffWrap64);
+  float *w32 = reinterpret_cast<float *>(base + kOffWrap32);
+  a.LX = w64;
+  a.LY = w64 + pp::kMaxSide;
+  a.fLX = w32;
+  a.fLY = w32 + pp::kMaxSide;
+  a.rep = d_rep;
+  a.rho = d_rho;
+  a.u = d_u;
+  a.v = d_v;
+  a.p_n = d_p_n;
+  a.p_nm1 = d_p_nm1;
+  a.p_out = d_p_out;
+
+  const int cells = pb->nx * pb->ny, entries = pb->nx + pb->ny - 1 + 2 * pp::kLevPad;
+  const int threads = 256, blocks = (std::max(cells, entries) + threads - 1) / threads;
+  hipLaunchKernelGGL(noalloc_layout_kernel, dim3(blocks), dim3(threads), 0, s, pb->nx, pb->ny, lev, cell, d_rep);
+  PP_HIP_CHECK(hipGetLastError());
+  auto *d_args = reinterpret_cast<pp::SysArgs *>(base + kOffArgs);
+  PP_HIP_CHECK(hipMemcpyAsync(d_args, &a, sizeof(a), hipMemcpyHostToDevice, s));
+  if (pb->precision == PP_MIXED)
+    PP_HIP_CHECK(pp::launch_mixed_solve(d_args, 1, a.ildlt != 0, s));
+  else
+    PP_HIP_CHECK(pp::launch_fp64_solve(d_args, 1, a.ildlt != 0, s, false));
+  pp::DevReport r;
+  PP_HIP_CHECK(hipMemcpyAsync(&r, d_rep, sizeof(r), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
+  copy_report(r, 0.0, report);
+  if (r.status != pp::kOk) return fail(ctx, report, r.status, status_text(r.status, pb->scheme));
+  return PP_OK;
 }
 
 }  // extern "C"

@@ -93,7 +93,7 @@ EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_
            "pp_solve_coeffs", "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device",
            "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device", "pp_solve_monitored",
            "pp_solve_device_monitored", "pp_solve_device_field", "pp_pressure_step", "pp_bicgstab",
-           "pp_bicgstab_device", "pp_set_wait_limit")
+           "pp_bicgstab_device", "pp_set_wait_limit", "pp_solve_device_noalloc")
 
 # One PCG monitor record (pp_monitor_record): the scalar fields of one CGMonitor call.
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
@@ -176,6 +176,9 @@ def lib():
         ip = ctypes.POINTER(ctypes.c_int)
         L.pp_set_tuning.argtypes = [vp, ctypes.c_int, ip, ip, ip]
         L.pp_set_tuning.restype = ctypes.c_int
+        L.pp_solve_device_noalloc.argtypes = [ctypes.POINTER(PPProblem), vp, vp, vp, vp, vp, vp,
+                                              ctypes.POINTER(PPReport), vp]
+        L.pp_solve_device_noalloc.restype = ctypes.c_int
         L.pp_set_wait_limit.argtypes = [vp, ctypes.c_int]
         L.pp_set_wait_limit.restype = ctypes.c_int
         vb = ctypes.POINTER(PPVelocityBC)

@@ -313,6 +313,26 @@ int pp_bicgstab(pp_ctx *ctx, int n, const int *row_ptr, const int *col_idx, cons
 int pp_set_tuning(pp_ctx *ctx, int count, const int *nx, const int *ny, const int *ring_depth);
 
 /*
+ * Allocation-free form of pp_solve_device, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in
+ * the reference. Same problem, inputs, output, report and error texts as pp_solve_device, on the
+ * current device and the given stream (null: the default stream), with the built-in sweep ring depth
+ * and wait budget (no context, so no tuning table or wait limit). Every buffer the kernels need beyond
+ * the inputs and the output is a fixed file-scope device array of PP_NOALLOC_SCRATCH_BYTES bytes that
+ * each GPU the library runs on holds until the process ends; grids of more than PP_NOALLOC_MAX_CELLS
+ * cells are rejected with PP_ERR_ARG. The call makes no hipMalloc, hipFree or other allocation; it
+ * synchronizes the stream to read the report (kernel_ms is 0). The first call on a device looks up the
+ * array's address there and zeroes it (one fill kernel and a device synchronization). All calls on one GPU share that GPU's
+ * array: calls that overlap on one GPU (several host threads or streams at once) race and must be
+ * serialized by the caller.
+ */
+#define PP_NOALLOC_MAX_CELLS 131072
+#define PP_NOALLOC_SCRATCH_BYTES 20635392
+int pp_solve_device_noalloc(const pp_problem *pb, const double *d_rho, const double *d_u, const double *d_v,
+                            const double *d_p_n, const double *d_p_nm1, double *d_p_out, pp_report *report,
+                            void *stream);
+
+/*
  * Watchdog of the wavefront sweeps (no counterpart in the reference, whose sweeps are sequential).
  * The waves of a solve hand boundary values and staged operands to each other through LDS counters;
  * each wait is bounded, and a wait that spins past `sleeps` sleeps fails the solve with
