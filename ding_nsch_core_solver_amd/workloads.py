@@ -272,3 +272,46 @@ def momentum_system(nx=192, ny=192, seed=0, density_ratio=1000.0, viscosity_rati
     cols = np.concatenate([a.ravel() for a in cols])
     vals = np.concatenate([a.ravel() for a in vals])
     return csr_from_coo(n, rows, cols, vals, rhs)
+
+
+# Full-run settings (the reference's Config fields as oracle.ref.RefRun takes them). "width" is the
+# interface width of the initial phase c = 0.5 - 0.5 tanh((r - radius) / width): sqrt(2) Cn gives the
+# profile phi = tanh((R - r) / (sqrt(2) Cn)), c = (1 + phi) / 2, that the workloads state (the
+# reference's own initializer uses 2 sqrt(2) Cn). Unstated values are the reference's defaults
+# (config.hpp: pe 100, ch_inner_iterations 30, ch_tolerance 1e-8, momentum 50 iterations to 1e-8).
+_SQRT2 = 2.0 ** 0.5
+
+
+def _run_spec(**kw):
+    spec = dict(nx=64, ny=64, lx=1.0, ly=1.0, dt=1.0e-4, periodic_x=1, periodic_y=1, re=1.0, ca=1.0, pe=100.0,
+                cn=0.01, density_ratio=1.0, viscosity_ratio=1.0, surface_tension_multiplier=1.0, body_force_x=0.0,
+                body_force_y=0.0, center_x=0.5, center_y=0.5, radius=0.25, width=0.0, ch_inner_iterations=30,
+                ch_tolerance=1.0e-8, momentum_iterations=50, momentum_tolerance=1.0e-8, poisson_iterations=400,
+                pressure_tolerance=1.0e-8)
+    spec.update(kw)
+    return spec
+
+
+# 64x64 periodic static droplet, R = 0.25, Cn = 0.01, density and viscosity ratio 1, surface tension
+# 1 (re = ca = 1), dt = 1e-4, icpcg to 1e-8; 10 steps.
+STATIC_DROPLET_64 = _run_spec(width=_SQRT2 * 0.01)
+STATIC_DROPLET_64_STEPS = 10
+
+# 192x384 planar rising bubble on [0,1]x[0,2]: R = 0.25 at (0.5, 0.5), density ratio 1000, viscosity
+# ratio 100, gravity 0.98 downwards, sigma = 1/(re ca) = 1.96 with re = 10, Cn = 0.01, no-slip walls,
+# dt = 5e-5, pressure to 1e-10; 500 steps. The initial interface has the reference initializer's
+# width 2 sqrt(2) Cn: with sqrt(2) Cn (2.7 cells at this resolution) the first CH step overshoots c to
+# 1.004, the density 1 - 1000 (1 - c) turns negative near the bubble and the reference's own ICC
+# factorization of the pressure matrix fails ("ICC preconditioner construction failed").
+RISING_BUBBLE_192X384 = _run_spec(nx=192, ny=384, ly=2.0, dt=5.0e-5, periodic_x=0, periodic_y=0, re=10.0,
+                                  ca=1.0 / (10.0 * 1.96), density_ratio=1000.0, viscosity_ratio=100.0,
+                                  body_force_y=-0.98, width=0.0, poisson_iterations=20000,
+                                  pressure_tolerance=1.0e-10)
+RISING_BUBBLE_STEPS = 500
+
+# The reference's 128x128 static-droplet validation run whose history is published with it
+# (testcases/92_static_droplet_surface_tension_validation_128.cfg with the icpcg pressure scheme):
+# restarted from its step-5000 snapshot.
+STATIC_DROPLET_128_RESTART = _run_spec(nx=128, ny=128, dt=2.0e-7, re=50.0, ca=1.0, pe=150.0, cn=0.02, radius=0.16,
+                                       ch_inner_iterations=80, momentum_iterations=200, poisson_iterations=1200,
+                                       pressure_tolerance=1.0e-10)

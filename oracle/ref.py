@@ -35,6 +35,23 @@ class RefVelocityBc(ctypes.Structure):
     ]
 
 
+class RefRunConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("dt", ctypes.c_double),
+        ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int),
+        ("re", ctypes.c_double), ("ca", ctypes.c_double), ("pe", ctypes.c_double), ("cn", ctypes.c_double),
+        ("density_ratio", ctypes.c_double), ("viscosity_ratio", ctypes.c_double),
+        ("surface_tension_multiplier", ctypes.c_double),
+        ("body_force_x", ctypes.c_double), ("body_force_y", ctypes.c_double),
+        ("center_x", ctypes.c_double), ("center_y", ctypes.c_double), ("radius", ctypes.c_double),
+        ("width", ctypes.c_double),
+        ("ch_inner_iterations", ctypes.c_int), ("ch_tolerance", ctypes.c_double),
+        ("momentum_iterations", ctypes.c_int), ("momentum_tolerance", ctypes.c_double),
+        ("poisson_iterations", ctypes.c_int), ("pressure_tolerance", ctypes.c_double),
+    ]
+
+
 def available() -> bool:
     return os.path.exists(LIB_PATH)
 
@@ -76,6 +93,22 @@ def lib():
         L.ref_bicgstab.argtypes = [ctypes.c_int, ip, ip, dp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                    ip, dp, dp, dp, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
         L.ref_bicgstab.restype = ctypes.c_int
+        vp = ctypes.c_void_p
+        L.ref_run_create.argtypes = [ctypes.POINTER(RefRunConfig), ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                     ctypes.c_int]
+        L.ref_run_create.restype = vp
+        L.ref_run_destroy.argtypes = [vp]
+        L.ref_run_destroy.restype = None
+        L.ref_run_step_index.argtypes = [vp]
+        L.ref_run_step_index.restype = ctypes.c_int
+        L.ref_run_phase_momentum.argtypes = [vp, dp, ctypes.c_char_p, ctypes.c_int]
+        L.ref_run_phase_momentum.restype = ctypes.c_int
+        L.ref_run_pressure.argtypes = [vp, ip, dp, dp, ctypes.c_char_p, ctypes.c_int]
+        L.ref_run_pressure.restype = ctypes.c_int
+        L.ref_run_finish.argtypes = [vp, dp, ctypes.c_char_p, ctypes.c_int]
+        L.ref_run_finish.restype = ctypes.c_int
+        L.ref_run_field.argtypes = [vp, ctypes.c_char_p, ip]
+        L.ref_run_field.restype = dp
         _lib = L
     return _lib
 
@@ -252,3 +285,69 @@ def bicgstab(system, x0=None, max_iterations: int = 200, tolerance: float = 1e-8
     if timing is not None:
         timing.append(float(seconds[0]))
     return x, it.value, ab.value, rel.value, itr.value
+
+
+class RefRun:
+    """A full run of the reference Solver (solver.cpp:224-283), stepped in three parts so that the
+    pressure step between the momentum predictor and the rest of the timestep can be the reference's
+    own or another one working on the same Field2D buffers:
+
+        run.phase_momentum()   # history shifts, CH step, momentum fluxes and predictor (u_star_, v_star_)
+        run.pressure()         # solve_pressure_correction + apply_pressure_velocity_correction
+        run.finish()           # chemical potential, diagnostics; the step counter advances
+
+    field(name) is a writable (nx + 2g, ny + 2g) view of a Solver Field2D buffer (u is (nx+1) x ny,
+    v nx x (ny+1) before the ghosts)."""
+
+    def __init__(self, spec: dict, scheme: str = "icpcg", restart_file: str | None = None):
+        rc = RefRunConfig()
+        for name, _ in RefRunConfig._fields_:
+            setattr(rc, name, spec[name])
+        err = ctypes.create_string_buffer(512)
+        self.h = lib().ref_run_create(ctypes.byref(rc), scheme.encode(),
+                                      restart_file.encode() if restart_file else None, err, 512)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        self.spec = dict(spec)
+
+    @property
+    def step(self) -> int:
+        return lib().ref_run_step_index(self.h)
+
+    def _call(self, fn, *args):
+        err = ctypes.create_string_buffer(512)
+        if fn(self.h, *args, err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def phase_momentum(self) -> float:
+        sec = ctypes.c_double(0.0)
+        self._call(lib().ref_run_phase_momentum, ctypes.byref(sec))
+        return sec.value
+
+    def pressure(self):
+        """The reference's pressure step. Returns (iterations, relative residual, seconds)."""
+        it, res, sec = ctypes.c_int(0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+        self._call(lib().ref_run_pressure, ctypes.byref(it), ctypes.byref(res), ctypes.byref(sec))
+        return it.value, res.value, sec.value
+
+    def finish(self) -> dict:
+        out = np.zeros(6)
+        self._call(lib().ref_run_finish, _dp(out))
+        return {"ch_residual": out[0], "momentum_residual": out[1], "momentum_iterations": int(out[2]),
+                "ch_iterations": int(out[3]), "mass": out[4], "max_velocity": out[5]}
+
+    def field(self, name: str) -> np.ndarray:
+        dims = (ctypes.c_int * 3)()
+        ptr = lib().ref_run_field(self.h, name.encode(), dims)
+        if not ptr:
+            raise KeyError(name)
+        nx, ny, g = dims[0], dims[1], dims[2]
+        return np.ctypeslib.as_array(ptr, shape=((nx + 2 * g) * (ny + 2 * g),)).reshape(nx + 2 * g, ny + 2 * g)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ref_run_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

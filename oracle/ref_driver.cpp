@@ -395,4 +395,203 @@ int ref_bicgstab(int n, const int *row_ptr, const int *col_idx, const double *va
   }
 }
 
+
+// ---- full runs (solver.cpp:224-283) split at the pressure step ---------------------------------------
+//
+// A reference Solver built from a RefRunConfig and stepped in three parts, so that a caller can put
+// another pressure step between the reference's phase / momentum stages and the rest of the
+// timestep: ref_run_phase_momentum (the history shifts, advance_phase_state, compute_momentum_fluxes
+// and solve_momentum_predictor of advance_one_timestep, solver.cpp:224-270), ref_run_pressure (the
+// reference's own solve_pressure_correction + apply_pressure_velocity_correction, 271-272) and
+// ref_run_finish (refresh_chemical_potential_for_current_phase and compute_diagnostics, 274-282).
+// ref_run_field exposes the Solver's Field2D buffers (ghost layers included) for reading and writing.
+
+struct RefRunConfig {
+  int nx, ny;
+  double lx, ly, dt;
+  int periodic_x, periodic_y;
+  double re, ca, pe, cn, density_ratio, viscosity_ratio, surface_tension_multiplier;
+  double body_force_x, body_force_y;
+  double center_x, center_y, radius;
+  double width;  // > 0: c = 0.5 - 0.5 tanh((r - radius) / width) replaces the initial phase
+  int ch_inner_iterations;
+  double ch_tolerance;
+  int momentum_iterations;
+  double momentum_tolerance;
+  int poisson_iterations;
+  double pressure_tolerance;
+};
+
+struct RefRun {
+  std::unique_ptr<ding::Solver> solver;
+  int step = 0;
+  double ch_residual = 0.0, momentum_residual = 0.0, pressure_residual = 0.0;
+};
+
+void *ref_run_create(const RefRunConfig *rc, const char *scheme, const char *restart_file, char *err, int errlen) {
+  try {
+    ding::Config cfg;
+    cfg.name = "oracle_ref_run";
+    cfg.mode = "coupled";
+    cfg.nx = rc->nx;
+    cfg.ny = rc->ny;
+    cfg.ghost = 3;
+    cfg.lx = rc->lx;
+    cfg.ly = rc->ly;
+    cfg.dt = rc->dt;
+    cfg.periodic_x = rc->periodic_x != 0;
+    cfg.periodic_y = rc->periodic_y != 0;
+    cfg.re = rc->re;
+    cfg.ca = rc->ca;
+    cfg.pe = rc->pe;
+    cfg.cn = rc->cn;
+    cfg.density_ratio = rc->density_ratio;
+    cfg.viscosity_ratio = rc->viscosity_ratio;
+    cfg.surface_tension_multiplier = rc->surface_tension_multiplier;
+    cfg.body_force_x = rc->body_force_x;
+    cfg.body_force_y = rc->body_force_y;
+    cfg.interface_center_x = rc->center_x;
+    cfg.interface_center_y = rc->center_y;
+    cfg.interface_radius = rc->radius;
+    cfg.ch_inner_iterations = rc->ch_inner_iterations;
+    cfg.ch_tolerance = rc->ch_tolerance;
+    cfg.momentum_iterations = rc->momentum_iterations;
+    cfg.momentum_tolerance = rc->momentum_tolerance;
+    cfg.poisson_iterations = rc->poisson_iterations;
+    cfg.pressure_tolerance = rc->pressure_tolerance;
+    cfg.pressure_scheme = scheme ? scheme : "icpcg";
+    cfg.verbose = false;
+    cfg.write_vtk = false;
+    cfg.write_restart = false;
+    if (restart_file && *restart_file) {
+      cfg.restart = true;
+      cfg.restart_file = restart_file;
+    }
+    auto run = std::make_unique<RefRun>();
+    run->solver = std::make_unique<ding::Solver>(cfg);
+    ding::Solver &s = *run->solver;
+    s.initialize();
+    if (s.restarted_from_snapshot_) {
+      run->step = s.restart_step_;
+    } else if (rc->width > 0.0) {
+      // the initial phase with another interface width, then the tail of Solver::initialize()
+      for (int i = 0; i < cfg.nx; ++i)
+        for (int j = 0; j < cfg.ny; ++j) {
+          const double x = (i + 0.5) * s.dx_, y = (j + 0.5) * s.dy_;
+          const double r = std::sqrt((x - rc->center_x) * (x - rc->center_x) + (y - rc->center_y) * (y - rc->center_y));
+          s.c_(i, j) = 0.5 - 0.5 * std::tanh((r - rc->radius) / rc->width);
+        }
+      s.apply_scalar_bc(s.c_);
+      s.c_previous_step_ = s.c_;
+      s.c_two_steps_back_ = s.c_;
+      s.update_materials();
+      s.rho_previous_step_ = s.rho_;
+      s.eta_previous_step_ = s.eta_;
+      s.rho_mid_ = s.rho_;
+      s.eta_mid_ = s.eta_;
+      s.refresh_chemical_potential_for_current_phase();
+      s.clear_surface_tension_force();
+      s.initial_mass_ = s.compute_mass(s.c_);
+    }
+    return run.release();
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return nullptr;
+  }
+}
+
+void ref_run_destroy(void *h) { delete static_cast<RefRun *>(h); }
+
+int ref_run_step_index(void *h) { return static_cast<RefRun *>(h)->step; }
+
+// advance_one_timestep(step) up to the pressure step (solver.cpp:224-269).
+int ref_run_phase_momentum(void *h, double *seconds, char *err, int errlen) {
+  auto *run = static_cast<RefRun *>(h);
+  ding::Solver &s = *run->solver;
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int step = run->step;
+    s.current_step_index_ = step;
+    s.c_two_steps_back_ = s.c_previous_step_;
+    s.c_previous_step_ = s.c_;
+    s.rho_previous_step_ = s.rho_;
+    s.eta_previous_step_ = s.eta_;
+    s.u_previous_step_ = s.u_;
+    s.v_previous_step_ = s.v_;
+    s.current_coupling_iteration_ = 0;
+    const ding::Solver::PhaseStepReport phase = s.advance_phase_state(step);
+    run->ch_residual = phase.ch_residual;
+    ding::Field2D u_adv(s.u_.nx, s.u_.ny, s.u_.ghost, 0.0);
+    ding::Field2D v_adv(s.v_.nx, s.v_.ny, s.v_.ghost, 0.0);
+    s.compute_momentum_fluxes(u_adv, v_adv);
+    run->momentum_residual = s.solve_momentum_predictor(u_adv, v_adv, step);
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// The reference's own pressure step (solver.cpp:271-272).
+int ref_run_pressure(void *h, int *iterations, double *residual, double *seconds, char *err, int errlen) {
+  auto *run = static_cast<RefRun *>(h);
+  ding::Solver &s = *run->solver;
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    run->pressure_residual = s.solve_pressure_correction();
+    s.apply_pressure_velocity_correction();
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (iterations) *iterations = s.last_pressure_iterations_;
+    if (residual) *residual = run->pressure_residual;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// The rest of advance_one_timestep (solver.cpp:274-282); the step counter advances.
+// out[0..5]: ch residual, momentum residual, momentum iterations, CH iterations, mass, max velocity.
+int ref_run_finish(void *h, double *out, char *err, int errlen) {
+  auto *run = static_cast<RefRun *>(h);
+  ding::Solver &s = *run->solver;
+  try {
+    s.refresh_chemical_potential_for_current_phase();
+    s.last_diag_ = s.compute_diagnostics();
+    if (out) {
+      out[0] = run->ch_residual;
+      out[1] = run->momentum_residual;
+      out[2] = s.last_momentum_iterations_;
+      out[3] = s.last_ch_iterations_;
+      out[4] = s.last_diag_.mass;
+      out[5] = s.last_diag_.max_velocity;
+    }
+    ++run->step;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// The Field2D buffer named `name` (nx, ny and ghost of the field in dims[0..2]); null if unknown.
+double *ref_run_field(void *h, const char *name, int *dims) {
+  ding::Solver &s = *static_cast<RefRun *>(h)->solver;
+  const std::map<std::string, ding::Field2D *> fields = {
+      {"c", &s.c_}, {"rho", &s.rho_}, {"rho_mid", &s.rho_mid_}, {"eta", &s.eta_}, {"eta_mid", &s.eta_mid_},
+      {"eta_previous_step", &s.eta_previous_step_}, {"mu", &s.mu_}, {"u", &s.u_}, {"v", &s.v_},
+      {"u_star", &s.u_star_}, {"v_star", &s.v_star_}, {"pressure", &s.pressure_},
+      {"pressure_previous_step", &s.pressure_previous_step_}, {"pressure_correction", &s.pressure_correction_},
+      {"surface_fx_u", &s.surface_fx_u_}, {"surface_fy_v", &s.surface_fy_v_},
+      {"momentum_u_rhs_prev", &s.momentum_u_rhs_prev_}, {"momentum_v_rhs_prev", &s.momentum_v_rhs_prev_}};
+  auto it = fields.find(name);
+  if (it == fields.end()) return nullptr;
+  dims[0] = it->second->nx;
+  dims[1] = it->second->ny;
+  dims[2] = it->second->ghost;
+  return it->second->data.data();
+}
+
 }  // extern "C"
+

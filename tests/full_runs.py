@@ -1,0 +1,83 @@
+"""Full runs of the reference's timestep loop with the HIP pressure step plugged in.
+
+oracle.ref.RefRun steps the reference Solver (compiled from its own sources) in three parts; between
+its CH + momentum stage and the rest of the timestep, `hybrid_run` calls the engine's
+pp_pressure_step on the Solver's own Field2D buffers (rho_mid_, u_star_, v_star_ in; u_, v_,
+pressure_, pressure_previous_step_ and pressure_correction_ out), which is what a reference build
+with a hip_* pressure scheme does at solver.cpp:271-272. `reference_run` is the same loop with the
+reference's own pressure step.
+"""
+import time
+
+import numpy as np
+
+from ding_nsch_core_solver_amd import engine as E
+from ding_nsch_core_solver_amd import workloads as W
+
+
+def pressure_case(spec, step):
+    return W.PressureCase(nx=spec["nx"], ny=spec["ny"], lx=spec["lx"], ly=spec["ly"], dt=spec["dt"],
+                          periodic_x=bool(spec["periodic_x"]), periodic_y=bool(spec["periodic_y"]),
+                          max_iterations=spec["poisson_iterations"], tolerance=spec["pressure_tolerance"],
+                          density_ratio=spec["density_ratio"], step_index=step)
+
+
+def centroid(run):
+    """Centroid of the c = 1 phase (the droplet / bubble) over the interior cells."""
+    spec = run.spec
+    c = run.field("c")[3:-3, 3:-3]
+    dx, dy = spec["lx"] / spec["nx"], spec["ly"] / spec["ny"]
+    x = (np.arange(spec["nx"]) + 0.5) * dx
+    y = (np.arange(spec["ny"]) + 0.5) * dy
+    m = c.sum()
+    return float((c.sum(axis=1) @ x) / m), float((c.sum(axis=0) @ y) / m)
+
+
+def snapshot(run):
+    return {k: run.field(k)[3:-3, 3:-3].copy() for k in ("pressure", "u", "v", "c")} | {"centroid": centroid(run)}
+
+
+def reference_run(ref, spec, steps, scheme="icpcg"):
+    run = ref.RefRun(spec, scheme)
+    log = []
+    for _ in range(steps):
+        run.phase_momentum()
+        it, res, sec = run.pressure()
+        run.finish()
+        log.append({"iterations": it, "residual": res, "pressure_s": sec})
+    out = snapshot(run)
+    run.close()
+    return out, log
+
+
+def hybrid_run(ref, eng, spec, steps, scheme=E.PP_ICPCG, precision=E.PP_FP64, ref_scheme="icpcg"):
+    run = ref.RefRun(spec, ref_scheme)
+    vbc = W.VelocityBC()
+    log = []
+    for _ in range(steps):
+        step = run.step
+        run.phase_momentum()
+        case = pressure_case(spec, step)
+        f = {k: run.field(k) for k in ("rho_mid", "u_star", "v_star", "u", "v", "pressure", "pressure_previous_step",
+                                        "pressure_correction")}
+        t0 = time.perf_counter()
+        p_corr, u, v, p, prev, _, rep = eng.pressure_step(case, vbc, f["rho_mid"], f["u_star"], f["v_star"], f["u"],
+                                                          f["v"], f["pressure"], f["pressure_previous_step"],
+                                                          scheme=scheme, precision=precision, ghost=3)
+        sec = time.perf_counter() - t0
+        for name, new in (("pressure_correction", p_corr), ("u", u), ("v", v), ("pressure", p),
+                          ("pressure_previous_step", prev)):
+            f[name][...] = new.reshape(f[name].shape)
+        run.finish()
+        log.append({"iterations": rep["iterations"], "residual": rep["relative_residual"], "pressure_s": sec,
+                    "kernel_ms": rep["kernel_ms"]})
+    out = snapshot(run)
+    run.close()
+    return out, log
+
+
+def deviation(a, b):
+    """Max-abs relative differences of pressure, u, v and the centroid shift (absolute)."""
+    rel = lambda x, y: float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300))  # noqa: E731
+    return {"pressure": rel(a["pressure"], b["pressure"]), "u": rel(a["u"], b["u"]), "v": rel(a["v"], b["v"]),
+            "centroid": float(np.hypot(a["centroid"][0] - b["centroid"][0], a["centroid"][1] - b["centroid"][1]))}
