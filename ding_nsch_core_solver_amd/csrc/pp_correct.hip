@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "pp_device.hpp"
+#include "pp_fields.hpp"
 #include "pp_kernels.hpp"
 #include "pp_solver.h"
 
@@ -29,26 +30,6 @@ constexpr int kSumStage = kSumChunk + kSumGroup;  // one stage plus the group th
 constexpr int kWaveSize = 64;
 constexpr int kFaceBlock = 256;  // threads per block of the face and history kernels
 constexpr int kMeanSlot = 2;     // work[0], work[1]: boundary speeds; work[2]: the pressure mean
-
-// Field2D element (i, j) of a field with `cols` interior columns and G ghost layers (global
-// address space, so the accesses are global_load / global_store rather than flat ones).
-struct Ghosted {
-  global_ptr<double> f;
-  int cols, G;
-  __device__ Ghosted(double *p, int c, int g) : f(as_global(p)), cols(c), G(g) {}
-  __device__ __forceinline__ __attribute__((address_space(1))) double &operator()(int i, int j) const {
-    return f[static_cast<size_t>(i + G) * (cols + 2 * G) + (j + G)];
-  }
-};
-
-struct GhostedConst {
-  global_ptr<const double> f;
-  int cols, G;
-  __device__ GhostedConst(const double *p, int c, int g) : f(as_global(p)), cols(c), G(g) {}
-  __device__ __forceinline__ double operator()(int i, int j) const {
-    return f[static_cast<size_t>(i + G) * (cols + 2 * G) + (j + G)];
-  }
-};
 
 __device__ __forceinline__ double block_max(double v, double *red) {
 #pragma unroll
@@ -106,88 +87,6 @@ __device__ void pressure_ghosts(const CorrArgs &a, const Ghosted &f) {
         const double to = 2.0 * (static_cast<double>(k) - 0.5) * a.p_value[3] * a.dy;
         f(i, -k) = a.p_type[2] == PP_BC_DIRICHLET ? 2.0 * a.p_value[2] - f(i, k - 1) : f(i, k - 1) + bo;
         f(i, ny - 1 + k) = a.p_type[3] == PP_BC_DIRICHLET ? 2.0 * a.p_value[3] - f(i, ny - k) : f(i, ny - k) + to;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// apply_u_bc (operators.cpp:182-233) on the (nx+1) x ny face field
-__device__ void u_bc(const CorrArgs &a, const Ghosted &u) {
-  const int fnx = a.nx + 1, ny = a.ny, G = a.G;
-  for (int j = threadIdx.x; j < ny; j += kThreads) {
-    if (a.periodic_x) {
-      u(fnx - 1, j) = u(0, j);
-    } else {
-      u(0, j) = a.u_type[0] == PP_BC_DIRICHLET ? a.u_value[0] : u(1, j) + a.u_value[0] * a.dx;
-      u(fnx - 1, j) = a.u_type[1] == PP_BC_DIRICHLET ? a.u_value[1] : u(fnx - 2, j) + a.u_value[1] * a.dx;
-    }
-  }
-  __syncthreads();
-  for (int k = 1; k <= G; ++k) {
-    for (int j = threadIdx.x; j < ny; j += kThreads) {
-      if (a.periodic_x) {
-        u(-k, j) = u(fnx - k - 1, j);
-        u(fnx - 1 + k, j) = u(k, j);
-      } else {
-        const double lo = 2.0 * static_cast<double>(k) * a.u_value[0] * a.dx;
-        const double ro = 2.0 * static_cast<double>(k) * a.u_value[1] * a.dx;
-        u(-k, j) = a.u_type[0] == PP_BC_DIRICHLET ? 2.0 * a.u_value[0] - u(k, j) : u(k, j) + lo;
-        u(fnx - 1 + k, j) = a.u_type[1] == PP_BC_DIRICHLET ? 2.0 * a.u_value[1] - u(fnx - 1 - k, j)
-                                                            : u(fnx - 1 - k, j) + ro;
-      }
-    }
-    __syncthreads();
-    for (int i = -G + static_cast<int>(threadIdx.x); i < fnx + G; i += kThreads) {
-      if (a.periodic_y) {
-        u(i, -k) = u(i, ny - k);
-        u(i, ny - 1 + k) = u(i, k - 1);
-      } else {
-        const double bo = 2.0 * (static_cast<double>(k) - 0.5) * a.u_value[2] * a.dy;
-        const double to = 2.0 * (static_cast<double>(k) - 0.5) * a.u_value[3] * a.dy;
-        u(i, -k) = a.u_type[2] == PP_BC_DIRICHLET ? 2.0 * a.u_value[2] - u(i, k - 1) : u(i, k - 1) + bo;
-        u(i, ny - 1 + k) = a.u_type[3] == PP_BC_DIRICHLET ? 2.0 * a.u_value[3] - u(i, ny - k) : u(i, ny - k) + to;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// apply_v_bc (operators.cpp:239-283) on the nx x (ny+1) face field
-__device__ void v_bc(const CorrArgs &a, const Ghosted &v) {
-  const int nx = a.nx, fny = a.ny + 1, G = a.G;
-  for (int i = threadIdx.x; i < nx; i += kThreads) {
-    if (a.periodic_y) {
-      v(i, fny - 1) = v(i, 0);
-    } else {
-      v(i, 0) = a.v_type[2] == PP_BC_DIRICHLET ? a.v_value[2] : v(i, 1) + a.v_value[2] * a.dy;
-      v(i, fny - 1) = a.v_type[3] == PP_BC_DIRICHLET ? a.v_value[3] : v(i, fny - 2) + a.v_value[3] * a.dy;
-    }
-  }
-  __syncthreads();
-  for (int k = 1; k <= G; ++k) {
-    for (int j = threadIdx.x; j < fny; j += kThreads) {
-      if (a.periodic_x) {
-        v(-k, j) = v(nx - k, j);
-        v(nx - 1 + k, j) = v(k - 1, j);
-      } else {
-        const double lo = 2.0 * (static_cast<double>(k) - 0.5) * a.v_value[0] * a.dx;
-        const double ro = 2.0 * (static_cast<double>(k) - 0.5) * a.v_value[1] * a.dx;
-        v(-k, j) = a.v_type[0] == PP_BC_DIRICHLET ? 2.0 * a.v_value[0] - v(k - 1, j) : v(k - 1, j) + lo;
-        v(nx - 1 + k, j) = a.v_type[1] == PP_BC_DIRICHLET ? 2.0 * a.v_value[1] - v(nx - k, j) : v(nx - k, j) + ro;
-      }
-    }
-    __syncthreads();
-    for (int i = -G + static_cast<int>(threadIdx.x); i < nx + G; i += kThreads) {
-      if (a.periodic_y) {
-        v(i, -k) = v(i, fny - k - 1);
-        v(i, fny - 1 + k) = v(i, k);
-      } else {
-        const double bo = 2.0 * static_cast<double>(k) * a.v_value[2] * a.dy;
-        const double to = 2.0 * static_cast<double>(k) * a.v_value[3] * a.dy;
-        v(i, -k) = a.v_type[2] == PP_BC_DIRICHLET ? 2.0 * a.v_value[2] - v(i, k) : v(i, k) + bo;
-        v(i, fny - 1 + k) = a.v_type[3] == PP_BC_DIRICHLET ? 2.0 * a.v_value[3] - v(i, fny - 1 - k)
-                                                            : v(i, fny - 1 - k) + to;
       }
     }
     __syncthreads();

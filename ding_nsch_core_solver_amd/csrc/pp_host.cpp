@@ -77,6 +77,15 @@ struct pp_ctx {
   int *bicg_stage_idx = nullptr;     // row_ptr then col_idx
   double *bicg_stage_val = nullptr;  // values, rhs, x
 
+  // momentum predictor (pp_momentum_predictor*): per-row entry slots, counts, scan block sums, CSR,
+  // right-hand side and solution for mom_cap rows; host-entry field staging; its own events
+  size_t mom_cap = 0;
+  int *mom_int = nullptr;         // slot_col (9 cap), count (cap), block sums, row_ptr (cap + 1), col (9 cap)
+  double *mom_dbl = nullptr;      // slot_val (9 cap), val (9 cap), rhs (cap), x (cap)
+  size_t mom_stage_cap = 0;
+  double *mom_stage = nullptr;
+  hipEvent_t mom_ev[2] = {nullptr, nullptr};
+
   // PCG monitor log of the host entry point (pp_solve_monitored)
   pp::MonitorRecord *monitor_log = nullptr;
   size_t monitor_cap = 0;       // capacity in records
@@ -718,6 +727,11 @@ void pp_destroy(pp_ctx *ctx) {
   if (ctx->h_bicg) (void)hipHostFree(ctx->h_bicg);
   if (ctx->bicg_ev[0]) (void)hipEventDestroy(ctx->bicg_ev[0]);
   if (ctx->bicg_ev[1]) (void)hipEventDestroy(ctx->bicg_ev[1]);
+  if (ctx->mom_int) (void)hipFree(ctx->mom_int);
+  if (ctx->mom_dbl) (void)hipFree(ctx->mom_dbl);
+  if (ctx->mom_stage) (void)hipFree(ctx->mom_stage);
+  if (ctx->mom_ev[0]) (void)hipEventDestroy(ctx->mom_ev[0]);
+  if (ctx->mom_ev[1]) (void)hipEventDestroy(ctx->mom_ev[1]);
   if (ctx->bicg_stage_idx) (void)hipFree(ctx->bicg_stage_idx);
   if (ctx->bicg_stage_val) (void)hipFree(ctx->bicg_stage_val);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
@@ -1398,6 +1412,192 @@ int pp_solve_device_noalloc(const pp_problem *pb, const double *d_rho, const dou
   PP_HIP_CHECK(hipStreamSynchronize(s));
   copy_report(r, 0.0, report);
   if (r.status != pp::kOk) return fail(ctx, report, r.status, status_text(r.status, pb->scheme));
+  return PP_OK;
+}
+
+}  // extern "C"
+
+// ---- momentum predictor (momentum.cpp:55-85, 157-502) ----------------------------------------------
+
+namespace {
+
+int validate_momentum(pp_ctx *ctx, const pp_momentum_problem *mp, const pp_velocity_bc *vbc, int ghost,
+                      pp_report *report) {
+  if (!ctx) return fail(ctx, report, PP_ERR_ARG, "null context");
+  if (!mp || !vbc) return fail(ctx, report, PP_ERR_ARG, "null momentum problem or velocity boundary conditions");
+  if (mp->nx < 3 || mp->ny < 3) return fail(ctx, report, PP_ERR_ARG, "momentum predictor needs nx, ny >= 3");
+  if (ghost < 1 || ghost > 8) return fail(ctx, report, PP_ERR_ARG, "ghost must be 1..8");
+  for (int k = 0; k < 4; ++k)
+    if (vbc->u_type[k] < PP_BC_UNSET || vbc->u_type[k] > PP_BC_NEUMANN || vbc->v_type[k] < PP_BC_UNSET ||
+        vbc->v_type[k] > PP_BC_NEUMANN)
+      return fail(ctx, report, PP_ERR_ARG, "unknown velocity boundary type");
+  // explicit velocity BCs on periodic axes are rejected by the reference (operators.cpp:11-34)
+  static const char *const kU[4] = {"u_bc_left", "u_bc_right", "u_bc_bottom", "u_bc_top"};
+  static const char *const kV[4] = {"v_bc_left", "v_bc_right", "v_bc_bottom", "v_bc_top"};
+  for (int axis = 0; axis < 2; ++axis) {
+    if (!(axis == 0 ? mp->periodic_x : mp->periodic_y)) continue;
+    const int lo = 2 * axis;
+    const char *names[4] = {kU[lo], kU[lo + 1], kV[lo], kV[lo + 1]};
+    const int types[4] = {vbc->u_type[lo], vbc->u_type[lo + 1], vbc->v_type[lo], vbc->v_type[lo + 1]};
+    for (int k = 0; k < 4; ++k)
+      if (types[k] != PP_BC_UNSET)
+        return fail(ctx, report, PP_ERR_ARG, std::string(names[k]) + " cannot be set when the direction is periodic");
+  }
+  return PP_OK;
+}
+
+int ensure_momentum(pp_ctx *ctx, size_t n, pp_report *report) {
+  if (!ctx->mom_ev[0]) PP_HIP_CHECK(hipEventCreate(&ctx->mom_ev[0]));
+  if (!ctx->mom_ev[1]) PP_HIP_CHECK(hipEventCreate(&ctx->mom_ev[1]));
+  if (n <= ctx->mom_cap) return PP_OK;
+  if (ctx->mom_int) (void)hipFree(ctx->mom_int);
+  if (ctx->mom_dbl) (void)hipFree(ctx->mom_dbl);
+  ctx->mom_int = nullptr;
+  ctx->mom_dbl = nullptr;
+  ctx->mom_cap = 0;
+  const size_t blocks = (n + pp::kMomScanBlock - 1) / pp::kMomScanBlock;
+  PP_HIP_CHECK(hipMalloc(&ctx->mom_int, (2 * pp::kMomRow * n + 2 * n + blocks + 1) * sizeof(int)));
+  PP_HIP_CHECK(hipMalloc(&ctx->mom_dbl, (2 * pp::kMomRow * n + 2 * n) * sizeof(double)));
+  ctx->mom_cap = n;
+  return PP_OK;
+}
+
+// The whole predictor on device Field2D buffers: system, BiCGSTAB from x0 = u / v, scatter.
+int run_momentum(pp_ctx *ctx, const pp_momentum_problem *mp, const pp_velocity_bc *vbc, int G, const double *d_u,
+                 const double *d_v, const double *d_rho, const double *d_eta, const double *d_eta_prev,
+                 const double *d_sfx, const double *d_sfy, double *d_u_prev, double *d_v_prev, double *d_u_star,
+                 double *d_v_star, pp_report *report, hipStream_t s) {
+  pp::MomArgs a{};
+  a.nx = mp->nx;
+  a.ny = mp->ny;
+  a.G = G;
+  a.dx = mp->dx;
+  a.dy = mp->dy;
+  a.dt = mp->dt;
+  a.alpha = 0.5 * mp->dt / mp->re;
+  a.body_force_x = mp->body_force_x;
+  a.body_force_y = mp->body_force_y;
+  a.periodic_x = mp->periodic_x ? 1 : 0;
+  a.periodic_y = mp->periodic_y ? 1 : 0;
+  a.step_index = mp->step_index;
+  for (int k = 0; k < 4; ++k) {  // effective_u_bc / effective_v_bc (operators.cpp:62-112)
+    const bool u_unset = vbc->u_type[k] == PP_BC_UNSET;
+    const double u_default = k == 2 ? vbc->bottom_wall_velocity_x : (k == 3 ? vbc->top_wall_velocity_x : 0.0);
+    a.u_type[k] = u_unset ? PP_BC_DIRICHLET : vbc->u_type[k];
+    a.u_value[k] = u_unset ? u_default : vbc->u_value[k];
+    const bool v_unset = vbc->v_type[k] == PP_BC_UNSET;
+    a.v_type[k] = v_unset ? PP_BC_DIRICHLET : vbc->v_type[k];
+    a.v_value[k] = v_unset ? 0.0 : vbc->v_value[k];
+  }
+  a.iu0 = a.periodic_x ? 0 : 1;
+  a.jv0 = a.periodic_y ? 0 : 1;
+  a.niu = a.nx - a.iu0;
+  a.njv = a.ny - a.jv0;
+  a.nu = a.niu * a.ny;
+  a.n = a.nu + a.nx * a.njv;
+  const size_t n = static_cast<size_t>(a.n);
+  if (int rc = ensure_momentum(ctx, n, report)) return rc;
+  const size_t blocks = (n + pp::kMomScanBlock - 1) / pp::kMomScanBlock;
+  int *ib = ctx->mom_int;
+  a.slot_col = ib;
+  a.count = ib + pp::kMomRow * n;
+  a.block_sum = a.count + n;
+  a.row_ptr = a.block_sum + blocks;
+  a.col = a.row_ptr + n + 1;
+  double *db = ctx->mom_dbl;
+  a.slot_val = db;
+  a.val = db + pp::kMomRow * n;
+  a.rhs = a.val + pp::kMomRow * n;
+  a.x = a.rhs + n;
+  a.u = d_u;
+  a.v = d_v;
+  a.rho = d_rho;
+  a.eta = d_eta;
+  a.eta_prev = d_eta_prev;
+  a.sfx = d_sfx;
+  a.sfy = d_sfy;
+  a.u_prev = d_u_prev;
+  a.v_prev = d_v_prev;
+  a.u_star = d_u_star;
+  a.v_star = d_v_star;
+  PP_HIP_CHECK(hipEventRecord(ctx->mom_ev[0], s));
+  PP_HIP_CHECK(pp::launch_momentum_system(a, s));
+  const int rc = run_bicgstab(ctx, a.n, a.row_ptr, a.col, a.val, a.rhs, a.x, mp->max_iterations, mp->tolerance, false,
+                              report, s);
+  if (rc != PP_OK) return rc;  // the reference throws before writing u_star_ / v_star_
+  PP_HIP_CHECK(pp::launch_momentum_scatter(a, s));
+  PP_HIP_CHECK(hipEventRecord(ctx->mom_ev[1], s));
+  PP_HIP_CHECK(hipEventSynchronize(ctx->mom_ev[1]));
+  float ms = 0.0f;
+  PP_HIP_CHECK(hipEventElapsedTime(&ms, ctx->mom_ev[0], ctx->mom_ev[1]));
+  ctx->last_kernel_ms = ms;
+  if (report) report->kernel_ms = ms;
+  ctx->last_error.clear();
+  return PP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_momentum_predictor_device(pp_ctx *ctx, const pp_momentum_problem *mp, const pp_velocity_bc *vbc, int ghost,
+                                 const double *d_u, const double *d_v, const double *d_rho_mid, const double *d_eta,
+                                 const double *d_eta_prev, const double *d_surface_fx_u, const double *d_surface_fy_v,
+                                 double *d_u_rhs_prev, double *d_v_rhs_prev, double *d_u_star, double *d_v_star,
+                                 pp_report *report, void *stream) {
+  if (int rc = validate_momentum(ctx, mp, vbc, ghost, report)) return rc;
+  if (!d_u || !d_v || !d_rho_mid || !d_eta || !d_eta_prev || !d_surface_fx_u || !d_surface_fy_v || !d_u_rhs_prev ||
+      !d_v_rhs_prev || !d_u_star || !d_v_star)
+    return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return run_momentum(ctx, mp, vbc, ghost, d_u, d_v, d_rho_mid, d_eta, d_eta_prev, d_surface_fx_u, d_surface_fy_v,
+                      d_u_rhs_prev, d_v_rhs_prev, d_u_star, d_v_star, report, s);
+}
+
+int pp_momentum_predictor(pp_ctx *ctx, const pp_momentum_problem *mp, const pp_velocity_bc *vbc, int ghost,
+                          const double *u, const double *v, const double *rho_mid, const double *eta,
+                          const double *eta_prev, const double *surface_fx_u, const double *surface_fy_v,
+                          double *u_rhs_prev, double *v_rhs_prev, double *u_star, double *v_star, pp_report *report) {
+  if (int rc = validate_momentum(ctx, mp, vbc, ghost, report)) return rc;
+  if (!u || !v || !rho_mid || !eta || !eta_prev || !surface_fx_u || !surface_fy_v || !u_rhs_prev || !v_rhs_prev ||
+      !u_star || !v_star)
+    return fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_HIP_CHECK(hipSetDevice(ctx->device));
+  const int G = ghost;
+  const size_t nc = static_cast<size_t>(mp->nx + 2 * G) * (mp->ny + 2 * G);
+  const size_t nu = static_cast<size_t>(mp->nx + 1 + 2 * G) * (mp->ny + 2 * G);
+  const size_t nv = static_cast<size_t>(mp->nx + 2 * G) * (mp->ny + 1 + 2 * G);
+  const size_t need = 3 * nc + 4 * nu + 4 * nv;
+  if (need > ctx->mom_stage_cap) {
+    if (ctx->mom_stage) (void)hipFree(ctx->mom_stage);
+    ctx->mom_stage = nullptr;
+    ctx->mom_stage_cap = 0;
+    PP_HIP_CHECK(hipMalloc(&ctx->mom_stage, need * sizeof(double)));
+    ctx->mom_stage_cap = need;
+  }
+  hipStream_t s = ctx->stream;
+  double *p = ctx->mom_stage;
+  double *d_rho = p, *d_eta = p + nc, *d_eta_prev = p + 2 * nc;
+  double *d_u = p + 3 * nc, *d_sfx = d_u + nu, *d_u_prev = d_sfx + nu, *d_u_star = d_u_prev + nu;
+  double *d_v = d_u_star + nu, *d_sfy = d_v + nv, *d_v_prev = d_sfy + nv, *d_v_star = d_v_prev + nv;
+  const std::pair<double *, const double *> up[] = {{d_rho, rho_mid}, {d_eta, eta}, {d_eta_prev, eta_prev}};
+  for (const auto &f : up) PP_HIP_CHECK(hipMemcpyAsync(f.first, f.second, nc * sizeof(double), hipMemcpyHostToDevice, s));
+  const std::pair<double *, const double *> uu[] = {{d_u, u}, {d_sfx, surface_fx_u}, {d_u_prev, u_rhs_prev}};
+  for (const auto &f : uu) PP_HIP_CHECK(hipMemcpyAsync(f.first, f.second, nu * sizeof(double), hipMemcpyHostToDevice, s));
+  const std::pair<double *, const double *> vv[] = {{d_v, v}, {d_sfy, surface_fy_v}, {d_v_prev, v_rhs_prev}};
+  for (const auto &f : vv) PP_HIP_CHECK(hipMemcpyAsync(f.first, f.second, nv * sizeof(double), hipMemcpyHostToDevice, s));
+  const int rc = run_momentum(ctx, mp, vbc, G, d_u, d_v, d_rho, d_eta, d_eta_prev, d_sfx, d_sfy, d_u_prev, d_v_prev,
+                              d_u_star, d_v_star, report, s);
+  if (rc != PP_OK) {
+    (void)hipStreamSynchronize(s);
+    return rc;
+  }
+  PP_HIP_CHECK(hipMemcpyAsync(u_rhs_prev, d_u_prev, nu * sizeof(double), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipMemcpyAsync(v_rhs_prev, d_v_prev, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipMemcpyAsync(u_star, d_u_star, nu * sizeof(double), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipMemcpyAsync(v_star, d_v_star, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+  PP_HIP_CHECK(hipStreamSynchronize(s));
   return PP_OK;
 }
 

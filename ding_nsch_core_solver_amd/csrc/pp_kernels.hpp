@@ -162,4 +162,38 @@ hipError_t launch_bicg_setup(const BicgArgs &a, hipStream_t stream);
 hipError_t launch_bicg_iteration(const BicgArgs &a, int iter, hipStream_t stream);
 hipError_t launch_bicg_report(const BicgArgs &a, int iterations, hipStream_t stream);
 
+// Momentum predictor (Solver::compute_momentum_fluxes + solve_momentum_predictor) on ghosted Field2D
+// buffers (G layers). Unknowns: the active u faces (i-major; i = iu0 .. nx-1, iu0 = 0 periodic else 1)
+// then the active v faces (j = jv0 .. ny-1); each row keeps up to kMomRow entries.
+constexpr int kMomRow = 9;
+constexpr int kMomScanBlock = 1024;  // rows per block of the row-pointer scan
+struct MomArgs {
+  int nx, ny, G;
+  double dx, dy, dt, alpha;           // alpha = dt / (2 re)
+  double body_force_x, body_force_y;
+  int periodic_x, periodic_y, step_index;
+  int u_type[4];                      // resolved velocity BCs (left, right, bottom, top)
+  double u_value[4];
+  int v_type[4];
+  double v_value[4];
+  int iu0, jv0, niu, njv, nu, n;      // row layout: u rows (i - iu0) * ny + j, v rows nu + i * njv + (j - jv0)
+  const double *u, *v, *rho, *eta, *eta_prev, *sfx, *sfy;
+  double *u_prev, *v_prev;            // in: the previous step's convection; out: this step's
+  double *u_star, *v_star;
+  // work (n rows): merged row entries, counts, CSR, right-hand side, solution (initial guess x0)
+  int *slot_col;
+  double *slot_val;
+  int *count;
+  int *block_sum;                     // per scan block, then its exclusive prefix
+  int *row_ptr, *col;
+  double *val, *rhs, *x;
+};
+
+// Assembly, row-pointer scan and CSR compaction, and this step's convection fields (with their ghost
+// fills) into u_prev / v_prev; all on `stream`. A row whose diagonal was dropped is emitted without
+// it, and the BiCGSTAB setup pass reports it (the reference's finalize_row_maps text).
+hipError_t launch_momentum_system(const MomArgs &a, hipStream_t stream);
+// u_star / v_star from u / v and the solution x, then their ghost fills.
+hipError_t launch_momentum_scatter(const MomArgs &a, hipStream_t stream);
+
 }  // namespace pp

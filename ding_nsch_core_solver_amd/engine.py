@@ -70,6 +70,28 @@ def make_velocity_bc(vbc) -> PPVelocityBC:
     return out
 
 
+class PPMomentumProblem(ctypes.Structure):
+    """pp_momentum_problem: grid, time step, Reynolds number, body force, periodicity, step index
+    and the BiCGSTAB controls of one momentum predictor step."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dt", ctypes.c_double), ("re", ctypes.c_double),
+        ("body_force_x", ctypes.c_double), ("body_force_y", ctypes.c_double),
+        ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int), ("step_index", ctypes.c_int),
+        ("max_iterations", ctypes.c_int), ("tolerance", ctypes.c_double),
+    ]
+
+
+def make_momentum_problem(mc) -> PPMomentumProblem:
+    mp = PPMomentumProblem()
+    mp.nx, mp.ny = mc.nx, mc.ny
+    mp.dx, mp.dy, mp.dt, mp.re = mc.lx / mc.nx, mc.ly / mc.ny, mc.dt, mc.re
+    mp.body_force_x, mp.body_force_y = mc.body_force_x, mc.body_force_y
+    mp.periodic_x, mp.periodic_y = int(mc.periodic_x), int(mc.periodic_y)
+    mp.step_index, mp.max_iterations, mp.tolerance = int(mc.step_index), int(mc.max_iterations), float(mc.tolerance)
+    return mp
+
+
 class PPReport(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int), ("outer_cycles", ctypes.c_int),
@@ -93,7 +115,8 @@ EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_
            "pp_solve_coeffs", "pp_solve_coeffs_device", "pp_adjoint", "pp_adjoint_device", "pp_solve_batched_device",
            "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device", "pp_solve_monitored",
            "pp_solve_device_monitored", "pp_solve_device_field", "pp_pressure_step", "pp_bicgstab",
-           "pp_bicgstab_device", "pp_set_wait_limit", "pp_solve_device_noalloc")
+           "pp_bicgstab_device", "pp_set_wait_limit", "pp_solve_device_noalloc", "pp_momentum_predictor",
+           "pp_momentum_predictor_device")
 
 # One PCG monitor record (pp_monitor_record): the scalar fields of one CGMonitor call.
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
@@ -176,6 +199,13 @@ def lib():
         ip = ctypes.POINTER(ctypes.c_int)
         L.pp_set_tuning.argtypes = [vp, ctypes.c_int, ip, ip, ip]
         L.pp_set_tuning.restype = ctypes.c_int
+        mpp = ctypes.POINTER(PPMomentumProblem)
+        L.pp_momentum_predictor_device.argtypes = [vp, mpp, ctypes.POINTER(PPVelocityBC), ctypes.c_int] + [vp] * 11 + [
+            ctypes.POINTER(PPReport), vp]
+        L.pp_momentum_predictor_device.restype = ctypes.c_int
+        L.pp_momentum_predictor.argtypes = [vp, mpp, ctypes.POINTER(PPVelocityBC), ctypes.c_int] + [dp] * 11 + [
+            ctypes.POINTER(PPReport)]
+        L.pp_momentum_predictor.restype = ctypes.c_int
         L.pp_solve_device_noalloc.argtypes = [ctypes.POINTER(PPProblem), vp, vp, vp, vp, vp, vp,
                                               ctypes.POINTER(PPReport), vp]
         L.pp_solve_device_noalloc.restype = ctypes.c_int
@@ -536,6 +566,37 @@ class Engine:
         rc = self._lib.pp_bicgstab_device(self.h, int(n), d_row_ptr, d_col_idx, d_values, d_rhs, d_x,
                                           int(max_iterations), float(tolerance), int(bool(use_relative)),
                                           ctypes.byref(rep), stream)
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return _report_dict(rep)
+
+    MOMENTUM_INPUTS = ("u", "v", "rho_mid", "eta", "eta_prev", "surface_fx_u", "surface_fy_v")
+
+    def momentum_predictor(self, mc):
+        """The momentum predictor (pp_momentum_predictor) on a workloads.MomentumCase's host Field2D
+        buffers. Returns (u_star, v_star, u_rhs_prev, v_rhs_prev, report): the predictor's fields and
+        this step's convection fields, all ghosted and flattened; the case is not modified."""
+        ins = [np.ascontiguousarray(getattr(mc, k), dtype=np.float64).reshape(-1) for k in self.MOMENTUM_INPUTS]
+        u_prev = np.array(mc.u_prev, dtype=np.float64, copy=True).reshape(-1)
+        v_prev = np.array(mc.v_prev, dtype=np.float64, copy=True).reshape(-1)
+        u_star, v_star = np.zeros_like(u_prev), np.zeros_like(v_prev)
+        rep = PPReport()
+        rc = self._lib.pp_momentum_predictor(self.h, ctypes.byref(make_momentum_problem(mc)),
+                                             ctypes.byref(make_velocity_bc(mc.vbc)), int(mc.ghost),
+                                             *[_dp(a) for a in ins], _dp(u_prev), _dp(v_prev), _dp(u_star),
+                                             _dp(v_star), ctypes.byref(rep))
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return u_star, v_star, u_prev, v_prev, _report_dict(rep)
+
+    def momentum_predictor_device(self, mc, d_u, d_v, d_rho_mid, d_eta, d_eta_prev, d_sfx, d_sfy, d_u_prev, d_v_prev,
+                                  d_u_star, d_v_star, stream=None) -> dict:
+        """Device-pointer form (pp_momentum_predictor_device); d_u_prev / d_v_prev are updated in place."""
+        rep = PPReport()
+        rc = self._lib.pp_momentum_predictor_device(self.h, ctypes.byref(make_momentum_problem(mc)),
+                                                    ctypes.byref(make_velocity_bc(mc.vbc)), int(mc.ghost), d_u, d_v,
+                                                    d_rho_mid, d_eta, d_eta_prev, d_sfx, d_sfy, d_u_prev, d_v_prev,
+                                                    d_u_star, d_v_star, ctypes.byref(rep), stream)
         if rc != 0:
             raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
         return _report_dict(rep)

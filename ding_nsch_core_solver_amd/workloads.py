@@ -315,3 +315,79 @@ RISING_BUBBLE_STEPS = 500
 STATIC_DROPLET_128_RESTART = _run_spec(nx=128, ny=128, dt=2.0e-7, re=50.0, ca=1.0, pe=150.0, cn=0.02, radius=0.16,
                                        ch_inner_iterations=80, momentum_iterations=200, poisson_iterations=1200,
                                        pressure_tolerance=1.0e-10)
+
+
+@dataclasses.dataclass
+class MomentumCase:
+    """Inputs of one momentum predictor step (Solver::compute_momentum_fluxes +
+    solve_momentum_predictor): ghosted Field2D buffers with `ghost` layers, flattened i-major:
+    u ((nx+1) x ny faces) and v (nx x (ny+1) faces) at step n, rho_mid / eta / eta_prev (nx x ny
+    cells: the midpoint density, the viscosity and the previous step's viscosity), the surface
+    tension forces on the u and v faces and the previous step's convection fields u_prev / v_prev
+    (momentum_u_rhs_prev_, momentum_v_rhs_prev_). dt, re, body forces, periodicity, velocity BCs and
+    the BiCGSTAB controls (momentum_iterations, momentum_tolerance) as the reference Config."""
+    nx: int
+    ny: int
+    lx: float = 1.0
+    ly: float = 1.0
+    dt: float = 1.0e-4
+    re: float = 100.0
+    body_force_x: float = 0.0
+    body_force_y: float = 0.0
+    periodic_x: bool = False
+    periodic_y: bool = False
+    vbc: VelocityBC = dataclasses.field(default_factory=VelocityBC)
+    step_index: int = 1
+    max_iterations: int = 50
+    tolerance: float = 1.0e-8
+    ghost: int = 3
+    u: np.ndarray | None = None
+    v: np.ndarray | None = None
+    rho_mid: np.ndarray | None = None
+    eta: np.ndarray | None = None
+    eta_prev: np.ndarray | None = None
+    surface_fx_u: np.ndarray | None = None
+    surface_fy_v: np.ndarray | None = None
+    u_prev: np.ndarray | None = None
+    v_prev: np.ndarray | None = None
+
+    @property
+    def unknowns(self) -> int:
+        nu = (self.nx if self.periodic_x else self.nx - 1) * self.ny
+        nv = self.nx * (self.ny if self.periodic_y else self.ny - 1)
+        return nu + nv
+
+
+def _scalar_ghosts(a: np.ndarray, g: int, periodic_x: bool, periodic_y: bool) -> np.ndarray:
+    """apply_scalar_bc (operators.cpp:130-141): periodic wrap or copy of the edge cell, x then y."""
+    a = np.pad(a, ((g, g), (0, 0)), mode="wrap" if periodic_x else "edge")
+    return np.pad(a, ((0, 0), (g, g)), mode="wrap" if periodic_y else "edge")
+
+
+def momentum_case(nx=192, ny=192, seed=0, density_ratio=1000.0, viscosity_ratio=100.0, radius=0.2, cn=0.01,
+                  lx=1.0, ly=1.0, dt=1.0e-4, re=100.0, body_force_y=0.0, periodic_x=False, periodic_y=False,
+                  vbc=None, step_index=1, velocity_scale=0.1, ghost=3, **kw) -> MomentumCase:
+    """A droplet (density `density_ratio`, viscosity `viscosity_ratio`, as c + (1 - c) ratio with c = 0
+    inside) in a unit bath; u, v, the surface forces and the previous convection fields are seeded
+    N(0, velocity_scale) (ghost layers included: the predictor reads them); rho, eta ghost-filled as
+    apply_scalar_bc leaves them, eta_prev a slightly perturbed eta."""
+    rng = np.random.default_rng(seed)
+    g = ghost
+    x = (np.arange(nx) + 0.5) * lx / nx
+    y = (np.arange(ny) + 0.5) * ly / ny
+    r = np.hypot(x[:, None] - 0.5 * lx, y[None, :] - 0.5 * ly)
+    c = 0.5 - 0.5 * np.tanh((radius - r) / (2.0 * math.sqrt(2.0) * cn))
+    rho = c + (1.0 - c) * density_ratio
+    eta = c + (1.0 - c) * viscosity_ratio
+    eta_prev = eta * (1.0 + 1e-3 * rng.standard_normal(eta.shape))
+    shape_u = (nx + 1 + 2 * g, ny + 2 * g)
+    shape_v = (nx + 2 * g, ny + 1 + 2 * g)
+    noise = lambda shape: velocity_scale * rng.standard_normal(shape)  # noqa: E731
+    return MomentumCase(nx=nx, ny=ny, lx=lx, ly=ly, dt=dt, re=re, body_force_y=body_force_y, periodic_x=periodic_x,
+                        periodic_y=periodic_y, vbc=vbc if vbc is not None else VelocityBC(), step_index=step_index,
+                        ghost=g, u=noise(shape_u).reshape(-1), v=noise(shape_v).reshape(-1),
+                        rho_mid=_scalar_ghosts(rho, g, periodic_x, periodic_y).reshape(-1),
+                        eta=_scalar_ghosts(eta, g, periodic_x, periodic_y).reshape(-1),
+                        eta_prev=_scalar_ghosts(eta_prev, g, periodic_x, periodic_y).reshape(-1),
+                        surface_fx_u=noise(shape_u).reshape(-1), surface_fy_v=noise(shape_v).reshape(-1),
+                        u_prev=noise(shape_u).reshape(-1), v_prev=noise(shape_v).reshape(-1), **kw)

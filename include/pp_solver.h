@@ -313,6 +313,42 @@ int pp_bicgstab(pp_ctx *ctx, int n, const int *row_ptr, const int *col_idx, cons
 int pp_set_tuning(pp_ctx *ctx, int count, const int *nx, const int *ny, const int *ring_depth);
 
 /*
+ * The momentum predictor (Solver::compute_momentum_fluxes + Solver::solve_momentum_predictor,
+ * momentum.cpp:55-85, 157-502; called at solver.cpp:266-267): the convection fields H(u), H(v), the
+ * implicit full-viscous-tensor u-v system with its right-hand side (momentum.cpp:157-468), assembled
+ * on the device into the CSR form finalize_row_maps produces, BiCGSTAB with the diagonal
+ * preconditioner from x0 = u^n (as pp_bicgstab_device), and the scatter into u_star_ / v_star_ with
+ * apply_u_velocity_bc / apply_v_bc. Fields are whole Field2D buffers with `ghost` layers (ghost
+ * cells are read, as the reference reads them): u, u_star, u_rhs_prev, surface_fx_u are
+ * (nx+1) x ny faces; v, v_star, v_rhs_prev, surface_fy_v are nx x (ny+1); rho_mid, eta (the
+ * Solver's eta_), eta_prev (eta_previous_step_) are nx x ny cells. u_rhs_prev / v_rhs_prev hold
+ * momentum_u_rhs_prev_ / momentum_v_rhs_prev_ on entry (read when step_index > 0) and this step's
+ * convection fields on return. The report carries the BiCGSTAB's iterations and residuals (the
+ * reference's return value is absolute_residual); kernel_ms is the device time of the whole step.
+ * Errors carry the reference's texts (BiCGSTAB breakdowns, a dropped diagonal; explicit u/v BCs on
+ * a periodic axis, as validate_boundary_configuration).
+ */
+typedef struct pp_momentum_problem {
+  int nx, ny;
+  double dx, dy, dt;
+  double re;                  /* the viscous weight is alpha = 0.5 dt / re (momentum.cpp:162) */
+  double body_force_x, body_force_y;
+  int periodic_x, periodic_y;
+  int step_index;             /* 0: convection H^n; else 1.5 H^n - 0.5 H^{n-1} */
+  int max_iterations;         /* momentum_iterations */
+  double tolerance;           /* momentum_tolerance (absolute, as momentum.cpp:470-471 passes it) */
+} pp_momentum_problem;
+int pp_momentum_predictor_device(pp_ctx *ctx, const pp_momentum_problem *mp, const pp_velocity_bc *vbc, int ghost,
+                                 const double *d_u, const double *d_v, const double *d_rho_mid, const double *d_eta,
+                                 const double *d_eta_prev, const double *d_surface_fx_u, const double *d_surface_fy_v,
+                                 double *d_u_rhs_prev, double *d_v_rhs_prev, double *d_u_star, double *d_v_star,
+                                 pp_report *report, void *stream);
+int pp_momentum_predictor(pp_ctx *ctx, const pp_momentum_problem *mp, const pp_velocity_bc *vbc, int ghost,
+                          const double *u, const double *v, const double *rho_mid, const double *eta,
+                          const double *eta_prev, const double *surface_fx_u, const double *surface_fy_v,
+                          double *u_rhs_prev, double *v_rhs_prev, double *u_star, double *v_star, pp_report *report);
+
+/*
  * Allocation-free form of pp_solve_device, for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in
  * the reference. Same problem, inputs, output, report and error texts as pp_solve_device, on the

@@ -42,6 +42,26 @@ class OrVelocityBc(ctypes.Structure):
     ]
 
 
+class OrMomentumProblem(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("dt", ctypes.c_double), ("re", ctypes.c_double),
+        ("body_force_x", ctypes.c_double), ("body_force_y", ctypes.c_double),
+        ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int), ("step_index", ctypes.c_int),
+        ("max_iterations", ctypes.c_int), ("tolerance", ctypes.c_double),
+    ]
+
+
+def momentum_problem(mc, struct=None):
+    mp = (struct or OrMomentumProblem)()
+    mp.nx, mp.ny = mc.nx, mc.ny
+    mp.lx, mp.ly, mp.dt, mp.re = mc.lx, mc.ly, mc.dt, mc.re
+    mp.body_force_x, mp.body_force_y = mc.body_force_x, mc.body_force_y
+    mp.periodic_x, mp.periodic_y = int(mc.periodic_x), int(mc.periodic_y)
+    mp.step_index, mp.max_iterations, mp.tolerance = mc.step_index, mc.max_iterations, mc.tolerance
+    return mp
+
+
 def available() -> bool:
     return os.path.exists(LIB_PATH)
 
@@ -96,6 +116,14 @@ def lib():
         L.or_bicgstab.argtypes = [ctypes.c_int, ip, ip, dp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.c_int, ip, dp, dp, dp, ctypes.c_char_p, ctypes.c_int]
         L.or_bicgstab.restype = ctypes.c_int
+        mp_ = ctypes.POINTER(OrMomentumProblem)
+        vb_ = ctypes.POINTER(OrVelocityBc)
+        L.or_momentum_system.argtypes = [mp_, vb_, ctypes.c_int] + [dp] * 9 + [ip, ip, ip, ip, dp, dp, dp,
+                                                                            ctypes.c_char_p, ctypes.c_int]
+        L.or_momentum_system.restype = ctypes.c_int
+        L.or_momentum_predictor.argtypes = [mp_, vb_, ctypes.c_int] + [dp] * 11 + [ctypes.c_int, ip, dp,
+                                                                                 ctypes.c_char_p, ctypes.c_int]
+        L.or_momentum_predictor.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -322,3 +350,47 @@ def bicgstab(system, x0=None, max_iterations: int = 200, tolerance: float = 1e-8
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return x, it.value, ab.value, rel.value, itr.value
+
+
+def _momentum_inputs(mc):
+    return [_f64(getattr(mc, k)) for k in ("u", "v", "rho_mid", "eta", "eta_prev", "surface_fx_u", "surface_fy_v")]
+
+
+def momentum_system(mc):
+    """The momentum predictor's assembled system (momentum.cpp:157-468, see pp_oracle.cpp):
+    a workloads.CsrSystem-like object with row_ptr, col_idx, values, rhs and x0 (the initial guess)."""
+    import types
+    L = lib()
+    ins = _momentum_inputs(mc)
+    prev = [_f64(mc.u_prev), _f64(mc.v_prev)]
+    n, nnz = ctypes.c_int(0), ctypes.c_int(0)
+    err = ctypes.create_string_buffer(512)
+    args = [ctypes.byref(momentum_problem(mc)), ctypes.byref(velocity_bc(mc.vbc)), int(mc.ghost)]
+    args += [_dp(a) for a in ins + prev]
+    rc = L.or_momentum_system(*args, ctypes.byref(n), ctypes.byref(nnz), None, None, None, None, None, err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    rp = np.zeros(n.value + 1, dtype=np.int32)
+    ci = np.zeros(nnz.value, dtype=np.int32)
+    val, rhs, x0 = np.zeros(nnz.value), np.zeros(n.value), np.zeros(n.value)
+    rc = L.or_momentum_system(*args, ctypes.byref(n), ctypes.byref(nnz), _ip(rp), _ip(ci), _dp(val), _dp(rhs),
+                              _dp(x0), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return types.SimpleNamespace(n=n.value, row_ptr=rp, col_idx=ci, values=val, rhs=rhs, x0=x0)
+
+
+def momentum_predictor(mc, reduction: int = SEQUENTIAL):
+    """The whole predictor step. Returns (u_star, v_star, u_prev, v_prev, iterations, absolute
+    residual): the ghosted predictor fields and this step's convection fields."""
+    L = lib()
+    u_prev, v_prev = np.array(mc.u_prev, dtype=np.float64, copy=True), np.array(mc.v_prev, dtype=np.float64, copy=True)
+    u_star, v_star = np.zeros_like(u_prev), np.zeros_like(v_prev)
+    it, ab = ctypes.c_int(0), ctypes.c_double(0.0)
+    err = ctypes.create_string_buffer(512)
+    rc = L.or_momentum_predictor(ctypes.byref(momentum_problem(mc)), ctypes.byref(velocity_bc(mc.vbc)), int(mc.ghost),
+                                 *[_dp(a) for a in _momentum_inputs(mc)], _dp(u_prev), _dp(v_prev), _dp(u_star),
+                                 _dp(v_star), int(reduction), ctypes.byref(it), ctypes.byref(ab), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return u_star, v_star, u_prev, v_prev, it.value, ab.value

@@ -930,6 +930,123 @@ int or_sys_pcg_monitored(void *h, int max_iterations, double tolerance, double *
   }
 }
 
+}  // extern "C"
+
+// Velocity ghost fills on ghosted face fields of G layers (Field2D layout).
+namespace {
+
+// apply_u_bc (operators.cpp:182-233) on an (nx+1) x ny face field; unset edges: no-slip, bottom/top
+// at the wall speed (62-86). Layer by layer: the boundary faces, then x ghosts, then y ghosts over the
+// whole ghosted x range, which reads x-ghost cells of deeper layers before they are rewritten.
+void fill_u_ghosts(const OrProblem &pbr, const OrVelocityBc &vbr, int G, double *u) {
+  const OrProblem *pb = &pbr;
+  const OrVelocityBc *vb = &vbr;
+  const Grid g(pbr);
+  const int nx = pb->nx, ny = pb->ny;
+  const int ld_u = ny + 2 * G;
+  auto uref = [&](double *f, int i, int j) -> double & { return f[static_cast<std::size_t>(i + G) * ld_u + (j + G)]; };
+  auto uval = [&](const double *f, int i, int j) -> double { return f[static_cast<std::size_t>(i + G) * ld_u + (j + G)]; };
+  auto ubc = [&](int s) -> Bc {
+    if (vb->u_type[s] != kUnset) return {vb->u_type[s], vb->u_value[s]};
+    if (s == kBottom) return {kDirichlet, vb->bottom_wall_velocity_x};
+    if (s == kTop) return {kDirichlet, vb->top_wall_velocity_x};
+    return {kDirichlet, 0.0};
+  };
+  {
+    const Bc L = ubc(kLeft), Rb = ubc(kRight), B = ubc(kBottom), T = ubc(kTop);
+    const int fnx = nx + 1;
+    for (int j = 0; j < ny; ++j) {
+      if (pb->periodic_x) {
+        uref(u, fnx - 1, j) = uval(u, 0, j);
+      } else {
+        uref(u, 0, j) = (L.type == kDirichlet) ? L.value : uval(u, 1, j) + L.value * g.dx;
+        uref(u, fnx - 1, j) = (Rb.type == kDirichlet) ? Rb.value : uval(u, fnx - 2, j) + Rb.value * g.dx;
+      }
+    }
+    for (int k = 1; k <= G; ++k) {
+      for (int j = 0; j < ny; ++j) {
+        if (pb->periodic_x) {
+          uref(u, -k, j) = uval(u, fnx - k - 1, j);
+          uref(u, fnx - 1 + k, j) = uval(u, k, j);
+        } else {
+          const double lo = 2.0 * static_cast<double>(k) * L.value * g.dx;
+          const double ro = 2.0 * static_cast<double>(k) * Rb.value * g.dx;
+          uref(u, -k, j) = (L.type == kDirichlet) ? 2.0 * L.value - uval(u, k, j) : uval(u, k, j) + lo;
+          uref(u, fnx - 1 + k, j) = (Rb.type == kDirichlet) ? 2.0 * Rb.value - uval(u, fnx - 1 - k, j)
+                                                            : uval(u, fnx - 1 - k, j) + ro;
+        }
+      }
+      for (int i = -G; i < fnx + G; ++i) {
+        if (pb->periodic_y) {
+          uref(u, i, -k) = uval(u, i, ny - k);
+          uref(u, i, ny - 1 + k) = uval(u, i, k - 1);
+        } else {
+          const double bo = 2.0 * (static_cast<double>(k) - 0.5) * B.value * g.dy;
+          const double to = 2.0 * (static_cast<double>(k) - 0.5) * T.value * g.dy;
+          uref(u, i, -k) = (B.type == kDirichlet) ? 2.0 * B.value - uval(u, i, k - 1) : uval(u, i, k - 1) + bo;
+          uref(u, i, ny - 1 + k) = (T.type == kDirichlet) ? 2.0 * T.value - uval(u, i, ny - k) : uval(u, i, ny - k) + to;
+        }
+      }
+    }
+  }
+}
+
+// apply_v_bc (operators.cpp:239-283) on an nx x (ny+1) face field; unset edges: no-slip (88-112).
+void fill_v_ghosts(const OrProblem &pbr, const OrVelocityBc &vbr, int G, double *v) {
+  const OrProblem *pb = &pbr;
+  const OrVelocityBc *vb = &vbr;
+  const Grid g(pbr);
+  const int nx = pb->nx, ny = pb->ny;
+  const int ld_v = ny + 1 + 2 * G;
+  auto vref = [&](double *f, int i, int j) -> double & { return f[static_cast<std::size_t>(i + G) * ld_v + (j + G)]; };
+  auto vval = [&](const double *f, int i, int j) -> double { return f[static_cast<std::size_t>(i + G) * ld_v + (j + G)]; };
+  auto vbc = [&](int s) -> Bc {
+    if (vb->v_type[s] != kUnset) return {vb->v_type[s], vb->v_value[s]};
+    return {kDirichlet, 0.0};
+  };
+  {
+    const Bc L = vbc(kLeft), Rb = vbc(kRight), B = vbc(kBottom), T = vbc(kTop);
+    const int fny = ny + 1;
+    for (int i = 0; i < nx; ++i) {
+      if (pb->periodic_y) {
+        vref(v, i, fny - 1) = vval(v, i, 0);
+      } else {
+        vref(v, i, 0) = (B.type == kDirichlet) ? B.value : vval(v, i, 1) + B.value * g.dy;
+        vref(v, i, fny - 1) = (T.type == kDirichlet) ? T.value : vval(v, i, fny - 2) + T.value * g.dy;
+      }
+    }
+    for (int k = 1; k <= G; ++k) {
+      for (int j = 0; j < fny; ++j) {
+        if (pb->periodic_x) {
+          vref(v, -k, j) = vval(v, nx - k, j);
+          vref(v, nx - 1 + k, j) = vval(v, k - 1, j);
+        } else {
+          const double lo = 2.0 * (static_cast<double>(k) - 0.5) * L.value * g.dx;
+          const double ro = 2.0 * (static_cast<double>(k) - 0.5) * Rb.value * g.dx;
+          vref(v, -k, j) = (L.type == kDirichlet) ? 2.0 * L.value - vval(v, k - 1, j) : vval(v, k - 1, j) + lo;
+          vref(v, nx - 1 + k, j) = (Rb.type == kDirichlet) ? 2.0 * Rb.value - vval(v, nx - k, j) : vval(v, nx - k, j) + ro;
+        }
+      }
+      for (int i = -G; i < nx + G; ++i) {
+        if (pb->periodic_y) {
+          vref(v, i, -k) = vval(v, i, fny - k - 1);
+          vref(v, i, fny - 1 + k) = vval(v, i, k);
+        } else {
+          const double bo = 2.0 * static_cast<double>(k) * B.value * g.dy;
+          const double to = 2.0 * static_cast<double>(k) * T.value * g.dy;
+          vref(v, i, -k) = (B.type == kDirichlet) ? 2.0 * B.value - vval(v, i, k) : vval(v, i, k) + bo;
+          vref(v, i, fny - 1 + k) = (T.type == kDirichlet) ? 2.0 * T.value - vval(v, i, fny - 1 - k)
+                                                           : vval(v, i, fny - 1 - k) + to;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
 // Velocity correction and pressure-history shift after a pressure solve
 // (Solver::apply_pressure_velocity_correction, operators.cpp:480-534), on ghosted fields of G
 // ghost layers in Field2D layout (element (i, j) of an a x b field at (i + G) * (b + 2G) + j + G):
@@ -1012,92 +1129,8 @@ void or_velocity_correction(int split, const OrProblem *pb, const OrVelocityBc *
       vref(v, i, j) = vval(v_star, i, j) - pb->dt * (im + ex);
     }
 
-  // apply_u_bc (operators.cpp:182-233); unset edges: no-slip, bottom/top at the wall speed (62-86)
-  auto ubc = [&](int s) -> Bc {
-    if (vb->u_type[s] != kUnset) return {vb->u_type[s], vb->u_value[s]};
-    if (s == kBottom) return {kDirichlet, vb->bottom_wall_velocity_x};
-    if (s == kTop) return {kDirichlet, vb->top_wall_velocity_x};
-    return {kDirichlet, 0.0};
-  };
-  {
-    const Bc L = ubc(kLeft), Rb = ubc(kRight), B = ubc(kBottom), T = ubc(kTop);
-    const int fnx = nx + 1;
-    for (int j = 0; j < ny; ++j) {
-      if (pb->periodic_x) {
-        uref(u, fnx - 1, j) = uval(u, 0, j);
-      } else {
-        uref(u, 0, j) = (L.type == kDirichlet) ? L.value : uval(u, 1, j) + L.value * g.dx;
-        uref(u, fnx - 1, j) = (Rb.type == kDirichlet) ? Rb.value : uval(u, fnx - 2, j) + Rb.value * g.dx;
-      }
-    }
-    for (int k = 1; k <= G; ++k) {
-      for (int j = 0; j < ny; ++j) {
-        if (pb->periodic_x) {
-          uref(u, -k, j) = uval(u, fnx - k - 1, j);
-          uref(u, fnx - 1 + k, j) = uval(u, k, j);
-        } else {
-          const double lo = 2.0 * static_cast<double>(k) * L.value * g.dx;
-          const double ro = 2.0 * static_cast<double>(k) * Rb.value * g.dx;
-          uref(u, -k, j) = (L.type == kDirichlet) ? 2.0 * L.value - uval(u, k, j) : uval(u, k, j) + lo;
-          uref(u, fnx - 1 + k, j) = (Rb.type == kDirichlet) ? 2.0 * Rb.value - uval(u, fnx - 1 - k, j)
-                                                            : uval(u, fnx - 1 - k, j) + ro;
-        }
-      }
-      for (int i = -G; i < fnx + G; ++i) {
-        if (pb->periodic_y) {
-          uref(u, i, -k) = uval(u, i, ny - k);
-          uref(u, i, ny - 1 + k) = uval(u, i, k - 1);
-        } else {
-          const double bo = 2.0 * (static_cast<double>(k) - 0.5) * B.value * g.dy;
-          const double to = 2.0 * (static_cast<double>(k) - 0.5) * T.value * g.dy;
-          uref(u, i, -k) = (B.type == kDirichlet) ? 2.0 * B.value - uval(u, i, k - 1) : uval(u, i, k - 1) + bo;
-          uref(u, i, ny - 1 + k) = (T.type == kDirichlet) ? 2.0 * T.value - uval(u, i, ny - k) : uval(u, i, ny - k) + to;
-        }
-      }
-    }
-  }
-  // apply_v_bc (operators.cpp:239-283); unset edges: no-slip (88-112)
-  auto vbc = [&](int s) -> Bc {
-    if (vb->v_type[s] != kUnset) return {vb->v_type[s], vb->v_value[s]};
-    return {kDirichlet, 0.0};
-  };
-  {
-    const Bc L = vbc(kLeft), Rb = vbc(kRight), B = vbc(kBottom), T = vbc(kTop);
-    const int fny = ny + 1;
-    for (int i = 0; i < nx; ++i) {
-      if (pb->periodic_y) {
-        vref(v, i, fny - 1) = vval(v, i, 0);
-      } else {
-        vref(v, i, 0) = (B.type == kDirichlet) ? B.value : vval(v, i, 1) + B.value * g.dy;
-        vref(v, i, fny - 1) = (T.type == kDirichlet) ? T.value : vval(v, i, fny - 2) + T.value * g.dy;
-      }
-    }
-    for (int k = 1; k <= G; ++k) {
-      for (int j = 0; j < fny; ++j) {
-        if (pb->periodic_x) {
-          vref(v, -k, j) = vval(v, nx - k, j);
-          vref(v, nx - 1 + k, j) = vval(v, k - 1, j);
-        } else {
-          const double lo = 2.0 * (static_cast<double>(k) - 0.5) * L.value * g.dx;
-          const double ro = 2.0 * (static_cast<double>(k) - 0.5) * Rb.value * g.dx;
-          vref(v, -k, j) = (L.type == kDirichlet) ? 2.0 * L.value - vval(v, k - 1, j) : vval(v, k - 1, j) + lo;
-          vref(v, nx - 1 + k, j) = (Rb.type == kDirichlet) ? 2.0 * Rb.value - vval(v, nx - k, j) : vval(v, nx - k, j) + ro;
-        }
-      }
-      for (int i = -G; i < nx + G; ++i) {
-        if (pb->periodic_y) {
-          vref(v, i, -k) = vval(v, i, fny - k - 1);
-          vref(v, i, fny - 1 + k) = vval(v, i, k);
-        } else {
-          const double bo = 2.0 * static_cast<double>(k) * B.value * g.dy;
-          const double to = 2.0 * static_cast<double>(k) * T.value * g.dy;
-          vref(v, i, -k) = (B.type == kDirichlet) ? 2.0 * B.value - vval(v, i, k) : vval(v, i, k) + bo;
-          vref(v, i, fny - 1 + k) = (T.type == kDirichlet) ? 2.0 * T.value - vval(v, i, fny - 1 - k)
-                                                           : vval(v, i, fny - 1 - k) + to;
-        }
-      }
-    }
-  }
+  fill_u_ghosts(*pb, *vb, G, u);
+  fill_v_ghosts(*pb, *vb, G, v);
   speeds[1] = speed(u, v);
 
   // pressure history (operators.cpp:524-533): previous <- pressure; pressure <- p_corr (interior),
@@ -1291,6 +1324,368 @@ int or_bicgstab(int n, const int *row_ptr, const int *col_idx, const double *val
     *abs_res = rep.abs_res;
     *rel_res = rep.rel_res;
     *iter_res = rep.iter_res;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+}  // extern "C"
+
+// ---- momentum predictor (Solver::compute_momentum_fluxes + solve_momentum_predictor) -------------
+//
+// momentum.cpp:55-85 (central convection H(u), H(v)), 87-155 (stress divergence), 157-502 (the
+// implicit full-viscous-tensor u-v system with its right-hand side, BiCGSTAB with the diagonal
+// preconditioner, scatter into u_star_ / v_star_ and their ghost fills). Face densities and
+// viscosities as operators.cpp:330-385. Fields are ghosted (G layers, Field2D layout): u (nx+1) x ny,
+// v nx x (ny+1), rho_mid / eta / eta_prev nx x ny (their ghosts filled by the caller, as
+// apply_scalar_bc leaves them), surface forces on the u and v faces.
+//
+// Unknowns: the active u faces (i-major, j inner; i = 1..nx-1 with walls, 0..nx-1 periodic) then the
+// active v faces (j = 1..ny-1 with walls, 0..ny-1 periodic). Each matrix row is the list of
+// contributions the reference inserts into its row map, in its order; a column's value is the sum of
+// its contributions in that order starting from 0.0, and the row is emitted in ascending columns
+// without entries below 1e-18 in magnitude (ch_sparse_krylov.hpp:105-133).
+
+extern "C" {
+
+struct OrMomentumProblem {
+  int nx, ny;
+  double lx, ly, dt, re;
+  double body_force_x, body_force_y;
+  int periodic_x, periodic_y;
+  int step_index;
+  int max_iterations;
+  double tolerance;
+};
+
+}  // extern "C"
+
+namespace {
+
+struct MomentumSystem {
+  int nu = 0, nv = 0;
+  std::vector<int> row_ptr, col;
+  std::vector<double> val, rhs, x0;
+  std::vector<double> u_adv, v_adv;  // ghosted, after their ghost fills
+};
+
+MomentumSystem momentum_system(const OrMomentumProblem &mp, const OrVelocityBc &vb, int G, const double *u,
+                               const double *v, const double *rho, const double *eta, const double *eta_prev,
+                               const double *sfx, const double *sfy, const double *u_prev, const double *v_prev) {
+  const int nx = mp.nx, ny = mp.ny;
+  const bool px = mp.periodic_x != 0, py = mp.periodic_y != 0;
+  const double dx = mp.lx / nx, dy = mp.ly / ny, dt = mp.dt;
+  const int ldc = ny + 2 * G, ldu = ny + 2 * G, ldv = ny + 1 + 2 * G;
+  auto C = [&](const double *f, int i, int j) { return f[static_cast<std::size_t>(i + G) * ldc + (j + G)]; };
+  auto U = [&](const double *f, int i, int j) { return f[static_cast<std::size_t>(i + G) * ldu + (j + G)]; };
+  auto V = [&](const double *f, int i, int j) { return f[static_cast<std::size_t>(i + G) * ldv + (j + G)]; };
+  // face averages (operators.cpp:330-385)
+  auto rho_u = [&](int i, int j) {
+    if (i <= 0) return px ? 0.5 * (C(rho, nx - 1, j) + C(rho, 0, j)) : C(rho, 0, j);
+    if (i >= nx) return px ? 0.5 * (C(rho, nx - 1, j) + C(rho, 0, j)) : C(rho, nx - 1, j);
+    return 0.5 * (C(rho, i - 1, j) + C(rho, i, j));
+  };
+  auto rho_v = [&](int i, int j) {
+    if (j <= 0) return py ? 0.5 * (C(rho, i, ny - 1) + C(rho, i, 0)) : C(rho, i, 0);
+    if (j >= ny) return py ? 0.5 * (C(rho, i, ny - 1) + C(rho, i, 0)) : C(rho, i, ny - 1);
+    return 0.5 * (C(rho, i, j - 1) + C(rho, i, j));
+  };
+  auto eta_u = [&](const double *e, int i, int j) {
+    if (i <= 0) return px ? 0.5 * (C(e, nx - 1, j) + C(e, 0, j)) : C(e, 0, j);
+    if (i >= nx) return px ? 0.5 * (C(e, nx - 1, j) + C(e, 0, j)) : C(e, nx - 1, j);
+    return 0.5 * (C(e, i - 1, j) + C(e, i, j));
+  };
+  auto eta_v = [&](const double *e, int i, int j) {
+    if (j <= 0) return py ? 0.5 * (C(e, i, ny - 1) + C(e, i, 0)) : C(e, i, 0);
+    if (j >= ny) return py ? 0.5 * (C(e, i, ny - 1) + C(e, i, 0)) : C(e, i, ny - 1);
+    return 0.5 * (C(e, i, j - 1) + C(e, i, j));
+  };
+  auto eta_c = [&](const double *e, int i, int j) {
+    return 0.25 * (C(e, i, j) + C(e, i - 1, j) + C(e, i, j - 1) + C(e, i - 1, j - 1));
+  };
+  // convection (momentum.cpp:55-85)
+  auto hu = [&](int i, int j) {
+    const double du_dx = (U(u, i + 1, j) - U(u, i - 1, j)) / (2.0 * dx);
+    const double du_dy = (U(u, i, j + 1) - U(u, i, j - 1)) / (2.0 * dy);
+    const double vf = 0.25 * (V(v, i - 1, j) + V(v, i, j) + V(v, i - 1, j + 1) + V(v, i, j + 1));
+    return U(u, i, j) * du_dx + vf * du_dy;
+  };
+  auto hv = [&](int i, int j) {
+    const double dv_dx = (V(v, i + 1, j) - V(v, i - 1, j)) / (2.0 * dx);
+    const double dv_dy = (V(v, i, j + 1) - V(v, i, j - 1)) / (2.0 * dy);
+    const double uf = 0.25 * (U(u, i, j - 1) + U(u, i, j) + U(u, i + 1, j - 1) + U(u, i + 1, j));
+    return uf * dv_dx + V(v, i, j) * dv_dy;
+  };
+  // stress divergence with the previous step's viscosity (momentum.cpp:91-155)
+  auto visc_u = [&](int i, int j) {
+    const double *e = eta_prev;
+    const double ee = eta_u(e, i + 1, j), ew = eta_u(e, i, j);
+    const double normal = (2.0 * ee * ((U(u, i + 1, j) - U(u, i, j)) / dx) -
+                           2.0 * ew * ((U(u, i, j) - U(u, i - 1, j)) / dx)) / dx;
+    const double en = eta_c(e, i, j + 1), es = eta_c(e, i, j);
+    const double shear = (en * ((U(u, i, j + 1) - U(u, i, j)) / dy) - es * ((U(u, i, j) - U(u, i, j - 1)) / dy)) / dy;
+    const double cross = (en * ((V(v, i, j + 1) - V(v, i - 1, j + 1)) / dx) - es * ((V(v, i, j) - V(v, i - 1, j)) / dx)) / dy;
+    return (normal + shear) + cross;
+  };
+  auto visc_v = [&](int i, int j) {
+    const double *e = eta_prev;
+    const double ee = eta_c(e, i + 1, j), ew = eta_c(e, i, j);
+    const double shear = (ee * ((V(v, i + 1, j) - V(v, i, j)) / dx) - ew * ((V(v, i, j) - V(v, i - 1, j)) / dx)) / dx;
+    const double en = eta_v(e, i, j + 1), es = eta_v(e, i, j);
+    const double normal = (2.0 * en * ((V(v, i, j + 1) - V(v, i, j)) / dy) -
+                           2.0 * es * ((V(v, i, j) - V(v, i, j - 1)) / dy)) / dy;
+    const double cross = (ee * ((U(u, i + 1, j) - U(u, i + 1, j - 1)) / dy) - ew * ((U(u, i, j) - U(u, i, j - 1)) / dy)) / dx;
+    return (shear + normal) + cross;
+  };
+
+  MomentumSystem M;
+  const int iu0 = px ? 0 : 1, iu1 = px ? nx - 1 : nx - 1;  // active u faces i in [iu0, iu1]
+  const int jv0 = py ? 0 : 1, jv1 = py ? ny - 1 : ny - 1;  // active v faces j in [jv0, jv1]
+  const int niu = iu1 - iu0 + 1, njv = jv1 - jv0 + 1;
+  M.nu = niu * ny;
+  M.nv = nx * njv;
+  auto u_active = [&](int i, int j) { return j >= 0 && j < ny && i >= iu0 && i <= iu1; };
+  auto v_active = [&](int i, int j) { return i >= 0 && i < nx && j >= jv0 && j <= jv1; };
+  auto u_row = [&](int i, int j) { return (i - iu0) * ny + j; };
+  auto v_row = [&](int i, int j) { return M.nu + i * njv + (j - jv0); };
+  auto wrap_ui = [&](int i) { return px ? (i < 0 ? nx - 1 : (i >= nx ? 0 : i)) : i; };
+  auto wrap_uj = [&](int j) { return py ? (j < 0 ? ny - 1 : (j >= ny ? 0 : j)) : j; };
+  auto wrap_vi = [&](int i) { return px ? (i < 0 ? nx - 1 : (i >= nx ? 0 : i)) : i; };
+  auto wrap_vj = [&](int j) { return py ? (j < 0 ? ny - 1 : (j >= ny ? 0 : j)) : j; };
+  // effective velocity BCs (operators.cpp:62-112)
+  auto ubc = [&](int s) -> Bc {
+    if (vb.u_type[s] != kUnset) return {vb.u_type[s], vb.u_value[s]};
+    if (s == kBottom) return {kDirichlet, vb.bottom_wall_velocity_x};
+    if (s == kTop) return {kDirichlet, vb.top_wall_velocity_x};
+    return {kDirichlet, 0.0};
+  };
+  auto vbcf = [&](int s) -> Bc {
+    if (vb.v_type[s] != kUnset) return {vb.v_type[s], vb.v_value[s]};
+    return {kDirichlet, 0.0};
+  };
+  const double alpha = 0.5 * dt / mp.re;
+  const int n = M.nu + M.nv;
+  M.rhs.assign(static_cast<std::size_t>(n), 0.0);
+  M.x0.assign(static_cast<std::size_t>(n), 0.0);
+  M.row_ptr.assign(1, 0);
+
+  struct Contribution {
+    int col;
+    double value;
+    bool subtract;  // entry -= value (a Dirichlet wall's reflected neighbour) instead of +=
+  };
+  auto emit = [&](const std::vector<Contribution> &list, int row) {
+    std::vector<std::pair<int, double>> merged;  // columns in first-insertion order, values folded
+    for (const Contribution &c : list) {
+      auto it = std::find_if(merged.begin(), merged.end(), [&](const std::pair<int, double> &m) { return m.first == c.col; });
+      if (it == merged.end()) {
+        merged.push_back({c.col, 0.0});
+        it = merged.end() - 1;
+      }
+      it->second = c.subtract ? it->second - c.value : it->second + c.value;
+    }
+    std::sort(merged.begin(), merged.end());
+    bool diag = false;
+    for (const auto &m : merged) {
+      if (std::abs(m.second) < 1.0e-18) continue;
+      M.col.push_back(m.first);
+      M.val.push_back(m.second);
+      diag = diag || m.first == row;
+    }
+    if (!diag) throw std::runtime_error("sparse matrix assembly failed to produce a diagonal entry");
+    M.row_ptr.push_back(static_cast<int>(M.col.size()));
+  };
+  // neighbour contributions with the boundary handling of momentum.cpp:287-381
+  auto add_u_x = [&](std::vector<Contribution> &list, double &b, int ui, int uj, double value) {
+    if (u_active(ui, uj)) return list.push_back({u_row(ui, uj), value, false});
+    if (!px && (ui <= 0 || ui >= nx)) {
+      const bool left = ui <= 0;
+      const Bc bc = ubc(left ? kLeft : kRight);
+      if (bc.type == kDirichlet) {
+        b -= value * bc.value;
+      } else {
+        list.push_back({u_row(left ? 1 : nx - 1, uj), value, false});
+        b -= value * bc.value * dx;
+      }
+      return;
+    }
+    throw std::runtime_error("unexpected inactive u x-neighbor in momentum predictor");
+  };
+  auto add_u_y = [&](std::vector<Contribution> &list, double &b, int ui, int uj, double value) {
+    if (u_active(ui, uj)) return list.push_back({u_row(ui, uj), value, false});
+    if (!py && (uj < 0 || uj >= ny)) {
+      const bool bottom = uj < 0;
+      const Bc bc = ubc(bottom ? kBottom : kTop);
+      const int jin = bottom ? 0 : ny - 1;
+      if (bc.type == kDirichlet) {
+        list.push_back({u_row(ui, jin), value, true});
+        b -= value * (2.0 * bc.value);
+      } else {
+        list.push_back({u_row(ui, jin), value, false});
+        b -= value * bc.value * dy;
+      }
+      return;
+    }
+    throw std::runtime_error("unexpected inactive u y-neighbor in momentum predictor");
+  };
+  auto add_v_y = [&](std::vector<Contribution> &list, double &b, int vi, int vj, double value) {
+    if (v_active(vi, vj)) return list.push_back({v_row(vi, vj), value, false});
+    if (!py && (vj <= 0 || vj >= ny)) {
+      const bool bottom = vj <= 0;
+      const Bc bc = vbcf(bottom ? kBottom : kTop);
+      if (bc.type == kDirichlet) {
+        b -= value * bc.value;
+      } else {
+        list.push_back({v_row(vi, bottom ? 1 : ny - 1), value, false});
+        b -= value * bc.value * dy;
+      }
+      return;
+    }
+    throw std::runtime_error("unexpected inactive v y-neighbor in momentum predictor");
+  };
+  auto add_v_x = [&](std::vector<Contribution> &list, double &b, int vi, int vj, double value) {
+    if (v_active(vi, vj)) return list.push_back({v_row(vi, vj), value, false});
+    if (!px && (vi < 0 || vi >= nx)) {
+      const bool left = vi < 0;
+      const Bc bc = vbcf(left ? kLeft : kRight);
+      const int iin = left ? 0 : nx - 1;
+      if (bc.type == kDirichlet) {
+        list.push_back({v_row(iin, vj), value, true});
+        b -= value * (2.0 * bc.value);
+      } else {
+        list.push_back({v_row(iin, vj), value, false});
+        b -= value * bc.value * dx;
+      }
+      return;
+    }
+    throw std::runtime_error("unexpected inactive v x-neighbor in momentum predictor");
+  };
+
+  std::vector<Contribution> list;
+  for (int i = iu0; i <= iu1; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const int row = u_row(i, j);
+      const double rf = rho_u(i, j);
+      const double conv = mp.step_index == 0 ? hu(i, j) : 1.5 * hu(i, j) - 0.5 * U(u_prev, i, j);
+      double b = rf * U(u, i, j) + dt * (-rf * conv + rf * mp.body_force_x + U(sfx, i, j)) + alpha * visc_u(i, j);
+      M.x0[static_cast<std::size_t>(row)] = U(u, i, j);
+      const double ee = eta_u(eta, i + 1, j), ew = eta_u(eta, i, j), en = eta_c(eta, i, j + 1), es = eta_c(eta, i, j);
+      const double ce = 2.0 * ee / (dx * dx), cw = 2.0 * ew / (dx * dx), cn = en / (dy * dy), cs = es / (dy * dy);
+      const double xn = en / (dx * dy), xs = es / (dx * dy);
+      list.clear();
+      list.push_back({row, rf + alpha * (ce + cw + cn + cs), false});
+      add_u_x(list, b, wrap_ui(i + 1), j, -alpha * ce);
+      add_u_x(list, b, wrap_ui(i - 1), j, -alpha * cw);
+      add_u_y(list, b, i, wrap_uj(j + 1), -alpha * cn);
+      add_u_y(list, b, i, wrap_uj(j - 1), -alpha * cs);
+      add_v_y(list, b, wrap_vi(i), wrap_vj(j + 1), -alpha * xn);
+      add_v_y(list, b, wrap_vi(i - 1), wrap_vj(j + 1), alpha * xn);
+      add_v_y(list, b, wrap_vi(i), wrap_vj(j), alpha * xs);
+      add_v_y(list, b, wrap_vi(i - 1), wrap_vj(j), -alpha * xs);
+      M.rhs[static_cast<std::size_t>(row)] = b;
+      emit(list, row);
+    }
+  for (int i = 0; i < nx; ++i)
+    for (int j = jv0; j <= jv1; ++j) {
+      const int row = v_row(i, j);
+      const double rf = rho_v(i, j);
+      const double conv = mp.step_index == 0 ? hv(i, j) : 1.5 * hv(i, j) - 0.5 * V(v_prev, i, j);
+      double b = rf * V(v, i, j) + dt * (-rf * conv + rf * mp.body_force_y + V(sfy, i, j)) + alpha * visc_v(i, j);
+      M.x0[static_cast<std::size_t>(row)] = V(v, i, j);
+      const double ee = eta_c(eta, i + 1, j), ew = eta_c(eta, i, j), en = eta_v(eta, i, j + 1), es = eta_v(eta, i, j);
+      const double ce = ee / (dx * dx), cw = ew / (dx * dx), cn = 2.0 * en / (dy * dy), cs = 2.0 * es / (dy * dy);
+      const double xe = ee / (dx * dy), xw = ew / (dx * dy);
+      list.clear();
+      list.push_back({row, rf + alpha * (ce + cw + cn + cs), false});
+      add_v_x(list, b, wrap_vi(i + 1), j, -alpha * ce);
+      add_v_x(list, b, wrap_vi(i - 1), j, -alpha * cw);
+      add_v_y(list, b, i, wrap_vj(j + 1), -alpha * cn);
+      add_v_y(list, b, i, wrap_vj(j - 1), -alpha * cs);
+      add_u_x(list, b, wrap_ui(i + 1), wrap_uj(j), -alpha * xe);
+      add_u_x(list, b, wrap_ui(i + 1), wrap_uj(j - 1), alpha * xe);
+      add_u_x(list, b, wrap_ui(i), wrap_uj(j), alpha * xw);
+      add_u_x(list, b, wrap_ui(i), wrap_uj(j - 1), -alpha * xw);
+      M.rhs[static_cast<std::size_t>(row)] = b;
+      emit(list, row);
+    }
+
+  // the convection fields as stored for the next step: interiors, then apply_u_bc / apply_v_bc
+  // on buffers that start at zero (compute_momentum_fluxes, momentum.cpp:55-85)
+  const OrProblem gp{nx, ny, mp.lx, mp.ly, dt, mp.periodic_x, mp.periodic_y, {0, 0, 0, 0}, {0, 0, 0, 0}, 0, 0.0, 1.0, 0};
+  M.u_adv.assign(static_cast<std::size_t>(nx + 1 + 2 * G) * ldu, 0.0);
+  M.v_adv.assign(static_cast<std::size_t>(nx + 2 * G) * ldv, 0.0);
+  for (int i = 0; i <= nx; ++i)
+    for (int j = 0; j < ny; ++j) M.u_adv[static_cast<std::size_t>(i + G) * ldu + (j + G)] = hu(i, j);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j <= ny; ++j) M.v_adv[static_cast<std::size_t>(i + G) * ldv + (j + G)] = hv(i, j);
+  fill_u_ghosts(gp, vb, G, M.u_adv.data());
+  fill_v_ghosts(gp, vb, G, M.v_adv.data());
+  return M;
+}
+
+}  // namespace
+
+extern "C" {
+
+// The momentum predictor's system: n = active u faces + active v faces; row_ptr (n + 1), col / val
+// (up to 9 n), rhs and x0 (n) are written when not null (sizes from or_momentum_sizes).
+int or_momentum_system(const OrMomentumProblem *mp, const OrVelocityBc *vb, int G, const double *u, const double *v,
+                       const double *rho, const double *eta, const double *eta_prev, const double *sfx,
+                       const double *sfy, const double *u_prev, const double *v_prev, int *n_out, int *nnz_out,
+                       int *row_ptr, int *col, double *val, double *rhs, double *x0, char *err, int errlen) {
+  try {
+    const MomentumSystem M = momentum_system(*mp, *vb, G, u, v, rho, eta, eta_prev, sfx, sfy, u_prev, v_prev);
+    *n_out = M.nu + M.nv;
+    *nnz_out = static_cast<int>(M.col.size());
+    if (row_ptr) std::copy(M.row_ptr.begin(), M.row_ptr.end(), row_ptr);
+    if (col) std::copy(M.col.begin(), M.col.end(), col);
+    if (val) std::copy(M.val.begin(), M.val.end(), val);
+    if (rhs) std::copy(M.rhs.begin(), M.rhs.end(), rhs);
+    if (x0) std::copy(M.x0.begin(), M.x0.end(), x0);
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// The whole predictor: system, BiCGSTAB from x0 = u / v (reduction 0: sequential dot products as the
+// reference, 3: the engine's chunked order), u_star / v_star = u / v with the active faces replaced by
+// the solution and their ghosts refilled (apply_u_velocity_bc / apply_v_bc), and u_prev / v_prev
+// replaced by this step's convection fields (momentum_u_rhs_prev_ = u_adv). Returns 0, or 1 with the
+// reference's text.
+int or_momentum_predictor(const OrMomentumProblem *mp, const OrVelocityBc *vb, int G, const double *u, const double *v,
+                          const double *rho, const double *eta, const double *eta_prev, const double *sfx,
+                          const double *sfy, double *u_prev, double *v_prev, double *u_star, double *v_star,
+                          int reduction, int *iterations, double *abs_res, char *err, int errlen) {
+  try {
+    const int nx = mp->nx, ny = mp->ny;
+    MomentumSystem M = momentum_system(*mp, *vb, G, u, v, rho, eta, eta_prev, sfx, sfy, u_prev, v_prev);
+    const int n = M.nu + M.nv;
+    std::vector<double> x = M.x0;
+    const BicgReport rep = bicgstab(reduction == kChunked ? kChunked : kSequential, n, M.row_ptr.data(), M.col.data(),
+                                    M.val.data(), M.rhs.data(), x, mp->max_iterations, mp->tolerance, false);
+    const int ldu = ny + 2 * G, ldv = ny + 1 + 2 * G;
+    const std::size_t su = static_cast<std::size_t>(nx + 1 + 2 * G) * ldu, sv = static_cast<std::size_t>(nx + 2 * G) * ldv;
+    std::copy(u, u + su, u_star);
+    std::copy(v, v + sv, v_star);
+    const bool px = mp->periodic_x != 0, py = mp->periodic_y != 0;
+    const int iu0 = px ? 0 : 1, jv0 = py ? 0 : 1, njv = py ? ny : ny - 1;
+    for (int i = iu0; i <= nx - 1; ++i)
+      for (int j = 0; j < ny; ++j) u_star[static_cast<std::size_t>(i + G) * ldu + (j + G)] = x[static_cast<std::size_t>((i - iu0) * ny + j)];
+    for (int i = 0; i < nx; ++i)
+      for (int j = jv0; j <= ny - 1; ++j)
+        v_star[static_cast<std::size_t>(i + G) * ldv + (j + G)] = x[static_cast<std::size_t>(M.nu + i * njv + (j - jv0))];
+    const OrProblem gp{nx, ny, mp->lx, mp->ly, mp->dt, mp->periodic_x, mp->periodic_y, {0, 0, 0, 0}, {0, 0, 0, 0}, 0,
+                       0.0, 1.0, 0};
+    fill_u_ghosts(gp, *vb, G, u_star);
+    fill_v_ghosts(gp, *vb, G, v_star);
+    std::copy(M.u_adv.begin(), M.u_adv.end(), u_prev);
+    std::copy(M.v_adv.begin(), M.v_adv.end(), v_prev);
+    *iterations = rep.iterations;
+    *abs_res = rep.abs_res;
     return 0;
   } catch (const std::exception &e) {
     set_err(err, errlen, e.what());

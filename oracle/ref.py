@@ -35,6 +35,16 @@ class RefVelocityBc(ctypes.Structure):
     ]
 
 
+class RefMomentumProblem(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("dt", ctypes.c_double), ("re", ctypes.c_double),
+        ("body_force_x", ctypes.c_double), ("body_force_y", ctypes.c_double),
+        ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int), ("step_index", ctypes.c_int),
+        ("max_iterations", ctypes.c_int), ("tolerance", ctypes.c_double),
+    ]
+
+
 class RefRunConfig(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int), ("ny", ctypes.c_int),
@@ -94,6 +104,10 @@ def lib():
                                    ip, dp, dp, dp, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
         L.ref_bicgstab.restype = ctypes.c_int
         vp = ctypes.c_void_p
+        L.ref_momentum_predictor.argtypes = [ctypes.POINTER(RefMomentumProblem), ctypes.POINTER(RefVelocityBc),
+                                             ctypes.c_int] + [dp] * 11 + [ip, dp, ctypes.c_int, dp, ctypes.c_char_p,
+                                                                          ctypes.c_int]
+        L.ref_momentum_predictor.restype = ctypes.c_int
         L.ref_run_create.argtypes = [ctypes.POINTER(RefRunConfig), ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                      ctypes.c_int]
         L.ref_run_create.restype = vp
@@ -103,6 +117,12 @@ def lib():
         L.ref_run_step_index.restype = ctypes.c_int
         L.ref_run_phase_momentum.argtypes = [vp, dp, ctypes.c_char_p, ctypes.c_int]
         L.ref_run_phase_momentum.restype = ctypes.c_int
+        L.ref_run_phase.argtypes = [vp, dp, ctypes.c_char_p, ctypes.c_int]
+        L.ref_run_phase.restype = ctypes.c_int
+        L.ref_run_momentum.argtypes = [vp, dp, ctypes.c_char_p, ctypes.c_int]
+        L.ref_run_momentum.restype = ctypes.c_int
+        L.ref_run_set_momentum_report.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+        L.ref_run_set_momentum_report.restype = None
         L.ref_run_pressure.argtypes = [vp, ip, dp, dp, ctypes.c_char_p, ctypes.c_int]
         L.ref_run_pressure.restype = ctypes.c_int
         L.ref_run_finish.argtypes = [vp, dp, ctypes.c_char_p, ctypes.c_int]
@@ -324,6 +344,37 @@ class RefRun:
         self._call(lib().ref_run_phase_momentum, ctypes.byref(sec))
         return sec.value
 
+    def phase(self) -> float:
+        """History shifts and the CH stage of the step (advance_phase_state)."""
+        sec = ctypes.c_double(0.0)
+        self._call(lib().ref_run_phase, ctypes.byref(sec))
+        return sec.value
+
+    def momentum(self) -> float:
+        """The reference's momentum predictor of the step (after phase())."""
+        sec = ctypes.c_double(0.0)
+        self._call(lib().ref_run_momentum, ctypes.byref(sec))
+        return sec.value
+
+    def set_momentum_report(self, residual: float, iterations: int):
+        """Record an external predictor's residual and iterations for the step's diagnostics."""
+        lib().ref_run_set_momentum_report(self.h, float(residual), int(iterations))
+
+    def momentum_case(self):
+        """The momentum predictor's inputs of the current step as a workloads.MomentumCase (copies
+        of the Solver's ghosted buffers; call after phase())."""
+        from ding_nsch_core_solver_amd import workloads as W
+        sp = self.spec
+        f = lambda k: self.field(k).reshape(-1).copy()  # noqa: E731
+        return W.MomentumCase(nx=sp["nx"], ny=sp["ny"], lx=sp["lx"], ly=sp["ly"], dt=sp["dt"], re=sp["re"],
+                              body_force_x=sp["body_force_x"], body_force_y=sp["body_force_y"],
+                              periodic_x=bool(sp["periodic_x"]), periodic_y=bool(sp["periodic_y"]),
+                              step_index=self.step, max_iterations=sp["momentum_iterations"],
+                              tolerance=sp["momentum_tolerance"], ghost=3, u=f("u"), v=f("v"), rho_mid=f("rho_mid"),
+                              eta=f("eta"), eta_prev=f("eta_previous_step"), surface_fx_u=f("surface_fx_u"),
+                              surface_fy_v=f("surface_fy_v"), u_prev=f("momentum_u_rhs_prev"),
+                              v_prev=f("momentum_v_rhs_prev"))
+
     def pressure(self):
         """The reference's pressure step. Returns (iterations, relative residual, seconds)."""
         it, res, sec = ctypes.c_int(0), ctypes.c_double(0.0), ctypes.c_double(0.0)
@@ -351,3 +402,37 @@ class RefRun:
 
     def __del__(self):
         self.close()
+
+
+def momentum_predictor(mc, repeats: int = 1, timing: list | None = None):
+    """The reference's compute_momentum_fluxes + solve_momentum_predictor on a workloads.MomentumCase.
+    Returns (u_star, v_star, u_prev, v_prev, iterations, absolute residual) like the oracle's; with a
+    timing list, the median seconds of one call over `repeats` is appended."""
+    b = RefVelocityBc()
+    vbc = mc.vbc
+    for k in range(4):
+        b.u_type[k], b.u_value[k] = int(vbc.u_type[k]), float(vbc.u_value[k])
+        b.v_type[k], b.v_value[k] = int(vbc.v_type[k]), float(vbc.v_value[k])
+    b.bottom_wall_velocity_x = float(vbc.bottom_wall_velocity_x)
+    b.top_wall_velocity_x = float(vbc.top_wall_velocity_x)
+    mp = RefMomentumProblem()
+    mp.nx, mp.ny = mc.nx, mc.ny
+    mp.lx, mp.ly, mp.dt, mp.re = mc.lx, mc.ly, mc.dt, mc.re
+    mp.body_force_x, mp.body_force_y = mc.body_force_x, mc.body_force_y
+    mp.periodic_x, mp.periodic_y = int(mc.periodic_x), int(mc.periodic_y)
+    mp.step_index, mp.max_iterations, mp.tolerance = mc.step_index, mc.max_iterations, mc.tolerance
+    ins = [np.ascontiguousarray(getattr(mc, k), dtype=np.float64)
+           for k in ("u", "v", "rho_mid", "eta", "eta_prev", "surface_fx_u", "surface_fy_v")]
+    u_prev, v_prev = np.array(mc.u_prev, dtype=np.float64, copy=True), np.array(mc.v_prev, dtype=np.float64, copy=True)
+    u_star, v_star = np.zeros_like(u_prev), np.zeros_like(v_prev)
+    it, ab = ctypes.c_int(0), ctypes.c_double(0.0)
+    seconds = np.zeros(1)
+    err = ctypes.create_string_buffer(512)
+    rc = lib().ref_momentum_predictor(ctypes.byref(mp), ctypes.byref(b), int(mc.ghost), *[_dp(a) for a in ins],
+                                      _dp(u_prev), _dp(v_prev), _dp(u_star), _dp(v_star), ctypes.byref(it),
+                                      ctypes.byref(ab), int(repeats), _dp(seconds), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    if timing is not None:
+        timing.append(float(seconds[0]))
+    return u_star, v_star, u_prev, v_prev, it.value, ab.value

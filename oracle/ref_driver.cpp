@@ -504,6 +504,57 @@ void ref_run_destroy(void *h) { delete static_cast<RefRun *>(h); }
 
 int ref_run_step_index(void *h) { return static_cast<RefRun *>(h)->step; }
 
+// advance_one_timestep(step) up to the momentum predictor: history shifts and advance_phase_state
+// (solver.cpp:224-263).
+int ref_run_phase(void *h, double *seconds, char *err, int errlen) {
+  auto *run = static_cast<RefRun *>(h);
+  ding::Solver &s = *run->solver;
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int step = run->step;
+    s.current_step_index_ = step;
+    s.c_two_steps_back_ = s.c_previous_step_;
+    s.c_previous_step_ = s.c_;
+    s.rho_previous_step_ = s.rho_;
+    s.eta_previous_step_ = s.eta_;
+    s.u_previous_step_ = s.u_;
+    s.v_previous_step_ = s.v_;
+    s.current_coupling_iteration_ = 0;
+    run->ch_residual = s.advance_phase_state(step).ch_residual;
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// compute_momentum_fluxes + solve_momentum_predictor of the current step (solver.cpp:265-267).
+int ref_run_momentum(void *h, double *seconds, char *err, int errlen) {
+  auto *run = static_cast<RefRun *>(h);
+  ding::Solver &s = *run->solver;
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    ding::Field2D u_adv(s.u_.nx, s.u_.ny, s.u_.ghost, 0.0);
+    ding::Field2D v_adv(s.v_.nx, s.v_.ny, s.v_.ghost, 0.0);
+    s.compute_momentum_fluxes(u_adv, v_adv);
+    run->momentum_residual = s.solve_momentum_predictor(u_adv, v_adv, run->step);
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// The momentum predictor's result fields after an external predictor wrote u_star_ / v_star_ and the
+// convection fields: records its residual and iteration count for the step's diagnostics.
+void ref_run_set_momentum_report(void *h, double residual, int iterations) {
+  auto *run = static_cast<RefRun *>(h);
+  run->momentum_residual = residual;
+  run->solver->last_momentum_iterations_ = iterations;
+}
+
 // advance_one_timestep(step) up to the pressure step (solver.cpp:224-269).
 int ref_run_phase_momentum(void *h, double *seconds, char *err, int errlen) {
   auto *run = static_cast<RefRun *>(h);
@@ -591,6 +642,92 @@ double *ref_run_field(void *h, const char *name, int *dims) {
   dims[1] = it->second->ny;
   dims[2] = it->second->ghost;
   return it->second->data.data();
+}
+
+
+// Solver::compute_momentum_fluxes + solve_momentum_predictor (momentum.cpp:55-85, 157-502) on the
+// whole ghosted buffers given (G layers): u, v, rho_mid, eta, eta_prev (eta_previous_step_),
+// surface_fx_u, surface_fy_v and u_prev / v_prev (momentum_u/v_rhs_prev_, replaced by this step's
+// convection fields). u_star / v_star receive the predictor's fields with their ghosts.
+struct RefMomentumProblem {
+  int nx, ny;
+  double lx, ly, dt, re;
+  double body_force_x, body_force_y;
+  int periodic_x, periodic_y;
+  int step_index;
+  int max_iterations;
+  double tolerance;
+};
+
+int ref_momentum_predictor(const RefMomentumProblem *mp, const RefVelocityBc *vb, int G, const double *u,
+                           const double *v, const double *rho_mid, const double *eta, const double *eta_prev,
+                           const double *sfx, const double *sfy, double *u_prev, double *v_prev, double *u_star,
+                           double *v_star, int *iterations, double *abs_res, int repeats, double *seconds, char *err,
+                           int errlen) {
+  try {
+    ding::Config cfg;
+    cfg.name = "oracle_ref_momentum";
+    cfg.nx = mp->nx;
+    cfg.ny = mp->ny;
+    cfg.ghost = G;
+    cfg.lx = mp->lx;
+    cfg.ly = mp->ly;
+    cfg.dt = mp->dt;
+    cfg.re = mp->re;
+    cfg.body_force_x = mp->body_force_x;
+    cfg.body_force_y = mp->body_force_y;
+    cfg.periodic_x = mp->periodic_x != 0;
+    cfg.periodic_y = mp->periodic_y != 0;
+    cfg.momentum_iterations = mp->max_iterations;
+    cfg.momentum_tolerance = mp->tolerance;
+    cfg.verbose = false;
+    cfg.write_vtk = false;
+    ding::BoundaryConditionSpec *ub[4] = {&cfg.u_bc_left, &cfg.u_bc_right, &cfg.u_bc_bottom, &cfg.u_bc_top};
+    ding::BoundaryConditionSpec *vbs[4] = {&cfg.v_bc_left, &cfg.v_bc_right, &cfg.v_bc_bottom, &cfg.v_bc_top};
+    for (int k = 0; k < 4; ++k) {
+      ub[k]->type = static_cast<ding::BoundaryConditionType>(vb->u_type[k]);
+      ub[k]->value = vb->u_value[k];
+      vbs[k]->type = static_cast<ding::BoundaryConditionType>(vb->v_type[k]);
+      vbs[k]->value = vb->v_value[k];
+    }
+    cfg.bottom_wall_velocity_x = vb->bottom_wall_velocity_x;
+    cfg.top_wall_velocity_x = vb->top_wall_velocity_x;
+    auto s = std::make_unique<ding::Solver>(cfg);
+    s->validate_boundary_configuration();
+    auto in = [](ding::Field2D &f, const double *src) { std::memcpy(f.data.data(), src, f.data.size() * sizeof(double)); };
+    auto out = [](const ding::Field2D &f, double *dst) { std::memcpy(dst, f.data.data(), f.data.size() * sizeof(double)); };
+    std::vector<double> times;
+    for (int r = 0; r < std::max(1, repeats); ++r) {
+      in(s->u_, u);
+      in(s->v_, v);
+      in(s->rho_mid_, rho_mid);
+      in(s->eta_, eta);
+      in(s->eta_previous_step_, eta_prev);
+      in(s->surface_fx_u_, sfx);
+      in(s->surface_fy_v_, sfy);
+      in(s->momentum_u_rhs_prev_, u_prev);
+      in(s->momentum_v_rhs_prev_, v_prev);
+      const auto t0 = std::chrono::steady_clock::now();
+      ding::Field2D u_adv(s->u_.nx, s->u_.ny, s->u_.ghost, 0.0);
+      ding::Field2D v_adv(s->v_.nx, s->v_.ny, s->v_.ghost, 0.0);
+      s->compute_momentum_fluxes(u_adv, v_adv);
+      *abs_res = s->solve_momentum_predictor(u_adv, v_adv, mp->step_index);
+      times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    if (seconds) {
+      std::sort(times.begin(), times.end());
+      *seconds = times[times.size() / 2];
+    }
+    *iterations = s->last_momentum_iterations_;
+    out(s->u_star_, u_star);
+    out(s->v_star_, v_star);
+    out(s->momentum_u_rhs_prev_, u_prev);
+    out(s->momentum_v_rhs_prev_, v_prev);
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
 }
 
 }  // extern "C"

@@ -41,22 +41,43 @@ def reference_run(ref, spec, steps, scheme="icpcg"):
     run = ref.RefRun(spec, scheme)
     log = []
     for _ in range(steps):
-        run.phase_momentum()
+        run.phase()
+        msec = run.momentum()
         it, res, sec = run.pressure()
-        run.finish()
-        log.append({"iterations": it, "residual": res, "pressure_s": sec})
+        info = run.finish()
+        log.append({"iterations": it, "residual": res, "pressure_s": sec, "momentum_s": msec,
+                    "momentum_iterations": info["momentum_iterations"]})
     out = snapshot(run)
     run.close()
     return out, log
 
 
-def hybrid_run(ref, eng, spec, steps, scheme=E.PP_ICPCG, precision=E.PP_FP64, ref_scheme="icpcg"):
+def hip_momentum(run, eng):
+    """The step's momentum predictor on the device (pp_momentum_predictor) on the Solver's own
+    fields: u_star_, v_star_ and the stored convection fields are written back."""
+    mc = run.momentum_case()
+    u_star, v_star, u_prev, v_prev, rep = eng.momentum_predictor(mc)
+    for name, new in (("u_star", u_star), ("v_star", v_star), ("momentum_u_rhs_prev", u_prev),
+                      ("momentum_v_rhs_prev", v_prev)):
+        f = run.field(name)
+        f[...] = new.reshape(f.shape)
+    run.set_momentum_report(rep["absolute_residual"], rep["iterations"])
+    return rep
+
+
+def hybrid_run(ref, eng, spec, steps, scheme=E.PP_ICPCG, precision=E.PP_FP64, ref_scheme="icpcg", momentum=False):
+    """The reference's loop with the HIP pressure step (and, with momentum=True, the HIP momentum
+    predictor too)."""
     run = ref.RefRun(spec, ref_scheme)
     vbc = W.VelocityBC()
     log = []
     for _ in range(steps):
         step = run.step
-        run.phase_momentum()
+        run.phase()
+        if momentum:
+            mom_rep = hip_momentum(run, eng)
+        else:
+            run.momentum()
         case = pressure_case(spec, step)
         f = {k: run.field(k) for k in ("rho_mid", "u_star", "v_star", "u", "v", "pressure", "pressure_previous_step",
                                         "pressure_correction")}
@@ -69,8 +90,12 @@ def hybrid_run(ref, eng, spec, steps, scheme=E.PP_ICPCG, precision=E.PP_FP64, re
                           ("pressure_previous_step", prev)):
             f[name][...] = new.reshape(f[name].shape)
         run.finish()
-        log.append({"iterations": rep["iterations"], "residual": rep["relative_residual"], "pressure_s": sec,
-                    "kernel_ms": rep["kernel_ms"]})
+        entry = {"iterations": rep["iterations"], "residual": rep["relative_residual"], "pressure_s": sec,
+                 "kernel_ms": rep["kernel_ms"]}
+        if momentum:
+            entry["momentum_iterations"] = mom_rep["iterations"]
+            entry["momentum_kernel_ms"] = mom_rep["kernel_ms"]
+        log.append(entry)
     out = snapshot(run)
     run.close()
     return out, log

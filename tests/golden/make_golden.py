@@ -11,7 +11,9 @@ vector (235-250, 444-513), the scalar CGMonitor log of the reference's PCG on th
 Solver::apply_pressure_velocity_correction (operators.cpp:480-534) on whole ghosted buffers, and
 of apply_pressure_bc (143-181) on a random ghosted field. The bicgstab_*.npz fixtures hold a
 momentum-shaped CSR system with the result of the reference's build_diagonal_preconditioner +
-solve_preconditioned_bicgstab (ch_sparse_krylov.hpp:421-431, 839-963) on it.
+solve_preconditioned_bicgstab (ch_sparse_krylov.hpp:421-431, 839-963) on it. The momentum_*.npz
+fixtures hold the inputs and outputs of Solver::compute_momentum_fluxes + solve_momentum_predictor
+(momentum.cpp:55-85, 157-502) on whole ghosted buffers.
 The files hold data only (no code) and load with numpy.load(allow_pickle=False).
 """
 import os
@@ -140,6 +142,38 @@ def write_bicgstab(name, kw, max_it, tol, rel, x0_seed):
     print(f"{name}: {it} iterations, |r| {ab:.3e}")
 
 
+def momentum_golden_cases():
+    """Momentum predictor steps (workloads.momentum_case) covering walls with wall speed and gravity,
+    Neumann and Dirichlet values, and a periodic stiff step needing many BiCGSTAB iterations."""
+    VB = W.VelocityBC
+    return [("momentum_12x10_lid_gravity", W.momentum_case(12, 10, seed=51, vbc=VB(top_wall_velocity_x=1.0),
+                                                           body_force_y=-0.98, re=10.0)),
+            ("momentum_10x8_neumann_values_step0", W.momentum_case(
+                10, 8, seed=52, step_index=0, vbc=VB(u_type=(N, N, D, N), u_value=(0.3, -0.2, 0.5, 0.1),
+                                                     v_type=(D, N, N, D), v_value=(0.2, 0.4, -0.3, 0.6)))),
+            ("momentum_9x7_periodic_xy_stiff", W.momentum_case(9, 7, seed=53, periodic_x=True, periodic_y=True,
+                                                               dt=1e-2, re=0.5, density_ratio=20.0))]
+
+
+MOMENTUM_SCALARS = ("nx", "ny", "lx", "ly", "dt", "re", "body_force_x", "body_force_y", "periodic_x", "periodic_y",
+                    "step_index", "max_iterations", "tolerance", "ghost")
+MOMENTUM_ARRAYS = ("u", "v", "rho_mid", "eta", "eta_prev", "surface_fx_u", "surface_fy_v", "u_prev", "v_prev")
+
+
+def write_momentum(name, mc):
+    u_star, v_star, u_prev, v_prev, it, ab = ref.momentum_predictor(mc)
+    vb = mc.vbc
+    np.savez_compressed(
+        os.path.join(HERE, name + ".npz"), **{k: getattr(mc, k) for k in MOMENTUM_SCALARS},
+        **{"in_" + k: getattr(mc, k) for k in MOMENTUM_ARRAYS},
+        u_type=np.array(vb.u_type, dtype=np.int32), u_value=np.array(vb.u_value, dtype=np.float64),
+        v_type=np.array(vb.v_type, dtype=np.int32), v_value=np.array(vb.v_value, dtype=np.float64),
+        bottom_wall_velocity_x=vb.bottom_wall_velocity_x, top_wall_velocity_x=vb.top_wall_velocity_x,
+        out_u_star=u_star, out_v_star=v_star, out_u_prev=u_prev, out_v_prev=v_prev, out_iterations=it,
+        out_absolute=ab)
+    print(f"{name}: {it} iterations, |r| {ab:.3e}")
+
+
 if __name__ == "__main__":
     if not ref.available():
         sys.exit("build the reference checker first: make -C oracle ref")
@@ -149,3 +183,5 @@ if __name__ == "__main__":
         write_velcorr(name, case, vbc, scheme, g)
     for args in bicgstab_cases():
         write_bicgstab(*args)
+    for name, mc in momentum_golden_cases():
+        write_momentum(name, mc)

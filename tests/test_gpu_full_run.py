@@ -42,3 +42,19 @@ def test_full_run_with_hip_pressure_step(engine, ref_lib, name, spec, steps):
     assert d64["centroid"] <= 1e-12
     assert max(dmix["pressure"], dmix["u"], dmix["v"]) <= 1e3 * tol
     assert dmix["centroid"] <= 1e-9
+
+
+@pytest.mark.parametrize("name,spec,steps", RUNS, ids=[r[0] for r in RUNS])
+def test_full_run_with_hip_momentum_and_pressure(engine, ref_lib, name, spec, steps):
+    """Both the momentum predictor and the pressure step on the device inside the reference's loop."""
+    ref_out, ref_log = reference_run(ref_lib, spec, steps)
+    out, log = hybrid_run(ref_lib, engine, spec, steps, momentum=True)
+    d = deviation(out, ref_out)
+    print(f"[full run] {name} with hip momentum + pressure: momentum iterations reference "
+          f"{[e['momentum_iterations'] for e in ref_log]}, hip {[e['momentum_iterations'] for e in log]}; "
+          f"deviation {d}; momentum step reference "
+          f"{sum(e['momentum_s'] for e in ref_log) / steps * 1e3:.2f} ms, hip device "
+          f"{sum(e['momentum_kernel_ms'] for e in log) / steps:.2f} ms")
+    assert [e["momentum_iterations"] for e in log] == [e["momentum_iterations"] for e in ref_log]
+    assert [e["iterations"] for e in log] == [e["iterations"] for e in ref_log]
+    assert max(d["pressure"], d["u"], d["v"]) <= 1e-7 and d["centroid"] <= 1e-12
