@@ -17,7 +17,7 @@ INCLUDE = os.path.join(REPO_DIR, "include")
 LIB_NAME = "libppsolve.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
-SOURCES = ["pp_fp64.hip", "pp_adjoint.hip", "pp_correct.hip", "pp_bicgstab.hip", "pp_momentum.hip", "pp_host.cpp"]
+SOURCES = ["pp_fp64.hip", "pp_adjoint.hip", "pp_correct.hip", "pp_bicgstab.hip", "pp_momentum.hip", "pp_ch.hip", "pp_host.cpp"]
 ARCH = os.environ.get("PP_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off keeps the reference's unfused double arithmetic (x86-64 baseline, no FMA).

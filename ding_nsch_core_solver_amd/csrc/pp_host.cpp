@@ -86,6 +86,22 @@ struct pp_ctx {
   double *mom_stage = nullptr;
   hipEvent_t mom_ev[2] = {nullptr, nullptr};
 
+  // Cahn-Hilliard solve (pp_ch_solve*): the cached operator with its schedule, factor and vectors
+  struct ChOperator {
+    bool valid = false;
+    int nx = 0, ny = 0, px = 0, py = 0;
+    double dx = 0.0, dy = 0.0, alpha0 = 0.0, beta1 = 0.0, beta2 = 0.0;
+    int n = 0, nnz = 0, nlev = 0;
+    int kind = 0;                  // PP_CH_ILDLT, PP_CH_DIAGONAL or PP_CH_DIAGONAL_FALLBACK
+    int *d_int = nullptr;          // row_ptr, col, diag_pos, lev_ptr, lev_rows, lt_ptr, lt_row, lt_pos
+    double *d_dbl = nullptr;       // val, lower, D, dinv, r, z, p, ap, y
+    pp::ChState *d_state = nullptr;
+    pp::ChState *h_state = nullptr;  // pinned
+    pp::ChArgs args{};
+    size_t stage_cap = 0;
+    double *stage = nullptr;       // host entry: rhs and c
+  } ch;
+
   // PCG monitor log of the host entry point (pp_solve_monitored)
   pp::MonitorRecord *monitor_log = nullptr;
   size_t monitor_cap = 0;       // capacity in records
@@ -727,6 +743,11 @@ void pp_destroy(pp_ctx *ctx) {
   if (ctx->h_bicg) (void)hipHostFree(ctx->h_bicg);
   if (ctx->bicg_ev[0]) (void)hipEventDestroy(ctx->bicg_ev[0]);
   if (ctx->bicg_ev[1]) (void)hipEventDestroy(ctx->bicg_ev[1]);
+  if (ctx->ch.d_int) (void)hipFree(ctx->ch.d_int);
+  if (ctx->ch.d_dbl) (void)hipFree(ctx->ch.d_dbl);
+  if (ctx->ch.d_state) (void)hipFree(ctx->ch.d_state);
+  if (ctx->ch.h_state) (void)hipHostFree(ctx->ch.h_state);
+  if (ctx->ch.stage) (void)hipFree(ctx->ch.stage);
   if (ctx->mom_int) (void)hipFree(ctx->mom_int);
   if (ctx->mom_dbl) (void)hipFree(ctx->mom_dbl);
   if (ctx->mom_stage) (void)hipFree(ctx->mom_stage);
@@ -1450,6 +1471,11 @@ int ensure_momentum(pp_ctx *ctx, size_t n, pp_report *report) {
   if (!ctx->mom_ev[0]) PP_HIP_CHECK(hipEventCreate(&ctx->mom_ev[0]));
   if (!ctx->mom_ev[1]) PP_HIP_CHECK(hipEventCreate(&ctx->mom_ev[1]));
   if (n <= ctx->mom_cap) return PP_OK;
+  if (ctx->ch.d_int) (void)hipFree(ctx->ch.d_int);
+  if (ctx->ch.d_dbl) (void)hipFree(ctx->ch.d_dbl);
+  if (ctx->ch.d_state) (void)hipFree(ctx->ch.d_state);
+  if (ctx->ch.h_state) (void)hipHostFree(ctx->ch.h_state);
+  if (ctx->ch.stage) (void)hipFree(ctx->ch.stage);
   if (ctx->mom_int) (void)hipFree(ctx->mom_int);
   if (ctx->mom_dbl) (void)hipFree(ctx->mom_dbl);
   ctx->mom_int = nullptr;
@@ -1599,6 +1625,335 @@ int pp_momentum_predictor(pp_ctx *ctx, const pp_momentum_problem *mp, const pp_v
   PP_HIP_CHECK(hipMemcpyAsync(v_star, d_v_star, nv * sizeof(double), hipMemcpyDeviceToHost, s));
   PP_HIP_CHECK(hipStreamSynchronize(s));
   return PP_OK;
+}
+
+}  // extern "C"
+
+// ---- Cahn-Hilliard linear solve (phase_field.cpp:437-503, ch_sparse_krylov.hpp:135-233) ------------
+
+namespace {
+
+// One CSR row built as std::map rows are: each column's value is the sum of its contributions in
+// insertion order starting from 0.0; emitted in ascending columns.
+struct MapRow {
+  std::vector<std::pair<int, double>> e;
+  void add(int col, double value) {
+    for (auto &x : e)
+      if (x.first == col) {
+        x.second += value;
+        return;
+      }
+    e.push_back({col, 0.0 + value});
+  }
+  void sort() { std::sort(e.begin(), e.end()); }
+};
+
+struct HostCsr {
+  std::vector<int> rp{0}, ci, diag;
+  std::vector<double> v;
+  bool push(MapRow &row, int r) {  // finalize_row_maps for one row (drop |v| < 1e-18, find the diagonal)
+    row.sort();
+    int d = -1;
+    for (const auto &x : row.e) {
+      if (std::abs(x.second) < 1.0e-18) continue;
+      if (x.first == r) d = static_cast<int>(v.size());
+      ci.push_back(x.first);
+      v.push_back(x.second);
+    }
+    diag.push_back(d);
+    rp.push_back(static_cast<int>(v.size()));
+    return d >= 0;
+  }
+};
+
+// build_laplacian_matrix, multiply and build_cahn_hilliard_operator (ch_sparse_krylov.hpp:135-233):
+// A = alpha0 I - beta1 L + beta2 L L with L the 5-point Laplacian of the grid.
+bool ch_assemble(int nx, int ny, double dx, double dy, bool px, bool py, double alpha0, double beta1, double beta2,
+                 HostCsr &A) {
+  const double ix = 1.0 / (dx * dx), iy = 1.0 / (dy * dy);
+  HostCsr L;
+  bool ok = true;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      MapRow row;
+      double diag = 0.0;
+      if (px || i > 0) {
+        row.add((px && i == 0 ? nx - 1 : i - 1) * ny + j, ix);
+        diag -= ix;
+      }
+      if (px || i < nx - 1) {
+        row.add((px && i == nx - 1 ? 0 : i + 1) * ny + j, ix);
+        diag -= ix;
+      }
+      if (py || j > 0) {
+        row.add(i * ny + (py && j == 0 ? ny - 1 : j - 1), iy);
+        diag -= iy;
+      }
+      if (py || j < ny - 1) {
+        row.add(i * ny + (py && j == ny - 1 ? 0 : j + 1), iy);
+        diag -= iy;
+      }
+      row.add(i * ny + j, diag);
+      ok = L.push(row, i * ny + j) && ok;
+    }
+  const int n = nx * ny;
+  for (int r = 0; r < n; ++r) {
+    MapRow l2;
+    for (int pa = L.rp[r]; pa < L.rp[r + 1]; ++pa) {
+      const int mid = L.ci[pa];
+      for (int pb = L.rp[mid]; pb < L.rp[mid + 1]; ++pb) l2.add(L.ci[pb], L.v[pa] * L.v[pb]);
+    }
+    l2.sort();
+    MapRow row;
+    row.add(r, alpha0);
+    for (int p = L.rp[r]; p < L.rp[r + 1]; ++p) row.add(L.ci[p], -beta1 * L.v[p]);
+    for (const auto &x : l2.e)
+      if (!(std::abs(x.second) < 1.0e-18)) row.add(x.first, beta2 * x.second);
+    ok = A.push(row, r) && ok;
+  }
+  return ok;
+}
+
+int validate_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report) {
+  pp_report *rep = nullptr;
+  (void)rep;
+  auto bad = [&](const char *msg) {
+    if (ctx) ctx->last_error = msg;
+    if (report) {
+      report->status = PP_ERR_ARG;
+      std::snprintf(report->message, sizeof(report->message), "%s", msg);
+    }
+    return PP_ERR_ARG;
+  };
+  if (!ctx) return bad("null context");
+  if (!cp) return bad("null problem");
+  if (cp->nx < 1 || cp->ny < 1) return bad("cannot solve an empty linear system");
+  return PP_OK;
+}
+
+int ch_fail(pp_ctx *ctx, pp_ch_report *report, int status, const std::string &msg) {
+  ctx->last_error = msg;
+  if (report) {
+    report->status = status;
+    std::snprintf(report->message, sizeof(report->message), "%s", msg.c_str());
+  }
+  return status;
+}
+
+#define PP_CH_CHECK(expr)                                                                            \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if (e_ != hipSuccess) return ch_fail(ctx, report, PP_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// The operator, its level schedule, lower-transpose lists and factor, rebuilt when the grid or the
+// coefficients change; alpha0 within 1e-14 max(1, |alpha0|) of the cached one reuses the cache, as
+// ensure_ch_linear_system does (phase_field.cpp:121-135).
+int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStream_t s) {
+  auto &C = ctx->ch;
+  const bool same = C.valid && C.nx == cp->nx && C.ny == cp->ny && C.px == (cp->periodic_x != 0) &&
+                    C.py == (cp->periodic_y != 0) && C.dx == cp->dx && C.dy == cp->dy && C.beta1 == cp->beta1 &&
+                    C.beta2 == cp->beta2 &&
+                    std::abs(cp->alpha0 - C.alpha0) <= 1.0e-14 * std::max(1.0, std::abs(cp->alpha0));
+  if (same) return PP_OK;
+  C.valid = false;
+  HostCsr A;
+  if (!ch_assemble(cp->nx, cp->ny, cp->dx, cp->dy, cp->periodic_x != 0, cp->periodic_y != 0, cp->alpha0, cp->beta1,
+                   cp->beta2, A))
+    return ch_fail(ctx, report, PP_ERR_NODIAG, "sparse matrix assembly failed to produce a diagonal entry");
+  const int n = cp->nx * cp->ny, nnz = static_cast<int>(A.v.size());
+  // levels: 1 + the largest level of the row's lower columns
+  std::vector<int> lvl(static_cast<size_t>(n), 0);
+  int nlev = 0;
+  for (int r = 0; r < n; ++r) {
+    int l = 0;
+    for (int p = A.rp[r]; p < A.rp[r + 1] && A.ci[p] < r; ++p) l = std::max(l, lvl[static_cast<size_t>(A.ci[p])] + 1);
+    lvl[static_cast<size_t>(r)] = l;
+    nlev = std::max(nlev, l + 1);
+  }
+  std::vector<int> lev_ptr(static_cast<size_t>(nlev) + 1, 0), lev_rows(static_cast<size_t>(n));
+  for (int r = 0; r < n; ++r) ++lev_ptr[static_cast<size_t>(lvl[static_cast<size_t>(r)]) + 1];
+  for (int k = 0; k < nlev; ++k) lev_ptr[static_cast<size_t>(k) + 1] += lev_ptr[static_cast<size_t>(k)];
+  {
+    std::vector<int> at(lev_ptr.begin(), lev_ptr.end() - 1);
+    for (int r = 0; r < n; ++r) lev_rows[static_cast<size_t>(at[static_cast<size_t>(lvl[static_cast<size_t>(r)])]++)] = r;
+  }
+  // children of each row in ascending order (lower_transpose, ch_sparse_krylov.hpp:322-332)
+  std::vector<int> lt_ptr(static_cast<size_t>(n) + 1, 0), lt_row, lt_pos;
+  for (int r = 0; r < n; ++r)
+    for (int p = A.rp[r]; p < A.rp[r + 1] && A.ci[p] < r; ++p) ++lt_ptr[static_cast<size_t>(A.ci[p]) + 1];
+  for (int r = 0; r < n; ++r) lt_ptr[static_cast<size_t>(r) + 1] += lt_ptr[static_cast<size_t>(r)];
+  lt_row.resize(static_cast<size_t>(lt_ptr[static_cast<size_t>(n)]));
+  lt_pos.resize(lt_row.size());
+  {
+    std::vector<int> at(lt_ptr.begin(), lt_ptr.end() - 1);
+    for (int r = 0; r < n; ++r)
+      for (int p = A.rp[r]; p < A.rp[r + 1] && A.ci[p] < r; ++p) {
+        const int k = at[static_cast<size_t>(A.ci[p])]++;
+        lt_row[static_cast<size_t>(k)] = r;
+        lt_pos[static_cast<size_t>(k)] = p;
+      }
+  }
+  // device buffers
+  if (C.d_int) (void)hipFree(C.d_int);
+  if (C.d_dbl) (void)hipFree(C.d_dbl);
+  C.d_int = nullptr;
+  C.d_dbl = nullptr;
+  const size_t nl = lt_row.size();
+  const size_t ints = (n + 1) + nnz + n + (nlev + 1) + n + (n + 1) + 2 * nl;
+  const size_t dbls = 2 * static_cast<size_t>(nnz) + 7 * static_cast<size_t>(n);
+  PP_CH_CHECK(hipMalloc(&C.d_int, ints * sizeof(int)));
+  PP_CH_CHECK(hipMalloc(&C.d_dbl, dbls * sizeof(double)));
+  if (!C.d_state) PP_CH_CHECK(hipMalloc(&C.d_state, sizeof(pp::ChState)));
+  if (!C.h_state) PP_CH_CHECK(hipHostMalloc(&C.h_state, sizeof(pp::ChState), hipHostMallocDefault));
+  int *ip = C.d_int;
+  double *dp = C.d_dbl;
+  pp::ChArgs &a = C.args;
+  a = pp::ChArgs{};
+  a.n = n;
+  a.nlev = nlev;
+  auto put_int = [&](const std::vector<int> &h, const int *&dst) -> int {
+    PP_CH_CHECK(hipMemcpyAsync(ip, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    dst = ip;
+    ip += h.size();
+    return PP_OK;
+  };
+  if (int rc = put_int(A.rp, a.row_ptr)) return rc;
+  if (int rc = put_int(A.ci, a.col)) return rc;
+  if (int rc = put_int(A.diag, a.diag_pos)) return rc;
+  if (int rc = put_int(lev_ptr, a.lev_ptr)) return rc;
+  if (int rc = put_int(lev_rows, a.lev_rows)) return rc;
+  if (int rc = put_int(lt_ptr, a.lt_ptr)) return rc;
+  if (int rc = put_int(lt_row, a.lt_row)) return rc;
+  if (int rc = put_int(lt_pos, a.lt_pos)) return rc;
+  PP_CH_CHECK(hipMemcpyAsync(dp, A.v.data(), A.v.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  a.val = dp;
+  a.lower = dp + nnz;
+  a.D = a.lower + nnz;
+  a.dinv = a.D + n;
+  a.r = a.dinv + n;
+  a.z = a.r + n;
+  a.p = a.z + n;
+  a.ap = a.p + n;
+  a.y = a.ap + n;
+  a.state = C.d_state;
+  PP_CH_CHECK(pp::launch_ch_factor(a, s));
+  PP_CH_CHECK(hipMemcpyAsync(C.h_state, C.d_state, sizeof(pp::ChState), hipMemcpyDeviceToHost, s));
+  PP_CH_CHECK(hipStreamSynchronize(s));
+  C.kind = C.h_state->ildlt_ok ? PP_CH_ILDLT : PP_CH_DIAGONAL;
+  C.nx = cp->nx;
+  C.ny = cp->ny;
+  C.px = cp->periodic_x != 0;
+  C.py = cp->periodic_y != 0;
+  C.dx = cp->dx;
+  C.dy = cp->dy;
+  C.alpha0 = cp->alpha0;
+  C.beta1 = cp->beta1;
+  C.beta2 = cp->beta2;
+  C.n = n;
+  C.nnz = nnz;
+  C.nlev = nlev;
+  C.valid = true;
+  return PP_OK;
+}
+
+const char *ch_status_text(int status) {
+  switch (status) {
+    case pp::kChRz0Fail: return "pcg preconditioner breakdown: non-positive r^T M^{-1} r";
+    case pp::kChRzFail: return "pcg preconditioner breakdown: non-positive new r^T M^{-1} r";
+    case pp::kChPapFail: return "pcg breakdown: p^T A p is invalid";
+    default: return "unknown device status";
+  }
+}
+
+int ch_status_code(int status) {
+  switch (status) {
+    case pp::kChRz0Fail: return PP_ERR_RZ0;
+    case pp::kChRzFail: return PP_ERR_RZ;
+    case pp::kChPapFail: return PP_ERR_PAP;
+    default: return PP_ERR_HIP;
+  }
+}
+
+int run_ch(pp_ctx *ctx, const pp_ch_problem *cp, const double *d_rhs, double *d_c, pp_ch_report *report,
+           hipStream_t s) {
+  if (int rc = ensure_ch(ctx, cp, report, s)) return rc;
+  auto &C = ctx->ch;
+  pp::ChArgs a = C.args;
+  a.rhs = d_rhs;
+  a.x = d_c;
+  a.max_iterations = cp->max_iterations;
+  a.tolerance = cp->tolerance;
+  PP_CH_CHECK(hipEventRecord(ctx->ev_start, s));
+  // solve with the cached preconditioner; a breakdown with ILDLT switches the operator to the
+  // diagonal preconditioner for good and solves again from the x the failed attempt left
+  // (phase_field.cpp:465-474)
+  bool fell_back = false;
+  for (;;) {
+    PP_CH_CHECK(pp::launch_ch_pcg(a, C.kind == PP_CH_ILDLT, s));
+    PP_CH_CHECK(hipMemcpyAsync(C.h_state, C.d_state, sizeof(pp::ChState), hipMemcpyDeviceToHost, s));
+    PP_CH_CHECK(hipStreamSynchronize(s));
+    if (C.h_state->status == pp::kChOk || C.kind != PP_CH_ILDLT) break;
+    C.kind = PP_CH_DIAGONAL;  // later solves report SparsePCG[Diagonal], as the reference's cached one
+    fell_back = true;
+  }
+  PP_CH_CHECK(hipEventRecord(ctx->ev_stop, s));
+  PP_CH_CHECK(hipEventSynchronize(ctx->ev_stop));
+  float ms = 0.0f;
+  PP_CH_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+  ctx->last_kernel_ms = ms;
+  const pp::ChState &st = *C.h_state;
+  if (report) {
+    std::memset(report, 0, sizeof(*report));
+    report->iterations = st.iterations;
+    report->preconditioner = fell_back ? PP_CH_DIAGONAL_FALLBACK : C.kind;
+    report->iterate_residual = st.iter_res;
+    report->absolute_residual = st.abs_res;
+    report->relative_residual = st.rel_res;
+    report->equation_residual = st.eq_res;
+    report->kernel_ms = ms;
+    report->levels = C.nlev;
+  }
+  if (st.status != pp::kChOk) return ch_fail(ctx, report, ch_status_code(st.status), ch_status_text(st.status));
+  ctx->last_error.clear();
+  return PP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_ch_solve_device(pp_ctx *ctx, const pp_ch_problem *cp, const double *d_rhs, double *d_c, pp_ch_report *report,
+                       void *stream) {
+  if (int rc = validate_ch(ctx, cp, report)) return rc;
+  if (!d_rhs || !d_c) return ch_fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_CH_CHECK(hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return run_ch(ctx, cp, d_rhs, d_c, report, s);
+}
+
+int pp_ch_solve(pp_ctx *ctx, const pp_ch_problem *cp, const double *rhs, double *c, pp_ch_report *report) {
+  if (int rc = validate_ch(ctx, cp, report)) return rc;
+  if (!rhs || !c) return ch_fail(ctx, report, PP_ERR_ARG, "null field pointer");
+  PP_CH_CHECK(hipSetDevice(ctx->device));
+  auto &C = ctx->ch;
+  const size_t n = static_cast<size_t>(cp->nx) * cp->ny;
+  if (2 * n > C.stage_cap) {
+    if (C.stage) (void)hipFree(C.stage);
+    C.stage = nullptr;
+    C.stage_cap = 0;
+    PP_CH_CHECK(hipMalloc(&C.stage, 2 * n * sizeof(double)));
+    C.stage_cap = 2 * n;
+  }
+  hipStream_t s = ctx->stream;
+  PP_CH_CHECK(hipMemcpyAsync(C.stage, rhs, n * sizeof(double), hipMemcpyHostToDevice, s));
+  PP_CH_CHECK(hipMemcpyAsync(C.stage + n, c, n * sizeof(double), hipMemcpyHostToDevice, s));
+  const int rc = run_ch(ctx, cp, C.stage, C.stage + n, report, s);
+  const std::string err = ctx->last_error;
+  PP_CH_CHECK(hipMemcpyAsync(c, C.stage + n, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  PP_CH_CHECK(hipStreamSynchronize(s));
+  ctx->last_error = err;
+  return rc;
 }
 
 }  // extern "C"

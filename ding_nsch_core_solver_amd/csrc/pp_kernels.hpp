@@ -162,6 +162,31 @@ hipError_t launch_bicg_setup(const BicgArgs &a, hipStream_t stream);
 hipError_t launch_bicg_iteration(const BicgArgs &a, int iter, hipStream_t stream);
 hipError_t launch_bicg_report(const BicgArgs &a, int iterations, hipStream_t stream);
 
+// Cahn-Hilliard linear solve (solve_phase_linear_system_eq25): the operator's CSR with its level
+// schedule and lower-transpose lists (set on the host), the ILDLT(0) factor and the PCG vectors, all
+// natural order. One workgroup of kThreads runs the factorization, and one the whole PCG.
+enum ChStatus : int { kChOk = 0, kChRz0Fail = 1, kChRzFail = 2, kChPapFail = 3 };
+struct ChState {
+  int status, iterations, ildlt_ok, pad;
+  double floor;                        // ILDLT pivot floor max(1, max |a_ii|) 1e-12
+  double abs_res, rel_res, iter_res, eq_res;
+};
+struct ChArgs {
+  int n, nlev;
+  const int *row_ptr, *col, *diag_pos;
+  const double *val;
+  const int *lev_ptr, *lev_rows;       // rows of level k: lev_rows[lev_ptr[k] .. lev_ptr[k+1])
+  const int *lt_ptr, *lt_row, *lt_pos; // children of row r in ascending order, with the CSR position of L(child, r)
+  double *lower, *D, *dinv;            // factor (aligned with the CSR) and the diagonal preconditioner
+  const double *rhs;
+  double *x, *r, *z, *p, *ap, *y;
+  int max_iterations;
+  double tolerance;
+  ChState *state;
+};
+hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream);
+hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream);
+
 // Momentum predictor (Solver::compute_momentum_fluxes + solve_momentum_predictor) on ghosted Field2D
 // buffers (G layers). Unknowns: the active u faces (i-major; i = iu0 .. nx-1, iu0 = 0 periodic else 1)
 // then the active v faces (j = jv0 .. ny-1); each row keeps up to kMomRow entries.

@@ -92,6 +92,50 @@ def make_momentum_problem(mc) -> PPMomentumProblem:
     return mp
 
 
+class PPChProblem(ctypes.Structure):
+    """pp_ch_problem: grid, alpha0 and the stabilization coefficients, ch_inner_iterations and
+    ch_tolerance of one Cahn-Hilliard linear solve."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int),
+        ("alpha0", ctypes.c_double), ("beta1", ctypes.c_double), ("beta2", ctypes.c_double),
+        ("max_iterations", ctypes.c_int), ("tolerance", ctypes.c_double),
+    ]
+
+
+class PPChReport(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int), ("preconditioner", ctypes.c_int),
+        ("iterate_residual", ctypes.c_double), ("absolute_residual", ctypes.c_double),
+        ("relative_residual", ctypes.c_double), ("equation_residual", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double), ("levels", ctypes.c_int), ("status", ctypes.c_int),
+        ("message", ctypes.c_char * 256),
+    ]
+
+
+PP_CH_DIAGONAL, PP_CH_ILDLT, PP_CH_DIAGONAL_FALLBACK = 0, 1, 2
+CH_SOLVER_NAMES = {PP_CH_DIAGONAL: "SparsePCG[Diagonal]", PP_CH_ILDLT: "SparsePCG[ILDLT]",
+                   PP_CH_DIAGONAL_FALLBACK: "SparsePCG[DiagonalFallback]"}
+
+
+def make_ch_problem(cc) -> PPChProblem:
+    cp = PPChProblem()
+    cp.nx, cp.ny = cc.nx, cc.ny
+    cp.dx, cp.dy = cc.lx / cc.nx, cc.ly / cc.ny
+    cp.periodic_x, cp.periodic_y = int(cc.periodic_x), int(cc.periodic_y)
+    cp.alpha0, cp.beta1, cp.beta2 = float(cc.alpha0), float(cc.beta1), float(cc.beta2)
+    cp.max_iterations, cp.tolerance = int(cc.max_iterations), float(cc.tolerance)
+    return cp
+
+
+def _ch_report_dict(rep: PPChReport) -> dict:
+    return {"iterations": rep.iterations, "preconditioner": rep.preconditioner,
+            "solver_name": CH_SOLVER_NAMES.get(rep.preconditioner, "?"), "iterate_residual": rep.iterate_residual,
+            "absolute_residual": rep.absolute_residual, "relative_residual": rep.relative_residual,
+            "equation_residual": rep.equation_residual, "kernel_ms": rep.kernel_ms, "levels": rep.levels,
+            "status": rep.status, "message": rep.message.decode(errors="replace")}
+
+
 class PPReport(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int), ("outer_cycles", ctypes.c_int),
@@ -116,7 +160,7 @@ EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_
            "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device", "pp_solve_monitored",
            "pp_solve_device_monitored", "pp_solve_device_field", "pp_pressure_step", "pp_bicgstab",
            "pp_bicgstab_device", "pp_set_wait_limit", "pp_solve_device_noalloc", "pp_momentum_predictor",
-           "pp_momentum_predictor_device")
+           "pp_momentum_predictor_device", "pp_ch_solve", "pp_ch_solve_device")
 
 # One PCG monitor record (pp_monitor_record): the scalar fields of one CGMonitor call.
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
@@ -199,6 +243,10 @@ def lib():
         ip = ctypes.POINTER(ctypes.c_int)
         L.pp_set_tuning.argtypes = [vp, ctypes.c_int, ip, ip, ip]
         L.pp_set_tuning.restype = ctypes.c_int
+        L.pp_ch_solve_device.argtypes = [vp, ctypes.POINTER(PPChProblem), vp, vp, ctypes.POINTER(PPChReport), vp]
+        L.pp_ch_solve_device.restype = ctypes.c_int
+        L.pp_ch_solve.argtypes = [vp, ctypes.POINTER(PPChProblem), dp, dp, ctypes.POINTER(PPChReport)]
+        L.pp_ch_solve.restype = ctypes.c_int
         mpp = ctypes.POINTER(PPMomentumProblem)
         L.pp_momentum_predictor_device.argtypes = [vp, mpp, ctypes.POINTER(PPVelocityBC), ctypes.c_int] + [vp] * 11 + [
             ctypes.POINTER(PPReport), vp]
@@ -569,6 +617,26 @@ class Engine:
         if rc != 0:
             raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
         return _report_dict(rep)
+
+    def ch_solve(self, cc, rhs=None, c0=None):
+        """The Cahn-Hilliard linear solve (pp_ch_solve) of a workloads.ChCase (or its rhs / c0
+        overridden). Returns (c, report); the context keeps the operator for the next solve."""
+        rhs = np.ascontiguousarray(cc.rhs if rhs is None else rhs, dtype=np.float64).reshape(-1)
+        c = np.array(cc.c0 if c0 is None else c0, dtype=np.float64, copy=True).reshape(-1)
+        rep = PPChReport()
+        rc = self._lib.pp_ch_solve(self.h, ctypes.byref(make_ch_problem(cc)), _dp(rhs), _dp(c), ctypes.byref(rep))
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return c, _ch_report_dict(rep)
+
+    def ch_solve_device(self, cc, d_rhs, d_c, stream=None) -> dict:
+        """Device-pointer form (pp_ch_solve_device); d_c is solved in place."""
+        rep = PPChReport()
+        rc = self._lib.pp_ch_solve_device(self.h, ctypes.byref(make_ch_problem(cc)), d_rhs, d_c, ctypes.byref(rep),
+                                          stream)
+        if rc != 0:
+            raise EngineError(rc, rep.message.decode(errors="replace") or self.last_error())
+        return _ch_report_dict(rep)
 
     MOMENTUM_INPUTS = ("u", "v", "rho_mid", "eta", "eta_prev", "surface_fx_u", "surface_fy_v")
 

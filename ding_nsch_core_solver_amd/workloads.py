@@ -391,3 +391,56 @@ def momentum_case(nx=192, ny=192, seed=0, density_ratio=1000.0, viscosity_ratio=
                         eta_prev=_scalar_ghosts(eta_prev, g, periodic_x, periodic_y).reshape(-1),
                         surface_fx_u=noise(shape_u).reshape(-1), surface_fy_v=noise(shape_v).reshape(-1),
                         u_prev=noise(shape_u).reshape(-1), v_prev=noise(shape_v).reshape(-1), **kw)
+
+
+def stabilization_coefficients(cn: float, pe: float):
+    """beta1 = a1 / Pe and beta2 = a2 / Pe with a1 = 3 sqrt(2) / (16 Cn), a2 = 3 sqrt(2) Cn / 4
+    (config.hpp:9-15, phase_field.cpp:129-130)."""
+    s2 = 1.4142135623730950488
+    a1 = (3.0 * s2 / (16.0 * cn)) if cn > 0.0 else 0.0
+    a2 = (3.0 * s2 * cn / 4.0) if cn > 0.0 else 0.0
+    return a1 / pe, a2 / pe
+
+
+@dataclasses.dataclass
+class ChCase:
+    """One Cahn-Hilliard linear solve (Solver::solve_phase_linear_system_eq25): the operator
+    alpha0 I - beta1 Lap + beta2 Lap^2 on an nx x ny grid (beta1, beta2 from Cn and Pe), the
+    right-hand side and the initial guess c0 (natural order i * ny + j), ch_inner_iterations and
+    ch_tolerance. alpha0 is 1/dt on the first step and 3/(2 dt) after (phase_field.cpp:678)."""
+    nx: int
+    ny: int
+    lx: float = 1.0
+    ly: float = 1.0
+    periodic_x: bool = False
+    periodic_y: bool = False
+    cn: float = 0.01
+    pe: float = 100.0
+    alpha0: float = 1.0e4
+    max_iterations: int = 30
+    tolerance: float = 1.0e-8
+    rhs: np.ndarray | None = None
+    c0: np.ndarray | None = None
+
+    @property
+    def beta1(self) -> float:
+        return stabilization_coefficients(self.cn, self.pe)[0]
+
+    @property
+    def beta2(self) -> float:
+        return stabilization_coefficients(self.cn, self.pe)[1]
+
+
+def ch_case(nx=192, ny=192, seed=0, radius=0.2, cn=0.01, pe=100.0, dt=1.0e-4, first_step=False, lx=1.0, ly=1.0,
+            periodic_x=False, periodic_y=False, **kw) -> ChCase:
+    """A droplet phase field c0 (c = 1 inside, tanh profile of width 2 sqrt(2) Cn) and a right-hand
+    side alpha0 c0 + a seeded perturbation, as the BDF2 right-hand side of a step carries."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) * lx / nx
+    y = (np.arange(ny) + 0.5) * ly / ny
+    r = np.hypot(x[:, None] - 0.5 * lx, y[None, :] - 0.5 * ly)
+    c0 = 0.5 - 0.5 * np.tanh((r - radius) / (2.0 * math.sqrt(2.0) * cn))
+    alpha0 = 1.0 / dt if first_step else 3.0 / (2.0 * dt)
+    rhs = alpha0 * c0 + 50.0 * rng.standard_normal(c0.shape)
+    return ChCase(nx=nx, ny=ny, lx=lx, ly=ly, periodic_x=periodic_x, periodic_y=periodic_y, cn=cn, pe=pe,
+                  alpha0=alpha0, rhs=rhs.reshape(-1), c0=c0.reshape(-1), **kw)

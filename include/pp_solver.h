@@ -349,6 +349,46 @@ int pp_momentum_predictor(pp_ctx *ctx, const pp_momentum_problem *mp, const pp_v
                           double *u_rhs_prev, double *v_rhs_prev, double *u_star, double *v_star, pp_report *report);
 
 /*
+ * The Cahn-Hilliard semi-implicit linear solve (Solver::solve_phase_linear_system_eq25,
+ * phase_field.cpp:437-503, called from solve_cahn_hilliard_semi_implicit at 690): the operator
+ * A = alpha0 I - beta1 Lap + beta2 Lap^2 on the grid (build_laplacian_matrix, multiply,
+ * build_cahn_hilliard_operator, ch_sparse_krylov.hpp:135-233), its ILDLT(0) preconditioner when it
+ * builds, else the diagonal one (build_preconditioner, 252-335, 421-442), and PCG without projection
+ * from x0 = c (515-660) to ch_tolerance within ch_inner_iterations. A PCG breakdown with the ILDLT
+ * preconditioner switches to the diagonal one and solves again from the iterate the failed attempt
+ * left, as the reference's catch does; that choice is kept with the cached operator. The context
+ * caches the operator, its level schedule and its factor, and reuses them while the grid and the
+ * coefficients stay the same (alpha0 within 1e-14 max(1, |alpha0|), as ensure_ch_linear_system):
+ * a run rebuilds them at its first two steps (alpha0 = 1/dt, then 3/(2 dt)). rhs and c are
+ * natural-order nx * ny arrays (row i * ny + j, the Field2D interiors); c holds the initial guess on
+ * entry and the solution on return. beta1 = a1 / Pe, beta2 = a2 / Pe with a1 = 3 sqrt(2) / (16 Cn),
+ * a2 = 3 sqrt(2) Cn / 4 (config.hpp:9-15).
+ */
+enum pp_ch_preconditioner { PP_CH_DIAGONAL = 0, PP_CH_ILDLT = 1, PP_CH_DIAGONAL_FALLBACK = 2 };
+typedef struct pp_ch_problem {
+  int nx, ny;
+  double dx, dy;
+  int periodic_x, periodic_y;
+  double alpha0, beta1, beta2;
+  int max_iterations;         /* ch_inner_iterations */
+  double tolerance;           /* ch_tolerance */
+} pp_ch_problem;
+typedef struct pp_ch_report {
+  int iterations;             /* last_ch_iterations_ */
+  int preconditioner;         /* pp_ch_preconditioner: SparsePCG[ILDLT] / [Diagonal] / [DiagonalFallback] */
+  double iterate_residual;    /* the reference's return value (report.iterate_residual) */
+  double absolute_residual, relative_residual;
+  double equation_residual;   /* |rhs - A c| / |rhs| of the returned c (last_ch_equation_residual_) */
+  double kernel_ms;           /* device time of the solve (the factorization of a new operator excluded) */
+  int levels;                 /* levels of the triangular sweeps */
+  int status;
+  char message[256];
+} pp_ch_report;
+int pp_ch_solve_device(pp_ctx *ctx, const pp_ch_problem *cp, const double *d_rhs, double *d_c, pp_ch_report *report,
+                       void *stream);
+int pp_ch_solve(pp_ctx *ctx, const pp_ch_problem *cp, const double *rhs, double *c, pp_ch_report *report);
+
+/*
  * Allocation-free form of pp_solve_device, for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in
  * the reference. Same problem, inputs, output, report and error texts as pp_solve_device, on the

@@ -42,6 +42,27 @@ class OrVelocityBc(ctypes.Structure):
     ]
 
 
+class OrChProblem(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+        ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int),
+        ("alpha0", ctypes.c_double), ("beta1", ctypes.c_double), ("beta2", ctypes.c_double),
+        ("max_iterations", ctypes.c_int), ("tolerance", ctypes.c_double),
+    ]
+
+
+NATURAL_TREE = 2  # CH solve reductions in the engine's order (512-thread tree over natural rows)
+
+
+def ch_problem(cc) -> OrChProblem:
+    cp = OrChProblem()
+    cp.nx, cp.ny, cp.lx, cp.ly = cc.nx, cc.ny, cc.lx, cc.ly
+    cp.periodic_x, cp.periodic_y = int(cc.periodic_x), int(cc.periodic_y)
+    cp.alpha0, cp.beta1, cp.beta2 = cc.alpha0, cc.beta1, cc.beta2
+    cp.max_iterations, cp.tolerance = cc.max_iterations, cc.tolerance
+    return cp
+
+
 class OrMomentumProblem(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int), ("ny", ctypes.c_int),
@@ -116,6 +137,14 @@ def lib():
         L.or_bicgstab.argtypes = [ctypes.c_int, ip, ip, dp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.c_int, ip, dp, dp, dp, ctypes.c_char_p, ctypes.c_int]
         L.or_bicgstab.restype = ctypes.c_int
+        chp = ctypes.POINTER(OrChProblem)
+        L.or_ch_operator.argtypes = [chp, ip, ip, ip, ip, dp, ctypes.c_char_p, ctypes.c_int]
+        L.or_ch_operator.restype = ctypes.c_int
+        L.or_ch_ildlt.argtypes = [chp, dp, dp, dp]
+        L.or_ch_ildlt.restype = ctypes.c_int
+        L.or_ch_solve.argtypes = [chp, dp, dp, ctypes.c_int, ctypes.c_int, ip, dp, dp, dp, ip, ctypes.c_char_p,
+                                  ctypes.c_int]
+        L.or_ch_solve.restype = ctypes.c_int
         mp_ = ctypes.POINTER(OrMomentumProblem)
         vb_ = ctypes.POINTER(OrVelocityBc)
         L.or_momentum_system.argtypes = [mp_, vb_, ctypes.c_int] + [dp] * 9 + [ip, ip, ip, ip, dp, dp, dp,
@@ -394,3 +423,42 @@ def momentum_predictor(mc, reduction: int = SEQUENTIAL):
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return u_star, v_star, u_prev, v_prev, it.value, ab.value
+
+
+def ch_operator(cc):
+    """The CH operator alpha0 I - beta1 Lap + beta2 Lap^2 as CSR (row_ptr, col_idx, values)."""
+    import types
+    L = lib()
+    n, nnz = ctypes.c_int(0), ctypes.c_int(0)
+    err = ctypes.create_string_buffer(512)
+    cp = ch_problem(cc)
+    if L.or_ch_operator(ctypes.byref(cp), ctypes.byref(n), ctypes.byref(nnz), None, None, None, err, 512):
+        raise RuntimeError(err.value.decode())
+    rp, ci, val = np.zeros(n.value + 1, dtype=np.int32), np.zeros(nnz.value, dtype=np.int32), np.zeros(nnz.value)
+    if L.or_ch_operator(ctypes.byref(cp), ctypes.byref(n), ctypes.byref(nnz), _ip(rp), _ip(ci), _dp(val), err, 512):
+        raise RuntimeError(err.value.decode())
+    return types.SimpleNamespace(n=n.value, row_ptr=rp, col_idx=ci, values=val)
+
+
+def ch_ildlt(cc):
+    """(ok, lower (nnz, aligned with the operator's CSR), D (n), floor) of the ILDLT(0) factor."""
+    A = ch_operator(cc)
+    lower, D, fl = np.zeros(len(A.values)), np.zeros(A.n), ctypes.c_double(0.0)
+    ok = lib().or_ch_ildlt(ctypes.byref(ch_problem(cc)), _dp(lower), _dp(D), ctypes.byref(fl))
+    return bool(ok), lower, D, fl.value
+
+
+def ch_solve(cc, rhs, c0, reduction: int = SEQUENTIAL, precond: int = -1):
+    """The CH linear solve (phase_field.cpp:437-503). Returns (c, iterations, absolute, relative,
+    iterate residual, preconditioner used: 1 ILDLT, 0 diagonal, 2 diagonal after a fallback)."""
+    L = lib()
+    c = np.array(c0, dtype=np.float64, copy=True).reshape(-1)
+    it, pre = ctypes.c_int(0), ctypes.c_int(0)
+    ab, rel, itr = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+    err = ctypes.create_string_buffer(512)
+    rc = L.or_ch_solve(ctypes.byref(ch_problem(cc)), _dp(_f64(rhs)), _dp(c), int(precond), int(reduction),
+                       ctypes.byref(it), ctypes.byref(ab), ctypes.byref(rel), ctypes.byref(itr), ctypes.byref(pre),
+                       err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return c, it.value, ab.value, rel.value, itr.value, pre.value

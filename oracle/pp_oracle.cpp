@@ -1694,3 +1694,353 @@ int or_momentum_predictor(const OrMomentumProblem *mp, const OrVelocityBc *vb, i
 }
 
 }  // extern "C"
+
+// ---- Cahn-Hilliard semi-implicit solve (Solver::solve_phase_linear_system_eq25) ------------------
+//
+// phase_field.cpp:437-503 with the cached operator and preconditioner of 109-135: the operator
+// A = alpha0 I - beta1 Lap + beta2 Lap^2 (ch_sparse_krylov.hpp:135-233: build_laplacian_matrix,
+// multiply, build_cahn_hilliard_operator, each through std::map rows, so every entry is the sum of
+// its contributions in map-insertion order from 0.0), the ILDLT(0) preconditioner when it builds
+// (build_preconditioner, 252-335, 434-442) else the diagonal one, and PCG without projection from
+// x0 = c (515-660). When that PCG throws, the preconditioner becomes the diagonal one (and stays so
+// for the operator) and PCG runs again from the x the failed attempt left.
+
+namespace {
+
+constexpr int kNaturalTree = 2;  // the engine's order: 512-thread canonical tree over natural rows
+
+template <class F>
+double ch_sum(int mode, std::size_t n, F f) {
+  if (mode != kNaturalTree) {
+    double s = 0.0;
+    for (std::size_t k = 0; k < n; ++k) s += f(k);
+    return s;
+  }
+  constexpr std::size_t T = 512, W = 8;
+  double part[T];
+  for (std::size_t t = 0; t < T; ++t) {
+    double s = 0.0;
+    for (std::size_t q = t; q < n; q += T) s = s + f(q);
+    part[t] = s;
+  }
+  double wsum[W];
+  for (std::size_t w = 0; w < W; ++w) {
+    double *v = part + 64 * w;
+    for (std::size_t off = 32; off >= 1; off >>= 1)
+      for (std::size_t l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+    wsum[w] = v[0];
+  }
+  for (std::size_t off = W / 2; off >= 1; off >>= 1)
+    for (std::size_t t = 0; t < off; ++t) wsum[t] = wsum[t] + wsum[t + off];
+  return wsum[0];
+}
+
+struct ChCsr {
+  int n = 0;
+  std::vector<int> rp{0}, ci, diag;
+  std::vector<double> v;
+};
+
+// Rows given as (column, value) lists already merged and sorted: drop |v| < 1e-18, record diagonals.
+void push_row(ChCsr &A, const std::vector<std::pair<int, double>> &row, int r) {
+  int d = -1;
+  for (const auto &e : row) {
+    if (std::abs(e.second) < 1.0e-18) continue;
+    if (e.first == r) d = static_cast<int>(A.v.size());
+    A.ci.push_back(e.first);
+    A.v.push_back(e.second);
+  }
+  if (d < 0) throw std::runtime_error("sparse matrix assembly failed to produce a diagonal entry");
+  A.diag.push_back(d);
+  A.rp.push_back(static_cast<int>(A.v.size()));
+  ++A.n;
+}
+
+// Accumulates contributions into a row: the first contribution to a column starts it at 0.0.
+struct RowAccum {
+  std::vector<std::pair<int, double>> e;  // insertion order
+  void add(int col, double value) {
+    for (auto &x : e)
+      if (x.first == col) {
+        x.second += value;
+        return;
+      }
+    e.push_back({col, 0.0 + value});
+  }
+  std::vector<std::pair<int, double>> sorted() const {
+    auto s = e;
+    std::sort(s.begin(), s.end());
+    return s;
+  }
+};
+
+ChCsr laplacian(int nx, int ny, double dx, double dy, bool px, bool py) {
+  const double ix = 1.0 / (dx * dx), iy = 1.0 / (dy * dy);
+  ChCsr L;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const int r = i * ny + j;
+      RowAccum a;
+      double diag = 0.0;
+      if (px || i > 0) {
+        a.add((px && i == 0 ? nx - 1 : i - 1) * ny + j, ix);
+        diag -= ix;
+      }
+      if (px || i < nx - 1) {
+        a.add((px && i == nx - 1 ? 0 : i + 1) * ny + j, ix);
+        diag -= ix;
+      }
+      if (py || j > 0) {
+        a.add(i * ny + (py && j == 0 ? ny - 1 : j - 1), iy);
+        diag -= iy;
+      }
+      if (py || j < ny - 1) {
+        a.add(i * ny + (py && j == ny - 1 ? 0 : j + 1), iy);
+        diag -= iy;
+      }
+      a.add(r, diag);
+      push_row(L, a.sorted(), r);
+    }
+  return L;
+}
+
+ChCsr ch_operator(int nx, int ny, double dx, double dy, bool px, bool py, double alpha0, double beta1, double beta2) {
+  const ChCsr L = laplacian(nx, ny, dx, dy, px, py);
+  ChCsr A;
+  for (int r = 0; r < L.n; ++r) {
+    RowAccum l2;  // (L L)(r, :) in the traversal order of multiply()
+    for (int pa = L.rp[r]; pa < L.rp[r + 1]; ++pa) {
+      const int mid = L.ci[pa];
+      for (int pb = L.rp[mid]; pb < L.rp[mid + 1]; ++pb) l2.add(L.ci[pb], L.v[pa] * L.v[pb]);
+    }
+    std::vector<std::pair<int, double>> l2row;  // finalize_row_maps of the product
+    for (const auto &e : l2.sorted())
+      if (!(std::abs(e.second) < 1.0e-18)) l2row.push_back(e);
+    RowAccum a;
+    a.add(r, alpha0);
+    for (int p = L.rp[r]; p < L.rp[r + 1]; ++p) a.add(L.ci[p], -beta1 * L.v[p]);
+    for (const auto &e : l2row) a.add(e.first, beta2 * e.second);
+    push_row(A, a.sorted(), r);
+  }
+  return A;
+}
+
+struct ChPrecond {
+  int type = 0;  // 0 diagonal, 1 ILDLT
+  std::vector<double> dinv, lower, D;
+  double floor = 0.0;
+};
+
+// try_build_incomplete_ldlt (ch_sparse_krylov.hpp:252-335)
+bool ildlt(const ChCsr &A, ChPrecond &P) {
+  P.lower.assign(A.v.size(), 0.0);
+  P.D.assign(static_cast<std::size_t>(A.n), 0.0);
+  double max_diag = 0.0;
+  for (int r = 0; r < A.n; ++r) max_diag = std::max(max_diag, std::abs(A.v[static_cast<std::size_t>(A.diag[r])]));
+  P.floor = std::max(1.0, max_diag) * 1.0e-12;
+  auto lower_of = [&](int row, int col) {  // L(row, col) for col < row, 0 when absent
+    for (int p = A.rp[row]; p < A.rp[row + 1] && A.ci[p] < row && A.ci[p] <= col; ++p)
+      if (A.ci[p] == col) return P.lower[static_cast<std::size_t>(p)];
+    return 0.0;
+  };
+  for (int r = 0; r < A.n; ++r) {
+    for (int p = A.rp[r]; p < A.rp[r + 1] && A.ci[p] < r; ++p) {
+      const int c = A.ci[p];
+      double s = A.v[static_cast<std::size_t>(p)];
+      for (int q = A.rp[r]; q < p && A.ci[q] < c; ++q) {
+        const int k = A.ci[q];
+        const double l_ck = lower_of(c, k);
+        if (l_ck != 0.0) s -= P.lower[static_cast<std::size_t>(q)] * P.D[static_cast<std::size_t>(k)] * l_ck;
+      }
+      P.lower[static_cast<std::size_t>(p)] = s / std::max(P.D[static_cast<std::size_t>(c)], P.floor);
+    }
+    double d = A.v[static_cast<std::size_t>(A.diag[r])];
+    for (int p = A.rp[r]; p < A.rp[r + 1] && A.ci[p] < r; ++p)
+      d -= (P.lower[static_cast<std::size_t>(p)] * P.lower[static_cast<std::size_t>(p)]) * P.D[static_cast<std::size_t>(A.ci[p])];
+    if (!std::isfinite(d) || d <= P.floor) return false;
+    P.D[static_cast<std::size_t>(r)] = d;
+  }
+  P.type = 1;
+  return true;
+}
+
+ChPrecond diagonal_precond(const ChCsr &A) {
+  ChPrecond P;
+  P.dinv.assign(static_cast<std::size_t>(A.n), 1.0);
+  for (int r = 0; r < A.n; ++r) P.dinv[static_cast<std::size_t>(r)] = 1.0 / std::max(std::abs(A.v[static_cast<std::size_t>(A.diag[r])]), 1.0e-30);
+  return P;
+}
+
+// apply_preconditioner, ILDLT branch (ch_sparse_krylov.hpp:486-512): unit-lower forward in CSR
+// order, division by max(D, floor), unit-upper backward over the children in ascending row order.
+void ch_apply(const ChCsr &A, const ChPrecond &P, const std::vector<double> &r, std::vector<double> &z) {
+  const std::size_t n = static_cast<std::size_t>(A.n);
+  if (P.type == 0) {
+    z.resize(n);
+    for (std::size_t k = 0; k < n; ++k) z[k] = P.dinv[k] * r[k];
+    return;
+  }
+  std::vector<double> y(n, 0.0);
+  for (int row = 0; row < A.n; ++row) {
+    double s = r[static_cast<std::size_t>(row)];
+    for (int p = A.rp[row]; p < A.rp[row + 1] && A.ci[p] < row; ++p)
+      s -= P.lower[static_cast<std::size_t>(p)] * y[static_cast<std::size_t>(A.ci[p])];
+    y[static_cast<std::size_t>(row)] = s;
+  }
+  for (std::size_t k = 0; k < n; ++k) y[k] /= std::max(P.D[k], P.floor);
+  // children lists in ascending child row order
+  std::vector<std::vector<std::pair<int, double>>> lt(n);
+  for (int row = 0; row < A.n; ++row)
+    for (int p = A.rp[row]; p < A.rp[row + 1] && A.ci[p] < row; ++p)
+      lt[static_cast<std::size_t>(A.ci[p])].push_back({row, P.lower[static_cast<std::size_t>(p)]});
+  z = y;
+  for (int row = A.n - 1; row >= 0; --row) {
+    double s = z[static_cast<std::size_t>(row)];
+    for (const auto &e : lt[static_cast<std::size_t>(row)]) s -= e.second * z[static_cast<std::size_t>(e.first)];
+    z[static_cast<std::size_t>(row)] = s;
+  }
+}
+
+struct ChReport {
+  int iterations = 0;
+  double abs_res = 0.0, rel_res = 0.0, iter_res = 0.0;
+};
+
+// solve_preconditioned_cg without projection (ch_sparse_krylov.hpp:515-660)
+ChReport ch_pcg(int mode, const ChCsr &A, const ChPrecond &P, const std::vector<double> &b, int max_it, double tol,
+                std::vector<double> &x) {
+  const std::size_t n = static_cast<std::size_t>(A.n);
+  if (A.n <= 0) throw std::runtime_error("cannot solve an empty linear system");
+  auto spmv = [&](const std::vector<double> &in, std::vector<double> &out) {
+    out.assign(n, 0.0);
+    for (int row = 0; row < A.n; ++row) {
+      double s = 0.0;
+      for (int p = A.rp[row]; p < A.rp[row + 1]; ++p) s += A.v[static_cast<std::size_t>(p)] * in[static_cast<std::size_t>(A.ci[p])];
+      out[static_cast<std::size_t>(row)] = s;
+    }
+  };
+  std::vector<double> ax, r(n), z, p, ap;
+  spmv(x, ax);
+  for (std::size_t k = 0; k < n; ++k) r[k] = b[k] - ax[k];
+  const double bn = std::sqrt(std::max(ch_sum(mode, n, [&](std::size_t k) { return b[k] * b[k]; }), 0.0));
+  ChReport rep;
+  rep.abs_res = std::sqrt(std::max(ch_sum(mode, n, [&](std::size_t k) { return r[k] * r[k]; }), 0.0));
+  rep.rel_res = rep.abs_res / std::max(bn, 1.0e-30);
+  if (rep.rel_res < tol || rep.abs_res < tol) return rep;
+  ch_apply(A, P, r, z);
+  double rz = ch_sum(mode, n, [&](std::size_t k) { return r[k] * z[k]; });
+  if (!std::isfinite(rz) || rz <= 0.0) throw std::runtime_error("pcg preconditioner breakdown: non-positive r^T M^{-1} r");
+  p = z;
+  const int limit = std::max(1, max_it);
+  for (int it = 0; it < limit; ++it) {
+    spmv(p, ap);
+    const double pap = ch_sum(mode, n, [&](std::size_t k) { return p[k] * ap[k]; });
+    if (!std::isfinite(pap) || std::abs(pap) < 1.0e-30) throw std::runtime_error("pcg breakdown: p^T A p is invalid");
+    const double alpha = rz / pap;
+    std::vector<double> delta(n);
+    for (std::size_t k = 0; k < n; ++k) {
+      delta[k] = alpha * p[k];
+      x[k] += delta[k];
+      r[k] -= alpha * ap[k];
+    }
+    const double up = ch_sum(mode, n, [&](std::size_t k) { return delta[k] * delta[k]; });
+    const double xx = ch_sum(mode, n, [&](std::size_t k) { return x[k] * x[k]; });
+    rep.iterations = it + 1;
+    rep.iter_res = std::sqrt(up / std::max(xx, 1.0e-30));
+    rep.abs_res = std::sqrt(std::max(ch_sum(mode, n, [&](std::size_t k) { return r[k] * r[k]; }), 0.0));
+    rep.rel_res = rep.abs_res / std::max(bn, 1.0e-30);
+    if (rep.rel_res < tol || rep.abs_res < tol) break;
+    ch_apply(A, P, r, z);
+    const double rzn = ch_sum(mode, n, [&](std::size_t k) { return r[k] * z[k]; });
+    if (!std::isfinite(rzn) || rzn <= 0.0)
+      throw std::runtime_error("pcg preconditioner breakdown: non-positive new r^T M^{-1} r");
+    const double beta = rzn / rz;
+    for (std::size_t k = 0; k < n; ++k) p[k] = z[k] + beta * p[k];
+    rz = rzn;
+  }
+  return rep;
+}
+
+}  // namespace
+
+extern "C" {
+
+struct OrChProblem {
+  int nx, ny;
+  double lx, ly;
+  int periodic_x, periodic_y;
+  double alpha0, beta1, beta2;
+  int max_iterations;
+  double tolerance;
+};
+
+// The CH operator's CSR (sizes first with null arrays).
+int or_ch_operator(const OrChProblem *cp, int *n, int *nnz, int *row_ptr, int *col, double *val, char *err, int errlen) {
+  try {
+    const ChCsr A = ch_operator(cp->nx, cp->ny, cp->lx / cp->nx, cp->ly / cp->ny, cp->periodic_x != 0, cp->periodic_y != 0,
+                              cp->alpha0, cp->beta1, cp->beta2);
+    *n = A.n;
+    *nnz = static_cast<int>(A.v.size());
+    if (row_ptr) std::copy(A.rp.begin(), A.rp.end(), row_ptr);
+    if (col) std::copy(A.ci.begin(), A.ci.end(), col);
+    if (val) std::copy(A.v.begin(), A.v.end(), val);
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// ILDLT(0) of the CH operator: returns 1 and fills lower (nnz, aligned with the CSR) and D (n) when it
+// builds, 0 when it fails (build_preconditioner then takes the diagonal one).
+int or_ch_ildlt(const OrChProblem *cp, double *lower, double *D, double *floor_out) {
+  const ChCsr A = ch_operator(cp->nx, cp->ny, cp->lx / cp->nx, cp->ly / cp->ny, cp->periodic_x != 0, cp->periodic_y != 0,
+                            cp->alpha0, cp->beta1, cp->beta2);
+  ChPrecond P;
+  const bool ok = ildlt(A, P);
+  std::copy(P.lower.begin(), P.lower.end(), lower);
+  std::copy(P.D.begin(), P.D.end(), D);
+  *floor_out = P.floor;
+  return ok ? 1 : 0;
+}
+
+// The solve: rhs and c (in: the initial guess, out: the solution) are natural-order nx*ny arrays.
+// precond_in: -1 build (ILDLT if it builds, else diagonal), 0 diagonal (the cached state after a
+// fallback); precond_out: 1 ILDLT, 0 diagonal, 2 diagonal after a fallback. reduction: 0 sequential
+// (the reference), 2 the engine's natural-order tree.
+int or_ch_solve(const OrChProblem *cp, const double *rhs, double *c, int precond_in, int reduction, int *iterations,
+                double *abs_res, double *rel_res, double *iter_res, int *precond_out, char *err, int errlen) {
+  try {
+    const ChCsr A = ch_operator(cp->nx, cp->ny, cp->lx / cp->nx, cp->ly / cp->ny, cp->periodic_x != 0, cp->periodic_y != 0,
+                              cp->alpha0, cp->beta1, cp->beta2);
+    ChPrecond P;
+    if (precond_in != 0 && ildlt(A, P)) {
+      P.type = 1;
+    } else {
+      P = diagonal_precond(A);
+    }
+    const std::vector<double> b(rhs, rhs + A.n);
+    std::vector<double> x(c, c + A.n);
+    ChReport rep;
+    int used = P.type;
+    try {
+      rep = ch_pcg(reduction, A, P, b, cp->max_iterations, cp->tolerance, x);
+    } catch (const std::runtime_error &) {
+      P = diagonal_precond(A);
+      used = 2;
+      rep = ch_pcg(reduction, A, P, b, cp->max_iterations, cp->tolerance, x);
+    }
+    std::copy(x.begin(), x.end(), c);
+    *iterations = rep.iterations;
+    *abs_res = rep.abs_res;
+    *rel_res = rep.rel_res;
+    *iter_res = rep.iter_res;
+    *precond_out = used;
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+}  // extern "C"

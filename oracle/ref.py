@@ -35,6 +35,14 @@ class RefVelocityBc(ctypes.Structure):
     ]
 
 
+class RefChConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+        ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int), ("cn", ctypes.c_double), ("pe", ctypes.c_double),
+        ("ch_inner_iterations", ctypes.c_int), ("ch_tolerance", ctypes.c_double),
+    ]
+
+
 class RefMomentumProblem(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int), ("ny", ctypes.c_int),
@@ -104,6 +112,15 @@ def lib():
                                    ip, dp, dp, dp, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
         L.ref_bicgstab.restype = ctypes.c_int
         vp = ctypes.c_void_p
+        L.ref_ch_create.argtypes = [ctypes.POINTER(RefChConfig), ctypes.c_char_p, ctypes.c_int]
+        L.ref_ch_create.restype = vp
+        L.ref_ch_destroy.argtypes = [vp]
+        L.ref_ch_destroy.restype = None
+        L.ref_ch_solve.argtypes = [vp, ctypes.c_double, dp, dp, ip, dp, dp, ctypes.c_char_p, ctypes.c_int,
+                                   ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
+        L.ref_ch_solve.restype = ctypes.c_int
+        L.ref_ch_operator.argtypes = [vp, ip, ip, ip, ip, dp, ip, dp, dp]
+        L.ref_ch_operator.restype = None
         L.ref_momentum_predictor.argtypes = [ctypes.POINTER(RefMomentumProblem), ctypes.POINTER(RefVelocityBc),
                                              ctypes.c_int] + [dp] * 11 + [ip, dp, ctypes.c_int, dp, ctypes.c_char_p,
                                                                           ctypes.c_int]
@@ -436,3 +453,55 @@ def momentum_predictor(mc, repeats: int = 1, timing: list | None = None):
     if timing is not None:
         timing.append(float(seconds[0]))
     return u_star, v_star, u_prev, v_prev, it.value, ab.value
+
+
+class RefCh:
+    """A reference Solver used for its CH linear solve (solve_phase_linear_system_eq25), keeping its
+    cached operator and preconditioner across solve() calls as a run does."""
+
+    def __init__(self, cc):
+        rc = RefChConfig()
+        rc.nx, rc.ny, rc.lx, rc.ly = cc.nx, cc.ny, cc.lx, cc.ly
+        rc.periodic_x, rc.periodic_y = int(cc.periodic_x), int(cc.periodic_y)
+        rc.cn, rc.pe = cc.cn, cc.pe
+        rc.ch_inner_iterations, rc.ch_tolerance = cc.max_iterations, cc.tolerance
+        err = ctypes.create_string_buffer(512)
+        self.h = lib().ref_ch_create(ctypes.byref(rc), err, 512)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+
+    def solve(self, alpha0, rhs, c0, repeats: int = 1, timing: list | None = None):
+        """Returns (c, iterations, iterate residual, equation residual, solver name)."""
+        c = np.array(c0, dtype=np.float64, copy=True).reshape(-1)
+        it, itr, eq = ctypes.c_int(0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+        name = ctypes.create_string_buffer(128)
+        sec = np.zeros(1)
+        err = ctypes.create_string_buffer(512)
+        rc = lib().ref_ch_solve(self.h, float(alpha0), _dp(np.ascontiguousarray(rhs, dtype=np.float64).reshape(-1)),
+                                _dp(c), ctypes.byref(it), ctypes.byref(itr), ctypes.byref(eq), name, 128, int(repeats),
+                                _dp(sec), err, 512)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        if timing is not None:
+            timing.append(float(sec[0]))
+        return c, it.value, itr.value, eq.value, name.value.decode()
+
+    def operator(self):
+        """(row_ptr, col_idx, values, kind (1 ILDLT, 0 diagonal), lower, D) of the cached operator."""
+        L = lib()
+        n, nnz, kind = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.ref_ch_operator(self.h, ctypes.byref(n), ctypes.byref(nnz), None, None, None, ctypes.byref(kind), None, None)
+        rp, ci = np.zeros(n.value + 1, dtype=np.int32), np.zeros(nnz.value, dtype=np.int32)
+        val, lower, D = np.zeros(nnz.value), np.zeros(nnz.value), np.zeros(n.value)
+        L.ref_ch_operator(self.h, ctypes.byref(n), ctypes.byref(nnz), rp.ctypes.data_as(ip), ci.ctypes.data_as(ip),
+                          _dp(val), ctypes.byref(kind), _dp(lower), _dp(D))
+        return rp, ci, val, kind.value, lower, D
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ref_ch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

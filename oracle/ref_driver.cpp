@@ -730,5 +730,96 @@ int ref_momentum_predictor(const RefMomentumProblem *mp, const RefVelocityBc *vb
   }
 }
 
+
+// The CH linear solve through the reference Solver's own solve_phase_linear_system_eq25
+// (phase_field.cpp:437-503), on a Solver that keeps its cached operator and preconditioner between
+// calls (phase_field.cpp:109-135), as in a run.
+struct RefChConfig {
+  int nx, ny;
+  double lx, ly;
+  int periodic_x, periodic_y;
+  double cn, pe;
+  int ch_inner_iterations;
+  double ch_tolerance;
+};
+
+void *ref_ch_create(const RefChConfig *cc, char *err, int errlen) {
+  try {
+    ding::Config cfg;
+    cfg.name = "oracle_ref_ch";
+    cfg.nx = cc->nx;
+    cfg.ny = cc->ny;
+    cfg.lx = cc->lx;
+    cfg.ly = cc->ly;
+    cfg.periodic_x = cc->periodic_x != 0;
+    cfg.periodic_y = cc->periodic_y != 0;
+    cfg.cn = cc->cn;
+    cfg.pe = cc->pe;
+    cfg.ch_inner_iterations = cc->ch_inner_iterations;
+    cfg.ch_tolerance = cc->ch_tolerance;
+    cfg.verbose = false;
+    cfg.write_vtk = false;
+    return new ding::Solver(cfg);
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return nullptr;
+  }
+}
+
+void ref_ch_destroy(void *h) { delete static_cast<ding::Solver *>(h); }
+
+// rhs and c are natural-order nx*ny arrays (c: the initial guess in, the solution out). name receives
+// last_ch_solver_name_. With repeats > 1 the solve runs that many times from the same c and rhs (the
+// operator cached after the first) and *seconds gets the median of one solve.
+int ref_ch_solve(void *h, double alpha0, const double *rhs, double *c, int *iterations, double *iterate_residual,
+                 double *equation_residual, char *name, int namelen, int repeats, double *seconds, char *err,
+                 int errlen) {
+  auto &s = *static_cast<ding::Solver *>(h);
+  try {
+    const int nx = s.cfg_.nx, ny = s.cfg_.ny;
+    ding::Field2D rhs_field(nx, ny, s.cfg_.ghost, 0.0);
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) rhs_field(i, j) = rhs[static_cast<std::size_t>(i) * ny + j];
+    std::vector<double> times;
+    ding::Field2D c_state(nx, ny, s.cfg_.ghost, 0.0);
+    for (int rep = 0; rep < std::max(1, repeats); ++rep) {
+      for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) c_state(i, j) = c[static_cast<std::size_t>(i) * ny + j];
+      const auto t0 = std::chrono::steady_clock::now();
+      s.solve_phase_linear_system_eq25(rhs_field, alpha0, 0.0, c_state, *iterate_residual, *equation_residual);
+      times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    if (seconds) {
+      std::sort(times.begin(), times.end());
+      *seconds = times[times.size() / 2];
+    }
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) c[static_cast<std::size_t>(i) * ny + j] = c_state(i, j);
+    *iterations = s.last_ch_iterations_;
+    if (name && namelen > 0) std::snprintf(name, static_cast<std::size_t>(namelen), "%s", s.last_ch_solver_name_.c_str());
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// The cached operator's CSR after a solve (sizes first with null arrays) and its preconditioner kind
+// (1 ILDLT, 0 diagonal); lower / D when ILDLT.
+void ref_ch_operator(void *h, int *n, int *nnz, int *row_ptr, int *col, double *val, int *kind, double *lower,
+                     double *D) {
+  auto &s = *static_cast<ding::Solver *>(h);
+  const auto &A = s.ch_linear_system_matrix_;
+  const auto &P = s.ch_linear_system_preconditioner_;
+  *n = A.n;
+  *nnz = static_cast<int>(A.values.size());
+  *kind = P.type == ding::ch_sparse_krylov::PreconditionerType::incomplete_ldlt ? 1 : 0;
+  if (row_ptr) std::copy(A.row_ptr.begin(), A.row_ptr.end(), row_ptr);
+  if (col) std::copy(A.col_idx.begin(), A.col_idx.end(), col);
+  if (val) std::copy(A.values.begin(), A.values.end(), val);
+  if (lower && *kind == 1) std::copy(P.lower.begin(), P.lower.end(), lower);
+  if (D && *kind == 1) std::copy(P.diag.begin(), P.diag.end(), D);
+}
+
 }  // extern "C"
 
