@@ -1471,13 +1471,8 @@ int ensure_momentum(pp_ctx *ctx, size_t n, pp_report *report) {
   if (!ctx->mom_ev[0]) PP_HIP_CHECK(hipEventCreate(&ctx->mom_ev[0]));
   if (!ctx->mom_ev[1]) PP_HIP_CHECK(hipEventCreate(&ctx->mom_ev[1]));
   if (n <= ctx->mom_cap) return PP_OK;
-  if (ctx->ch.d_int) (void)hipFree(ctx->ch.d_int);
-  if (ctx->ch.d_dbl) (void)hipFree(ctx->ch.d_dbl);
-  if (ctx->ch.d_state) (void)hipFree(ctx->ch.d_state);
-  if (ctx->ch.h_state) (void)hipHostFree(ctx->ch.h_state);
-  if (ctx->ch.stage) (void)hipFree(ctx->ch.stage);
-  if (ctx->mom_int) (void)hipFree(ctx->mom_int);
-  if (ctx->mom_dbl) (void)hipFree(ctx->mom_dbl);
+  if (ctx->mom_int) PP_HIP_CHECK(hipFree(ctx->mom_int));
+  if (ctx->mom_dbl) PP_HIP_CHECK(hipFree(ctx->mom_dbl));
   ctx->mom_int = nullptr;
   ctx->mom_dbl = nullptr;
   ctx->mom_cap = 0;
@@ -1596,7 +1591,7 @@ int pp_momentum_predictor(pp_ctx *ctx, const pp_momentum_problem *mp, const pp_v
   const size_t nv = static_cast<size_t>(mp->nx + 2 * G) * (mp->ny + 1 + 2 * G);
   const size_t need = 3 * nc + 4 * nu + 4 * nv;
   if (need > ctx->mom_stage_cap) {
-    if (ctx->mom_stage) (void)hipFree(ctx->mom_stage);
+    if (ctx->mom_stage) PP_HIP_CHECK(hipFree(ctx->mom_stage));
     ctx->mom_stage = nullptr;
     ctx->mom_stage_cap = 0;
     PP_HIP_CHECK(hipMalloc(&ctx->mom_stage, need * sizeof(double)));

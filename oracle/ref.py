@@ -53,6 +53,11 @@ class RefMomentumProblem(ctypes.Structure):
     ]
 
 
+RefChSolveFn = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+
+
 class RefRunConfig(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int), ("ny", ctypes.c_int),
@@ -136,6 +141,8 @@ def lib():
         L.ref_run_phase_momentum.restype = ctypes.c_int
         L.ref_run_phase.argtypes = [vp, dp, ctypes.c_char_p, ctypes.c_int]
         L.ref_run_phase.restype = ctypes.c_int
+        L.ref_run_phase_external_ch.argtypes = [vp, RefChSolveFn, vp, dp, ctypes.c_char_p, ctypes.c_int]
+        L.ref_run_phase_external_ch.restype = ctypes.c_int
         L.ref_run_momentum.argtypes = [vp, dp, ctypes.c_char_p, ctypes.c_int]
         L.ref_run_momentum.restype = ctypes.c_int
         L.ref_run_set_momentum_report.argtypes = [vp, ctypes.c_double, ctypes.c_int]
@@ -365,6 +372,30 @@ class RefRun:
         """History shifts and the CH stage of the step (advance_phase_state)."""
         sec = ctypes.c_double(0.0)
         self._call(lib().ref_run_phase, ctypes.byref(sec))
+        return sec.value
+
+    def phase_external_ch(self, solve) -> float:
+        """phase() with the CH linear solve done by solve(alpha0, rhs, c) -> (iterations, iterate
+        residual, equation residual), which updates c (natural order) in place."""
+        def cb(user, alpha0, rhs_p, c_p, n, it_p, itr_p, eq_p):
+            try:
+                rhs = np.ctypeslib.as_array(rhs_p, shape=(n,))
+                c = np.ctypeslib.as_array(c_p, shape=(n,))
+                it, itr, eq = solve(alpha0, rhs, c)
+                it_p[0], itr_p[0], eq_p[0] = int(it), float(itr), float(eq)
+                return 0
+            except Exception as e:  # noqa: BLE001 - reported through the driver's error text
+                self._cb_error = e
+                return 1
+        fn = RefChSolveFn(cb)
+        sec = ctypes.c_double(0.0)
+        self._cb_error = None
+        try:
+            self._call(lib().ref_run_phase_external_ch, fn, None, ctypes.byref(sec))
+        except RuntimeError:
+            if self._cb_error is not None:
+                raise self._cb_error
+            raise
         return sec.value
 
     def momentum(self) -> float:

@@ -529,6 +529,74 @@ int ref_run_phase(void *h, double *seconds, char *err, int errlen) {
   }
 }
 
+// ref_run_phase with another CH linear solve: the CH step of solve_cahn_hilliard_semi_implicit
+// (phase_field.cpp:667-708) and the rest of advance_phase_state (283-294), with the linear solve of
+// solve_phase_linear_system_eq25 (437-503) handed to `solve` on the interiors (natural order):
+// solve(user, alpha0, rhs, c, n, &iterations, &iterate_residual, &equation_residual) returns 0 or an
+// error. Runs that do not solve CH take the reference's own path.
+typedef int (*RefChSolveFn)(void *user, double alpha0, const double *rhs, double *c, int n, int *iterations,
+                            double *iterate_residual, double *equation_residual);
+
+int ref_run_phase_external_ch(void *h, RefChSolveFn solve, void *user, double *seconds, char *err, int errlen) {
+  auto *run = static_cast<RefRun *>(h);
+  ding::Solver &s = *run->solver;
+  try {
+    if (!s.should_solve_cahn_hilliard()) return ref_run_phase(h, seconds, err, errlen);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int step = run->step;
+    s.current_step_index_ = step;
+    s.c_two_steps_back_ = s.c_previous_step_;
+    s.c_previous_step_ = s.c_;
+    s.rho_previous_step_ = s.rho_;
+    s.eta_previous_step_ = s.eta_;
+    s.u_previous_step_ = s.u_;
+    s.v_previous_step_ = s.v_;
+    s.current_coupling_iteration_ = 0;
+    const ding::Config &cfg = s.cfg_;
+    const int nx = cfg.nx, ny = cfg.ny;
+    ding::Field2D rhs_field(nx, ny, cfg.ghost, 0.0);
+    ding::Field2D explicit_n(nx, ny, cfg.ghost, 0.0);
+    ding::Field2D c_state = s.c_;
+    s.build_phase_explicit_operator(s.c_, s.u_, s.v_, explicit_n);
+    const double alpha0 = (step == 0) ? (1.0 / cfg.dt) : (3.0 / (2.0 * cfg.dt));
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        rhs_field(i, j) = (step == 0) ? s.c_previous_step_(i, j) / cfg.dt + explicit_n(i, j)
+                                      : (4.0 * s.c_previous_step_(i, j) - s.c_two_steps_back_(i, j)) / (2.0 * cfg.dt) +
+                                            2.0 * explicit_n(i, j) - s.phase_explicit_operator_prev_(i, j);
+    s.apply_scalar_bc(rhs_field);
+    std::vector<double> rhs(static_cast<std::size_t>(nx) * ny), c(rhs.size());
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        rhs[static_cast<std::size_t>(i) * ny + j] = rhs_field(i, j);
+        c[static_cast<std::size_t>(i) * ny + j] = c_state(i, j);
+      }
+    int iterations = 0;
+    double iterate_residual = 0.0, equation_residual = 0.0;
+    if (solve(user, alpha0, rhs.data(), c.data(), nx * ny, &iterations, &iterate_residual, &equation_residual) != 0)
+      throw std::runtime_error("external CH solve failed");
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) c_state(i, j) = c[static_cast<std::size_t>(i) * ny + j];
+    s.apply_scalar_bc(c_state);
+    s.last_ch_iterations_ = iterations;
+    s.c_ = c_state;
+    s.apply_scalar_bc(s.c_);
+    s.update_chemical_potential(s.c_, s.mu_);
+    s.last_ch_equation_residual_ = equation_residual;
+    s.phase_advection_rhs_prev_.fill(0.0);
+    s.phase_explicit_operator_prev_ = explicit_n;
+    run->ch_residual = iterate_residual;
+    s.update_materials();
+    s.update_midpoint_materials();
+    s.update_surface_tension_force(s.c_previous_step_, s.c_);
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
 // compute_momentum_fluxes + solve_momentum_predictor of the current step (solver.cpp:265-267).
 int ref_run_momentum(void *h, double *seconds, char *err, int errlen) {
   auto *run = static_cast<RefRun *>(h);

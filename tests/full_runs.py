@@ -46,7 +46,7 @@ def reference_run(ref, spec, steps, scheme="icpcg"):
         it, res, sec = run.pressure()
         info = run.finish()
         log.append({"iterations": it, "residual": res, "pressure_s": sec, "momentum_s": msec,
-                    "momentum_iterations": info["momentum_iterations"]})
+                    "momentum_iterations": info["momentum_iterations"], "ch_iterations": info["ch_iterations"]})
     out = snapshot(run)
     run.close()
     return out, log
@@ -65,15 +65,37 @@ def hip_momentum(run, eng):
     return rep
 
 
-def hybrid_run(ref, eng, spec, steps, scheme=E.PP_ICPCG, precision=E.PP_FP64, ref_scheme="icpcg", momentum=False):
-    """The reference's loop with the HIP pressure step (and, with momentum=True, the HIP momentum
-    predictor too)."""
+def ch_case_of(spec):
+    """The run's CH linear-solve settings as a workloads.ChCase (alpha0 set per step)."""
+    return W.ChCase(nx=spec["nx"], ny=spec["ny"], lx=spec["lx"], ly=spec["ly"], periodic_x=bool(spec["periodic_x"]),
+                    periodic_y=bool(spec["periodic_y"]), cn=spec["cn"], pe=spec["pe"],
+                    max_iterations=spec["ch_inner_iterations"], tolerance=spec["ch_tolerance"])
+
+
+def hybrid_run(ref, eng, spec, steps, scheme=E.PP_ICPCG, precision=E.PP_FP64, ref_scheme="icpcg", momentum=False,
+               ch=False):
+    """The reference's loop with the HIP pressure step (with momentum=True also the HIP momentum
+    predictor, with ch=True also the HIP CH linear solve)."""
+    import dataclasses
     run = ref.RefRun(spec, ref_scheme)
     vbc = W.VelocityBC()
     log = []
+    ch_log = []
+    cc0 = ch_case_of(spec)
+
+    def ch_solve(alpha0, rhs, c):
+        cc = dataclasses.replace(cc0, alpha0=alpha0)
+        c_new, rep = eng.ch_solve(cc, rhs=rhs, c0=c)
+        c[...] = c_new
+        ch_log.append(rep)
+        return rep["iterations"], rep["iterate_residual"], rep["equation_residual"]
+
     for _ in range(steps):
         step = run.step
-        run.phase()
+        if ch:
+            run.phase_external_ch(ch_solve)
+        else:
+            run.phase()
         if momentum:
             mom_rep = hip_momentum(run, eng)
         else:
@@ -95,6 +117,9 @@ def hybrid_run(ref, eng, spec, steps, scheme=E.PP_ICPCG, precision=E.PP_FP64, re
         if momentum:
             entry["momentum_iterations"] = mom_rep["iterations"]
             entry["momentum_kernel_ms"] = mom_rep["kernel_ms"]
+        if ch:
+            entry["ch_iterations"] = ch_log[-1]["iterations"]
+            entry["ch_kernel_ms"] = ch_log[-1]["kernel_ms"]
         log.append(entry)
     out = snapshot(run)
     run.close()

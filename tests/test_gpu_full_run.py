@@ -58,3 +58,20 @@ def test_full_run_with_hip_momentum_and_pressure(engine, ref_lib, name, spec, st
     assert [e["momentum_iterations"] for e in log] == [e["momentum_iterations"] for e in ref_log]
     assert [e["iterations"] for e in log] == [e["iterations"] for e in ref_log]
     assert max(d["pressure"], d["u"], d["v"]) <= 1e-7 and d["centroid"] <= 1e-12
+
+
+@pytest.mark.parametrize("name,spec,steps", RUNS, ids=[r[0] for r in RUNS])
+def test_full_run_with_hip_ch_momentum_and_pressure(engine, ref_lib, name, spec, steps):
+    """The CH linear solve, the momentum predictor and the pressure step on the device inside the
+    reference's loop: the same iteration counts of all three solvers on every step, and the final
+    state within reordering noise of the reference-only run."""
+    ref_out, ref_log = reference_run(ref_lib, spec, steps)
+    out, log = hybrid_run(ref_lib, engine, spec, steps, momentum=True, ch=True)
+    d = deviation(out, ref_out)
+    print(f"[full run] {name} with hip CH + momentum + pressure: CH iterations reference "
+          f"{[e['ch_iterations'] for e in ref_log]}, hip {[e['ch_iterations'] for e in log]}; deviation {d}; "
+          f"CH device {sum(e['ch_kernel_ms'] for e in log) / steps:.2f} ms/step")
+    assert [e["ch_iterations"] for e in log] == [e["ch_iterations"] for e in ref_log]
+    assert [e["momentum_iterations"] for e in log] == [e["momentum_iterations"] for e in ref_log]
+    assert [e["iterations"] for e in log] == [e["iterations"] for e in ref_log]
+    assert max(d["pressure"], d["u"], d["v"]) <= 1e-7 and d["centroid"] <= 1e-12
