@@ -68,9 +68,10 @@ def test_oracle_operators_match_reference_bitwise(oracle_lib, ref_lib, name, cas
 
 def _golden_files():
     """The pressure-solve fixtures (velcorr_*.npz hold the correction step, test_velocity_oracle.py;
-    bicgstab_*.npz the momentum solve, test_bicgstab_oracle.py)."""
+    bicgstab_*.npz the momentum solve, test_bicgstab_oracle.py; momentum_*.npz the momentum predictor,
+    test_momentum_oracle.py; timestep_*.npz whole timestep states, test_timestep_oracle.py)."""
     return sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith(("velcorr_", "bicgstab_")))
+                  if not os.path.basename(p).startswith(("velcorr_", "bicgstab_", "momentum_", "timestep_")))
 
 
 def _case_from_npz(z):
