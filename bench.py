@@ -340,6 +340,182 @@ def momentum_leg(eng, torch, dev, steps, cpu_seconds, cpu=True, side=192):
     return out
 
 
+def momentum_predictor_bytes(n, nnz, iterations, cells_u, cells_v):
+    """Algorithmic HBM bytes of one device momentum predictor (pp_momentum_predictor_device) with n
+    unknowns, nnz matrix entries and ghosted face fields of cells_u / cells_v doubles:
+    assembly reads u, v, rho, eta, eta_prev, the surface force and the previous convection field of
+    each face (9 doubles per unknown) and writes the row slots (9 columns + 9 values), the count,
+    rhs and x0; the count / scan / compact passes read the counts twice and the slots once and write
+    row_ptr and the CSR; the convection pass reads u, v and writes u_prev, v_prev; BiCGSTAB moves
+    bicgstab_bytes per iteration plus its setup (b, x, CSR, M^-1, r, r_hat); the scatter reads x, u,
+    v and writes u_star, v_star."""
+    assembly = n * (9 * 8 + 9 * 12 + 4 + 8 + 8)
+    compact = n * (2 * 4 + 9 * 12) + 4 * (n + 1) + 12 * nnz
+    convection = 2 * 8 * (cells_u + cells_v)
+    setup = 12 * nnz + 4 * (n + 1) + 6 * 8 * n
+    scatter = 8 * n + 2 * 8 * (cells_u + cells_v)
+    return assembly + compact + convection + setup + iterations * bicgstab_bytes(n, nnz) + scatter
+
+
+def momentum_predictor_leg(eng, torch, dev, steps, cpu_seconds, cpu=True):
+    """The whole momentum predictor on the device (Solver::compute_momentum_fluxes +
+    solve_momentum_predictor, momentum.cpp:55-502): assembly of the implicit u-v system from the
+    Field2D buffers, BiCGSTAB from x0 = u / v (50 iterations, absolute 1e-8), scatter and ghost fill
+    of u_star_ / v_star_. The grid and fluid are the rising-bubble run's: 192x384 on [0,1]x[0,2],
+    density ratio 1000, viscosity ratio 100, Re 10, gravity -0.98, dt 5e-5, no-slip walls; seeded
+    velocity, surface-force and convection fields. One pp_momentum_predictor_device call per step
+    (the stored convection fields restored before each)."""
+    mc = W.momentum_case(192, 384, ly=2.0, density_ratio=1000.0, viscosity_ratio=100.0, re=10.0, dt=5.0e-5,
+                         body_force_y=-0.98, radius=0.25, velocity_scale=0.05)
+    names = ("u", "v", "rho_mid", "eta", "eta_prev", "surface_fx_u", "surface_fy_v", "u_prev", "v_prev")
+    d = {k: torch.from_numpy(np.ascontiguousarray(getattr(mc, k), dtype=np.float64)).to(dev) for k in names}
+    keep = {k: d[k].clone() for k in ("u_prev", "v_prev")}
+    d["u_star"], d["v_star"] = torch.zeros_like(d["u"]), torch.zeros_like(d["v"])
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def run():
+        d["u_prev"].copy_(keep["u_prev"])
+        d["v_prev"].copy_(keep["v_prev"])
+        return eng.momentum_predictor_device(mc, *(d[k].data_ptr() for k in names), d["u_star"].data_ptr(),
+                                             d["v_star"].data_ptr(), stream=stream)
+
+    for _ in range(3):
+        rep = run()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rep = run()
+        kernel_ms.append(rep["kernel_ms"])
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t0) / steps
+    kms = float(np.mean(kernel_ms))
+    n = mc.unknowns
+    nnz = 9 * n  # upper bound: 5 own-component and 4 cross entries per row
+    alg = momentum_predictor_bytes(n, nnz, rep["iterations"], d["u"].numel(), d["v"].numel())
+    out = {"value": 1.0 / elapsed, "unit": "predictor steps/s", "ms_per_step": elapsed * 1e3, "device_ms": kms,
+           "iterations": rep["iterations"], "absolute_residual": rep["absolute_residual"], "unknowns": n,
+           "grid": "192x384 rising-bubble fluid (density ratio 1000, viscosity ratio 100, Re 10, dt 5e-5)",
+           "data": "synthetic fields of the rising-bubble run's shape",
+           "kernels": "momentum_assemble/count/block_scan/compact/convection/ghost + BiCGSTAB passes + "
+                      "momentum_scatter/ghost",
+           "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "algorithmic_bytes_per_step": alg,
+                        "note": "whole predictor (assembly, solve, scatter); nnz taken as its bound 9 n"}}
+    if cpu:
+        try:
+            from oracle import ref
+            if not ref.available():
+                raise FileNotFoundError
+        except (ImportError, FileNotFoundError):
+            log("[bench] WARNING: momentum predictor CPU baseline skipped: oracle/_ref/libdingref.so is not present")
+            return out
+        u_gpu, v_gpu = d["u_star"].cpu().numpy(), d["v_star"].cpu().numpy()
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+        try:
+            u_r, v_r, _, _, it_r, _ = ref.momentum_predictor(mc)
+            timing = []
+            reps = max(3, int(cpu_seconds / 0.25))
+            ref.momentum_predictor(mc, repeats=reps, timing=timing)
+        finally:
+            os.sched_setaffinity(0, old)
+        scale = max(np.abs(u_r).max(), np.abs(v_r).max())
+        out["parity"] = {"iterations_gpu": rep["iterations"], "iterations_cpu": it_r,
+                         "max_abs_diff_over_max_abs": float(max(np.abs(u_gpu - u_r).max(), np.abs(v_gpu - v_r).max())
+                                                            / scale)}
+        out["cpu_baseline"] = {"value": 1.0 / timing[0], "unit": "predictor steps/s", "cores": 1, "kind": "reference",
+                               "sample": f"{reps} calls of compute_momentum_fluxes + solve_momentum_predictor on the "
+                                         f"same fields, median {timing[0] * 1e3:.1f} ms"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
+def ch_diag_bytes(n, nnz, iterations):
+    """Algorithmic HBM bytes of one CH solve with the diagonal preconditioner as full-chip passes:
+    setup (CSR 12 nnz + row_ptr, x, b, M^-1 read; r, z, p written); per iteration pass A (CSR, z and
+    p_old read, p and Ap written) and pass B (p, x, r, Ap, M^-1 read; x, r, z written); the closing
+    residual pass (CSR, x, b)."""
+    csr = 12 * nnz + 4 * (n + 1)
+    setup = csr + 3 * 8 * n + 3 * 8 * n
+    per_it = (csr + 2 * 8 * n + 2 * 8 * n) + (5 * 8 * n + 3 * 8 * n)
+    return setup + iterations * per_it + csr + 2 * 8 * n
+
+
+def ch_leg(eng, torch, dev, steps, cpu_seconds, cpu=True):
+    """The Cahn-Hilliard semi-implicit linear solve (Solver::solve_phase_linear_system_eq25,
+    phase_field.cpp:437-503) on the device: PCG on the 13-point operator alpha0 I - beta1 Lap +
+    beta2 Lap^2 with the ILDLT(0) preconditioner, or the diagonal one where ILDLT(0) fails (as it does
+    on this grid, in the reference too), from x0 = c, 30 iterations to 1e-8 (the reference's
+    ch_inner_iterations / ch_tolerance defaults). Grid 192x384 on [0,1]x[0,2] (the rising-bubble
+    run), Cn 0.01, Pe 100, dt 5e-5 (alpha0 = 3 / (2 dt)); seeded right-hand side around the bubble's
+    phase field. The operator and factor are built once and cached, as the reference's solver caches
+    them across steps; one pp_ch_solve_device call per step, c restored before each."""
+    cc = W.ch_case(192, 384, ly=2.0, radius=0.25, dt=5.0e-5, seed=0)
+    d_rhs = torch.from_numpy(np.ascontiguousarray(cc.rhs)).to(dev)
+    c0 = torch.from_numpy(np.ascontiguousarray(cc.c0)).to(dev)
+    d_c = c0.clone()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def run():
+        d_c.copy_(c0)
+        return eng.ch_solve_device(cc, d_rhs.data_ptr(), d_c.data_ptr(), stream=stream)
+
+    for _ in range(2):
+        rep = run()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rep = run()
+        kernel_ms.append(rep["kernel_ms"])
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t0) / steps
+    kms = float(np.mean(kernel_ms))
+    n = cc.nx * cc.ny
+    nnz = 13 * n  # interior rows: the 13-point stencil (walls trim a few)
+    alg = ch_diag_bytes(n, nnz, rep["iterations"])
+    out = {"value": 1.0 / elapsed, "unit": "CH solves/s", "ms_per_step": elapsed * 1e3, "device_ms": kms,
+           "iterations": rep["iterations"], "preconditioner": rep["solver_name"],
+           "equation_residual": rep["equation_residual"], "levels": rep["levels"], "unknowns": n,
+           "data": "synthetic phase field and right-hand side on the rising-bubble grid",
+           "kernels": "ch_diag_setup + per iteration ch_diag_pass_a / ch_diag_pass_b + ch_diag_eq / finish",
+           "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "algorithmic_bytes_per_solve": alg,
+                        "note": "ILDLT(0) of this operator fails at this grid, as in the reference, so the solve "
+                                "is the diagonal-preconditioned CG (full-chip passes); whole solve over HIP-event "
+                                "time"}}
+    if cpu:
+        try:
+            from oracle import ref
+            if not ref.available():
+                raise FileNotFoundError
+        except (ImportError, FileNotFoundError):
+            log("[bench] WARNING: CH CPU baseline skipped: oracle/_ref/libdingref.so is not present")
+            return out
+        c_gpu = d_c.cpu().numpy()
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+        try:
+            rch = ref.RefCh(cc)
+            c_r, it_r, _, _, name_r = rch.solve(cc.alpha0, cc.rhs, cc.c0)
+            timing = []
+            reps = max(3, int(cpu_seconds / 0.1))
+            rch.solve(cc.alpha0, cc.rhs, cc.c0, repeats=reps, timing=timing)
+            rch.close()
+        finally:
+            os.sched_setaffinity(0, old)
+        out["parity"] = {"iterations_gpu": rep["iterations"], "iterations_cpu": it_r, "solver_cpu": name_r,
+                         "max_abs_diff_over_max_abs": float(np.abs(c_gpu - c_r).max() / np.abs(c_r).max())}
+        out["cpu_baseline"] = {"value": 1.0 / timing[0], "unit": "CH solves/s", "cores": 1, "kind": "reference",
+                               "sample": f"{reps} solves with the reference's cached operator, median "
+                                         f"{timing[0] * 1e3:.1f} ms"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def batched_cpu_baseline(cases, threads):
     """Reference icpcg on every system of the batch, one system per thread (the reference is
     single-threaded; independent solves run concurrently), wall time of the whole batch."""
@@ -524,6 +700,15 @@ def main():
                                                            side=1024)
     log(f"[bench] bicgstab 1024: {line['momentum_bicgstab']['large_1024']['device_ms']:.3f} ms, "
         f"{line['momentum_bicgstab']['large_1024']['roofline']['frac'] * 100:.1f}% of HBM")
+
+    log("[bench] momentum predictor on the device (assembly + BiCGSTAB + scatter), 192x384")
+    line["momentum_predictor"] = momentum_predictor_leg(eng, torch, dev, args.steps * 2, 3.0,
+                                                        cpu=not args.no_cpu_baseline)
+    log(f"[bench] momentum predictor: {line['momentum_predictor']['device_ms']:.3f} ms, "
+        f"{line['momentum_predictor']['iterations']} iterations")
+    log("[bench] CH linear solve on the device, 192x384")
+    line["ch_solve"] = ch_leg(eng, torch, dev, args.steps, 3.0, cpu=not args.no_cpu_baseline)
+    log(f"[bench] CH solve: {line['ch_solve']['device_ms']:.3f} ms, {line['ch_solve']['iterations']} iterations")
 
     batch_cases = None
     if not args.no_batched:

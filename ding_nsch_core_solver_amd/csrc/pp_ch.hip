@@ -14,11 +14,19 @@
 // (2 nx + ny - 2 of them); a periodic y direction chains (i, 0) to (i - 1, ny - 1) and the levels
 // become the rows themselves.
 //
-// Reductions (dot products, norms) use the 512-thread canonical tree over the natural row order:
-// thread t sums rows t, t + 512, ..., then the 64-lane tree (offsets 32..1) and the 8-wave tree
-// (offsets 4..1). The CSR row sums of the SpMV follow apply_matrix (235-250).
+// Reductions (dot products, norms) of this one-workgroup PCG use the 512-thread canonical tree
+// over the natural row order: thread t sums rows t, t + 512, ..., then the 64-lane tree (offsets
+// 32..1) and the 8-wave tree (offsets 4..1). The CSR row sums of the SpMV follow apply_matrix
+// (235-250).
+//
+// With the diagonal preconditioner (the one the reference falls back to whenever ILDLT(0) fails,
+// which is the case at the production grid sizes) nothing is sequential, and the CG runs as
+// full-chip passes instead (ch_diag_* below): one 512-row chunk per workgroup, the chunked
+// reductions of pp_chunked.hpp, the scalar recurrence and the reference's stop and breakdown tests
+// on the device.
 #include <hip/hip_runtime.h>
 
+#include "pp_chunked.hpp"
 #include "pp_device.hpp"
 #include "pp_kernels.hpp"
 
@@ -276,6 +284,200 @@ __global__ void __launch_bounds__(kThreads) ch_pcg_kernel(ChArgs a, int ildlt) {
   }
 }
 
+// ---- diagonal-preconditioned CG as full-chip passes ------------------------------------------
+
+using chunked::row_chunk;
+using chunked::chunk_partials;
+using chunked::dot_sums;
+constexpr int kRows = chunked::kRows;
+static_assert(kRows == kChRows, "the host sizes the CH chunks with kChRows");
+
+__device__ __forceinline__ double max0(double v) { return v < 0.0 ? 0.0 : v; }            // std::max(v, 0.0)
+__device__ __forceinline__ double floor30(double v) { return v < 1.0e-30 ? 1.0e-30 : v; }  // std::max(v, 1e-30)
+
+// The state's stop flag as one value per workgroup (thread 0 reads it, the barrier hands it on).
+__device__ __forceinline__ bool block_stopped(const ChDiagArgs &a) {
+  __shared__ int flag;
+  if (threadIdx.x == 0) flag = a.state->stopped;
+  __syncthreads();
+  return flag != 0;
+}
+
+// The K sums of partial slots [slot, slot + K), summed here or taken from the sum pass.
+template <int K>
+__device__ __forceinline__ void ch_slot_sums(const ChDiagArgs &a, int slot, double (&out)[K]) {
+  if (a.presummed) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = a.state->sums[slot + k];
+  } else {
+    dot_sums<K>(a.partials + static_cast<size_t>(slot) * a.chunks, a.chunks, out);
+  }
+}
+
+__device__ __forceinline__ double csr_row(const ChDiagArgs &a, const double *x, int row) {
+  double s = 0.0;
+  const int end = a.row_ptr[row + 1];
+  for (int p = a.row_ptr[row]; p < end; ++p) s += a.val[p] * x[a.col[p]];
+  return s;
+}
+
+// Workgroup 0 records a stop with the report fields.
+__device__ __forceinline__ void ch_record(const ChDiagArgs &a, int status, int iterations, double abs_res,
+                                          double rel_res, double iter_res) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ChDiagState &s = *a.state;
+    s.iterations = iterations;
+    s.abs_res = abs_res;
+    s.rel_res = rel_res;
+    s.iter_res = iter_res;
+    s.status = status;
+    s.stopped = 1;
+  }
+}
+
+// r = b - A x, z = M^-1 r, p = z (ch_sparse_krylov.hpp:530-583); partials b.b, r.r, r.z. z and r.z
+// are formed before the initial convergence test, which pass A of iteration 0 makes; they go unused
+// when it stops.
+__global__ void __launch_bounds__(kRows) ch_diag_setup_kernel(ChDiagArgs a) {
+  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (q < a.n) {
+    const double b = a.rhs[q];
+    const double r = b - csr_row(a, a.x, q);
+    const double z = a.dinv[q] * r;
+    a.r[q] = r;
+    a.z[q] = z;
+    a.p[0][q] = z;
+    v[0] = v[0] + b * b;
+    v[1] = v[1] + r * r;
+    v[2] = v[2] + r * z;
+  }
+  chunk_partials<3>(v, a.partials + kChSlotSetup * static_cast<size_t>(a.chunks), a.chunks);
+}
+
+// Pass A of iteration k (final: the report after the limit's last iteration). The tests of the
+// previous step: at k = 0 the initial residual and r.z > 0 (540-583); after an iteration its
+// residuals (603-612), then the new r.z > 0 and beta (640-652). Then p_k = z + beta p_{k-1} (p_0 = z),
+// A p_k with the neighbours' p_k formed from their z and p_{k-1} by the same expression, and the
+// partial p.Ap.
+__global__ void __launch_bounds__(kRows) ch_diag_pass_a_kernel(ChDiagArgs a, int k, int final) {
+  if (block_stopped(a)) return;
+  const double tol = a.tolerance;
+  double beta = 0.0;
+  if (k == 0) {
+    double s[3];
+    ch_slot_sums<3>(a, kChSlotSetup, s);
+    const double bnorm = sqrt(max0(s[0]));
+    const double abs_res = sqrt(max0(s[1]));
+    const double rel = abs_res / floor30(bnorm);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.state->bnorm = bnorm;
+      a.state->bsq = s[0];
+      a.state->rz[0] = s[2];
+    }
+    if (rel < tol || abs_res < tol) {
+      ch_record(a, kChOk, 0, abs_res, rel, 0.0);
+      return;
+    }
+    if (!isfinite(s[2]) || s[2] <= 0.0) {
+      ch_record(a, kChRz0Fail, 0, abs_res, rel, 0.0);
+      return;
+    }
+  } else {
+    double u[4];
+    ch_slot_sums<4>(a, kChSlotUpd, u);
+    const double iter_res = sqrt(u[0] / floor30(u[1]));
+    const double abs_res = sqrt(max0(u[2]));
+    const double rel = abs_res / floor30(a.state->bnorm);
+    if (rel < tol || abs_res < tol || final) {
+      ch_record(a, kChOk, k, abs_res, rel, iter_res);
+      return;
+    }
+    if (!isfinite(u[3]) || u[3] <= 0.0) {
+      ch_record(a, kChRzFail, k, abs_res, rel, iter_res);
+      return;
+    }
+    beta = u[3] / a.state->rz[(k - 1) & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.state->rz[k & 1] = u[3];
+  }
+  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  const double *__restrict__ z = a.z;
+  const double *__restrict__ po = a.p[(k - 1) & 1];
+  double v[1] = {0.0};
+  if (q < a.n) {
+    double s = 0.0;
+    const int end = a.row_ptr[q + 1];
+    for (int e = a.row_ptr[q]; e < end; ++e) {
+      const int c = a.col[e];
+      const double pc = k == 0 ? z[c] : z[c] + beta * po[c];
+      s += a.val[e] * pc;
+    }
+    const double pq = k == 0 ? z[q] : z[q] + beta * po[q];
+    a.p[k & 1][q] = pq;
+    a.ap[q] = s;
+    v[0] = v[0] + pq * s;
+  }
+  chunk_partials<1>(v, a.partials + kChSlotPap * static_cast<size_t>(a.chunks), a.chunks);
+}
+
+// Pass B of iteration k: p^T A p valid (588-594), alpha, x += alpha p, r -= alpha A p, z = M^-1 r
+// (595-601, 614-621); partials delta.delta, x.x, r.r, r.z.
+__global__ void __launch_bounds__(kRows) ch_diag_pass_b_kernel(ChDiagArgs a, int k) {
+  if (block_stopped(a)) return;
+  double pap[1];
+  ch_slot_sums<1>(a, kChSlotPap, pap);
+  if (!isfinite(pap[0]) || fabs(pap[0]) < 1.0e-30) {
+    ch_record(a, kChPapFail, k, 0.0, 0.0, 0.0);
+    return;
+  }
+  const double alpha = a.state->rz[k & 1] / pap[0];
+  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (q < a.n) {
+    const double delta = alpha * a.p[k & 1][q];
+    const double xn = a.x[q] + delta;
+    const double rn = a.r[q] - alpha * a.ap[q];
+    const double zn = a.dinv[q] * rn;
+    a.x[q] = xn;
+    a.r[q] = rn;
+    a.z[q] = zn;
+    v[0] = v[0] + delta * delta;
+    v[1] = v[1] + xn * xn;
+    v[2] = v[2] + rn * rn;
+    v[3] = v[3] + rn * zn;
+  }
+  chunk_partials<4>(v, a.partials + kChSlotUpd * static_cast<size_t>(a.chunks), a.chunks);
+}
+
+// |rhs - A x|^2 of the returned x (phase_field.cpp:481-502), per chunk.
+__global__ void __launch_bounds__(kRows) ch_diag_eq_kernel(ChDiagArgs a) {
+  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  double v[1] = {0.0};
+  if (q < a.n) {
+    const double d = a.rhs[q] - csr_row(a, a.x, q);
+    v[0] = v[0] + d * d;
+  }
+  chunk_partials<1>(v, a.partials + kChSlotEq * static_cast<size_t>(a.chunks), a.chunks);
+}
+
+__global__ void __launch_bounds__(kRows) ch_diag_finish_kernel(ChDiagArgs a) {
+  double eq[1];
+  dot_sums<1>(a.partials + kChSlotEq * static_cast<size_t>(a.chunks), a.chunks, eq);
+  if (threadIdx.x == 0) a.state->eq_res = sqrt(eq[0] / floor30(a.state->bsq));
+}
+
+// Large systems: the K sums of slots [first, first + K) once, into the state.
+template <int K>
+__global__ void __launch_bounds__(kRows) ch_diag_sum_kernel(ChDiagArgs a, int first) {
+  if (block_stopped(a)) return;
+  double d[K];
+  dot_sums<K>(a.partials + static_cast<size_t>(first) * a.chunks, a.chunks, d);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) a.state->sums[first + k] = d[k];
+  }
+}
+
 }  // namespace
 
 hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream) {
@@ -285,6 +487,41 @@ hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream) {
 
 hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream) {
   hipLaunchKernelGGL(ch_pcg_kernel, dim3(1), dim3(kThreads), 0, stream, a, ildlt ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_ch_diag_setup(const ChDiagArgs &a, hipStream_t stream) {
+  ch_diag_setup_kernel<<<dim3(a.chunks), dim3(kRows), 0, stream>>>(a);
+  return hipGetLastError();
+}
+
+// Before pass A of iteration k: the sums it reads (the setup's at k = 0, else pass B's).
+static void ch_presum_a(const ChDiagArgs &a, int k, hipStream_t stream) {
+  if (!a.presummed) return;
+  if (k == 0)
+    ch_diag_sum_kernel<3><<<dim3(1), dim3(kRows), 0, stream>>>(a, kChSlotSetup);
+  else
+    ch_diag_sum_kernel<4><<<dim3(1), dim3(kRows), 0, stream>>>(a, kChSlotUpd);
+}
+
+hipError_t launch_ch_diag_iteration(const ChDiagArgs &a, int k, hipStream_t stream) {
+  const dim3 grid(a.chunks), block(kRows);
+  ch_presum_a(a, k, stream);
+  ch_diag_pass_a_kernel<<<grid, block, 0, stream>>>(a, k, 0);
+  if (a.presummed) ch_diag_sum_kernel<1><<<dim3(1), block, 0, stream>>>(a, kChSlotPap);
+  ch_diag_pass_b_kernel<<<grid, block, 0, stream>>>(a, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_ch_diag_report(const ChDiagArgs &a, int iterations, hipStream_t stream) {
+  ch_presum_a(a, iterations, stream);
+  ch_diag_pass_a_kernel<<<dim3(a.chunks), dim3(kRows), 0, stream>>>(a, iterations, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_ch_diag_finish(const ChDiagArgs &a, hipStream_t stream) {
+  ch_diag_eq_kernel<<<dim3(a.chunks), dim3(kRows), 0, stream>>>(a);
+  ch_diag_finish_kernel<<<dim3(1), dim3(kRows), 0, stream>>>(a);
   return hipGetLastError();
 }
 

@@ -100,6 +100,11 @@ struct pp_ctx {
     pp::ChArgs args{};
     size_t stage_cap = 0;
     double *stage = nullptr;       // host entry: rhs and c
+    // full-chip diagonal CG: partials, state (pinned copies: two groups and the final state), events
+    double *d_part = nullptr;
+    pp::ChDiagState *d_dstate = nullptr;
+    pp::ChDiagState *h_dstate = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
   } ch;
 
   // PCG monitor log of the host entry point (pp_solve_monitored)
@@ -748,6 +753,11 @@ void pp_destroy(pp_ctx *ctx) {
   if (ctx->ch.d_state) (void)hipFree(ctx->ch.d_state);
   if (ctx->ch.h_state) (void)hipHostFree(ctx->ch.h_state);
   if (ctx->ch.stage) (void)hipFree(ctx->ch.stage);
+  if (ctx->ch.d_part) (void)hipFree(ctx->ch.d_part);
+  if (ctx->ch.d_dstate) (void)hipFree(ctx->ch.d_dstate);
+  if (ctx->ch.h_dstate) (void)hipHostFree(ctx->ch.h_dstate);
+  if (ctx->ch.ev[0]) (void)hipEventDestroy(ctx->ch.ev[0]);
+  if (ctx->ch.ev[1]) (void)hipEventDestroy(ctx->ch.ev[1]);
   if (ctx->mom_int) (void)hipFree(ctx->mom_int);
   if (ctx->mom_dbl) (void)hipFree(ctx->mom_dbl);
   if (ctx->mom_stage) (void)hipFree(ctx->mom_stage);
@@ -1792,8 +1802,10 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
   // device buffers
   if (C.d_int) (void)hipFree(C.d_int);
   if (C.d_dbl) (void)hipFree(C.d_dbl);
+  if (C.d_part) (void)hipFree(C.d_part);
   C.d_int = nullptr;
   C.d_dbl = nullptr;
+  C.d_part = nullptr;
   const size_t nl = lt_row.size();
   const size_t ints = (n + 1) + nnz + n + (nlev + 1) + n + (n + 1) + 2 * nl;
   const size_t dbls = 2 * static_cast<size_t>(nnz) + 7 * static_cast<size_t>(n);
@@ -1801,6 +1813,12 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
   PP_CH_CHECK(hipMalloc(&C.d_dbl, dbls * sizeof(double)));
   if (!C.d_state) PP_CH_CHECK(hipMalloc(&C.d_state, sizeof(pp::ChState)));
   if (!C.h_state) PP_CH_CHECK(hipHostMalloc(&C.h_state, sizeof(pp::ChState), hipHostMallocDefault));
+  const size_t chunks = (static_cast<size_t>(n) + pp::kChRows - 1) / pp::kChRows;
+  PP_CH_CHECK(hipMalloc(&C.d_part, pp::kChSlots * chunks * sizeof(double)));
+  if (!C.d_dstate) PP_CH_CHECK(hipMalloc(&C.d_dstate, sizeof(pp::ChDiagState)));
+  if (!C.h_dstate) PP_CH_CHECK(hipHostMalloc(&C.h_dstate, 3 * sizeof(pp::ChDiagState), hipHostMallocDefault));
+  for (hipEvent_t &e : C.ev)
+    if (!e) PP_CH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   int *ip = C.d_int;
   double *dp = C.d_dbl;
   pp::ChArgs &a = C.args;
@@ -1870,6 +1888,61 @@ int ch_status_code(int status) {
   }
 }
 
+constexpr int kChQueue = 8;  // CG iterations queued between the host's state checks
+
+// The diagonal-preconditioned CG as full-chip passes from x = d_c (solve_preconditioned_cg with the
+// diagonal preconditioner): the passes run the reference's recurrence and tests on the device; the
+// host queues iterations in groups and reads the state between them, with the next group queued.
+// Leaves the outcome in C.h_dstate[2].
+int run_ch_diag(pp_ctx *ctx, const pp_ch_problem *cp, const double *d_rhs, double *d_c, pp_ch_report *report,
+                hipStream_t s) {
+  auto &C = ctx->ch;
+  pp::ChDiagArgs a{};
+  a.n = C.n;
+  a.chunks = (C.n + pp::kChRows - 1) / pp::kChRows;
+  a.presummed = a.chunks > pp::kChPresumChunks ? 1 : 0;
+  a.max_iterations = cp->max_iterations;
+  a.tolerance = cp->tolerance;
+  a.row_ptr = C.args.row_ptr;
+  a.col = C.args.col;
+  a.val = C.args.val;
+  a.dinv = C.args.dinv;
+  a.rhs = d_rhs;
+  a.x = d_c;
+  a.r = C.args.r;
+  a.z = C.args.z;
+  a.p[0] = C.args.p;
+  a.p[1] = C.args.y;  // y is the ILDLT sweeps' vector; the diagonal CG uses it as p of odd iterations
+  a.ap = C.args.ap;
+  a.partials = C.d_part;
+  a.state = C.d_dstate;
+  const int limit = std::max(1, cp->max_iterations);
+  PP_CH_CHECK(hipMemsetAsync(a.state, 0, sizeof(pp::ChDiagState), s));
+  PP_CH_CHECK(pp::launch_ch_diag_setup(a, s));
+  pp::ChDiagState *hs = C.h_dstate;
+  const int groups = (limit + kChQueue - 1) / kChQueue;
+  auto queue_group = [&](int g) -> int {
+    for (int k = g * kChQueue; k < std::min(limit, (g + 1) * kChQueue); ++k)
+      PP_CH_CHECK(pp::launch_ch_diag_iteration(a, k, s));
+    PP_CH_CHECK(hipMemcpyAsync(&hs[g & 1], a.state, sizeof(pp::ChDiagState), hipMemcpyDeviceToHost, s));
+    PP_CH_CHECK(hipEventRecord(C.ev[g & 1], s));
+    return PP_OK;
+  };
+  if (int rc = queue_group(0)) return rc;
+  bool stopped = false;
+  for (int g = 0; g < groups && !stopped; ++g) {
+    if (g + 1 < groups)
+      if (int rc = queue_group(g + 1)) return rc;
+    PP_CH_CHECK(hipEventSynchronize(C.ev[g & 1]));
+    stopped = hs[g & 1].stopped != 0;
+  }
+  if (!stopped) PP_CH_CHECK(pp::launch_ch_diag_report(a, limit, s));
+  PP_CH_CHECK(pp::launch_ch_diag_finish(a, s));
+  PP_CH_CHECK(hipMemcpyAsync(&hs[2], a.state, sizeof(pp::ChDiagState), hipMemcpyDeviceToHost, s));
+  PP_CH_CHECK(hipStreamSynchronize(s));
+  return PP_OK;
+}
+
 int run_ch(pp_ctx *ctx, const pp_ch_problem *cp, const double *d_rhs, double *d_c, pp_ch_report *report,
            hipStream_t s) {
   if (int rc = ensure_ch(ctx, cp, report, s)) return rc;
@@ -1880,36 +1953,55 @@ int run_ch(pp_ctx *ctx, const pp_ch_problem *cp, const double *d_rhs, double *d_
   a.max_iterations = cp->max_iterations;
   a.tolerance = cp->tolerance;
   PP_CH_CHECK(hipEventRecord(ctx->ev_start, s));
-  // solve with the cached preconditioner; a breakdown with ILDLT switches the operator to the
-  // diagonal preconditioner for good and solves again from the x the failed attempt left
-  // (phase_field.cpp:465-474)
+  // With ILDLT(0) the one-workgroup PCG; a breakdown there switches the operator to the diagonal
+  // preconditioner for good, and the diagonal CG runs again from the x the failed attempt left
+  // (phase_field.cpp:465-474). The diagonal CG runs as full-chip passes.
   bool fell_back = false;
-  for (;;) {
-    PP_CH_CHECK(pp::launch_ch_pcg(a, C.kind == PP_CH_ILDLT, s));
+  int status = pp::kChOk, iterations = 0;
+  double abs_res = 0.0, rel_res = 0.0, iter_res = 0.0, eq_res = 0.0;
+  if (C.kind == PP_CH_ILDLT) {
+    PP_CH_CHECK(pp::launch_ch_pcg(a, true, s));
     PP_CH_CHECK(hipMemcpyAsync(C.h_state, C.d_state, sizeof(pp::ChState), hipMemcpyDeviceToHost, s));
     PP_CH_CHECK(hipStreamSynchronize(s));
-    if (C.h_state->status == pp::kChOk || C.kind != PP_CH_ILDLT) break;
-    C.kind = PP_CH_DIAGONAL;  // later solves report SparsePCG[Diagonal], as the reference's cached one
-    fell_back = true;
+    const pp::ChState &st = *C.h_state;
+    status = st.status;
+    iterations = st.iterations;
+    abs_res = st.abs_res;
+    rel_res = st.rel_res;
+    iter_res = st.iter_res;
+    eq_res = st.eq_res;
+    if (status != pp::kChOk) {
+      C.kind = PP_CH_DIAGONAL;  // later solves report SparsePCG[Diagonal], as the reference's cached one
+      fell_back = true;
+    }
+  }
+  if (C.kind != PP_CH_ILDLT) {
+    if (int rc = run_ch_diag(ctx, cp, d_rhs, d_c, report, s)) return rc;
+    const pp::ChDiagState &st = C.h_dstate[2];
+    status = st.status;
+    iterations = st.iterations;
+    abs_res = st.abs_res;
+    rel_res = st.rel_res;
+    iter_res = st.iter_res;
+    eq_res = st.eq_res;
   }
   PP_CH_CHECK(hipEventRecord(ctx->ev_stop, s));
   PP_CH_CHECK(hipEventSynchronize(ctx->ev_stop));
   float ms = 0.0f;
   PP_CH_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
   ctx->last_kernel_ms = ms;
-  const pp::ChState &st = *C.h_state;
   if (report) {
     std::memset(report, 0, sizeof(*report));
-    report->iterations = st.iterations;
+    report->iterations = iterations;
     report->preconditioner = fell_back ? PP_CH_DIAGONAL_FALLBACK : C.kind;
-    report->iterate_residual = st.iter_res;
-    report->absolute_residual = st.abs_res;
-    report->relative_residual = st.rel_res;
-    report->equation_residual = st.eq_res;
+    report->iterate_residual = iter_res;
+    report->absolute_residual = abs_res;
+    report->relative_residual = rel_res;
+    report->equation_residual = eq_res;
     report->kernel_ms = ms;
     report->levels = C.nlev;
   }
-  if (st.status != pp::kChOk) return ch_fail(ctx, report, ch_status_code(st.status), ch_status_text(st.status));
+  if (status != pp::kChOk) return ch_fail(ctx, report, ch_status_code(status), ch_status_text(status));
   ctx->last_error.clear();
   return PP_OK;
 }

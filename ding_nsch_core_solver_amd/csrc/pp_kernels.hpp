@@ -187,6 +187,39 @@ struct ChArgs {
 hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream);
 hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream);
 
+// The CH solve's CG with the diagonal preconditioner as full-chip passes (the chunked reductions of
+// pp_chunked.hpp): a setup pass, then per iteration pass A (the stop tests of the previous step,
+// beta, p = z + beta p and A p) and pass B (alpha, the x / r update, z = M^-1 r), and after the
+// stop the residual of the returned x. Scalars live in the state block; a stop recorded there
+// makes every later pass return at once.
+// Partial slots (each `chunks` long): setup b.b, r.r, r.z; pass A p.Ap; pass B delta.delta, x.x,
+// r.r, r.z; the closing |rhs - A x|^2.
+constexpr int kChRows = 512;  // rows per workgroup of the full-chip passes
+constexpr int kChSlotSetup = 0, kChSlotPap = 3, kChSlotUpd = 4, kChSlotEq = 8, kChSlots = 9;
+constexpr int kChPresumChunks = 256;  // above this many workgroups a one-workgroup pass sums the partials
+struct ChDiagState {
+  int stopped, status, iterations, pad;  // status: ChStatus of the stop
+  double rz[2];                          // r.z of the iterations of each parity
+  double bnorm, bsq;                     // |b| and b.b (the setup's sums)
+  double abs_res, rel_res, iter_res, eq_res;
+  double sums[kChSlots];                 // presummed slots
+};
+struct ChDiagArgs {
+  int n, chunks, presummed, max_iterations;
+  double tolerance;
+  const int *row_ptr, *col;
+  const double *val, *dinv, *rhs;
+  double *x, *r, *z, *p[2], *ap;  // p by iteration parity
+  double *partials;               // kChSlots x chunks
+  ChDiagState *state;             // zeroed before the setup pass
+};
+hipError_t launch_ch_diag_setup(const ChDiagArgs &a, hipStream_t stream);
+hipError_t launch_ch_diag_iteration(const ChDiagArgs &a, int k, hipStream_t stream);
+// The stop test after `iterations` iterations when the limit ended the loop.
+hipError_t launch_ch_diag_report(const ChDiagArgs &a, int iterations, hipStream_t stream);
+// |rhs - A x| of the returned x over |b|.
+hipError_t launch_ch_diag_finish(const ChDiagArgs &a, hipStream_t stream);
+
 // Momentum predictor (Solver::compute_momentum_fluxes + solve_momentum_predictor) on ghosted Field2D
 // buffers (G layers). Unknowns: the active u faces (i-major; i = iu0 .. nx-1, iu0 = 0 periodic else 1)
 // then the active v faces (j = jv0 .. ny-1); each row keeps up to kMomRow entries.

@@ -51,7 +51,9 @@ class OrChProblem(ctypes.Structure):
     ]
 
 
-NATURAL_TREE = 2  # CH solve reductions in the engine's order (512-thread tree over natural rows)
+# CH solve reductions in the engine's orders: the ILDLT PCG's 512-thread tree over natural rows, the
+# diagonal CG's 512-row chunk trees (BiCGSTAB's order), each attempt by its preconditioner
+CH_ENGINE = 2
 
 
 def ch_problem(cc) -> OrChProblem:

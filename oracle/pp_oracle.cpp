@@ -1707,10 +1707,14 @@ int or_momentum_predictor(const OrMomentumProblem *mp, const OrVelocityBc *vb, i
 
 namespace {
 
-constexpr int kNaturalTree = 2;  // the engine's order: 512-thread canonical tree over natural rows
+// The engine's orders: the one-workgroup ILDLT PCG sums with the 512-thread canonical tree over
+// natural rows (kNaturalTree); the full-chip diagonal CG with BiCGSTAB's chunked order (kChunked,
+// bicg_sum). or_ch_solve's reduction 2 selects each attempt's order by its preconditioner.
+constexpr int kNaturalTree = 2;
 
 template <class F>
 double ch_sum(int mode, std::size_t n, F f) {
+  if (mode == kChunked) return bicg_sum(kChunked, n, f);
   if (mode != kNaturalTree) {
     double s = 0.0;
     for (std::size_t k = 0; k < n; ++k) s += f(k);
@@ -2007,7 +2011,8 @@ int or_ch_ildlt(const OrChProblem *cp, double *lower, double *D, double *floor_o
 // The solve: rhs and c (in: the initial guess, out: the solution) are natural-order nx*ny arrays.
 // precond_in: -1 build (ILDLT if it builds, else diagonal), 0 diagonal (the cached state after a
 // fallback); precond_out: 1 ILDLT, 0 diagonal, 2 diagonal after a fallback. reduction: 0 sequential
-// (the reference), 2 the engine's natural-order tree.
+// (the reference), 2 the engine's orders (the ILDLT PCG's natural-order tree, the diagonal CG's
+// chunked order), 3 the chunked order for every attempt.
 int or_ch_solve(const OrChProblem *cp, const double *rhs, double *c, int precond_in, int reduction, int *iterations,
                 double *abs_res, double *rel_res, double *iter_res, int *precond_out, char *err, int errlen) {
   try {
@@ -2023,12 +2028,14 @@ int or_ch_solve(const OrChProblem *cp, const double *rhs, double *c, int precond
     std::vector<double> x(c, c + A.n);
     ChReport rep;
     int used = P.type;
+    // reduction 2: the engine's order of each attempt (ILDLT: natural tree, diagonal: chunked)
+    auto order = [&](const ChPrecond &Q) { return reduction == kNaturalTree && Q.type == 0 ? kChunked : reduction; };
     try {
-      rep = ch_pcg(reduction, A, P, b, cp->max_iterations, cp->tolerance, x);
+      rep = ch_pcg(order(P), A, P, b, cp->max_iterations, cp->tolerance, x);
     } catch (const std::runtime_error &) {
       P = diagonal_precond(A);
       used = 2;
-      rep = ch_pcg(reduction, A, P, b, cp->max_iterations, cp->tolerance, x);
+      rep = ch_pcg(order(P), A, P, b, cp->max_iterations, cp->tolerance, x);
     }
     std::copy(x.begin(), x.end(), c);
     *iterations = rep.iterations;

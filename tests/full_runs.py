@@ -37,7 +37,8 @@ def snapshot(run):
     return {k: run.field(k)[3:-3, 3:-3].copy() for k in ("pressure", "u", "v", "c")} | {"centroid": centroid(run)}
 
 
-def reference_run(ref, spec, steps, scheme="icpcg"):
+def reference_run(ref, spec, steps, scheme="icpcg", on_step=None):
+    """The reference's own loop; on_step(k, log_entry) is called after each step."""
     run = ref.RefRun(spec, scheme)
     log = []
     for _ in range(steps):
@@ -47,6 +48,8 @@ def reference_run(ref, spec, steps, scheme="icpcg"):
         info = run.finish()
         log.append({"iterations": it, "residual": res, "pressure_s": sec, "momentum_s": msec,
                     "momentum_iterations": info["momentum_iterations"], "ch_iterations": info["ch_iterations"]})
+        if on_step is not None:
+            on_step(len(log) - 1, log[-1])
     out = snapshot(run)
     run.close()
     return out, log
@@ -73,9 +76,10 @@ def ch_case_of(spec):
 
 
 def hybrid_run(ref, eng, spec, steps, scheme=E.PP_ICPCG, precision=E.PP_FP64, ref_scheme="icpcg", momentum=False,
-               ch=False):
+               ch=False, on_step=None):
     """The reference's loop with the HIP pressure step (with momentum=True also the HIP momentum
-    predictor, with ch=True also the HIP CH linear solve)."""
+    predictor, with ch=True also the HIP CH linear solve); on_step(k, log_entry) is called after
+    each step."""
     import dataclasses
     run = ref.RefRun(spec, ref_scheme)
     vbc = W.VelocityBC()
@@ -121,6 +125,8 @@ def hybrid_run(ref, eng, spec, steps, scheme=E.PP_ICPCG, precision=E.PP_FP64, re
             entry["ch_iterations"] = ch_log[-1]["iterations"]
             entry["ch_kernel_ms"] = ch_log[-1]["kernel_ms"]
         log.append(entry)
+        if on_step is not None:
+            on_step(len(log) - 1, entry)
     out = snapshot(run)
     run.close()
     return out, log

@@ -26,7 +26,7 @@ def _check(engine, oracle_lib, cc, precond=-1, operator_alpha0=None):
     """operator_alpha0: the alpha0 the cached operator was built with, when the context reuses it."""
     c, rep = engine.ch_solve(cc)
     occ = cc if operator_alpha0 is None else dataclasses.replace(cc, alpha0=operator_alpha0)
-    want = oracle_lib.ch_solve(occ, cc.rhs, cc.c0, oracle_lib.NATURAL_TREE, precond)
+    want = oracle_lib.ch_solve(occ, cc.rhs, cc.c0, oracle_lib.CH_ENGINE, precond)
     assert np.array_equal(c, want[0]), f"max diff {np.abs(c - want[0]).max():.3e}"
     assert rep["iterations"] == want[1]
     assert rep["absolute_residual"] == want[2] and rep["relative_residual"] == want[3]
@@ -78,7 +78,7 @@ def test_device_entry(engine, oracle_lib):
     d_c = torch.from_numpy(cc.c0.copy()).to(dev)
     rep = engine.ch_solve_device(cc, d_rhs.data_ptr(), d_c.data_ptr(),
                                  stream=torch.cuda.current_stream(dev).cuda_stream)
-    want = oracle_lib.ch_solve(cc, cc.rhs, cc.c0, oracle_lib.NATURAL_TREE)
+    want = oracle_lib.ch_solve(cc, cc.rhs, cc.c0, oracle_lib.CH_ENGINE)
     assert rep["iterations"] == want[1] and np.array_equal(d_c.cpu().numpy(), want[0])
 
 
@@ -89,7 +89,7 @@ def test_breakdown_falls_back_then_reports_the_reference_text(engine, oracle_lib
     cc.rhs = cc.rhs.copy()
     cc.rhs[5] = np.nan
     with pytest.raises(RuntimeError) as e_or:
-        oracle_lib.ch_solve(cc, cc.rhs, cc.c0, oracle_lib.NATURAL_TREE)
+        oracle_lib.ch_solve(cc, cc.rhs, cc.c0, oracle_lib.CH_ENGINE)
     with pytest.raises(E.EngineError) as e_gpu:
         engine.ch_solve(cc)
     assert str(e_gpu.value) == str(e_or.value)
