@@ -180,6 +180,110 @@ __device__ void ch_precondition(const ChArgs &a, bool ildlt, double floor) {
   }
 }
 
+// Lane k's double for every lane (k uniform).
+__device__ __forceinline__ double readlane_f64(double v, int k) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffu), k));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), k));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// z = M^{-1} r with the ILDLT factor by batches of B rows (apply_preconditioner, 486-512), for
+// factors whose level schedule is a chain. Wave 0 does the sweeps; ys (LDS, n doubles) holds y,
+// then z. Forward, per batch: lane k subtracts row b0+k's lower entries outside the batch from r in
+// CSR order (their y are final), then the rows run one after the other, each subtracting its
+// in-batch entries L(r, r-2) y_{r-2}, L(r, r-1) y_{r-1} (the last ones in CSR order) with every lane
+// holding the running values. Backward, batches and rows descending: each row subtracts the
+// in-batch children r+1, r+2 first (they lead the ascending child list), then the products of its
+// children outside the batch, formed beforehand by lane k in list order. Every operation is the
+// reference's, in its order; only who computes it changes.
+__device__ void ch_precondition_seq(const ChArgs &a, double floor, double *ys) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = a.n, B = a.seq_batch;
+  if (wave == 0) {
+    for (int b0 = 0; b0 < n; b0 += B) {
+      const int cnt = min(B, n - b0);
+      const int r = b0 + lane;
+      const bool own = lane < cnt;
+      double s = 0.0, l1 = 0.0, l2 = 0.0;
+      int f = 0;
+      if (own) {
+        s = a.r[r];
+        const int end = a.fsplit[r];
+        for (int p = a.row_ptr[r]; p < end; ++p) s -= a.lower[p] * ys[a.col[p]];
+        const int q2 = a.fpos[2 * r], q1 = a.fpos[2 * r + 1];
+        if (q2 >= 0) {
+          l2 = a.lower[q2];
+          f |= 2;
+        }
+        if (q1 >= 0) {
+          l1 = a.lower[q1];
+          f |= 1;
+        }
+      }
+      double y1 = 0.0, y2 = 0.0, mine = 0.0;
+      for (int k = 0; k < cnt; ++k) {
+        double t = readlane_f64(s, k);
+        const int fk = __builtin_amdgcn_readlane(f, k);
+        if (fk & 2) t -= readlane_f64(l2, k) * y2;
+        if (fk & 1) t -= readlane_f64(l1, k) * y1;
+        y2 = y1;
+        y1 = t;
+        mine = lane == k ? t : mine;
+      }
+      if (own) ys[r] = mine;
+    }
+  }
+  __syncthreads();
+  for (int r = tid; r < n; r += kThreads) {
+    const double d = a.D[r];
+    ys[r] = ys[r] / (d < floor ? floor : d);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    for (int b0 = ((n - 1) / B) * B; b0 >= 0; b0 -= B) {
+      const int hi = min(b0 + B, n) - 1, cnt = hi - b0 + 1;
+      const int r = hi - lane;  // lane k: the k-th row of the batch from the top
+      const bool own = lane < cnt;
+      double zr = 0.0, c1 = 0.0, c2 = 0.0;
+      double far[kChMaxFar];
+#pragma unroll
+      for (int q = 0; q < kChMaxFar; ++q) far[q] = 0.0;
+      int f = 0, nf = 0;
+      if (own) {
+        zr = ys[r];
+        f = a.bflag[r];
+        int t = a.lt_ptr[r];
+        const int t1 = a.lt_ptr[r + 1];
+        if (f & 1) c1 = a.lower[a.lt_pos[t++]];
+        if (f & 2) c2 = a.lower[a.lt_pos[t++]];
+        nf = t1 - t;
+#pragma unroll
+        for (int q = 0; q < kChMaxFar; ++q)
+          if (q < nf) far[q] = a.lower[a.lt_pos[t + q]] * ys[a.lt_row[t + q]];
+      }
+      double z1 = 0.0, z2 = 0.0, mine = 0.0;
+      for (int k = 0; k < cnt; ++k) {
+        double t = readlane_f64(zr, k);
+        const int fk = __builtin_amdgcn_readlane(f, k);
+        const int nk = __builtin_amdgcn_readlane(nf, k);
+        if (fk & 1) t -= readlane_f64(c1, k) * z1;
+        if (fk & 2) t -= readlane_f64(c2, k) * z2;
+#pragma unroll
+        for (int q = 0; q < kChMaxFar; ++q)
+          if (q < nk) t -= readlane_f64(far[q], k);
+        z2 = z1;
+        z1 = t;
+        mine = lane == k ? t : mine;
+      }
+      if (own) ys[r] = mine;
+    }
+  }
+  __syncthreads();
+  for (int r = tid; r < n; r += kThreads) a.z[r] = ys[r];
+  __syncthreads();
+}
+
 __device__ __forceinline__ double row_times(const ChArgs &a, const double *x, int row) {
   double s = 0.0;
   for (int p = a.row_ptr[row]; p < a.row_ptr[row + 1]; ++p) s += a.val[p] * x[a.col[p]];
@@ -190,6 +294,7 @@ __device__ __forceinline__ double row_times(const ChArgs &a, const double *x, in
 // reference's. ildlt selects the preconditioner.
 __global__ void __launch_bounds__(kThreads) ch_pcg_kernel(ChArgs a, int ildlt) {
   __shared__ ChShared sh;
+  extern __shared__ double ch_ys[];  // the sequential sweeps' vector (a.seq_batch > 0)
   const int tid = threadIdx.x, n = a.n;
   int parity = 0;
   const double floor = a.state->floor;
@@ -209,7 +314,10 @@ __global__ void __launch_bounds__(kThreads) ch_pcg_kernel(ChArgs a, int ildlt) {
   const double tol = a.tolerance;
   int iterations = 0, status = kChOk;
   if (!(rel_res < tol || abs_res < tol)) {
-    ch_precondition(a, use_ildlt, floor);
+    if (use_ildlt && a.seq_batch > 0)
+      ch_precondition_seq(a, floor, ch_ys);
+    else
+      ch_precondition(a, use_ildlt, floor);
     double rz = 0.0;
     for (int r = tid; r < n; r += kThreads) {
       rz = rz + a.r[r] * a.z[r];
@@ -250,7 +358,10 @@ __global__ void __launch_bounds__(kThreads) ch_pcg_kernel(ChArgs a, int ildlt) {
         abs_res = sqrt(up[2] < 0.0 ? 0.0 : up[2]);
         rel_res = abs_res / bden;
         if (rel_res < tol || abs_res < tol) break;
-        ch_precondition(a, use_ildlt, floor);
+        if (use_ildlt && a.seq_batch > 0)
+          ch_precondition_seq(a, floor, ch_ys);
+        else
+          ch_precondition(a, use_ildlt, floor);
         double rzn = 0.0;
         for (int r = tid; r < n; r += kThreads) rzn = rzn + a.r[r] * a.z[r];
         rzn = ch_sum1(rzn, sh, parity);
@@ -486,7 +597,8 @@ hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream) {
 }
 
 hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream) {
-  hipLaunchKernelGGL(ch_pcg_kernel, dim3(1), dim3(kThreads), 0, stream, a, ildlt ? 1 : 0);
+  const size_t lds = ildlt && a.seq_batch > 0 ? static_cast<size_t>(a.n) * sizeof(double) : 0;
+  hipLaunchKernelGGL(ch_pcg_kernel, dim3(1), dim3(kThreads), lds, stream, a, ildlt ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -1799,6 +1799,55 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
         lt_pos[static_cast<size_t>(k)] = p;
       }
   }
+  // Batched sequential sweeps when the level schedule is close to a chain (periodic y makes row
+  // (i, 0) depend on (i-1, ny-1)) and the vectors fit in LDS: the largest batch B for which every
+  // row's in-batch lower entries are L(r, r-2), L(r, r-1) and its in-batch children r+1, r+2 (with at
+  // most kChMaxFar children outside).
+  int seq_batch = 0;
+  std::vector<int> fsplit, fpos, bflag;
+  if (n <= pp::kChSeqMaxRows && nlev > n / 16) {
+    for (int B : {64, 32, 16, 8}) {
+      fsplit.assign(static_cast<size_t>(n), 0);
+      fpos.assign(2 * static_cast<size_t>(n), -1);
+      bflag.assign(static_cast<size_t>(n), 0);
+      bool ok = true;
+      for (int r = 0; r < n && ok; ++r) {
+        const int b0 = (r / B) * B, hi = std::min(b0 + B, n) - 1;
+        int p = A.rp[r], split = -1;
+        for (; p < A.rp[r + 1] && A.ci[p] < r; ++p) {
+          const int c = A.ci[p];
+          if (c < b0) continue;
+          if (split < 0) split = p;
+          if (c == r - 2 && fpos[2 * r] < 0 && fpos[2 * r + 1] < 0) fpos[2 * r] = p;
+          else if (c == r - 1 && fpos[2 * r + 1] < 0) fpos[2 * r + 1] = p;
+          else ok = false;
+        }
+        fsplit[static_cast<size_t>(r)] = split < 0 ? p : split;
+        int t = lt_ptr[static_cast<size_t>(r)], f = 0;
+        const int t1 = lt_ptr[static_cast<size_t>(r) + 1];
+        if (t < t1 && lt_row[static_cast<size_t>(t)] == r + 1 && r + 1 <= hi) {
+          f |= 1;
+          ++t;
+        }
+        if (t < t1 && lt_row[static_cast<size_t>(t)] == r + 2 && r + 2 <= hi) {
+          f |= 2;
+          ++t;
+        }
+        if (t < t1 && lt_row[static_cast<size_t>(t)] <= hi) ok = false;  // another in-batch child
+        if (t1 - t > pp::kChMaxFar) ok = false;
+        bflag[static_cast<size_t>(r)] = f;
+      }
+      if (ok) {
+        seq_batch = B;
+        break;
+      }
+    }
+  }
+  if (!seq_batch) {
+    fsplit.clear();
+    fpos.clear();
+    bflag.clear();
+  }
   // device buffers
   if (C.d_int) (void)hipFree(C.d_int);
   if (C.d_dbl) (void)hipFree(C.d_dbl);
@@ -1807,7 +1856,7 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
   C.d_dbl = nullptr;
   C.d_part = nullptr;
   const size_t nl = lt_row.size();
-  const size_t ints = (n + 1) + nnz + n + (nlev + 1) + n + (n + 1) + 2 * nl;
+  const size_t ints = (n + 1) + nnz + n + (nlev + 1) + n + (n + 1) + 2 * nl + fsplit.size() + fpos.size() + bflag.size();
   const size_t dbls = 2 * static_cast<size_t>(nnz) + 7 * static_cast<size_t>(n);
   PP_CH_CHECK(hipMalloc(&C.d_int, ints * sizeof(int)));
   PP_CH_CHECK(hipMalloc(&C.d_dbl, dbls * sizeof(double)));
@@ -1839,6 +1888,12 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
   if (int rc = put_int(lt_ptr, a.lt_ptr)) return rc;
   if (int rc = put_int(lt_row, a.lt_row)) return rc;
   if (int rc = put_int(lt_pos, a.lt_pos)) return rc;
+  a.seq_batch = seq_batch;
+  if (seq_batch) {
+    if (int rc = put_int(fsplit, a.fsplit)) return rc;
+    if (int rc = put_int(fpos, a.fpos)) return rc;
+    if (int rc = put_int(bflag, a.bflag)) return rc;
+  }
   PP_CH_CHECK(hipMemcpyAsync(dp, A.v.data(), A.v.size() * sizeof(double), hipMemcpyHostToDevice, s));
   a.val = dp;
   a.lower = dp + nnz;

@@ -183,7 +183,16 @@ struct ChArgs {
   int max_iterations;
   double tolerance;
   ChState *state;
+  // Batched sequential sweeps (seq_batch = B > 0, used when the level schedule degenerates into a
+  // chain, as periodic y makes it): rows in batches of B, the vectors in LDS. fsplit[r]: the CSR
+  // position where row r's lower entries inside its batch start (they are L(r, r-2), L(r, r-1) at
+  // fpos[2r], fpos[2r+1], -1 when absent); bflag[r]: bit 0 / 1 when the children r+1 / r+2 lie in the
+  // batch (they lead the ascending child list), the rest of the list (at most kChMaxFar) outside.
+  int seq_batch;
+  const int *fsplit, *fpos, *bflag;
 };
+constexpr int kChMaxFar = 12;         // children outside the batch a sequential backward row may have
+constexpr int kChSeqMaxRows = 7936;   // the sequential sweeps keep n doubles in LDS
 hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream);
 hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream);
 
