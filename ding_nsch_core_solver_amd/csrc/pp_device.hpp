@@ -206,6 +206,29 @@ __device__ __forceinline__ float wave_shl1(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 
+// Lane l receives lane l-1's value and lane 0 receives `fill` (wave_shr with bound_ctrl off: the
+// lane without a source keeps the old value, which is fill).
+__device__ __forceinline__ double shr1_fill(double v, double fill) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x138, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Lane l receives lane l+1's value and lane 63 receives `fill`.
+__device__ __forceinline__ double shl1_fill(double v, double fill) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x130, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ float shr1_fill(float v, float fill) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float shl1_fill(float v, float fill) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
 // Value held by lane k (compile-time k), broadcast through SGPRs.
 __device__ __forceinline__ double lane_value(double v, int k) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
@@ -242,6 +265,25 @@ __device__ __forceinline__ T receive_batch(int stream_wave, int lane, int k0, in
     if (lane == 0) *lds(&sh.ack[stream_wave]) = khi + 1;
   }
   return hv;
+}
+
+// The same batch with every lane holding all P values: hk[k] = stream value k0 + k (a broadcast LDS
+// read per value), for the lane that takes it through shr1_fill / shl1_fill. Slots outside the
+// published range are read too; their values only reach lanes whose cell is outside the grid at that
+// level (the stream index is the boundary lane's column), which compute on padding anyway.
+template <typename T, int P>
+__device__ __forceinline__ void receive_chunk(int stream_wave, int k0, int count, SweepShared &sh, T (&hk)[P]) {
+  const int klo = k0 > 0 ? k0 : 0;
+  const int khi = (k0 + P - 1) < (count - 1) ? (k0 + P - 1) : (count - 1);
+  if (klo <= khi) {
+    wait_at_least(&sh.cnt[stream_wave], khi + 1, sh);
+#pragma unroll
+    for (int k = 0; k < P; ++k) hk[k] = static_cast<T>(*lds(&sh.hand[stream_wave][(k0 + k) & (kRing - 1)]));
+    if ((threadIdx.x & 63) == 0) *lds(&sh.ack[stream_wave]) = khi + 1;
+  } else {
+#pragma unroll
+    for (int k = 0; k < P; ++k) hk[k] = T(0);
+  }
 }
 
 // Before a batch whose last published index is k_last: make sure its slot has been consumed.

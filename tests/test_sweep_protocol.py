@@ -139,7 +139,7 @@ def loader_units(me, S, ny, R, backward):
     L, U, Tn = WAVES - S, unit_size(R), stage_steps(ny)
     if S <= L:  # dedicated loaders
         rank = me % S
-        w_own = rank if backward else S - 1 - rank
+        w_own = S - 1 - rank if backward else rank
         n_own, k_own = (L - 1 - rank) // S + 1, me // S
         m = 0
         while U * (k_own + n_own * m) < Tn:
