@@ -209,10 +209,44 @@ __device__ double sequential_sum(const GhostedConst &f, int nx, int ny, double (
   return acc;
 }
 
+// Sum of the nx x ny interior of f in the tree order: thread t of the kFaceBlock threads adds the
+// cells of i-outer index e = t, t + kFaceBlock, ... from 0.0, eight loads in flight at a time; the
+// 64-lane tree (offsets 32..1) and the wave tree (offsets 2, 1) combine the threads. Returns the
+// sum in thread 0.
+__device__ double tree_sum(const GhostedConst &f, int nx, int ny, double *red) {
+  const int tid = threadIdx.x, n = nx * ny;
+  constexpr int kAhead = 8;
+  double acc = 0.0;
+  for (int e0 = tid; e0 < n; e0 += kAhead * kFaceBlock) {
+    double val[kAhead];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const int e = min(e0 + k * kFaceBlock, n - 1), i = e / ny, j = e - i * ny;
+      val[k] = f(i, j);
+    }
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k)
+      if (e0 + k * kFaceBlock < n) acc = acc + val[k];
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
+  const int lane = tid & 63, wave = tid >> 6;
+  if (lane == 0) red[wave] = acc;
+  __syncthreads();
+  double total = 0.0;
+  if (tid == 0) {
+    static_assert(kFaceBlock == 256, "the wave tree below is written for four waves");
+    const double w0 = red[0] + red[2], w1 = red[1] + red[3];
+    total = w0 + w1;
+  }
+  return total;
+}
+
 }  // namespace
 
-// Block 0: the sequential mean of the correction (when no pressure edge is Dirichlet) into
-// work[kMeanSlot]. Other blocks: one face each, u = u* - dt (implicit + explicit)
+// Block 0: the mean of the correction (when no pressure edge is Dirichlet), in the reference's
+// sequential order or the tree order (CorrArgs::exact_mean), into work[kMeanSlot].
+// Other blocks: one face each, u = u* - dt (implicit + explicit)
 // (operators.cpp:490-519), every face of the (nx+1) x ny and nx x (ny+1) fields including the
 // boundary faces, which the finish kernel then overwrites as the reference does.
 __global__ void __launch_bounds__(kFaceBlock) correction_faces_kernel(const CorrArgs *__restrict__ args) {
@@ -222,7 +256,7 @@ __global__ void __launch_bounds__(kFaceBlock) correction_faces_kernel(const Corr
   const GhostedConst rho{a.rho_mid, ny, G}, p_corr{a.p_corr, ny, G};
   if (blockIdx.x == 0) {
     if (!a.has_dirichlet) {
-      const double sum = sequential_sum(p_corr, nx, ny, stage);
+      const double sum = a.exact_mean ? sequential_sum(p_corr, nx, ny, stage) : tree_sum(p_corr, nx, ny, stage[0]);
       if (threadIdx.x == 0) a.work[kMeanSlot] = sum / static_cast<double>(nx * ny);
     }
     return;

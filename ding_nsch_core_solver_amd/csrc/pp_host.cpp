@@ -49,6 +49,8 @@ struct pp_ctx {
   std::map<std::pair<int, int>, int> tuning;
   // sleeps a hand-off wait may spend before the solve fails (pp_set_wait_limit; 0: built-in)
   int wait_limit = 0;
+  // velocity correction's pressure mean: 1 the reference's sequential order, 0 the tree (pp_set_exact_mean)
+  int exact_mean = 0;
 
   // batched solves: per-system work slab, wrap lines, arguments, reports, order and counters
   size_t batch_cells = 0;       // capacity of batch_work in cells (sum of n + kPad)
@@ -561,6 +563,7 @@ int run_velocity_correction(pp_ctx *ctx, const pp_problem *pb, const pp_velocity
   const int nx = pb->nx, ny = pb->ny, G = ghost;
   if (int rc = ensure_corr(ctx)) return rc;
   pp::CorrArgs c = corr_args(pb, vbc, G);
+  c.exact_mean = ctx->exact_mean;
   c.rho_mid = d_rho_mid;
   c.u_star = d_u_star;
   c.v_star = d_v_star;
@@ -1009,6 +1012,12 @@ int pp_set_tuning(pp_ctx *ctx, int count, const int *nx, const int *ny, const in
     table[std::make_pair(nx[k], ny[k])] = ring_depth[k];
   }
   ctx->tuning.swap(table);
+  return PP_OK;
+}
+
+int pp_set_exact_mean(pp_ctx *ctx, int exact) {
+  if (!ctx) return PP_ERR_ARG;
+  ctx->exact_mean = exact ? 1 : 0;
   return PP_OK;
 }
 

@@ -106,6 +106,7 @@ struct CorrArgs {
   int v_type[4];
   double v_value[4];
   int split, step_index, has_dirichlet;
+  int exact_mean;  // 1: the pressure mean in the reference's sequential order; 0: the 256-thread tree
   double rho_ref;
   const double *rho_mid, *u_star, *v_star, *p_corr;
   double *u, *v, *pressure, *pressure_prev;

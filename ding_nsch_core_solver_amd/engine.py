@@ -160,7 +160,7 @@ EXPORTS = ("pp_create", "pp_destroy", "pp_last_error", "pp_last_kernel_ms", "pp_
            "pp_set_tuning", "pp_velocity_correction", "pp_velocity_correction_device", "pp_solve_monitored",
            "pp_solve_device_monitored", "pp_solve_device_field", "pp_pressure_step", "pp_bicgstab",
            "pp_bicgstab_device", "pp_set_wait_limit", "pp_solve_device_noalloc", "pp_momentum_predictor",
-           "pp_momentum_predictor_device", "pp_ch_solve", "pp_ch_solve_device")
+           "pp_momentum_predictor_device", "pp_ch_solve", "pp_ch_solve_device", "pp_set_exact_mean")
 
 # One PCG monitor record (pp_monitor_record): the scalar fields of one CGMonitor call.
 MONITOR_DTYPE = np.dtype([("iteration", "<i4"), ("pad", "<i4"), ("alpha", "<f8"), ("beta", "<f8"),
@@ -259,6 +259,8 @@ def lib():
         L.pp_solve_device_noalloc.restype = ctypes.c_int
         L.pp_set_wait_limit.argtypes = [vp, ctypes.c_int]
         L.pp_set_wait_limit.restype = ctypes.c_int
+        L.pp_set_exact_mean.argtypes = [vp, ctypes.c_int]
+        L.pp_set_exact_mean.restype = ctypes.c_int
         vb = ctypes.POINTER(PPVelocityBC)
         L.pp_velocity_correction_device.argtypes = [vp, ctypes.POINTER(PPProblem), vb, ctypes.c_int, vp, vp, vp, vp,
                                                     vp, vp, vp, vp, dp, vp]
@@ -353,6 +355,13 @@ class Engine:
         """Sleeps a sweep hand-off wait may spend before the solve fails with PP_ERR_SWEEP
         (0: the built-in budget)."""
         rc = self._lib.pp_set_wait_limit(self.h, int(sleeps))
+        if rc != 0:
+            raise EngineError(rc, self.last_error())
+
+    def set_exact_mean(self, exact: bool = True):
+        """Velocity correction's pressure mean in the reference's sequential order (exact=True, its
+        bits) or the tree order (exact=False, the default: within rounding, microseconds)."""
+        rc = self._lib.pp_set_exact_mean(self.h, int(bool(exact)))
         if rc != 0:
             raise EngineError(rc, self.last_error())
 

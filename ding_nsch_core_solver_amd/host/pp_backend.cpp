@@ -185,6 +185,8 @@ double PressureBackend::solve(const std::string &scheme, const PressureSettings 
   return report_.relative_residual;
 }
 
+void PressureBackend::set_exact_mean(bool exact) { pp_set_exact_mean(ctx_, exact ? 1 : 0); }
+
 void PressureBackend::correct_velocity(const std::string &scheme, const PressureSettings &s,
                                        const VelocitySettings &vel, const FieldView &rho_mid,
                                        const FieldView &u_star, const FieldView &v_star,

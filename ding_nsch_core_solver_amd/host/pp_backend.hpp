@@ -131,6 +131,11 @@ class PressureBackend {
   LinearSolveReport solve_bicgstab(const SparseMatrixCSR &matrix, const std::vector<double> &rhs, int max_iterations,
                                    double tolerance, std::vector<double> &x, bool use_relative_tolerance = true);
 
+  // Summation order of the pressure mean in correct_velocity / pressure_step: the reference's
+  // sequential sum (exact = true, its bits) or the engine's tree (false, the default; within
+  // rounding, microseconds instead of one dependent addition per cell). pp_set_exact_mean.
+  void set_exact_mean(bool exact);
+
   int last_iterations() const { return last_iterations_; }
   const pp_report &last_report() const { return report_; }
   // last_boundary_speed_pre_correction_ / _post_correction_ of the last correct_velocity

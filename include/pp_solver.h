@@ -420,6 +420,18 @@ int pp_solve_device_noalloc(const pp_problem *pb, const double *d_rho, const dou
  */
 int pp_set_wait_limit(pp_ctx *ctx, int sleeps);
 
+/*
+ * Summation order of the pressure mean in the velocity correction (pp_velocity_correction[_device],
+ * pp_pressure_step): subtract_mean (operators.cpp:447-460, called at 526-532) sums the correction
+ * sequentially, i outer and j inner, one dependent addition per cell. exact = 1 keeps that order
+ * and the reference's bits (a single lane runs the chain: about 180 us of the 210 us correction at
+ * 192x192). exact = 0, the default, sums in a 256-thread tree (thread t adds cells t, t + 256, ...
+ * of the i-outer order, then the 64-lane and 4-wave trees), which moves the mean by rounding only
+ * (measured within 1e-12 of the reference's pressure, u and v relative to their maxima) and takes a
+ * few microseconds. Applies to later calls on the context.
+ */
+int pp_set_exact_mean(pp_ctx *ctx, int exact);
+
 #ifdef __cplusplus
 }
 #endif

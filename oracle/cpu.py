@@ -136,6 +136,10 @@ def lib():
         L.or_velocity_correction.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem), ctypes.POINTER(OrVelocityBc),
                                              ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.or_velocity_correction.restype = None
+        L.or_velocity_correction_order.argtypes = [ctypes.c_int, ctypes.POINTER(OrProblem),
+                                                   ctypes.POINTER(OrVelocityBc), ctypes.c_int, dp, dp, dp, dp, dp, dp,
+                                                   dp, dp, dp, ctypes.c_int]
+        L.or_velocity_correction_order.restype = None
         L.or_bicgstab.argtypes = [ctypes.c_int, ip, ip, dp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.c_int, ip, dp, dp, dp, ctypes.c_char_p, ctypes.c_int]
         L.or_bicgstab.restype = ctypes.c_int
@@ -346,15 +350,17 @@ def velocity_bc(vbc) -> OrVelocityBc:
 
 
 def velocity_correction(case, vbc, split: bool, ghost: int, rho_mid, u_star, v_star, p_corr, u, v, pressure,
-                        pressure_prev):
+                        pressure_prev, tree_mean: bool = False):
     """Solver::apply_pressure_velocity_correction on ghosted Field2D buffers (see pp_oracle.cpp).
     Returns the new (u, v, pressure, pressure_prev, (speed_pre, speed_post)); the inputs are not
-    modified."""
+    modified. tree_mean: the pressure mean in the engine's tree order instead of the reference's
+    sequential sum."""
     u, v, pressure, prev = (np.array(a, dtype=np.float64, copy=True) for a in (u, v, pressure, pressure_prev))
     speeds = np.zeros(2)
     args = [_f64(a) for a in (rho_mid, u_star, v_star, p_corr)]
-    lib().or_velocity_correction(int(split), ctypes.byref(problem(case)), ctypes.byref(velocity_bc(vbc)), int(ghost),
-                                 *[_dp(a) for a in args], _dp(u), _dp(v), _dp(pressure), _dp(prev), _dp(speeds))
+    lib().or_velocity_correction_order(int(split), ctypes.byref(problem(case)), ctypes.byref(velocity_bc(vbc)),
+                                       int(ghost), *[_dp(a) for a in args], _dp(u), _dp(v), _dp(pressure), _dp(prev),
+                                       _dp(speeds), int(bool(tree_mean)))
     return u, v, pressure, prev, (float(speeds[0]), float(speeds[1]))
 
 

@@ -1056,9 +1056,24 @@ extern "C" {
 // pressure_prev holds the previous step's pressure on entry (the split extrapolation reads it) and
 // the pressure before this correction, ghosts included, on return. speeds[0] / speeds[1] get the
 // periodic boundary speed before / after (operators.cpp:463-478).
+void or_velocity_correction_order(int split, const OrProblem *pb, const OrVelocityBc *vb, int G, const double *rho_mid,
+                                  const double *u_star, const double *v_star, const double *p_corr, double *u,
+                                  double *v, double *pressure, double *pressure_prev, double *speeds, int tree_mean);
+
 void or_velocity_correction(int split, const OrProblem *pb, const OrVelocityBc *vb, int G, const double *rho_mid,
                             const double *u_star, const double *v_star, const double *p_corr, double *u, double *v,
                             double *pressure, double *pressure_prev, double *speeds) {
+  or_velocity_correction_order(split, pb, vb, G, rho_mid, u_star, v_star, p_corr, u, v, pressure, pressure_prev, speeds,
+                               0);
+}
+
+// The same with the pressure mean's summation order chosen: tree_mean = 0 the reference's
+// sequential sum (subtract_mean, operators.cpp:447-460), 1 the engine's tree order (256 threads:
+// thread t adds the cells of i-outer index t, t + 256, ... from 0.0; lanes of each 64-lane wave
+// combine with offsets 32..1, the four wave sums as (w0 + w2) + (w1 + w3)).
+void or_velocity_correction_order(int split, const OrProblem *pb, const OrVelocityBc *vb, int G, const double *rho_mid,
+                                  const double *u_star, const double *v_star, const double *p_corr, double *u,
+                                  double *v, double *pressure, double *pressure_prev, double *speeds, int tree_mean) {
   const Grid g(*pb);
   const int nx = pb->nx, ny = pb->ny;
   const int ld_c = ny + 2 * G, ld_u = ny + 2 * G, ld_v = ny + 1 + 2 * G;
@@ -1140,8 +1155,27 @@ void or_velocity_correction(int split, const OrProblem *pb, const OrVelocityBc *
     for (int j = 0; j < ny; ++j) cref(pressure, i, j) = cell(p_corr, i, j);
   if (!g.has_dirichlet()) {
     double mean = 0.0;
-    for (int i = 0; i < nx; ++i)
-      for (int j = 0; j < ny; ++j) mean += cell(pressure, i, j);
+    if (tree_mean) {
+      constexpr int T = 256;
+      const int n = nx * ny;
+      double part[T];
+      for (int t = 0; t < T; ++t) {
+        double acc = 0.0;
+        for (int e = t; e < n; e += T) acc = acc + cell(pressure, e / ny, e % ny);
+        part[t] = acc;
+      }
+      double wsum[T / 64];
+      for (int w = 0; w < T / 64; ++w) {
+        double *v = part + 64 * w;
+        for (int off = 32; off >= 1; off >>= 1)
+          for (int l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+        wsum[w] = v[0];
+      }
+      mean = (wsum[0] + wsum[2]) + (wsum[1] + wsum[3]);
+    } else {
+      for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) mean += cell(pressure, i, j);
+    }
     mean /= static_cast<double>(nx * ny);
     for (int i = 0; i < nx; ++i)
       for (int j = 0; j < ny; ++j) cref(pressure, i, j) -= mean;

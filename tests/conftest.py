@@ -33,8 +33,12 @@ def ref_lib():
 
 @pytest.fixture(scope="session")
 def engine():
-    """One engine context on cuda:0 for the GPU tests; fails loudly if the library is missing."""
+    """One engine context on cuda:0 for the GPU tests; fails loudly if the library is missing. The
+    velocity correction's pressure mean is set to the reference's sequential order here, so the
+    tests compare it with the reference bit for bit; the tree order has its own tests
+    (test_gpu_velocity.py, tree_engine)."""
     from ding_nsch_core_solver_amd import engine as E
     eng = E.Engine(0)
+    eng.set_exact_mean(True)
     yield eng
     eng.close()

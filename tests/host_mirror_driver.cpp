@@ -163,6 +163,7 @@ int main(int argc, char **argv) {
     const FieldView p{take(cell), nx, ny, g}, pm{take(cell), nx, ny, g};
     try {
       ppamd::PressureBackend backend(0);
+      backend.set_exact_mean(true);  // compared with the reference bit for bit
       std::vector<double> out;
       if (mode == "step") {
         const double res = backend.pressure_step(argv[4], c.s, vel, rho, us, vs, c.step, pc, u, v, p, pm);

@@ -96,3 +96,48 @@ def test_bad_arguments_rejected(engine):
         engine.velocity_correction(case, W.VelocityBC(), ghost=0, **f)
     with pytest.raises(E.EngineError, match="velocity boundary type"):
         engine.velocity_correction(case, W.VelocityBC(u_type=(7, 0, 0, 0)), ghost=G, **f)
+
+
+@pytest.fixture(scope="module")
+def tree_engine():
+    """A context with the default pressure-mean order: the 256-thread tree."""
+    eng = E.Engine(0)
+    yield eng
+    eng.close()
+
+
+@pytest.mark.parametrize("pname,vname,scheme_step", CASES, ids=[f"{p}-{v}-{s}{k}" for p, v, (s, k) in CASES])
+def test_tree_mean_matches_oracle_tree_order(tree_engine, oracle_lib, pname, vname, scheme_step):
+    """The default (tree-order) pressure mean: bitwise equal to the oracle's restatement of that order,
+    and within 1e-12 (relative to each field's maximum) of the reference's sequential order."""
+    scheme, step = scheme_step
+    case = _case(pname, step=step)
+    f = fields(case, seed=zlib.crc32(f"{pname}/{vname}/{scheme}/{step}".encode()))
+    vbc = VELOCITY_BC[vname]
+    split = scheme.startswith("split")
+    got = tree_engine.velocity_correction(case, vbc, scheme=CODES[scheme], ghost=G, **f)
+    _check(got, oracle_lib.velocity_correction(case, vbc, split, G, tree_mean=True, **f))
+    seq = oracle_lib.velocity_correction(case, vbc, split, G, **f)
+    for a, b in zip(got[:4], seq[:4]):
+        assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300)
+
+
+def test_tree_mean_after_the_192_solve(tree_engine, engine, oracle_lib, ref_lib):
+    """The 192x192 density-ratio-1000 pair (solve, then correction) with the tree-order mean: within
+    1e-12 of the compiled reference's correction and bitwise equal to the oracle's tree order."""
+    case = W.droplet_case(192, 192, seed=0)
+    p, _ = engine.solve(case, E.PP_ICPCG)
+    nx, ny = case.nx, case.ny
+    pc = np.zeros(W.ghosted_shape(nx, ny, G))
+    pc[G:-G, G:-G] = p.reshape(nx, ny)
+    oracle_lib.apply_pressure_bc(case, pc.reshape(-1))
+    f = fields(case, seed=5)
+    f["p_corr"] = pc.reshape(-1)
+    vbc = W.VelocityBC()
+    got = tree_engine.velocity_correction(case, vbc, scheme=E.PP_ICPCG, ghost=G, **f)
+    _check(got, oracle_lib.velocity_correction(case, vbc, False, G, tree_mean=True, **f))
+    ref = ref_lib.velocity_correction("icpcg", case, vbc, G, **f)
+    dev = [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(got[:4], ref[:4])]
+    print(f"[velocity] tree-order mean vs the reference at 192x192: u {dev[0]:.1e}, v {dev[1]:.1e}, "
+          f"pressure {dev[2]:.1e}, pressure_prev {dev[3]:.1e}")
+    assert max(dev) <= 1e-12
