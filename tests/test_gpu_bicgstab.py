@@ -166,3 +166,31 @@ def test_against_reference_golden(engine, oracle_lib, path):
         assert rep["iterations"] == int(z["out_iterations"])
         assert np.abs(x - z["out_x"]).max() <= 1e-12 * np.abs(z["out_x"]).max()
         _check_equal(x, rep, oracle_lib.bicgstab(s, z["x0"], reduction=oracle_lib.CHUNKED, **kw))
+
+
+def test_presummed_path_converges(engine, oracle_lib):
+    """Above kBicgPresumChunks workgroups (n > 131072) the dot products go through the one-workgroup
+    sum pass and the state's sums. A 272x272 momentum-shaped system with a small time step (the
+    viscous part weak, 147 152 unknowns, 288 workgroups) converges at the reference's absolute 1e-8
+    through the full step: equal to the oracle's chunked order bit for bit."""
+    s = W.momentum_system(272, 272, seed=7, dt=1.0e-5)
+    assert s.n > 256 * 512
+    x, rep = engine.bicgstab(s, max_iterations=50, tolerance=1e-8)
+    expect = oracle_lib.bicgstab(s, max_iterations=50, tolerance=1e-8, reduction=oracle_lib.CHUNKED)
+    _check_equal(x, rep, expect)
+    assert 1 <= rep["iterations"] < 50 and rep["absolute_residual"] < 1e-8
+    print(f"[bicgstab] presummed 272x272: {rep['iterations']} iterations", flush=True)
+
+
+def test_presummed_path_half_step_exit(engine, oracle_lib):
+    """A diagonal system of 150 000 unknowns on the sum-pass path: the first half step already meets
+    the tolerance (x = b / d), the half-step exit of the reference, bit for bit against the oracle."""
+    n = 150_000
+    rng = np.random.default_rng(9)
+    d = 1.0 + rng.random(n)
+    idx = np.arange(n)
+    s = W.csr_from_coo(n, idx, idx, d, d * rng.standard_normal(n))
+    x, rep = engine.bicgstab(s, max_iterations=50, tolerance=1e-8)
+    _check_equal(x, rep, oracle_lib.bicgstab(s, max_iterations=50, tolerance=1e-8, reduction=oracle_lib.CHUNKED))
+    assert rep["iterations"] == 1
+    assert np.abs(x - s.rhs / d).max() <= 1e-15 * np.abs(x).max()
