@@ -180,58 +180,134 @@ __device__ void ch_precondition(const ChArgs &a, bool ildlt, double floor) {
   }
 }
 
-// Lane k's double for every lane (k uniform).
-__device__ __forceinline__ double readlane_f64(double v, int k) {
-  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
-  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffu), k));
-  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), k));
-  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-
 // z = M^{-1} r with the ILDLT factor by batches of B rows (apply_preconditioner, 486-512), for
 // factors whose level schedule is a chain. Wave 0 does the sweeps; ys (LDS, n doubles) holds y,
-// then z. Forward, per batch: lane k subtracts row b0+k's lower entries outside the batch from r in
-// CSR order (their y are final), then the rows run one after the other, each subtracting its
-// in-batch entries L(r, r-2) y_{r-2}, L(r, r-1) y_{r-1} (the last ones in CSR order) with every lane
-// holding the running values. Backward, batches and rows descending: each row subtracts the
+// then z. Forward, per batch: lane k subtracts row b0+k's lower entries that lie before the batch
+// from r, in CSR order (their y are final), then the rows run one after the other, each subtracting
+// its in-batch entries L(r, r-2) y_{r-2}, L(r, r-1) y_{r-1} (the last ones in CSR order) with every
+// lane holding the running values. Backward, batches and rows descending: each row subtracts the
 // in-batch children r+1, r+2 first (they lead the ascending child list), then the products of its
 // children outside the batch, formed beforehand by lane k in list order. Every operation is the
-// reference's, in its order; only who computes it changes.
-__device__ void ch_precondition_seq(const ChArgs &a, double floor, double *ys) {
+// reference's, in its order; only who computes it changes. A batch's operands are packed per row
+// and loaded while the previous batch runs, so only the LDS reads of y / z wait on the batch before.
+struct SeqFwd {
+  double r, l2, l1;
+  int cnt, flags;
+  int col[kChSeqWidth];
+  double val[kChSeqWidth];
+};
+
+struct SeqBwd {
+  double c1, c2;
+  int cnt, flags;
+  int row[kChSeqWidth];
+  double val[kChSeqWidth];
+};
+
+__device__ __forceinline__ void load_fwd(const ChArgs &a, int r, bool own, SeqFwd &o) {
+  const int rr = own ? r : 0;
+  const size_t base = static_cast<size_t>(rr) * kChSeqWidth;
+  o.r = a.r[rr];
+  o.cnt = own ? a.fo_cnt[rr] : 0;
+  o.l2 = a.fin_val[2 * rr];
+  o.l1 = a.fin_val[2 * rr + 1];
+  o.flags = (a.fin_pos[2 * rr] >= 0 ? 2 : 0) | (a.fin_pos[2 * rr + 1] >= 0 ? 1 : 0);
+#pragma unroll
+  for (int e = 0; e < kChSeqWidth; ++e) {
+    o.col[e] = a.fo_col[base + e];
+    o.val[e] = a.fo_val[base + e];
+  }
+}
+
+__device__ __forceinline__ void load_bwd(const ChArgs &a, int r, bool own, SeqBwd &o) {
+  const int rr = own ? r : 0;
+  const size_t base = static_cast<size_t>(rr) * kChSeqWidth;
+  o.cnt = own ? a.bf_cnt[rr] : 0;
+  o.c1 = a.bin_val[2 * rr];
+  o.c2 = a.bin_val[2 * rr + 1];
+  o.flags = (a.bin_pos[2 * rr] >= 0 ? 1 : 0) | (a.bin_pos[2 * rr + 1] >= 0 ? 2 : 0);
+#pragma unroll
+  for (int e = 0; e < kChSeqWidth; ++e) {
+    o.row[e] = a.bf_row[base + e];
+    o.val[e] = a.bf_val[base + e];
+  }
+}
+
+// Row records of a batch in LDS for the sequential part: lane k writes its row's operands, then
+// every lane reads row k's record with one broadcast read per pair of doubles (no lane-to-lane
+// moves, and the reads run ahead of the dependency chain).
+constexpr int kRecF = 4;                  // forward: s, L(r, r-2), L(r, r-1), flags
+constexpr int kRecB = 4 + kChSeqWidth;    // backward: z, L(r+1, r), L(r+2, r), flags, far products
+
+using lds_double = __attribute__((address_space(3))) double;
+
+// The backward batch's chain: row k of the batch (from the top) subtracts its in-batch children
+// (flags bit 0: r+1, bit 1: r+2) and then its NF far products (+0.0 past its own list). Returns row
+// k's z in lane k.
+template <int NF>
+__device__ __forceinline__ double bwd_chain(const lds_double *rec, int cnt, int lane) {
+  double z1 = 0.0, z2 = 0.0, mine = 0.0;
+#pragma unroll 2
+  for (int k = 0; k < cnt; ++k) {
+    const lds_double *q = rec + k * kRecB;
+    double f[NF > 0 ? NF : 1];
+#pragma unroll
+    for (int e = 0; e < NF; ++e) f[e] = q[4 + e];
+    double t = q[0];
+    const double c1 = q[1], c2 = q[2];
+    const int fk = static_cast<int>(q[3]);
+    const double t1 = t - c1 * z1;
+    t = (fk & 1) ? t1 : t;
+    const double t2 = t - c2 * z2;
+    t = (fk & 2) ? t2 : t;
+#pragma unroll
+    for (int e = 0; e < NF; ++e) t -= f[e];
+    z2 = z1;
+    z1 = t;
+    mine = lane == k ? t : mine;
+  }
+  return mine;
+}
+
+__device__ void ch_precondition_seq(const ChArgs &a, double floor, lds_double *ys, lds_double *rec) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = a.n, B = a.seq_batch;
   if (wave == 0) {
+    SeqFwd cur, nxt;
+    load_fwd(a, lane, lane < min(B, n), cur);
     for (int b0 = 0; b0 < n; b0 += B) {
       const int cnt = min(B, n - b0);
       const int r = b0 + lane;
       const bool own = lane < cnt;
-      double s = 0.0, l1 = 0.0, l2 = 0.0;
-      int f = 0;
+      double s = cur.r;
+#pragma unroll
+      for (int e = 0; e < kChSeqWidth; ++e)
+        if (e < cur.cnt) s -= cur.val[e] * ys[cur.col[e]];
       if (own) {
-        s = a.r[r];
-        const int end = a.fsplit[r];
-        for (int p = a.row_ptr[r]; p < end; ++p) s -= a.lower[p] * ys[a.col[p]];
-        const int q2 = a.fpos[2 * r], q1 = a.fpos[2 * r + 1];
-        if (q2 >= 0) {
-          l2 = a.lower[q2];
-          f |= 2;
-        }
-        if (q1 >= 0) {
-          l1 = a.lower[q1];
-          f |= 1;
-        }
+        lds_double *q = rec + lane * kRecF;
+        q[0] = s;
+        q[1] = cur.l2;
+        q[2] = cur.l1;
+        q[3] = static_cast<double>(cur.flags);
       }
       double y1 = 0.0, y2 = 0.0, mine = 0.0;
+#pragma unroll 4
       for (int k = 0; k < cnt; ++k) {
-        double t = readlane_f64(s, k);
-        const int fk = __builtin_amdgcn_readlane(f, k);
-        if (fk & 2) t -= readlane_f64(l2, k) * y2;
-        if (fk & 1) t -= readlane_f64(l1, k) * y1;
+        const lds_double *q = rec + k * kRecF;
+        double t = q[0];
+        const double l2 = q[1], l1 = q[2];
+        const int fk = static_cast<int>(q[3]);
+        const double t2 = t - l2 * y2;
+        t = (fk & 2) ? t2 : t;
+        const double t1 = t - l1 * y1;
+        t = (fk & 1) ? t1 : t;
         y2 = y1;
         y1 = t;
         mine = lane == k ? t : mine;
       }
+      if (b0 + B < n) load_fwd(a, b0 + B + lane, lane < min(B, n - b0 - B), nxt);
       if (own) ys[r] = mine;
+      cur = nxt;
     }
   }
   __syncthreads();
@@ -241,47 +317,72 @@ __device__ void ch_precondition_seq(const ChArgs &a, double floor, double *ys) {
   }
   __syncthreads();
   if (wave == 0) {
-    for (int b0 = ((n - 1) / B) * B; b0 >= 0; b0 -= B) {
+    const int top = ((n - 1) / B) * B;
+    SeqBwd cur, nxt;
+    {
+      const int hi = n - 1;
+      load_bwd(a, hi - lane, lane < hi - top + 1, cur);
+    }
+    for (int b0 = top; b0 >= 0; b0 -= B) {
       const int hi = min(b0 + B, n) - 1, cnt = hi - b0 + 1;
       const int r = hi - lane;  // lane k: the k-th row of the batch from the top
       const bool own = lane < cnt;
-      double zr = 0.0, c1 = 0.0, c2 = 0.0;
-      double far[kChMaxFar];
+      int nfar = own ? cur.cnt : 0;  // the batch's longest list of far children
 #pragma unroll
-      for (int q = 0; q < kChMaxFar; ++q) far[q] = 0.0;
-      int f = 0, nf = 0;
+      for (int off = 32; off >= 1; off >>= 1) nfar = max(nfar, __shfl_xor(nfar, off, 64));
+      nfar = __builtin_amdgcn_readfirstlane(nfar);
       if (own) {
-        zr = ys[r];
-        f = a.bflag[r];
-        int t = a.lt_ptr[r];
-        const int t1 = a.lt_ptr[r + 1];
-        if (f & 1) c1 = a.lower[a.lt_pos[t++]];
-        if (f & 2) c2 = a.lower[a.lt_pos[t++]];
-        nf = t1 - t;
+        lds_double *q = rec + lane * kRecB;
+        q[0] = ys[r];
+        q[1] = cur.c1;
+        q[2] = cur.c2;
+        q[3] = static_cast<double>(cur.flags);
+        // products of the children outside the batch in list order; +0.0 past the row's list
+        // (t - (+0.0) leaves t unchanged bit for bit, -0.0 included)
 #pragma unroll
-        for (int q = 0; q < kChMaxFar; ++q)
-          if (q < nf) far[q] = a.lower[a.lt_pos[t + q]] * ys[a.lt_row[t + q]];
+        for (int e = 0; e < kChSeqWidth; ++e) q[4 + e] = e < cur.cnt ? cur.val[e] * ys[cur.row[e]] : 0.0;
       }
-      double z1 = 0.0, z2 = 0.0, mine = 0.0;
-      for (int k = 0; k < cnt; ++k) {
-        double t = readlane_f64(zr, k);
-        const int fk = __builtin_amdgcn_readlane(f, k);
-        const int nk = __builtin_amdgcn_readlane(nf, k);
-        if (fk & 1) t -= readlane_f64(c1, k) * z1;
-        if (fk & 2) t -= readlane_f64(c2, k) * z2;
-#pragma unroll
-        for (int q = 0; q < kChMaxFar; ++q)
-          if (q < nk) t -= readlane_f64(far[q], k);
-        z2 = z1;
-        z1 = t;
-        mine = lane == k ? t : mine;
+      double mine = 0.0;
+      switch (nfar) {  // the chain with the batch's far count fixed: reads issued together, no branches
+        case 0: mine = bwd_chain<0>(rec, cnt, lane); break;
+        case 1: mine = bwd_chain<1>(rec, cnt, lane); break;
+        case 2: mine = bwd_chain<2>(rec, cnt, lane); break;
+        case 3: mine = bwd_chain<3>(rec, cnt, lane); break;
+        case 4: mine = bwd_chain<4>(rec, cnt, lane); break;
+        case 5: mine = bwd_chain<5>(rec, cnt, lane); break;
+        case 6: mine = bwd_chain<6>(rec, cnt, lane); break;
+        case 7: mine = bwd_chain<7>(rec, cnt, lane); break;
+        case 8: mine = bwd_chain<8>(rec, cnt, lane); break;
+        case 9: mine = bwd_chain<9>(rec, cnt, lane); break;
+        case 10: mine = bwd_chain<10>(rec, cnt, lane); break;
+        case 11: mine = bwd_chain<11>(rec, cnt, lane); break;
+        default: mine = bwd_chain<kChSeqWidth>(rec, cnt, lane); break;
       }
+      if (b0 > 0) load_bwd(a, b0 - 1 - lane, lane < B, nxt);
       if (own) ys[r] = mine;
+      cur = nxt;
     }
   }
   __syncthreads();
   for (int r = tid; r < n; r += kThreads) a.z[r] = ys[r];
   __syncthreads();
+}
+
+// Gathers the factor's values into the sequential sweeps' packed per-row layout (once per operator).
+__global__ void __launch_bounds__(kThreads) ch_pack_kernel(ChArgs a) {
+  for (int r = blockIdx.x * kThreads + threadIdx.x; r < a.n; r += gridDim.x * kThreads) {
+    const size_t base = static_cast<size_t>(r) * kChSeqWidth;
+    for (int e = 0; e < kChSeqWidth; ++e) {
+      const int pf = a.fo_pos[base + e], pb = a.bf_pos[base + e];
+      a.fo_val[base + e] = pf >= 0 ? a.lower[pf] : 0.0;
+      a.bf_val[base + e] = pb >= 0 ? a.lower[pb] : 0.0;
+    }
+    for (int e = 0; e < 2; ++e) {
+      const int pi = a.fin_pos[2 * r + e], pc = a.bin_pos[2 * r + e];
+      a.fin_val[2 * r + e] = pi >= 0 ? a.lower[pi] : 0.0;
+      a.bin_val[2 * r + e] = pc >= 0 ? a.lower[pc] : 0.0;
+    }
+  }
 }
 
 __device__ __forceinline__ double row_times(const ChArgs &a, const double *x, int row) {
@@ -294,7 +395,7 @@ __device__ __forceinline__ double row_times(const ChArgs &a, const double *x, in
 // reference's. ildlt selects the preconditioner.
 __global__ void __launch_bounds__(kThreads) ch_pcg_kernel(ChArgs a, int ildlt) {
   __shared__ ChShared sh;
-  extern __shared__ double ch_ys[];  // the sequential sweeps' vector (a.seq_batch > 0)
+  extern __shared__ double ch_ys[];  // the sequential sweeps' vector and row records (a.seq_batch > 0)
   const int tid = threadIdx.x, n = a.n;
   int parity = 0;
   const double floor = a.state->floor;
@@ -315,7 +416,7 @@ __global__ void __launch_bounds__(kThreads) ch_pcg_kernel(ChArgs a, int ildlt) {
   int iterations = 0, status = kChOk;
   if (!(rel_res < tol || abs_res < tol)) {
     if (use_ildlt && a.seq_batch > 0)
-      ch_precondition_seq(a, floor, ch_ys);
+      ch_precondition_seq(a, floor, (lds_double *)ch_ys, (lds_double *)ch_ys + a.n);
     else
       ch_precondition(a, use_ildlt, floor);
     double rz = 0.0;
@@ -359,7 +460,7 @@ __global__ void __launch_bounds__(kThreads) ch_pcg_kernel(ChArgs a, int ildlt) {
         rel_res = abs_res / bden;
         if (rel_res < tol || abs_res < tol) break;
         if (use_ildlt && a.seq_batch > 0)
-          ch_precondition_seq(a, floor, ch_ys);
+          ch_precondition_seq(a, floor, (lds_double *)ch_ys, (lds_double *)ch_ys + a.n);
         else
           ch_precondition(a, use_ildlt, floor);
         double rzn = 0.0;
@@ -597,8 +698,14 @@ hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream) {
 }
 
 hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream) {
-  const size_t lds = ildlt && a.seq_batch > 0 ? static_cast<size_t>(a.n) * sizeof(double) : 0;
+  const size_t lds = ildlt && a.seq_batch > 0 ? (static_cast<size_t>(a.n) + 64 * kRecB) * sizeof(double) : 0;
   hipLaunchKernelGGL(ch_pcg_kernel, dim3(1), dim3(kThreads), lds, stream, a, ildlt ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_ch_pack(const ChArgs &a, hipStream_t stream) {
+  const int blocks = (a.n + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(ch_pack_kernel, dim3(blocks < 64 ? blocks : 64), dim3(kThreads), 0, stream, a);
   return hipGetLastError();
 }
 

@@ -1809,42 +1809,58 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
       }
   }
   // Batched sequential sweeps when the level schedule is close to a chain (periodic y makes row
-  // (i, 0) depend on (i-1, ny-1)) and the vectors fit in LDS: the largest batch B for which every
-  // row's in-batch lower entries are L(r, r-2), L(r, r-1) and its in-batch children r+1, r+2 (with at
-  // most kChMaxFar children outside).
+  // (i, 0) depend on (i-1, ny-1)) and the vectors fit in LDS: the largest batch B <= 64 for which every
+  // row's in-batch lower entries are L(r, r-2), L(r, r-1), its in-batch children are r+1, r+2, and
+  // at most kChSeqWidth entries of each list lie outside the batch. The operands are packed per row.
+  constexpr size_t W = pp::kChSeqWidth;
   int seq_batch = 0;
-  std::vector<int> fsplit, fpos, bflag;
+  std::vector<int> fo_cnt, fo_col, fo_pos, fin_pos, bf_cnt, bf_row, bf_pos, bin_pos;
   if (n <= pp::kChSeqMaxRows && nlev > n / 16) {
-    for (int B : {64, 32, 16, 8}) {
-      fsplit.assign(static_cast<size_t>(n), 0);
-      fpos.assign(2 * static_cast<size_t>(n), -1);
-      bflag.assign(static_cast<size_t>(n), 0);
+    for (int B = 64; B >= 8; --B) {
+      const size_t nn = static_cast<size_t>(n);
+      fo_cnt.assign(nn, 0);
+      fo_col.assign(nn * W, 0);
+      fo_pos.assign(nn * W, -1);
+      fin_pos.assign(2 * nn, -1);
+      bf_cnt.assign(nn, 0);
+      bf_row.assign(nn * W, 0);
+      bf_pos.assign(nn * W, -1);
+      bin_pos.assign(2 * nn, -1);
       bool ok = true;
       for (int r = 0; r < n && ok; ++r) {
+        const size_t ru = static_cast<size_t>(r);
         const int b0 = (r / B) * B, hi = std::min(b0 + B, n) - 1;
-        int p = A.rp[r], split = -1;
-        for (; p < A.rp[r + 1] && A.ci[p] < r; ++p) {
+        int older = 0;
+        for (int p = A.rp[r]; p < A.rp[r + 1] && A.ci[p] < r; ++p) {
           const int c = A.ci[p];
-          if (c < b0) continue;
-          if (split < 0) split = p;
-          if (c == r - 2 && fpos[2 * r] < 0 && fpos[2 * r + 1] < 0) fpos[2 * r] = p;
-          else if (c == r - 1 && fpos[2 * r + 1] < 0) fpos[2 * r + 1] = p;
-          else ok = false;
+          if (c < b0) {  // before the batch: packed in CSR order (they precede the in-batch ones)
+            if (older >= static_cast<int>(W)) {
+              ok = false;
+              break;
+            }
+            fo_col[ru * W + static_cast<size_t>(older)] = c;
+            fo_pos[ru * W + static_cast<size_t>(older)] = p;
+            ++older;
+          } else if (c == r - 2 && fin_pos[2 * ru] < 0 && fin_pos[2 * ru + 1] < 0) {
+            fin_pos[2 * ru] = p;
+          } else if (c == r - 1 && fin_pos[2 * ru + 1] < 0) {
+            fin_pos[2 * ru + 1] = p;
+          } else {
+            ok = false;
+          }
         }
-        fsplit[static_cast<size_t>(r)] = split < 0 ? p : split;
-        int t = lt_ptr[static_cast<size_t>(r)], f = 0;
-        const int t1 = lt_ptr[static_cast<size_t>(r) + 1];
-        if (t < t1 && lt_row[static_cast<size_t>(t)] == r + 1 && r + 1 <= hi) {
-          f |= 1;
-          ++t;
-        }
-        if (t < t1 && lt_row[static_cast<size_t>(t)] == r + 2 && r + 2 <= hi) {
-          f |= 2;
-          ++t;
-        }
+        fo_cnt[ru] = older;
+        int t = lt_ptr[ru];
+        const int t1 = lt_ptr[ru + 1];
+        if (t < t1 && lt_row[static_cast<size_t>(t)] == r + 1 && r + 1 <= hi) bin_pos[2 * ru] = lt_pos[static_cast<size_t>(t++)];
+        if (t < t1 && lt_row[static_cast<size_t>(t)] == r + 2 && r + 2 <= hi) bin_pos[2 * ru + 1] = lt_pos[static_cast<size_t>(t++)];
         if (t < t1 && lt_row[static_cast<size_t>(t)] <= hi) ok = false;  // another in-batch child
-        if (t1 - t > pp::kChMaxFar) ok = false;
-        bflag[static_cast<size_t>(r)] = f;
+        if (t1 - t > static_cast<int>(W)) ok = false;
+        for (int e = 0; ok && t + e < t1; ++e) {
+          bf_row[ru * W + static_cast<size_t>(e)] = lt_row[static_cast<size_t>(t + e)];
+          bf_pos[ru * W + static_cast<size_t>(e)] = lt_pos[static_cast<size_t>(t + e)];
+        }
+        bf_cnt[ru] = t1 - t;
       }
       if (ok) {
         seq_batch = B;
@@ -1852,11 +1868,8 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
       }
     }
   }
-  if (!seq_batch) {
-    fsplit.clear();
-    fpos.clear();
-    bflag.clear();
-  }
+  if (!seq_batch)
+    for (auto *v : {&fo_cnt, &fo_col, &fo_pos, &fin_pos, &bf_cnt, &bf_row, &bf_pos, &bin_pos}) v->clear();
   // device buffers
   if (C.d_int) (void)hipFree(C.d_int);
   if (C.d_dbl) (void)hipFree(C.d_dbl);
@@ -1865,8 +1878,11 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
   C.d_dbl = nullptr;
   C.d_part = nullptr;
   const size_t nl = lt_row.size();
-  const size_t ints = (n + 1) + nnz + n + (nlev + 1) + n + (n + 1) + 2 * nl + fsplit.size() + fpos.size() + bflag.size();
-  const size_t dbls = 2 * static_cast<size_t>(nnz) + 7 * static_cast<size_t>(n);
+  const size_t seq_ints = fo_cnt.size() + fo_col.size() + fo_pos.size() + fin_pos.size() + bf_cnt.size() +
+                          bf_row.size() + bf_pos.size() + bin_pos.size();
+  const size_t ints = (n + 1) + nnz + n + (nlev + 1) + n + (n + 1) + 2 * nl + seq_ints;
+  const size_t seq_dbls = seq_batch ? (2 * W + 4) * static_cast<size_t>(n) : 0;  // fo_val, bf_val, fin_val, bin_val
+  const size_t dbls = 2 * static_cast<size_t>(nnz) + 7 * static_cast<size_t>(n) + seq_dbls;
   PP_CH_CHECK(hipMalloc(&C.d_int, ints * sizeof(int)));
   PP_CH_CHECK(hipMalloc(&C.d_dbl, dbls * sizeof(double)));
   if (!C.d_state) PP_CH_CHECK(hipMalloc(&C.d_state, sizeof(pp::ChState)));
@@ -1899,9 +1915,14 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
   if (int rc = put_int(lt_pos, a.lt_pos)) return rc;
   a.seq_batch = seq_batch;
   if (seq_batch) {
-    if (int rc = put_int(fsplit, a.fsplit)) return rc;
-    if (int rc = put_int(fpos, a.fpos)) return rc;
-    if (int rc = put_int(bflag, a.bflag)) return rc;
+    if (int rc = put_int(fo_cnt, a.fo_cnt)) return rc;
+    if (int rc = put_int(fo_col, a.fo_col)) return rc;
+    if (int rc = put_int(fo_pos, a.fo_pos)) return rc;
+    if (int rc = put_int(fin_pos, a.fin_pos)) return rc;
+    if (int rc = put_int(bf_cnt, a.bf_cnt)) return rc;
+    if (int rc = put_int(bf_row, a.bf_row)) return rc;
+    if (int rc = put_int(bf_pos, a.bf_pos)) return rc;
+    if (int rc = put_int(bin_pos, a.bin_pos)) return rc;
   }
   PP_CH_CHECK(hipMemcpyAsync(dp, A.v.data(), A.v.size() * sizeof(double), hipMemcpyHostToDevice, s));
   a.val = dp;
@@ -1913,11 +1934,21 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
   a.p = a.z + n;
   a.ap = a.p + n;
   a.y = a.ap + n;
+  if (seq_batch) {
+    a.fo_val = a.y + n;
+    a.bf_val = a.fo_val + W * static_cast<size_t>(n);
+    a.fin_val = a.bf_val + W * static_cast<size_t>(n);
+    a.bin_val = a.fin_val + 2 * static_cast<size_t>(n);
+  }
   a.state = C.d_state;
   PP_CH_CHECK(pp::launch_ch_factor(a, s));
   PP_CH_CHECK(hipMemcpyAsync(C.h_state, C.d_state, sizeof(pp::ChState), hipMemcpyDeviceToHost, s));
   PP_CH_CHECK(hipStreamSynchronize(s));
   C.kind = C.h_state->ildlt_ok ? PP_CH_ILDLT : PP_CH_DIAGONAL;
+  if (seq_batch && C.kind == PP_CH_ILDLT) {
+    PP_CH_CHECK(pp::launch_ch_pack(a, s));
+    PP_CH_CHECK(hipStreamSynchronize(s));
+  }
   C.nx = cp->nx;
   C.ny = cp->ny;
   C.px = cp->periodic_x != 0;

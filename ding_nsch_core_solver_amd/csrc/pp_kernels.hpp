@@ -185,15 +185,21 @@ struct ChArgs {
   double tolerance;
   ChState *state;
   // Batched sequential sweeps (seq_batch = B > 0, used when the level schedule degenerates into a
-  // chain, as periodic y makes it): rows in batches of B, the vectors in LDS. fsplit[r]: the CSR
-  // position where row r's lower entries inside its batch start (they are L(r, r-2), L(r, r-1) at
-  // fpos[2r], fpos[2r+1], -1 when absent); bflag[r]: bit 0 / 1 when the children r+1 / r+2 lie in the
-  // batch (they lead the ascending child list), the rest of the list (at most kChMaxFar) outside.
+  // chain, as periodic y makes it): rows in batches of B, the vectors in LDS, every row's operands
+  // packed (kChSeqWidth per row, in the reference's order) so a batch's loads are independent and
+  // can be issued while the previous batch runs. Forward: fo_cnt[r] lower entries before the batch
+  // (columns fo_col, CSR positions fo_pos), then the in-batch L(r, r-2), L(r, r-1) (CSR positions
+  // fin_pos[2r], fin_pos[2r+1], -1 when absent). Backward: the in-batch children r+1, r+2 (positions
+  // bin_pos[2r], bin_pos[2r+1] of L(r+1, r), L(r+2, r)), which lead the ascending child list, then
+  // bf_cnt[r] children outside the batch (rows bf_row, positions bf_pos). The values are gathered
+  // from the factor once per operator (launch_ch_pack) into fo_val, fin_val, bf_val, bin_val.
   int seq_batch;
-  const int *fsplit, *fpos, *bflag;
+  const int *fo_cnt, *fo_col, *fo_pos, *fin_pos, *bf_cnt, *bf_row, *bf_pos, *bin_pos;
+  double *fo_val, *fin_val, *bf_val, *bin_val;
 };
-constexpr int kChMaxFar = 12;         // children outside the batch a sequential backward row may have
-constexpr int kChSeqMaxRows = 7936;   // the sequential sweeps keep n doubles in LDS
+constexpr int kChSeqWidth = 12;       // entries of a row outside its batch (the 13-point stencil has at most 12)
+constexpr int kChSeqMaxRows = 6912;   // the sequential sweeps keep n doubles (and 8 KB of row records) in LDS
+hipError_t launch_ch_pack(const ChArgs &a, hipStream_t stream);
 hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream);
 hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream);
 
