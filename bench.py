@@ -36,6 +36,9 @@ BATCH_TRAFFIC_FILE = os.path.join(REPO, "profiles", "batched_256_uneven_mixed_tr
 BATCH64_TRAFFIC_FILE = os.path.join(REPO, "profiles", "batched_256_uneven_fp64_traffic.txt")
 CORR_TRAFFIC_FILE = os.path.join(REPO, "profiles", "velocity_correction_192x192_traffic.txt")
 BICG_TRAFFIC_FILE = os.path.join(REPO, "profiles", "bicgstab_momentum_192x192_traffic.txt")
+BICG1024_TRAFFIC_FILE = os.path.join(REPO, "profiles", "bicgstab_momentum_1024x1024_traffic.txt")
+MOMENTUM_TRAFFIC_FILE = os.path.join(REPO, "profiles", "momentum_predictor_192x384_traffic.txt")
+CH_TRAFFIC_FILE = os.path.join(REPO, "profiles", "ch_solve_192x384_traffic.txt")
 
 
 def log(msg):
@@ -335,8 +338,10 @@ def momentum_leg(eng, torch, dev, steps, cpu_seconds, cpu=True, side=192):
                         "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": read_traffic(BICG_TRAFFIC_FILE),
                         "algorithmic_bytes_per_solve": alg,
                         "note": "whole solve (all passes and the host's state checks); traffic per solve"}}
-    if side != 192:
-        out["roofline"]["traffic"] = None  # the measured traffic file covers the 192x192 system
+    if side == 1024:
+        out["roofline"]["traffic"] = read_traffic(BICG1024_TRAFFIC_FILE)
+    elif side != 192:
+        out["roofline"]["traffic"] = None
     if cpu:
         try:
             from oracle import ref
@@ -422,7 +427,7 @@ def momentum_predictor_leg(eng, torch, dev, steps, cpu_seconds, cpu=True):
            "kernels": "momentum_assemble/count/block_scan/compact/convection/ghost + BiCGSTAB passes + "
                       "momentum_scatter/ghost",
            "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": read_traffic(MOMENTUM_TRAFFIC_FILE),
                         "algorithmic_bytes_per_step": alg,
                         "note": "whole predictor (assembly, solve, scatter); nnz taken as its bound 9 n"}}
     if cpu:
@@ -504,7 +509,7 @@ def ch_leg(eng, torch, dev, steps, cpu_seconds, cpu=True):
            "data": "synthetic phase field and right-hand side on the rising-bubble grid",
            "kernels": "ch_diag_setup + per iteration ch_diag_pass_a / ch_diag_pass_b + ch_diag_eq / finish",
            "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": read_traffic(CH_TRAFFIC_FILE),
                         "algorithmic_bytes_per_solve": alg,
                         "note": "ILDLT(0) of this operator fails at this grid, as in the reference, so the solve "
                                 "is the diagonal-preconditioned CG (full-chip passes); whole solve over HIP-event "
@@ -601,6 +606,92 @@ def copy_calibration(nbytes=256 << 20, repeats=3):
     log(f"[bench] copied {nbytes} bytes x{repeats}")
 
 
+def _read_csv(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _find_csv(directory, suffix):
+    for root, _, files in os.walk(directory):
+        for name in sorted(files):
+            if name.endswith(suffix):
+                return os.path.join(root, name)
+    raise FileNotFoundError(f"no *{suffix} under {directory}")
+
+
+def _short_name(name):
+    return name.replace("void ", "").replace("(anonymous namespace)::", "").split("(")[0]
+
+
+def summarize_trace(directory, match=None):
+    """Per-kernel dispatch statistics of a `rocprofv3 --kernel-trace --output-format csv` run, grouped
+    by kernel name and grid size: calls, total / average / min / max duration in ms, and the kernel's
+    resources (LDS, scratch, VGPR, SGPR, workgroup)."""
+    rows = _read_csv(_find_csv(directory, "kernel_trace.csv"))
+    groups = {}
+    for r in rows:
+        name = r["Kernel_Name"]
+        if match and not any(m in name for m in match):
+            continue
+        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        g = groups.setdefault((name, grid), {"d": [], "r": r})
+        g["d"].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+    lines = [f"{'kernel':<62} {'grid':>8} {'calls':>6} {'total_ms':>10} {'avg_ms':>10} {'min_ms':>10} {'max_ms':>10}"]
+    for (name, grid), g in sorted(groups.items(), key=lambda kv: -sum(kv[1]["d"])):
+        d = g["d"]
+        short = _short_name(name)
+        lines.append(f"{short:<62} {grid:>8} {len(d):>6} {sum(d):>10.3f} {sum(d) / len(d):>10.4f} {min(d):>10.4f} "
+                     f"{max(d):>10.4f}")
+    lines.append("")
+    for (name, grid), g in sorted(groups.items(), key=lambda kv: -sum(kv[1]["d"])):
+        r = g["r"]
+        short = _short_name(name)
+        lines.append(f"{short} grid {grid}: workgroup {r['Workgroup_Size_X']}, LDS {r['LDS_Block_Size']} B, "
+                     f"scratch {r['Scratch_Size']} B/lane, VGPR {r['VGPR_Count']}, SGPR {r['SGPR_Count']}")
+    return "\n".join(lines)
+
+
+def summarize_traffic(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, match=None, grids=None, calls=None):
+    """HBM bytes per dispatch from two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs (KiB per
+    dispatch), each scaled by what the counter reads for the --calibrate-copy run's 256 MiB device
+    copy (its largest dispatches read and write 256 MiB), grouped by kernel name and grid size."""
+    def per_kernel(directory):
+        out = {}
+        for r in _read_csv(_find_csv(directory, "counter_collection.csv")):
+            name = r["Kernel_Name"]
+            if match and not any(m in name for m in match):
+                continue
+            if grids and int(r["Grid_Size"]) not in grids:
+                continue
+            out.setdefault((name, int(r["Grid_Size"])), []).append(float(r["Counter_Value"]))
+        return out
+
+    def copy_value(directory):
+        vals = [float(r["Counter_Value"]) for r in _read_csv(_find_csv(directory, "counter_collection.csv"))]
+        return max(vals)
+
+    mib256_kib = 256 * 1024.0
+    f_scale = mib256_kib / copy_value(copy_fetch_dir)
+    w_scale = mib256_kib / copy_value(copy_write_dir)
+    fetch, write = per_kernel(fetch_dir), per_kernel(write_dir)
+    lines = [f"copy calibration (256 MiB device copy): FETCH_SIZE {copy_value(copy_fetch_dir):.1f} KiB, "
+             f"WRITE_SIZE {copy_value(copy_write_dir):.1f} KiB -> scale factors fetch x{f_scale:.4f}, "
+             f"write x{w_scale:.4f} (bytes moved = counter x factor)",
+             f"{'kernel':<62} {'grid':>8} {'calls':>6} {'FETCH_KiB':>12} {'WRITE_KiB':>12} {'bytes/dispatch':>16}"]
+    for key in sorted(fetch, key=lambda k: -sum(fetch[k])):
+        if key not in write:
+            continue
+        fk, wk = np.mean(fetch[key]), np.mean(write[key])
+        b = (fk * f_scale + wk * w_scale) * 1024.0
+        short = _short_name(key[0])
+        lines.append(f"{short:<62} {key[1]:>8} {len(fetch[key]):>6} {fk:>12.1f} {wk:>12.1f} {b:>16.0f}")
+    if calls:
+        total = sum(sum(fetch[k]) * f_scale + sum(write.get(k, [0.0])) * w_scale for k in fetch) * 1024.0
+        lines.append(f"all listed dispatches: {total:.0f} bytes over {calls} calls -> {total / calls:.0f} bytes per call")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -613,7 +704,20 @@ def main():
     ap.add_argument("--batched-repeats", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print the per-kernel summary of a rocprofv3 --kernel-trace --output-format csv run")
+    ap.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="print HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE csv runs")
+    ap.add_argument("--kernels", nargs="*", help="with --summarize-*: only kernels whose name contains one of these")
+    ap.add_argument("--grids", nargs="*", type=int, help="with --summarize-traffic: only dispatches of these grid sizes")
+    ap.add_argument("--calls", type=int, help="with --summarize-traffic: also print the listed dispatches' bytes per call")
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace, args.kernels))
+        return
+    if args.summarize_traffic:
+        print(summarize_traffic(*args.summarize_traffic, args.kernels, args.grids, args.calls))
+        return
 
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this engine runs one pressure system per "
