@@ -1020,8 +1020,12 @@ __device__ __forceinline__ double win_level(const Stencil<T> &A, const global_pt
                                             double acc, F &epi) {
   const int nx = A.nx, ny = A.ny;
   const int b0 = lb[d];
+  const int row_lo = level_lo(d, ny), row_hi = d < nx - 1 ? d : nx - 1;  // the level's rows
 #pragma unroll
   for (int c = 0; c < S; ++c) {
+    // a chunk with no row on this level contributes nothing (its loads were issued anyway, clamped
+    // to real cells, so the prefetch stays unconditional); the branch is wave-uniform
+    if (64 * c > row_hi || 64 * c + 63 < row_lo) continue;
     const int i = 64 * c + lane, j = d - i;
     const bool in_i = i < nx;
     const bool in_j = static_cast<unsigned>(j) < static_cast<unsigned>(ny);
