@@ -3,7 +3,7 @@
 // The reference solves it on the host with build_diagonal_preconditioner (ch_sparse_krylov.hpp:
 // 421-431) and solve_preconditioned_bicgstab (839-963), called at momentum.cpp:469-472. Here every
 // vector pass is a full-chip kernel: 512-thread workgroups, each owning one chunk of 512 rows
-// (row_chunk: contiguous chunks per XCD), one thread per row. SpMV row sums run over the row's
+// (paired_chunk: each XCD takes interleaved u and v chunks of the same cells), one thread per row. SpMV row sums run over the row's
 // CSR entries in order from 0.0 (apply_matrix, 235-250), and every vector update is the
 // reference's expression term for term, so only the dot products differ from the reference, by
 // summation order: each pass leaves one partial per chunk (the canonical 64-lane and 8-wave trees
@@ -28,7 +28,7 @@ namespace {
 
 constexpr int kRows = kBicgRows;  // rows per workgroup, one per thread
 static_assert(kRows == chunked::kRows, "BiCGSTAB chunks are the shared 512-row chunks");
-using chunked::row_chunk;
+using chunked::paired_chunk;
 using chunked::chunk_partials;
 using chunked::dot_sums;
 
@@ -93,7 +93,7 @@ __device__ __forceinline__ void record(const BicgArgs &a, int status, int iterat
 // finalize_row_maps records it), r = b - A x, r_hat = r, p = v = 0 (both parities); partials b.b,
 // r.r.
 __global__ void __launch_bounds__(kRows) bicg_setup_kernel(BicgArgs a) {
-  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  const int chunk = paired_chunk(a.chunks), q = chunk * kRows + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (q < a.n) {
     int diag = -1;
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(kRows) bicg_setup_kernel(BicgArgs a) {
     v[0] = v[0] + b * b;
     v[1] = v[1] + r * r;
   }
-  chunk_partials<2>(v, a.p_setup, a.chunks);
+  chunk_partials<2>(v, a.p_setup, a.chunks, chunk);
 }
 
 // Pass A of iteration k (or the report after the last iteration, final): the stop test of the
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_a_kernel(BicgArgs a, int k, i
   const double *__restrict__ v_old = a.v[(k + 1) & 1];
   const double *__restrict__ r = a.r;
   const double *__restrict__ dinv = a.dinv;
-  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  const int chunk = paired_chunk(a.chunks), q = chunk * kRows + threadIdx.x;
   int beg = 0, end = 0;
   double rq = 0.0, pq_old = 0.0, vq_old = 0.0, dq = 0.0, rhq = 0.0;
   if (q < a.n) {
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_a_kernel(BicgArgs a, int k, i
     a.v[k & 1][q] = sum;
     v[0] = v[0] + rhq * sum;
   }
-  chunk_partials<1>(v, a.p_v, a.chunks);
+  chunk_partials<1>(v, a.p_v, a.chunks, chunk);
 }
 
 // Pass B: alpha = rho / r_hat.v (with its check); s = r - alpha v; update = alpha p_hat;
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_b_kernel(BicgArgs a, int k) {
   const double *__restrict__ vk = a.v[k & 1];
   const double *__restrict__ r = a.r;
   const double *__restrict__ dinv = a.dinv;
-  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  const int chunk = paired_chunk(a.chunks), q = chunk * kRows + threadIdx.x;
   int beg = 0, end = 0;
   double rq = 0.0, vq = 0.0, dq = 0.0, phq = 0.0;
   if (q < a.n) {  // issued before the partial sums, as in pass A
@@ -250,14 +250,14 @@ __global__ void __launch_bounds__(kRows) bicg_pass_b_kernel(BicgArgs a, int k) {
     v[1] = v[1] + sum * sum;
     v[2] = v[2] + sum * sq;
   }
-  chunk_partials<3>(v, a.p_b, a.chunks);
+  chunk_partials<3>(v, a.p_b, a.chunks, chunk);
 }
 
 // Pass C: the half-step test on |s| (converged: x += update, partials update.update, x.x, and
 // workgroup 0 marks the half exit for the next pass A); else omega = t.s / t.t with both checks,
 // update += omega s_hat; x += update; r = s - omega t; partials update.update, x.x, r.r, r_hat.r.
 __global__ void __launch_bounds__(kRows) bicg_pass_c_kernel(BicgArgs a, int k) {
-  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  const int chunk = paired_chunk(a.chunks), q = chunk * kRows + threadIdx.x;
   double uq = 0.0, xq = 0.0, shq = 0.0, sq = 0.0, tq = 0.0, rhq = 0.0;
   if (q < a.n) {  // issued before the partial sums, as in pass A
     uq = a.update[q];
@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_c_kernel(BicgArgs a, int k) {
       v[0] = v[0] + u * u;
       v[1] = v[1] + x * x;
     }
-    chunk_partials<2>(v, a.p_exit, a.chunks);
+    chunk_partials<2>(v, a.p_exit, a.chunks, chunk);
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // status stays running: every workgroup does its rows
       a.state->abs_res = s_norm;
       a.state->rel_res = s_norm / floor30(bn);
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(kRows) bicg_pass_c_kernel(BicgArgs a, int k) {
     v[2] = v[2] + r * r;
     v[3] = v[3] + rhq * r;
   }
-  chunk_partials<4>(v, a.p_x, a.chunks);
+  chunk_partials<4>(v, a.p_x, a.chunks, chunk);
 }
 
 // Sum pass for large systems: one workgroup reduces partial slots [first, first + count) as

@@ -551,7 +551,7 @@ __device__ __forceinline__ void ch_record(const ChDiagArgs &a, int status, int i
 // are formed before the initial convergence test, which pass A of iteration 0 makes; they go unused
 // when it stops.
 __global__ void __launch_bounds__(kRows) ch_diag_setup_kernel(ChDiagArgs a) {
-  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  const int chunk = row_chunk(a.chunks), q = chunk * kRows + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
   if (q < a.n) {
     const double b = a.rhs[q];
@@ -564,7 +564,7 @@ __global__ void __launch_bounds__(kRows) ch_diag_setup_kernel(ChDiagArgs a) {
     v[1] = v[1] + r * r;
     v[2] = v[2] + r * z;
   }
-  chunk_partials<3>(v, a.partials + kChSlotSetup * static_cast<size_t>(a.chunks), a.chunks);
+  chunk_partials<3>(v, a.partials + kChSlotSetup * static_cast<size_t>(a.chunks), a.chunks, chunk);
 }
 
 // Pass A of iteration k (final: the report after the limit's last iteration). The tests of the
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(kRows) ch_diag_pass_a_kernel(ChDiagArgs a, int
     beta = u[3] / a.state->rz[(k - 1) & 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) a.state->rz[k & 1] = u[3];
   }
-  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  const int chunk = row_chunk(a.chunks), q = chunk * kRows + threadIdx.x;
   const double *__restrict__ z = a.z;
   const double *__restrict__ po = a.p[(k - 1) & 1];
   double v[1] = {0.0};
@@ -629,7 +629,7 @@ __global__ void __launch_bounds__(kRows) ch_diag_pass_a_kernel(ChDiagArgs a, int
     a.ap[q] = s;
     v[0] = v[0] + pq * s;
   }
-  chunk_partials<1>(v, a.partials + kChSlotPap * static_cast<size_t>(a.chunks), a.chunks);
+  chunk_partials<1>(v, a.partials + kChSlotPap * static_cast<size_t>(a.chunks), a.chunks, chunk);
 }
 
 // Pass B of iteration k: p^T A p valid (588-594), alpha, x += alpha p, r -= alpha A p, z = M^-1 r
@@ -643,7 +643,7 @@ __global__ void __launch_bounds__(kRows) ch_diag_pass_b_kernel(ChDiagArgs a, int
     return;
   }
   const double alpha = a.state->rz[k & 1] / pap[0];
-  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  const int chunk = row_chunk(a.chunks), q = chunk * kRows + threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (q < a.n) {
     const double delta = alpha * a.p[k & 1][q];
@@ -658,18 +658,18 @@ __global__ void __launch_bounds__(kRows) ch_diag_pass_b_kernel(ChDiagArgs a, int
     v[2] = v[2] + rn * rn;
     v[3] = v[3] + rn * zn;
   }
-  chunk_partials<4>(v, a.partials + kChSlotUpd * static_cast<size_t>(a.chunks), a.chunks);
+  chunk_partials<4>(v, a.partials + kChSlotUpd * static_cast<size_t>(a.chunks), a.chunks, chunk);
 }
 
 // |rhs - A x|^2 of the returned x (phase_field.cpp:481-502), per chunk.
 __global__ void __launch_bounds__(kRows) ch_diag_eq_kernel(ChDiagArgs a) {
-  const int q = row_chunk(a.chunks) * kRows + threadIdx.x;
+  const int chunk = row_chunk(a.chunks), q = chunk * kRows + threadIdx.x;
   double v[1] = {0.0};
   if (q < a.n) {
     const double d = a.rhs[q] - csr_row(a, a.x, q);
     v[0] = v[0] + d * d;
   }
-  chunk_partials<1>(v, a.partials + kChSlotEq * static_cast<size_t>(a.chunks), a.chunks);
+  chunk_partials<1>(v, a.partials + kChSlotEq * static_cast<size_t>(a.chunks), a.chunks, chunk);
 }
 
 __global__ void __launch_bounds__(kRows) ch_diag_finish_kernel(ChDiagArgs a) {

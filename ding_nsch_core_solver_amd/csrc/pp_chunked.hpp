@@ -24,10 +24,22 @@ __device__ __forceinline__ int row_chunk(int chunks) {
   return x * per + (x < rem ? x : rem) + i;
 }
 
+// Row chunk of this workgroup for a system made of two blocks of nearly equal size (the u and v
+// unknowns of the momentum system, the v row of a face ~n/2 after the u rows it couples to): the
+// contiguous run of positions row_chunk gives XCD x is mapped to interleaved chunks, position 2m to
+// chunk m and 2m + 1 to chunk ceil(chunks/2) + m, so a u chunk and the v chunk it couples to are
+// worked on by the same XCD at about the same time and the coupling gathers hit its L2. A
+// bijection on [0, chunks) like row_chunk, so the partials (indexed by chunk) sum the same way.
+__device__ __forceinline__ int paired_chunk(int chunks) {
+  const int pos = row_chunk(chunks), half = (chunks + 1) >> 1;
+  return (pos & 1) ? half + (pos >> 1) : (pos >> 1);
+}
+
 // K per-thread values summed over the workgroup: lanes with offsets 32..1, then the 8 wave sums
-// with offsets 4, 2, 1. Thread 0 stores sum k at partials[k * chunks + row_chunk(chunks)].
+// with offsets 4, 2, 1. Thread 0 stores sum k at partials[k * chunks + chunk], chunk being the row
+// chunk the workgroup worked on.
 template <int K>
-__device__ __forceinline__ void chunk_partials(double (&v)[K], double *partials, int chunks) {
+__device__ __forceinline__ void chunk_partials(double (&v)[K], double *partials, int chunks, int chunk) {
   __shared__ double red[K * 8];
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -48,7 +60,7 @@ __device__ __forceinline__ void chunk_partials(double (&v)[K], double *partials,
       for (int t = 0; t < 4; ++t) w[t] = red[k * 8 + t] + red[k * 8 + t + 4];
       w[0] = w[0] + w[2];
       w[1] = w[1] + w[3];
-      partials[k * chunks + row_chunk(chunks)] = w[0] + w[1];
+      partials[k * chunks + chunk] = w[0] + w[1];
     }
   }
 }
