@@ -12,7 +12,7 @@ import pytest
 from ding_nsch_core_solver_amd import build as B
 
 # bytes of spill scratch per lane allowed for each kernel family
-BUDGET = {"fp64_solve_kernel": 64, "mixed_solve_kernel": 128, "correction_faces_kernel": 0,
+BUDGET = {"fp64_solve_kernel": 32, "mixed_solve_kernel": 96, "correction_faces_kernel": 0,
           "correction_history_kernel": 0, "correction_finish_kernel": 0, "pressure_ghost_kernel": 0,
           "bicg_": 0, "momentum_": 0}
 
