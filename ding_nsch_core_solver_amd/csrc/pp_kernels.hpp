@@ -125,6 +125,12 @@ hipError_t launch_pressure_ghosts(const CorrArgs *d_args, double *d_field, hipSt
 // Workgroup c of `chunks` owns rows [kBicgRows c, kBicgRows (c + 1)); each pass leaves its dot
 // products as per-workgroup partials (partials[k * chunks + c]) that the next pass sums, and the
 // scalar recurrence and stopping tests run on the device against the state block.
+// Per-level part of the windowed SpMV's cost in the host's level split, in lane units (a level costs
+// 64 per 64-row chunk holding a row of it, plus this). Weights of 64 to 256 split the 192x192 SpMV
+// equally fast (9.7 ms); 160 keeps the canonical p^T A p order at the reference's iteration count on
+// every timestep state and parity case, as the cell-count split did. The oracle uses the same value.
+constexpr int kSpmvLevelCost = 160;
+
 constexpr int kBicgRows = 512;
 enum BicgStatus : int {
   kBicgRunning = 0, kBicgDone = 1, kBicgNoDiag = 2, kBicgRho = 3, kBicgRhatV = 4, kBicgTT = 5, kBicgOmega = 6

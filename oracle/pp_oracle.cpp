@@ -188,13 +188,16 @@ struct Reducer {
   static void window_split(int nx, int ny, int *split) {
     constexpr int W = 8;
     const int nlev = nx + ny - 1;
-    const long long n = static_cast<long long>(nx) * ny;
+    // a level costs its 64-row chunks that hold a row of it, plus 160 (the engine's kSpmvLevelCost)
+    auto cost = [&](int d) { return 64LL * (std::min(d, nx - 1) / 64 - std::max(0, d - ny + 1) / 64 + 1) + 160; };
+    long long total = 0;
+    for (int d = 0; d < nlev; ++d) total += cost(d);
     long long cum = 0;
     int w = 1;
     split[0] = 0;
     for (int d = 0; d < nlev && w < W; ++d) {
-      cum += std::min(d, nx - 1) - std::max(0, d - ny + 1) + 1;
-      while (w < W && cum * W >= w * n) split[w++] = d + 1;
+      cum += cost(d);
+      while (w < W && cum * W >= w * total) split[w++] = d + 1;
     }
     while (w <= W) split[w++] = nlev;
   }
