@@ -1657,7 +1657,9 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
       clk.mark(1);
       if (!isfinite(pap) || fabs(pap) < 1.0e-30) break;
       const float alpha = static_cast<float>(rz / pap);
-      double up[2] = {0.0, 0.0};
+      // sums of d, r and r^2 in one pass: |r - m_r|^2 = sum r^2 - (sum r)^2 / n. r is projected
+      // every iteration, so its mean is at rounding level and the subtraction cancels nothing.
+      double up[3] = {0.0, 0.0, 0.0};
       for (int base = tid; base < n; base += kThreads * kUnroll) {
         float dq[kUnroll], rq[kUnroll], pq[kUnroll], aq[kUnroll];
 #pragma unroll
@@ -1679,49 +1681,22 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
             r32[q] = rn;
             up[0] = up[0] + static_cast<double>(dn);
             up[1] = up[1] + static_cast<double>(rn);
+            up[2] = up[2] + static_cast<double>(rn) * static_cast<double>(rn);
           }
         }
       }
-      block_sum<2>(up, sh, parity);
+      block_sum<3>(up, sh, parity);
       m_d = projected ? static_cast<float>(up[0] / nd) : 0.0f;
       m_r = projected ? static_cast<float>(up[1] / nd) : 0.0f;
-      double rri = 0.0;
-      for (int base = tid; base < n; base += kThreads * kUnroll) {
-        float rq[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          const int q = base + u * kThreads;
-          rq[u] = r32[q < n ? q : 0];
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          if (base + u * kThreads < n) {
-            const double tq = static_cast<double>(rq[u] - m_r);
-            rri = rri + tq * tq;
-          }
-        }
-      }
-      rri = block_sum1(rri, sh, parity);
+      const double rri = projected ? up[2] - up[1] * (up[1] / nd) : up[2];
       clk.mark(3);
       ++inner;
       ++total;
       if (sqrt(rri < 0.0 ? 0.0 : rri) < cycle_tol * rn0 || inner >= inner_cap || total >= limit) break;
       precondition<float, kIldlt>(a, fwd, bwd, sh, m_r, 0.0f, clk.ticks, clk.t);
-      zs = 0.0;
-      for (int base = tid; base < n; base += kThreads * kUnroll) {
-        float zq[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          const int q = base + u * kThreads;
-          zq[u] = z32[q < n ? q : 0];
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
-          if (base + u * kThreads < n) zs = zs + static_cast<double>(zq[u]);
-      }
-      zs = block_sum1(zs, sh, parity);
-      m_z = projected ? static_cast<float>(zs / nd) : 0.0f;
-      double rzn = 0.0;
+      // mean(z) and (r - m_r).(z - m_z) in one pass: sum z, sum (r - m_r) z and sum (r - m_r), then
+      // (r - m_r).(z - m_z) = sum (r - m_r) z - m_z sum (r - m_r)
+      double zr[3] = {0.0, 0.0, 0.0};
       for (int base = tid; base < n; base += kThreads * kUnroll) {
         float zq[kUnroll], rq[kUnroll];
 #pragma unroll
@@ -1733,10 +1708,18 @@ __device__ __forceinline__ void mixed_solve(const SysArgs &a, SweepShared &sh) {
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u)
-          if (base + u * kThreads < n)
-            rzn = rzn + static_cast<double>(rq[u] - m_r) * static_cast<double>(zq[u] - m_z);
+          if (base + u * kThreads < n) {
+            const double rc = static_cast<double>(rq[u] - m_r), zc = static_cast<double>(zq[u]);
+            zr[0] = zr[0] + zc;
+            zr[1] = zr[1] + rc * zc;
+            zr[2] = zr[2] + rc;
+          }
       }
-      rzn = block_sum1(rzn, sh, parity);
+      block_sum<3>(zr, sh, parity);
+      zs = zr[0];
+      const double mz = projected ? zs / nd : 0.0;
+      m_z = static_cast<float>(mz);
+      const double rzn = zr[1] - mz * zr[2];
       clk.mark(7);
       if (!isfinite(rzn) || rzn <= 0.0) break;
       const float beta = static_cast<float>(rzn / rz);
