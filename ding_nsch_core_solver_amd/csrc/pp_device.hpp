@@ -27,8 +27,8 @@ namespace pp {
 constexpr int kThreads = 512;             // threads per system block (8 waves, 256 VGPRs per lane)
 constexpr int kWaves = kThreads / 64;     // 8
 constexpr int kMaxRows = 64 * kWaves;     // nx <= 512: one sweep wave per 64 rows
-constexpr int kMaxSide = 1024;            // ny <= 1024
-constexpr int kMaxLevels = 2 * kMaxSide - 1;
+constexpr int kMaxSide = 1024;            // ny <= 1024 where the periodic-x wrap line sits in LDS
+constexpr int kMaxColumns = 65535;        // ny <= 65535 otherwise (the (i << 16) | j cell code)
 constexpr int kRing = 64;                 // hand-off ring depth between sweep waves
 constexpr int kPad = 1024;                // padding after every work array (lanes outside the grid)
 constexpr int kStageDoubles = 16384;      // 128 KB of LDS for the sweep operand ring

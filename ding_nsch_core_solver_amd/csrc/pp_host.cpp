@@ -187,8 +187,12 @@ const char *status_text(int status, int scheme) {
 // Checks of the problem itself; ctx may be null (the error text then only goes to the report).
 int validate_problem(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
   if (!pb) return fail(ctx, report, PP_ERR_ARG, "null problem");
-  if (pb->nx < 1 || pb->ny < 1 || pb->nx > pp::kMaxRows || pb->ny > pp::kMaxSide)
-    return fail(ctx, report, PP_ERR_ARG, "grid size outside nx 1..512, ny 1..1024");
+  // nx: one row per lane of the 8 waves. ny: the (i << 16) | j cell code; a periodic-x grid also
+  // keeps its wrap line (ny values) in LDS, sized for kMaxSide.
+  if (pb->nx < 1 || pb->ny < 1 || pb->nx > pp::kMaxRows || pb->ny > pp::kMaxColumns)
+    return fail(ctx, report, PP_ERR_ARG, "grid size outside nx 1..512, ny 1..65535");
+  if (pb->periodic_x && pb->ny > pp::kMaxSide)
+    return fail(ctx, report, PP_ERR_ARG, "a grid periodic in x takes ny 1..1024");
   if ((pb->periodic_x && pb->nx < 4) || (pb->periodic_y && pb->ny < 4))
     return fail(ctx, report, PP_ERR_ARG, "periodic direction needs at least 4 cells");
   if (pb->scheme < PP_ICPCG || pb->scheme > PP_SPLIT_ILDLT_PCG) return fail(ctx, report, PP_ERR_ARG, "unknown scheme");
@@ -1014,7 +1018,7 @@ int pp_set_tuning(pp_ctx *ctx, int count, const int *nx, const int *ny, const in
   if (count < 0 || (count > 0 && (!nx || !ny || !ring_depth))) return fail(ctx, report, PP_ERR_ARG, "bad tuning table");
   std::map<std::pair<int, int>, int> table;
   for (int k = 0; k < count; ++k) {
-    if (nx[k] < 1 || ny[k] < 1 || nx[k] > pp::kMaxRows || ny[k] > pp::kMaxSide ||
+    if (nx[k] < 1 || ny[k] < 1 || nx[k] > pp::kMaxRows || ny[k] > pp::kMaxColumns ||
         !pp::valid_ring((nx[k] + 63) / 64, ring_depth[k]))
       return fail(ctx, report, PP_ERR_ARG, "invalid tuning entry");
     table[std::make_pair(nx[k], ny[k])] = ring_depth[k];
@@ -1412,6 +1416,8 @@ int pp_solve_device_noalloc(const pp_problem *pb, const double *d_rho, const dou
     return fail(ctx, report, PP_ERR_ARG,
                 "grid has more cells than the allocation-free solve supports (nx * ny <= " +
                     std::to_string(kNoAllocMaxCells) + ")");
+  if (pb->ny > pp::kMaxSide)  // its level-base table is sized for nx + ny - 1 <= 512 + 1024 - 1 levels
+    return fail(ctx, report, PP_ERR_ARG, "the allocation-free solve takes ny 1..1024");
   if (!d_rho && !(pb->scheme == PP_SPLIT_ICPCG || pb->scheme == PP_SPLIT_ILDLT_PCG))
     return fail(ctx, report, PP_ERR_ARG, "rho is required");
   if (!d_u || !d_v || !d_p_out) return fail(ctx, report, PP_ERR_ARG, "null field pointer");

@@ -47,7 +47,8 @@ enum pp_precision {
 enum pp_bc_type { PP_BC_UNSET = 0, PP_BC_DIRICHLET = 1, PP_BC_NEUMANN = 2 };
 
 typedef struct pp_problem {
-  int nx, ny;             /* cells; nx 1..512, ny 1..1024, >= 4 along a periodic direction     */
+  int nx, ny;             /* cells; nx 1..512, ny 1..65535 (1..1024 if periodic in x), >= 4     */
+                          /* along a periodic direction                                         */
   double dx, dy;          /* the Solver's dx_, dy_ (lx / nx, ly / ny)                            */
   double dt;              /* cfg_.dt                                                              */
   int periodic_x, periodic_y;

@@ -122,3 +122,27 @@ def test_status_codes_match_header():
     body = body[:body.index("};")]
     codes = dict(re.findall(r"(PP_\w+)\s*=\s*(\d+)", body))
     assert codes and all(getattr(E, k) == int(v) for k, v in codes.items()), codes
+
+
+@pytest.mark.parametrize("nx,ny,periodic_x,text", [
+    (600, 8, False, "grid size outside nx 1..512, ny 1..65535"),
+    (40, 70000, False, "grid size outside nx 1..512, ny 1..65535"),
+    (40, 1100, True, "a grid periodic in x takes ny 1..1024"),
+    (40, 1100, False, "the allocation-free solve takes ny 1..1024"),
+])
+def test_grid_limits_are_checked_before_any_device_call(nx, ny, periodic_x, text):
+    """The problem checks run on the host before any HIP call, so they answer without a GPU: nx up
+    to 512 (one row per lane of the 8 waves), ny up to 65535, 1024 where the grid is periodic in x
+    (its wrap line sits in LDS) and in the allocation-free entry (its level table is sized so)."""
+    from ding_nsch_core_solver_amd import engine as E
+    from ding_nsch_core_solver_amd import workloads as W
+    if not E.library_available():
+        pytest.skip("engine not built (python -m ding_nsch_core_solver_amd.build)")
+    L = E.lib()
+    case = W.PressureCase(nx=nx, ny=ny, lx=1.0, ly=1.0, dt=1e-4, periodic_x=periodic_x, periodic_y=False,
+                          tolerance=1e-8, max_iterations=10, density_ratio=1.0, rho=None, u=None, v=None)
+    pb = E.make_problem(case)
+    rep = E.PPReport()
+    rc = L.pp_solve_device_noalloc(ctypes.byref(pb), None, None, None, None, None, None, ctypes.byref(rep), None)
+    assert rc == E.PP_ERR_ARG
+    assert rep.message.decode() == text

@@ -117,6 +117,43 @@ def test_argument_errors(engine):
     case = W.droplet_case(3, 8, seed=0, periodic_x=True)
     with pytest.raises(E.EngineError, match="periodic direction"):
         engine.solve(case, E.PP_ICPCG)
+    case = W.droplet_case(40, 1100, seed=0, periodic_x=True)
+    with pytest.raises(E.EngineError, match="a grid periodic in x takes ny 1..1024"):
+        engine.solve(case, E.PP_ICPCG)
+
+
+TALL = [("32x1100_walls", {"nx": 32, "ny": 1100}), ("70x1030_periodic_y", {"nx": 70, "ny": 1030, "periodic_y": True}),
+        ("200x1500_walls", {"nx": 200, "ny": 1500})]
+
+
+@pytest.mark.parametrize("scheme", ["icpcg", "ildlt_pcg"])
+@pytest.mark.parametrize("name,kw", TALL, ids=[n for n, _ in TALL])
+def test_grids_taller_than_1024(engine, oracle_lib, name, kw, scheme):
+    """ny above 1024, which the engine takes where the grid is not periodic in x: bitwise equal to
+    the oracle's replay of the engine order, the reference's iteration count, and within three
+    times the summation-order spread of the reference-order result."""
+    case = W.droplet_case(seed=3, density_ratio=10.0, **kw)
+    p_gpu, rep = engine.solve(case, CODES[scheme])
+    p_can, it_can, res_can = oracle_lib.pressure_solve(scheme, case, oracle_lib.CANONICAL)
+    p_seq, it_seq, res_seq = oracle_lib.pressure_solve(scheme, case, oracle_lib.SEQUENTIAL)
+    p_pw, _, _ = oracle_lib.pressure_solve(scheme, case, oracle_lib.PAIRWISE)
+    assert rep["iterations"] == it_can and rep["relative_residual"] == res_can
+    assert np.array_equal(p_gpu, p_can)
+    _check_iterations(it_can, res_can, it_seq, res_seq, case.tolerance)
+    assert _rel(p_gpu, p_seq) <= 3.0 * max(_rel(p_pw, p_seq), _rel(p_can, p_seq), 1.0e-15)
+
+
+def test_tall_grid_mixed_precision(engine, oracle_lib):
+    """The mixed-precision solve on a grid taller than 1024 rows meets the tolerance in the FP64
+    residual of the oracle's assembled system (the solution itself may differ from the FP64 one by
+    more than the tolerance: on a 200x1500 domain the smooth modes are poorly conditioned)."""
+    case = W.droplet_case(200, 1500, seed=3, density_ratio=10.0)
+    pm, rep = engine.solve(case, E.PP_ICPCG, precision=E.PP_MIXED)
+    assert rep["relative_residual"] < case.tolerance
+    s = oracle_lib.OracleSystem(case, use_split=False)
+    r = s.rhs - s.apply_matrix(pm)
+    r = r - r.mean()  # Neumann walls: the consistent part
+    assert np.linalg.norm(r) / np.linalg.norm(s.rhs) < 2.0 * case.tolerance
 
 
 @pytest.mark.parametrize("edge", [0, 1, 2, 3])
