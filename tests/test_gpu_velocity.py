@@ -44,6 +44,18 @@ def test_engine_matches_oracle_shapes(engine, oracle_lib, shape):
         _check(got, expect)
 
 
+@pytest.mark.parametrize("shape", [(40, 1300), (2, 3000)])
+def test_engine_matches_oracle_taller_than_1024(engine, oracle_lib, shape):
+    """Grids taller than 1024 rows (taken where the grid is not periodic in x)."""
+    case = W.droplet_case(*shape, seed=1, density_ratio=7.0, periodic_y=True, step_index=1)
+    f = fields(case, seed=shape[0] * 1000 + shape[1])
+    vbc = VELOCITY_BC["neumann_outflow"]
+    for scheme in ("icpcg", "split_icpcg"):
+        expect = oracle_lib.velocity_correction(case, vbc, scheme == "split_icpcg", G, **f)
+        got = engine.velocity_correction(case, vbc, scheme=CODES[scheme], ghost=G, **f)
+        _check(got, expect)
+
+
 def test_ghost_depth_two(engine, oracle_lib):
     case = W.droplet_case(12, 9, seed=2, density_ratio=5.0, periodic_y=True, step_index=2)
     g = 2
