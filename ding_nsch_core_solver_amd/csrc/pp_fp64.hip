@@ -1031,13 +1031,11 @@ __device__ __forceinline__ double win_level(const Stencil<T> &A, const global_pt
     const bool in_j = static_cast<unsigned>(j) < static_cast<unsigned>(ny);
     const bool act = in_i && in_j;
     const bool has_w = i >= 1, has_s = j >= 1, has_n = j <= ny - 2, has_e = i <= nx - 2;
-    T pw = wave_shr1(prev.p[c]);  // p - m_p of the west neighbour
-    if (c > 0) pw = write_lane(pw, lane_value(prev.p[c - 1], 63), 0);
-    T pe = wave_shl1(next.p[c]), ew = wave_shl1(next.w[c]);
-    if (c + 1 < S) {
-      pe = write_lane(pe, lane_value(next.p[c + 1], 0), 63);
-      ew = write_lane(ew, lane_value(next.w[c + 1], 0), 63);
-    }
+    // p - m_p of the west neighbour: lane l-1, and lane 0 takes lane 63 of the previous chunk inside
+    // the DPP shift itself (the lane without a source keeps the fill); likewise east
+    const T pw = shr1_fill(prev.p[c], c > 0 ? lane_value(prev.p[c - 1], 63) : T(0));
+    const T pe = shl1_fill(next.p[c], c + 1 < S ? lane_value(next.p[c + 1], 0) : T(0));
+    const T ew = shl1_fill(next.w[c], c + 1 < S ? lane_value(next.w[c + 1], 0) : T(0));
     const T cw = has_w ? cur.w[c] : T(0), cs = has_s ? cur.s[c] : T(0);
     const T cn = has_n ? next.s[c] : T(0), ce = has_e ? ew : T(0);
     const T pc = cur.p[c];
