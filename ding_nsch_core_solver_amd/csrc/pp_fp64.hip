@@ -1036,7 +1036,11 @@ __device__ __forceinline__ double win_level(const Stencil<T> &A, const global_pt
     const T pw = shr1_fill(prev.p[c], c > 0 ? lane_value(prev.p[c - 1], 63) : T(0));
     const T pe = shl1_fill(next.p[c], c + 1 < S ? lane_value(next.p[c + 1], 0) : T(0));
     const T ew = shl1_fill(next.w[c], c + 1 < S ? lane_value(next.w[c + 1], 0) : T(0));
-    const T cw = has_w ? cur.w[c] : T(0), cs = has_s ? cur.s[c] : T(0);
+    // Without periodicity the assembly stores aW = 0 at i = 0 and aS = 0 at j = 0 (no such entry),
+    // and the values they multiply there are finite (the DPP fill, a clamped real cell), so the
+    // unmasked product is the same signed zero the masked one gives. Periodic rows hold their wrap
+    // coefficient in those slots, so there the masks stay.
+    const T cw = (!kPeriodic || has_w) ? cur.w[c] : T(0), cs = (!kPeriodic || has_s) ? cur.s[c] : T(0);
     const T cn = has_n ? next.s[c] : T(0), ce = has_e ? ew : T(0);
     const T pc = cur.p[c];
     T s = T(0);
