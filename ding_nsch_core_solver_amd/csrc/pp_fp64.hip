@@ -632,6 +632,17 @@ __device__ __forceinline__ void await_chunk(SweepShared &sh, int w, int R, int t
   }
 }
 
+// The chunk's units are staged and (when the wave publishes) the receiving wave has consumed enough
+// of its hand-off ring, read in one LDS round trip. Counters only grow, so a true answer stays true
+// and both waits may be skipped.
+__device__ __forceinline__ bool staged_and_room(SweepShared &sh, int w, int R, int t0, int send_w, int send_need) {
+  const int last1 = t0 + kPrefetch - 1, last0 = t0 + stage_unit(R) - 1;  // one or two units per chunk
+  const int r1 = *lds(&sh.ready[w * kMaxRing + (last1 & (R - 1))]);
+  const int r0 = *lds(&sh.ready[w * kMaxRing + (last0 & (R - 1))]);
+  const int ak = *lds(&sh.ack[send_w < 0 ? 0 : send_w]);
+  return (r1 > last1) & (r0 > last0) & ((send_w < 0) | (ak >= send_need));
+}
+
 template <typename T>
 struct ChunkOps {
   T f[kPrefetch][4];
@@ -663,7 +674,10 @@ __device__ __forceinline__ void staged_forward(const SysArgs &a, const SweepArra
   T prev = T(0), first = T(0);
   for (int t0 = 0; t0 < Tn; t0 += kPrefetch) {
     const int d0 = wave * 64 + t0;
-    await_chunk(sh, wave, R, t0);
+    const int send_last = min(d0 + kPrefetch - 1 - (wave * 64 + 63), ny - 1);
+    const bool send = wave + 1 < S && send_last >= 0;
+    const bool ready = staged_and_room(sh, wave, R, t0, send ? wave + 1 : -1, send_last + 1 - kRing);
+    if (!ready) await_chunk(sh, wave, R, t0);
     ChunkOps<T> c;
     read_chunk<T>(sh, wave, R, t0, lane, c);
     T hk[kPrefetch];  // west inputs of lane 0 (row 64w-1 of wave w-1) at the chunk's levels
@@ -673,7 +687,7 @@ __device__ __forceinline__ void staged_forward(const SysArgs &a, const SweepArra
 #pragma unroll
       for (int k = 0; k < kPrefetch; ++k) hk[k] = T(0);
     }
-    if (wave + 1 < S) reserve_batch(wave + 1, min(d0 + kPrefetch - 1 - (wave * 64 + 63), ny - 1), sh);
+    if (!ready && send) reserve_batch(wave + 1, send_last, sh);
     int base[kPrefetch];  // level bases of the chunk, scalar loads from the constant table
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) base[k] = lev_base[d0 + k];
@@ -733,7 +747,10 @@ __device__ __forceinline__ void staged_backward(const SysArgs &a, const SweepArr
   for (int t0 = 0; t0 < Tn; t0 += kPrefetch) {
     const int d0 = wave * 64 + Tn - 1 - t0;        // levels descend
     const int j63_first = d0 - (wave * 64 + 63);  // column of lane 63 at level d0
-    await_chunk(sh, wave, R, t0);
+    const int send_last = min(ny - 1 - (d0 - kPrefetch + 1 - wave * 64), ny - 1);
+    const bool send = wave > 0 && send_last >= 0;
+    const bool ready = staged_and_room(sh, wave, R, t0, send ? wave - 1 : -1, send_last + 1 - kRing);
+    if (!ready) await_chunk(sh, wave, R, t0);
     ChunkOps<T> c;
     read_chunk<T>(sh, wave, R, t0, lane, c);
     T hk[kPrefetch];  // east inputs of lane 63 (row 64w+64 of wave w+1) at the chunk's levels
@@ -743,7 +760,7 @@ __device__ __forceinline__ void staged_backward(const SysArgs &a, const SweepArr
 #pragma unroll
       for (int k = 0; k < kPrefetch; ++k) hk[k] = T(0);
     }
-    if (wave > 0) reserve_batch(wave - 1, min(ny - 1 - (d0 - kPrefetch + 1 - wave * 64), ny - 1), sh);
+    if (!ready && send) reserve_batch(wave - 1, send_last, sh);
     int base[kPrefetch];  // level bases of the chunk, scalar loads from the constant table
 #pragma unroll
     for (int k = 0; k < kPrefetch; ++k) base[k] = lev_base[d0 - k];
