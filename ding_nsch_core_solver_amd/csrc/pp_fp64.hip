@@ -648,14 +648,21 @@ struct ChunkOps {
   T f[kPrefetch][4];
 };
 
+// Plain loads after a compiler-only fence: they stay behind the flag reads and waits written before
+// them (LDS executes a wave's operations in order), and the compiler may pair them into
+// ds_read2st64_b64 (fields 64 doubles apart).
 template <typename T>
 __device__ __forceinline__ void read_chunk(SweepShared &sh, int w, int R, int t0, int lane, ChunkOps<T> &c) {
+  __asm__ volatile("" ::: "memory");
+  const __attribute__((address_space(3))) T *st =
+      (const __attribute__((address_space(3))) T *)(reinterpret_cast<T *>(&sh.stage[0]));
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k) {
-    volatile __attribute__((address_space(3))) T *p = stage_slot<T>(sh, w, R, (t0 + k) & (R - 1), lane);
+    const __attribute__((address_space(3))) T *p = st + ((w * R + ((t0 + k) & (R - 1))) * 4) * 64 + lane;
 #pragma unroll
     for (int f = 0; f < 4; ++f) c.f[k][f] = p[f * 64];
   }
+  __asm__ volatile("" ::: "memory");
 }
 
 template <typename T, bool kIldlt, bool kPeriodic>
