@@ -158,6 +158,14 @@ def batched_leg(eng, torch, dev, repeats):
     diff = max(float(np.abs(t["out"].cpu().numpy() - q).max() / np.abs(q).max()) for t, q in zip(tens, p64))
     alg = sum(mixed_bytes(c.n, r["iterations"], r["outer_cycles"]) for c, r in zip(cases, reps))
     achieved = alg / (kms * 1e-3) / 1e9
+    # CU occupancy: each system runs on one CU for the sum of its device phase timers, so busy CU
+    # time over (batch kernel time x CUs) is the share of the chip the batch keeps working
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+
+    def occupancy(rs, batch_ms):
+        busy = [sum(r["phase_ms"].values()) for r in rs]
+        return {"cus": cus, "mean_system_ms": float(np.mean(busy)), "max_system_ms": float(np.max(busy)),
+                "busy_fraction": float(sum(busy) / (batch_ms * cus))}
     out = {"systems": len(cases), "cells": cells, "precision": "fp32 inner / fp64 refinement",
            "solves_per_s": len(cases) / elapsed, "ms_per_batch": elapsed * 1e3, "kernel_ms": kms,
            "inner_iterations_total": int(sum(r["iterations"] for r in reps)),
@@ -167,7 +175,9 @@ def batched_leg(eng, torch, dev, repeats):
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(BATCH_TRAFFIC_FILE),
                         "algorithmic_bytes_per_launch": alg},
+           "cu_occupancy": occupancy(reps, kms),
            "fp64": {"solves_per_s": len(cases) / elapsed64, "ms_per_batch": elapsed64 * 1e3, "kernel_ms": kms64,
+                    "cu_occupancy": occupancy(reps64, kms64),
                     "iterations_total": int(sum(r["iterations"] for r in reps64)),
                     "roofline": {"bound": "hbm", "achieved": alg64 / (kms64 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                                  "unit": "GB/s", "frac": alg64 / (kms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -841,7 +851,8 @@ def main():
         log("[bench] 256-system batch")
         line["batched"], batch_cases = batched_leg(eng, torch, dev, args.batched_repeats)
         log(f"[bench] batch: {line['batched']['solves_per_s']:.1f} solves/s, kernel {line['batched']['kernel_ms']:.1f} ms, "
-            f"{line['batched']['roofline']['achieved']:.0f} GB/s algorithmic")
+            f"{line['batched']['roofline']['achieved']:.0f} GB/s algorithmic, CUs busy "
+            f"{line['batched']['cu_occupancy']['busy_fraction'] * 100:.0f}% of the batch")
 
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline (~{args.cpu_seconds:.0f} s on one core)")
