@@ -16,7 +16,9 @@ SYSTEMS = [("momentum_12x10", dict(nx=12, ny=10, seed=1)),
            ("momentum_24x20_dr1000", dict(nx=24, ny=20, seed=2)),
            ("momentum_33x17_visc1000", dict(nx=33, ny=17, seed=3, viscosity_ratio=1000.0)),
            ("momentum_40x40_dt1e-2", dict(nx=40, ny=40, seed=4, dt=1.0e-2)),
-           ("momentum_96x80", dict(nx=96, ny=80, seed=5))]
+           ("momentum_96x80", dict(nx=96, ny=80, seed=5)),
+           # 19 800 unknowns, 39 row chunks: an odd count through the paired u / v chunk map
+           ("momentum_100x100_odd_chunks", dict(nx=100, ny=100, seed=6))]
 
 
 def _check_equal(got_x, rep, expect):
