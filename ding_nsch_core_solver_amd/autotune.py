@@ -1,9 +1,10 @@
-"""Autotuner for the sweep operand ring depth (pp_problem.ring_depth) on the local GPU.
+"""Autotuner for the sweep wave roles (pp_problem.ring_depth) on the local GPU.
 
 For each grid shape the FP64 solve runs a fixed number of PCG iterations (tolerance 0, so every
-candidate does the same work) with every valid ring depth, and the depth with the lowest device
-time wins. Ring depth only changes how far the loader waves run ahead of the sweep waves, never
-the arithmetic, so every candidate returns the same bits (checked here too). The table is written
+candidate does the same work) with every choice of wave roles, and the fastest wins: the staged
+sweeps with each valid ring depth (how far the 8 - S loader waves run ahead of the S sweep waves)
+and the direct sweeps (ring_depth -1: no loaders, every sweep wave loads its own operands). None of
+them changes the arithmetic, so every candidate returns the same bits (checked here too). The table is written
 as JSON (default: tuning_gfx950.json beside this module), which Engine installs at creation.
 
     python -m ding_nsch_core_solver_amd.autotune [--iterations 40] [--repeats 3] [--out PATH]
@@ -25,11 +26,16 @@ DEFAULT_SHAPES = ((64, 64), (96, 192), (128, 128), (192, 192), (256, 256), (192,
 MAX_STAGED_WAVES = 6
 
 
+DIRECT = -1  # pp_problem.ring_depth for direct sweeps: every sweep wave loads its own operands
+
+
 def candidates(nx: int) -> list[int]:
+    """Wave-role choices for a grid nx wide: the staged sweeps with each accepted ring depth (8 - S
+    loader waves), and the direct sweeps (no loaders). Wider grids than 6 waves only run direct."""
     S = (nx + 63) // 64
     if S > MAX_STAGED_WAVES:
         return []
-    return [r for r in (8, 16, 32, 64) if S * r <= 64]
+    return [r for r in (8, 16, 32, 64) if S * r <= 64] + [DIRECT]
 
 
 def tune_shape(eng: E.Engine, nx: int, ny: int, iterations: int, repeats: int, log=print) -> dict:

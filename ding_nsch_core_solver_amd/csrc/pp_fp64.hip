@@ -248,6 +248,15 @@ struct SweepArrays {
   const double *LY;   // periodic-y wrap entries (FP64 factor)
 };
 
+// s - c * x: separately rounded in FP64 (the reference's arithmetic), fused in FP32.
+template <typename T>
+__device__ __forceinline__ T sub_mul(T s, T c, T x) {
+  if constexpr (sizeof(T) == 8)
+    return s - c * x;
+  else
+    return __builtin_fmaf(-c, x, s);
+}
+
 template <typename T>
 __device__ __forceinline__ T pivot_apply(T s, T d, T floor) {
   if constexpr (sizeof(T) == 8)
@@ -321,9 +330,9 @@ __device__ __forceinline__ void forward_solve(const SysArgs &a, const SweepArray
         const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
         s = (wrap_row && act) ? s - static_cast<T>(sh.lx[jc]) * wx : s;
       }
-      s = has_w ? s - o.lw * west : s;
+      s = has_w ? sub_mul(s, o.lw, west) : s;  // the staged sweeps' arithmetic (fused in FP32)
       if (py) s = j == ny - 1 ? s - ly * first : s;
-      s = j > 0 ? s - o.ls * prev : s;
+      s = j > 0 ? sub_mul(s, o.ls, prev) : s;
       T cur, out;
       if (kIldlt) {
         cur = s;
@@ -394,9 +403,9 @@ __device__ __forceinline__ void backward_solve(const SysArgs &a, const SweepArra
         east = lane == 63 ? h : east;
       }
       T s = o.y;
-      s = j + 1 < ny ? s - o.ls_n * prev : s;
+      s = j + 1 < ny ? sub_mul(s, o.ls_n, prev) : s;  // the staged sweeps' arithmetic (fused in FP32)
       if (py) s = j == 0 ? s - ly * first : s;
-      s = has_e ? s - o.lw_e * east : s;
+      s = has_e ? sub_mul(s, o.lw_e, east) : s;
       if (px) {
         const int jc = min(max(j, 0), ny - 1);
         const T wx = static_cast<T>(*lds(&sh.wrap[jc]));
@@ -459,15 +468,6 @@ __device__ __forceinline__ T staged_pivot(T d, T floor) {
     return floor_max(d, floor);
   else
     return d;
-}
-
-// s - c * x: separately rounded in FP64 (the reference's arithmetic), fused in FP32.
-template <typename T>
-__device__ __forceinline__ T sub_mul(T s, T c, T x) {
-  if constexpr (sizeof(T) == 8)
-    return s - c * x;
-  else
-    return __builtin_fmaf(-c, x, s);
 }
 
 template <typename T>
@@ -1166,7 +1166,7 @@ __device__ __forceinline__ void precondition(const SysArgs &a, const SweepArrays
                                              unsigned long long &t) {
   const int S = (a.nx + 63) >> 6;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool staged = S <= kMaxStagedWaves;
+  const bool staged = S <= kMaxStagedWaves && a.ring != kDirectSweeps;
   reset_handoff(sh);
   reset_stage(sh);
   __syncthreads();

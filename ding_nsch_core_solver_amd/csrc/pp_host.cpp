@@ -210,7 +210,8 @@ int validate_problem(pp_ctx *ctx, const pp_problem *pb, pp_report *report) {
       return fail(ctx, report, PP_ERR_ARG, std::string(kEdgeNames[s]) + " cannot be set when the direction is periodic");
   }
   if (pb->ring_depth != 0 && !pp::valid_ring((pb->nx + 63) / 64, pb->ring_depth))
-    return fail(ctx, report, PP_ERR_ARG, "ring_depth must be a power of two >= 8 with ceil(nx/64) * ring_depth <= 64");
+    return fail(ctx, report, PP_ERR_ARG,
+                "ring_depth must be a power of two >= 8 with ceil(nx/64) * ring_depth <= 64, or -1 for direct sweeps");
   return PP_OK;
 }
 

@@ -74,8 +74,13 @@ hipError_t launch_mixed_solve(const SysArgs *d_systems, int count, bool ildlt, h
 // sweep chunk (the sweep releases ring space once per chunk, so a shallower ring would stall its
 // own loaders), and S * ring * 256 doubles within the LDS stage.
 inline int default_ring(int S) { return S <= 4 ? 16 : 8; }
+// Sweep wave roles: a ring depth picks the staged sweeps (8 - S loader waves fill an LDS ring of
+// that many levels per sweep wave); kDirectSweeps picks the direct ones (every sweep wave loads its
+// own operands, no loaders), which wide grids use anyway. Both give the same bits.
+constexpr int kDirectSweeps = -1;
+
 inline bool valid_ring(int S, int ring) {
-  return ring >= 8 && ring <= 64 && (ring & (ring - 1)) == 0 && S * ring <= 64;
+  return ring == kDirectSweeps || (ring >= 8 && ring <= 64 && (ring & (ring - 1)) == 0 && S * ring <= 64);
 }
 
 // Face gradients of the coefficient-mode adjoint: given the solution p and the adjoint solution lam

@@ -62,9 +62,11 @@ typedef struct pp_problem {
   int step_index;         /* split schemes: 0 -> p_ext = p^n, else 2 p^n - p^{n-1} (operators.cpp:413-422) */
   int inner_max_iterations; /* PP_MIXED: cap per inner FP32 solve (<= 0: engine default)         */
   double inner_tolerance;   /* PP_MIXED: inner relative reduction (<= 0: engine default)         */
-  int ring_depth;           /* sweep operand ring depth in levels: 0 = the context's tuning table
-                               (pp_set_tuning) or the built-in default; else a power of two >= 8
-                               with ceil(nx/64) * ring_depth <= 64. Does not change results.     */
+  int ring_depth;           /* sweep wave roles: 0 = the context's tuning table (pp_set_tuning)
+                               or the built-in default; a power of two >= 8 with
+                               ceil(nx/64) * ring_depth <= 64: loader waves stage that many
+                               levels per sweep wave; -1: direct sweeps, no loader waves (grids
+                               wider than 384 always). Does not change results.                  */
 } pp_problem;
 
 typedef struct pp_report {
@@ -307,8 +309,8 @@ int pp_bicgstab(pp_ctx *ctx, int n, const int *row_ptr, const int *col_idx, cons
 
 /*
  * Tuning table of the context (written by the project's autotuner, ding_nsch_core_solver_amd/
- * autotune.py): the sweep operand ring depth to use for grids of exactly nx[k] x ny[k] when
- * pp_problem.ring_depth is 0. Replaces the previous table; entries with an invalid depth are
+ * autotune.py): the sweep wave roles (ring depth, or -1 for direct sweeps) to use for grids of
+ * exactly nx[k] x ny[k] when pp_problem.ring_depth is 0. Replaces the previous table; entries with an invalid depth are
  * rejected. count = 0 clears it.
  */
 int pp_set_tuning(pp_ctx *ctx, int count, const int *nx, const int *ny, const int *ring_depth);

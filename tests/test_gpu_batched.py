@@ -1,4 +1,4 @@
-"""Batched solve (pp_solve_batched_device) and the ring-depth tuning knob.
+"""Batched solve (pp_solve_batched_device) and the sweep wave-role tuning knob.
 
 The batch launches the same solve kernel as pp_solve_device with one workgroup per system, so every
 system must equal its single solve bit for bit (and hence the oracle in the engine's order)."""
@@ -36,13 +36,19 @@ def _batch(engine, cases, schemes, precision=E.PP_FP64):
     return [d["out"].cpu().numpy() for d in t], reps
 
 
-@pytest.mark.parametrize("shape", [(64, 256), (96, 192), (192, 192), (130, 70), (320, 100)])
-def test_ring_depth_does_not_change_results(engine, shape):
-    from ding_nsch_core_solver_amd.autotune import candidates
-    case = W.droplet_case(*shape, seed=1, density_ratio=300.0)
-    ref, rep0 = engine.solve(case, E.PP_ICPCG)
+@pytest.mark.parametrize("precision", [E.PP_FP64, E.PP_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("shape,periodic", [((64, 256), False), ((96, 192), False), ((192, 192), False),
+                                            ((130, 70), False), ((320, 100), False), ((130, 70), True)],
+                         ids=["64x256", "96x192", "192x192", "130x70", "320x100", "130x70_periodic"])
+def test_wave_roles_do_not_change_results(engine, shape, periodic, precision):
+    """Every choice the autotuner weighs (each ring depth of the staged sweeps, and the direct sweeps
+    with no loader waves) returns the same bits, in FP64 and in the FP32 inner sweeps."""
+    from ding_nsch_core_solver_amd.autotune import DIRECT, candidates
+    case = W.droplet_case(*shape, seed=1, density_ratio=300.0, periodic_x=periodic, periodic_y=periodic)
+    ref, rep0 = engine.solve(case, E.PP_ICPCG, precision=precision)
+    assert DIRECT in candidates(shape[0])
     for ring in candidates(shape[0]):
-        p, rep = engine.solve(case, E.PP_ICPCG, ring_depth=ring)
+        p, rep = engine.solve(case, E.PP_ICPCG, precision=precision, ring_depth=ring)
         assert rep["iterations"] == rep0["iterations"]
         assert np.array_equal(p, ref), ring
 

@@ -166,8 +166,9 @@ def loader(me, S, ny, R, backward):
 
 
 def sweep_waves(nx, ny, backward, R=None, factor=False):
+    """R = -1 (pp::kDirectSweeps) runs the direct sweeps at any width, as wider grids always do."""
     S = (nx + 63) // 64
-    if factor or S > MAX_STAGED:
+    if factor or S > MAX_STAGED or R == -1:
         return [direct_sweep(w, S, nx, ny, backward, factor) for w in range(S)]
     waves = [staged_sweep(w, S, ny, R, backward) for w in range(S)]
     waves += [loader(me, S, ny, R, backward) for me in range(WAVES - S)]
@@ -195,7 +196,7 @@ SHAPES = sorted({
 def test_sweeps_finish_for_every_accepted_ring(nx, ny):
     S = (nx + 63) // 64
     run(sweep_waves(nx, ny, backward=False, factor=True))
-    rings = fp64_rings(S) + [fp32_ring(S)] if S <= MAX_STAGED else [None]
+    rings = fp64_rings(S) + [fp32_ring(S), -1] if S <= MAX_STAGED else [None]
     for R in rings:
         for backward in (False, True):
             run(sweep_waves(nx, ny, backward, R))
