@@ -12,7 +12,8 @@
 // max(D, floor) after the whole forward sweep, the backward sum over the children in ascending row
 // order. For the 13-point stencil in natural order the levels are the anti-diagonals d = 2i + j
 // (2 nx + ny - 2 of them); a periodic y direction chains (i, 0) to (i - 1, ny - 1) and the levels
-// become the rows themselves.
+// become the rows themselves. Such chains are factored and swept by batches of rows in one wavefront
+// instead (ch_factor_seq_kernel, ch_precondition_seq).
 //
 // Reductions (dot products, norms) of this one-workgroup PCG use the 512-thread canonical tree
 // over the natural row order: thread t sums rows t, t + 512, ..., then the 64-lane tree (offsets
@@ -143,6 +144,142 @@ __global__ void __launch_bounds__(kThreads) ch_factor_kernel(ChArgs a) {
   if (tid == 0) a.state->ildlt_ok = sh.flag ? 0 : 1;
 }
 
+using lds_double = __attribute__((address_space(3))) double;
+
+// ILDLT(0) by batches of B rows, for factors whose level schedule is a chain (the batch structure of
+// the sequential sweeps below: a row's lower entries inside its batch are L(r, r-2) and L(r, r-1)
+// only). Wave 0 does the work, lane k holding row b0 + k of the batch; D of the finished rows is kept
+// in LDS (Dl, n doubles). Per batch:
+//  1. every lane factors its row's entries before the batch, in CSR order: their cross terms
+//     L(c, k) (c and k before the batch, final; positions fx_pos) and D are final, and the row's
+//     own earlier entries are its own work (kept in the lane's LDS record Lr);
+//  2. the numerators of the in-batch entries, over the row's entries before the batch: L(r-2, k) and
+//     L(r-1, k) are entries of rows r-2, r-1 before the batch, found in their lanes' records (fab);
+//  3. the rows one after the other, every lane holding the running values: L(r, r-2), then
+//     L(r, r-1) with its last cross term (L(r, r-2) D(r-2)) L(r-1, r-2), then D(r), whose
+//     subtractions end with the two in-batch entries (the far ones were subtracted in step 1).
+// Every operation is the reference's (ch_sparse_krylov.hpp:278-320), in its order; the skip of a zero
+// L(c, k) included.
+__global__ void __launch_bounds__(kThreads) ch_factor_seq_kernel(ChArgs a) {
+  __shared__ ChShared sh;
+  extern __shared__ double ch_fac_lds[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int W = kChSeqWidth;
+  lds_double *Dl = (lds_double *)ch_fac_lds;
+  lds_double *Lr = Dl + a.n;       // the batch's entries before the batch: lane k at k * W
+  lds_double *rec = Lr + 64 * W;   // chain records: numerator of L(r, r-2), of L(r, r-1), D so far, flags
+  double m = 0.0;
+  for (int r = tid; r < a.n; r += kThreads) {
+    const double d = a.val[a.diag_pos[r]];
+    const double ad = fabs(d);
+    a.dinv[r] = 1.0 / (ad < 1.0e-30 ? 1.0e-30 : ad);
+    m = m < ad ? ad : m;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_down(m, off, 64);
+    m = m < o ? o : m;
+  }
+  if (tid == 0) sh.flag = 0;
+  if ((tid & 63) == 0) sh.red[0][tid >> 6] = m;
+  __syncthreads();
+  double max_diag = sh.red[0][0];
+  for (int w = 1; w < kWaves; ++w) max_diag = max_diag < sh.red[0][w] ? sh.red[0][w] : max_diag;
+  const double floor = (1.0 < max_diag ? max_diag : 1.0) * 1.0e-12;
+  if (tid == 0) a.state->floor = floor;
+  if (wave == 0) {
+    const int n = a.n, B = a.seq_batch;
+    bool failed = false;
+    for (int b0 = 0; b0 < n; b0 += B) {
+      const int cnt = min(B, n - b0);
+      const bool own = lane < cnt;
+      const int r = own ? b0 + lane : 0;
+      const size_t base = static_cast<size_t>(r) * W;
+      const int fc = own ? a.fo_cnt[r] : 0;
+      lds_double *mine = Lr + lane * W;
+      // 1. entries before the batch, and D's subtractions for them
+      double d = a.val[a.diag_pos[r]];
+      for (int e = 0; e < fc; ++e) {
+        const int p = a.fo_pos[base + e];
+        double s = a.val[p];
+        for (int e2 = 0; e2 < e; ++e2) {
+          const int pos = a.fx_pos[(base + e) * W + e2];
+          const double l_ck = pos >= 0 ? a.lower[pos] : 0.0;
+          if (l_ck != 0.0) s -= mine[e2] * Dl[a.fo_col[base + e2]] * l_ck;
+        }
+        const double dc = Dl[a.fo_col[base + e]];
+        const double l = s / (dc < floor ? floor : dc);
+        mine[e] = l;
+        a.lower[p] = l;
+        d -= (l * l) * dc;
+      }
+      __asm__ volatile("" ::: "memory");  // the lanes' records are complete (LDS runs a wave's accesses in order)
+      // 2. numerators of L(r, r-2) and L(r, r-1) over the entries before the batch
+      const int p2 = own ? a.fin_pos[2 * r] : -1, p1 = own ? a.fin_pos[2 * r + 1] : -1;
+      const lds_double *row2 = Lr + (lane >= 2 ? lane - 2 : 0) * W, *row1 = Lr + (lane >= 1 ? lane - 1 : 0) * W;
+      double s2 = p2 >= 0 ? a.val[p2] : 0.0, s1 = p1 >= 0 ? a.val[p1] : 0.0;
+      for (int e2 = 0; e2 < fc; ++e2) {
+        const int code = a.fab[base + e2];
+        const int i2 = (code & 0xff) - 1, i1 = (code >> 8) - 1;
+        const double t = mine[e2] * Dl[a.fo_col[base + e2]];
+        if (p2 >= 0 && i2 >= 0) {
+          const double l_ck = row2[i2];
+          if (l_ck != 0.0) s2 -= t * l_ck;
+        }
+        if (p1 >= 0 && i1 >= 0) {
+          const double l_ck = row1[i1];
+          if (l_ck != 0.0) s1 -= t * l_ck;
+        }
+      }
+      if (own) {
+        lds_double *q = rec + lane * 4;
+        q[0] = s2;
+        q[1] = s1;
+        q[2] = d;
+        q[3] = static_cast<double>((p2 >= 0 ? 2 : 0) | (p1 >= 0 ? 1 : 0));
+      }
+      __asm__ volatile("" ::: "memory");
+      // 3. the rows of the batch in order
+      double dm1 = 0.0, dm2 = 0.0, lm = 0.0;  // D(r-1), D(r-2), L(r-1, r-2) of the running row
+      double my_l2 = 0.0, my_l1 = 0.0, my_d = 0.0;
+      for (int k = 0; k < cnt; ++k) {
+        const lds_double *q = rec + k * 4;
+        const double n2 = q[0], n1 = q[1], dk = q[2];
+        const int fk = static_cast<int>(q[3]);
+        double l2 = 0.0, l1 = 0.0, t = n1, dd = dk;
+        if (fk & 2) {
+          l2 = n2 / (dm2 < floor ? floor : dm2);
+          if (lm != 0.0) t -= l2 * dm2 * lm;
+        }
+        if (fk & 1) l1 = t / (dm1 < floor ? floor : dm1);
+        if (fk & 2) dd -= (l2 * l2) * dm2;
+        if (fk & 1) dd -= (l1 * l1) * dm1;
+        dm2 = dm1;
+        dm1 = dd;
+        lm = l1;
+        if (lane == k) {
+          my_l2 = l2;
+          my_l1 = l1;
+          my_d = dd;
+        }
+      }
+      if (own) {
+        if (p2 >= 0) a.lower[p2] = my_l2;
+        if (p1 >= 0) a.lower[p1] = my_l1;
+        a.D[b0 + lane] = my_d;
+        Dl[b0 + lane] = my_d;
+        failed = failed || !isfinite(my_d) || my_d <= floor;
+      }
+      // the batch's factor entries are read from memory by later batches' lanes
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    if (failed) sh.flag = 1;
+  }
+  __syncthreads();
+  if (tid == 0) a.state->ildlt_ok = sh.flag ? 0 : 1;
+}
+
 // z = M^{-1} r with the ILDLT factor (ch_sparse_krylov.hpp:486-512) or the diagonal (446-452).
 __device__ void ch_precondition(const ChArgs &a, bool ildlt, double floor) {
   const int tid = threadIdx.x;
@@ -188,8 +325,8 @@ __device__ void ch_precondition(const ChArgs &a, bool ildlt, double floor) {
 // lane holding the running values. Backward, batches and rows descending: each row subtracts the
 // in-batch children r+1, r+2 first (they lead the ascending child list), then the products of its
 // children outside the batch, formed beforehand by lane k in list order. Every operation is the
-// reference's, in its order; only who computes it changes. A batch's operands are packed per row
-// and loaded while the previous batch runs, so only the LDS reads of y / z wait on the batch before.
+// reference's, in its order; only who computes it changes. A batch's operands are packed per row,
+// so its loads do not depend on the sweep's values; they are issued at the end of the batch before.
 struct SeqFwd {
   double r, l2, l1;
   int cnt, flags;
@@ -238,8 +375,6 @@ __device__ __forceinline__ void load_bwd(const ChArgs &a, int r, bool own, SeqBw
 // moves, and the reads run ahead of the dependency chain).
 constexpr int kRecF = 4;                  // forward: s, L(r, r-2), L(r, r-1), flags
 constexpr int kRecB = 4 + kChSeqWidth;    // backward: z, L(r+1, r), L(r+2, r), flags, far products
-
-using lds_double = __attribute__((address_space(3))) double;
 
 // The backward batch's chain: row k of the batch (from the top) subtracts its in-batch children
 // (flags bit 0: r+1, bit 1: r+2) and then its NF far products (+0.0 past its own list). Returns row
@@ -693,7 +828,12 @@ __global__ void __launch_bounds__(kRows) ch_diag_sum_kernel(ChDiagArgs a, int fi
 }  // namespace
 
 hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream) {
-  hipLaunchKernelGGL(ch_factor_kernel, dim3(1), dim3(kThreads), 0, stream, a);
+  if (a.seq_batch > 0) {
+    const size_t lds = (static_cast<size_t>(a.n) + 64 * (kChSeqWidth + 4)) * sizeof(double);
+    hipLaunchKernelGGL(ch_factor_seq_kernel, dim3(1), dim3(kThreads), lds, stream, a);
+  } else {
+    hipLaunchKernelGGL(ch_factor_kernel, dim3(1), dim3(kThreads), 0, stream, a);
+  }
   return hipGetLastError();
 }
 

@@ -1885,6 +1885,39 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
   }
   if (!seq_batch)
     for (auto *v : {&fo_cnt, &fo_col, &fo_pos, &fin_pos, &bf_cnt, &bf_row, &bf_pos, &bin_pos}) v->clear();
+  // The batched factorization's cross-term positions: L(c, k) for each pair of a row's entries before
+  // its batch (find_lower_entry of the factor loop), and where L(r-2, k), L(r-1, k) sit among the
+  // entries of rows r-2, r-1 before the same batch.
+  std::vector<int> fx_pos, fab;
+  if (seq_batch) {
+    const size_t nn = static_cast<size_t>(n);
+    fx_pos.assign(nn * W * W, -1);
+    fab.assign(nn * W, 0);
+    auto lower_pos = [&](int row, int target) {
+      for (int p = A.rp[row]; p < A.rp[row + 1] && A.ci[p] < row && A.ci[p] <= target; ++p)
+        if (A.ci[p] == target) return p;
+      return -1;
+    };
+    auto far_index = [&](int row, int k) {
+      for (int e = 0; e < fo_cnt[static_cast<size_t>(row)]; ++e)
+        if (fo_col[static_cast<size_t>(row) * W + static_cast<size_t>(e)] == k) return e;
+      return -1;
+    };
+    for (int r = 0; r < n; ++r) {
+      const size_t base = static_cast<size_t>(r) * W;
+      const int fc = fo_cnt[static_cast<size_t>(r)];
+      for (int e = 0; e < fc; ++e)
+        for (int e2 = 0; e2 < e; ++e2)
+          fx_pos[(base + static_cast<size_t>(e)) * W + static_cast<size_t>(e2)] =
+              lower_pos(fo_col[base + static_cast<size_t>(e)], fo_col[base + static_cast<size_t>(e2)]);
+      for (int e2 = 0; e2 < fc; ++e2) {
+        const int k = fo_col[base + static_cast<size_t>(e2)];
+        const int i2 = fin_pos[2 * static_cast<size_t>(r)] >= 0 ? far_index(r - 2, k) : -1;
+        const int i1 = fin_pos[2 * static_cast<size_t>(r) + 1] >= 0 ? far_index(r - 1, k) : -1;
+        fab[base + static_cast<size_t>(e2)] = (i2 + 1) | ((i1 + 1) << 8);
+      }
+    }
+  }
   // device buffers
   if (C.d_int) (void)hipFree(C.d_int);
   if (C.d_dbl) (void)hipFree(C.d_dbl);
@@ -1894,7 +1927,7 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
   C.d_part = nullptr;
   const size_t nl = lt_row.size();
   const size_t seq_ints = fo_cnt.size() + fo_col.size() + fo_pos.size() + fin_pos.size() + bf_cnt.size() +
-                          bf_row.size() + bf_pos.size() + bin_pos.size();
+                          bf_row.size() + bf_pos.size() + bin_pos.size() + fx_pos.size() + fab.size();
   const size_t ints = (n + 1) + nnz + n + (nlev + 1) + n + (n + 1) + 2 * nl + seq_ints;
   const size_t seq_dbls = seq_batch ? (2 * W + 4) * static_cast<size_t>(n) : 0;  // fo_val, bf_val, fin_val, bin_val
   const size_t dbls = 2 * static_cast<size_t>(nnz) + 7 * static_cast<size_t>(n) + seq_dbls;
@@ -1938,6 +1971,8 @@ int ensure_ch(pp_ctx *ctx, const pp_ch_problem *cp, pp_ch_report *report, hipStr
     if (int rc = put_int(bf_row, a.bf_row)) return rc;
     if (int rc = put_int(bf_pos, a.bf_pos)) return rc;
     if (int rc = put_int(bin_pos, a.bin_pos)) return rc;
+    if (int rc = put_int(fx_pos, a.fx_pos)) return rc;
+    if (int rc = put_int(fab, a.fab)) return rc;
   }
   PP_CH_CHECK(hipMemcpyAsync(dp, A.v.data(), A.v.size() * sizeof(double), hipMemcpyHostToDevice, s));
   a.val = dp;

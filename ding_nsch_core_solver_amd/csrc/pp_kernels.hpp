@@ -204,8 +204,12 @@ struct ChArgs {
   // bin_pos[2r], bin_pos[2r+1] of L(r+1, r), L(r+2, r)), which lead the ascending child list, then
   // bf_cnt[r] children outside the batch (rows bf_row, positions bf_pos). The values are gathered
   // from the factor once per operator (launch_ch_pack) into fo_val, fin_val, bf_val, bin_val.
+  // The batched factorization also takes, for entry e of row r before the batch and each earlier
+  // entry e2 of the row, the CSR position of L(fo_col[e], fo_col[e2]) (fx_pos[(r W + e) W + e2],
+  // -1 when absent), and for entry e2 the index + 1 of its column among the entries of rows r-2 and
+  // r-1 before the batch (fab[r W + e2], bits 0-7 and 8-15, 0 when absent).
   int seq_batch;
-  const int *fo_cnt, *fo_col, *fo_pos, *fin_pos, *bf_cnt, *bf_row, *bf_pos, *bin_pos;
+  const int *fo_cnt, *fo_col, *fo_pos, *fin_pos, *bf_cnt, *bf_row, *bf_pos, *bin_pos, *fx_pos, *fab;
   double *fo_val, *fin_val, *bf_val, *bin_val;
 };
 constexpr int kChSeqWidth = 12;       // entries of a row outside its batch (the 13-point stencil has at most 12)

@@ -16,4 +16,10 @@ def small_ch_cases():
         ("16x16_ildlt_fails_diagonal", W.ch_case(16, 16, seed=8, cn=0.05, pe=1.0, dt=1e-2)),
         ("18x14_ildlt_fails_periodic_x", W.ch_case(18, 14, seed=9, cn=0.3, pe=100.0, dt=1e3, periodic_x=True)),
         ("33x70_rectangular", W.ch_case(33, 70, seed=10, lx=1.0, ly=2.0, dt=1e-3, pe=10.0)),
+        # periodic y chains the level schedule, so these take the batched factorization and sweeps
+        ("37x29_periodic_y_odd_batches", W.ch_case(37, 29, seed=12, periodic_y=True)),
+        ("24x20_periodic_xy_many_iterations", W.ch_case(24, 20, seed=16, dt=1.0, pe=1.0, periodic_x=True,
+                                                        periodic_y=True)),
+        ("16x16_ildlt_fails_periodic_y", W.ch_case(16, 16, seed=8, cn=0.05, pe=1.0, dt=1e-2, periodic_y=True)),
+        ("96x72_periodic_y_largest_batched", W.ch_case(96, 72, seed=15, periodic_y=True)),
     ]
