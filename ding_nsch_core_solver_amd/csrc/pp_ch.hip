@@ -326,7 +326,8 @@ __device__ void ch_precondition(const ChArgs &a, bool ildlt, double floor) {
 // in-batch children r+1, r+2 first (they lead the ascending child list), then the products of its
 // children outside the batch, formed beforehand by lane k in list order. Every operation is the
 // reference's, in its order; only who computes it changes. A batch's operands are packed per row,
-// so its loads do not depend on the sweep's values; they are issued at the end of the batch before.
+// so its loads do not depend on the sweep's values: wave 1 stages them in LDS during the batch
+// before.
 struct SeqFwd {
   double r, l2, l1;
   int cnt, flags;
@@ -372,93 +373,239 @@ __device__ __forceinline__ void load_bwd(const ChArgs &a, int r, bool own, SeqBw
 
 // Row records of a batch in LDS for the sequential part: lane k writes its row's operands, then
 // every lane reads row k's record with one broadcast read per pair of doubles (no lane-to-lane
-// moves, and the reads run ahead of the dependency chain).
-constexpr int kRecF = 4;                  // forward: s, L(r, r-2), L(r, r-1), flags
-constexpr int kRecB = 4 + kChSeqWidth;    // backward: z, L(r+1, r), L(r+2, r), flags, far products
+// moves, and the reads run ahead of the dependency chain). Which in-batch entries a row has is a
+// bit per row in a wave-uniform mask (scalar registers), and lane 0 stores each row's value into
+// the LDS vector as the chain produces it; the other lanes store into their own slot of a sink, so the
+// store needs no branch and the loop stays one block in which the record reads run ahead.
+constexpr int kRecF = 4;                  // forward: s, L(r, r-2), L(r, r-1), unused
+constexpr int kRecB = 4 + kChSeqWidth;    // backward: z, L(r+1, r), L(r+2, r), unused, far products
 
-// The backward batch's chain: row k of the batch (from the top) subtracts its in-batch children
-// (flags bit 0: r+1, bit 1: r+2) and then its NF far products (+0.0 past its own list). Returns row
-// k's z in lane k.
+// The chains: one wave steps through the batch's rows, every lane computing each row. A wave issues
+// at most one instruction every four cycles, so the chains are cut to few instructions per row:
+// groups of rows whose in-batch entries are all present take a path without flag tests or selects
+// (most rows of a 13-point operator), the others test their bits (has1, has2: bit k for row k of
+// the batch) and select. The results are stored by lane 0; the other lanes store into their own sink
+// slots (G per lane), which keeps the stores free of branches.
+
+// The backward batch's chain: row k of the batch (from the top, row top - k) subtracts its in-batch
+// children (bit k of has1: r+1, of has2: r+2) and then its NF far products (+0.0 past its own list),
+// and lane 0 stores its z at ys[top - k].
 template <int NF>
-__device__ __forceinline__ double bwd_chain(const lds_double *rec, int cnt, int lane) {
-  double z1 = 0.0, z2 = 0.0, mine = 0.0;
-#pragma unroll 2
-  for (int k = 0; k < cnt; ++k) {
+__device__ __forceinline__ void bwd_chain(const lds_double *__restrict__ rec, int cnt, int lane,
+                                          unsigned long long has1, unsigned long long has2,
+                                          lds_double *__restrict__ ys, int top, lds_double *__restrict__ sink) {
+  constexpr int G = 2;
+  const unsigned long long both = has1 & has2;
+  double z1 = 0.0, z2 = 0.0;
+  int k = 0;
+  for (; k + G <= cnt; k += G) {
+    double v[G][3 + NF];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const lds_double *q = rec + (k + j) * kRecB;
+      v[j][0] = q[0];
+      v[j][1] = q[1];
+      v[j][2] = q[2];
+#pragma unroll
+      for (int e = 0; e < NF; ++e) v[j][3 + e] = q[4 + e];
+    }
+    // the group's rows top-k .. top-k-G+1 from one base: lane 0 at the lowest row, the others in
+    // their G sink slots
+    lds_double *base = lane == 0 ? ys + (top - k - (G - 1)) : sink + G * lane;
+    if (((both >> k) & 3) == 3) {
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        double t = v[j][0] - v[j][1] * z1;
+        t = t - v[j][2] * z2;
+#pragma unroll
+        for (int e = 0; e < NF; ++e) t -= v[j][3 + e];
+        z2 = z1;
+        z1 = t;
+        base[G - 1 - j] = t;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        double t = v[j][0];
+        const double t1 = t - v[j][1] * z1;
+        t = (has1 >> (k + j)) & 1 ? t1 : t;
+        const double t2 = t - v[j][2] * z2;
+        t = (has2 >> (k + j)) & 1 ? t2 : t;
+#pragma unroll
+        for (int e = 0; e < NF; ++e) t -= v[j][3 + e];
+        z2 = z1;
+        z1 = t;
+        base[G - 1 - j] = t;
+      }
+    }
+  }
+  for (; k < cnt; ++k) {
     const lds_double *q = rec + k * kRecB;
-    double f[NF > 0 ? NF : 1];
-#pragma unroll
-    for (int e = 0; e < NF; ++e) f[e] = q[4 + e];
     double t = q[0];
-    const double c1 = q[1], c2 = q[2];
-    const int fk = static_cast<int>(q[3]);
-    const double t1 = t - c1 * z1;
-    t = (fk & 1) ? t1 : t;
-    const double t2 = t - c2 * z2;
-    t = (fk & 2) ? t2 : t;
+    const double t1 = t - q[1] * z1;
+    t = (has1 >> k) & 1 ? t1 : t;
+    const double t2 = t - q[2] * z2;
+    t = (has2 >> k) & 1 ? t2 : t;
 #pragma unroll
-    for (int e = 0; e < NF; ++e) t -= f[e];
+    for (int e = 0; e < NF; ++e) t -= q[4 + e];
     z2 = z1;
     z1 = t;
-    mine = lane == k ? t : mine;
+    *(lane == 0 ? ys + (top - k) : sink + lane) = t;
   }
-  return mine;
+}
+
+// The forward batch's chain: row k of the batch subtracts its in-batch entries L(r, r-2) y(r-2)
+// (bit k of has2) and L(r, r-1) y(r-1) (bit k of has1), and lane 0 stores its y at out[k].
+__device__ __forceinline__ void fwd_chain(const lds_double *__restrict__ rec, int cnt, int lane,
+                                          unsigned long long has1, unsigned long long has2,
+                                          lds_double *__restrict__ out, lds_double *__restrict__ sink) {
+  constexpr int G = 4;
+  const unsigned long long both = has1 & has2;
+  double y1 = 0.0, y2 = 0.0;
+  int k = 0;
+  for (; k + G <= cnt; k += G) {
+    double v[G][3];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const lds_double *q = rec + (k + j) * kRecF;
+      v[j][0] = q[0];
+      v[j][1] = q[1];
+      v[j][2] = q[2];
+    }
+    lds_double *base = lane == 0 ? out + k : sink + G * lane;  // lane 0: the group's rows, the others their sink slots
+    if (((both >> k) & 15) == 15) {
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        double t = v[j][0] - v[j][1] * y2;
+        t = t - v[j][2] * y1;
+        y2 = y1;
+        y1 = t;
+        base[j] = t;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        double t = v[j][0];
+        const double t2 = t - v[j][1] * y2;
+        t = (has2 >> (k + j)) & 1 ? t2 : t;
+        const double t1 = t - v[j][2] * y1;
+        t = (has1 >> (k + j)) & 1 ? t1 : t;
+        y2 = y1;
+        y1 = t;
+        base[j] = t;
+      }
+    }
+  }
+  for (; k < cnt; ++k) {
+    const lds_double *q = rec + k * kRecF;
+    double t = q[0];
+    const double t2 = t - q[1] * y2;
+    t = (has2 >> k) & 1 ? t2 : t;
+    const double t1 = t - q[2] * y1;
+    t = (has1 >> k) & 1 ? t1 : t;
+    y2 = y1;
+    y1 = t;
+    *(lane == 0 ? out + k : sink + lane) = t;
+  }
+}
+
+// A batch's operands are staged in LDS by wave 1 while wave 0 sweeps the batch before (two buffers,
+// one workgroup barrier per batch), so wave 0 reads them at LDS latency. One buffer: per row the
+// doubles (r or L(r+1, r), L(r, r-2) or L(r+2, r), L(r, r-1), the 12 packed values) then the ints
+// (count, flags, the 12 columns or rows), field-major so each lane's reads and writes are stride 1.
+using lds_int = __attribute__((address_space(3))) int;
+constexpr int kStageDbl = 3 + kChSeqWidth, kStageInt = 2 + kChSeqWidth;
+constexpr int kChStageDoubles = 64 * kStageDbl + 64 * kStageInt / 2;
+
+__device__ __forceinline__ void put_stage(lds_double *buf, int lane, double d0, double d1, double d2,
+                                          const double (&val)[kChSeqWidth], int cnt, int flags,
+                                          const int (&idx)[kChSeqWidth]) {
+  lds_int *bi = (lds_int *)(buf + 64 * kStageDbl);
+  buf[lane] = d0;
+  buf[64 + lane] = d1;
+  buf[128 + lane] = d2;
+#pragma unroll
+  for (int e = 0; e < kChSeqWidth; ++e) buf[(3 + e) * 64 + lane] = val[e];
+  bi[lane] = cnt;
+  bi[64 + lane] = flags;
+#pragma unroll
+  for (int e = 0; e < kChSeqWidth; ++e) bi[(2 + e) * 64 + lane] = idx[e];
+}
+
+__device__ __forceinline__ void get_stage(const lds_double *buf, int lane, double &d0, double &d1, double &d2,
+                                          double (&val)[kChSeqWidth], int &cnt, int &flags,
+                                          int (&idx)[kChSeqWidth]) {
+  const lds_int *bi = (const lds_int *)(buf + 64 * kStageDbl);
+  d0 = buf[lane];
+  d1 = buf[64 + lane];
+  d2 = buf[128 + lane];
+#pragma unroll
+  for (int e = 0; e < kChSeqWidth; ++e) val[e] = buf[(3 + e) * 64 + lane];
+  cnt = bi[lane];
+  flags = bi[64 + lane];
+#pragma unroll
+  for (int e = 0; e < kChSeqWidth; ++e) idx[e] = bi[(2 + e) * 64 + lane];
+}
+
+__device__ __forceinline__ void stage_fwd(const ChArgs &a, int r, bool own, lds_double *buf, int lane) {
+  SeqFwd o;
+  load_fwd(a, r, own, o);
+  put_stage(buf, lane, o.r, o.l2, o.l1, o.val, o.cnt, o.flags, o.col);
+}
+
+__device__ __forceinline__ void stage_bwd(const ChArgs &a, int r, bool own, lds_double *buf, int lane) {
+  SeqBwd o;
+  load_bwd(a, r, own, o);
+  put_stage(buf, lane, o.c1, o.c2, 0.0, o.val, o.cnt, o.flags, o.row);
 }
 
 __device__ void ch_precondition_seq(const ChArgs &a, double floor, lds_double *ys, lds_double *rec) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int n = a.n, B = a.seq_batch;
-  if (wave == 0) {
-    SeqFwd cur, nxt;
-    load_fwd(a, lane, lane < min(B, n), cur);
-    for (int b0 = 0; b0 < n; b0 += B) {
+  lds_double *sink = rec + 64 * kRecB;  // 4 slots per lane after the records
+  lds_double *stage = sink + 4 * 64;    // two staging buffers
+  const int n = a.n, B = a.seq_batch, nb = (n + B - 1) / B;
+  if (wave == 1) stage_fwd(a, lane, lane < min(B, n), stage, lane);
+  __syncthreads();
+  for (int bi = 0; bi < nb; ++bi) {  // the same barriers in every wave
+    const int b0 = bi * B;
+    if (wave == 0) {
+      SeqFwd cur;
+      get_stage(stage + (bi & 1) * kChStageDoubles, lane, cur.r, cur.l2, cur.l1, cur.val, cur.cnt, cur.flags, cur.col);
       const int cnt = min(B, n - b0);
-      const int r = b0 + lane;
       const bool own = lane < cnt;
-      double s = cur.r;
+      double s = cur.r, g[kChSeqWidth];
 #pragma unroll
-      for (int e = 0; e < kChSeqWidth; ++e)
-        if (e < cur.cnt) s -= cur.val[e] * ys[cur.col[e]];
+      for (int e = 0; e < kChSeqWidth; ++e) g[e] = ys[cur.col[e]];  // column 0 past the row's list
+#pragma unroll
+      for (int e = 0; e < kChSeqWidth; ++e) s = e < cur.cnt ? s - cur.val[e] * g[e] : s;
+      const unsigned long long has2 = __ballot(own && (cur.flags & 2)), has1 = __ballot(own && (cur.flags & 1));
       if (own) {
         lds_double *q = rec + lane * kRecF;
         q[0] = s;
         q[1] = cur.l2;
         q[2] = cur.l1;
-        q[3] = static_cast<double>(cur.flags);
       }
-      double y1 = 0.0, y2 = 0.0, mine = 0.0;
-#pragma unroll 4
-      for (int k = 0; k < cnt; ++k) {
-        const lds_double *q = rec + k * kRecF;
-        double t = q[0];
-        const double l2 = q[1], l1 = q[2];
-        const int fk = static_cast<int>(q[3]);
-        const double t2 = t - l2 * y2;
-        t = (fk & 2) ? t2 : t;
-        const double t1 = t - l1 * y1;
-        t = (fk & 1) ? t1 : t;
-        y2 = y1;
-        y1 = t;
-        mine = lane == k ? t : mine;
-      }
-      if (b0 + B < n) load_fwd(a, b0 + B + lane, lane < min(B, n - b0 - B), nxt);
-      if (own) ys[r] = mine;
-      cur = nxt;
+      fwd_chain(rec, cnt, lane, has1, has2, ys + b0, sink);
+    } else if (wave == 1 && bi + 1 < nb) {
+      stage_fwd(a, b0 + B + lane, lane < min(B, n - b0 - B), stage + ((bi + 1) & 1) * kChStageDoubles, lane);
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int r = tid; r < n; r += kThreads) {
     const double d = a.D[r];
     ys[r] = ys[r] / (d < floor ? floor : d);
   }
+  if (wave == 1) {
+    const int top = (nb - 1) * B;
+    stage_bwd(a, n - 1 - lane, lane < n - top, stage, lane);
+  }
   __syncthreads();
-  if (wave == 0) {
-    const int top = ((n - 1) / B) * B;
-    SeqBwd cur, nxt;
-    {
-      const int hi = n - 1;
-      load_bwd(a, hi - lane, lane < hi - top + 1, cur);
-    }
-    for (int b0 = top; b0 >= 0; b0 -= B) {
+  for (int t = 0; t < nb; ++t) {  // batches from the top
+    const int b0 = (nb - 1 - t) * B;
+    if (wave == 0) {
+      SeqBwd cur;
+      double unused;
+      get_stage(stage + (t & 1) * kChStageDoubles, lane, cur.c1, cur.c2, unused, cur.val, cur.cnt, cur.flags, cur.row);
       const int hi = min(b0 + B, n) - 1, cnt = hi - b0 + 1;
       const int r = hi - lane;  // lane k: the k-th row of the batch from the top
       const bool own = lane < cnt;
@@ -466,39 +613,40 @@ __device__ void ch_precondition_seq(const ChArgs &a, double floor, lds_double *y
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) nfar = max(nfar, __shfl_xor(nfar, off, 64));
       nfar = __builtin_amdgcn_readfirstlane(nfar);
+      const unsigned long long has1 = __ballot(own && (cur.flags & 1)), has2 = __ballot(own && (cur.flags & 2));
       if (own) {
         lds_double *q = rec + lane * kRecB;
         q[0] = ys[r];
         q[1] = cur.c1;
         q[2] = cur.c2;
-        q[3] = static_cast<double>(cur.flags);
         // products of the children outside the batch in list order; +0.0 past the row's list
         // (t - (+0.0) leaves t unchanged bit for bit, -0.0 included)
+        double g[kChSeqWidth];
 #pragma unroll
-        for (int e = 0; e < kChSeqWidth; ++e) q[4 + e] = e < cur.cnt ? cur.val[e] * ys[cur.row[e]] : 0.0;
+        for (int e = 0; e < kChSeqWidth; ++e) g[e] = ys[cur.row[e]];  // row 0 past the row's list
+#pragma unroll
+        for (int e = 0; e < kChSeqWidth; ++e) q[4 + e] = e < cur.cnt ? cur.val[e] * g[e] : 0.0;
       }
-      double mine = 0.0;
       switch (nfar) {  // the chain with the batch's far count fixed: reads issued together, no branches
-        case 0: mine = bwd_chain<0>(rec, cnt, lane); break;
-        case 1: mine = bwd_chain<1>(rec, cnt, lane); break;
-        case 2: mine = bwd_chain<2>(rec, cnt, lane); break;
-        case 3: mine = bwd_chain<3>(rec, cnt, lane); break;
-        case 4: mine = bwd_chain<4>(rec, cnt, lane); break;
-        case 5: mine = bwd_chain<5>(rec, cnt, lane); break;
-        case 6: mine = bwd_chain<6>(rec, cnt, lane); break;
-        case 7: mine = bwd_chain<7>(rec, cnt, lane); break;
-        case 8: mine = bwd_chain<8>(rec, cnt, lane); break;
-        case 9: mine = bwd_chain<9>(rec, cnt, lane); break;
-        case 10: mine = bwd_chain<10>(rec, cnt, lane); break;
-        case 11: mine = bwd_chain<11>(rec, cnt, lane); break;
-        default: mine = bwd_chain<kChSeqWidth>(rec, cnt, lane); break;
+        case 0: bwd_chain<0>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 1: bwd_chain<1>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 2: bwd_chain<2>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 3: bwd_chain<3>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 4: bwd_chain<4>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 5: bwd_chain<5>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 6: bwd_chain<6>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 7: bwd_chain<7>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 8: bwd_chain<8>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 9: bwd_chain<9>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 10: bwd_chain<10>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 11: bwd_chain<11>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        default: bwd_chain<kChSeqWidth>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
       }
-      if (b0 > 0) load_bwd(a, b0 - 1 - lane, lane < B, nxt);
-      if (own) ys[r] = mine;
-      cur = nxt;
+    } else if (wave == 1 && t + 1 < nb) {
+      stage_bwd(a, b0 - 1 - lane, lane < B, stage + ((t + 1) & 1) * kChStageDoubles, lane);
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int r = tid; r < n; r += kThreads) a.z[r] = ys[r];
   __syncthreads();
 }
@@ -520,9 +668,23 @@ __global__ void __launch_bounds__(kThreads) ch_pack_kernel(ChArgs a) {
   }
 }
 
+// (A x)_row in CSR order (apply_matrix, 235-250). The first 13 entries (the whole 13-point row) are
+// loaded together, positions past the row clamped to its last entry and their products left out,
+// so a row costs two load round trips rather than two per entry.
 __device__ __forceinline__ double row_times(const ChArgs &a, const double *x, int row) {
+  constexpr int kW = 13;
+  const int beg = a.row_ptr[row], end = a.row_ptr[row + 1];  // every row holds its diagonal
+  double v[kW], xv[kW];
+#pragma unroll
+  for (int e = 0; e < kW; ++e) {
+    const int p = min(beg + e, end - 1);
+    v[e] = a.val[p];
+    xv[e] = x[a.col[p]];
+  }
   double s = 0.0;
-  for (int p = a.row_ptr[row]; p < a.row_ptr[row + 1]; ++p) s += a.val[p] * x[a.col[p]];
+#pragma unroll
+  for (int e = 0; e < kW; ++e) s = beg + e < end ? s + v[e] * xv[e] : s;
+  for (int p = beg + kW; p < end; ++p) s += a.val[p] * x[a.col[p]];
   return s;
 }
 
@@ -530,7 +692,7 @@ __device__ __forceinline__ double row_times(const ChArgs &a, const double *x, in
 // reference's. ildlt selects the preconditioner.
 __global__ void __launch_bounds__(kThreads) ch_pcg_kernel(ChArgs a, int ildlt) {
   __shared__ ChShared sh;
-  extern __shared__ double ch_ys[];  // the sequential sweeps' vector and row records (a.seq_batch > 0)
+  extern __shared__ double ch_ys[];  // the sequential sweeps' vector, row records, sink and staging (a.seq_batch > 0)
   const int tid = threadIdx.x, n = a.n;
   int parity = 0;
   const double floor = a.state->floor;
@@ -838,7 +1000,8 @@ hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream) {
 }
 
 hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream) {
-  const size_t lds = ildlt && a.seq_batch > 0 ? (static_cast<size_t>(a.n) + 64 * kRecB) * sizeof(double) : 0;
+  const size_t lds =
+      ildlt && a.seq_batch > 0 ? (static_cast<size_t>(a.n) + 64 * kRecB + 4 * 64 + 2 * kChStageDoubles) * sizeof(double) : 0;
   hipLaunchKernelGGL(ch_pcg_kernel, dim3(1), dim3(kThreads), lds, stream, a, ildlt ? 1 : 0);
   return hipGetLastError();
 }
