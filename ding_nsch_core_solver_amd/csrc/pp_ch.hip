@@ -394,7 +394,7 @@ template <int NF>
 __device__ __forceinline__ void bwd_chain(const lds_double *__restrict__ rec, int cnt, int lane,
                                           unsigned long long has1, unsigned long long has2,
                                           lds_double *__restrict__ ys, int top, lds_double *__restrict__ sink) {
-  constexpr int G = 2;
+  constexpr int G = 4;
   const unsigned long long both = has1 & has2;
   double z1 = 0.0, z2 = 0.0;
   int k = 0;
@@ -412,7 +412,7 @@ __device__ __forceinline__ void bwd_chain(const lds_double *__restrict__ rec, in
     // the group's rows top-k .. top-k-G+1 from one base: lane 0 at the lowest row, the others in
     // their G sink slots
     lds_double *base = lane == 0 ? ys + (top - k - (G - 1)) : sink + G * lane;
-    if (((both >> k) & 3) == 3) {
+    if (((both >> k) & 15) == 15) {
 #pragma unroll
       for (int j = 0; j < G; ++j) {
         double t = v[j][0] - v[j][1] * z1;
