@@ -372,11 +372,8 @@ __device__ __forceinline__ void load_bwd(const ChArgs &a, int r, bool own, SeqBw
 }
 
 // Row records of a batch in LDS for the sequential part: lane k writes its row's operands, then
-// every lane reads row k's record with one broadcast read per pair of doubles (no lane-to-lane
-// moves, and the reads run ahead of the dependency chain). Which in-batch entries a row has is a
-// bit per row in a wave-uniform mask (scalar registers), and lane 0 stores each row's value into
-// the LDS vector as the chain produces it; the other lanes store into their own slot of a sink, so the
-// store needs no branch and the loop stays one block in which the record reads run ahead.
+// every lane reads row k's record with broadcast reads (no lane-to-lane moves). Which in-batch
+// entries a row has is a bit per row in a wave-uniform mask (scalar registers).
 constexpr int kRecF = 4;                  // forward: s, L(r, r-2), L(r, r-1), unused
 constexpr int kRecB = 4 + kChSeqWidth;    // backward: z, L(r+1, r), L(r+2, r), unused, far products
 
@@ -511,8 +508,9 @@ __device__ __forceinline__ void fwd_chain(const lds_double *__restrict__ rec, in
 
 // A batch's operands are staged in LDS by wave 1 while wave 0 sweeps the batch before (two buffers,
 // one workgroup barrier per batch), so wave 0 reads them at LDS latency. One buffer: per row the
-// doubles (r or L(r+1, r), L(r, r-2) or L(r+2, r), L(r, r-1), the 12 packed values) then the ints
-// (count, flags, the 12 columns or rows), field-major so each lane's reads and writes are stride 1.
+// doubles (forward: r, L(r, r-2), L(r, r-1); backward: L(r+1, r), L(r+2, r), unused; then the 12
+// packed values) and the ints (count, flags, the 12 columns or rows), field-major so each lane's
+// reads and writes are stride 1.
 using lds_int = __attribute__((address_space(3))) int;
 constexpr int kStageDbl = 3 + kChSeqWidth, kStageInt = 2 + kChSeqWidth;
 constexpr int kChStageDoubles = 64 * kStageDbl + 64 * kStageInt / 2;
