@@ -381,16 +381,18 @@ constexpr int kRecB = 4 + kChSeqWidth;    // backward: z, L(r+1, r), L(r+2, r), 
 // at most one instruction every four cycles, so the chains are cut to few instructions per row:
 // groups of rows whose in-batch entries are all present take a path without flag tests or selects
 // (most rows of a 13-point operator), the others test their bits (has1, has2: bit k for row k of
-// the batch) and select. The results are stored by lane 0; the other lanes store into their own sink
-// slots (G per lane), which keeps the stores free of branches.
+// the batch) and select. Every lane stores the rows' values (the same in all lanes) to their slots of
+// the LDS vector after each group: of the ways to hand the values on, this one costs the chain least
+// (in a one-wave timing of the forward recurrence, cycles per row: 72, against 82 storing each row as
+// it is computed, 107 for a lane-0 store beside a sink, 151 for an exec-masked store, 82 for a
+// per-lane select, 75 for a DPP chain kept in registers; 42 with no values handed on at all).
 
 // The backward batch's chain: row k of the batch (from the top, row top - k) subtracts its in-batch
 // children (bit k of has1: r+1, of has2: r+2) and then its NF far products (+0.0 past its own list),
 // and lane 0 stores its z at ys[top - k].
 template <int NF>
-__device__ __forceinline__ void bwd_chain(const lds_double *__restrict__ rec, int cnt, int lane,
-                                          unsigned long long has1, unsigned long long has2,
-                                          lds_double *__restrict__ ys, int top, lds_double *__restrict__ sink) {
+__device__ __forceinline__ void bwd_chain(const lds_double *__restrict__ rec, int cnt, unsigned long long has1,
+                                          unsigned long long has2, lds_double *__restrict__ ys, int top) {
   constexpr int G = 4;
   const unsigned long long both = has1 & has2;
   double z1 = 0.0, z2 = 0.0;
@@ -406,9 +408,7 @@ __device__ __forceinline__ void bwd_chain(const lds_double *__restrict__ rec, in
 #pragma unroll
       for (int e = 0; e < NF; ++e) v[j][3 + e] = q[4 + e];
     }
-    // the group's rows top-k .. top-k-G+1 from one base: lane 0 at the lowest row, the others in
-    // their G sink slots
-    lds_double *base = lane == 0 ? ys + (top - k - (G - 1)) : sink + G * lane;
+    double res[G];
     if (((both >> k) & 15) == 15) {
 #pragma unroll
       for (int j = 0; j < G; ++j) {
@@ -418,7 +418,7 @@ __device__ __forceinline__ void bwd_chain(const lds_double *__restrict__ rec, in
         for (int e = 0; e < NF; ++e) t -= v[j][3 + e];
         z2 = z1;
         z1 = t;
-        base[G - 1 - j] = t;
+        res[j] = t;
       }
     } else {
 #pragma unroll
@@ -432,9 +432,13 @@ __device__ __forceinline__ void bwd_chain(const lds_double *__restrict__ rec, in
         for (int e = 0; e < NF; ++e) t -= v[j][3 + e];
         z2 = z1;
         z1 = t;
-        base[G - 1 - j] = t;
+        res[j] = t;
       }
     }
+    // stored after the group, so no store waits on a result just computed
+    lds_double *base = ys + (top - k - (G - 1));  // the group's rows top-k .. top-k-G+1
+#pragma unroll
+    for (int j = 0; j < G; ++j) base[G - 1 - j] = res[j];
   }
   for (; k < cnt; ++k) {
     const lds_double *q = rec + k * kRecB;
@@ -447,15 +451,14 @@ __device__ __forceinline__ void bwd_chain(const lds_double *__restrict__ rec, in
     for (int e = 0; e < NF; ++e) t -= q[4 + e];
     z2 = z1;
     z1 = t;
-    *(lane == 0 ? ys + (top - k) : sink + lane) = t;
+    ys[top - k] = t;
   }
 }
 
 // The forward batch's chain: row k of the batch subtracts its in-batch entries L(r, r-2) y(r-2)
 // (bit k of has2) and L(r, r-1) y(r-1) (bit k of has1), and lane 0 stores its y at out[k].
-__device__ __forceinline__ void fwd_chain(const lds_double *__restrict__ rec, int cnt, int lane,
-                                          unsigned long long has1, unsigned long long has2,
-                                          lds_double *__restrict__ out, lds_double *__restrict__ sink) {
+__device__ __forceinline__ void fwd_chain(const lds_double *__restrict__ rec, int cnt, unsigned long long has1,
+                                          unsigned long long has2, lds_double *__restrict__ out) {
   constexpr int G = 4;
   const unsigned long long both = has1 & has2;
   double y1 = 0.0, y2 = 0.0;
@@ -469,7 +472,7 @@ __device__ __forceinline__ void fwd_chain(const lds_double *__restrict__ rec, in
       v[j][1] = q[1];
       v[j][2] = q[2];
     }
-    lds_double *base = lane == 0 ? out + k : sink + G * lane;  // lane 0: the group's rows, the others their sink slots
+    double res[G];
     if (((both >> k) & 15) == 15) {
 #pragma unroll
       for (int j = 0; j < G; ++j) {
@@ -477,7 +480,7 @@ __device__ __forceinline__ void fwd_chain(const lds_double *__restrict__ rec, in
         t = t - v[j][2] * y1;
         y2 = y1;
         y1 = t;
-        base[j] = t;
+        res[j] = t;
       }
     } else {
 #pragma unroll
@@ -489,9 +492,11 @@ __device__ __forceinline__ void fwd_chain(const lds_double *__restrict__ rec, in
         t = (has1 >> (k + j)) & 1 ? t1 : t;
         y2 = y1;
         y1 = t;
-        base[j] = t;
+        res[j] = t;
       }
     }
+#pragma unroll
+    for (int j = 0; j < G; ++j) out[k + j] = res[j];  // after the group, as in the backward chain
   }
   for (; k < cnt; ++k) {
     const lds_double *q = rec + k * kRecF;
@@ -502,7 +507,7 @@ __device__ __forceinline__ void fwd_chain(const lds_double *__restrict__ rec, in
     t = (has1 >> k) & 1 ? t1 : t;
     y2 = y1;
     y1 = t;
-    *(lane == 0 ? out + k : sink + lane) = t;
+    out[k] = t;
   }
 }
 
@@ -559,8 +564,7 @@ __device__ __forceinline__ void stage_bwd(const ChArgs &a, int r, bool own, lds_
 
 __device__ void ch_precondition_seq(const ChArgs &a, double floor, lds_double *ys, lds_double *rec) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  lds_double *sink = rec + 64 * kRecB;  // 4 slots per lane after the records
-  lds_double *stage = sink + 4 * 64;    // two staging buffers
+  lds_double *stage = rec + 64 * kRecB;  // two staging buffers after the records
   const int n = a.n, B = a.seq_batch, nb = (n + B - 1) / B;
   if (wave == 1) stage_fwd(a, lane, lane < min(B, n), stage, lane);
   __syncthreads();
@@ -583,7 +587,7 @@ __device__ void ch_precondition_seq(const ChArgs &a, double floor, lds_double *y
         q[1] = cur.l2;
         q[2] = cur.l1;
       }
-      fwd_chain(rec, cnt, lane, has1, has2, ys + b0, sink);
+      fwd_chain(rec, cnt, has1, has2, ys + b0);
     } else if (wave == 1 && bi + 1 < nb) {
       stage_fwd(a, b0 + B + lane, lane < min(B, n - b0 - B), stage + ((bi + 1) & 1) * kChStageDoubles, lane);
     }
@@ -626,19 +630,19 @@ __device__ void ch_precondition_seq(const ChArgs &a, double floor, lds_double *y
         for (int e = 0; e < kChSeqWidth; ++e) q[4 + e] = e < cur.cnt ? cur.val[e] * g[e] : 0.0;
       }
       switch (nfar) {  // the chain with the batch's far count fixed: reads issued together, no branches
-        case 0: bwd_chain<0>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 1: bwd_chain<1>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 2: bwd_chain<2>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 3: bwd_chain<3>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 4: bwd_chain<4>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 5: bwd_chain<5>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 6: bwd_chain<6>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 7: bwd_chain<7>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 8: bwd_chain<8>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 9: bwd_chain<9>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 10: bwd_chain<10>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        case 11: bwd_chain<11>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
-        default: bwd_chain<kChSeqWidth>(rec, cnt, lane, has1, has2, ys, hi, sink); break;
+        case 0: bwd_chain<0>(rec, cnt, has1, has2, ys, hi); break;
+        case 1: bwd_chain<1>(rec, cnt, has1, has2, ys, hi); break;
+        case 2: bwd_chain<2>(rec, cnt, has1, has2, ys, hi); break;
+        case 3: bwd_chain<3>(rec, cnt, has1, has2, ys, hi); break;
+        case 4: bwd_chain<4>(rec, cnt, has1, has2, ys, hi); break;
+        case 5: bwd_chain<5>(rec, cnt, has1, has2, ys, hi); break;
+        case 6: bwd_chain<6>(rec, cnt, has1, has2, ys, hi); break;
+        case 7: bwd_chain<7>(rec, cnt, has1, has2, ys, hi); break;
+        case 8: bwd_chain<8>(rec, cnt, has1, has2, ys, hi); break;
+        case 9: bwd_chain<9>(rec, cnt, has1, has2, ys, hi); break;
+        case 10: bwd_chain<10>(rec, cnt, has1, has2, ys, hi); break;
+        case 11: bwd_chain<11>(rec, cnt, has1, has2, ys, hi); break;
+        default: bwd_chain<kChSeqWidth>(rec, cnt, has1, has2, ys, hi); break;
       }
     } else if (wave == 1 && t + 1 < nb) {
       stage_bwd(a, b0 - 1 - lane, lane < B, stage + ((t + 1) & 1) * kChStageDoubles, lane);
@@ -690,7 +694,7 @@ __device__ __forceinline__ double row_times(const ChArgs &a, const double *x, in
 // reference's. ildlt selects the preconditioner.
 __global__ void __launch_bounds__(kThreads) ch_pcg_kernel(ChArgs a, int ildlt) {
   __shared__ ChShared sh;
-  extern __shared__ double ch_ys[];  // the sequential sweeps' vector, row records, sink and staging (a.seq_batch > 0)
+  extern __shared__ double ch_ys[];  // the sequential sweeps' vector, row records and staging (a.seq_batch > 0)
   const int tid = threadIdx.x, n = a.n;
   int parity = 0;
   const double floor = a.state->floor;
@@ -999,7 +1003,7 @@ hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream) {
 
 hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream) {
   const size_t lds =
-      ildlt && a.seq_batch > 0 ? (static_cast<size_t>(a.n) + 64 * kRecB + 4 * 64 + 2 * kChStageDoubles) * sizeof(double) : 0;
+      ildlt && a.seq_batch > 0 ? (static_cast<size_t>(a.n) + 64 * kRecB + 2 * kChStageDoubles) * sizeof(double) : 0;
   hipLaunchKernelGGL(ch_pcg_kernel, dim3(1), dim3(kThreads), lds, stream, a, ildlt ? 1 : 0);
   return hipGetLastError();
 }

@@ -213,7 +213,7 @@ struct ChArgs {
   double *fo_val, *fin_val, *bf_val, *bin_val;
 };
 constexpr int kChSeqWidth = 12;       // entries of a row outside its batch (the 13-point stencil has at most 12)
-constexpr int kChSeqMaxRows = 6912;   // the sequential sweeps keep n doubles (and 31 KB of row records, sink, staging) in LDS
+constexpr int kChSeqMaxRows = 6912;   // the sequential sweeps keep n doubles (and 30 KB of row records and staging) in LDS
 hipError_t launch_ch_pack(const ChArgs &a, hipStream_t stream);
 hipError_t launch_ch_factor(const ChArgs &a, hipStream_t stream);
 hipError_t launch_ch_pcg(const ChArgs &a, bool ildlt, hipStream_t stream);
