@@ -14,7 +14,7 @@ from ding_nsch_core_solver_amd import build as B
 # bytes of spill scratch per lane allowed for each kernel family
 BUDGET = {"fp64_solve_kernel": 32, "mixed_solve_kernel": 96, "correction_faces_kernel": 0,
           "correction_history_kernel": 0, "correction_finish_kernel": 0, "pressure_ghost_kernel": 0,
-          "bicg_": 0, "momentum_": 0}
+          "bicg_": 0, "momentum_": 0, "ch_pcg_kernel": 0, "ch_factor_seq_kernel": 0, "ch_diag_": 0}
 
 
 def _usage(source, tmp_path):
@@ -38,7 +38,7 @@ def _usage(source, tmp_path):
     return usage
 
 
-@pytest.mark.parametrize("source", ["pp_fp64.hip", "pp_correct.hip", "pp_bicgstab.hip", "pp_momentum.hip"])
+@pytest.mark.parametrize("source", ["pp_fp64.hip", "pp_correct.hip", "pp_bicgstab.hip", "pp_momentum.hip", "pp_ch.hip"])
 def test_kernel_spill_scratch_within_budget(tmp_path, source):
     if shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"):
         pytest.skip("hipcc not available")
