@@ -1,6 +1,6 @@
-"""HIP adjoint (pp_adjoint / pp_solve_coeffs through the C ABI) against the oracle's FP64 adjoint,
-against central finite differences at the 256x256 density-ratio-100 adjoint workload, and through
-torch.autograd."""
+"""HIP adjoint (pp_adjoint / pp_solve_coeffs through the C ABI) against the oracle's FP64 adjoint (bit
+for bit, up to the 256x256 density-ratio-100 adjoint workload itself), against central finite
+differences at that workload, and through torch.autograd."""
 import numpy as np
 import pytest
 
@@ -17,6 +17,8 @@ CASES = {
     "periodic_xy_20x20": dict(nx=20, ny=20, periodic_x=True, periodic_y=True),
     "droplet_96x80": dict(nx=96, ny=80),
     "droplet_130x70_periodic_y": dict(nx=130, ny=70, periodic_y=True),
+    # the adjoint workload itself (256x256, density ratio 100): ~400 PCG iterations each way
+    "droplet_256x256_adjoint_workload": dict(nx=256, ny=256),
 }
 
 
