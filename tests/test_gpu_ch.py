@@ -46,9 +46,11 @@ def test_ch_solve_bitwise_vs_oracle(engine, oracle_lib, name, cc):
           f"{rep['kernel_ms']:.3f} ms, deviation from the reference order {_rel(c, seq[0]):.1e}")
 
 
-def test_operator_cache_follows_alpha0(engine, oracle_lib):
-    """A run's sequence: 1/dt at step 0, then 3/(2 dt) (a rebuild), then the cached operator again."""
-    base = W.ch_case(40, 30, seed=11, first_step=True, dt=1e-3, pe=10.0)
+@pytest.mark.parametrize("periodic_y", [False, True], ids=["walls", "periodic_y_batched_factor"])
+def test_operator_cache_follows_alpha0(engine, oracle_lib, periodic_y):
+    """A run's sequence: 1/dt at step 0, then 3/(2 dt) (a rebuild), then the cached operator again.
+    With periodic y the factor is built by batches of rows (a chained level schedule)."""
+    base = W.ch_case(40, 30, seed=11, first_step=True, dt=1e-3, pe=10.0, periodic_y=periodic_y)
     cached = None
     for k, alpha0 in enumerate((1e3, 1.5e3, 1.5e3, 1.5e3 * (1 + 1e-15))):
         # ensure_ch_linear_system keeps the operator while alpha0 stays within 1e-14 max(1, |alpha0|)
@@ -58,9 +60,12 @@ def test_operator_cache_follows_alpha0(engine, oracle_lib):
         _check(engine, oracle_lib, cc, operator_alpha0=cached)
 
 
-def test_ildlt_failure_then_cache_keeps_diagonal(engine, oracle_lib):
-    cc = W.ch_case(16, 16, seed=8, cn=0.05, pe=1.0, dt=1e-2)
+@pytest.mark.parametrize("periodic_y", [False, True], ids=["walls", "periodic_y_batched_factor"])
+def test_ildlt_failure_then_cache_keeps_diagonal(engine, oracle_lib, periodic_y):
+    cc = W.ch_case(16, 16, seed=8, cn=0.05, pe=1.0, dt=1e-2, periodic_y=periodic_y)
     _, rep = _check(engine, oracle_lib, cc)
+    assert rep["preconditioner"] == E.PP_CH_DIAGONAL
+    _, rep = _check(engine, oracle_lib, cc, precond=0)  # the cached operator keeps the diagonal one
     assert rep["preconditioner"] == E.PP_CH_DIAGONAL
 
 
